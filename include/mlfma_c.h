@@ -1,0 +1,224 @@
+/*
+ * mlfma_c.h — C-ABI of the MI355X (gfx950) Helmholtz MLFMA matvec and GMRES
+ * Method-of-Moments solve.
+ *
+ * The reference (ellilu11/MLFMA) has no FFI: its boundary is a C++ class
+ * contract (src/mlfma.cpp:21-82).  Each entry point below replaces one step of
+ * that contract; the citation names the reference code it stands in for.
+ *
+ * Conventions
+ *   - Complex numbers are interleaved FP64 (re, im) pairs.
+ *   - Unknowns are numbered as in the input files (the reference's Source::idx,
+ *     src/sources/source.h:36-38); the library reorders internally.
+ *   - Return codes: 0 on success, a negative MLFMA_ERR_* code on failure; the
+ *     message of the last failure on this thread is given by mlfma_last_error().
+ *     No C++ exception crosses this boundary.
+ *   - One plan = one HIP device.  Plans are independent of each other; calls on
+ *     one plan must not overlap (the plan owns its scratch buffers and, in the
+ *     reference-compatible mode, the persistent local coefficients).
+ */
+#ifndef MLFMA_C_H
+#define MLFMA_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define MLFMA_OK 0
+#define MLFMA_ERR_ARG (-1)          /* bad argument                               */
+#define MLFMA_ERR_OOM (-2)          /* device or host allocation failed           */
+#define MLFMA_ERR_HIP (-3)          /* HIP runtime error                          */
+#define MLFMA_ERR_BREAKDOWN (-4)    /* GMRES breakdown (zero or non-finite norm)  */
+#define MLFMA_ERR_NONFINITE (-5)    /* non-finite value in the input              */
+#define MLFMA_ERR_STATE (-6)        /* call not valid for this plan (host-only)   */
+#define MLFMA_ERR_GEOMETRY (-7)     /* geometry outside the root box, bad mesh    */
+#define MLFMA_ERR_NOT_CONVERGED (-8)/* GMRES stopped at maxit above tolerance     */
+
+/* ------------------------------------------------------------ inputs */
+#define MLFMA_SRC_DIPOLE 0   /* src/sources/dipole.h */
+#define MLFMA_SRC_RWG 1      /* src/sources/rwg.{h,cpp} */
+
+/* Discretisation.  Replaces importFromConfig / importDipoles / importRWG
+ * (src/fileio.h:106-202, 220-263): the caller passes the parsed arrays. */
+typedef struct mlfma_geometry {
+    int32_t kind;            /* MLFMA_SRC_DIPOLE or MLFMA_SRC_RWG                   */
+    int32_t reserved;
+    int64_t n;               /* number of unknowns                                  */
+    const double* dip_pos;   /* dipoles: n x 3 positions                            */
+    const double* dip_mom;   /* dipoles: n x 3 moment vectors (Dipole pol)          */
+    int64_t nverts;          /* RWG: vertices                                       */
+    const double* verts;     /*      nverts x 3                                     */
+    int64_t ntris;           /* RWG: triangles                                      */
+    const int32_t* tris;     /*      ntris x 3 vertex indices (0-based)             */
+    const int32_t* rwgs;     /* RWG: n x 4 = v0 v1 t_plus t_minus                   */
+} mlfma_geometry;
+
+/* Incident plane wave (src/excite.h:6-28, src/fileio.h:8-27). */
+typedef struct mlfma_planewave {
+    double amplitude;
+    double pol[3];
+    double khat[3];
+    double wavenum;
+} mlfma_planewave;
+
+#define MLFMA_FP64 0
+#define MLFMA_FP32 1         /* far-field stages in FP32 complex, FP32 accumulate */
+
+/* flags */
+#define MLFMA_FLAG_FRESH_LOCALS 1u  /* zero the locals of boxes with an empty
+                                       interaction list every matvec instead of
+                                       keeping the previous matvec's values
+                                       (the reference keeps them, src/fmm/node.cpp:106) */
+
+/* Numerical configuration (src/config.h:50-73) plus device selection. */
+typedef struct mlfma_config {
+    int32_t digits;          /* excess-bandwidth digits (src/fmm/angles.cpp:9-10)  */
+    int32_t interp_order;    /* stencil half-width p; 2p taps (src/fmm/tables.cpp)  */
+    double over_interp;      /* psi oversampling (src/fmm/tables.cpp:97)            */
+    double root_leng;        /* root box side, centred at the origin                */
+    int32_t max_node_srcs;   /* split a box while it holds more sources            */
+    int32_t max_level;       /* -1: no cap (reference rule); else also stop at this level */
+    int32_t quad_prec;       /* 0,1,2,3 = 1,3,7,13-point triangle rule              */
+    int32_t device;          /* HIP device ordinal; -1 = host-only plan             */
+    int32_t precision;       /* MLFMA_FP64 or MLFMA_FP32                            */
+    uint32_t flags;          /* MLFMA_FLAG_*                                        */
+} mlfma_config;
+
+typedef struct mlfma_plan mlfma_plan;
+
+/* Plan summary (counts printed by src/mlfma.cpp:34-36 and
+ * src/fmm/angles.cpp:30, plus sizes of the device-resident data). */
+typedef struct mlfma_info {
+    int64_t n;
+    int32_t max_level;
+    int32_t num_levels_m2l;        /* levels with M2L work                          */
+    int64_t num_nodes;
+    int64_t num_leaves;
+    int64_t nth[16], nph[16];      /* angular grid per level                        */
+    int64_t boxes[16];             /* non-empty boxes per level                     */
+    int64_t m2l_pairs[16];         /* interaction-list pairs per level              */
+    int64_t near_nnz;              /* stored near-field entries (both directions)   */
+    int64_t radpat_entries;        /* sum over leaves of srcs x directions          */
+    int64_t stale_boxes;           /* non-empty boxes (level >= 2) with empty iList */
+    int64_t truncated_l2l;         /* boxes whose parent's iList is empty (level>=3)*/
+    int64_t psi_merged;            /* psi keys merged by the translation-table dedup*/
+    int64_t device_bytes;          /* device memory held by the plan                */
+} mlfma_info;
+
+/* --------------------------------------------------------------- plan */
+
+/* Build everything the matvec needs, in the order of src/mlfma.cpp:21-64:
+ * sources and RHS (Node::initStatic / Source::buildVoltage), tree
+ * (Stem/Leaf ctors, src/fmm/stem.cpp:3-26), lists (buildLists,
+ * src/fmm/node.cpp:61-99, src/fmm/nbor.cpp), near field (Leaf::buildNearRads,
+ * src/fmm/leaf.cpp:108-180), angular grids and tables (Node::buildTables,
+ * src/fmm/node.cpp:46-56), coefficient storage (resizeCoeffs) and radiation
+ * patterns (Leaf::buildRadPats, src/fmm/leaf.cpp:185-207).  With
+ * config->device = -1 only the host part (tree, lists, grids, interpolation
+ * tables) is built; no HIP call is made. */
+int mlfma_plan_create(const mlfma_geometry* geom, const mlfma_planewave* pw,
+                      const mlfma_config* cfg, mlfma_plan** out);
+
+int mlfma_plan_destroy(mlfma_plan* plan);
+
+int mlfma_plan_info(const mlfma_plan* plan, mlfma_info* out);
+
+/* Excitation vector V (Source::getVoltage, src/sources/source.h:20), host
+ * buffer of n complex values.  The reference solves Z I = -V
+ * (src/solver.cpp:31-32). */
+int mlfma_voltage(const mlfma_plan* plan, double* v_host);
+
+#define MLFMA_PTR_HOST 0
+#define MLFMA_PTR_DEVICE 1
+
+/* y = Z x, or y += Z x when accumulate != 0 — the drop-in for
+ * Solver::evalRvec (src/solver.cpp:41-57), which reads Node::lvec and adds into
+ * Node::rvec.  x and y hold nrhs column vectors of n complex values each
+ * (column stride n).  ptr_mode says whether x and y are host or device
+ * pointers; stream is a hipStream_t (NULL = the plan's own stream).  The call
+ * returns after the result is complete in y. */
+int mlfma_matvec(mlfma_plan* plan, const double* x, double* y, int32_t nrhs,
+                 int32_t accumulate, int32_t ptr_mode, void* stream);
+
+/* Dense reference matvec over the same near-field kernel, all pairs
+ * (the reference's evalDirect path, src/mlfma.cpp:84-107); for small n. */
+int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t ptr_mode,
+                        void* stream);
+
+/* --------------------------------------------------------------- solve */
+#define MLFMA_GMRES_COMPAT 0   /* src/solver.cpp: unrestarted, complex Givens without
+                                  conjugates, stops on |g_k+1|/g0                  */
+#define MLFMA_GMRES_RESTART 1  /* GMRES(m), unitary Givens, true residual at restart */
+#define MLFMA_FGMRES_IR 2      /* FP64 flexible GMRES outer, FP32 far-field inner   */
+
+typedef struct mlfma_solve_opts {
+    int32_t method;       /* MLFMA_GMRES_*                                          */
+    int32_t restart;      /* m for GMRES(m) / inner iterations for FGMRES           */
+    int32_t maxit;        /* total iteration cap (reference: 500)                   */
+    int32_t use_voltage;  /* 1: b = -V (reference), 0: b from the argument          */
+    double tol;           /* reference: 1e-6                                        */
+} mlfma_solve_opts;
+
+typedef struct mlfma_solve_report {
+    int32_t iterations;   /* matvecs spent in the Krylov loop                       */
+    int32_t status;       /* MLFMA_OK, MLFMA_ERR_BREAKDOWN, MLFMA_ERR_NOT_CONVERGED */
+    double resid_est;     /* final |g|/g0 (compat) or implicit residual             */
+    double resid_true;    /* |b - Z x| / |b| evaluated with one extra FP64 matvec    */
+    double seconds;       /* wall time of the solve                                 */
+} mlfma_solve_report;
+
+/* Solve Z x = b — the drop-in for Solver::solve (src/solver.cpp:116-146)
+ * followed by the currents of Solver::getCurrents.  b and x are host
+ * buffers of n complex values; b may be NULL when opts->use_voltage = 1.
+ * resid_hist (optional, host, maxit doubles) receives the per-iteration
+ * residual estimate. */
+int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_opts* opts,
+                mlfma_solve_report* report, double* resid_hist);
+
+/* --------------------------------------------------------------- far field */
+
+/* E(rhat_d) = C k (I - rhat rhat^T) sum_n I_n f_n(k rhat_d) for ndir unit
+ * directions (generalises Node::printFarSols, src/fmm/node.cpp:147-178, to any
+ * direction list).  I: n complex (host); dirs: ndir x 3 (host);
+ * E: ndir x 3 complex (host). */
+int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E);
+
+/* --------------------------------------------------------------- introspection */
+
+/* Per-stage device time of the last matvec in milliseconds (the reference's
+ * ClockTimes, src/clock.h:8-17): [0] gather+S2M, [1] M2M, [2] M2L, [3] L2L,
+ * [4] L2T, [5] near field, [6] total. */
+int mlfma_stage_times(const mlfma_plan* plan, double* ms7);
+
+/* Boxes of one level in the plan's order: coords (nbox x 3 int32 integer box
+ * coordinates at that level), node kind (1 leaf, 0 stem) and source count. */
+int mlfma_level_boxes(const mlfma_plan* plan, int32_t level, int64_t* nbox, int32_t* coords,
+                      int32_t* is_leaf, int32_t* nsrc, int64_t cap);
+
+/* Interaction lists of one level as (target, source) box-index pairs into the
+ * order of mlfma_level_boxes: kind 0 = M2L (list 2), 1 = near leaf pairs
+ * (list 1, unordered, both leaves), 2 = list-4/list-3 pairs (leaf, node). */
+int mlfma_level_pairs(const mlfma_plan* plan, int32_t level, int32_t kind, int64_t* npairs,
+                      int32_t* pairs, int32_t* src_level, int64_t cap);
+
+/* Multipole (which = 0) or local (which = 1) coefficients of one level after
+ * the last matvec, nbox x ndir x 2 complex (theta, phi components), boxes in
+ * the order of mlfma_level_boxes (empty boxes hold zeros).  FP64 plans only. */
+int mlfma_get_coeffs(const mlfma_plan* plan, int32_t level, int32_t which, double* out,
+                     int64_t cap);
+
+/* Translation operators of one level, 316 x ndir complex, offsets in the
+ * order of Math::getINodeDistVecs (src/math.cpp:109-124), without the M2L
+ * quadrature weights. */
+int mlfma_get_translation(const mlfma_plan* plan, int32_t level, double* out, int64_t cap);
+
+const char* mlfma_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MLFMA_C_H */

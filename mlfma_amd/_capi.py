@@ -1,0 +1,179 @@
+"""ctypes mirror of ``include/mlfma_c.h``.
+
+The structure layouts are the C-ABI's; ``load_library`` finds the in-tree
+``libmlfma_amd.so`` built by ``mlfma_amd/build.py`` and raises if it is absent
+(there is no fallback implementation)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmlfma_amd.so")
+
+MLFMA_SRC_DIPOLE = 0
+MLFMA_SRC_RWG = 1
+MLFMA_FP64 = 0
+MLFMA_FP32 = 1
+MLFMA_FLAG_FRESH_LOCALS = 1
+MLFMA_PTR_HOST = 0
+MLFMA_PTR_DEVICE = 1
+MLFMA_GMRES_COMPAT = 0
+MLFMA_GMRES_RESTART = 1
+MLFMA_FGMRES_IR = 2
+
+ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_OOM", -3: "ERR_HIP", -4: "ERR_BREAKDOWN",
+          -5: "ERR_NONFINITE", -6: "ERR_STATE", -7: "ERR_GEOMETRY", -8: "ERR_NOT_CONVERGED"}
+
+dptr = C.POINTER(C.c_double)
+iptr = C.POINTER(C.c_int32)
+
+
+class Geometry(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("n", C.c_int64),
+                ("dip_pos", dptr), ("dip_mom", dptr),
+                ("nverts", C.c_int64), ("verts", dptr),
+                ("ntris", C.c_int64), ("tris", iptr), ("rwgs", iptr)]
+
+
+class PlaneWave(C.Structure):
+    _fields_ = [("amplitude", C.c_double), ("pol", C.c_double * 3), ("khat", C.c_double * 3),
+                ("wavenum", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("digits", C.c_int32), ("interp_order", C.c_int32), ("over_interp", C.c_double),
+                ("root_leng", C.c_double), ("max_node_srcs", C.c_int32), ("max_level", C.c_int32),
+                ("quad_prec", C.c_int32), ("device", C.c_int32), ("precision", C.c_int32),
+                ("flags", C.c_uint32)]
+
+
+class Info(C.Structure):
+    _fields_ = [("n", C.c_int64), ("max_level", C.c_int32), ("num_levels_m2l", C.c_int32),
+                ("num_nodes", C.c_int64), ("num_leaves", C.c_int64),
+                ("nth", C.c_int64 * 16), ("nph", C.c_int64 * 16), ("boxes", C.c_int64 * 16),
+                ("m2l_pairs", C.c_int64 * 16), ("near_nnz", C.c_int64),
+                ("radpat_entries", C.c_int64), ("stale_boxes", C.c_int64),
+                ("truncated_l2l", C.c_int64), ("psi_merged", C.c_int64),
+                ("device_bytes", C.c_int64)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("restart", C.c_int32), ("maxit", C.c_int32),
+                ("use_voltage", C.c_int32), ("tol", C.c_double)]
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("resid_est", C.c_double),
+                ("resid_true", C.c_double), ("seconds", C.c_double)]
+
+
+# exported symbols and their signatures (kept in step with include/mlfma_c.h)
+SIGNATURES = {
+    "mlfma_plan_create": (C.c_int, [C.POINTER(Geometry), C.POINTER(PlaneWave), C.POINTER(Config),
+                                    C.POINTER(C.c_void_p)]),
+    "mlfma_plan_destroy": (C.c_int, [C.c_void_p]),
+    "mlfma_plan_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
+    "mlfma_voltage": (C.c_int, [C.c_void_p, dptr]),
+    "mlfma_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                               C.c_int32, C.c_void_p]),
+    "mlfma_matvec_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "mlfma_solve": (C.c_int, [C.c_void_p, dptr, dptr, C.POINTER(SolveOpts), C.POINTER(SolveReport),
+                              dptr]),
+    "mlfma_farfield": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
+    "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),
+    "mlfma_level_boxes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), iptr, iptr, iptr,
+                                    C.c_int64]),
+    "mlfma_level_pairs": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64), iptr,
+                                    iptr, C.c_int64]),
+    "mlfma_get_coeffs": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, dptr, C.c_int64]),
+    "mlfma_get_translation": (C.c_int, [C.c_void_p, C.c_int32, dptr, C.c_int64]),
+    "mlfma_last_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                           " (or python mlfma_amd/build.py); there is no CPU fallback")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def as_dptr(a: np.ndarray):
+    return a.ctypes.data_as(dptr)
+
+
+def as_iptr(a: np.ndarray):
+    return a.ctypes.data_as(iptr)
+
+
+class GeometryArrays:
+    """Owns the numpy arrays behind a ``Geometry`` struct."""
+
+    def __init__(self, kind, *, pos=None, mom=None, verts=None, faces=None, rwgs=None):
+        self.kind = kind
+        g = Geometry()
+        g.kind = kind
+        if kind == MLFMA_SRC_DIPOLE:
+            self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+            self.mom = np.ascontiguousarray(mom, dtype=np.float64)
+            g.n = len(self.pos)
+            g.dip_pos = as_dptr(self.pos)
+            g.dip_mom = as_dptr(self.mom)
+        else:
+            self.verts = np.ascontiguousarray(verts, dtype=np.float64)
+            self.faces = np.ascontiguousarray(faces, dtype=np.int32)
+            self.rwgs = np.ascontiguousarray(rwgs, dtype=np.int32)
+            g.n = len(self.rwgs)
+            g.nverts = len(self.verts)
+            g.verts = as_dptr(self.verts)
+            g.ntris = len(self.faces)
+            g.tris = as_iptr(self.faces)
+            g.rwgs = as_iptr(self.rwgs)
+        self.struct = g
+
+    @property
+    def n(self):
+        return self.struct.n
+
+
+def make_planewave(amplitude=1.0, pol=(1.0, 0.0, 0.0), khat=(0.0, 0.0, 1.0), wavenum=2 * np.pi):
+    p = PlaneWave()
+    p.amplitude = amplitude
+    for i in range(3):
+        p.pol[i] = pol[i]
+        p.khat[i] = khat[i]
+    p.wavenum = wavenum
+    return p
+
+
+def make_config(*, digits=3, interp_order=6, over_interp=4.0, root_leng=4.0, max_node_srcs=16,
+                max_level=-1, quad_prec=2, device=0, precision=MLFMA_FP64, flags=0):
+    c = Config()
+    c.digits = digits
+    c.interp_order = interp_order
+    c.over_interp = over_interp
+    c.root_leng = root_leng
+    c.max_node_srcs = max_node_srcs
+    c.max_level = max_level
+    c.quad_prec = quad_prec
+    c.device = device
+    c.precision = precision
+    c.flags = flags
+    return c

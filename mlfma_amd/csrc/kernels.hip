@@ -1,0 +1,757 @@
+// kernels.hip — HIP kernels for gfx950 (MI355X) of the MLFMA matvec and its
+// setup generators.  Wave64 throughout; every kernel is a streaming pass whose
+// roofline is HBM (or L2 for the M2L source reads), see the launchers for the
+// per-item work each block does.
+#include "kernels.hpp"
+
+namespace mlfma {
+
+constexpr int BS = 256;
+constexpr double kPiD = 3.14159265358979323846;
+
+// ------------------------------------------------------------------ complex helpers
+template <typename R>
+__device__ __forceinline__ cplx<R> mk(R a, R b) {
+    cplx<R> c;
+    c.re = a;
+    c.im = b;
+    return c;
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return mk(a.re - b.re, a.im - b.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
+    return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> operator*(R s, cplx<R> a) { return mk(s * a.re, s * a.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> operator*(cplx<R> a, R s) { return mk(s * a.re, s * a.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> conjmul(cplx<R> a, cplx<R> b) {  // conj(a) * b
+    return mk(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> cdiv(cplx<R> a, R s) { return mk(a.re / s, a.im / s); }
+__device__ __forceinline__ cplx<double> cexpi(double a) {
+    double s, c;
+    sincos(a, &s, &c);
+    return mk(c, s);
+}
+template <typename R, typename S>
+__device__ __forceinline__ cplx<R> cast(cplx<S> a) { return mk((R)a.re, (R)a.im); }
+
+// ------------------------------------------------------------------ source kernels
+
+// Closed-form integral of e^{i k.r} times the RWG basis over its two triangles
+// (the analytic branch of src/sources/rwg.cpp:46-86, including its small-angle
+// branches at |alpha|, |beta|, |gamma| < 1e-6).
+__device__ void rwgPlaneWave(const RwgDev& r, const TriDev* tris, const double kv[3], cplx<double> out[3]) {
+    const cplx<double> I = mk(0.0, 1.0);
+    out[0] = out[1] = out[2] = mk(0.0, 0.0);
+    for (int t = 0; t < 2; ++t) {
+        const TriDev& T = tris[r.tri[t]];
+        const double alpha = kv[0] * T.D0[0] + kv[1] * T.D0[1] + kv[2] * T.D0[2];
+        const double beta = -(kv[0] * T.D2[0] + kv[1] * T.D2[1] + kv[2] * T.D2[2]);
+        const double gamma = alpha - beta;
+        const double asq = alpha * alpha, bsq = beta * beta;
+        const cplx<double> eA = cexpi(alpha), eB = cexpi(beta);
+        const bool za = fabs(alpha) < 1e-6, zb = fabs(beta) < 1e-6, zg = fabs(gamma) < 1e-6;
+        const cplx<double> one = mk(1.0, 0.0);
+        const cplx<double> f0a = za ? mk(0.0, -1.0) : cdiv(one - eA, alpha);
+        const cplx<double> f0b = zb ? mk(0.0, -1.0) : cdiv(one - eB, beta);
+        const cplx<double> f1a = za ? mk(-0.5, 0.0) : cdiv(one - mk(1.0, -alpha) * eA, asq);
+        const cplx<double> f1b = zb ? mk(-0.5, 0.0) : cdiv(one - mk(1.0, -beta) * eB, bsq);
+        double A[3];
+        for (int i = 0; i < 3; ++i) A[i] = T.X0[i] - r.xpm[t][i];
+        cplx<double> rv[3];
+        if (zg) {
+            const cplx<double> f2 = cdiv(eA * mk(asq - 2.0, 2.0 * alpha) + mk(2.0, 0.0), 2.0 * alpha * asq);
+            const cplx<double> if2 = I * f2;
+            for (int i = 0; i < 3; ++i) rv[i] = mk(-f1a.re * A[i], -f1a.im * A[i]) - (T.D0[i] - T.D2[i]) * if2;
+        } else {
+            const cplx<double> I0 = cdiv(f0a - f0b, gamma);
+            const cplx<double> I1 = I * (I0 + f1a);
+            const cplx<double> I2 = mk(0.0, -1.0) * (I0 + f1b);
+            for (int i = 0; i < 3; ++i)
+                rv[i] = A[i] * I0 + cdiv(T.D0[i] * I1 - T.D2[i] * I2, gamma);
+        }
+        const cplx<double> ph = cexpi(kv[0] * T.X0[0] + kv[1] * T.X0[1] + kv[2] * T.X0[2]);
+        const double s = t ? -1.0 : 1.0;
+        for (int i = 0; i < 3; ++i) out[i] = out[i] + s * (ph * rv[i]);
+    }
+    for (int i = 0; i < 3; ++i) out[i] = r.leng * out[i];
+}
+
+// Radiation pattern of one source about X along k khat, as (theta, phi)
+// components (src/fmm/leaf.cpp:185-207; dipole.h:47-51; rwg.h:23-25).
+__device__ void radiation(const SrcDev& s, int64_t t, const double X[3], const double kv[3], const double* M,
+                          cplx<double>& o0, cplx<double>& o1) {
+    cplx<double> r[3];
+    if (s.kind == 0) {
+        const DipoleDev& d = s.dip[t];
+        const cplx<double> e =
+            cexpi(kv[0] * (X[0] - d.pos[0]) + kv[1] * (X[1] - d.pos[1]) + kv[2] * (X[2] - d.pos[2]));
+        for (int i = 0; i < 3; ++i) r[i] = d.phat[i] * e;
+    } else {
+        cplx<double> f[3];
+        rwgPlaneWave(s.rwg[t], s.tri, kv, f);
+        const cplx<double> e = cexpi(kv[0] * X[0] + kv[1] * X[1] + kv[2] * X[2]);
+        for (int i = 0; i < 3; ++i) r[i] = e * mk(f[i].re, -f[i].im);
+    }
+    o0 = M[0] * r[0] + M[1] * r[1] + M[2] * r[2];
+    o1 = M[3] * r[0] + M[4] * r[1] + M[5] * r[2];
+}
+
+__global__ void kRadpat(SrcDev s, const LeafItem* __restrict__ items, const LevelDev* __restrict__ lv, double k,
+                        cplx<double>* __restrict__ rp) {
+    const LeafItem it = items[blockIdx.y];
+    const int d = blockIdx.x * BS + threadIdx.x;
+    if (d >= it.ndir) return;
+    const LevelDev L = lv[it.level];
+    const double kv[3] = {k * L.khat[3 * d], k * L.khat[3 * d + 1], k * L.khat[3 * d + 2]};
+    const double X[3] = {it.cx, it.cy, it.cz};
+    const double* M = L.thph + 6 * d;
+    for (int q = 0; q < it.cnt; ++q) {
+        cplx<double> a, b;
+        radiation(s, it.sbeg + q, X, kv, M, a, b);
+        const int64_t o = it.rpOff + ((int64_t)q * it.ndir + d) * 2;
+        rp[o] = a;
+        rp[o + 1] = b;
+    }
+}
+
+void launchRadpat(const SrcDev& s, const LeafItem* items, int nitems, const LevelDev* lv, double k,
+                  cplx<double>* rp, int maxndir, hipStream_t st) {
+    if (nitems <= 0) return;
+    dim3 g((maxndir + BS - 1) / BS, nitems);
+    hipLaunchKernelGGL(kRadpat, g, dim3(BS), 0, st, s, items, lv, k, rp);
+}
+
+// Galerkin entry <f_obs, G f_src> without the C k factor
+// (src/sources/dipole.h:62-70, src/sources/rwg.cpp:92-139): the dyadic Green's
+// function e^{ikr}/r [(1 + i/kr - 1/kr^2) I - (1 + 3i/kr - 3/kr^2) rr]
+// (src/math.h:132-142) contracted with the two test/basis vectors.
+__device__ __forceinline__ cplx<double> greenContract(const double ro[3], const double rs[3], const double vo[3],
+                                                      const double vs[3], double k) {
+    const double X[3] = {ro[0] - rs[0], ro[1] - rs[1], ro[2] - rs[2]};
+    const double r = sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+    const double kr = k * r, inv = 1.0 / kr, inv2 = inv * inv;
+    const cplx<double> e = cexpi(kr);
+    const double ir = 1.0 / r;
+    const double vovs = vo[0] * vs[0] + vo[1] * vs[1] + vo[2] * vs[2];
+    const double vor = (vo[0] * X[0] + vo[1] * X[1] + vo[2] * X[2]) * ir;
+    const double vsr = (vs[0] * X[0] + vs[1] * X[1] + vs[2] * X[2]) * ir;
+    // a (vo.vs) - b (vo.r)(vs.r) with a = 1 - inv2 + i inv, b = 1 - 3 inv2 + 3 i inv
+    const double pr = vor * vsr;
+    const cplx<double> c = mk((1.0 - inv2) * vovs - (1.0 - 3.0 * inv2) * pr, inv * vovs - 3.0 * inv * pr);
+    return (ir)*(e * c);
+}
+
+__device__ cplx<double> entry(const SrcDev& s, int64_t o, int64_t q, double k) {
+    if (s.kind == 0) {
+        const DipoleDev& a = s.dip[o];
+        const DipoleDev& b = s.dip[q];
+        if (a.pos[0] == b.pos[0] && a.pos[1] == b.pos[1] && a.pos[2] == b.pos[2]) return mk(0.0, 0.0);
+        return greenContract(a.pos, b.pos, a.phat, b.phat, k);
+    }
+    const RwgDev& a = s.rwg[o];
+    const RwgDev& b = s.rwg[q];
+    if (a.ctr[0] == b.ctr[0] && a.ctr[1] == b.ctr[1] && a.ctr[2] == b.ctr[2]) return mk(0.0, 0.0);
+    cplx<double> acc = mk(0.0, 0.0);
+    for (int ot = 0; ot < 2; ++ot) {
+        const TriDev& TO = s.tri[a.tri[ot]];
+        for (int st = 0; st < 2; ++st) {
+            if (a.tri[ot] == b.tri[st]) continue;  // coincident triangles are skipped
+            const TriDev& TS = s.tri[b.tri[st]];
+            const double sg = (ot ^ st) ? -1.0 : 1.0;
+            cplx<double> part = mk(0.0, 0.0);
+            for (int i = 0; i < TO.nq; ++i) {
+                const double ro[3] = {TO.q[i][0], TO.q[i][1], TO.q[i][2]};
+                const double vo[3] = {ro[0] - a.xpm[ot][0], ro[1] - a.xpm[ot][1], ro[2] - a.xpm[ot][2]};
+                cplx<double> inner = mk(0.0, 0.0);
+                for (int j = 0; j < TS.nq; ++j) {
+                    const double rs[3] = {TS.q[j][0], TS.q[j][1], TS.q[j][2]};
+                    const double vs[3] = {rs[0] - b.xpm[st][0], rs[1] - b.xpm[st][1], rs[2] - b.xpm[st][2]};
+                    inner = inner + TS.q[j][3] * greenContract(ro, rs, vo, vs, k);
+                }
+                part = part + TO.q[i][3] * inner;
+            }
+            acc = acc + sg * part;
+        }
+    }
+    return (a.leng * b.leng) * acc;
+}
+
+__global__ void kNearBuild(SrcDev s, const PanelItem* __restrict__ panels, int npanels,
+                           const int64_t* __restrict__ cols, int64_t nnz, double k, cplx<double> Ck,
+                           cplx<double>* __restrict__ val) {
+    const int64_t e = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (e >= nnz) return;
+    int lo = 0, hi = npanels - 1;
+    while (lo < hi) {  // last panel with voff <= e
+        const int mid = (lo + hi + 1) >> 1;
+        if (panels[mid].voff <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const PanelItem P = panels[lo];
+    const int64_t loc = e - P.voff;
+    const int64_t c = loc / P.nrows, r = loc - c * P.nrows;
+    const cplx<double> v = entry(s, P.rbeg + r, cols[P.cptr + c], k);
+    val[e] = Ck * v;
+}
+
+void launchNearBuild(const SrcDev& s, const PanelItem* panels, int npanels, const int64_t* cols, int64_t nnz,
+                     double k, cplx<double> Ck, cplx<double>* val, hipStream_t st) {
+    if (nnz <= 0) return;
+    const int64_t nb = (nnz + BS - 1) / BS;
+    hipLaunchKernelGGL(kNearBuild, dim3((unsigned)nb), dim3(BS), 0, st, s, panels, npanels, cols, nnz, k, Ck, val);
+}
+
+// ------------------------------------------------------------------ translation operators
+
+// alpha_d(psi_i) = (ik/4pi) sum_{l<=L} i^l (2l+1) h_l(k d a) P_l(cos psi_i),
+// psi_i = pi i/(nps-1) (src/fmm/tables.cpp:90-127); one thread per (distance, psi).
+__global__ void kAlpha(int nps, int L, double k, double a, const double* __restrict__ dists,
+                       cplx<double>* __restrict__ alpha) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    const int di = blockIdx.y;
+    if (i >= nps) return;
+    const double x = cos(kPiD * i / (double)(nps - 1));
+    const double kr = k * dists[di] * a;
+    // spherical Hankel h_l^(1)(kr) by upward recurrence (src/math.cpp:68-84)
+    const cplx<double> e = cexpi(kr);
+    cplx<double> hm = mk(e.im / kr, -e.re / kr);  // -i e^{ix}/x
+    cplx<double> h = mk(-(e.re * (1.0 / kr) - e.im * (1.0 / (kr * kr))), -(e.im * (1.0 / kr) + e.re * (1.0 / (kr * kr))));
+    double pm = 1.0, p = x;  // P_0, P_1
+    // l = 0 term: i^0 * 1 * h_0 * P_0
+    cplx<double> sum = hm;
+    if (L >= 1) sum = sum + 3.0 * (mk(0.0, 1.0) * h) * p;
+    for (int l = 1; l < L; ++l) {
+        const cplx<double> hn = ((2.0 * l + 1.0) / kr) * h - hm;
+        hm = h;
+        h = hn;
+        const double pn = ((2.0 * l + 1) * x * p - l * pm) / (double)(l + 1);
+        pm = p;
+        p = pn;
+        const int m = (l + 1) & 3;
+        const cplx<double> il = m == 0 ? mk(1.0, 0.0) : (m == 1 ? mk(0.0, 1.0) : (m == 2 ? mk(-1.0, 0.0) : mk(0.0, -1.0)));
+        sum = sum + (2.0 * (l + 1) + 1.0) * (il * h) * p;
+    }
+    alpha[(int64_t)di * nps + i] = mk(0.0, k / (4.0 * kPiD)) * sum;
+}
+
+void launchAlpha(int nps, int L, double k, double nodeLeng, const double* dists, cplx<double>* alpha, hipStream_t st) {
+    dim3 g((nps + BS - 1) / BS, 15);
+    hipLaunchKernelGGL(kAlpha, g, dim3(BS), 0, st, nps, L, k, nodeLeng, dists, alpha);
+}
+
+// T(dir) = sum_q alpha[reflect(near+1-p+q)] l_q(psi) on the uniform psi grid with
+// the half-sample index reflection of src/fmm/tables.cpp:200-213, then times the
+// M2L quadrature weight of its direction (src/fmm/node.cpp:122-132).
+__global__ void kTransl(int ndir, int nps, int p, const double* __restrict__ psi, const int32_t* __restrict__ nearIdx,
+                        const int32_t* __restrict__ distIdx, const cplx<double>* __restrict__ alpha,
+                        const double* __restrict__ dirWeight, cplx<double>* __restrict__ T) {
+    const int d = blockIdx.x * BS + threadIdx.x;
+    const int o = blockIdx.y;
+    if (d >= ndir) return;
+    const int64_t e = (int64_t)o * ndir + d;
+    const double ps = psi[e];
+    const int near = nearIdx[e];
+    const cplx<double>* al = alpha + (int64_t)distIdx[o] * nps;
+    const double h = kPiD / (double)(nps - 1);
+    cplx<double> acc = mk(0.0, 0.0);
+    for (int q = 0; q < 2 * p; ++q) {
+        const int iq = near + 1 - p + q;
+        const double xq = kPiD * iq / (double)(nps - 1);
+        double w = 1.0;
+        for (int j = 0; j < 2 * p; ++j) {
+            if (j == q) continue;
+            const double xj = kPiD * (near + 1 - p + j) / (double)(nps - 1);
+            w *= (ps - xj) / (xq - xj);
+        }
+        const int f = iq < 0 ? -iq - 1 : (iq >= nps ? 2 * nps - iq - 1 : iq);
+        acc = acc + w * al[f];
+    }
+    (void)h;
+    T[e] = dirWeight[d] * acc;
+}
+
+void launchTransl(int ndir, int nps, int p, const double* psi, const int32_t* nearIdx, const int32_t* distIdx,
+                  const cplx<double>* alpha, const double* dirWeight, cplx<double>* T, hipStream_t st) {
+    dim3 g((ndir + BS - 1) / BS, 316);
+    hipLaunchKernelGGL(kTransl, g, dim3(BS), 0, st, ndir, nps, p, psi, nearIdx, distIdx, alpha, dirWeight, T);
+}
+
+// ------------------------------------------------------------------ excitation and far field
+
+// V = -E0 e^{ik.r} (p0 x . pol) for dipoles (the reference evaluates the voltage
+// with the default moment p0 x, src/sources/dipole.h:11-16,29-32) and
+// V = -E0 sum_i F_i pol_i for RWG (src/sources/rwg.h:18-21).
+__global__ void kVoltage(SrcDev s, int64_t n, double w0, double w1, double w2, double p0, double p1, double p2,
+                         double amp, cplx<double>* __restrict__ V) {
+    const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    cplx<double> v;
+    if (s.kind == 0) {
+        const DipoleDev& d = s.dip[t];
+        const cplx<double> e = cexpi(w0 * d.pos[0] + w1 * d.pos[1] + w2 * d.pos[2]);
+        v = (-amp * 1.0e-8 * p0) * e;
+    } else {
+        const double kv[3] = {w0, w1, w2};
+        cplx<double> f[3];
+        rwgPlaneWave(s.rwg[t], s.tri, kv, f);
+        v = (-amp) * (p0 * f[0] + p1 * f[1] + p2 * f[2]);
+    }
+    V[t] = v;
+}
+
+void launchVoltage(const SrcDev& s, int64_t n, const double* wv, const double* pol, double amp, cplx<double>* V,
+                   hipStream_t st) {
+    const int64_t nb = (n + BS - 1) / BS;
+    hipLaunchKernelGGL(kVoltage, dim3((unsigned)nb), dim3(BS), 0, st, s, n, wv[0], wv[1], wv[2], pol[0], pol[1],
+                       pol[2], amp, V);
+}
+
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wsum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// E(rhat) = C k (I - rr) sum_n I_n f_n(k rhat) with f_n = conj(F_n) (RWG) or
+// e^{-ik rhat.r} p (dipole) (src/fmm/node.cpp:147-178); one block per direction.
+__global__ void kFarfield(SrcDev s, int64_t n, const cplx<double>* __restrict__ I, const double* __restrict__ dirs,
+                          double k, cplx<double> Ck, cplx<double>* __restrict__ E) {
+    const int d = blockIdx.x;
+    const double rh[3] = {dirs[3 * d], dirs[3 * d + 1], dirs[3 * d + 2]};
+    const double kv[3] = {k * rh[0], k * rh[1], k * rh[2]};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t t = threadIdx.x; t < n; t += BS) {
+        cplx<double> f[3];
+        if (s.kind == 0) {
+            const DipoleDev& dp = s.dip[t];
+            const cplx<double> e = cexpi(-(dp.pos[0] * kv[0] + dp.pos[1] * kv[1] + dp.pos[2] * kv[2]));
+            for (int i = 0; i < 3; ++i) f[i] = dp.phat[i] * e;
+        } else {
+            rwgPlaneWave(s.rwg[t], s.tri, kv, f);
+            for (int i = 0; i < 3; ++i) f[i] = mk(f[i].re, -f[i].im);
+        }
+        for (int i = 0; i < 3; ++i) {
+            const cplx<double> v = I[t] * f[i];
+            acc[2 * i] += v.re;
+            acc[2 * i + 1] += v.im;
+        }
+    }
+    __shared__ double red[BS / 64][6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int j = 0; j < 6; ++j) {
+        const double v = wsum(acc[j]);
+        if (lane == 0) red[w][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cplx<double> a[3];
+        for (int i = 0; i < 3; ++i) {
+            double re = 0, im = 0;
+            for (int q = 0; q < BS / 64; ++q) {
+                re += red[q][2 * i];
+                im += red[q][2 * i + 1];
+            }
+            a[i] = mk(re, im);
+        }
+        for (int i = 0; i < 3; ++i) {
+            cplx<double> v = mk(0.0, 0.0);
+            for (int j = 0; j < 3; ++j) v = v + ((i == j ? 1.0 : 0.0) - rh[i] * rh[j]) * a[j];
+            E[3 * d + i] = Ck * v;
+        }
+    }
+}
+
+void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const double* dirs, int ndir, double k,
+                    cplx<double> Ck, cplx<double>* E, hipStream_t st) {
+    if (ndir <= 0) return;
+    hipLaunchKernelGGL(kFarfield, dim3(ndir), dim3(BS), 0, st, s, n, I, dirs, k, Ck, E);
+}
+
+// ------------------------------------------------------------------ matvec stages
+
+__global__ void kGather(const cplx<double>* __restrict__ x, const int64_t* __restrict__ perm,
+                        cplx<double>* __restrict__ xt, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const int64_t r = blockIdx.y;
+    if (t >= n) return;
+    xt[r * n + t] = x[r * n + perm[t]];
+}
+
+void launchGather(const cplx<double>* x, const int64_t* perm, cplx<double>* xt, int64_t n, int nrhs, hipStream_t st) {
+    dim3 g((unsigned)((n + BS - 1) / BS), nrhs);
+    hipLaunchKernelGGL(kGather, g, dim3(BS), 0, st, x, perm, xt, n);
+}
+
+__global__ void kScatter(const cplx<double>* __restrict__ yt, const int64_t* __restrict__ perm,
+                         cplx<double>* __restrict__ y, int64_t n, int acc) {
+    const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const int64_t r = blockIdx.y;
+    if (t >= n) return;
+    const int64_t i = r * n + perm[t];
+    const cplx<double> v = yt[r * n + t];
+    y[i] = acc ? y[i] + v : v;
+}
+
+void launchScatter(const cplx<double>* yt, const int64_t* perm, cplx<double>* y, int64_t n, int nrhs, int accumulate,
+                   hipStream_t st) {
+    dim3 g((unsigned)((n + BS - 1) / BS), nrhs);
+    hipLaunchKernelGGL(kScatter, g, dim3(BS), 0, st, yt, perm, y, n, accumulate);
+}
+
+// S2M: M_b[d] = sum_s x_s rp_b[s][d] (src/fmm/leaf.cpp:212-230).  Block = one
+// leaf x 256 directions; each lane streams its direction's column of the
+// leaf's radiation patterns (consecutive lanes read consecutive 2x16 B pairs).
+template <typename R>
+__global__ void kS2M(const LeafItem* __restrict__ items, const cplx<R>* __restrict__ rp,
+                     const cplx<double>* __restrict__ xt, cplx<R>* __restrict__ M) {
+    const LeafItem it = items[blockIdx.y];
+    const int d = blockIdx.x * BS + threadIdx.x;
+    if (d >= it.ndir) return;
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+    const cplx<R>* p = rp + it.rpOff + 2 * d;
+    for (int q = 0; q < it.cnt; ++q) {
+        const cplx<R> x = cast<R>(xt[it.sbeg + q]);
+        const cplx<R> r0 = p[(int64_t)q * it.ndir * 2];
+        const cplx<R> r1 = p[(int64_t)q * it.ndir * 2 + 1];
+        a0 = a0 + x * r0;
+        a1 = a1 + x * r1;
+    }
+    M[it.fOff + 2 * d] = a0;
+    M[it.fOff + 2 * d + 1] = a1;
+}
+
+template <typename R>
+void launchS2M(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* xt, cplx<R>* M,
+               hipStream_t st) {
+    if (nitems <= 0) return;
+    dim3 g((maxndir + BS - 1) / BS, nitems);
+    hipLaunchKernelGGL(kS2M<R>, g, dim3(BS), 0, st, items, rp, xt, M);
+}
+
+// M2M theta pass (src/fmm/stem.cpp:72-103,141-171): for one parent, the sum over
+// its non-empty children of the phase-shifted child field interpolated in theta
+// onto the parent's theta nodes, on the child's phi grid.  Across a pole the
+// stencil reads the reflected theta node at phi + pi with both components negated.
+template <typename R>
+__global__ void kM2MTheta(const int32_t* __restrict__ ptr, const int32_t* __restrict__ child,
+                          const int32_t* __restrict__ oct, int nthP, int mthC, int mphC, int p,
+                          const int32_t* __restrict__ thNear, const double* __restrict__ thW,
+                          const cplx<R>* __restrict__ phaseUp, const cplx<R>* __restrict__ Mc,
+                          cplx<R>* __restrict__ inner) {
+    const int item = blockIdx.y;
+    const int e = blockIdx.x * BS + threadIdx.x;
+    if (e >= nthP * mphC) return;
+    const int jth = e / mphC, iph = e - jth * mphC;
+    const int ndirC = mthC * mphC;
+    const int near = thNear[jth];
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+    for (int c = ptr[item]; c < ptr[item + 1]; ++c) {
+        const cplx<R>* Mb = Mc + (int64_t)child[c] * ndirC * 2;
+        const cplx<R>* ph = phaseUp + (int64_t)oct[c] * ndirC;
+        for (int q = 0; q < 2 * p; ++q) {
+            const int i = near + 1 - p + q;
+            const int f = i < 0 ? -i - 1 : (i >= mthC ? 2 * mthC - i - 1 : i);
+            const bool flip = (i != f);
+            const int ip = flip ? (iph + mphC / 2) % mphC : iph;
+            const int d = f * mphC + ip;
+            const R w = (R)(flip ? -thW[jth * 2 * p + q] : thW[jth * 2 * p + q]);
+            const cplx<R> s = ph[d];
+            a0 = a0 + w * (s * Mb[2 * d]);
+            a1 = a1 + w * (s * Mb[2 * d + 1]);
+        }
+    }
+    cplx<R>* o = inner + ((int64_t)item * nthP * mphC + e) * 2;
+    o[0] = a0;
+    o[1] = a1;
+}
+
+template <typename R>
+void launchM2MTheta(int nitems, const int32_t* /*stemSlot*/, const int32_t* ptr, const int32_t* child,
+                    const int32_t* oct, int nthP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
+                    const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* inner, hipStream_t st) {
+    if (nitems <= 0) return;
+    dim3 g((nthP * mphC + BS - 1) / BS, nitems);
+    hipLaunchKernelGGL(kM2MTheta<R>, g, dim3(BS), 0, st, ptr, child, oct, nthP, mthC, mphC, p, thNear, thW, phaseUp,
+                       Mc, inner);
+}
+
+// M2M phi pass (src/fmm/stem.cpp:173-183): periodic Lagrange interpolation in phi.
+template <typename R>
+__global__ void kM2MPhi(const int32_t* __restrict__ stemSlot, int nthP, int nphP, int mphC, int p,
+                        const int32_t* __restrict__ phNear, const double* __restrict__ phW,
+                        const cplx<R>* __restrict__ inner, cplx<R>* __restrict__ Mp) {
+    const int item = blockIdx.y;
+    const int e = blockIdx.x * BS + threadIdx.x;
+    if (e >= nthP * nphP) return;
+    const int jth = e / nphP, jph = e - jth * nphP;
+    const cplx<R>* in = inner + (int64_t)item * nthP * mphC * 2 + (int64_t)jth * mphC * 2;
+    const int near = phNear[jph];
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+    for (int q = 0; q < 2 * p; ++q) {
+        int i = near + 1 - p + q;
+        i = i < 0 ? i + mphC : (i >= mphC ? i - mphC : i);
+        const R w = (R)phW[jph * 2 * p + q];
+        a0 = a0 + w * in[2 * i];
+        a1 = a1 + w * in[2 * i + 1];
+    }
+    cplx<R>* o = Mp + ((int64_t)stemSlot[item] * nthP * nphP + e) * 2;
+    o[0] = a0;
+    o[1] = a1;
+}
+
+template <typename R>
+void launchM2MPhi(int nitems, const int32_t* stemSlot, int nthP, int nphP, int mphC, int p, const int32_t* phNear,
+                  const double* phW, const cplx<R>* inner, cplx<R>* Mp, hipStream_t st) {
+    if (nitems <= 0) return;
+    dim3 g((nthP * nphP + BS - 1) / BS, nitems);
+    hipLaunchKernelGGL(kM2MPhi<R>, g, dim3(BS), 0, st, stemSlot, nthP, nphP, mphC, p, phNear, phW, inner, Mp);
+}
+
+// M2L: L_t[d] = sum over the interaction list of T_off[d] M_s[d] (src/fmm/node.cpp:105-134);
+// the quadrature weights are folded into T.  Block = one target x 256 directions.
+template <typename R>
+__global__ void kM2L(const int32_t* __restrict__ tgtSlot, const int32_t* __restrict__ ptr,
+                     const int32_t* __restrict__ src, const int32_t* __restrict__ off, int ndir,
+                     const cplx<R>* __restrict__ T, const cplx<R>* __restrict__ M, cplx<R>* __restrict__ Lc) {
+    const int item = blockIdx.y;
+    const int d = blockIdx.x * BS + threadIdx.x;
+    if (d >= ndir) return;
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+    for (int q = ptr[item]; q < ptr[item + 1]; ++q) {
+        const cplx<R> t = T[(int64_t)off[q] * ndir + d];
+        const cplx<R>* m = M + ((int64_t)src[q] * ndir + d) * 2;
+        a0 = a0 + t * m[0];
+        a1 = a1 + t * m[1];
+    }
+    cplx<R>* o = Lc + ((int64_t)tgtSlot[item] * ndir + d) * 2;
+    o[0] = a0;
+    o[1] = a1;
+}
+
+template <typename R>
+void launchM2L(int ntgt, const int32_t* tgtSlot, const int32_t* ptr, const int32_t* src, const int32_t* off, int ndir,
+               const cplx<R>* T, const cplx<R>* M, cplx<R>* Lc, hipStream_t st) {
+    if (ntgt <= 0) return;
+    dim3 g((ndir + BS - 1) / BS, ntgt);
+    hipLaunchKernelGGL(kM2L<R>, g, dim3(BS), 0, st, tgtSlot, ptr, src, off, ndir, T, M, Lc);
+}
+
+template <typename R>
+__global__ void kZeroSlots(const int32_t* __restrict__ slots, int ndir, cplx<R>* __restrict__ Lc) {
+    const int e = blockIdx.x * BS + threadIdx.x;
+    if (e >= 2 * ndir) return;
+    Lc[(int64_t)slots[blockIdx.y] * ndir * 2 + e] = mk<R>(0, 0);
+}
+
+template <typename R>
+void launchZeroSlots(int n, const int32_t* slots, int ndir, cplx<R>* Lc, hipStream_t st) {
+    if (n <= 0) return;
+    dim3 g((2 * ndir + BS - 1) / BS, n);
+    hipLaunchKernelGGL(kZeroSlots<R>, g, dim3(BS), 0, st, slots, ndir, Lc);
+}
+
+// L2L phi pass (transpose of the M2M phi pass, src/fmm/stem.cpp:109-131,202-216):
+// phase shift of the parent local on the parent grid, then the gather form of
+// the phi anterpolation onto the child's phi nodes.
+template <typename R>
+__global__ void kL2LPhi(const int32_t* __restrict__ parent, const int32_t* __restrict__ oct, int mthP, int mphP,
+                        int nphC, const int32_t* __restrict__ atPtr, const int32_t* __restrict__ atSrc,
+                        const double* __restrict__ atW, const cplx<R>* __restrict__ phaseDn,
+                        const cplx<R>* __restrict__ Lp, cplx<R>* __restrict__ inner) {
+    const int item = blockIdx.y;
+    const int e = blockIdx.x * BS + threadIdx.x;
+    if (e >= mthP * nphC) return;
+    const int ith = e / nphC, jph = e - ith * nphC;
+    const int ndirP = mthP * mphP;
+    const cplx<R>* Lb = Lp + (int64_t)parent[item] * ndirP * 2;
+    const cplx<R>* ph = phaseDn + (int64_t)oct[item] * ndirP;
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+    for (int q = atPtr[jph]; q < atPtr[jph + 1]; ++q) {
+        const int d = ith * mphP + atSrc[q];
+        const R w = (R)atW[q];
+        const cplx<R> s = ph[d];
+        a0 = a0 + w * (s * Lb[2 * d]);
+        a1 = a1 + w * (s * Lb[2 * d + 1]);
+    }
+    cplx<R>* o = inner + ((int64_t)item * mthP * nphC + e) * 2;
+    o[0] = a0;
+    o[1] = a1;
+}
+
+template <typename R>
+void launchL2LPhi(int nitems, const int32_t* parent, const int32_t* oct, int mthP, int mphP, int nphC,
+                  const int32_t* atPtr, const int32_t* atSrc, const double* atW, const cplx<R>* phaseDn,
+                  const cplx<R>* Lp, cplx<R>* inner, hipStream_t st) {
+    if (nitems <= 0) return;
+    dim3 g((mthP * nphC + BS - 1) / BS, nitems);
+    hipLaunchKernelGGL(kL2LPhi<R>, g, dim3(BS), 0, st, parent, oct, mthP, mphP, nphC, atPtr, atSrc, atW, phaseDn, Lp,
+                       inner);
+}
+
+// L2L theta pass (src/fmm/stem.cpp:218-243): gather form of the theta
+// anterpolation with the pole reflection, added into the child's local.
+template <typename R>
+__global__ void kL2LTheta(const int32_t* __restrict__ child, int mthP, int nthC, int nphC,
+                          const int32_t* __restrict__ atPtr, const int32_t* __restrict__ atSrc,
+                          const double* __restrict__ atW, const int32_t* __restrict__ atFlip,
+                          const cplx<R>* __restrict__ inner, cplx<R>* __restrict__ Lc) {
+    const int item = blockIdx.y;
+    const int e = blockIdx.x * BS + threadIdx.x;
+    if (e >= nthC * nphC) return;
+    const int jf = e / nphC, js = e - jf * nphC;
+    const cplx<R>* in = inner + (int64_t)item * mthP * nphC * 2;
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+    const int jsf = (js + nphC / 2) % nphC;
+    for (int q = atPtr[jf]; q < atPtr[jf + 1]; ++q) {
+        const bool flip = atFlip[q] != 0;
+        const R w = (R)(flip ? -atW[q] : atW[q]);
+        const int d = atSrc[q] * nphC + (flip ? jsf : js);
+        a0 = a0 + w * in[2 * d];
+        a1 = a1 + w * in[2 * d + 1];
+    }
+    cplx<R>* o = Lc + ((int64_t)child[item] * nthC * nphC + e) * 2;
+    o[0] = o[0] + a0;
+    o[1] = o[1] + a1;
+}
+
+template <typename R>
+void launchL2LTheta(int nitems, const int32_t* child, int mthP, int nthC, int nphC, const int32_t* atPtr,
+                    const int32_t* atSrc, const double* atW, const int32_t* atFlip, const cplx<R>* inner, cplx<R>* Lc,
+                    hipStream_t st) {
+    if (nitems <= 0) return;
+    dim3 g((nthC * nphC + BS - 1) / BS, nitems);
+    hipLaunchKernelGGL(kL2LTheta<R>, g, dim3(BS), 0, st, child, mthP, nthC, nphC, atPtr, atSrc, atW, atFlip, inner,
+                       Lc);
+}
+
+// L2T: y_s += C k sum_d conj(rp_s[d]) . L[d] (src/fmm/leaf.cpp:257-280).  One
+// wave per source: lanes stride the directions, then a wave reduction.
+template <typename R>
+__global__ void kL2T(const LeafItem* __restrict__ items, const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc,
+                     cplx<double> Ck, cplx<double>* __restrict__ yt) {
+    const LeafItem it = items[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const cplx<R>* Lb = Lc + it.fOff;
+    for (int q = w; q < it.cnt; q += BS / 64) {
+        const cplx<R>* r = rp + it.rpOff + (int64_t)q * it.ndir * 2;
+        R re = 0, im = 0;
+        for (int d = lane; d < it.ndir; d += 64) {
+            const cplx<R> v = conjmul(r[2 * d], Lb[2 * d]) + conjmul(r[2 * d + 1], Lb[2 * d + 1]);
+            re += v.re;
+            im += v.im;
+        }
+        re = wsum(re);
+        im = wsum(im);
+        if (lane == 0) {
+            const cplx<double> v = Ck * mk((double)re, (double)im);
+            yt[it.sbeg + q] = yt[it.sbeg + q] + v;
+        }
+    }
+}
+
+template <typename R>
+void launchL2T(const LeafItem* items, int nitems, const cplx<R>* rp, const cplx<R>* Lc, cplx<double> Ck,
+               cplx<double>* yt, hipStream_t st) {
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL(kL2T<R>, dim3(nitems), dim3(BS), 0, st, items, rp, Lc, Ck, yt);
+}
+
+// Near field: y_rows = Z_panel x_cols for one leaf's row panel (column-major,
+// C k folded into the values).  Threads are (row, column-group) so that one
+// sweep of the block reads a contiguous run of the panel.
+__global__ void kNear(const PanelItem* __restrict__ panels, const int64_t* __restrict__ cols,
+                      const cplx<double>* __restrict__ val, const cplx<double>* __restrict__ xt,
+                      cplx<double>* __restrict__ yt) {
+    const PanelItem P = panels[blockIdx.x];
+    __shared__ double sre[BS], sim[BS];
+    for (int r0 = 0; r0 < P.nrows; r0 += BS) {
+        const int nr = min(BS, P.nrows - r0);
+        const int G = BS / nr;
+        const int r = threadIdx.x % nr, g = threadIdx.x / nr;
+        double re = 0, im = 0;
+        if (g < G) {
+            for (int c = g; c < P.ncols; c += G) {
+                const cplx<double> z = val[P.voff + (int64_t)c * P.nrows + r0 + r];
+                const cplx<double> x = xt[cols[P.cptr + c]];
+                re += z.re * x.re - z.im * x.im;
+                im += z.re * x.im + z.im * x.re;
+            }
+        }
+        sre[threadIdx.x] = re;
+        sim[threadIdx.x] = im;
+        __syncthreads();
+        if (threadIdx.x < nr) {
+            double a = 0, b = 0;
+            for (int q = 0; q < G; ++q) {
+                a += sre[q * nr + threadIdx.x];
+                b += sim[q * nr + threadIdx.x];
+            }
+            yt[P.rbeg + r0 + threadIdx.x] = mk(a, b);
+        }
+        __syncthreads();
+    }
+}
+
+void launchNear(const PanelItem* panels, int npanels, const int64_t* cols, const cplx<double>* val,
+                const cplx<double>* xt, cplx<double>* yt, hipStream_t st) {
+    if (npanels <= 0) return;
+    hipLaunchKernelGGL(kNear, dim3(npanels), dim3(BS), 0, st, panels, cols, val, xt, yt);
+}
+
+// Dense matvec over all pairs with the near-field kernel (the reference's
+// evalDirect path, src/mlfma.cpp:84-107); one thread per row, for small n.
+__global__ void kDirect(SrcDev s, const cplx<double>* __restrict__ x, int64_t n, double k, cplx<double> Ck,
+                        cplx<double>* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    cplx<double> acc = mk(0.0, 0.0);
+    for (int64_t j = 0; j < n; ++j) {
+        if (j == i) continue;
+        acc = acc + entry(s, i, j, k) * x[j];
+    }
+    y[i] = Ck * acc;
+}
+
+void launchDirect(const SrcDev& s, const cplx<double>* xt, int64_t n, double k, cplx<double> Ck, cplx<double>* yt,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(kDirect, dim3((unsigned)((n + BS - 1) / BS)), dim3(BS), 0, st, s, xt, n, k, Ck, yt);
+}
+
+// explicit instantiations
+#define INST(R)                                                                                                   \
+    template void launchS2M<R>(const LeafItem*, int, int, const cplx<R>*, const cplx<double>*, cplx<R>*,           \
+                               hipStream_t);                                                                      \
+    template void launchM2MTheta<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int,  \
+                                    int, int, const int32_t*, const double*, const cplx<R>*, const cplx<R>*,       \
+                                    cplx<R>*, hipStream_t);                                                       \
+    template void launchM2MPhi<R>(int, const int32_t*, int, int, int, int, const int32_t*, const double*,          \
+                                  const cplx<R>*, cplx<R>*, hipStream_t);                                         \
+    template void launchM2L<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int,           \
+                               const cplx<R>*, const cplx<R>*, cplx<R>*, hipStream_t);                            \
+    template void launchZeroSlots<R>(int, const int32_t*, int, cplx<R>*, hipStream_t);                             \
+    template void launchL2LPhi<R>(int, const int32_t*, const int32_t*, int, int, int, const int32_t*,              \
+                                  const int32_t*, const double*, const cplx<R>*, const cplx<R>*, cplx<R>*,        \
+                                  hipStream_t);                                                                   \
+    template void launchL2LTheta<R>(int, const int32_t*, int, int, int, const int32_t*, const int32_t*,            \
+                                    const double*, const int32_t*, const cplx<R>*, cplx<R>*, hipStream_t);        \
+    template void launchL2T<R>(const LeafItem*, int, const cplx<R>*, const cplx<R>*, cplx<double>, cplx<double>*,   \
+                               hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace mlfma

@@ -1,0 +1,128 @@
+// kernels.hpp — device data layouts and kernel launchers of the MI355X MLFMA
+// library.  All arrays live in HBM and are indexed in tree order of the
+// unknowns (sources of one leaf contiguous, leaves in depth-first order).
+//
+// Complex numbers are interleaved (re, im).  A "field" of a box is
+// ndir x 2 complex values: for each direction (theta-major, phi-minor, as
+// src/fmm/angles.cpp:41-54) the theta and phi components.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mlfma {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cplx {
+    R re, im;
+};
+
+// ---- per-leaf far-field work item (S2M, L2T, radiation patterns)
+struct LeafItem {
+    int64_t sbeg;    // first tree position of the leaf's sources
+    int32_t cnt;     // number of sources
+    int32_t ndir;    // directions of the leaf's level
+    int64_t rpOff;   // first complex pair of the leaf's radiation patterns
+    int64_t fOff;    // first complex pair of the leaf's field (multipole/local)
+    int32_t level;
+    int32_t pad;
+    double cx, cy, cz;  // leaf centre
+};
+
+// ---- near-field row panel
+struct PanelItem {
+    int64_t rbeg;    // first row (tree position)
+    int32_t nrows;
+    int32_t ncols;
+    int64_t cptr;    // first column index entry
+    int64_t voff;    // first value (column-major nrows x ncols)
+};
+
+// ---- source descriptions in tree order
+struct DipoleDev {
+    double pos[3];
+    double phat[3];
+};
+struct RwgDev {
+    double ctr[3];
+    double leng;
+    int32_t tri[2];
+    double xpm[2][3];
+};
+struct TriDev {
+    double X0[3], D0[3], D2[3];
+    int32_t nq;
+    int32_t pad;
+    double q[13][4];
+};
+
+struct SrcDev {
+    int kind;
+    const DipoleDev* dip;
+    const RwgDev* rwg;
+    const TriDev* tri;
+};
+
+// ---- one level's angular data
+struct LevelDev {
+    int32_t nth, nph, ndir, nps;
+    const double* khat;   // ndir x 3
+    const double* thph;   // ndir x 6 (2x3 projector rows)
+};
+
+// ----------------------------------------------------------------- launchers
+// setup
+void launchRadpat(const SrcDev& s, const LeafItem* items, int nitems, const LevelDev* lv, double k,
+                  cplx<double>* rp, int maxndir, hipStream_t st);
+void launchNearBuild(const SrcDev& s, const PanelItem* panels, int npanels, const int64_t* cols,
+                     int64_t nnz, double k, cplx<double> Ck, cplx<double>* val, hipStream_t st);
+void launchAlpha(int nps, int L, double k, double nodeLeng, const double* dists, cplx<double>* alpha,
+                 hipStream_t st);
+void launchTransl(int ndir, int nps, int p, const double* psi, const int32_t* nearIdx, const int32_t* distIdx,
+                  const cplx<double>* alpha, const double* dirWeight, cplx<double>* T, hipStream_t st);
+void launchVoltage(const SrcDev& s, int64_t n, const double* wavevec, const double* pol, double amp,
+                   cplx<double>* V, hipStream_t st);
+void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const double* dirs, int ndir, double k,
+                    cplx<double> Ck, cplx<double>* E, hipStream_t st);
+
+// matvec stages, templated on the real type of the far field
+void launchGather(const cplx<double>* x, const int64_t* perm, cplx<double>* xt, int64_t n, int nrhs,
+                  hipStream_t st);
+void launchScatter(const cplx<double>* yt, const int64_t* perm, cplx<double>* y, int64_t n, int nrhs,
+                   int accumulate, hipStream_t st);
+
+template <typename R>
+void launchS2M(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* xt,
+               cplx<R>* M, hipStream_t st);
+template <typename R>
+void launchM2MTheta(int nitems, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child,
+                    const int32_t* oct, int nthP, int mthC, int mphC, int p, const int32_t* thNear,
+                    const double* thW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* inner,
+                    hipStream_t st);
+template <typename R>
+void launchM2MPhi(int nitems, const int32_t* stemSlot, int nthP, int nphP, int mphC, int p,
+                  const int32_t* phNear, const double* phW, const cplx<R>* inner, cplx<R>* Mp,
+                  hipStream_t st);
+template <typename R>
+void launchM2L(int ntgt, const int32_t* tgtSlot, const int32_t* ptr, const int32_t* src, const int32_t* off,
+               int ndir, const cplx<R>* T, const cplx<R>* M, cplx<R>* Lc, hipStream_t st);
+template <typename R>
+void launchZeroSlots(int n, const int32_t* slots, int ndir, cplx<R>* Lc, hipStream_t st);
+template <typename R>
+void launchL2LPhi(int nitems, const int32_t* parent, const int32_t* oct, int mthP, int mphP, int nphC,
+                  const int32_t* atPtr, const int32_t* atSrc, const double* atW, const cplx<R>* phaseDn,
+                  const cplx<R>* Lp, cplx<R>* inner, hipStream_t st);
+template <typename R>
+void launchL2LTheta(int nitems, const int32_t* child, int mthP, int nthC, int nphC, const int32_t* atPtr,
+                    const int32_t* atSrc, const double* atW, const int32_t* atFlip, const cplx<R>* inner,
+                    cplx<R>* Lc, hipStream_t st);
+template <typename R>
+void launchL2T(const LeafItem* items, int nitems, const cplx<R>* rp, const cplx<R>* Lc, cplx<double> Ck,
+               cplx<double>* yt, hipStream_t st);
+void launchDirect(const SrcDev& s, const cplx<double>* xt, int64_t n, double k, cplx<double> Ck, cplx<double>* yt,
+                  hipStream_t st);
+void launchNear(const PanelItem* panels, int npanels, const int64_t* cols, const cplx<double>* val,
+                const cplx<double>* xt, cplx<double>* yt, hipStream_t st);
+
+}  // namespace mlfma

@@ -1,0 +1,751 @@
+// plan.hip — device plan and the C-ABI of include/mlfma_c.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace mlfma {
+
+thread_local std::string g_lastError;
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(MLFMA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static const double kPi = std::acos(-1.0);
+
+void* DevicePlan::alloc(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) throw Error(MLFMA_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(p);
+    bytesHeld += (int64_t)bytes;
+    return p;
+}
+
+void DevicePlan::release(void* p) {
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) {
+        (void)hipFree(p);
+        allocs.erase(it);
+    }
+}
+
+DevicePlan::~DevicePlan() {
+    if (hostOnly) return;
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocs) (void)hipFree(p);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename T>
+T* DevicePlan::upload(const std::vector<T>& v) {
+    T* d = (T*)alloc(v.size() * sizeof(T));
+    if (!v.empty()) hipCheck(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    return d;
+}
+
+static std::vector<double> inodeDistances() {  // 15 distinct |d| of the M2L offsets (src/math.cpp:86-107)
+    std::vector<double> d;
+    for (int z = 0; z <= 3; ++z)
+        for (int y = 0; y <= 3; ++y)
+            for (int x = 0; x <= 3; ++x)
+                if (std::max(x, std::max(y, z)) > 1)
+                    d.push_back(std::sqrt((double)x * x + (double)y * y + (double)z * z));
+    std::sort(d.begin(), d.end());
+    d.erase(std::unique(d.begin(), d.end()), d.end());
+    return d;
+}
+
+template <typename R>
+static std::vector<cplx<R>> toR(const std::vector<cplx<double>>& v) {
+    std::vector<cplx<R>> o(v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+        o[i].re = (R)v[i].re;
+        o[i].im = (R)v[i].im;
+    }
+    return o;
+}
+
+void DevicePlan::build() {
+    HostPlan& h = hp;
+    hipCheck(hipSetDevice(device), "hipSetDevice");
+    hipCheck(hipStreamCreate(&stream), "hipStreamCreate");
+    for (auto& e : ev) hipCheck(hipEventCreate(&e), "hipEventCreate");
+    const int64_t n = h.src.n;
+    const Tree& t = h.tree;
+    const int ML = t.maxLevel;
+    useFp32 = (h.cfg.precision == MLFMA_FP32);
+
+    // sources in tree order
+    src.kind = h.src.kind;
+    if (h.src.kind == MLFMA_SRC_DIPOLE) {
+        std::vector<DipoleDev> d(n);
+        for (int64_t q = 0; q < n; ++q) {
+            const int64_t i = t.perm[q];
+            d[q].pos[0] = h.src.pos[i].x; d[q].pos[1] = h.src.pos[i].y; d[q].pos[2] = h.src.pos[i].z;
+            d[q].phat[0] = h.src.phat[i].x; d[q].phat[1] = h.src.phat[i].y; d[q].phat[2] = h.src.phat[i].z;
+        }
+        src.dip = upload(d);
+    } else {
+        std::vector<RwgDev> r(n);
+        for (int64_t q = 0; q < n; ++q) {
+            const int64_t i = t.perm[q];
+            r[q].ctr[0] = h.src.ctr[i].x; r[q].ctr[1] = h.src.ctr[i].y; r[q].ctr[2] = h.src.ctr[i].z;
+            r[q].leng = h.src.leng[i];
+            for (int s = 0; s < 2; ++s) {
+                r[q].tri[s] = h.src.tri[s][i];
+                r[q].xpm[s][0] = h.src.xpm[s][i].x;
+                r[q].xpm[s][1] = h.src.xpm[s][i].y;
+                r[q].xpm[s][2] = h.src.xpm[s][i].z;
+            }
+        }
+        std::vector<TriDev> tr(h.src.tris.size());
+        for (size_t q = 0; q < tr.size(); ++q) {
+            const TriQuad& T = h.src.tris[q];
+            std::memset(&tr[q], 0, sizeof(TriDev));
+            for (int a = 0; a < 3; ++a) {
+                tr[q].X0[a] = T.X[0][a];
+                tr[q].D0[a] = T.D0[a];
+                tr[q].D2[a] = T.D2[a];
+            }
+            tr[q].nq = T.nq;
+            for (int j = 0; j < T.nq; ++j)
+                for (int a = 0; a < 4; ++a) tr[q].q[j][a] = T.q[j][a];
+        }
+        src.rwg = upload(r);
+        src.tri = upload(tr);
+    }
+    perm = upload(t.perm);
+
+    // angular data per level
+    levels.assign(ML + 1, LevelBufs());
+    std::vector<LevelDev> lvd(ML + 1);
+    for (int l = 0; l <= ML; ++l) {
+        const Grid& g = h.grids[l];
+        LevelBufs& L = levels[l];
+        L.nth = g.nth;
+        L.nph = g.nph;
+        L.ndir = g.ndir();
+        L.khat.resize((size_t)3 * L.ndir);
+        std::vector<double> thph((size_t)6 * L.ndir);
+        for (int i = 0; i < g.nth; ++i)
+            for (int j = 0; j < g.nph; ++j) {
+                const int d = i * g.nph + j;
+                const double th = g.theta[i], ph = g.phi[j];
+                L.khat[3 * d] = std::sin(th) * std::cos(ph);
+                L.khat[3 * d + 1] = std::sin(th) * std::sin(ph);
+                L.khat[3 * d + 2] = std::cos(th);
+                double* M = &thph[6 * d];
+                M[0] = std::cos(th) * std::cos(ph);
+                M[1] = std::cos(th) * std::sin(ph);
+                M[2] = -std::sin(th);
+                M[3] = -std::sin(ph);
+                M[4] = std::cos(ph);
+                M[5] = 0.0;
+            }
+        lvd[l].nth = g.nth;
+        lvd[l].nph = g.nph;
+        lvd[l].ndir = g.ndir();
+        lvd[l].khat = upload(L.khat);
+        lvd[l].thph = upload(thph);
+    }
+    levelDev = upload(lvd);
+
+    // coefficient storage: levels >= 2, non-empty boxes only
+    int64_t off = 0;
+    for (int l = 2; l <= ML; ++l) {
+        levels[l].Moff = off;
+        off += (int64_t)h.boxes[l] * levels[l].ndir * 2;
+    }
+    fieldLen = off;
+    M64 = (cplx<double>*)alloc(std::max<int64_t>(fieldLen, 1) * sizeof(cplx<double>));
+    L64 = (cplx<double>*)alloc(std::max<int64_t>(fieldLen, 1) * sizeof(cplx<double>));
+    hipCheck(hipMemset(M64, 0, std::max<int64_t>(fieldLen, 1) * sizeof(cplx<double>)), "memset");
+    hipCheck(hipMemset(L64, 0, std::max<int64_t>(fieldLen, 1) * sizeof(cplx<double>)), "memset");
+    if (useFp32) {
+        M32 = (cplx<float>*)alloc(std::max<int64_t>(fieldLen, 1) * sizeof(cplx<float>));
+        L32 = (cplx<float>*)alloc(std::max<int64_t>(fieldLen, 1) * sizeof(cplx<float>));
+        hipCheck(hipMemset(M32, 0, std::max<int64_t>(fieldLen, 1) * sizeof(cplx<float>)), "memset");
+        hipCheck(hipMemset(L32, 0, std::max<int64_t>(fieldLen, 1) * sizeof(cplx<float>)), "memset");
+    }
+
+    // far-field leaves and their radiation patterns
+    std::vector<LeafItem> items;
+    maxLeafNdir = 0;
+    for (const auto& f : h.farLeaves) {
+        const Node& nd = t.nodes[f.node];
+        LeafItem it;
+        it.sbeg = f.sbeg;
+        it.cnt = (int32_t)f.cnt;
+        it.ndir = levels[f.level].ndir;
+        it.rpOff = 2 * f.rpOff;
+        it.fOff = levels[f.level].Moff + (int64_t)f.slot * it.ndir * 2;
+        it.level = f.level;
+        it.pad = 0;
+        it.cx = nd.center.x;
+        it.cy = nd.center.y;
+        it.cz = nd.center.z;
+        items.push_back(it);
+        maxLeafNdir = std::max(maxLeafNdir, it.ndir);
+    }
+    nLeafItems = (int)items.size();
+    leafItems = upload(items);
+    rp64 = (cplx<double>*)alloc(std::max<int64_t>(2 * h.radpatEntries, 1) * sizeof(cplx<double>));
+    launchRadpat(src, leafItems, nLeafItems, levelDev, h.k, rp64, maxLeafNdir, stream);
+    hipCheck(hipGetLastError(), "radpat launch");
+
+    // translation operators (levels >= 2), quadrature weights folded in
+    const std::vector<double> dists = inodeDistances();
+    double* dDists = upload(dists);
+    for (int l = 2; l <= ML; ++l) {
+        LevelBufs& L = levels[l];
+        const Grid& g = h.grids[l];
+        const int nps = (int)std::floor(h.cfg.over_interp * (g.nth - 1));
+        const int p = h.cfg.interp_order;
+        if (nps < 2 * p || nps < 2) throw Error(MLFMA_ERR_ARG, "over_interp too small for the psi stencil");
+        L.nps = nps;
+        std::vector<double> psi((size_t)316 * L.ndir);
+        std::vector<int32_t> nearIdx((size_t)316 * L.ndir), distIdx(316);
+        for (int o = 0; o < 316; ++o) {
+            int dx, dy, dz;
+            offsetVector(o, dx, dy, dz);
+            const double r = std::sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+            const double rh[3] = {dx / r, dy / r, dz / r};
+            int di = 0;
+            for (size_t q = 0; q < dists.size(); ++q)
+                if (std::fabs(dists[q] - r) < 1e-9) di = (int)q;
+            distIdx[o] = di;
+            for (int d = 0; d < L.ndir; ++d) {
+                const double c = L.khat[3 * d] * rh[0] + L.khat[3 * d + 1] * rh[1] + L.khat[3 * d + 2] * rh[2];
+                const double ps = std::acos(c);
+                psi[(size_t)o * L.ndir + d] = ps;
+                nearIdx[(size_t)o * L.ndir + d] = (int32_t)std::floor((nps - 1) * ps / kPi);
+            }
+        }
+        std::vector<double> dirW(L.ndir);
+        const double phw = 2.0 * kPi / static_cast<double>(g.nph);
+        for (int i = 0; i < g.nth; ++i)
+            for (int j = 0; j < g.nph; ++j) dirW[i * g.nph + j] = g.weight[i] * phw;
+        L.dirW = dirW;
+        double* dPsi = upload(psi);
+        int32_t* dNear = upload(nearIdx);
+        int32_t* dDist = upload(distIdx);
+        double* dW = upload(dirW);
+        cplx<double>* dAlpha = (cplx<double>*)alloc((size_t)15 * nps * sizeof(cplx<double>));
+        L.T64 = (cplx<double>*)alloc((size_t)316 * L.ndir * sizeof(cplx<double>));
+        const double a = h.cfg.root_leng / std::pow(2.0, l);
+        launchAlpha(nps, g.L, h.k, a, dDists, dAlpha, stream);
+        launchTransl(L.ndir, nps, p, dPsi, dNear, dDist, dAlpha, dW, L.T64, stream);
+        hipCheck(hipGetLastError(), "translation launch");
+        hipCheck(hipStreamSynchronize(stream), "translation");
+        release(dPsi);
+        release(dNear);
+        release(dDist);
+        release(dW);
+        release(dAlpha);
+    }
+    release(dDists);
+
+    // phase tables: M2M on the child grid, L2L on the parent grid
+    for (int l = 2; l <= ML; ++l) {
+        LevelBufs& L = levels[l];
+        const double hl = h.cfg.root_leng / std::pow(2.0, l) / 2.0;  // half side of a level-l box
+        std::vector<cplx<double>> up((size_t)8 * L.ndir), dn;
+        for (int o = 0; o < 8; ++o) {
+            const double pm[3] = {(o & 1) ? 1.0 : -1.0, (o & 2) ? 1.0 : -1.0, (o & 4) ? 1.0 : -1.0};
+            for (int d = 0; d < L.ndir; ++d) {
+                const double kv[3] = {L.khat[3 * d] * h.k, L.khat[3 * d + 1] * h.k, L.khat[3 * d + 2] * h.k};
+                // parent centre minus this (child) centre = -hl * pm
+                const double ph = kv[0] * (-hl * pm[0]) + kv[1] * (-hl * pm[1]) + kv[2] * (-hl * pm[2]);
+                up[(size_t)o * L.ndir + d] = cplx<double>{std::cos(ph), std::sin(ph)};
+            }
+        }
+        if (l < ML) {
+            const double hc = h.cfg.root_leng / std::pow(2.0, l + 1) / 2.0;  // child half side
+            dn.resize((size_t)8 * L.ndir);
+            for (int o = 0; o < 8; ++o) {
+                const double pm[3] = {(o & 1) ? 1.0 : -1.0, (o & 2) ? 1.0 : -1.0, (o & 4) ? 1.0 : -1.0};
+                for (int d = 0; d < L.ndir; ++d) {
+                    const double kv[3] = {L.khat[3 * d] * h.k, L.khat[3 * d + 1] * h.k, L.khat[3 * d + 2] * h.k};
+                    const double ph = kv[0] * (hc * pm[0]) + kv[1] * (hc * pm[1]) + kv[2] * (hc * pm[2]);
+                    dn[(size_t)o * L.ndir + d] = cplx<double>{std::cos(ph), std::sin(ph)};
+                }
+            }
+        }
+        L.phUp64 = upload(up);
+        if (!dn.empty()) L.phDn64 = upload(dn);
+        if (useFp32) {
+            L.phUp32 = upload(toR<float>(up));
+            if (!dn.empty()) L.phDn32 = upload(toR<float>(dn));
+            std::vector<cplx<double>> Th((size_t)316 * L.ndir);
+            hipCheck(hipMemcpy(Th.data(), L.T64, Th.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost), "T copy");
+            L.T32 = upload(toR<float>(Th));
+        }
+    }
+
+    // interpolation tables and worklists
+    int64_t innerLen = 1;
+    for (int l = 2; l <= ML; ++l) {
+        LevelBufs& L = levels[l];
+        if (l < ML) {
+            const InterpTable& I = h.interp[l];
+            L.thNear = upload(I.thNear);
+            L.thW = upload(I.thW);
+            L.phNear = upload(I.phNear);
+            L.phW = upload(I.phW);
+            L.atPhPtr = upload(I.atPhPtr);
+            L.atPhSrc = upload(I.atPhSrc);
+            L.atPhW = upload(I.atPhW);
+            L.atThPtr = upload(I.atThPtr);
+            L.atThSrc = upload(I.atThSrc);
+            L.atThW = upload(I.atThW);
+            L.atThFlip = upload(I.atThFlip);
+            // M2M into level l: stems with non-empty children
+            std::vector<int32_t> stem, ptr{0}, ch, oc;
+            const auto& P = h.m2mPtr[l];
+            for (size_t s = 0; s + 1 < P.size(); ++s) {
+                if (P[s + 1] == P[s]) continue;
+                stem.push_back((int32_t)s);
+                for (int q = P[s]; q < P[s + 1]; ++q) {
+                    ch.push_back(h.m2mChild[l][q]);
+                    oc.push_back(h.m2mOct[l][q]);
+                }
+                ptr.push_back((int32_t)ch.size());
+            }
+            L.nstem = (int)stem.size();
+            L.stemSlot = upload(stem);
+            L.mPtr = upload(ptr);
+            L.mChild = upload(ch);
+            L.mOct = upload(oc);
+            innerLen = std::max<int64_t>(innerLen, (int64_t)L.nstem * L.nth * levels[l + 1].nph * 2);
+        }
+        // M2L targets with a non-empty interaction list; stale boxes otherwise
+        std::vector<int32_t> tgt, ptr{0}, sr, of, stale;
+        const auto& P = h.m2lPtr[l];
+        for (size_t s = 0; s + 1 < P.size(); ++s) {
+            if (P[s + 1] == P[s]) {
+                stale.push_back((int32_t)s);
+                continue;
+            }
+            tgt.push_back((int32_t)s);
+            for (int q = P[s]; q < P[s + 1]; ++q) {
+                sr.push_back(h.m2lSrc[l][q]);
+                of.push_back(h.m2lOff[l][q]);
+            }
+            ptr.push_back((int32_t)sr.size());
+        }
+        L.ntgt = (int)tgt.size();
+        L.tgtSlot = upload(tgt);
+        L.lPtr = upload(ptr);
+        L.lSrc = upload(sr);
+        L.lOff = upload(of);
+        L.nstale = (int)stale.size();
+        L.staleSlots = upload(stale);
+        // L2L into level l (from l-1)
+        if (l >= 3) {
+            L.nl2l = (int)h.l2lChild[l].size();
+            L.l2lChild = upload(h.l2lChild[l]);
+            L.l2lParent = upload(h.l2lParent[l]);
+            L.l2lOct = upload(h.l2lOct[l]);
+            innerLen = std::max<int64_t>(innerLen, (int64_t)L.nl2l * levels[l - 1].nth * L.nph * 2);
+        }
+    }
+    inner64 = (cplx<double>*)alloc(innerLen * sizeof(cplx<double>));
+    if (useFp32) inner32 = (cplx<float>*)alloc(innerLen * sizeof(cplx<float>));
+
+    // near field
+    std::vector<PanelItem> pan(h.panels.size());
+    for (size_t q = 0; q < pan.size(); ++q) {
+        const auto& P = h.panels[q];
+        pan[q].rbeg = P.rbeg;
+        pan[q].nrows = (int32_t)P.nrows;
+        pan[q].ncols = (int32_t)P.ncols;
+        pan[q].cptr = P.cptr;
+        pan[q].voff = P.voff;
+    }
+    nPanels = (int)pan.size();
+    panels = upload(pan);
+    panelCols = upload(h.panelCols);
+    nearVal = (cplx<double>*)alloc(std::max<int64_t>(h.nearNnz, 1) * sizeof(cplx<double>));
+    const std::complex<double> C = std::complex<double>(0.0, -1.0) * 299792458.0 * 1.256637E-6 / (4.0 * kPi);
+    const std::complex<double> Ckc = C * h.k;
+    Ck = cplx<double>{Ckc.real(), Ckc.imag()};
+    launchNearBuild(src, panels, nPanels, panelCols, h.nearNnz, h.k, Ck, nearVal, stream);
+    hipCheck(hipGetLastError(), "near build launch");
+
+    // radiation patterns in FP32 for the mixed-precision far field
+    if (useFp32) {
+        std::vector<cplx<double>> tmp(2 * h.radpatEntries);
+        hipCheck(hipStreamSynchronize(stream), "radpat");
+        if (!tmp.empty())
+            hipCheck(hipMemcpy(tmp.data(), rp64, tmp.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost), "rp copy");
+        rp32 = upload(toR<float>(tmp));
+    }
+
+    // excitation
+    cplx<double>* dV = (cplx<double>*)alloc(n * sizeof(cplx<double>));
+    const double wv[3] = {h.pw.wavenum * h.pw.khat[0], h.pw.wavenum * h.pw.khat[1], h.pw.wavenum * h.pw.khat[2]};
+    launchVoltage(src, n, wv, h.pw.pol, h.pw.amplitude, dV, stream);
+    hipCheck(hipGetLastError(), "voltage launch");
+    std::vector<cplx<double>> vt(n);
+    hipCheck(hipStreamSynchronize(stream), "setup");
+    hipCheck(hipMemcpy(vt.data(), dV, n * sizeof(cplx<double>), hipMemcpyDeviceToHost), "voltage copy");
+    release(dV);
+    Vin.resize(n);
+    for (int64_t q = 0; q < n; ++q) Vin[t.perm[q]] = std::complex<double>(vt[q].re, vt[q].im);
+
+    ensureRhs(1);
+}
+
+void DevicePlan::ensureRhs(int nrhs) {
+    if (nrhs <= rhsCap) return;
+    if (xt) release(xt);
+    if (yt) release(yt);
+    if (xs) release(xs);
+    if (ys) release(ys);
+    const int64_t n = hp.src.n;
+    xt = (cplx<double>*)alloc((size_t)n * nrhs * sizeof(cplx<double>));
+    yt = (cplx<double>*)alloc((size_t)n * nrhs * sizeof(cplx<double>));
+    xs = (cplx<double>*)alloc((size_t)n * nrhs * sizeof(cplx<double>));
+    ys = (cplx<double>*)alloc((size_t)n * nrhs * sizeof(cplx<double>));
+    rhsCap = nrhs;
+}
+
+// One matvec on tree-ordered vectors: yt = Z xt (single right-hand side).
+template <typename R>
+void DevicePlan::farField(const cplx<double>* x, cplx<double>* y, hipStream_t st, bool timed) {
+    const int ML = hp.tree.maxLevel;
+    cplx<R>* M = fieldM<R>();
+    cplx<R>* Lc = fieldL<R>();
+    cplx<R>* inner = scratch<R>();
+    const cplx<R>* rp = radpat<R>();
+    if (timed) hipCheck(hipEventRecord(ev[1], st), "event");
+    launchS2M<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, M, st);
+    if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
+    for (int l = ML - 1; l >= 2; --l) {
+        LevelBufs& P = levels[l];
+        LevelBufs& C = levels[l + 1];
+        launchM2MTheta<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, C.nth, C.nph, hp.cfg.interp_order,
+                          P.thNear, P.thW, C.phUp<R>(), M + C.Moff, inner, st);
+        launchM2MPhi<R>(P.nstem, P.stemSlot, P.nth, P.nph, C.nph, hp.cfg.interp_order, P.phNear, P.phW, inner,
+                        M + P.Moff, st);
+    }
+    if (timed) hipCheck(hipEventRecord(ev[3], st), "event");
+    for (int l = 2; l <= ML; ++l) {
+        LevelBufs& L = levels[l];
+        launchM2L<R>(L.ntgt, L.tgtSlot, L.lPtr, L.lSrc, L.lOff, L.ndir, L.T<R>(), M + L.Moff, Lc + L.Moff, st);
+        if (hp.cfg.flags & MLFMA_FLAG_FRESH_LOCALS) launchZeroSlots<R>(L.nstale, L.staleSlots, L.ndir, Lc + L.Moff, st);
+    }
+    if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
+    for (int l = 3; l <= ML; ++l) {
+        LevelBufs& P = levels[l - 1];
+        LevelBufs& C = levels[l];
+        launchL2LPhi<R>(C.nl2l, C.l2lParent, C.l2lOct, P.nth, P.nph, C.nph, P.atPhPtr, P.atPhSrc, P.atPhW,
+                        P.phDn<R>(), Lc + P.Moff, inner, st);
+        launchL2LTheta<R>(C.nl2l, C.l2lChild, P.nth, C.nth, C.nph, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, inner,
+                          Lc + C.Moff, st);
+    }
+    if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
+    launchNear(panels, nPanels, panelCols, nearVal, x, y, st);
+    if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
+    launchL2T<R>(leafItems, nLeafItems, rp, Lc, Ck, y, st);
+    if (timed) hipCheck(hipEventRecord(ev[7], st), "event");
+}
+
+void DevicePlan::applyTree(const cplx<double>* x, cplx<double>* y, hipStream_t st, bool fp32, bool timed) {
+    if (fp32) {
+        if (!useFp32) throw Error(MLFMA_ERR_STATE, "plan was not built for FP32");
+        farField<float>(x, y, st, timed);
+    } else
+        farField<double>(x, y, st, timed);
+}
+
+void DevicePlan::matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int accumulate, hipStream_t st, bool fp32) {
+    ensureRhs(nrhs);
+    const int64_t n = hp.src.n;
+    hipCheck(hipEventRecord(ev[0], st), "event");
+    for (int r = 0; r < nrhs; ++r) {
+        launchGather(x + (size_t)r * n, perm, xt, n, 1, st);
+        applyTree(xt, yt, st, fp32, r == 0);
+        launchScatter(yt, perm, y + (size_t)r * n, n, 1, accumulate, st);
+    }
+    hipCheck(hipEventRecord(ev[8], st), "event");
+    hipCheck(hipGetLastError(), "matvec launch");
+    timesValid = true;
+}
+
+}  // namespace mlfma
+
+// ====================================================================== C-ABI
+using namespace mlfma;
+
+#define API_BEGIN try {
+#define API_END                                  \
+    }                                            \
+    catch (const Error& e) {                     \
+        g_lastError = e.what();                  \
+        return e.code;                           \
+    }                                            \
+    catch (const std::bad_alloc&) {              \
+        g_lastError = "host allocation failed";  \
+        return MLFMA_ERR_OOM;                    \
+    }                                            \
+    catch (const std::exception& e) {            \
+        g_lastError = e.what();                  \
+        return MLFMA_ERR_ARG;                    \
+    }
+
+static DevicePlan& P(mlfma_plan* p) {
+    if (!p || !p->d) throw Error(MLFMA_ERR_ARG, "null plan");
+    return *p->d;
+}
+static const DevicePlan& P(const mlfma_plan* p) {
+    if (!p || !p->d) throw Error(MLFMA_ERR_ARG, "null plan");
+    return *p->d;
+}
+static void needDevice(const DevicePlan& d) {
+    if (d.hostOnly) throw Error(MLFMA_ERR_STATE, "host-only plan (device = -1) has no device data");
+}
+
+extern "C" {
+
+const char* mlfma_last_error(void) { return g_lastError.c_str(); }
+
+int mlfma_plan_create(const mlfma_geometry* geom, const mlfma_planewave* pw, const mlfma_config* cfg,
+                      mlfma_plan** out) {
+    API_BEGIN
+    if (!geom || !pw || !cfg || !out) throw Error(MLFMA_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto plan = std::make_unique<mlfma_plan>();
+    plan->d = std::make_unique<DevicePlan>();
+    DevicePlan& d = *plan->d;
+    buildHostPlan(d.hp, *geom, *pw, *cfg);
+    d.hostOnly = cfg->device < 0;
+    d.device = cfg->device;
+    if (!d.hostOnly) d.build();
+    *out = plan.release();
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_plan_destroy(mlfma_plan* plan) {
+    API_BEGIN
+    delete plan;
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_plan_info(const mlfma_plan* plan, mlfma_info* o) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    if (!o) throw Error(MLFMA_ERR_ARG, "null info");
+    std::memset(o, 0, sizeof(*o));
+    const HostPlan& h = d.hp;
+    o->n = h.src.n;
+    o->max_level = h.tree.maxLevel;
+    o->num_nodes = (int64_t)h.tree.nodes.size();
+    for (const auto& nd : h.tree.nodes)
+        if (nd.leaf()) ++o->num_leaves;
+    for (int l = 0; l <= h.tree.maxLevel && l < 16; ++l) {
+        o->nth[l] = h.grids[l].nth;
+        o->nph[l] = h.grids[l].nph;
+        o->boxes[l] = h.boxes[l];
+        o->m2l_pairs[l] = l < (int)h.m2lPairs.size() ? h.m2lPairs[l] : 0;
+        if (o->m2l_pairs[l] > 0) ++o->num_levels_m2l;
+    }
+    o->near_nnz = h.nearNnz;
+    o->radpat_entries = h.radpatEntries;
+    o->stale_boxes = h.staleBoxes;
+    o->truncated_l2l = h.truncatedL2L;
+    o->psi_merged = 0;
+    o->device_bytes = d.bytesHeld;
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_voltage(const mlfma_plan* plan, double* v) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    needDevice(d);
+    if (!v) throw Error(MLFMA_ERR_ARG, "null output");
+    for (size_t i = 0; i < d.Vin.size(); ++i) {
+        v[2 * i] = d.Vin[i].real();
+        v[2 * i + 1] = d.Vin[i].imag();
+    }
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_matvec(mlfma_plan* plan, const double* x, double* y, int32_t nrhs, int32_t accumulate, int32_t ptr_mode,
+                 void* stream) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    needDevice(d);
+    if (!x || !y || nrhs < 1) throw Error(MLFMA_ERR_ARG, "bad matvec arguments");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    hipStream_t st = stream ? (hipStream_t)stream : d.stream;
+    const int64_t n = d.hp.src.n;
+    const bool fp32 = d.useFp32;
+    if (ptr_mode == MLFMA_PTR_DEVICE) {
+        d.matvec((const cplx<double>*)x, (cplx<double>*)y, nrhs, accumulate, st, fp32);
+        hipCheck(hipStreamSynchronize(st), "matvec");
+    } else if (ptr_mode == MLFMA_PTR_HOST) {
+        d.ensureRhs(nrhs);
+        const size_t bytes = (size_t)n * nrhs * sizeof(cplx<double>);
+        hipCheck(hipMemcpyAsync(d.xs, x, bytes, hipMemcpyHostToDevice, st), "x upload");
+        if (accumulate) hipCheck(hipMemcpyAsync(d.ys, y, bytes, hipMemcpyHostToDevice, st), "y upload");
+        d.matvec(d.xs, d.ys, nrhs, accumulate, st, fp32);
+        hipCheck(hipMemcpyAsync(y, d.ys, bytes, hipMemcpyDeviceToHost, st), "y download");
+        hipCheck(hipStreamSynchronize(st), "matvec");
+    } else
+        throw Error(MLFMA_ERR_ARG, "bad ptr_mode");
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_stage_times(const mlfma_plan* plan, double* ms) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    needDevice(d);
+    if (!d.timesValid) throw Error(MLFMA_ERR_STATE, "no matvec has run");
+    auto el = [&](int a, int b) {
+        float t = 0;
+        hipCheck(hipEventElapsedTime(&t, d.ev[a], d.ev[b]), "hipEventElapsedTime");
+        return (double)t;
+    };
+    ms[0] = el(0, 2);  // gather + S2M
+    ms[1] = el(2, 3);  // M2M
+    ms[2] = el(3, 4);  // M2L
+    ms[3] = el(4, 5);  // L2L
+    ms[4] = el(6, 7);  // L2T
+    ms[5] = el(5, 6);  // near
+    ms[6] = el(0, 8);  // total
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_level_boxes(const mlfma_plan* plan, int32_t level, int64_t* nbox, int32_t* coords, int32_t* is_leaf,
+                      int32_t* nsrc, int64_t cap) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    const Tree& t = d.hp.tree;
+    if (level < 0 || level > t.maxLevel) throw Error(MLFMA_ERR_ARG, "level out of range");
+    const int b = t.lvlBeg[level], e = t.lvlBeg[level + 1];
+    *nbox = e - b;
+    if (coords || is_leaf || nsrc) {
+        if (cap < e - b) throw Error(MLFMA_ERR_ARG, "capacity too small");
+        for (int i = b; i < e; ++i) {
+            const Node& nd = t.nodes[i];
+            if (coords) {
+                coords[3 * (i - b)] = nd.ix;
+                coords[3 * (i - b) + 1] = nd.iy;
+                coords[3 * (i - b) + 2] = nd.iz;
+            }
+            if (is_leaf) is_leaf[i - b] = nd.leaf() ? 1 : 0;
+            if (nsrc) nsrc[i - b] = (int32_t)nd.count();
+        }
+    }
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_level_pairs(const mlfma_plan* plan, int32_t level, int32_t kind, int64_t* npairs, int32_t* pairs,
+                      int32_t* src_level, int64_t cap) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    const Tree& t = d.hp.tree;
+    const Lists& L = d.hp.lists;
+    if (level < 0 || level > t.maxLevel) throw Error(MLFMA_ERR_ARG, "level out of range");
+    const int b = t.lvlBeg[level];
+    std::vector<std::array<int, 3>> out;
+    auto rel = [&](int node) { return node - t.lvlBeg[t.nodes[node].level]; };
+    if (kind == 0) {
+        for (int i = b; i < t.lvlBeg[level + 1]; ++i)
+            for (int s : L.iList[i]) out.push_back({i - b, rel(s), t.nodes[s].level});
+    } else if (kind == 1) {
+        for (auto& p : L.nearPairs)
+            if (t.nodes[p.first].level == level) out.push_back({rel(p.first), rel(p.second), t.nodes[p.second].level});
+    } else if (kind == 2) {
+        for (auto& p : L.nonNearPairs)
+            if (t.nodes[p.first].level == level) out.push_back({rel(p.first), rel(p.second), t.nodes[p.second].level});
+    } else
+        throw Error(MLFMA_ERR_ARG, "bad pair kind");
+    *npairs = (int64_t)out.size();
+    if (pairs) {
+        if (cap < (int64_t)out.size()) throw Error(MLFMA_ERR_ARG, "capacity too small");
+        for (size_t q = 0; q < out.size(); ++q) {
+            pairs[2 * q] = out[q][0];
+            pairs[2 * q + 1] = out[q][1];
+            if (src_level) src_level[q] = out[q][2];
+        }
+    }
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_get_coeffs(const mlfma_plan* plan, int32_t level, int32_t which, double* out, int64_t cap) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    needDevice(d);
+    const Tree& t = d.hp.tree;
+    if (level < 2 || level > t.maxLevel) throw Error(MLFMA_ERR_ARG, "coefficients exist for levels >= 2");
+    const LevelBufs& L = d.levels[level];
+    const int b = t.lvlBeg[level], e = t.lvlBeg[level + 1];
+    const int64_t per = (int64_t)L.ndir * 2;
+    if (cap < (int64_t)(e - b) * per * 2) throw Error(MLFMA_ERR_ARG, "capacity too small");
+    std::vector<cplx<double>> buf((size_t)d.hp.boxes[level] * per);
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    hipCheck(hipStreamSynchronize(d.stream), "sync");
+    if (d.useFp32) {
+        std::vector<cplx<float>> f(buf.size());
+        const cplx<float>* src = (which == 0 ? d.M32 : d.L32) + L.Moff;
+        if (!f.empty()) hipCheck(hipMemcpy(f.data(), src, f.size() * sizeof(cplx<float>), hipMemcpyDeviceToHost), "copy");
+        for (size_t i = 0; i < f.size(); ++i) buf[i] = cplx<double>{f[i].re, f[i].im};
+    } else {
+        const cplx<double>* src = (which == 0 ? d.M64 : d.L64) + L.Moff;
+        if (!buf.empty()) hipCheck(hipMemcpy(buf.data(), src, buf.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost), "copy");
+    }
+    std::memset(out, 0, sizeof(double) * (size_t)(e - b) * per * 2);
+    for (int i = b; i < e; ++i) {
+        const Node& nd = t.nodes[i];
+        if (nd.slot < 0) continue;
+        std::memcpy(out + (size_t)(i - b) * per * 2, buf.data() + (size_t)nd.slot * per, per * sizeof(cplx<double>));
+    }
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_get_translation(const mlfma_plan* plan, int32_t level, double* out, int64_t cap) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    needDevice(d);
+    if (level < 2 || level > d.hp.tree.maxLevel) throw Error(MLFMA_ERR_ARG, "translation tables exist for levels >= 2");
+    const LevelBufs& L = d.levels[level];
+    if (cap < (int64_t)316 * L.ndir * 2) throw Error(MLFMA_ERR_ARG, "capacity too small");
+    std::vector<cplx<double>> T((size_t)316 * L.ndir);
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    hipCheck(hipMemcpy(T.data(), L.T64, T.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost), "copy");
+    for (int o = 0; o < 316; ++o)
+        for (int q = 0; q < L.ndir; ++q) {
+            const cplx<double> v = T[(size_t)o * L.ndir + q];
+            out[2 * ((size_t)o * L.ndir + q)] = v.re / L.dirW[q];
+            out[2 * ((size_t)o * L.ndir + q) + 1] = v.im / L.dirW[q];
+        }
+    return MLFMA_OK;
+    API_END
+}
+
+}  // extern "C"

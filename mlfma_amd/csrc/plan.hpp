@@ -1,0 +1,143 @@
+// plan.hpp — device-resident plan of the MLFMA matvec (one plan = one device,
+// one stream).  Layout in HBM:
+//   * sources and vectors in tree order (perm maps tree position -> unknown);
+//   * radiation patterns: per far-field leaf, [source][direction][theta|phi];
+//   * multipole / local fields: per level >= 2, [box slot][direction][theta|phi];
+//   * translation operators: per level, [316 offsets][direction], weights folded;
+//   * near field: per leaf row panel, column-major values (C k folded).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <complex>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "kernels.hpp"
+
+namespace mlfma {
+
+extern thread_local std::string g_lastError;
+void hipCheck(hipError_t e, const char* what);
+
+struct LevelBufs {
+    int nth = 0, nph = 0, ndir = 0, nps = 0;
+    std::vector<double> khat, dirW;  // host copies
+    int64_t Moff = 0;                // first complex of this level in the field arrays
+    cplx<double>* T64 = nullptr;
+    cplx<float>* T32 = nullptr;
+    cplx<double>*phUp64 = nullptr, *phDn64 = nullptr;
+    cplx<float>*phUp32 = nullptr, *phDn32 = nullptr;
+    // interpolation tables (this level as the coarse one)
+    int32_t *thNear = nullptr, *phNear = nullptr;
+    double *thW = nullptr, *phW = nullptr;
+    int32_t *atPhPtr = nullptr, *atPhSrc = nullptr, *atThPtr = nullptr, *atThSrc = nullptr, *atThFlip = nullptr;
+    double *atPhW = nullptr, *atThW = nullptr;
+    // M2M into this level
+    int nstem = 0;
+    int32_t *stemSlot = nullptr, *mPtr = nullptr, *mChild = nullptr, *mOct = nullptr;
+    // M2L targets of this level
+    int ntgt = 0;
+    int32_t *tgtSlot = nullptr, *lPtr = nullptr, *lSrc = nullptr, *lOff = nullptr;
+    int nstale = 0;
+    int32_t* staleSlots = nullptr;
+    // L2L into this level
+    int nl2l = 0;
+    int32_t *l2lChild = nullptr, *l2lParent = nullptr, *l2lOct = nullptr;
+
+    template <typename R>
+    const cplx<R>* T() const;
+    template <typename R>
+    const cplx<R>* phUp() const;
+    template <typename R>
+    const cplx<R>* phDn() const;
+};
+template <>
+inline const cplx<double>* LevelBufs::T<double>() const { return T64; }
+template <>
+inline const cplx<float>* LevelBufs::T<float>() const { return T32; }
+template <>
+inline const cplx<double>* LevelBufs::phUp<double>() const { return phUp64; }
+template <>
+inline const cplx<float>* LevelBufs::phUp<float>() const { return phUp32; }
+template <>
+inline const cplx<double>* LevelBufs::phDn<double>() const { return phDn64; }
+template <>
+inline const cplx<float>* LevelBufs::phDn<float>() const { return phDn32; }
+
+struct DevicePlan {
+    HostPlan hp;
+    bool hostOnly = true;
+    int device = -1;
+    bool useFp32 = false;
+    hipStream_t stream = nullptr;
+    std::array<hipEvent_t, 9> ev{};
+    bool timesValid = false;
+    std::vector<void*> allocs;
+    int64_t bytesHeld = 0;
+
+    SrcDev src{};
+    int64_t* perm = nullptr;
+    std::vector<LevelBufs> levels;
+    LevelDev* levelDev = nullptr;
+    int64_t fieldLen = 0;
+    cplx<double>*M64 = nullptr, *L64 = nullptr, *inner64 = nullptr, *rp64 = nullptr;
+    cplx<float>*M32 = nullptr, *L32 = nullptr, *inner32 = nullptr, *rp32 = nullptr;
+    LeafItem* leafItems = nullptr;
+    int nLeafItems = 0, maxLeafNdir = 0;
+    PanelItem* panels = nullptr;
+    int nPanels = 0;
+    int64_t* panelCols = nullptr;
+    cplx<double>* nearVal = nullptr;
+    cplx<double> Ck{0, 0};
+    std::vector<std::complex<double>> Vin;  // excitation, input order
+    cplx<double>*xt = nullptr, *yt = nullptr, *xs = nullptr, *ys = nullptr;
+    int rhsCap = 0;
+
+    ~DevicePlan();
+    void* alloc(size_t bytes);
+    void release(void* p);
+    template <typename T>
+    T* upload(const std::vector<T>& v);
+    void build();
+    void ensureRhs(int nrhs);
+    template <typename R>
+    void farField(const cplx<double>* x, cplx<double>* y, hipStream_t st, bool timed);
+    void applyTree(const cplx<double>* x, cplx<double>* y, hipStream_t st, bool fp32, bool timed);
+    void matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int accumulate, hipStream_t st, bool fp32);
+
+    template <typename R>
+    cplx<R>* fieldM();
+    template <typename R>
+    cplx<R>* fieldL();
+    template <typename R>
+    cplx<R>* scratch();
+    template <typename R>
+    const cplx<R>* radpat();
+};
+template <>
+inline cplx<double>* DevicePlan::fieldM<double>() { return M64; }
+template <>
+inline cplx<float>* DevicePlan::fieldM<float>() { return M32; }
+template <>
+inline cplx<double>* DevicePlan::fieldL<double>() { return L64; }
+template <>
+inline cplx<float>* DevicePlan::fieldL<float>() { return L32; }
+template <>
+inline cplx<double>* DevicePlan::scratch<double>() { return inner64; }
+template <>
+inline cplx<float>* DevicePlan::scratch<float>() { return inner32; }
+template <>
+inline const cplx<double>* DevicePlan::radpat<double>() { return rp64; }
+template <>
+inline const cplx<float>* DevicePlan::radpat<float>() { return rp32; }
+
+}  // namespace mlfma
+
+// Opaque handle of the C-ABI.
+struct mlfma_plan {
+    std::unique_ptr<mlfma::DevicePlan> d;
+};

@@ -1,0 +1,543 @@
+// solver.hip — GMRES on the device (Arnoldi with wave-reduction dot products),
+// far field and the dense direct matvec.  Reference: src/solver.cpp:5-146.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace mlfma {
+
+constexpr int KB = 256;
+
+__device__ __forceinline__ double waveSum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide sum of a complex pair; result valid in thread 0.
+__device__ __forceinline__ void blockSum2(double& a, double& b) {
+    __shared__ double sa[KB / 64], sb[KB / 64];
+    a = waveSum(a);
+    b = waveSum(b);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = 0;
+        b = 0;
+        for (int q = 0; q < KB / 64; ++q) {
+            a += sa[q];
+            b += sb[q];
+        }
+    }
+}
+
+// partial[i * nb + block] = sum over the block's rows of conj(Q_i) w, i < ncol
+__global__ void kDotPartial(const cplx<double>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ w,
+                            int64_t n, cplx<double>* __restrict__ partial) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    const int nb = gridDim.x;
+    cplx<double> wv{0, 0};
+    if (r < n) wv = w[r];
+    for (int i = 0; i < ncol; ++i) {
+        double re = 0, im = 0;
+        if (r < n) {
+            const cplx<double> q = Q[(int64_t)i * ld + r];
+            re = q.re * wv.re + q.im * wv.im;
+            im = q.re * wv.im - q.im * wv.re;
+        }
+        blockSum2(re, im);
+        if (threadIdx.x == 0) partial[(int64_t)i * nb + blockIdx.x] = cplx<double>{re, im};
+    }
+}
+
+// h[i] (+)= sum_b partial[i * nb + b]; one block per column
+__global__ void kReduce(const cplx<double>* __restrict__ partial, int nb, cplx<double>* __restrict__ h, int add) {
+    const int i = blockIdx.x;
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < nb; b += KB) {
+        re += partial[(int64_t)i * nb + b].re;
+        im += partial[(int64_t)i * nb + b].im;
+    }
+    blockSum2(re, im);
+    if (threadIdx.x == 0) {
+        if (add) {
+            h[i].re += re;
+            h[i].im += im;
+        } else
+            h[i] = cplx<double>{re, im};
+    }
+}
+
+// w -= Q h (ncol columns); partial[block] = sum |w|^2 of the block when norm != 0
+__global__ void kUpdate(const cplx<double>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ h,
+                        cplx<double>* __restrict__ w, int64_t n, cplx<double>* __restrict__ partial, int norm) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    double s2 = 0;
+    if (r < n) {
+        cplx<double> a = w[r];
+        for (int i = 0; i < ncol; ++i) {
+            const cplx<double> q = Q[(int64_t)i * ld + r];
+            const cplx<double> c = h[i];
+            a.re -= q.re * c.re - q.im * c.im;
+            a.im -= q.re * c.im + q.im * c.re;
+        }
+        w[r] = a;
+        s2 = a.re * a.re + a.im * a.im;
+    }
+    if (norm) {
+        double z = 0;
+        blockSum2(s2, z);
+        if (threadIdx.x == 0) partial[blockIdx.x] = cplx<double>{s2, 0.0};
+    }
+}
+
+// out = in / s (s real, on the host)
+__global__ void kScale(const cplx<double>* __restrict__ in, double s, cplx<double>* __restrict__ out, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    if (r < n) out[r] = cplx<double>{in[r].re / s, in[r].im / s};
+}
+
+// x = (accumulate ? x : 0) + Q y
+__global__ void kCombine(const cplx<double>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ y,
+                         cplx<double>* __restrict__ x, int64_t n, int accumulate) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    if (r >= n) return;
+    cplx<double> a = accumulate ? x[r] : cplx<double>{0, 0};
+    for (int i = 0; i < ncol; ++i) {
+        const cplx<double> q = Q[(int64_t)i * ld + r];
+        const cplx<double> c = y[i];
+        a.re += q.re * c.re - q.im * c.im;
+        a.im += q.re * c.im + q.im * c.re;
+    }
+    x[r] = a;
+}
+
+// r = b - a
+__global__ void kSub(const cplx<double>* __restrict__ b, const cplx<double>* __restrict__ a, cplx<double>* __restrict__ r,
+                     int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x;
+    if (i < n) r[i] = cplx<double>{b[i].re - a[i].re, b[i].im - a[i].im};
+}
+
+// Krylov workspace and helpers bound to one plan
+struct Krylov {
+    DevicePlan& d;
+    hipStream_t st;
+    int64_t n;
+    int nb;
+    cplx<double>* partial = nullptr;
+    cplx<double>* h = nullptr;  // device column
+    Krylov(DevicePlan& dp, hipStream_t s, int maxcol) : d(dp), st(s), n(dp.hp.src.n) {
+        nb = (int)((n + KB - 1) / KB);
+        partial = (cplx<double>*)d.alloc((size_t)nb * (maxcol + 2) * sizeof(cplx<double>));
+        h = (cplx<double>*)d.alloc((size_t)2 * (maxcol + 2) * sizeof(cplx<double>));
+    }
+    ~Krylov() {
+        d.release(partial);
+        d.release(h);
+    }
+    double norm(const cplx<double>* v) {
+        hipLaunchKernelGGL(kUpdate, dim3(nb), dim3(KB), 0, st, (const cplx<double>*)nullptr, (int64_t)0, 0,
+                           (const cplx<double>*)nullptr, const_cast<cplx<double>*>(v), n, partial, 1);
+        hipLaunchKernelGGL(kReduce, dim3(1), dim3(KB), 0, st, partial, nb, h, 0);
+        cplx<double> r;
+        hipCheck(hipMemcpyAsync(&r, h, sizeof(r), hipMemcpyDeviceToHost, st), "norm copy");
+        hipCheck(hipStreamSynchronize(st), "norm");
+        return std::sqrt(r.re);
+    }
+    // classical Gram-Schmidt applied twice against Q[:, 0..ncol); returns h and |w|
+    void orthogonalize(const cplx<double>* Q, int ncol, cplx<double>* w, std::vector<std::complex<double>>& hout,
+                       double& wnorm) {
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL(kDotPartial, dim3(nb), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)w, n, partial);
+            hipLaunchKernelGGL(kReduce, dim3(ncol), dim3(KB), 0, st, partial, nb, h + (pass ? ncol + 1 : 0), 0);
+            hipLaunchKernelGGL(kUpdate, dim3(nb), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)(h + (pass ? ncol + 1 : 0)),
+                               w, n, partial + (size_t)nb * (ncol + 1), pass);
+        }
+        // |w|^2 of the final vector is in partial[nb*(ncol+1) ...]
+        hipLaunchKernelGGL(kReduce, dim3(1), dim3(KB), 0, st, partial + (size_t)nb * (ncol + 1), nb, h + ncol, 0);
+        std::vector<cplx<double>> hh(2 * ncol + 1);
+        hipCheck(hipMemcpyAsync(hh.data(), h, hh.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "h copy");
+        hipCheck(hipStreamSynchronize(st), "arnoldi");
+        hout.resize(ncol);
+        for (int i = 0; i < ncol; ++i) hout[i] = std::complex<double>(hh[i].re + hh[ncol + 1 + i].re, hh[i].im + hh[ncol + 1 + i].im);
+        wnorm = std::sqrt(hh[ncol].re);
+    }
+    void scale(const cplx<double>* in, double s, cplx<double>* out) {
+        hipLaunchKernelGGL(kScale, dim3(nb), dim3(KB), 0, st, in, s, out, n);
+    }
+    void combine(const cplx<double>* Q, int ncol, const std::vector<std::complex<double>>& y, cplx<double>* x, int acc) {
+        std::vector<cplx<double>> yy(ncol);
+        for (int i = 0; i < ncol; ++i) yy[i] = cplx<double>{y[i].real(), y[i].imag()};
+        hipCheck(hipMemcpyAsync(h, yy.data(), ncol * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "y upload");
+        hipLaunchKernelGGL(kCombine, dim3(nb), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)h, x, n, acc);
+        hipCheck(hipStreamSynchronize(st), "combine");
+    }
+    void sub(const cplx<double>* b, const cplx<double>* a, cplx<double>* r) {
+        hipLaunchKernelGGL(kSub, dim3(nb), dim3(KB), 0, st, b, a, r, n);
+    }
+};
+
+using zc = std::complex<double>;
+
+// Back substitution with the upper-triangular R stored by columns (H[j][i]).
+static std::vector<zc> backSolve(const std::vector<std::vector<zc>>& H, const std::vector<zc>& g, int m) {
+    std::vector<zc> y(m);
+    for (int i = m - 1; i >= 0; --i) {
+        zc s = g[i];
+        for (int j = i + 1; j < m; ++j) s -= H[j][i] * y[j];
+        y[i] = s / H[i][i];
+    }
+    return y;
+}
+
+struct SolveOut {
+    int iters = 0;
+    int status = MLFMA_OK;
+    double est = 0;
+};
+
+// Reference-compatible GMRES (src/solver.cpp:116-146): unrestarted, complex
+// Givens rotation t = sqrt(z^2 + w^2) without conjugates (src/math.h:168-172),
+// stops when |g_{k+1}|/g0 <= tol or after maxit iterations.
+static SolveOut gmresCompat(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int maxit, double tol,
+                            double* hist) {
+    SolveOut out;
+    const int64_t n = d.hp.src.n;
+    cplx<double>* Q = (cplx<double>*)d.alloc((size_t)n * (maxit + 1) * sizeof(cplx<double>));
+    cplx<double>* w = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    const double g0 = K.norm(b);
+    if (!(g0 > 0) || !std::isfinite(g0)) {
+        hipCheck(hipMemsetAsync(x, 0, n * sizeof(cplx<double>), K.st), "zero x");
+        hipCheck(hipStreamSynchronize(K.st), "sync");
+        d.release(Q);
+        d.release(w);
+        out.status = MLFMA_ERR_BREAKDOWN;
+        return out;
+    }
+    K.scale(b, g0, Q);
+    std::vector<std::vector<zc>> H;
+    std::vector<zc> g(maxit + 1, 0.0), cs(maxit), sn(maxit);
+    g[0] = g0;
+    int it = 0;
+    for (;;) {
+        const int j = it;
+        d.applyTree(Q + (size_t)j * n, w, K.st, false, false);
+        std::vector<zc> h;
+        double hn;
+        K.orthogonalize(Q, j + 1, w, h, hn);
+        K.scale(w, hn, Q + (size_t)(j + 1) * n);
+        h.push_back(hn);
+        for (int i = 0; i < j; ++i) {
+            const zc a = cs[i] * h[i] + sn[i] * h[i + 1];
+            h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+            h[i] = a;
+        }
+        const zc t = std::sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+        cs[j] = h[j] / t;
+        sn[j] = h[j + 1] / t;
+        h[j] = cs[j] * h[j] + sn[j] * h[j + 1];
+        h[j + 1] = 0.0;
+        H.push_back(h);
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        ++it;
+        const double res = std::abs(g[it]) / g0;
+        if (hist) hist[it - 1] = res;
+        out.est = res;
+        if (!std::isfinite(res)) {
+            out.status = MLFMA_ERR_BREAKDOWN;
+            break;
+        }
+        if (!(res > tol && it < maxit)) break;
+    }
+    out.iters = it;
+    if (out.status == MLFMA_OK && out.est > tol) out.status = MLFMA_ERR_NOT_CONVERGED;
+    const std::vector<zc> y = backSolve(H, g, it);
+    K.combine(Q, it, y, x, 0);
+    d.release(Q);
+    d.release(w);
+    return out;
+}
+
+// Restarted GMRES(m) with unitary complex Givens rotations; convergence is
+// checked on the true residual at every restart.  With fp32 the far field of
+// every matvec runs in FP32 (used as the inner solver of the refinement loop).
+static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit,
+                             double tol, bool fp32, bool x0zero, double* hist, int histOff) {
+    SolveOut out;
+    const int64_t n = d.hp.src.n;
+    m = std::max(1, std::min(m, maxit));
+    cplx<double>* Q = (cplx<double>*)d.alloc((size_t)n * (m + 1) * sizeof(cplx<double>));
+    cplx<double>* w = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    const double bn = K.norm(b);
+    if (!(bn > 0) || !std::isfinite(bn)) {
+        hipCheck(hipMemsetAsync(x, 0, n * sizeof(cplx<double>), K.st), "zero x");
+        hipCheck(hipStreamSynchronize(K.st), "sync");
+        d.release(Q);
+        d.release(w);
+        out.status = bn == 0 ? MLFMA_OK : MLFMA_ERR_NONFINITE;
+        return out;
+    }
+    if (x0zero) hipCheck(hipMemsetAsync(x, 0, n * sizeof(cplx<double>), K.st), "zero x");
+    int total = 0;
+    double res = 1.0;
+    for (;;) {
+        // r = b - A x
+        double beta;
+        if (x0zero && total == 0) {
+            hipCheck(hipMemcpyAsync(w, b, n * sizeof(cplx<double>), hipMemcpyDeviceToDevice, K.st), "copy");
+            beta = bn;
+        } else {
+            d.applyTree(x, w, K.st, fp32, false);
+            K.sub(b, w, w);
+            beta = K.norm(w);
+        }
+        res = beta / bn;
+        out.est = res;
+        if (res <= tol || total >= maxit) break;
+        K.scale(w, beta, Q);
+        std::vector<std::vector<zc>> H;
+        std::vector<zc> g(m + 1, 0.0), cs(m), sn(m);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m && total < maxit; ++j) {
+            d.applyTree(Q + (size_t)j * n, w, K.st, fp32, false);
+            std::vector<zc> h;
+            double hn;
+            K.orthogonalize(Q, j + 1, w, h, hn);
+            h.push_back(hn);
+            for (int i = 0; i < j; ++i) {
+                const zc a = std::conj(cs[i]) * h[i] + std::conj(sn[i]) * h[i + 1];
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                h[i] = a;
+            }
+            const double rho = std::sqrt(std::norm(h[j]) + std::norm(h[j + 1]));
+            if (rho == 0) {
+                out.status = MLFMA_ERR_BREAKDOWN;
+                break;
+            }
+            cs[j] = h[j] / rho;
+            sn[j] = h[j + 1] / rho;
+            h[j] = rho;
+            h[j + 1] = 0.0;
+            H.push_back(h);
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = std::conj(cs[j]) * g[j];
+            ++total;
+            const double est = std::abs(g[j + 1]) / bn;
+            if (hist) hist[histOff + total - 1] = est;
+            if (est <= tol || hn == 0) {
+                ++j;
+                break;
+            }
+            K.scale(w, hn, Q + (size_t)(j + 1) * n);
+        }
+        if (j > 0) {
+            const std::vector<zc> y = backSolve(H, g, j);
+            K.combine(Q, j, y, x, 1);
+        }
+        if (out.status != MLFMA_OK) break;
+    }
+    out.iters = total;
+    if (out.status == MLFMA_OK && res > tol) out.status = MLFMA_ERR_NOT_CONVERGED;
+    d.release(Q);
+    d.release(w);
+    return out;
+}
+
+// Mixed precision: FP64 iterative refinement (true residual with the FP64
+// matvec) around inner GMRES(m) solves whose far field is FP32.
+static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit, double tol,
+                        double* hist) {
+    SolveOut out;
+    const int64_t n = d.hp.src.n;
+    cplx<double>* r = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    cplx<double>* dx = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    cplx<double>* ax = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    const double bn = K.norm(b);
+    hipCheck(hipMemsetAsync(x, 0, n * sizeof(cplx<double>), K.st), "zero x");
+    int total = 0;
+    double res = 1.0;
+    const double inner = 1e-4;
+    for (int outer = 0; outer < 20; ++outer) {
+        if (outer == 0) {
+            hipCheck(hipMemcpyAsync(r, b, n * sizeof(cplx<double>), hipMemcpyDeviceToDevice, K.st), "copy");
+            res = 1.0;
+        } else {
+            d.applyTree(x, ax, K.st, false, false);
+            K.sub(b, ax, r);
+            res = K.norm(r) / bn;
+        }
+        out.est = res;
+        if (res <= tol || total >= maxit) break;
+        const double itol = std::max(inner, 0.5 * tol / res);
+        SolveOut s = gmresRestart(d, K, r, dx, m, maxit - total, itol, true, true, hist, total);
+        total += s.iters;
+        // x += dx
+        std::vector<zc> one{1.0};
+        K.combine(dx, 1, one, x, 1);
+        if (s.status == MLFMA_ERR_BREAKDOWN) {
+            out.status = s.status;
+            break;
+        }
+    }
+    out.iters = total;
+    if (out.status == MLFMA_OK && res > tol) out.status = MLFMA_ERR_NOT_CONVERGED;
+    d.release(r);
+    d.release(dx);
+    d.release(ax);
+    return out;
+}
+
+}  // namespace mlfma
+
+using namespace mlfma;
+
+#define API_BEGIN try {
+#define API_END                                  \
+    }                                            \
+    catch (const Error& e) {                     \
+        g_lastError = e.what();                  \
+        return e.code;                           \
+    }                                            \
+    catch (const std::bad_alloc&) {              \
+        g_lastError = "host allocation failed";  \
+        return MLFMA_ERR_OOM;                    \
+    }                                            \
+    catch (const std::exception& e) {            \
+        g_lastError = e.what();                  \
+        return MLFMA_ERR_ARG;                    \
+    }
+
+static DevicePlan& Pl(mlfma_plan* p) {
+    if (!p || !p->d) throw Error(MLFMA_ERR_ARG, "null plan");
+    if (p->d->hostOnly) throw Error(MLFMA_ERR_STATE, "host-only plan (device = -1) has no device data");
+    return *p->d;
+}
+
+extern "C" {
+
+int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_opts* opts, mlfma_solve_report* rep,
+                double* hist) {
+    API_BEGIN
+    DevicePlan& d = Pl(plan);
+    if (!x || !opts) throw Error(MLFMA_ERR_ARG, "null argument");
+    if (!opts->use_voltage && !b) throw Error(MLFMA_ERR_ARG, "b is NULL and use_voltage = 0");
+    if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
+    if (opts->method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t n = d.hp.src.n;
+    hipStream_t st = d.stream;
+    d.ensureRhs(1);
+    // b (input order) -> tree order
+    std::vector<cplx<double>> bh(n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (opts->use_voltage) {
+            bh[i].re = -d.Vin[i].real();
+            bh[i].im = -d.Vin[i].imag();
+        } else {
+            bh[i].re = b[2 * i];
+            bh[i].im = b[2 * i + 1];
+        }
+        if (!std::isfinite(bh[i].re) || !std::isfinite(bh[i].im)) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
+    }
+    hipCheck(hipMemcpyAsync(d.xs, bh.data(), n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "b upload");
+    cplx<double>* bt = (cplx<double>*)d.alloc(n * sizeof(cplx<double>));
+    cplx<double>* xtree = (cplx<double>*)d.alloc(n * sizeof(cplx<double>));
+    launchGather(d.xs, d.perm, bt, n, 1, st);
+    Krylov K(d, st, std::max(opts->maxit, opts->restart) + 2);
+    SolveOut o;
+    if (opts->method == MLFMA_GMRES_COMPAT)
+        o = gmresCompat(d, K, bt, xtree, opts->maxit, opts->tol, hist);
+    else if (opts->method == MLFMA_GMRES_RESTART)
+        o = gmresRestart(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, false, true,
+                         hist, 0);
+    else if (opts->method == MLFMA_FGMRES_IR)
+        o = gmresIR(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, hist);
+    else
+        throw Error(MLFMA_ERR_ARG, "unknown solve method");
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // true residual with the FP64 matvec
+    d.applyTree(xtree, d.yt, st, false, false);
+    K.sub(bt, d.yt, d.yt);
+    const double rn = K.norm(d.yt), bn = K.norm(bt);
+    launchScatter(xtree, d.perm, d.ys, n, 1, 0, st);
+    hipCheck(hipMemcpyAsync(x, d.ys, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "x download");
+    hipCheck(hipStreamSynchronize(st), "solve");
+    d.release(bt);
+    d.release(xtree);
+    if (rep) {
+        rep->iterations = o.iters;
+        rep->status = o.status;
+        rep->resid_est = o.est;
+        rep->resid_true = bn > 0 ? rn / bn : 0.0;
+        rep->seconds = secs;
+    }
+    return o.status == MLFMA_ERR_BREAKDOWN ? MLFMA_ERR_BREAKDOWN : MLFMA_OK;
+    API_END
+}
+
+int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E) {
+    API_BEGIN
+    DevicePlan& d = Pl(plan);
+    if (!I || !dirs || !E || ndir < 0) throw Error(MLFMA_ERR_ARG, "bad farfield arguments");
+    if (ndir == 0) return MLFMA_OK;
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const int64_t n = d.hp.src.n;
+    hipStream_t st = d.stream;
+    d.ensureRhs(1);
+    hipCheck(hipMemcpyAsync(d.xs, I, n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "I upload");
+    launchGather(d.xs, d.perm, d.xt, n, 1, st);
+    double* dd = (double*)d.alloc((size_t)ndir * 3 * sizeof(double));
+    cplx<double>* dE = (cplx<double>*)d.alloc((size_t)ndir * 3 * sizeof(cplx<double>));
+    hipCheck(hipMemcpyAsync(dd, dirs, (size_t)ndir * 3 * sizeof(double), hipMemcpyHostToDevice, st), "dirs upload");
+    launchFarfield(d.src, n, d.xt, dd, (int)ndir, d.hp.k, d.Ck, dE, st);
+    hipCheck(hipGetLastError(), "farfield launch");
+    hipCheck(hipMemcpyAsync(E, dE, (size_t)ndir * 3 * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "E download");
+    hipCheck(hipStreamSynchronize(st), "farfield");
+    d.release(dd);
+    d.release(dE);
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t ptr_mode, void* stream) {
+    API_BEGIN
+    DevicePlan& d = Pl(plan);
+    if (!x || !y) throw Error(MLFMA_ERR_ARG, "null argument");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const int64_t n = d.hp.src.n;
+    hipStream_t st = stream ? (hipStream_t)stream : d.stream;
+    d.ensureRhs(1);
+    const cplx<double>* xin = (const cplx<double>*)x;
+    cplx<double>* yout = (cplx<double>*)y;
+    if (ptr_mode == MLFMA_PTR_HOST) {
+        hipCheck(hipMemcpyAsync(d.xs, x, n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "x upload");
+        xin = d.xs;
+        yout = d.ys;
+    } else if (ptr_mode != MLFMA_PTR_DEVICE)
+        throw Error(MLFMA_ERR_ARG, "bad ptr_mode");
+    launchGather(xin, d.perm, d.xt, n, 1, st);
+    launchDirect(d.src, d.xt, n, d.hp.k, d.Ck, d.yt, st);
+    hipCheck(hipGetLastError(), "direct launch");
+    launchScatter(d.yt, d.perm, yout, n, 1, 0, st);
+    if (ptr_mode == MLFMA_PTR_HOST)
+        hipCheck(hipMemcpyAsync(y, d.ys, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "y download");
+    hipCheck(hipStreamSynchronize(st), "direct");
+    return MLFMA_OK;
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+"""Python mirror of the C-ABI (``include/mlfma_c.h``).
+
+``Plan`` follows the reference's setup/matvec/solve contract
+(src/mlfma.cpp:21-82): building a plan runs the whole setup; ``matvec`` is
+``Solver::evalRvec`` (src/solver.cpp:41-57); ``solve`` is ``Solver::solve``
+(src/solver.cpp:116-146); ``farfield`` generalises ``Node::printFarSols``
+(src/fmm/node.cpp:147-178).  Every call goes to the HIP library; a missing
+library raises."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+
+
+class MlfmaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{A.ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _check(lib, rc):
+    if rc != 0:
+        raise MlfmaError(rc, lib.mlfma_last_error().decode())
+
+
+def _cvec(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    if n is not None and a.shape[0] != n:
+        raise ValueError("vector length does not match the number of unknowns")
+    return a
+
+
+class Plan:
+    def __init__(self, geom: A.GeometryArrays, pw: A.PlaneWave | None = None, cfg: A.Config | None = None,
+                 **cfg_kw):
+        self.lib = A.load_library()
+        self.geom = geom
+        self.pw = pw or A.make_planewave()
+        self.cfg = cfg or A.make_config(**cfg_kw)
+        self._h = C.c_void_p()
+        _check(self.lib, self.lib.mlfma_plan_create(C.byref(geom.struct), C.byref(self.pw), C.byref(self.cfg),
+                                                    C.byref(self._h)))
+        self.n = geom.n
+
+    def close(self):
+        if self._h and self._h.value:
+            self.lib.mlfma_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- info
+    def info(self) -> dict:
+        i = A.Info()
+        _check(self.lib, self.lib.mlfma_plan_info(self._h, C.byref(i)))
+        ml = i.max_level
+        return {
+            "n": i.n, "max_level": ml, "num_nodes": i.num_nodes, "num_leaves": i.num_leaves,
+            "nth": list(i.nth[: ml + 1]), "nph": list(i.nph[: ml + 1]), "boxes": list(i.boxes[: ml + 1]),
+            "m2l_pairs": list(i.m2l_pairs[: ml + 1]), "near_nnz": i.near_nnz,
+            "radpat_entries": i.radpat_entries, "stale_boxes": i.stale_boxes,
+            "truncated_l2l": i.truncated_l2l, "device_bytes": i.device_bytes,
+        }
+
+    def voltage(self) -> np.ndarray:
+        v = np.zeros(self.n, dtype=np.complex128)
+        _check(self.lib, self.lib.mlfma_voltage(self._h, v.ctypes.data_as(A.dptr)))
+        return v
+
+    # ---------------------------------------------------------------- matvec
+    def matvec(self, x, y=None, accumulate=False, stream=None):
+        """y = Z x (or y += Z x).  numpy arrays (n,) or (nrhs, n) go through host
+        buffers; torch CUDA tensors (complex128, contiguous) are used in place."""
+        try:
+            import torch
+            is_t = isinstance(x, torch.Tensor)
+        except Exception:  # torch absent
+            is_t = False
+        if is_t:
+            if x.dtype != torch.complex128 or not x.is_cuda or not x.is_contiguous():
+                raise ValueError("device matvec needs a contiguous complex128 CUDA tensor")
+            nrhs = 1 if x.dim() == 1 else x.shape[0]
+            if y is None:
+                y = torch.empty_like(x)
+            st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+            _check(self.lib, self.lib.mlfma_matvec(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
+                                                   nrhs, 1 if accumulate else 0, A.MLFMA_PTR_DEVICE,
+                                                   C.c_void_p(st)))
+            return y
+        x = _cvec(x)
+        nrhs = 1 if x.ndim == 1 else x.shape[0]
+        if x.shape[-1] != self.n:
+            raise ValueError("vector length does not match the number of unknowns")
+        if y is None:
+            y = np.zeros_like(x)
+        elif accumulate:
+            y = _cvec(y)
+        _check(self.lib, self.lib.mlfma_matvec(self._h, C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data), nrhs,
+                                               1 if accumulate else 0, A.MLFMA_PTR_HOST, None))
+        return y
+
+    def matvec_direct(self, x):
+        x = _cvec(x, self.n)
+        y = np.zeros_like(x)
+        _check(self.lib, self.lib.mlfma_matvec_direct(self._h, C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data),
+                                                      A.MLFMA_PTR_HOST, None))
+        return y
+
+    def stage_times(self) -> dict:
+        t = np.zeros(7)
+        _check(self.lib, self.lib.mlfma_stage_times(self._h, t.ctypes.data_as(A.dptr)))
+        return dict(zip(["S2M", "M2M", "M2L", "L2L", "L2T", "near", "total"], t.tolist()))
+
+    # ---------------------------------------------------------------- solve
+    def solve(self, b=None, method=A.MLFMA_GMRES_COMPAT, restart=50, maxit=500, tol=1e-6, history=False):
+        o = A.SolveOpts()
+        o.method = method
+        o.restart = restart
+        o.maxit = maxit
+        o.use_voltage = 1 if b is None else 0
+        o.tol = tol
+        rep = A.SolveReport()
+        x = np.zeros(self.n, dtype=np.complex128)
+        hist = np.zeros(maxit)
+        bp = None
+        if b is not None:
+            b = _cvec(b, self.n)
+            bp = b.ctypes.data_as(A.dptr)
+        _check(self.lib, self.lib.mlfma_solve(self._h, bp, x.ctypes.data_as(A.dptr), C.byref(o), C.byref(rep),
+                                              hist.ctypes.data_as(A.dptr)))
+        report = {"iterations": rep.iterations, "status": A.ERRORS.get(rep.status, rep.status),
+                  "resid_est": rep.resid_est, "resid_true": rep.resid_true, "seconds": rep.seconds}
+        if history:
+            report["history"] = hist[: rep.iterations].copy()
+        return x, report
+
+    # ---------------------------------------------------------------- far field
+    def farfield(self, I, dirs):
+        I = _cvec(I, self.n)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        E = np.zeros((len(dirs), 3), dtype=np.complex128)
+        _check(self.lib, self.lib.mlfma_farfield(self._h, I.ctypes.data_as(A.dptr), dirs.ctypes.data_as(A.dptr),
+                                                 len(dirs), E.ctypes.data_as(A.dptr)))
+        return E
+
+    # ---------------------------------------------------------------- introspection
+    def level_boxes(self, level):
+        nb = C.c_int64()
+        _check(self.lib, self.lib.mlfma_level_boxes(self._h, level, C.byref(nb), None, None, None, 0))
+        co = np.zeros((nb.value, 3), np.int32)
+        lf = np.zeros(nb.value, np.int32)
+        ns = np.zeros(nb.value, np.int32)
+        _check(self.lib, self.lib.mlfma_level_boxes(self._h, level, C.byref(nb), co.ctypes.data_as(A.iptr),
+                                                    lf.ctypes.data_as(A.iptr), ns.ctypes.data_as(A.iptr), nb.value))
+        return co, lf, ns
+
+    def level_pairs(self, level, kind):
+        npr = C.c_int64()
+        _check(self.lib, self.lib.mlfma_level_pairs(self._h, level, kind, C.byref(npr), None, None, 0))
+        pr = np.zeros((max(npr.value, 1), 2), np.int32)
+        sl = np.zeros(max(npr.value, 1), np.int32)
+        _check(self.lib, self.lib.mlfma_level_pairs(self._h, level, kind, C.byref(npr), pr.ctypes.data_as(A.iptr),
+                                                    sl.ctypes.data_as(A.iptr), pr.size))
+        return pr[: npr.value], sl[: npr.value]
+
+    def coeffs(self, level, which):
+        info = self.info()
+        nd = info["nth"][level] * info["nph"][level]
+        nb = len(self.level_boxes(level)[0])
+        out = np.zeros((nb, nd, 2), dtype=np.complex128)
+        _check(self.lib, self.lib.mlfma_get_coeffs(self._h, level, which, out.ctypes.data_as(A.dptr), out.size * 2))
+        return out
+
+    def translation(self, level):
+        info = self.info()
+        nd = info["nth"][level] * info["nph"][level]
+        T = np.zeros((316, nd), dtype=np.complex128)
+        _check(self.lib, self.lib.mlfma_get_translation(self._h, level, T.ctypes.data_as(A.dptr), T.size * 2))
+        return T

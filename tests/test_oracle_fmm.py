@@ -1,0 +1,287 @@
+"""Pins the oracle's FMM stages, sources and GMRES against closed forms,
+brute force and the reference's own test ideas (test/tInterp.cpp,
+test/tFarfield.cpp, test/tNearfield.cpp, test/tRwg.cpp).  CPU only."""
+import numpy as np
+import pytest
+from scipy.special import eval_legendre, spherical_jn, spherical_yn
+
+from mlfma_amd import _capi as A
+from mlfma_amd import geometry as G
+from tests import _oracle as O
+from tests import cases
+
+K = 2 * np.pi
+
+
+def _oracle(name, **over):
+    geom, kw = cases.make(name)
+    kw.update(over)
+    return geom, kw, O.Oracle(geom, cases.planewave_z(), A.make_config(device=-1, **kw))
+
+
+@pytest.fixture(scope="module")
+def dip():
+    return _oracle("dipole_fixed")
+
+
+def _grid_dirs(a):
+    th, ph = a["thetas"], a["phis"]
+    TH, PH = np.meshgrid(th, ph, indexing="ij")
+    return TH.ravel(), PH.ravel()
+
+
+def test_phi_interp_kat(dip):
+    # test/tInterp.cpp:28-66: <e^{i phi}, I e^{i phi}> over the parent phi grid = 2 pi
+    _, kw, o = dip
+    p = kw["interp_order"]
+    co, near = o.interp(2, 1, p)
+    fine = o.angles(3)["phis"]
+    coarse = o.angles(2)["phis"]
+    m = len(fine)
+    vals = np.exp(1j * fine)
+    out = np.array([sum(co[j, q] * vals[(near[j] + 1 - p + q) % m] for q in range(2 * p)) for j in range(len(coarse))])
+    integ = (2 * np.pi / len(coarse)) * np.sum(np.conj(np.exp(1j * coarse)) * out)
+    assert abs(integ - 2 * np.pi) <= 1e-8
+    assert np.abs(out - np.exp(1j * coarse)).max() <= 1e-8
+
+
+def test_theta_interp_kat(dip):
+    # test/tInterp.cpp:127-166: scalar cos(theta) interpolated with the reflected
+    # stencil, sum w cos I cos * 2 pi = 4 pi / 3
+    _, kw, o = dip
+    p = kw["interp_order"]
+    co, near = o.interp(2, 0, p)
+    fa, ca = o.angles(3), o.angles(2)
+    m = len(fa["thetas"])
+    vals = np.cos(fa["thetas"])
+    out = []
+    for j in range(len(ca["thetas"])):
+        s = 0
+        for q in range(2 * p):
+            i = near[j] + 1 - p + q
+            f = -i - 1 if i < 0 else (2 * m - i - 1 if i >= m else i)
+            s += co[j, q] * vals[f]
+        out.append(s)
+    integ = 2 * np.pi * np.sum(ca["weights"] * np.cos(ca["thetas"]) * np.array(out))
+    assert abs(integ - 4 * np.pi / 3) <= 1e-5
+
+
+def _tangent(a, V):
+    th, ph = _grid_dirs(a)
+    et = np.stack([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)], -1)
+    ep = np.stack([-np.sin(ph), np.cos(ph), np.zeros_like(ph)], -1)
+    return np.stack([et @ V, ep @ V], -1).astype(complex)
+
+
+def test_vector_interp_of_tangent_field(dip):
+    # the theta/phi components of a constant vector are band-limited and obey the
+    # pole rule of src/fmm/stem.cpp:150-170 (phi + pi, both components negated)
+    _, kw, o = dip
+    V = np.array([0.3, -1.1, 0.7])
+    fine, coarse = o.angles(3), o.angles(2)
+    out = o.apply_interp(2, 0, _tangent(fine, V))
+    ref = _tangent(coarse, V)
+    assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max()
+    th, _ = _grid_dirs(coarse)
+    w = np.repeat(coarse["weights"], len(coarse["phis"])) * 2 * np.pi / len(coarse["phis"])
+    integ = np.sum(w[:, None] * np.conj(ref) * out)
+    assert abs(integ - 8 * np.pi / 3 * V @ V) <= 1e-4 * abs(8 * np.pi / 3 * V @ V)
+
+
+def test_anterpolation_is_exact_transpose(dip):
+    # Q10: anterpolation applies the interpolation tables transposed
+    # (src/fmm/stem.cpp:186-244): <A u, v> = <u, I v> with the plain dot product
+    _, kw, o = dip
+    rng = np.random.default_rng(0)
+    nf = len(o.angles(3)["thetas"]) * len(o.angles(3)["phis"])
+    nc = len(o.angles(2)["thetas"]) * len(o.angles(2)["phis"])
+    u = rng.normal(size=(nc, 2)) + 1j * rng.normal(size=(nc, 2))
+    v = rng.normal(size=(nf, 2)) + 1j * rng.normal(size=(nf, 2))
+    Au = o.apply_interp(2, 1, u)
+    Iv = o.apply_interp(2, 0, v)
+    lhs = np.sum(Au * v)
+    rhs = np.sum(u * Iv)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_alpha_series_vs_scipy(dip):
+    # alpha_d(psi_i) = ik/4pi sum_l i^l (2l+1) h_l(k d a) P_l(cos psi_i)  (src/fmm/tables.cpp:90-127)
+    _, kw, o = dip
+    d = O.inode_distances()
+    for lv in (2, 3):
+        al = o.alpha(lv)
+        a = kw["root_leng"] / 2 ** lv
+        L = o.angles(lv)["L"]
+        nps = al.shape[1]
+        psi = np.pi * np.arange(nps) / (nps - 1)
+        for di in [0, 7, 14]:
+            kr = K * d[di] * a
+            s = sum((1j ** l) * (2 * l + 1) * (spherical_jn(l, kr) + 1j * spherical_yn(l, kr)) *
+                    eval_legendre(l, np.cos(psi)) for l in range(L + 1))
+            ref = 1j * K / (4 * np.pi) * s
+            assert np.abs(al[di] - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_translation_vs_direct_series(dip):
+    # T_dX(khat) interpolated in psi agrees with the series evaluated directly at
+    # psi = acos(khat . rhat) to the psi-interpolation error (which the half-sample
+    # reflection at the psi ends, src/fmm/tables.cpp:200-213, keeps near 1e-3)
+    _, kw, o = dip
+    dv = O.inode_distvecs()
+    for lv in (2, 3):
+        T, ex = o.translation(lv)
+        a_ = o.angles(lv)
+        th, ph = _grid_dirs(a_)
+        kh = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1)
+        a = kw["root_leng"] / 2 ** lv
+        L = a_["L"]
+        err = []
+        for oi in range(0, 316, 7):
+            r = np.linalg.norm(dv[oi])
+            x = kh @ (dv[oi] / r)
+            kr = K * r * a
+            s = sum((1j ** l) * (2 * l + 1) * (spherical_jn(l, kr) + 1j * spherical_yn(l, kr)) * eval_legendre(l, x)
+                    for l in range(L + 1))
+            ref = 1j * K / (4 * np.pi) * s
+            err.append(np.linalg.norm(T[oi] - ref) / np.linalg.norm(ref))
+        assert max(err) <= 2e-3
+        assert ex["psi_keys"] > 0
+
+
+def test_fmm_matvec_vs_dense_dipoles(dip):
+    # the reference's own O(N^2) check (evalDirect, src/mlfma.cpp:84-107): 3 digits
+    geom, kw, o = dip
+    x = G.complex_gaussian(geom.n, 0)
+    y = o.matvec(x)
+    yd = o.dense_matvec(x)
+    assert cases.relerr(y, yd) <= 1e-2
+    # linearity
+    x2 = G.complex_gaussian(geom.n, 1)
+    y2 = o.matvec(x2)
+    y12 = o.matvec(x + 2j * x2)
+    assert cases.relerr(y12, y + 2j * y2) <= 1e-13
+
+
+def test_fmm_matvec_vs_dense_adaptive():
+    geom, kw, o = _oracle("dipole_adaptive")
+    assert o.info()["nonnear_pairs"] > 0  # exercises lists 3 and 4
+    x = G.complex_gaussian(geom.n, 0)
+    assert cases.relerr(o.matvec(x), o.dense_matvec(x)) <= 1e-2
+
+
+def test_fmm_matvec_vs_dense_rwg():
+    g = cases.sphere(10, 0.9)
+    cfg = A.make_config(root_leng=2.0, max_node_srcs=0, max_level=3, interp_order=6, digits=3, quad_prec=0, device=-1)
+    o = O.Oracle(g, cases.planewave_z(), cfg)
+    x = G.complex_gaussian(g.n, 0)
+    # RWG supports reach ~0.1 wavelength past the 0.25-wavelength leaves
+    assert cases.relerr(o.matvec(x), o.dense_matvec(x)) <= 3e-2
+
+
+def test_near_entries_reciprocal():
+    g = cases.sphere(4, 0.9)
+    o = O.Oracle(g, cases.planewave_z(),
+                 A.make_config(root_leng=2.0, max_node_srcs=0, max_level=2, quad_prec=2, device=-1))
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        i, j = rng.integers(0, g.n, 2)
+        a, b = o.entry(i, j), o.entry(j, i)
+        if i == j:
+            assert a == 0  # Q3: self term dropped
+        else:
+            assert abs(a - b) <= 1e-12 * abs(a)
+
+
+def _tri_integral(P0, P1, P2, f, n=24):
+    # product Gauss rule on the reference triangle (collapsed square), area 1/2
+    x, w = np.polynomial.legendre.leggauss(n)
+    u = (x + 1) / 2
+    wu = w / 2
+    tot = 0
+    for a, wa in zip(u, wu):
+        for b, wb in zip(u, wu):
+            s, t = a, b * (1 - a)
+            jac = 1 - a
+            r = P0 + s * (P1 - P0) + t * (P2 - P0)
+            tot = tot + wa * wb * jac * f(r)
+    return tot
+
+
+def test_rwg_plane_wave_integral_closed_form():
+    # test/tRwg.cpp idea: the closed form of src/sources/rwg.cpp:46-86 equals a
+    # high-order numerical integral of leng * sum_pm (+-)(r - X_pm) e^{ik.r}
+    v, f = G.icosphere(3, 0.9)
+    r = G.rwg_edges(f)
+    g = A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=r)
+    o = O.Oracle(g, cases.planewave_z(), A.make_config(root_leng=2.0, max_node_srcs=0, max_level=1, device=-1))
+    rng = np.random.default_rng(2)
+    for i in [0, 5, 17, 40]:
+        kvec = K * rng.normal(size=3)
+        kvec *= K / np.linalg.norm(kvec)
+        ana = o.rwg_ipw(i, kvec)
+        v0, v1, tp, tm = r[i]
+        leng = np.linalg.norm(v[v0] - v[v1])
+        ref = 0
+        for t, sg in [(tp, 1.0), (tm, -1.0)]:
+            X = v[f[t]]
+            free = [q for q in f[t] if q not in (v0, v1)][0]
+            ref = ref + sg * _tri_integral(X[0], X[1], X[2], lambda p: np.exp(1j * kvec @ p) * (p - v[free]))
+        ref = leng * ref
+        assert np.abs(ana - ref).max() <= 1e-10 * np.abs(ref).max()
+        num = o.rwg_ipw(i, kvec, numeric=True)  # the 7-point rule of the reference
+        assert np.abs(num - ref).max() <= 1e-3 * np.abs(ref).max()
+
+
+def test_farfield_is_limit_of_near_kernel():
+    # E(rhat) = C k (I - rr) sum I_n f_n(k rhat) (src/fmm/node.cpp:147-178) is the
+    # r -> infinity limit of C k sum I_n G(r rhat - r_n) p_n r e^{-ikr}
+    pos, mom = G.uniform_dipoles(20, 1.0, 4)
+    mom = np.random.default_rng(5).normal(size=(20, 3))
+    g = A.GeometryArrays(A.MLFMA_SRC_DIPOLE, pos=pos, mom=mom)
+    o = O.Oracle(g, cases.planewave_z(), A.make_config(root_leng=2.0, max_node_srcs=4, device=-1))
+    I = G.complex_gaussian(20, 3)
+    dirs = np.array([[0.3, 0.4, np.sqrt(1 - 0.25)], [1.0, 0, 0], [0, -0.6, -0.8]])
+    E = o.farfield(I, dirs)
+    Ck = -1j * 299792458.0 * 1.256637e-6 / (4 * np.pi) * K
+    R = 1e6
+    phat = mom / np.linalg.norm(mom, axis=1, keepdims=True)
+    for d, rh in enumerate(dirs):
+        acc = 0
+        for n in range(20):
+            acc = acc + I[n] * (O.dyadic(R * rh - pos[n], K) @ phat[n])
+        ref = Ck * acc * R * np.exp(-1j * K * R)
+        assert np.abs(E[d] - ref).max() <= 1e-4 * np.abs(ref).max()
+
+
+def test_compat_gmres_invariants(dip):
+    # src/solver.cpp: Arnoldi basis orthonormal, Z Q_k = Q_{k+1} H_k, and since
+    # the complex Givens rotations of src/math.h:168-172 are complex-orthogonal
+    # (G^T G = I), y solves H^T H y = H^T (g0 e1)
+    geom, kw, _ = dip
+    o = O.Oracle(geom, cases.planewave_z(), A.make_config(device=-1, **kw))
+    rec = o.solve_record(12, 1e-12)
+    k = rec["iters"]
+    Q, H, y = rec["Q"], rec["H"], rec["y"]
+    assert np.abs(Q.conj().T @ Q - np.eye(k + 1)).max() <= 1e-12
+    o2 = O.Oracle(geom, cases.planewave_z(), A.make_config(device=-1, **kw))
+    ZQ = np.stack([o2.matvec(Q[:, j]) for j in range(k)], 1)
+    assert np.linalg.norm(ZQ - Q @ H) <= 1e-11 * np.linalg.norm(ZQ)
+    e1 = np.zeros(k + 1, complex)
+    e1[0] = rec["g0"]
+    lhs = H.T @ H @ y
+    rhs = H.T @ e1
+    assert np.linalg.norm(lhs - rhs) <= 1e-9 * np.linalg.norm(rhs)
+    assert np.abs(rec["x"] - Q[:, :k] @ y).max() <= 1e-12 * np.abs(rec["x"]).max()
+    # b = -V (src/solver.cpp:31-32)
+    assert np.abs(Q[:, 0] * rec["g0"] + o.voltage()).max() <= 1e-12 * rec["g0"]
+
+
+def test_dipole_voltage_uses_default_moment():
+    # the 4-argument Dipole constructor sets pol after the voltage was built with
+    # the default moment (p0, 0, 0) (src/sources/dipole.h:11-27)
+    pos, mom = G.uniform_dipoles(10, 1.0, 0)
+    g = A.GeometryArrays(A.MLFMA_SRC_DIPOLE, pos=pos, mom=mom)
+    pw = A.make_planewave(2.0, (0.6, 0.8, 0.0), (0.0, 0.0, 1.0), K)
+    o = O.Oracle(g, pw, A.make_config(root_leng=2.0, max_node_srcs=4, device=-1))
+    ref = -2.0 * np.exp(1j * K * pos[:, 2]) * 1e-8 * 0.6
+    assert np.abs(o.voltage() - ref).max() <= 1e-15 * np.abs(ref).max() + 1e-30
