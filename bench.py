@@ -1,0 +1,226 @@
+"""Benchmark: MLFMA matvec throughput (and GMRES solve time) on one MI355X.
+
+Workload (default): plain FP64 RWG EFIE on a PEC sphere of radius 2 wavelengths,
+geodesic icosphere of frequency 32 (20,480 triangles, 30,720 RWG unknowns),
+7-point triangle quadrature, leaf boxes of 0.25 wavelength (root 4, levels
+0-4), interpolation order 6, 3 digits, plane wave theta = 0 x-polarised.
+
+A step is one full matvec y = Z x (gather, S2M, M2M, M2L, L2L, L2T, near field,
+scatter) with x and y resident in HBM.  The driver contract: W untimed warmup
+steps, then K timed steps bracketed by torch.cuda.synchronize(); one JSON line.
+
+The line also carries:
+  roofline      the dominant kernel's achieved algorithmic GB/s (HIP events on
+                the library's stream over the timed steps) against 8 TB/s;
+  cpu_baseline  the CPU oracle (FP64 restatement of the reference, one core)
+                timed on a bounded number of matvecs of the same workload;
+  solve         GMRES(50) to 1e-6 in FP64 and the FP32-far-field refinement
+                solve, with iteration counts and true residuals.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from mlfma_amd import _capi as A
+    from mlfma_amd import geometry as G
+    if name == "sphere30k":
+        v, f = G.icosphere(32, 2.0)
+        r = G.rwg_edges(f)
+        geom = A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=r)
+        kw = dict(root_leng=4.0, max_node_srcs=0, max_level=4, interp_order=6, digits=3, over_interp=4.0,
+                  quad_prec=2)
+        pw = A.make_planewave(1.0, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2 * np.pi)
+        desc = {"workload": "rwg_sphere_R2_icosphere32", "unknowns": int(len(r)), "triangles": int(len(f)),
+                "mean_edge_wavelengths": round(G.mesh_stats(v, f, r)["mean_edge"], 4), "leaf_wavelengths": 0.25,
+                "levels": "0-4", "interp_order": 6, "digits": 3, "over_interp": 4.0, "quad_points": 7,
+                "excitation": "plane wave theta=0 x-pol"}
+        return geom, pw, kw, desc
+    if name == "dipoles4k":
+        pos, mom = G.uniform_dipoles(4096, 4.0, 0)
+        geom = A.GeometryArrays(A.MLFMA_SRC_DIPOLE, pos=pos, mom=mom)
+        kw = dict(root_leng=4.0, max_node_srcs=0, max_level=3, interp_order=6, digits=3, over_interp=4.0)
+        pw = A.make_planewave(1.0, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2 * np.pi)
+        desc = {"workload": "dipoles_4096_cube4", "unknowns": 4096, "leaf_wavelengths": 0.5, "levels": "0-3",
+                "interp_order": 6, "digits": 3}
+        return geom, pw, kw, desc
+    raise SystemExit(f"unknown workload {name}")
+
+
+def measured_traffic(kernel, tag):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated against a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{tag}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        k, v = line.split()[:2]
+        vals[k] = v
+    return float(vals["traffic_bytes"]) if "traffic_bytes" in vals else None
+
+
+def near_bytes(info):
+    # per launch: the panel values (16 B), their column indices (8 B) and the
+    # output rows (16 B); x reads are gathers that stay in cache
+    return info["near_nnz"] * 16
+
+
+def cpu_baseline(geom, pw, kw, budget_s=20.0, y_gpu=None):
+    from mlfma_amd import _capi as A
+    from tests import _oracle as O
+    from mlfma_amd import geometry as G
+    t0 = time.time()
+    nt = max(1, min(16, os.cpu_count() or 1))
+    o = O.Oracle(geom, pw, A.make_config(device=-1, **kw), threads=nt)
+    setup = time.time() - t0
+    x = G.complex_gaussian(geom.n, 0)
+    y_cpu = o.matvec(x)  # warm; also the parity check of the timed GPU matvec
+    parity = None
+    if y_gpu is not None:
+        parity = float(np.linalg.norm(y_gpu - y_cpu) / np.linalg.norm(y_cpu))
+    times = []
+    t_start = time.time()
+    while time.time() - t_start < budget_s and len(times) < 20:
+        t = time.time()
+        o.matvec(x)
+        times.append(time.time() - t)
+    med = float(np.median(times))
+    return {"value": round(1.0 / med, 4), "unit": "matvec/s", "cores": 1, "kind": "port",
+            "gpu_vs_oracle_rel_diff": parity,
+            "sample": f"{len(times)} FP64 matvecs of the same workload by the CPU oracle (single-threaded "
+                      f"restatement of the reference); median {med:.3f} s; oracle setup {setup:.1f} s on {nt} threads "
+                      f"(not timed)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="sphere30k")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-solve", action="store_true", help="skip the GMRES solves")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--mode", default="matvec", choices=["matvec", "copy"],
+                    help="copy: only run a plain 256 MiB device copy (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (see DESIGN.md for how it would "
+              "shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mlfma_amd import _capi as A
+    from mlfma_amd import geometry as G
+    from mlfma_amd.plan import Plan
+
+    if args.mode == "copy":
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")   # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_read_per_launch": src.numel() * 8,
+                          "bytes_written_per_launch": src.numel() * 8, "launches": 10}))
+        return
+
+    geom, pw, kw, desc = workload(args.workload)
+    log("workload", desc)
+    t0 = time.time()
+    plan = Plan(geom, pw, A.make_config(device=0, **kw))
+    setup_s = time.time() - t0
+    info = plan.info()
+    log("plan", {k: info[k] for k in ["max_level", "boxes", "m2l_pairs", "near_nnz", "radpat_entries",
+                                      "stale_boxes", "truncated_l2l", "device_bytes"]}, "setup s", round(setup_s, 2))
+    n = geom.n
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(G.complex_gaussian(n, 0)).to(dev)
+    y = torch.empty_like(x)
+    stages = {k: 0.0 for k in ["S2M", "M2M", "M2L", "L2L", "L2T", "near", "total"]}
+    for _ in range(args.warmup):
+        plan.matvec(x, y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.matvec(x, y)
+        st = plan.stage_times()
+        for k in stages:
+            stages[k] += st[k]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = args.steps / elapsed
+    stage_ms = {k: round(v / args.steps, 4) for k, v in stages.items()}
+    log("stage ms", stage_ms)
+
+    # roofline of the near-field kernel (one launch per matvec; HIP events on the
+    # library's stream around that launch)
+    near_ms = stage_ms["near"]
+    achieved = near_bytes(info) / (near_ms * 1e-3) / 1e9
+    traffic = measured_traffic("kNear", "sphere30k_fp64") if args.workload == "sphere30k" else None
+    roofline = {"bound": "hbm", "kernel": "kNear", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic) if traffic else None,
+                "bytes_per_launch": near_bytes(info), "avg_launch_ms": near_ms}
+
+    result = {
+        "metric": "MLFMA matvec/s (FP64, one MI355X)", "value": round(value, 3), "unit": "matvec/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": desc, "roofline": roofline, "stage_ms": stage_ms, "setup_s": round(setup_s, 2),
+    }
+
+    if not args.no_solve:
+        sol = {}
+        p64 = Plan(geom, pw, A.make_config(device=0, **kw))
+        x64, r64 = p64.solve(method=A.MLFMA_GMRES_RESTART, restart=50, maxit=500, tol=1e-6)
+        sol["fp64_gmres50"] = {k: r64[k] for k in ["iterations", "status", "resid_true", "seconds"]}
+        p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, **kw))
+        x32, r32 = p32.solve(method=A.MLFMA_FGMRES_IR, restart=50, maxit=500, tol=1e-6)
+        sol["mixed_fp32_ir"] = {k: r32[k] for k in ["iterations", "status", "resid_true", "seconds"]}
+        sol["mixed_vs_fp64_solution_rel"] = float(np.linalg.norm(x32 - x64) / max(np.linalg.norm(x64), 1e-300))
+        # FP32 far-field matvec throughput
+        xx = torch.from_numpy(G.complex_gaussian(n, 0)).to(dev)
+        yy = torch.empty_like(xx)
+        for _ in range(3):
+            p32.matvec(xx, yy)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            p32.matvec(xx, yy)
+        torch.cuda.synchronize()
+        sol["fp32_farfield_matvec_per_s"] = round(args.steps / (time.perf_counter() - t1), 3)
+        result["solve"] = sol
+        log("solve", sol)
+
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        y_gpu = plan.matvec(G.complex_gaussian(n, 0))
+        result["cpu_baseline"] = cpu_baseline(geom, pw, kw, args.cpu_budget, y_gpu)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -96,11 +96,23 @@ SIGNATURES = {
 _lib = None
 
 
+def _bind_one_hip_runtime():
+    """torch ships its own libamdhip64 (same SONAME as /opt/rocm's).  Whichever
+    is loaded first is the one the library binds to; loading torch first keeps
+    one HIP runtime in the process, so torch tensors and the library can share
+    device memory and streams."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def load_library(path: str | None = None):
     """Load the HIP library; raise if it has not been built."""
     global _lib
     if _lib is not None and path is None:
         return _lib
+    _bind_one_hip_runtime()
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"

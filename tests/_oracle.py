@@ -15,20 +15,24 @@ from mlfma_amd import _capi as A
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+# the same restatement compiled with FMA contraction: only used to measure how
+# much the reference's own floating-point result moves under another valid
+# compilation
+LIB_FMA = os.path.join(ORACLE_DIR, "_build", "liboracle_fma.so")
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(variant=""):
+    path = LIB_FMA if variant == "fma" else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         vp = C.c_void_p
         d = A.dptr
         i32 = A.iptr
@@ -67,8 +71,8 @@ def lib():
         L.orc_rwg_ipw.argtypes = [vp, C.c_int, d, C.c_int, d]
         L.orc_entry.argtypes = [vp, C.c_int, C.c_int, d]
         L.orc_radpat.argtypes = [vp, C.c_int, d, d, d]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _c(a):
@@ -83,8 +87,8 @@ class Oracle:
     """One restated reference model (FMM tree, or direct=True for evalDirect)."""
 
     def __init__(self, geom: A.GeometryArrays, pw: A.PlaneWave, cfg: A.Config, direct=False,
-                 threads=None):
-        self.L = lib()
+                 threads=None, variant=""):
+        self.L = lib(variant)
         self.geom = geom
         self.h = C.c_void_p()
         nt = threads or min(8, os.cpu_count() or 1)
