@@ -125,8 +125,8 @@ def main():
                     help="copy: only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (see DESIGN.md for how it would "
-              "shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one MI355X: the library runs the matvec on one GPU and has no multi-GPU "
+              "path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -159,6 +159,9 @@ def main():
     stages = {k: 0.0 for k in ["S2M", "M2M", "M2L", "L2L", "L2T", "near", "total"]}
     for _ in range(args.warmup):
         plan.matvec(x, y)
+    # timed region: K matvecs with the per-stage HIP events recorded on the
+    # library's stream (their device cost is included in the throughput)
+    plan.set_flags(A.MLFMA_FLAG_STAGE_TIMES)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -172,6 +175,15 @@ def main():
     value = args.steps / elapsed
     stage_ms = {k: round(v / args.steps, 4) for k, v in stages.items()}
     log("stage ms", stage_ms)
+    # the same K matvecs without events, reported beside the timed value
+    plan.set_flags(0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.matvec(x, y)
+    torch.cuda.synchronize()
+    no_events = args.steps / (time.perf_counter() - t1)
+    log("matvec/s with stage events", round(value, 2), "without", round(no_events, 2))
 
     # roofline of the near-field kernel (one launch per matvec; HIP events on the
     # library's stream around that launch)
@@ -188,6 +200,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": desc, "roofline": roofline, "stage_ms": stage_ms, "setup_s": round(setup_s, 2),
+        "value_without_stage_events": round(no_events, 3),
     }
 
     if not args.no_solve:

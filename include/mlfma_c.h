@@ -72,6 +72,11 @@ typedef struct mlfma_planewave {
                                        interaction list every matvec instead of
                                        keeping the previous matvec's values
                                        (the reference keeps them, src/fmm/node.cpp:106) */
+#define MLFMA_FLAG_STAGE_TIMES 2u   /* record per-stage device events on the first
+                                       right-hand side of each matvec (the
+                                       reference's ClockTimes, src/clock.h:8-17);
+                                       each event costs a few microseconds of
+                                       device idle time */
 
 /* Numerical configuration (src/config.h:50-73) plus device selection. */
 typedef struct mlfma_config {
@@ -188,10 +193,16 @@ int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_
 
 /* --------------------------------------------------------------- introspection */
 
-/* Per-stage device time of the last matvec in milliseconds (the reference's
- * ClockTimes, src/clock.h:8-17): [0] gather+S2M, [1] M2M, [2] M2L, [3] L2L,
- * [4] L2T, [5] near field, [6] total. */
+/* Per-stage device time of the last matvec run with MLFMA_FLAG_STAGE_TIMES in
+ * milliseconds (the reference's ClockTimes, src/clock.h:8-17): [0] gather+S2M,
+ * [1] M2M, [2] M2L, [3] L2L, [4] L2T, [5] near field, [6] total.
+ * MLFMA_ERR_STATE if no such matvec has run. */
 int mlfma_stage_times(const mlfma_plan* plan, double* ms7);
+
+/* Replace the plan's MLFMA_FLAG_* set for later matvecs and solves (the flags
+ * act at apply time; the plan itself does not change).  No reference
+ * counterpart: the reference fixes them at compile time. */
+int mlfma_plan_set_flags(mlfma_plan* plan, uint32_t flags);
 
 /* Boxes of one level in the plan's order: coords (nbox x 3 int32 integer box
  * coordinates at that level), node kind (1 leaf, 0 stem) and source count. */

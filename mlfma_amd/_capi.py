@@ -18,6 +18,7 @@ MLFMA_SRC_RWG = 1
 MLFMA_FP64 = 0
 MLFMA_FP32 = 1
 MLFMA_FLAG_FRESH_LOCALS = 1
+MLFMA_FLAG_STAGE_TIMES = 2
 MLFMA_PTR_HOST = 0
 MLFMA_PTR_DEVICE = 1
 MLFMA_GMRES_COMPAT = 0
@@ -84,6 +85,7 @@ SIGNATURES = {
                               dptr]),
     "mlfma_farfield": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
     "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),
+    "mlfma_plan_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mlfma_level_boxes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), iptr, iptr, iptr,
                                     C.c_int64]),
     "mlfma_level_pairs": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64), iptr,

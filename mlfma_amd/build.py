@@ -26,7 +26,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
-DEVICE_SOURCES = ["kernels.hip", "plan.hip", "solver.hip"]
+DEVICE_SOURCES = ["kernels.hip", "kernels_mv.hip", "plan.hip", "solver.hip"]
 HOST_SOURCES = ["host.cpp"]
 
 

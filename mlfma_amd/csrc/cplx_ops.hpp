@@ -1,0 +1,56 @@
+// cplx_ops.hpp — device complex arithmetic on interleaved (re, im) pairs.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mlfma {
+
+template <typename R>
+__device__ __forceinline__ cplx<R> mk(R a, R b) {
+    cplx<R> c;
+    c.re = a;
+    c.im = b;
+    return c;
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return mk(a.re - b.re, a.im - b.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
+    return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> operator*(R s, cplx<R> a) { return mk(s * a.re, s * a.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> operator*(cplx<R> a, R s) { return mk(s * a.re, s * a.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> conjmul(cplx<R> a, cplx<R> b) {  // conj(a) * b
+    return mk(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> cdiv(cplx<R> a, R s) { return mk(a.re / s, a.im / s); }
+__device__ __forceinline__ cplx<double> cexpi(double a) {
+    double s, c;
+    sincos(a, &s, &c);
+    return mk(c, s);
+}
+template <typename R, typename S>
+__device__ __forceinline__ cplx<R> cast(cplx<S> a) { return mk((R)a.re, (R)a.im); }
+
+
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wsum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block index -> work item such that the items of each XCD are contiguous:
+// workgroups are dispatched round-robin over the 8 XCDs, so block b runs on
+// XCD b % 8; with a grid of 8 * per blocks XCD x gets items [x*per, (x+1)*per).
+__device__ __forceinline__ int xcdItem(int b, int per) { return (b & 7) * per + (b >> 3); }
+
+}  // namespace mlfma

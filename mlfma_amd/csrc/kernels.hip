@@ -3,45 +3,12 @@
 // roofline is HBM (or L2 for the M2L source reads), see the launchers for the
 // per-item work each block does.
 #include "kernels.hpp"
+#include "cplx_ops.hpp"
 
 namespace mlfma {
 
 constexpr int BS = 256;
 constexpr double kPiD = 3.14159265358979323846;
-
-// ------------------------------------------------------------------ complex helpers
-template <typename R>
-__device__ __forceinline__ cplx<R> mk(R a, R b) {
-    cplx<R> c;
-    c.re = a;
-    c.im = b;
-    return c;
-}
-template <typename R>
-__device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return mk(a.re + b.re, a.im + b.im); }
-template <typename R>
-__device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return mk(a.re - b.re, a.im - b.im); }
-template <typename R>
-__device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
-    return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
-}
-template <typename R>
-__device__ __forceinline__ cplx<R> operator*(R s, cplx<R> a) { return mk(s * a.re, s * a.im); }
-template <typename R>
-__device__ __forceinline__ cplx<R> operator*(cplx<R> a, R s) { return mk(s * a.re, s * a.im); }
-template <typename R>
-__device__ __forceinline__ cplx<R> conjmul(cplx<R> a, cplx<R> b) {  // conj(a) * b
-    return mk(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);
-}
-template <typename R>
-__device__ __forceinline__ cplx<R> cdiv(cplx<R> a, R s) { return mk(a.re / s, a.im / s); }
-__device__ __forceinline__ cplx<double> cexpi(double a) {
-    double s, c;
-    sincos(a, &s, &c);
-    return mk(c, s);
-}
-template <typename R, typename S>
-__device__ __forceinline__ cplx<R> cast(cplx<S> a) { return mk((R)a.re, (R)a.im); }
 
 // ------------------------------------------------------------------ source kernels
 
@@ -317,15 +284,6 @@ void launchVoltage(const SrcDev& s, int64_t n, const double* wv, const double* p
                        pol[2], amp, V);
 }
 
-__device__ __forceinline__ double wsum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ float wsum(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // E(rhat) = C k (I - rr) sum_n I_n f_n(k rhat) with f_n = conj(F_n) (RWG) or
 // e^{-ik rhat.r} p (dipole) (src/fmm/node.cpp:147-178); one block per direction.
 __global__ void kFarfield(SrcDev s, int64_t n, const cplx<double>* __restrict__ I, const double* __restrict__ dirs,
@@ -412,36 +370,6 @@ void launchScatter(const cplx<double>* yt, const int64_t* perm, cplx<double>* y,
     hipLaunchKernelGGL(kScatter, g, dim3(BS), 0, st, yt, perm, y, n, accumulate);
 }
 
-// S2M: M_b[d] = sum_s x_s rp_b[s][d] (src/fmm/leaf.cpp:212-230).  Block = one
-// leaf x 256 directions; each lane streams its direction's column of the
-// leaf's radiation patterns (consecutive lanes read consecutive 2x16 B pairs).
-template <typename R>
-__global__ void kS2M(const LeafItem* __restrict__ items, const cplx<R>* __restrict__ rp,
-                     const cplx<double>* __restrict__ xt, cplx<R>* __restrict__ M) {
-    const LeafItem it = items[blockIdx.y];
-    const int d = blockIdx.x * BS + threadIdx.x;
-    if (d >= it.ndir) return;
-    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
-    const cplx<R>* p = rp + it.rpOff + 2 * d;
-    for (int q = 0; q < it.cnt; ++q) {
-        const cplx<R> x = cast<R>(xt[it.sbeg + q]);
-        const cplx<R> r0 = p[(int64_t)q * it.ndir * 2];
-        const cplx<R> r1 = p[(int64_t)q * it.ndir * 2 + 1];
-        a0 = a0 + x * r0;
-        a1 = a1 + x * r1;
-    }
-    M[it.fOff + 2 * d] = a0;
-    M[it.fOff + 2 * d + 1] = a1;
-}
-
-template <typename R>
-void launchS2M(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* xt, cplx<R>* M,
-               hipStream_t st) {
-    if (nitems <= 0) return;
-    dim3 g((maxndir + BS - 1) / BS, nitems);
-    hipLaunchKernelGGL(kS2M<R>, g, dim3(BS), 0, st, items, rp, xt, M);
-}
-
 // M2M theta pass (src/fmm/stem.cpp:72-103,141-171): for one parent, the sum over
 // its non-empty children of the phase-shifted child field interpolated in theta
 // onto the parent's theta nodes, on the child's phi grid.  Across a pole the
@@ -519,35 +447,6 @@ void launchM2MPhi(int nitems, const int32_t* stemSlot, int nthP, int nphP, int m
     if (nitems <= 0) return;
     dim3 g((nthP * nphP + BS - 1) / BS, nitems);
     hipLaunchKernelGGL(kM2MPhi<R>, g, dim3(BS), 0, st, stemSlot, nthP, nphP, mphC, p, phNear, phW, inner, Mp);
-}
-
-// M2L: L_t[d] = sum over the interaction list of T_off[d] M_s[d] (src/fmm/node.cpp:105-134);
-// the quadrature weights are folded into T.  Block = one target x 256 directions.
-template <typename R>
-__global__ void kM2L(const int32_t* __restrict__ tgtSlot, const int32_t* __restrict__ ptr,
-                     const int32_t* __restrict__ src, const int32_t* __restrict__ off, int ndir,
-                     const cplx<R>* __restrict__ T, const cplx<R>* __restrict__ M, cplx<R>* __restrict__ Lc) {
-    const int item = blockIdx.y;
-    const int d = blockIdx.x * BS + threadIdx.x;
-    if (d >= ndir) return;
-    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
-    for (int q = ptr[item]; q < ptr[item + 1]; ++q) {
-        const cplx<R> t = T[(int64_t)off[q] * ndir + d];
-        const cplx<R>* m = M + ((int64_t)src[q] * ndir + d) * 2;
-        a0 = a0 + t * m[0];
-        a1 = a1 + t * m[1];
-    }
-    cplx<R>* o = Lc + ((int64_t)tgtSlot[item] * ndir + d) * 2;
-    o[0] = a0;
-    o[1] = a1;
-}
-
-template <typename R>
-void launchM2L(int ntgt, const int32_t* tgtSlot, const int32_t* ptr, const int32_t* src, const int32_t* off, int ndir,
-               const cplx<R>* T, const cplx<R>* M, cplx<R>* Lc, hipStream_t st) {
-    if (ntgt <= 0) return;
-    dim3 g((ndir + BS - 1) / BS, ntgt);
-    hipLaunchKernelGGL(kM2L<R>, g, dim3(BS), 0, st, tgtSlot, ptr, src, off, ndir, T, M, Lc);
 }
 
 template <typename R>
@@ -638,80 +537,6 @@ void launchL2LTheta(int nitems, const int32_t* child, int mthP, int nthC, int np
                        Lc);
 }
 
-// L2T: y_s += C k sum_d conj(rp_s[d]) . L[d] (src/fmm/leaf.cpp:257-280).  One
-// wave per source: lanes stride the directions, then a wave reduction.
-template <typename R>
-__global__ void kL2T(const LeafItem* __restrict__ items, const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc,
-                     cplx<double> Ck, cplx<double>* __restrict__ yt) {
-    const LeafItem it = items[blockIdx.x];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const cplx<R>* Lb = Lc + it.fOff;
-    for (int q = w; q < it.cnt; q += BS / 64) {
-        const cplx<R>* r = rp + it.rpOff + (int64_t)q * it.ndir * 2;
-        R re = 0, im = 0;
-        for (int d = lane; d < it.ndir; d += 64) {
-            const cplx<R> v = conjmul(r[2 * d], Lb[2 * d]) + conjmul(r[2 * d + 1], Lb[2 * d + 1]);
-            re += v.re;
-            im += v.im;
-        }
-        re = wsum(re);
-        im = wsum(im);
-        if (lane == 0) {
-            const cplx<double> v = Ck * mk((double)re, (double)im);
-            yt[it.sbeg + q] = yt[it.sbeg + q] + v;
-        }
-    }
-}
-
-template <typename R>
-void launchL2T(const LeafItem* items, int nitems, const cplx<R>* rp, const cplx<R>* Lc, cplx<double> Ck,
-               cplx<double>* yt, hipStream_t st) {
-    if (nitems <= 0) return;
-    hipLaunchKernelGGL(kL2T<R>, dim3(nitems), dim3(BS), 0, st, items, rp, Lc, Ck, yt);
-}
-
-// Near field: y_rows = Z_panel x_cols for one leaf's row panel (column-major,
-// C k folded into the values).  Threads are (row, column-group) so that one
-// sweep of the block reads a contiguous run of the panel.
-__global__ void kNear(const PanelItem* __restrict__ panels, const int64_t* __restrict__ cols,
-                      const cplx<double>* __restrict__ val, const cplx<double>* __restrict__ xt,
-                      cplx<double>* __restrict__ yt) {
-    const PanelItem P = panels[blockIdx.x];
-    __shared__ double sre[BS], sim[BS];
-    for (int r0 = 0; r0 < P.nrows; r0 += BS) {
-        const int nr = min(BS, P.nrows - r0);
-        const int G = BS / nr;
-        const int r = threadIdx.x % nr, g = threadIdx.x / nr;
-        double re = 0, im = 0;
-        if (g < G) {
-            for (int c = g; c < P.ncols; c += G) {
-                const cplx<double> z = val[P.voff + (int64_t)c * P.nrows + r0 + r];
-                const cplx<double> x = xt[cols[P.cptr + c]];
-                re += z.re * x.re - z.im * x.im;
-                im += z.re * x.im + z.im * x.re;
-            }
-        }
-        sre[threadIdx.x] = re;
-        sim[threadIdx.x] = im;
-        __syncthreads();
-        if (threadIdx.x < nr) {
-            double a = 0, b = 0;
-            for (int q = 0; q < G; ++q) {
-                a += sre[q * nr + threadIdx.x];
-                b += sim[q * nr + threadIdx.x];
-            }
-            yt[P.rbeg + r0 + threadIdx.x] = mk(a, b);
-        }
-        __syncthreads();
-    }
-}
-
-void launchNear(const PanelItem* panels, int npanels, const int64_t* cols, const cplx<double>* val,
-                const cplx<double>* xt, cplx<double>* yt, hipStream_t st) {
-    if (npanels <= 0) return;
-    hipLaunchKernelGGL(kNear, dim3(npanels), dim3(BS), 0, st, panels, cols, val, xt, yt);
-}
-
 // Dense matvec over all pairs with the near-field kernel (the reference's
 // evalDirect path, src/mlfma.cpp:84-107); one thread per row, for small n.
 __global__ void kDirect(SrcDev s, const cplx<double>* __restrict__ x, int64_t n, double k, cplx<double> Ck,
@@ -733,23 +558,17 @@ void launchDirect(const SrcDev& s, const cplx<double>* xt, int64_t n, double k, 
 
 // explicit instantiations
 #define INST(R)                                                                                                   \
-    template void launchS2M<R>(const LeafItem*, int, int, const cplx<R>*, const cplx<double>*, cplx<R>*,           \
-                               hipStream_t);                                                                      \
     template void launchM2MTheta<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int,  \
                                     int, int, const int32_t*, const double*, const cplx<R>*, const cplx<R>*,       \
                                     cplx<R>*, hipStream_t);                                                       \
     template void launchM2MPhi<R>(int, const int32_t*, int, int, int, int, const int32_t*, const double*,          \
                                   const cplx<R>*, cplx<R>*, hipStream_t);                                         \
-    template void launchM2L<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int,           \
-                               const cplx<R>*, const cplx<R>*, cplx<R>*, hipStream_t);                            \
     template void launchZeroSlots<R>(int, const int32_t*, int, cplx<R>*, hipStream_t);                             \
     template void launchL2LPhi<R>(int, const int32_t*, const int32_t*, int, int, int, const int32_t*,              \
                                   const int32_t*, const double*, const cplx<R>*, const cplx<R>*, cplx<R>*,        \
                                   hipStream_t);                                                                   \
     template void launchL2LTheta<R>(int, const int32_t*, int, int, int, const int32_t*, const int32_t*,            \
-                                    const double*, const int32_t*, const cplx<R>*, cplx<R>*, hipStream_t);        \
-    template void launchL2T<R>(const LeafItem*, int, const cplx<R>*, const cplx<R>*, cplx<double>, cplx<double>*,   \
-                               hipStream_t);
+                                    const double*, const int32_t*, const cplx<R>*, cplx<R>*, hipStream_t);
 INST(double)
 INST(float)
 #undef INST

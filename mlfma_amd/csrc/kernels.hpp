@@ -93,9 +93,6 @@ void launchScatter(const cplx<double>* yt, const int64_t* perm, cplx<double>* y,
                    int accumulate, hipStream_t st);
 
 template <typename R>
-void launchS2M(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* xt,
-               cplx<R>* M, hipStream_t st);
-template <typename R>
 void launchM2MTheta(int nitems, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child,
                     const int32_t* oct, int nthP, int mthC, int mphC, int p, const int32_t* thNear,
                     const double* thW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* inner,
@@ -104,9 +101,6 @@ template <typename R>
 void launchM2MPhi(int nitems, const int32_t* stemSlot, int nthP, int nphP, int mphC, int p,
                   const int32_t* phNear, const double* phW, const cplx<R>* inner, cplx<R>* Mp,
                   hipStream_t st);
-template <typename R>
-void launchM2L(int ntgt, const int32_t* tgtSlot, const int32_t* ptr, const int32_t* src, const int32_t* off,
-               int ndir, const cplx<R>* T, const cplx<R>* M, cplx<R>* Lc, hipStream_t st);
 template <typename R>
 void launchZeroSlots(int n, const int32_t* slots, int ndir, cplx<R>* Lc, hipStream_t st);
 template <typename R>
@@ -117,12 +111,7 @@ template <typename R>
 void launchL2LTheta(int nitems, const int32_t* child, int mthP, int nthC, int nphC, const int32_t* atPtr,
                     const int32_t* atSrc, const double* atW, const int32_t* atFlip, const cplx<R>* inner,
                     cplx<R>* Lc, hipStream_t st);
-template <typename R>
-void launchL2T(const LeafItem* items, int nitems, const cplx<R>* rp, const cplx<R>* Lc, cplx<double> Ck,
-               cplx<double>* yt, hipStream_t st);
 void launchDirect(const SrcDev& s, const cplx<double>* xt, int64_t n, double k, cplx<double> Ck, cplx<double>* yt,
                   hipStream_t st);
-void launchNear(const PanelItem* panels, int npanels, const int64_t* cols, const cplx<double>* val,
-                const cplx<double>* xt, cplx<double>* yt, hipStream_t st);
 
 }  // namespace mlfma

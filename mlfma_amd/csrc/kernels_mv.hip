@@ -1,0 +1,433 @@
+// kernels_mv.hip — the per-matvec kernels (S2M, fused M2M, M2L, fused L2L,
+// near field, L2T) for gfx950.  Vectors x and y are in input order; the tree
+// permutation is applied inside the kernels that touch them (S2M reads, near
+// field and L2T writes), so a matvec needs no separate gather/scatter pass.
+#include "cplx_ops.hpp"
+#include "kernels_mv.hpp"
+
+namespace mlfma {
+
+constexpr int TB = 256;
+
+// ------------------------------------------------------------------ S2M
+// M_b[d] = sum_s x_s rp_b[s][d]  (src/fmm/leaf.cpp:212-230).  Block = one leaf
+// x blockDim directions; lanes stream consecutive 32-byte (theta, phi) pairs
+// of one source row, two source rows in flight per iteration.
+template <typename R>
+__global__ void kS2Mx(const LeafItem* __restrict__ items, int nitems, int per, const cplx<R>* __restrict__ rp,
+                      const cplx<double>* __restrict__ x, const int64_t* __restrict__ perm, cplx<R>* __restrict__ M) {
+    const int item = xcdItem(blockIdx.x, per);
+    if (item >= nitems) return;
+    const LeafItem it = items[item];
+    const int d = blockIdx.y * blockDim.x + threadIdx.x;
+    if (d >= it.ndir) return;
+    const cplx<R>* p = rp + it.rpOff + 2 * d;
+    const int64_t stride = (int64_t)it.ndir * 2;
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
+    int q = 0;
+    for (; q + 1 < it.cnt; q += 2) {
+        const cplx<R> x0 = cast<R>(x[perm[it.sbeg + q]]);
+        const cplx<R> x1 = cast<R>(x[perm[it.sbeg + q + 1]]);
+        const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
+        const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
+        a0 = a0 + x0 * r00;
+        a1 = a1 + x0 * r01;
+        b0 = b0 + x1 * r10;
+        b1 = b1 + x1 * r11;
+    }
+    if (q < it.cnt) {
+        const cplx<R> x0 = cast<R>(x[perm[it.sbeg + q]]);
+        a0 = a0 + x0 * p[q * stride];
+        a1 = a1 + x0 * p[q * stride + 1];
+    }
+    M[it.fOff + 2 * d] = a0 + b0;
+    M[it.fOff + 2 * d + 1] = a1 + b1;
+}
+
+template <typename R>
+void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* x,
+                const int64_t* perm, cplx<R>* M, hipStream_t st) {
+    if (nitems <= 0) return;
+    const int bs = std::min(TB, ((maxndir + 63) / 64) * 64);
+    const int per = (nitems + 7) / 8;
+    dim3 g(8 * per, (maxndir + bs - 1) / bs);
+    hipLaunchKernelGGL(kS2Mx<R>, g, dim3(bs), 0, st, items, nitems, per, rp, x, perm, M);
+}
+
+// ------------------------------------------------------------------ fused M2M
+// One block per parent box (src/fmm/stem.cpp:72-103,133-184): every non-empty
+// child field is phase-shifted into LDS, the theta interpolation (with the
+// pole rule: reflected node, phi + pi, both components negated) accumulates
+// in registers, the result goes to LDS and the periodic phi interpolation
+// writes the parent field.
+template <typename R, int EPT>
+__global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stemSlot, const int32_t* __restrict__ ptr,
+                                                const int32_t* __restrict__ child, const int32_t* __restrict__ oct,
+                                                int nstem, int per, int rows, int nthP, int nphP, int mthC, int mphC, int p,
+                                                const int32_t* __restrict__ thNear, const double* __restrict__ thW,
+                                                const int32_t* __restrict__ phNear, const double* __restrict__ phW,
+                                                const cplx<R>* __restrict__ phaseUp, const cplx<R>* __restrict__ Mc,
+                                                cplx<R>* __restrict__ Mp) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int item = xcdItem(blockIdx.x, per);
+    if (item >= nstem) return;
+    // this block computes parent theta rows [j0, j0 + nr)
+    const int j0 = blockIdx.y * rows;
+    const int nr = min(rows, nthP - j0);
+    const int ndirC = mthC * mphC;
+    const int nInner = nr * mphC;
+    cplx<R>* sC = reinterpret_cast<cplx<R>*>(smem);
+    cplx<R>* sI = sC + 2 * ndirC;
+    cplx<R> acc[EPT][2];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) acc[k][0] = acc[k][1] = mk<R>(0, 0);
+    for (int c = ptr[item]; c < ptr[item + 1]; ++c) {
+        __syncthreads();
+        const cplx<R>* Mb = Mc + (int64_t)child[c] * ndirC * 2;
+        const cplx<R>* ph = phaseUp + (int64_t)oct[c] * ndirC;
+        for (int d = threadIdx.x; d < ndirC; d += TB) {
+            const cplx<R> s = ph[d];
+            sC[2 * d] = s * Mb[2 * d];
+            sC[2 * d + 1] = s * Mb[2 * d + 1];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = threadIdx.x + k * TB;
+            if (e < nInner) {
+                const int jth = j0 + e / mphC, iph = e - (e / mphC) * mphC;
+                const int near = thNear[jth];
+                const int iphf = iph + (iph < mphC / 2 ? mphC / 2 : -mphC / 2);
+                for (int q = 0; q < 2 * p; ++q) {
+                    const int i = near + 1 - p + q;
+                    const int f = i < 0 ? -i - 1 : (i >= mthC ? 2 * mthC - i - 1 : i);
+                    const bool flip = (i != f);
+                    const int dd = f * mphC + (flip ? iphf : iph);
+                    const R w = (R)(flip ? -thW[jth * 2 * p + q] : thW[jth * 2 * p + q]);
+                    acc[k][0] = acc[k][0] + w * sC[2 * dd];
+                    acc[k][1] = acc[k][1] + w * sC[2 * dd + 1];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int e = threadIdx.x + k * TB;
+        if (e < nInner) {
+            sI[2 * e] = acc[k][0];
+            sI[2 * e + 1] = acc[k][1];
+        }
+    }
+    __syncthreads();
+    cplx<R>* out = Mp + ((int64_t)stemSlot[item] * nthP + j0) * nphP * 2;
+    for (int e = threadIdx.x; e < nr * nphP; e += TB) {
+        const int jr = e / nphP, jph = e - jr * nphP;
+        const int near = phNear[jph];
+        const cplx<R>* in = sI + 2 * jr * mphC;
+        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+        for (int q = 0; q < 2 * p; ++q) {
+            int i = near + 1 - p + q;
+            i = i < 0 ? i + mphC : (i >= mphC ? i - mphC : i);
+            const R w = (R)phW[jph * 2 * p + q];
+            a0 = a0 + w * in[2 * i];
+            a1 = a1 + w * in[2 * i + 1];
+        }
+        out[2 * e] = a0;
+        out[2 * e + 1] = a1;
+    }
+}
+
+template <typename R>
+bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
+                    int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
+                    const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
+                    size_t ldsMax, hipStream_t st) {
+    if (nstem <= 0) return true;
+    // split the parent's theta rows so that there are enough blocks for the
+    // device (each block re-reads the children's fields from L2)
+    int rows = nthP;
+    while (rows > 2 && (int64_t)nstem * ((nthP + rows - 1) / rows) < 1024) rows = (rows + 1) / 2;
+    const size_t lds = sizeof(cplx<R>) * 2 * ((size_t)mthC * mphC + (size_t)rows * mphC);
+    const int nInner = rows * mphC;
+    const int ept = (nInner + TB - 1) / TB;
+    if (lds > ldsMax || ept > 16) return false;
+    const int per = (nstem + 7) / 8;
+    dim3 g(8 * per, (nthP + rows - 1) / rows);
+#define M2M_CASE(E)                                                                                              \
+    hipLaunchKernelGGL((kM2MFused<R, E>), g, dim3(TB), lds, st, stemSlot, ptr, child, oct, nstem, per, rows, nthP, \
+                       nphP, mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp)
+    if (ept <= 1) M2M_CASE(1);
+    else if (ept <= 2) M2M_CASE(2);
+    else if (ept <= 4) M2M_CASE(4);
+    else if (ept <= 8) M2M_CASE(8);
+    else M2M_CASE(16);
+#undef M2M_CASE
+    return true;
+}
+
+// ------------------------------------------------------------------ M2L
+// L_t[d] = sum over the interaction list of T_off[d] M_s[d]  (src/fmm/node.cpp:105-134),
+// quadrature weights folded into T.  One launch covers every level: a block
+// is (level, target box, chunk of 256 directions); the (source, offset) list
+// is read with uniform loads, two pairs in flight; within a level the targets
+// of one XCD are spatially contiguous so their shared sources stay in that
+// XCD's L2.
+template <typename R>
+__global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
+    int b = blockIdx.x, l = 0;
+    while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
+    b -= lv.blk[l];
+    const int nch = lv.nch[l];
+    const int item = xcdItem(b / nch, lv.per[l]);
+    const int chunk = b - (b / nch) * nch;
+    if (item >= lv.ntgt[l]) return;
+    const int ndir = lv.ndir[l];
+    const int d = chunk * TB + threadIdx.x;
+    if (d >= ndir) return;
+    const int32_t* src = lv.src[l];
+    const int32_t* off = lv.off[l];
+    const cplx<R>* T = lv.T[l];
+    const cplx<R>* M = lv.M[l];
+    const int q0 = lv.ptr[l][item], q1 = lv.ptr[l][item + 1];
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
+    int q = q0;
+    for (; q + 1 < q1; q += 2) {
+        const cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
+        const cplx<R> t1 = T[(int64_t)off[q + 1] * ndir + d];
+        const cplx<R>* m0 = M + ((int64_t)src[q] * ndir + d) * 2;
+        const cplx<R>* m1 = M + ((int64_t)src[q + 1] * ndir + d) * 2;
+        a0 = a0 + t0 * m0[0];
+        a1 = a1 + t0 * m0[1];
+        b0 = b0 + t1 * m1[0];
+        b1 = b1 + t1 * m1[1];
+    }
+    if (q < q1) {
+        const cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
+        const cplx<R>* m0 = M + ((int64_t)src[q] * ndir + d) * 2;
+        a0 = a0 + t0 * m0[0];
+        a1 = a1 + t0 * m0[1];
+    }
+    cplx<R>* out = lv.L[l] + ((int64_t)lv.tgt[l][item] * ndir + d) * 2;
+    out[0] = a0 + b0;
+    out[1] = a1 + b1;
+}
+
+template <typename R>
+void launchM2Lx(M2LLevels<R> lv, hipStream_t st) {
+    int total = 0;
+    for (int l = 0; l < lv.nlev; ++l) {
+        lv.nch[l] = (lv.ndir[l] + TB - 1) / TB;
+        lv.per[l] = (lv.ntgt[l] + 7) / 8;
+        lv.blk[l] = total;
+        total += 8 * lv.per[l] * lv.nch[l];
+    }
+    if (total <= 0) return;
+    hipLaunchKernelGGL(kM2Lx<R>, dim3(total), dim3(TB), 0, st, lv);
+}
+
+// ------------------------------------------------------------------ fused L2L
+// One block per child box whose parent has a non-empty interaction list
+// (src/fmm/stem.cpp:109-131,186-244): the parent local, phase-shifted to the
+// child centre on the parent grid, is staged in LDS; the phi anterpolation
+// (gather form of the transpose) goes to LDS; the theta anterpolation with the
+// pole rule is added into the child local.
+template <typename R>
+__global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ parent, const int32_t* __restrict__ child,
+                                                const int32_t* __restrict__ oct, int nitems, int per, int mthP,
+                                                int mphP, int nthC, int nphC, const int32_t* __restrict__ atPhPtr,
+                                                const int32_t* __restrict__ atPhSrc, const double* __restrict__ atPhW,
+                                                const int32_t* __restrict__ atThPtr, const int32_t* __restrict__ atThSrc,
+                                                const double* __restrict__ atThW, const int32_t* __restrict__ atThFlip,
+                                                const cplx<R>* __restrict__ phaseDn, const cplx<R>* __restrict__ Lp,
+                                                cplx<R>* __restrict__ Lc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int item = xcdItem(blockIdx.x, per);
+    if (item >= nitems) return;
+    const int ndirP = mthP * mphP;
+    const int nInner = mthP * nphC;
+    cplx<R>* sP = reinterpret_cast<cplx<R>*>(smem);
+    cplx<R>* sI = sP + 2 * ndirP;
+    const cplx<R>* Lb = Lp + (int64_t)parent[item] * ndirP * 2;
+    const cplx<R>* ph = phaseDn + (int64_t)oct[item] * ndirP;
+    for (int d = threadIdx.x; d < ndirP; d += TB) {
+        const cplx<R> s = ph[d];
+        sP[2 * d] = s * Lb[2 * d];
+        sP[2 * d + 1] = s * Lb[2 * d + 1];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nInner; e += TB) {
+        const int ith = e / nphC, jph = e - ith * nphC;
+        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+        for (int q = atPhPtr[jph]; q < atPhPtr[jph + 1]; ++q) {
+            const int d = ith * mphP + atPhSrc[q];
+            const R w = (R)atPhW[q];
+            a0 = a0 + w * sP[2 * d];
+            a1 = a1 + w * sP[2 * d + 1];
+        }
+        sI[2 * e] = a0;
+        sI[2 * e + 1] = a1;
+    }
+    __syncthreads();
+    cplx<R>* out = Lc + (int64_t)child[item] * nthC * nphC * 2;
+    for (int e = threadIdx.x; e < nthC * nphC; e += TB) {
+        const int jf = e / nphC, js = e - jf * nphC;
+        const int jsf = js + (js < nphC / 2 ? nphC / 2 : -nphC / 2);
+        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+        for (int q = atThPtr[jf]; q < atThPtr[jf + 1]; ++q) {
+            const bool flip = atThFlip[q] != 0;
+            const R w = (R)(flip ? -atThW[q] : atThW[q]);
+            const int d = atThSrc[q] * nphC + (flip ? jsf : js);
+            a0 = a0 + w * sI[2 * d];
+            a1 = a1 + w * sI[2 * d + 1];
+        }
+        out[2 * e] = out[2 * e] + a0;
+        out[2 * e + 1] = out[2 * e + 1] + a1;
+    }
+}
+
+template <typename R>
+bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
+                    int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW,
+                    const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
+                    const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax, hipStream_t st) {
+    if (nitems <= 0) return true;
+    const size_t lds = sizeof(cplx<R>) * 2 * ((size_t)mthP * mphP + (size_t)mthP * nphC);
+    if (lds > ldsMax) return false;
+    const int per = (nitems + 7) / 8;
+    hipLaunchKernelGGL(kL2LFused<R>, dim3(8 * per), dim3(TB), lds, st, parent, child, oct, nitems, per, mthP, mphP,
+                       nthC, nphC, atPhPtr, atPhSrc, atPhW, atThPtr, atThSrc, atThW, atThFlip, phaseDn, Lp, Lc);
+    return true;
+}
+
+// ------------------------------------------------------------------ near field
+// y_rows (=|+=) Z_panel x_cols for one leaf's row panel (values column-major,
+// C k folded in).  The panel's x entries are staged in LDS (chunks of NCH
+// columns), then lanes (row, column group) stream the panel with four
+// independent loads in flight; a block-level reduction over column groups
+// finishes the rows.
+constexpr int NCH = 2048;
+
+__global__ __launch_bounds__(TB) void kNearX(const PanelItem* __restrict__ panels, int npanels,
+                                             const int32_t* __restrict__ cols, const cplx<double>* __restrict__ val,
+                                             const cplx<double>* __restrict__ x, const int64_t* __restrict__ perm,
+                                             cplx<double>* __restrict__ y, int accumulate) {
+    __shared__ cplx<double> sx[NCH];
+    __shared__ double sre[TB], sim[TB];
+    // panels are stored largest first, so the long blocks start first
+    const PanelItem P = panels[blockIdx.x];
+    for (int r0 = 0; r0 < P.nrows; r0 += TB) {
+        const int nr = min(TB, P.nrows - r0);
+        const int G = TB / nr;
+        const int r = threadIdx.x % nr, g = threadIdx.x / nr;
+        double re = 0, im = 0, re2 = 0, im2 = 0;
+        for (int c0 = 0; c0 < P.ncols; c0 += NCH) {
+            const int nc = min(NCH, P.ncols - c0);
+            __syncthreads();
+            for (int c = threadIdx.x; c < nc; c += TB) sx[c] = x[cols[P.cptr + c0 + c]];
+            __syncthreads();
+            if (g < G) {
+                const cplx<double>* v = val + P.voff + (int64_t)c0 * P.nrows + r0 + r;
+                const int64_t ld = (int64_t)G * P.nrows;
+                int c = g;
+                for (; c + 3 * G < nc; c += 4 * G) {
+                    const cplx<double> z0 = v[(int64_t)c * P.nrows];
+                    const cplx<double> z1 = v[(int64_t)c * P.nrows + ld];
+                    const cplx<double> z2 = v[(int64_t)c * P.nrows + 2 * ld];
+                    const cplx<double> z3 = v[(int64_t)c * P.nrows + 3 * ld];
+                    const cplx<double> x0 = sx[c], x1 = sx[c + G], x2 = sx[c + 2 * G], x3 = sx[c + 3 * G];
+                    re += z0.re * x0.re - z0.im * x0.im;
+                    im += z0.re * x0.im + z0.im * x0.re;
+                    re2 += z1.re * x1.re - z1.im * x1.im;
+                    im2 += z1.re * x1.im + z1.im * x1.re;
+                    re += z2.re * x2.re - z2.im * x2.im;
+                    im += z2.re * x2.im + z2.im * x2.re;
+                    re2 += z3.re * x3.re - z3.im * x3.im;
+                    im2 += z3.re * x3.im + z3.im * x3.re;
+                }
+                for (; c < nc; c += G) {
+                    const cplx<double> z = v[(int64_t)c * P.nrows];
+                    const cplx<double> xx = sx[c];
+                    re += z.re * xx.re - z.im * xx.im;
+                    im += z.re * xx.im + z.im * xx.re;
+                }
+            }
+        }
+        sre[threadIdx.x] = re + re2;
+        sim[threadIdx.x] = im + im2;
+        __syncthreads();
+        if (threadIdx.x < nr) {
+            double a = 0, b = 0;
+            for (int q = 0; q < G; ++q) {
+                a += sre[q * nr + threadIdx.x];
+                b += sim[q * nr + threadIdx.x];
+            }
+            const int64_t o = perm[P.rbeg + r0 + threadIdx.x];
+            const cplx<double> v = mk(a, b);
+            y[o] = accumulate ? y[o] + v : v;
+        }
+        __syncthreads();
+    }
+}
+
+void launchNearX(const PanelItem* panels, int npanels, const int32_t* cols, const cplx<double>* val,
+                 const cplx<double>* x, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st) {
+    if (npanels <= 0) return;
+    hipLaunchKernelGGL(kNearX, dim3(npanels), dim3(TB), 0, st, panels, npanels, cols, val, x, perm, y, accumulate);
+}
+
+// ------------------------------------------------------------------ L2T
+// y_s += C k sum_d conj(rp_s[d]) . L[d]  (src/fmm/leaf.cpp:257-280).  One wave
+// per source row across the whole grid; the row's leaf gives its local field.
+template <typename R>
+__global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, const int32_t* __restrict__ rowItem,
+                                            int64_t nrows, const LeafItem* __restrict__ items,
+                                            const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc,
+                                            cplx<double> Ck, const int64_t* __restrict__ perm,
+                                            cplx<double>* __restrict__ y) {
+    const int64_t w = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nrows) return;  // uniform per wave
+    const int64_t t = rowPos[w];
+    const LeafItem it = items[rowItem[w]];
+    const cplx<R>* r = rp + it.rpOff + (t - it.sbeg) * it.ndir * 2;
+    const cplx<R>* Lb = Lc + it.fOff;
+    R re = 0, im = 0;
+    for (int d = lane; d < it.ndir; d += 64) {
+        const cplx<R> v = conjmul(r[2 * d], Lb[2 * d]) + conjmul(r[2 * d + 1], Lb[2 * d + 1]);
+        re += v.re;
+        im += v.im;
+    }
+    re = wsum(re);
+    im = wsum(im);
+    if (lane == 0) {
+        const int64_t o = perm[t];
+        y[o] = y[o] + Ck * mk((double)re, (double)im);
+    }
+}
+
+template <typename R>
+void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,
+                const cplx<R>* Lc, cplx<double> Ck, const int64_t* perm, cplx<double>* y, hipStream_t st) {
+    if (nrows <= 0) return;
+    const int64_t nb = (nrows + TB / 64 - 1) / (TB / 64);
+    hipLaunchKernelGGL(kL2Tx<R>, dim3((unsigned)nb), dim3(TB), 0, st, rowPos, rowItem, nrows, items, rp, Lc, Ck, perm, y);
+}
+
+#define INST(R)                                                                                                     \
+    template void launchS2Mx<R>(const LeafItem*, int, int, const cplx<R>*, const cplx<double>*, const int64_t*,     \
+                                cplx<R>*, hipStream_t);                                                             \
+    template bool launchM2MFused<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
+                                    int, int, const int32_t*, const double*, const int32_t*, const double*,         \
+                                    const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t);                 \
+    template void launchM2Lx<R>(M2LLevels<R>, hipStream_t);                                                         \
+    template bool launchL2LFused<R>(int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,         \
+                                    const int32_t*, const int32_t*, const double*, const int32_t*, const int32_t*,  \
+                                    const double*, const int32_t*, const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, \
+                                    hipStream_t);                                                                   \
+    template void launchL2Tx<R>(const int64_t*, const int32_t*, int64_t, const LeafItem*, const cplx<R>*,           \
+                                const cplx<R>*, cplx<double>, const int64_t*, cplx<double>*, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace mlfma

@@ -1,0 +1,51 @@
+// kernels_mv.hpp — launchers of the per-matvec kernels (kernels_mv.hip).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mlfma {
+
+template <typename R>
+void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* x,
+                const int64_t* perm, cplx<R>* M, hipStream_t st);
+
+// returns false when the level's fields do not fit the LDS limit (caller
+// then uses the two-pass kernels)
+template <typename R>
+bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
+                    int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
+                    const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
+                    size_t ldsMax, hipStream_t st);
+
+// M2L work of every level, passed by value to one launch
+constexpr int kMaxLevels = 16;
+template <typename R>
+struct M2LLevels {
+    int nlev = 0;
+    int ndir[kMaxLevels], ntgt[kMaxLevels], nch[kMaxLevels], per[kMaxLevels], blk[kMaxLevels];
+    const int32_t* tgt[kMaxLevels];
+    const int32_t* ptr[kMaxLevels];
+    const int32_t* src[kMaxLevels];
+    const int32_t* off[kMaxLevels];
+    const cplx<R>* T[kMaxLevels];
+    const cplx<R>* M[kMaxLevels];
+    cplx<R>* L[kMaxLevels];
+};
+
+template <typename R>
+void launchM2Lx(M2LLevels<R> lv, hipStream_t st);
+
+template <typename R>
+bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
+                    int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW,
+                    const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
+                    const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax, hipStream_t st);
+
+void launchNearX(const PanelItem* panels, int npanels, const int32_t* cols, const cplx<double>* val,
+                 const cplx<double>* x, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st);
+
+template <typename R>
+void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,
+                const cplx<R>* Lc, cplx<double> Ck, const int64_t* perm, cplx<double>* y, hipStream_t st);
+
+}  // namespace mlfma

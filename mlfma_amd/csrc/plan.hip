@@ -10,6 +10,7 @@
 
 #include "host.hpp"
 #include "kernels.hpp"
+#include "kernels_mv.hpp"
 #include "plan.hpp"
 
 namespace mlfma {
@@ -83,7 +84,13 @@ void DevicePlan::build() {
     HostPlan& h = hp;
     hipCheck(hipSetDevice(device), "hipSetDevice");
     hipCheck(hipStreamCreate(&stream), "hipStreamCreate");
-    for (auto& e : ev) hipCheck(hipEventCreate(&e), "hipEventCreate");
+    {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+            ldsMax = (size_t)v;
+    }
+    // stage events need no system-scope fence: they order work of one stream
+    for (auto& e : ev) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
     const int64_t n = h.src.n;
     const Tree& t = h.tree;
     const int ML = t.maxLevel;
@@ -203,6 +210,18 @@ void DevicePlan::build() {
     }
     nLeafItems = (int)items.size();
     leafItems = upload(items);
+    {
+        std::vector<int64_t> rows;
+        std::vector<int32_t> ritem;
+        for (int i = 0; i < nLeafItems; ++i)
+            for (int q = 0; q < items[i].cnt; ++q) {
+                rows.push_back(items[i].sbeg + q);
+                ritem.push_back(i);
+            }
+        nFarRows = (int64_t)rows.size();
+        farRowPos = upload(rows);
+        farRowItem = upload(ritem);
+    }
     rp64 = (cplx<double>*)alloc(std::max<int64_t>(2 * h.radpatEntries, 1) * sizeof(cplx<double>));
     launchRadpat(src, leafItems, nLeafItems, levelDev, h.k, rp64, maxLeafNdir, stream);
     hipCheck(hipGetLastError(), "radpat launch");
@@ -376,15 +395,34 @@ void DevicePlan::build() {
         pan[q].cptr = P.cptr;
         pan[q].voff = P.voff;
     }
+    // the build kernel locates an entry's panel by binary search over voff,
+    // so its list stays in value order
+    for (size_t q = 0; q < pan.size(); ++q) {
+        const int64_t end = pan[q].voff + (int64_t)pan[q].nrows * pan[q].ncols;
+        if ((q > 0 && pan[q].voff < pan[q - 1].voff) || end > h.nearNnz)
+            throw Error(MLFMA_ERR_STATE, "near-field panels out of value order");
+    }
     nPanels = (int)pan.size();
-    panels = upload(pan);
+    PanelItem* panelsByValue = upload(pan);
     panelCols = upload(h.panelCols);
+    {
+        std::vector<int32_t> ci(h.panelCols.size());
+        for (size_t q = 0; q < ci.size(); ++q) ci[q] = (int32_t)t.perm[h.panelCols[q]];
+        panelColsIn = upload(ci);
+    }
     nearVal = (cplx<double>*)alloc(std::max<int64_t>(h.nearNnz, 1) * sizeof(cplx<double>));
     const std::complex<double> C = std::complex<double>(0.0, -1.0) * 299792458.0 * 1.256637E-6 / (4.0 * kPi);
     const std::complex<double> Ckc = C * h.k;
     Ck = cplx<double>{Ckc.real(), Ckc.imag()};
-    launchNearBuild(src, panels, nPanels, panelCols, h.nearNnz, h.k, Ck, nearVal, stream);
+    launchNearBuild(src, panelsByValue, nPanels, panelCols, h.nearNnz, h.k, Ck, nearVal, stream);
     hipCheck(hipGetLastError(), "near build launch");
+    hipCheck(hipStreamSynchronize(stream), "near build");
+    release(panelsByValue);
+    // the apply kernel takes one block per panel: largest panels first
+    std::stable_sort(pan.begin(), pan.end(), [](const PanelItem& a, const PanelItem& b) {
+        return (int64_t)a.nrows * a.ncols > (int64_t)b.nrows * b.ncols;
+    });
+    panels = upload(pan);
 
     // radiation patterns in FP32 for the mixed-precision far field
     if (useFp32) {
@@ -424,67 +462,88 @@ void DevicePlan::ensureRhs(int nrhs) {
     rhsCap = nrhs;
 }
 
-// One matvec on tree-ordered vectors: yt = Z xt (single right-hand side).
+// One matvec y (=|+=) Z x on input-order device vectors (single right-hand side).
 template <typename R>
-void DevicePlan::farField(const cplx<double>* x, cplx<double>* y, hipStream_t st, bool timed) {
+void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed) {
     const int ML = hp.tree.maxLevel;
+    const int p = hp.cfg.interp_order;
     cplx<R>* M = fieldM<R>();
     cplx<R>* Lc = fieldL<R>();
     cplx<R>* inner = scratch<R>();
     const cplx<R>* rp = radpat<R>();
+    if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
+    launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st);
     if (timed) hipCheck(hipEventRecord(ev[1], st), "event");
-    launchS2M<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, M, st);
-    if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
     for (int l = ML - 1; l >= 2; --l) {
         LevelBufs& P = levels[l];
         LevelBufs& C = levels[l + 1];
-        launchM2MTheta<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, C.nth, C.nph, hp.cfg.interp_order,
-                          P.thNear, P.thW, C.phUp<R>(), M + C.Moff, inner, st);
-        launchM2MPhi<R>(P.nstem, P.stemSlot, P.nth, P.nph, C.nph, hp.cfg.interp_order, P.phNear, P.phW, inner,
-                        M + P.Moff, st);
+        if (!launchM2MFused<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph, p, P.thNear,
+                               P.thW, P.phNear, P.phW, C.phUp<R>(), M + C.Moff, M + P.Moff, ldsMax, st)) {
+            launchM2MTheta<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, C.nth, C.nph, p, P.thNear, P.thW,
+                              C.phUp<R>(), M + C.Moff, inner, st);
+            launchM2MPhi<R>(P.nstem, P.stemSlot, P.nth, P.nph, C.nph, p, P.phNear, P.phW, inner, M + P.Moff, st);
+        }
     }
+    if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
+    {
+        M2LLevels<R> lv;
+        for (int l = 2; l <= ML; ++l) {
+            LevelBufs& L = levels[l];
+            if (L.ntgt == 0) continue;
+            if (lv.nlev == kMaxLevels) throw Error(MLFMA_ERR_ARG, "too many tree levels");
+            const int k = lv.nlev++;
+            lv.ndir[k] = L.ndir;
+            lv.ntgt[k] = L.ntgt;
+            lv.tgt[k] = L.tgtSlot;
+            lv.ptr[k] = L.lPtr;
+            lv.src[k] = L.lSrc;
+            lv.off[k] = L.lOff;
+            lv.T[k] = L.T<R>();
+            lv.M[k] = M + L.Moff;
+            lv.L[k] = Lc + L.Moff;
+        }
+        launchM2Lx<R>(lv, st);
+    }
+    if (flags & MLFMA_FLAG_FRESH_LOCALS)
+        for (int l = 2; l <= ML; ++l) {
+            LevelBufs& L = levels[l];
+            launchZeroSlots<R>(L.nstale, L.staleSlots, L.ndir, Lc + L.Moff, st);
+        }
     if (timed) hipCheck(hipEventRecord(ev[3], st), "event");
-    for (int l = 2; l <= ML; ++l) {
-        LevelBufs& L = levels[l];
-        launchM2L<R>(L.ntgt, L.tgtSlot, L.lPtr, L.lSrc, L.lOff, L.ndir, L.T<R>(), M + L.Moff, Lc + L.Moff, st);
-        if (hp.cfg.flags & MLFMA_FLAG_FRESH_LOCALS) launchZeroSlots<R>(L.nstale, L.staleSlots, L.ndir, Lc + L.Moff, st);
-    }
-    if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
     for (int l = 3; l <= ML; ++l) {
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];
-        launchL2LPhi<R>(C.nl2l, C.l2lParent, C.l2lOct, P.nth, P.nph, C.nph, P.atPhPtr, P.atPhSrc, P.atPhW,
-                        P.phDn<R>(), Lc + P.Moff, inner, st);
-        launchL2LTheta<R>(C.nl2l, C.l2lChild, P.nth, C.nth, C.nph, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, inner,
-                          Lc + C.Moff, st);
+        if (!launchL2LFused<R>(C.nl2l, C.l2lParent, C.l2lChild, C.l2lOct, P.nth, P.nph, C.nth, C.nph, P.atPhPtr,
+                               P.atPhSrc, P.atPhW, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, P.phDn<R>(),
+                               Lc + P.Moff, Lc + C.Moff, ldsMax, st)) {
+            launchL2LPhi<R>(C.nl2l, C.l2lParent, C.l2lOct, P.nth, P.nph, C.nph, P.atPhPtr, P.atPhSrc, P.atPhW,
+                            P.phDn<R>(), Lc + P.Moff, inner, st);
+            launchL2LTheta<R>(C.nl2l, C.l2lChild, P.nth, C.nth, C.nph, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip,
+                              inner, Lc + C.Moff, st);
+        }
     }
+    if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
+    launchNearX(panels, nPanels, panelColsIn, nearVal, x, perm, y, accumulate, st);
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
-    launchNear(panels, nPanels, panelCols, nearVal, x, y, st);
+    launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, Lc, Ck, perm, y, st);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
-    launchL2T<R>(leafItems, nLeafItems, rp, Lc, Ck, y, st);
-    if (timed) hipCheck(hipEventRecord(ev[7], st), "event");
 }
 
-void DevicePlan::applyTree(const cplx<double>* x, cplx<double>* y, hipStream_t st, bool fp32, bool timed) {
+void DevicePlan::applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
     if (fp32) {
         if (!useFp32) throw Error(MLFMA_ERR_STATE, "plan was not built for FP32");
-        farField<float>(x, y, st, timed);
+        apply<float>(x, y, accumulate, st, timed);
     } else
-        farField<double>(x, y, st, timed);
+        apply<double>(x, y, accumulate, st, timed);
 }
 
 void DevicePlan::matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int accumulate, hipStream_t st, bool fp32) {
-    ensureRhs(nrhs);
     const int64_t n = hp.src.n;
-    hipCheck(hipEventRecord(ev[0], st), "event");
-    for (int r = 0; r < nrhs; ++r) {
-        launchGather(x + (size_t)r * n, perm, xt, n, 1, st);
-        applyTree(xt, yt, st, fp32, r == 0);
-        launchScatter(yt, perm, y + (size_t)r * n, n, 1, accumulate, st);
-    }
-    hipCheck(hipEventRecord(ev[8], st), "event");
+    const bool timed = (flags & MLFMA_FLAG_STAGE_TIMES) != 0;
+    for (int r = 0; r < nrhs; ++r)
+        applyOp(x + (size_t)r * n, y + (size_t)r * n, accumulate, st, fp32, timed && r == 0);
     hipCheck(hipGetLastError(), "matvec launch");
-    timesValid = true;
+    if (timed) timesValid = true;
 }
 
 }  // namespace mlfma
@@ -533,6 +592,7 @@ int mlfma_plan_create(const mlfma_geometry* geom, const mlfma_planewave* pw, con
     plan->d = std::make_unique<DevicePlan>();
     DevicePlan& d = *plan->d;
     buildHostPlan(d.hp, *geom, *pw, *cfg);
+    d.flags = cfg->flags;
     d.hostOnly = cfg->device < 0;
     d.device = cfg->device;
     if (!d.hostOnly) d.build();
@@ -620,19 +680,28 @@ int mlfma_stage_times(const mlfma_plan* plan, double* ms) {
     API_BEGIN
     const DevicePlan& d = P(plan);
     needDevice(d);
-    if (!d.timesValid) throw Error(MLFMA_ERR_STATE, "no matvec has run");
+    if (!d.timesValid) throw Error(MLFMA_ERR_STATE, "no matvec has run with MLFMA_FLAG_STAGE_TIMES set");
     auto el = [&](int a, int b) {
         float t = 0;
         hipCheck(hipEventElapsedTime(&t, d.ev[a], d.ev[b]), "hipEventElapsedTime");
         return (double)t;
     };
-    ms[0] = el(0, 2);  // gather + S2M
-    ms[1] = el(2, 3);  // M2M
-    ms[2] = el(3, 4);  // M2L
-    ms[3] = el(4, 5);  // L2L
-    ms[4] = el(6, 7);  // L2T
-    ms[5] = el(5, 6);  // near
-    ms[6] = el(0, 8);  // total
+    ms[0] = el(0, 1);  // S2M
+    ms[1] = el(1, 2);  // M2M
+    ms[2] = el(2, 3);  // M2L
+    ms[3] = el(3, 4);  // L2L
+    ms[4] = el(5, 6);  // L2T
+    ms[5] = el(4, 5);  // near
+    ms[6] = el(0, 6);  // total (first right-hand side)
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_plan_set_flags(mlfma_plan* plan, uint32_t flags) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES)) throw Error(MLFMA_ERR_ARG, "unknown flag bits");
+    d.flags = flags;
     return MLFMA_OK;
     API_END
 }

@@ -76,6 +76,7 @@ struct DevicePlan {
     hipStream_t stream = nullptr;
     std::array<hipEvent_t, 9> ev{};
     bool timesValid = false;
+    uint32_t flags = 0;  // MLFMA_FLAG_* in effect for matvecs (settable after creation)
     std::vector<void*> allocs;
     int64_t bytesHeld = 0;
 
@@ -90,7 +91,12 @@ struct DevicePlan {
     int nLeafItems = 0, maxLeafNdir = 0;
     PanelItem* panels = nullptr;
     int nPanels = 0;
-    int64_t* panelCols = nullptr;
+    int64_t* panelCols = nullptr;      // tree positions (near-field build)
+    int32_t* panelColsIn = nullptr;    // input-order indices (near-field apply)
+    int64_t* farRowPos = nullptr;      // tree positions of the sources of far-field leaves
+    int32_t* farRowItem = nullptr;     // their leaf item
+    int64_t nFarRows = 0;
+    size_t ldsMax = 65536;
     cplx<double>* nearVal = nullptr;
     cplx<double> Ck{0, 0};
     std::vector<std::complex<double>> Vin;  // excitation, input order
@@ -104,9 +110,10 @@ struct DevicePlan {
     T* upload(const std::vector<T>& v);
     void build();
     void ensureRhs(int nrhs);
+    // y (=|+=) Z x on device vectors in input order
     template <typename R>
-    void farField(const cplx<double>* x, cplx<double>* y, hipStream_t st, bool timed);
-    void applyTree(const cplx<double>* x, cplx<double>* y, hipStream_t st, bool fp32, bool timed);
+    void apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed);
+    void applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed);
     void matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int accumulate, hipStream_t st, bool fp32);
 
     template <typename R>

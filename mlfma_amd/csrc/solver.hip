@@ -232,7 +232,7 @@ static SolveOut gmresCompat(DevicePlan& d, Krylov& K, const cplx<double>* b, cpl
     int it = 0;
     for (;;) {
         const int j = it;
-        d.applyTree(Q + (size_t)j * n, w, K.st, false, false);
+        d.applyOp(Q + (size_t)j * n, w, 0, K.st, false, false);
         std::vector<zc> h;
         double hn;
         K.orthogonalize(Q, j + 1, w, h, hn);
@@ -299,7 +299,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
             hipCheck(hipMemcpyAsync(w, b, n * sizeof(cplx<double>), hipMemcpyDeviceToDevice, K.st), "copy");
             beta = bn;
         } else {
-            d.applyTree(x, w, K.st, fp32, false);
+            d.applyOp(x, w, 0, K.st, fp32, false);
             K.sub(b, w, w);
             beta = K.norm(w);
         }
@@ -312,7 +312,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         g[0] = beta;
         int j = 0;
         for (; j < m && total < maxit; ++j) {
-            d.applyTree(Q + (size_t)j * n, w, K.st, fp32, false);
+            d.applyOp(Q + (size_t)j * n, w, 0, K.st, fp32, false);
             std::vector<zc> h;
             double hn;
             K.orthogonalize(Q, j + 1, w, h, hn);
@@ -375,7 +375,7 @@ static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<do
             hipCheck(hipMemcpyAsync(r, b, n * sizeof(cplx<double>), hipMemcpyDeviceToDevice, K.st), "copy");
             res = 1.0;
         } else {
-            d.applyTree(x, ax, K.st, false, false);
+            d.applyOp(x, ax, 0, K.st, false, false);
             K.sub(b, ax, r);
             res = K.norm(r) / bn;
         }
@@ -453,10 +453,11 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
         }
         if (!std::isfinite(bh[i].re) || !std::isfinite(bh[i].im)) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
     }
-    hipCheck(hipMemcpyAsync(d.xs, bh.data(), n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "b upload");
+    // Krylov vectors stay in input order: the matvec kernels apply the tree
+    // permutation themselves
     cplx<double>* bt = (cplx<double>*)d.alloc(n * sizeof(cplx<double>));
     cplx<double>* xtree = (cplx<double>*)d.alloc(n * sizeof(cplx<double>));
-    launchGather(d.xs, d.perm, bt, n, 1, st);
+    hipCheck(hipMemcpyAsync(bt, bh.data(), n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "b upload");
     Krylov K(d, st, std::max(opts->maxit, opts->restart) + 2);
     SolveOut o;
     if (opts->method == MLFMA_GMRES_COMPAT)
@@ -470,11 +471,10 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
         throw Error(MLFMA_ERR_ARG, "unknown solve method");
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // true residual with the FP64 matvec
-    d.applyTree(xtree, d.yt, st, false, false);
+    d.applyOp(xtree, d.yt, 0, st, false, false);
     K.sub(bt, d.yt, d.yt);
     const double rn = K.norm(d.yt), bn = K.norm(bt);
-    launchScatter(xtree, d.perm, d.ys, n, 1, 0, st);
-    hipCheck(hipMemcpyAsync(x, d.ys, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "x download");
+    hipCheck(hipMemcpyAsync(x, xtree, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "x download");
     hipCheck(hipStreamSynchronize(st), "solve");
     d.release(bt);
     d.release(xtree);

@@ -113,6 +113,10 @@ class Plan:
                                                       A.MLFMA_PTR_HOST, None))
         return y
 
+    def set_flags(self, flags: int) -> None:
+        """Replace the MLFMA_FLAG_* set used by later matvecs and solves."""
+        _check(self.lib, self.lib.mlfma_plan_set_flags(self._h, int(flags)))
+
     def stage_times(self) -> dict:
         t = np.zeros(7)
         _check(self.lib, self.lib.mlfma_stage_times(self._h, t.ctypes.data_as(A.dptr)))

@@ -145,3 +145,21 @@ def test_fp32_far_field_matvec():
     e = cases.relerr(p32.matvec(x), p64.matvec(x))
     say("fp32 far field vs fp64", e)
     assert 1e-12 < e <= 1e-6
+
+
+def test_stage_times_and_runtime_flags():
+    geom, kw = cases.make("dipole_fixed")
+    p = Plan(geom, cases.planewave_z(), A.make_config(device=0, **kw))
+    x = G.complex_gaussian(geom.n, 4)
+    y0 = p.matvec(x)
+    with pytest.raises(Exception):
+        p.stage_times()                    # no matvec has run with events
+    p.set_flags(A.MLFMA_FLAG_STAGE_TIMES)
+    y1 = p.matvec(x)
+    assert cases.relerr(y1, y0) == 0.0     # events do not change results
+    st = p.stage_times()
+    parts = sum(st[k] for k in ["S2M", "M2M", "M2L", "L2L", "L2T", "near"])
+    assert all(v >= 0 for v in st.values()) and st["total"] > 0
+    assert abs(parts - st["total"]) <= 1e-3 + 1e-6 * st["total"]
+    with pytest.raises(Exception):
+        p.set_flags(1 << 7)                # unknown flag bit
