@@ -105,7 +105,8 @@ typedef struct mlfma_info {
     int64_t nth[16], nph[16];      /* angular grid per level                        */
     int64_t boxes[16];             /* non-empty boxes per level                     */
     int64_t m2l_pairs[16];         /* interaction-list pairs per level              */
-    int64_t near_nnz;              /* stored near-field entries (both directions)   */
+    int64_t near_nnz;              /* stored near-field entries: each unordered pair 
+                                      once, self blocks as full squares             */
     int64_t radpat_entries;        /* sum over leaves of srcs x directions          */
     int64_t stale_boxes;           /* non-empty boxes (level >= 2) with empty iList */
     int64_t truncated_l2l;         /* boxes whose parent's iList is empty (level>=3)*/
@@ -195,9 +196,17 @@ int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_
 
 /* Per-stage device time of the last matvec run with MLFMA_FLAG_STAGE_TIMES in
  * milliseconds (the reference's ClockTimes, src/clock.h:8-17): [0] gather+S2M,
- * [1] M2M, [2] M2L, [3] L2L, [4] L2T, [5] near field, [6] total.
+ * [1] M2M, [2] M2L, [3] L2L, [4] L2T (with the near-field row sums folded
+ * in), [5] near field (pair products), [6] total.
  * MLFMA_ERR_STATE if no such matvec has run. */
 int mlfma_stage_times(const mlfma_plan* plan, double* ms7);
+
+/* The same events one launch group at a time: [0] S2M, [1] M2M, [2] M2L,
+ * [3] L2L, [4] near-field pair products (kNearSym), [5] near-field sums of
+ * rows outside every far-field leaf (kNearSum), [6] L2T with the near-field
+ * sums of the other rows folded in (kL2Tx).  Writes min(cap, *count) values;
+ * *count = 7.  No reference counterpart (finer than ClockTimes). */
+int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, int32_t* count);
 
 /* Replace the plan's MLFMA_FLAG_* set for later matvecs and solves (the flags
  * act at apply time; the plan itself does not change).  No reference

@@ -86,6 +86,7 @@ SIGNATURES = {
     "mlfma_farfield": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
     "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),
     "mlfma_plan_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mlfma_stage_times_detail": (C.c_int, [C.c_void_p, dptr, C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_level_boxes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), iptr, iptr, iptr,
                                     C.c_int64]),
     "mlfma_level_pairs": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64), iptr,

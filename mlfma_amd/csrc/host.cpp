@@ -509,13 +509,93 @@ InterpTable buildInterp(const Grid& fine, const Grid& coarse, int p) {
 
 // ---------------------------------------------------------------- plan
 
-static void collectLeaves(const Tree& t, int id, std::vector<int>& out) {
-    const Node& n = t.nodes[id];
-    if (n.leaf()) {
-        if (n.count() > 0) out.push_back(id);
-        return;
+// Creation order of the reference's nodes: a stem creates its eight children
+// in octant order and each child builds its subtree before the next sibling
+// (src/fmm/stem.cpp:3-26), i.e. depth-first preorder.  The reference heads a
+// near pair with the leaf that compares lower (src/fmm/leaf.cpp:76); creation
+// order stands in for that comparison.
+static std::vector<int> creationOrder(const Tree& t) {
+    std::vector<int> ord(t.nodes.size(), -1), stack{0};
+    int next = 0;
+    while (!stack.empty()) {
+        const int id = stack.back();
+        stack.pop_back();
+        ord[id] = next++;
+        const Node& n = t.nodes[id];
+        if (!n.leaf())
+            for (int c = 7; c >= 0; --c) stack.push_back(n.child0 + c);
     }
-    for (int c = 0; c < 8; ++c) collectLeaves(t, n.child0 + c, out);
+    return ord;
+}
+
+static void buildNearItems(HostPlan& hp) {
+    const Tree& t = hp.tree;
+    const Lists& L = hp.lists;
+    const std::vector<int> ord = creationOrder(t);
+    // per head leaf: the column nodes of the pairs it heads
+    std::vector<std::vector<int>> heads(t.nodes.size());
+    for (auto& pr : L.nearPairs) {
+        const bool firstHeads = ord[pr.first] < ord[pr.second];
+        heads[firstHeads ? pr.first : pr.second].push_back(firstHeads ? pr.second : pr.first);
+    }
+    for (auto& pr : L.nonNearPairs) {  // (leaf, list-3 node): the leaf is the observer
+        if (t.nodes[pr.first].count() == 0 || t.nodes[pr.second].count() == 0) continue;
+        heads[pr.first].push_back(pr.second);
+    }
+    int64_t nnz = 0;
+    for (int id = 0; id < (int)t.nodes.size(); ++id) {
+        const Node& n = t.nodes[id];
+        if (n.leaf() && n.count() > 0) {
+            int64_t nc = n.count();
+            for (int cn : heads[id]) nc += t.nodes[cn].count();
+            nnz += n.count() * nc;
+        }
+    }
+    // entries per item: enough items to spread over the device, none too long
+    const int64_t cap = std::max<int64_t>(4096, (nnz + 8191) / 8192);
+    int64_t voff = 0, parts = 0;
+    std::vector<std::vector<int64_t>> sums(t.perm.size());
+    for (int id = 0; id < (int)t.nodes.size(); ++id) {
+        const Node& n = t.nodes[id];
+        if (!n.leaf() || n.count() == 0) continue;
+        auto& hc = heads[id];
+        std::sort(hc.begin(), hc.end(), [&](int a, int b) { return t.nodes[a].sbeg < t.nodes[b].sbeg; });
+        const int64_t base = (int64_t)hp.nearCols.size();
+        for (int64_t q = n.sbeg; q < n.send; ++q) hp.nearCols.push_back(q);
+        for (int cn : hc)
+            for (int64_t q = t.nodes[cn].sbeg; q < t.nodes[cn].send; ++q) hp.nearCols.push_back(q);
+        const int64_t ncol = (int64_t)hp.nearCols.size() - base;
+        const int64_t nself = n.count();
+        for (int64_t r0 = 0; r0 < n.count(); r0 += 256) {
+            const int64_t nr = std::min<int64_t>(256, n.count() - r0);
+            const int64_t cw = std::max<int64_t>(1, cap / nr);
+            for (int64_t c0 = 0; c0 < ncol; c0 += cw) {
+                const int64_t c1 = std::min(ncol, c0 + cw);
+                HostPlan::NearItem it;
+                it.rbeg = n.sbeg + r0;
+                it.nr = (int)nr;
+                it.nc = (int)(c1 - c0);
+                it.nself = (int)std::max<int64_t>(0, std::min(c1, nself) - c0);
+                it.cptr = base + c0;
+                it.voff = voff;
+                it.rpo = parts;
+                it.cpo = parts + nr;
+                for (int64_t r = 0; r < nr; ++r) sums[it.rbeg + r].push_back(it.rpo + r);
+                for (int64_t c = it.nself; c < it.nc; ++c) sums[hp.nearCols[it.cptr + c]].push_back(it.cpo + c - it.nself);
+                parts += nr + (it.nc - it.nself);
+                voff += nr * it.nc;
+                hp.nearItems.push_back(it);
+            }
+        }
+    }
+    if (parts >= ((int64_t)1 << 31)) throw Error(MLFMA_ERR_ARG, "near field too large for one plan");
+    hp.nearNnz = voff;
+    hp.nearParts = parts;
+    hp.nearSumPtr.assign(1, 0);
+    for (auto& v : sums) {
+        for (int64_t q : v) hp.nearSumIdx.push_back((int32_t)q);
+        hp.nearSumPtr.push_back((int64_t)hp.nearSumIdx.size());
+    }
 }
 
 void buildHostPlan(HostPlan& hp, const mlfma_geometry& g, const mlfma_planewave& pw, const mlfma_config& cfg) {
@@ -614,41 +694,7 @@ void buildHostPlan(HostPlan& hp, const mlfma_geometry& g, const mlfma_planewave&
             hp.l2lParent[l].push_back(t.nodes[p].slot);
             hp.l2lOct[l].push_back(n.branch);
         }
-    // near-field panels (src/fmm/leaf.cpp:108-180 stored both ways)
-    std::vector<std::vector<int>> cols(t.nodes.size());   // per leaf: column nodes
-    for (int id = 0; id < (int)t.nodes.size(); ++id)
-        if (t.nodes[id].leaf() && t.nodes[id].count() > 0) cols[id].push_back(id);
-    for (auto& pr : L.nearPairs) {
-        cols[pr.first].push_back(pr.second);
-        cols[pr.second].push_back(pr.first);
-    }
-    for (auto& pr : L.nonNearPairs) {
-        const int l4 = pr.first, x = pr.second;
-        if (t.nodes[l4].count() == 0 || t.nodes[x].count() == 0) continue;
-        cols[l4].push_back(x);
-        std::vector<int> lv;
-        collectLeaves(t, x, lv);
-        for (int leaf : lv) cols[leaf].push_back(l4);
-    }
-    int64_t voff = 0;
-    for (int id = 0; id < (int)t.nodes.size(); ++id) {
-        const Node& n = t.nodes[id];
-        if (!n.leaf() || n.count() == 0) continue;
-        auto& c = cols[id];
-        std::sort(c.begin(), c.end(), [&](int a, int b) { return t.nodes[a].sbeg < t.nodes[b].sbeg; });
-        HostPlan::Panel P;
-        P.node = id;
-        P.rbeg = n.sbeg;
-        P.nrows = n.count();
-        P.cptr = (int64_t)hp.panelCols.size();
-        for (int cn : c)
-            for (int64_t q = t.nodes[cn].sbeg; q < t.nodes[cn].send; ++q) hp.panelCols.push_back(q);
-        P.ncols = (int64_t)hp.panelCols.size() - P.cptr;
-        P.voff = voff;
-        voff += P.nrows * P.ncols;
-        hp.panels.push_back(P);
-    }
-    hp.nearNnz = voff;
+    buildNearItems(hp);
 }
 
 }  // namespace mlfma

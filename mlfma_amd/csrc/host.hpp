@@ -155,11 +155,25 @@ struct HostPlan {
     std::vector<std::vector<int>> m2mPtr, m2mChild, m2mOct;
     // L2L per level l+1 (target): (child slot, parent slot, octant)
     std::vector<std::vector<int>> l2lChild, l2lParent, l2lOct;
-    // near field panels: per non-empty leaf row block
-    struct Panel { int node; int64_t rbeg, nrows, cptr, ncols, voff; };
-    std::vector<Panel> panels;
-    std::vector<int64_t> panelCols;            // tree positions of the panel columns
-    int64_t nearNnz = 0;
+    // near field, stored once per unordered pair as the reference stores it
+    // (src/fmm/leaf.cpp:108-180) and applied in both directions
+    // (src/fmm/leaf.cpp:329-383).  Every non-empty leaf heads one row block:
+    // its self block (full square, zero diagonal, applied to the rows only)
+    // followed by the columns of the pairs it heads.  Row blocks are cut into
+    // work items of at most 256 rows and a bounded number of entries; values
+    // are column-major per item.  Each item writes partial sums (nr for its
+    // rows, nc - nself for its pair columns); nearSumPtr/nearSumIdx list the
+    // partials of every tree position in a fixed order.
+    struct NearItem {
+        int64_t rbeg;  // first row (tree position)
+        int nr, nc, nself;
+        int64_t cptr, voff, rpo, cpo;
+    };
+    std::vector<NearItem> nearItems;           // in value order
+    std::vector<int64_t> nearCols;             // tree positions of the columns
+    std::vector<int64_t> nearSumPtr;           // per tree position (n + 1)
+    std::vector<int32_t> nearSumIdx;
+    int64_t nearNnz = 0, nearParts = 0;
     int64_t staleBoxes = 0, truncatedL2L = 0;
 };
 

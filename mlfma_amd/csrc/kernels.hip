@@ -152,31 +152,40 @@ __device__ cplx<double> entry(const SrcDev& s, int64_t o, int64_t q, double k) {
     return (a.leng * b.leng) * acc;
 }
 
-__global__ void kNearBuild(SrcDev s, const PanelItem* __restrict__ panels, int npanels,
+// One thread per stored entry.  Pair columns hold entry(row, column) with the
+// head leaf's source as the observer (src/fmm/leaf.cpp:115-128); self columns
+// hold the reference's strict lower triangle mirrored, entry(later, earlier)
+// in leaf order (src/fmm/leaf.cpp:361-383), and zero on the diagonal.
+__global__ void kNearBuild(SrcDev s, const NearItem* __restrict__ items, int nitems,
                            const int64_t* __restrict__ cols, int64_t nnz, double k, cplx<double> Ck,
                            cplx<double>* __restrict__ val) {
     const int64_t e = (int64_t)blockIdx.x * BS + threadIdx.x;
     if (e >= nnz) return;
-    int lo = 0, hi = npanels - 1;
-    while (lo < hi) {  // last panel with voff <= e
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {  // last item with voff <= e (items are in value order)
         const int mid = (lo + hi + 1) >> 1;
-        if (panels[mid].voff <= e)
+        if (items[mid].voff <= e)
             lo = mid;
         else
             hi = mid - 1;
     }
-    const PanelItem P = panels[lo];
-    const int64_t loc = e - P.voff;
-    const int64_t c = loc / P.nrows, r = loc - c * P.nrows;
-    const cplx<double> v = entry(s, P.rbeg + r, cols[P.cptr + c], k);
+    const NearItem I = items[lo];
+    const int64_t loc = e - I.voff;
+    const int64_t c = loc / I.nr, r = loc - c * I.nr;
+    const int64_t row = I.rbeg + r, col = cols[I.cptr + c];
+    cplx<double> v;
+    if (c < I.nself)
+        v = row == col ? mk(0.0, 0.0) : entry(s, row > col ? row : col, row > col ? col : row, k);
+    else
+        v = entry(s, row, col, k);
     val[e] = Ck * v;
 }
 
-void launchNearBuild(const SrcDev& s, const PanelItem* panels, int npanels, const int64_t* cols, int64_t nnz,
-                     double k, cplx<double> Ck, cplx<double>* val, hipStream_t st) {
-    if (nnz <= 0) return;
+void launchNearBuild(const SrcDev& s, const NearItem* items, int nitems, const int64_t* cols, int64_t nnz, double k,
+                     cplx<double> Ck, cplx<double>* val, hipStream_t st) {
+    if (nnz <= 0 || nitems <= 0) return;
     const int64_t nb = (nnz + BS - 1) / BS;
-    hipLaunchKernelGGL(kNearBuild, dim3((unsigned)nb), dim3(BS), 0, st, s, panels, npanels, cols, nnz, k, Ck, val);
+    hipLaunchKernelGGL(kNearBuild, dim3((unsigned)nb), dim3(BS), 0, st, s, items, nitems, cols, nnz, k, Ck, val);
 }
 
 // ------------------------------------------------------------------ translation operators

@@ -30,13 +30,17 @@ struct LeafItem {
     double cx, cy, cz;  // leaf centre
 };
 
-// ---- near-field row panel
-struct PanelItem {
-    int64_t rbeg;    // first row (tree position)
-    int32_t nrows;
-    int32_t ncols;
-    int64_t cptr;    // first column index entry
-    int64_t voff;    // first value (column-major nrows x ncols)
+// ---- near-field work item (one per row block piece; see HostPlan::NearItem)
+struct NearItem {
+    int64_t rbeg;   // first row (tree position)
+    int32_t nr;     // rows, <= 256
+    int32_t nc;     // columns
+    int32_t nself;  // leading columns of the self block (applied to the rows only)
+    int32_t pad;
+    int64_t cptr;   // first column entry
+    int64_t voff;   // first value (column-major nr x nc)
+    int64_t rpo;    // row partial sums [rpo, rpo + nr)
+    int64_t cpo;    // column partial sums [cpo, cpo + nc - nself)
 };
 
 // ---- source descriptions in tree order
@@ -75,8 +79,8 @@ struct LevelDev {
 // setup
 void launchRadpat(const SrcDev& s, const LeafItem* items, int nitems, const LevelDev* lv, double k,
                   cplx<double>* rp, int maxndir, hipStream_t st);
-void launchNearBuild(const SrcDev& s, const PanelItem* panels, int npanels, const int64_t* cols,
-                     int64_t nnz, double k, cplx<double> Ck, cplx<double>* val, hipStream_t st);
+void launchNearBuild(const SrcDev& s, const NearItem* items, int nitems, const int64_t* cols, int64_t nnz,
+                     double k, cplx<double> Ck, cplx<double>* val, hipStream_t st);
 void launchAlpha(int nps, int L, double k, double nodeLeng, const double* dists, cplx<double>* alpha,
                  hipStream_t st);
 void launchTransl(int ndir, int nps, int p, const double* psi, const int32_t* nearIdx, const int32_t* distIdx,

@@ -300,90 +300,152 @@ bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, con
 }
 
 // ------------------------------------------------------------------ near field
-// y_rows (=|+=) Z_panel x_cols for one leaf's row panel (values column-major,
-// C k folded in).  The panel's x entries are staged in LDS (chunks of NCH
-// columns), then lanes (row, column group) stream the panel with four
-// independent loads in flight; a block-level reduction over column groups
-// finishes the rows.
-constexpr int NCH = 2048;
+// One block per work item (rows of one leaf x a run of its columns, values
+// column-major with C k folded in).  Each value is read from HBM once and
+// used twice, as the reference uses each stored pair entry
+// (src/fmm/leaf.cpp:329-355): row sums Z x_cols for the item's rows and, for
+// the pair columns, column sums Z^T x_rows.  Tiles of NT values are staged in
+// LDS with a leading dimension made odd (conflict-free column reads); the
+// partial sums go to a buffer that kNearSum folds in a fixed order.
+constexpr int NT = 6 * TB;  // six values per lane per tile
 
-__global__ __launch_bounds__(TB) void kNearX(const PanelItem* __restrict__ panels, int npanels,
-                                             const int32_t* __restrict__ cols, const cplx<double>* __restrict__ val,
-                                             const cplx<double>* __restrict__ x, const int64_t* __restrict__ perm,
-                                             cplx<double>* __restrict__ y, int accumulate) {
-    __shared__ cplx<double> sx[NCH];
+__global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
+                                               const int64_t* __restrict__ perm, const cplx<double>* __restrict__ val,
+                                               const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
+    __shared__ cplx<double> sT[NT];
+    __shared__ cplx<double> sxr[TB], sxc[TB];
     __shared__ double sre[TB], sim[TB];
-    // panels are stored largest first, so the long blocks start first
-    const PanelItem P = panels[blockIdx.x];
-    for (int r0 = 0; r0 < P.nrows; r0 += TB) {
-        const int nr = min(TB, P.nrows - r0);
-        const int G = TB / nr;
-        const int r = threadIdx.x % nr, g = threadIdx.x / nr;
-        double re = 0, im = 0, re2 = 0, im2 = 0;
-        for (int c0 = 0; c0 < P.ncols; c0 += NCH) {
-            const int nc = min(NCH, P.ncols - c0);
-            __syncthreads();
-            for (int c = threadIdx.x; c < nc; c += TB) sx[c] = x[cols[P.cptr + c0 + c]];
-            __syncthreads();
-            if (g < G) {
-                const cplx<double>* v = val + P.voff + (int64_t)c0 * P.nrows + r0 + r;
-                const int64_t ld = (int64_t)G * P.nrows;
-                int c = g;
-                for (; c + 3 * G < nc; c += 4 * G) {
-                    const cplx<double> z0 = v[(int64_t)c * P.nrows];
-                    const cplx<double> z1 = v[(int64_t)c * P.nrows + ld];
-                    const cplx<double> z2 = v[(int64_t)c * P.nrows + 2 * ld];
-                    const cplx<double> z3 = v[(int64_t)c * P.nrows + 3 * ld];
-                    const cplx<double> x0 = sx[c], x1 = sx[c + G], x2 = sx[c + 2 * G], x3 = sx[c + 3 * G];
-                    re += z0.re * x0.re - z0.im * x0.im;
-                    im += z0.re * x0.im + z0.im * x0.re;
-                    re2 += z1.re * x1.re - z1.im * x1.im;
-                    im2 += z1.re * x1.im + z1.im * x1.re;
-                    re += z2.re * x2.re - z2.im * x2.im;
-                    im += z2.re * x2.im + z2.im * x2.re;
-                    re2 += z3.re * x3.re - z3.im * x3.im;
-                    im2 += z3.re * x3.im + z3.im * x3.re;
-                }
-                for (; c < nc; c += G) {
-                    const cplx<double> z = v[(int64_t)c * P.nrows];
-                    const cplx<double> xx = sx[c];
-                    re += z.re * xx.re - z.im * xx.im;
-                    im += z.re * xx.im + z.im * xx.re;
-                }
-            }
-        }
-        sre[threadIdx.x] = re + re2;
-        sim[threadIdx.x] = im + im2;
+    const NearItem I = items[blockIdx.x];
+    const int nr = I.nr;        // 1..256
+    const int ld = nr | 1;      // odd leading dimension in LDS
+    const int cwMax = min(TB, NT / ld);
+    if (threadIdx.x < nr) sxr[threadIdx.x] = x[perm[I.rbeg + threadIdx.x]];
+    const int G = TB / nr;
+    const int r = threadIdx.x % nr, g = threadIdx.x / nr;
+    double re = 0, im = 0;
+    for (int c0 = 0; c0 < I.nc; c0 += cwMax) {
+        const int cw = min(cwMax, I.nc - c0);
+        const int ne = cw * nr;
+        const cplx<double>* src = val + I.voff + (int64_t)c0 * nr;
+        // six independent loads in flight per lane, held as plain doubles (a
+        // struct or array here is left in scratch memory by the compiler); the
+        // index is clamped into the tile so every load is valid
+        const int t0 = threadIdx.x;
+        const double* sd = reinterpret_cast<const double*>(src);
+#define NEAR_LD(K) \
+    const int e##K = t0 + K * TB; \
+    const int q##K = 2 * min(e##K, ne - 1); \
+    const double r##K = sd[q##K], i##K = sd[q##K + 1];
+        NEAR_LD(0) NEAR_LD(1) NEAR_LD(2) NEAR_LD(3) NEAR_LD(4) NEAR_LD(5)
+#undef NEAR_LD
+        cplx<double> xc = mk(0.0, 0.0);
+        if (threadIdx.x < cw) xc = x[colsIn[I.cptr + c0 + threadIdx.x]];
+        __syncthreads();  // the previous tile is consumed
+        double* sTd = reinterpret_cast<double*>(sT);
+#define NEAR_ST(K) \
+    if (e##K < ne) { \
+        const int c = e##K / nr; \
+        const int a = 2 * (c * ld + (e##K - c * nr)); \
+        sTd[a] = r##K; \
+        sTd[a + 1] = i##K; \
+    }
+        NEAR_ST(0) NEAR_ST(1) NEAR_ST(2) NEAR_ST(3) NEAR_ST(4) NEAR_ST(5)
+#undef NEAR_ST
+        if (threadIdx.x < cw) sxc[threadIdx.x] = xc;
         __syncthreads();
-        if (threadIdx.x < nr) {
-            double a = 0, b = 0;
-            for (int q = 0; q < G; ++q) {
-                a += sre[q * nr + threadIdx.x];
-                b += sim[q * nr + threadIdx.x];
+        if (g < G)
+            for (int c = g; c < cw; c += G) {
+                const cplx<double> z = sT[c * ld + r], xx = sxc[c];
+                re += z.re * xx.re - z.im * xx.im;
+                im += z.re * xx.im + z.im * xx.re;
             }
-            const int64_t o = perm[P.rbeg + r0 + threadIdx.x];
-            const cplx<double> v = mk(a, b);
-            y[o] = accumulate ? y[o] + v : v;
+        if (threadIdx.x < cw && c0 + (int)threadIdx.x >= I.nself) {
+            const cplx<double>* col = sT + threadIdx.x * ld;
+            double cr = 0, ci = 0, cr2 = 0, ci2 = 0;
+            int q = 0;
+            for (; q + 1 < nr; q += 2) {
+                const cplx<double> z0 = col[q], x0 = sxr[q], z1 = col[q + 1], x1 = sxr[q + 1];
+                cr += z0.re * x0.re - z0.im * x0.im;
+                ci += z0.re * x0.im + z0.im * x0.re;
+                cr2 += z1.re * x1.re - z1.im * x1.im;
+                ci2 += z1.re * x1.im + z1.im * x1.re;
+            }
+            if (q < nr) {
+                const cplx<double> z0 = col[q], x0 = sxr[q];
+                cr += z0.re * x0.re - z0.im * x0.im;
+                ci += z0.re * x0.im + z0.im * x0.re;
+            }
+            part[I.cpo + c0 + threadIdx.x - I.nself] = mk(cr + cr2, ci + ci2);
         }
-        __syncthreads();
+    }
+    sre[threadIdx.x] = (g < G) ? re : 0.0;
+    sim[threadIdx.x] = (g < G) ? im : 0.0;
+    __syncthreads();
+    if (threadIdx.x < nr) {
+        double a = 0, b = 0;
+        for (int q = 0; q < G; ++q) {
+            a += sre[q * nr + threadIdx.x];
+            b += sim[q * nr + threadIdx.x];
+        }
+        part[I.rpo + threadIdx.x] = mk(a, b);
     }
 }
 
-void launchNearX(const PanelItem* panels, int npanels, const int32_t* cols, const cplx<double>* val,
-                 const cplx<double>* x, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st) {
-    if (npanels <= 0) return;
-    hipLaunchKernelGGL(kNearX, dim3(npanels), dim3(TB), 0, st, panels, npanels, cols, val, x, perm, y, accumulate);
+// Sum of the near-field partials of tree position t, split over the 64 lanes
+// of a wave (same lane assignment and butterfly every call: deterministic).
+__device__ __forceinline__ cplx<double> nearFold(const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                                 const cplx<double>* __restrict__ part, int64_t t, int lane) {
+    double a = 0, b = 0;
+    for (int64_t q = ptr[t] + lane; q < ptr[t + 1]; q += 64) {
+        const cplx<double> v = part[idx[q]];
+        a += v.re;
+        b += v.im;
+    }
+    return mk(wsum(a), wsum(b));
+}
+
+// y[perm[t]] (=|+=) near-field sum, one wave per listed row (rows that have no
+// far-field leaf; the far-field rows are finished by kL2Tx).
+__global__ __launch_bounds__(TB) void kNearSum(const int64_t* __restrict__ rows, int64_t nrows,
+                                               const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                               const cplx<double>* __restrict__ part, const int64_t* __restrict__ perm,
+                                               cplx<double>* __restrict__ y, int accumulate) {
+    const int64_t w = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nrows) return;  // uniform per wave
+    const int64_t t = rows[w];
+    const cplx<double> v = nearFold(ptr, idx, part, t, lane);
+    if (lane == 0) {
+        const int64_t o = perm[t];
+        y[o] = accumulate ? y[o] + v : v;
+    }
+}
+
+void launchNearSym(const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+                   const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st) {
+    if (nitems > 0) hipLaunchKernelGGL(kNearSym, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
+}
+
+void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
+                   const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st) {
+    if (nrows <= 0) return;
+    const int64_t nb = (nrows + TB / 64 - 1) / (TB / 64);
+    hipLaunchKernelGGL(kNearSum, dim3((unsigned)nb), dim3(TB), 0, st, rows, nrows, sumPtr, sumIdx, part, perm, y,
+                       accumulate);
 }
 
 // ------------------------------------------------------------------ L2T
-// y_s += C k sum_d conj(rp_s[d]) . L[d]  (src/fmm/leaf.cpp:257-280).  One wave
-// per source row across the whole grid; the row's leaf gives its local field.
+// y_s (=|+=) near_s + C k sum_d conj(rp_s[d]) . L[d]  (src/fmm/leaf.cpp:257-280,
+// with the near-field partial sums of the row folded in).  One wave per source
+// row of a far-field leaf; the row's leaf gives its local field.
 template <typename R>
 __global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, const int32_t* __restrict__ rowItem,
                                             int64_t nrows, const LeafItem* __restrict__ items,
                                             const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc,
-                                            cplx<double> Ck, const int64_t* __restrict__ perm,
-                                            cplx<double>* __restrict__ y) {
+                                            cplx<double> Ck, const int64_t* __restrict__ sumPtr,
+                                            const int32_t* __restrict__ sumIdx, const cplx<double>* __restrict__ part,
+                                            const int64_t* __restrict__ perm, cplx<double>* __restrict__ y,
+                                            int accumulate) {
     const int64_t w = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w >= nrows) return;  // uniform per wave
@@ -399,18 +461,22 @@ __global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, 
     }
     re = wsum(re);
     im = wsum(im);
+    const cplx<double> nearv = nearFold(sumPtr, sumIdx, part, t, lane);
     if (lane == 0) {
         const int64_t o = perm[t];
-        y[o] = y[o] + Ck * mk((double)re, (double)im);
+        const cplx<double> v = nearv + Ck * mk((double)re, (double)im);
+        y[o] = accumulate ? y[o] + v : v;
     }
 }
 
 template <typename R>
 void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,
-                const cplx<R>* Lc, cplx<double> Ck, const int64_t* perm, cplx<double>* y, hipStream_t st) {
+                const cplx<R>* Lc, cplx<double> Ck, const int64_t* sumPtr, const int32_t* sumIdx,
+                const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st) {
     if (nrows <= 0) return;
     const int64_t nb = (nrows + TB / 64 - 1) / (TB / 64);
-    hipLaunchKernelGGL(kL2Tx<R>, dim3((unsigned)nb), dim3(TB), 0, st, rowPos, rowItem, nrows, items, rp, Lc, Ck, perm, y);
+    hipLaunchKernelGGL(kL2Tx<R>, dim3((unsigned)nb), dim3(TB), 0, st, rowPos, rowItem, nrows, items, rp, Lc, Ck, sumPtr,
+                       sumIdx, part, perm, y, accumulate);
 }
 
 #define INST(R)                                                                                                     \
@@ -425,7 +491,8 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
                                     const double*, const int32_t*, const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, \
                                     hipStream_t);                                                                   \
     template void launchL2Tx<R>(const int64_t*, const int32_t*, int64_t, const LeafItem*, const cplx<R>*,           \
-                                const cplx<R>*, cplx<double>, const int64_t*, cplx<double>*, hipStream_t);
+                                const cplx<R>*, cplx<double>, const int64_t*, const int32_t*, const cplx<double>*,  \
+                                const int64_t*, cplx<double>*, int, hipStream_t);
 INST(double)
 INST(float)
 #undef INST

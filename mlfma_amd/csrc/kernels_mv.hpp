@@ -41,11 +41,17 @@ bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, con
                     const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
                     const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax, hipStream_t st);
 
-void launchNearX(const PanelItem* panels, int npanels, const int32_t* cols, const cplx<double>* val,
-                 const cplx<double>* x, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st);
+// near field: kNearSym writes partial sums; kL2Tx folds them into the rows of
+// far-field leaves and kNearSum into the listed remaining rows
+void launchNearSym(const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+                   const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st);
+void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
+                   const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st);
 
+// y (=|+=) near + L2T for the rows of far-field leaves
 template <typename R>
 void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,
-                const cplx<R>* Lc, cplx<double> Ck, const int64_t* perm, cplx<double>* y, hipStream_t st);
+                const cplx<R>* Lc, cplx<double> Ck, const int64_t* sumPtr, const int32_t* sumIdx,
+                const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st);
 
 }  // namespace mlfma

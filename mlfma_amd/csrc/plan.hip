@@ -221,6 +221,14 @@ void DevicePlan::build() {
         nFarRows = (int64_t)rows.size();
         farRowPos = upload(rows);
         farRowItem = upload(ritem);
+        // rows that no far-field leaf covers get their near-field sum alone
+        std::vector<char> far(h.src.n, 0);
+        for (int64_t q : rows) far[q] = 1;
+        std::vector<int64_t> only;
+        for (int64_t q = 0; q < h.src.n; ++q)
+            if (!far[q]) only.push_back(q);
+        nNearOnlyRows = (int64_t)only.size();
+        nearOnlyRows = upload(only);
     }
     rp64 = (cplx<double>*)alloc(std::max<int64_t>(2 * h.radpatEntries, 1) * sizeof(cplx<double>));
     launchRadpat(src, leafItems, nLeafItems, levelDev, h.k, rp64, maxLeafNdir, stream);
@@ -386,43 +394,45 @@ void DevicePlan::build() {
     if (useFp32) inner32 = (cplx<float>*)alloc(innerLen * sizeof(cplx<float>));
 
     // near field
-    std::vector<PanelItem> pan(h.panels.size());
-    for (size_t q = 0; q < pan.size(); ++q) {
-        const auto& P = h.panels[q];
-        pan[q].rbeg = P.rbeg;
-        pan[q].nrows = (int32_t)P.nrows;
-        pan[q].ncols = (int32_t)P.ncols;
-        pan[q].cptr = P.cptr;
-        pan[q].voff = P.voff;
+    std::vector<NearItem> nitemsH(h.nearItems.size());
+    for (size_t q = 0; q < nitemsH.size(); ++q) {
+        const auto& I = h.nearItems[q];
+        if (I.nr < 1 || I.nr > 256 || I.nc < 1 || I.nself < 0 || I.nself > I.nc)
+            throw Error(MLFMA_ERR_STATE, "near-field item out of range");
+        nitemsH[q] = NearItem{I.rbeg, I.nr, I.nc, I.nself, 0, I.cptr, I.voff, I.rpo, I.cpo};
     }
-    // the build kernel locates an entry's panel by binary search over voff,
-    // so its list stays in value order
-    for (size_t q = 0; q < pan.size(); ++q) {
-        const int64_t end = pan[q].voff + (int64_t)pan[q].nrows * pan[q].ncols;
-        if ((q > 0 && pan[q].voff < pan[q - 1].voff) || end > h.nearNnz)
-            throw Error(MLFMA_ERR_STATE, "near-field panels out of value order");
+    // the build kernel locates an entry's item by binary search over voff, so
+    // its list stays in value order
+    for (size_t q = 0; q < nitemsH.size(); ++q) {
+        const int64_t end = nitemsH[q].voff + (int64_t)nitemsH[q].nr * nitemsH[q].nc;
+        if ((q > 0 && nitemsH[q].voff < nitemsH[q - 1].voff) || end > h.nearNnz)
+            throw Error(MLFMA_ERR_STATE, "near-field items out of value order");
     }
-    nPanels = (int)pan.size();
-    PanelItem* panelsByValue = upload(pan);
-    panelCols = upload(h.panelCols);
+    nNearItems = (int)nitemsH.size();
+    NearItem* itemsByValue = upload(nitemsH);
+    int64_t* colsTree = upload(h.nearCols);
     {
-        std::vector<int32_t> ci(h.panelCols.size());
-        for (size_t q = 0; q < ci.size(); ++q) ci[q] = (int32_t)t.perm[h.panelCols[q]];
-        panelColsIn = upload(ci);
+        std::vector<int32_t> ci(h.nearCols.size());
+        for (size_t q = 0; q < ci.size(); ++q) ci[q] = (int32_t)t.perm[h.nearCols[q]];
+        nearColsIn = upload(ci);
     }
+    nearSumPtr = upload(h.nearSumPtr);
+    nearSumIdx = upload(h.nearSumIdx);
+    nearPart = (cplx<double>*)alloc(std::max<int64_t>(h.nearParts, 1) * sizeof(cplx<double>));
     nearVal = (cplx<double>*)alloc(std::max<int64_t>(h.nearNnz, 1) * sizeof(cplx<double>));
     const std::complex<double> C = std::complex<double>(0.0, -1.0) * 299792458.0 * 1.256637E-6 / (4.0 * kPi);
     const std::complex<double> Ckc = C * h.k;
     Ck = cplx<double>{Ckc.real(), Ckc.imag()};
-    launchNearBuild(src, panelsByValue, nPanels, panelCols, h.nearNnz, h.k, Ck, nearVal, stream);
+    launchNearBuild(src, itemsByValue, nNearItems, colsTree, h.nearNnz, h.k, Ck, nearVal, stream);
     hipCheck(hipGetLastError(), "near build launch");
     hipCheck(hipStreamSynchronize(stream), "near build");
-    release(panelsByValue);
-    // the apply kernel takes one block per panel: largest panels first
-    std::stable_sort(pan.begin(), pan.end(), [](const PanelItem& a, const PanelItem& b) {
-        return (int64_t)a.nrows * a.ncols > (int64_t)b.nrows * b.ncols;
+    release(itemsByValue);
+    release(colsTree);
+    // the apply kernel takes one block per item: longest items first
+    std::stable_sort(nitemsH.begin(), nitemsH.end(), [](const NearItem& a, const NearItem& b) {
+        return (int64_t)a.nr * a.nc > (int64_t)b.nr * b.nc;
     });
-    panels = upload(pan);
+    nearItems = upload(nitemsH);
 
     // radiation patterns in FP32 for the mixed-precision far field
     if (useFp32) {
@@ -523,10 +533,13 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         }
     }
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
-    launchNearX(panels, nPanels, panelColsIn, nearVal, x, perm, y, accumulate, st);
+    launchNearSym(nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
-    launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, Lc, Ck, perm, y, st);
+    launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
+    launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, Lc, Ck, nearSumPtr, nearSumIdx, nearPart, perm, y,
+                  accumulate, st);
+    if (timed) hipCheck(hipEventRecord(ev[7], st), "event");
 }
 
 void DevicePlan::applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
@@ -690,9 +703,25 @@ int mlfma_stage_times(const mlfma_plan* plan, double* ms) {
     ms[1] = el(1, 2);  // M2M
     ms[2] = el(2, 3);  // M2L
     ms[3] = el(3, 4);  // L2L
-    ms[4] = el(5, 6);  // L2T
-    ms[5] = el(4, 5);  // near
-    ms[6] = el(0, 6);  // total (first right-hand side)
+    ms[4] = el(6, 7);  // L2T
+    ms[5] = el(4, 6);  // near field (pair products + fold)
+    ms[6] = el(0, 7);  // total (first right-hand side)
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, int32_t* count) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    needDevice(d);
+    if (!ms || !count) throw Error(MLFMA_ERR_ARG, "null argument");
+    if (!d.timesValid) throw Error(MLFMA_ERR_STATE, "no matvec has run with MLFMA_FLAG_STAGE_TIMES set");
+    *count = 7;
+    for (int q = 0; q < 7 && q < cap; ++q) {
+        float t = 0;
+        hipCheck(hipEventElapsedTime(&t, d.ev[q], d.ev[q + 1]), "hipEventElapsedTime");
+        ms[q] = t;
+    }
     return MLFMA_OK;
     API_END
 }

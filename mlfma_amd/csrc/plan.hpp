@@ -4,7 +4,8 @@
 //   * radiation patterns: per far-field leaf, [source][direction][theta|phi];
 //   * multipole / local fields: per level >= 2, [box slot][direction][theta|phi];
 //   * translation operators: per level, [316 offsets][direction], weights folded;
-//   * near field: per leaf row panel, column-major values (C k folded).
+//   * near field: each unordered pair stored once (self blocks full), cut into
+//     work items of <= 256 rows; column-major values (C k folded).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -89,13 +90,17 @@ struct DevicePlan {
     cplx<float>*M32 = nullptr, *L32 = nullptr, *inner32 = nullptr, *rp32 = nullptr;
     LeafItem* leafItems = nullptr;
     int nLeafItems = 0, maxLeafNdir = 0;
-    PanelItem* panels = nullptr;
-    int nPanels = 0;
-    int64_t* panelCols = nullptr;      // tree positions (near-field build)
-    int32_t* panelColsIn = nullptr;    // input-order indices (near-field apply)
+    NearItem* nearItems = nullptr;     // longest first
+    int nNearItems = 0;
+    int32_t* nearColsIn = nullptr;     // input-order indices of the item columns
+    int64_t* nearSumPtr = nullptr;     // per tree position: its partial sums
+    int32_t* nearSumIdx = nullptr;
+    cplx<double>* nearPart = nullptr;  // partial sums of one near-field apply
     int64_t* farRowPos = nullptr;      // tree positions of the sources of far-field leaves
     int32_t* farRowItem = nullptr;     // their leaf item
     int64_t nFarRows = 0;
+    int64_t* nearOnlyRows = nullptr;   // tree positions outside every far-field leaf
+    int64_t nNearOnlyRows = 0;
     size_t ldsMax = 65536;
     cplx<double>* nearVal = nullptr;
     cplx<double> Ck{0, 0};

@@ -122,6 +122,13 @@ class Plan:
         _check(self.lib, self.lib.mlfma_stage_times(self._h, t.ctypes.data_as(A.dptr)))
         return dict(zip(["S2M", "M2M", "M2L", "L2L", "L2T", "near", "total"], t.tolist()))
 
+    def stage_times_detail(self) -> dict:
+        """Device ms of each launch group of the last timed matvec."""
+        t = np.zeros(7)
+        cnt = C.c_int32(0)
+        _check(self.lib, self.lib.mlfma_stage_times_detail(self._h, t.ctypes.data_as(A.dptr), 7, C.byref(cnt)))
+        return dict(zip(["S2M", "M2M", "M2L", "L2L", "near_pairs", "near_sum", "L2T"], t.tolist()))
+
     # ---------------------------------------------------------------- solve
     def solve(self, b=None, method=A.MLFMA_GMRES_COMPAT, restart=50, maxit=500, tol=1e-6, history=False):
         o = A.SolveOpts()

@@ -161,5 +161,8 @@ def test_stage_times_and_runtime_flags():
     parts = sum(st[k] for k in ["S2M", "M2M", "M2L", "L2L", "L2T", "near"])
     assert all(v >= 0 for v in st.values()) and st["total"] > 0
     assert abs(parts - st["total"]) <= 1e-3 + 1e-6 * st["total"]
+    d = p.stage_times_detail()
+    assert abs(d["near_pairs"] + d["near_sum"] - st["near"]) <= 1e-3
+    assert abs(sum(d.values()) - st["total"]) <= 1e-3
     with pytest.raises(Exception):
         p.set_flags(1 << 7)                # unknown flag bit
