@@ -552,7 +552,10 @@ static void buildNearItems(HostPlan& hp) {
         }
     }
     // entries per item: enough items to spread over the device, none too long
-    const int64_t cap = std::max<int64_t>(4096, (nnz + 8191) / 8192);
+#ifndef MLFMA_NEAR_ITEM_CAP
+#define MLFMA_NEAR_ITEM_CAP 8192
+#endif
+    const int64_t cap = std::max<int64_t>(MLFMA_NEAR_ITEM_CAP, (nnz + 8191) / 8192);
     int64_t voff = 0, parts = 0;
     std::vector<std::vector<int64_t>> sums(t.perm.size());
     for (int id = 0; id < (int)t.nodes.size(); ++id) {

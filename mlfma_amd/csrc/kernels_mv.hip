@@ -9,6 +9,17 @@ namespace mlfma {
 
 constexpr int TB = 256;
 
+// launch-shape parameters (compile-time; defaults measured on MI355X)
+#ifndef MLFMA_M2M_LDS_BUDGET
+#define MLFMA_M2M_LDS_BUDGET 32768   // LDS bytes per M2M block for staged children
+#endif
+#ifndef MLFMA_M2M_MIN_BLOCKS
+#define MLFMA_M2M_MIN_BLOCKS 1024    // split parent theta rows until this many blocks
+#endif
+#ifndef MLFMA_NEAR_LOADS
+#define MLFMA_NEAR_LOADS 6           // near-field values in flight per lane (6 or 8)
+#endif
+
 // ------------------------------------------------------------------ S2M
 // M_b[d] = sum_s x_s rp_b[s][d]  (src/fmm/leaf.cpp:212-230).  Block = one leaf
 // x blockDim directions; lanes stream consecutive 32-byte (theta, phi) pairs
@@ -55,57 +66,71 @@ void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* r
 }
 
 // ------------------------------------------------------------------ fused M2M
-// One block per parent box (src/fmm/stem.cpp:72-103,133-184): every non-empty
-// child field is phase-shifted into LDS, the theta interpolation (with the
-// pole rule: reflected node, phi + pi, both components negated) accumulates
-// in registers, the result goes to LDS and the periodic phi interpolation
-// writes the parent field.
+// One block per (parent box, run of parent theta rows) (src/fmm/stem.cpp:
+// 72-103,133-184).  The interpolation tables of the rows are staged in LDS;
+// the non-empty children are phase-shifted into LDS a batch at a time (one
+// global round trip per batch); the theta interpolation (pole rule: reflected
+// node, phi + pi, both components negated) accumulates in registers over all
+// children, goes to LDS, and the periodic phi interpolation writes the parent.
 template <typename R, int EPT>
 __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stemSlot, const int32_t* __restrict__ ptr,
                                                 const int32_t* __restrict__ child, const int32_t* __restrict__ oct,
-                                                int nstem, int per, int rows, int nthP, int nphP, int mthC, int mphC, int p,
-                                                const int32_t* __restrict__ thNear, const double* __restrict__ thW,
-                                                const int32_t* __restrict__ phNear, const double* __restrict__ phW,
-                                                const cplx<R>* __restrict__ phaseUp, const cplx<R>* __restrict__ Mc,
-                                                cplx<R>* __restrict__ Mp) {
+                                                int nstem, int per, int rows, int cb, int nthP, int nphP, int mthC,
+                                                int mphC, int p, const int32_t* __restrict__ thNear,
+                                                const double* __restrict__ thW, const int32_t* __restrict__ phNear,
+                                                const double* __restrict__ phW, const cplx<R>* __restrict__ phaseUp,
+                                                const cplx<R>* __restrict__ Mc, cplx<R>* __restrict__ Mp) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int item = xcdItem(blockIdx.x, per);
     if (item >= nstem) return;
-    // this block computes parent theta rows [j0, j0 + nr)
-    const int j0 = blockIdx.y * rows;
+    const int j0 = blockIdx.y * rows;  // parent theta rows [j0, j0 + nr)
     const int nr = min(rows, nthP - j0);
     const int ndirC = mthC * mphC;
     const int nInner = nr * mphC;
-    cplx<R>* sC = reinterpret_cast<cplx<R>*>(smem);
-    cplx<R>* sI = sC + 2 * ndirC;
+    const int tp = 2 * p;
+    cplx<R>* sC = reinterpret_cast<cplx<R>*>(smem);       // cb children x ndirC x 2
+    cplx<R>* sI = sC + (size_t)cb * 2 * ndirC;             // rows x mphC x 2
+    R* tThW = reinterpret_cast<R*>(sI + (size_t)2 * rows * mphC);
+    R* tPhW = tThW + rows * tp;
+    int* tThNear = reinterpret_cast<int*>(tPhW + nphP * tp);
+    int* tPhNear = tThNear + rows;
+    for (int i = threadIdx.x; i < nr * tp; i += TB) tThW[i] = (R)thW[j0 * tp + i];
+    for (int i = threadIdx.x; i < nphP * tp; i += TB) tPhW[i] = (R)phW[i];
+    for (int i = threadIdx.x; i < nr; i += TB) tThNear[i] = thNear[j0 + i];
+    for (int i = threadIdx.x; i < nphP; i += TB) tPhNear[i] = phNear[i];
     cplx<R> acc[EPT][2];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) acc[k][0] = acc[k][1] = mk<R>(0, 0);
-    for (int c = ptr[item]; c < ptr[item + 1]; ++c) {
-        __syncthreads();
-        const cplx<R>* Mb = Mc + (int64_t)child[c] * ndirC * 2;
-        const cplx<R>* ph = phaseUp + (int64_t)oct[c] * ndirC;
-        for (int d = threadIdx.x; d < ndirC; d += TB) {
-            const cplx<R> s = ph[d];
-            sC[2 * d] = s * Mb[2 * d];
-            sC[2 * d + 1] = s * Mb[2 * d + 1];
+    const int c0 = ptr[item], c1 = ptr[item + 1];
+    for (int cbeg = c0; cbeg < c1; cbeg += cb) {
+        const int nb = min(cb, c1 - cbeg);
+        __syncthreads();  // tables staged / previous batch consumed
+        for (int e = threadIdx.x; e < nb * ndirC; e += TB) {
+            const int k = e / ndirC, d = e - k * ndirC;
+            const int c = cbeg + k;
+            const cplx<R>* Mb = Mc + (int64_t)child[c] * ndirC * 2;
+            const cplx<R> s = phaseUp[(int64_t)oct[c] * ndirC + d];
+            sC[2 * e] = s * Mb[2 * d];
+            sC[2 * e + 1] = s * Mb[2 * d + 1];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = threadIdx.x + k * TB;
             if (e < nInner) {
-                const int jth = j0 + e / mphC, iph = e - (e / mphC) * mphC;
-                const int near = thNear[jth];
+                const int jr = e / mphC, iph = e - jr * mphC;
+                const int near = tThNear[jr];
                 const int iphf = iph + (iph < mphC / 2 ? mphC / 2 : -mphC / 2);
-                for (int q = 0; q < 2 * p; ++q) {
+                for (int q = 0; q < tp; ++q) {
                     const int i = near + 1 - p + q;
                     const int f = i < 0 ? -i - 1 : (i >= mthC ? 2 * mthC - i - 1 : i);
                     const bool flip = (i != f);
-                    const int dd = f * mphC + (flip ? iphf : iph);
-                    const R w = (R)(flip ? -thW[jth * 2 * p + q] : thW[jth * 2 * p + q]);
-                    acc[k][0] = acc[k][0] + w * sC[2 * dd];
-                    acc[k][1] = acc[k][1] + w * sC[2 * dd + 1];
+                    const int dd = 2 * (f * mphC + (flip ? iphf : iph));
+                    const R w = flip ? -tThW[jr * tp + q] : tThW[jr * tp + q];
+                    for (int b = 0; b < nb; ++b) {
+                        acc[k][0] = acc[k][0] + w * sC[b * 2 * ndirC + dd];
+                        acc[k][1] = acc[k][1] + w * sC[b * 2 * ndirC + dd + 1];
+                    }
                 }
             }
         }
@@ -122,13 +147,13 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
     cplx<R>* out = Mp + ((int64_t)stemSlot[item] * nthP + j0) * nphP * 2;
     for (int e = threadIdx.x; e < nr * nphP; e += TB) {
         const int jr = e / nphP, jph = e - jr * nphP;
-        const int near = phNear[jph];
+        const int near = tPhNear[jph];
         const cplx<R>* in = sI + 2 * jr * mphC;
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
-        for (int q = 0; q < 2 * p; ++q) {
+        for (int q = 0; q < tp; ++q) {
             int i = near + 1 - p + q;
             i = i < 0 ? i + mphC : (i >= mphC ? i - mphC : i);
-            const R w = (R)phW[jph * 2 * p + q];
+            const R w = tPhW[jph * tp + q];
             a0 = a0 + w * in[2 * i];
             a1 = a1 + w * in[2 * i + 1];
         }
@@ -146,16 +171,27 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
     // split the parent's theta rows so that there are enough blocks for the
     // device (each block re-reads the children's fields from L2)
     int rows = nthP;
-    while (rows > 2 && (int64_t)nstem * ((nthP + rows - 1) / rows) < 1024) rows = (rows + 1) / 2;
-    const size_t lds = sizeof(cplx<R>) * 2 * ((size_t)mthC * mphC + (size_t)rows * mphC);
+    while (rows > 2 && (int64_t)nstem * ((nthP + rows - 1) / rows) < MLFMA_M2M_MIN_BLOCKS) rows = (rows + 1) / 2;
+    const int tp = 2 * p;
+    const size_t fixed = sizeof(cplx<R>) * 2 * (size_t)rows * mphC + sizeof(R) * ((size_t)rows + nphP) * tp +
+                        sizeof(int) * ((size_t)rows + nphP);
+    const size_t perChild = sizeof(cplx<R>) * 2 * (size_t)mthC * mphC;
+    // children per batch: as many as fit the LDS budget (several blocks per CU)
+    const size_t budget = std::min<size_t>(ldsMax, MLFMA_M2M_LDS_BUDGET);
+    int cb = fixed < budget ? (int)std::min<size_t>(8, (budget - fixed) / perChild) : 0;
+    if (cb < 1) {
+        if (fixed + perChild > ldsMax) return false;
+        cb = 1;
+    }
+    const size_t lds = fixed + perChild * cb;
     const int nInner = rows * mphC;
     const int ept = (nInner + TB - 1) / TB;
-    if (lds > ldsMax || ept > 16) return false;
+    if (ept > 16) return false;
     const int per = (nstem + 7) / 8;
     dim3 g(8 * per, (nthP + rows - 1) / rows);
-#define M2M_CASE(E)                                                                                              \
-    hipLaunchKernelGGL((kM2MFused<R, E>), g, dim3(TB), lds, st, stemSlot, ptr, child, oct, nstem, per, rows, nthP, \
-                       nphP, mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp)
+#define M2M_CASE(E)                                                                                                \
+    hipLaunchKernelGGL((kM2MFused<R, E>), g, dim3(TB), lds, st, stemSlot, ptr, child, oct, nstem, per, rows, cb,  \
+                       nthP, nphP, mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp)
     if (ept <= 1) M2M_CASE(1);
     else if (ept <= 2) M2M_CASE(2);
     else if (ept <= 4) M2M_CASE(4);
@@ -227,17 +263,18 @@ void launchM2Lx(M2LLevels<R> lv, hipStream_t st) {
 
 // ------------------------------------------------------------------ fused L2L
 // One block per child box whose parent has a non-empty interaction list
-// (src/fmm/stem.cpp:109-131,186-244): the parent local, phase-shifted to the
-// child centre on the parent grid, is staged in LDS; the phi anterpolation
-// (gather form of the transpose) goes to LDS; the theta anterpolation with the
-// pole rule is added into the child local.
+// (src/fmm/stem.cpp:109-131,186-244).  The gather-form anterpolation tables
+// are staged in LDS; the parent local, phase-shifted to the child centre on
+// the parent grid, is staged in LDS; the phi anterpolation goes to LDS; the
+// theta anterpolation with the pole rule is added into the child local.
 template <typename R>
 __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ parent, const int32_t* __restrict__ child,
                                                 const int32_t* __restrict__ oct, int nitems, int per, int mthP,
                                                 int mphP, int nthC, int nphC, const int32_t* __restrict__ atPhPtr,
                                                 const int32_t* __restrict__ atPhSrc, const double* __restrict__ atPhW,
-                                                const int32_t* __restrict__ atThPtr, const int32_t* __restrict__ atThSrc,
-                                                const double* __restrict__ atThW, const int32_t* __restrict__ atThFlip,
+                                                int nAtPh, const int32_t* __restrict__ atThPtr,
+                                                const int32_t* __restrict__ atThSrc, const double* __restrict__ atThW,
+                                                const int32_t* __restrict__ atThFlip, int nAtTh,
                                                 const cplx<R>* __restrict__ phaseDn, const cplx<R>* __restrict__ Lp,
                                                 cplx<R>* __restrict__ Lc) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -247,6 +284,23 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     const int nInner = mthP * nphC;
     cplx<R>* sP = reinterpret_cast<cplx<R>*>(smem);
     cplx<R>* sI = sP + 2 * ndirP;
+    R* tPhW = reinterpret_cast<R*>(sI + 2 * nInner);
+    R* tThW = tPhW + nAtPh;                       // pole flips folded into the sign
+    int* tPhPtr = reinterpret_cast<int*>(tThW + nAtTh);
+    int* tPhSrc = tPhPtr + nphC + 1;
+    int* tThPtr = tPhSrc + nAtPh;
+    int* tThSrc = tThPtr + nthC + 1;              // 2 * coarse theta + flip
+    for (int i = threadIdx.x; i < nAtPh; i += TB) {
+        tPhW[i] = (R)atPhW[i];
+        tPhSrc[i] = atPhSrc[i];
+    }
+    for (int i = threadIdx.x; i < nAtTh; i += TB) {
+        const bool flip = atThFlip[i] != 0;
+        tThW[i] = (R)(flip ? -atThW[i] : atThW[i]);
+        tThSrc[i] = 2 * atThSrc[i] + (flip ? 1 : 0);
+    }
+    for (int i = threadIdx.x; i <= nphC; i += TB) tPhPtr[i] = atPhPtr[i];
+    for (int i = threadIdx.x; i <= nthC; i += TB) tThPtr[i] = atThPtr[i];
     const cplx<R>* Lb = Lp + (int64_t)parent[item] * ndirP * 2;
     const cplx<R>* ph = phaseDn + (int64_t)oct[item] * ndirP;
     for (int d = threadIdx.x; d < ndirP; d += TB) {
@@ -257,12 +311,13 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     __syncthreads();
     for (int e = threadIdx.x; e < nInner; e += TB) {
         const int ith = e / nphC, jph = e - ith * nphC;
+        const cplx<R>* row = sP + 2 * ith * mphP;
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
-        for (int q = atPhPtr[jph]; q < atPhPtr[jph + 1]; ++q) {
-            const int d = ith * mphP + atPhSrc[q];
-            const R w = (R)atPhW[q];
-            a0 = a0 + w * sP[2 * d];
-            a1 = a1 + w * sP[2 * d + 1];
+        for (int q = tPhPtr[jph]; q < tPhPtr[jph + 1]; ++q) {
+            const int d = 2 * tPhSrc[q];
+            const R w = tPhW[q];
+            a0 = a0 + w * row[d];
+            a1 = a1 + w * row[d + 1];
         }
         sI[2 * e] = a0;
         sI[2 * e + 1] = a1;
@@ -273,12 +328,12 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
         const int jf = e / nphC, js = e - jf * nphC;
         const int jsf = js + (js < nphC / 2 ? nphC / 2 : -nphC / 2);
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
-        for (int q = atThPtr[jf]; q < atThPtr[jf + 1]; ++q) {
-            const bool flip = atThFlip[q] != 0;
-            const R w = (R)(flip ? -atThW[q] : atThW[q]);
-            const int d = atThSrc[q] * nphC + (flip ? jsf : js);
-            a0 = a0 + w * sI[2 * d];
-            a1 = a1 + w * sI[2 * d + 1];
+        for (int q = tThPtr[jf]; q < tThPtr[jf + 1]; ++q) {
+            const int sf = tThSrc[q];
+            const int d = 2 * ((sf >> 1) * nphC + ((sf & 1) ? jsf : js));
+            const R w = tThW[q];
+            a0 = a0 + w * sI[d];
+            a1 = a1 + w * sI[d + 1];
         }
         out[2 * e] = out[2 * e] + a0;
         out[2 * e + 1] = out[2 * e + 1] + a1;
@@ -287,15 +342,19 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
 
 template <typename R>
 bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
-                    int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW,
+                    int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW, int nAtPh,
                     const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
-                    const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax, hipStream_t st) {
+                    int nAtTh, const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax,
+                    hipStream_t st) {
     if (nitems <= 0) return true;
-    const size_t lds = sizeof(cplx<R>) * 2 * ((size_t)mthP * mphP + (size_t)mthP * nphC);
+    const size_t lds = sizeof(cplx<R>) * 2 * ((size_t)mthP * mphP + (size_t)mthP * nphC) +
+                       sizeof(R) * ((size_t)nAtPh + nAtTh) +
+                       sizeof(int) * ((size_t)nphC + 1 + nAtPh + nthC + 1 + nAtTh);
     if (lds > ldsMax) return false;
     const int per = (nitems + 7) / 8;
     hipLaunchKernelGGL(kL2LFused<R>, dim3(8 * per), dim3(TB), lds, st, parent, child, oct, nitems, per, mthP, mphP,
-                       nthC, nphC, atPhPtr, atPhSrc, atPhW, atThPtr, atThSrc, atThW, atThFlip, phaseDn, Lp, Lc);
+                       nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip, nAtTh, phaseDn,
+                       Lp, Lc);
     return true;
 }
 
@@ -307,7 +366,8 @@ bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, con
 // the pair columns, column sums Z^T x_rows.  Tiles of NT values are staged in
 // LDS with a leading dimension made odd (conflict-free column reads); the
 // partial sums go to a buffer that kNearSum folds in a fixed order.
-constexpr int NT = 6 * TB;  // six values per lane per tile
+static_assert(MLFMA_NEAR_LOADS == 6 || MLFMA_NEAR_LOADS == 8, "near-field loads per lane: 6 or 8");
+constexpr int NT = MLFMA_NEAR_LOADS * TB;  // values per tile
 
 __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
                                                const int64_t* __restrict__ perm, const cplx<double>* __restrict__ val,
@@ -323,44 +383,73 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     const int G = TB / nr;
     const int r = threadIdx.x % nr, g = threadIdx.x / nr;
     double re = 0, im = 0;
-    for (int c0 = 0; c0 < I.nc; c0 += cwMax) {
-        const int cw = min(cwMax, I.nc - c0);
-        const int ne = cw * nr;
-        const cplx<double>* src = val + I.voff + (int64_t)c0 * nr;
-        // six independent loads in flight per lane, held as plain doubles (a
-        // struct or array here is left in scratch memory by the compiler); the
-        // index is clamped into the tile so every load is valid
-        const int t0 = threadIdx.x;
-        const double* sd = reinterpret_cast<const double*>(src);
-#define NEAR_LD(K) \
-    const int e##K = t0 + K * TB; \
-    const int q##K = 2 * min(e##K, ne - 1); \
-    const double r##K = sd[q##K], i##K = sd[q##K + 1];
-        NEAR_LD(0) NEAR_LD(1) NEAR_LD(2) NEAR_LD(3) NEAR_LD(4) NEAR_LD(5)
-#undef NEAR_LD
-        cplx<double> xc = mk(0.0, 0.0);
-        if (threadIdx.x < cw) xc = x[colsIn[I.cptr + c0 + threadIdx.x]];
+    // Software pipeline: the six loads per lane of tile k+1 are issued before
+    // the LDS work of tile k.  Values are held as plain doubles (a struct or
+    // array here is left in scratch memory by the compiler); indices are
+    // clamped into the tile so every load is valid.
+    const int t0 = threadIdx.x;
+    int c0 = 0;
+    int cw = min(cwMax, I.nc);
+    int ne = cw * nr;
+    double r0, r1, r2, r3, r4, r5, i0, i1, i2, i3, i4, i5;
+#if MLFMA_NEAR_LOADS == 8
+    double r6, r7, i6, i7;
+#endif
+    cplx<double> xc = mk(0.0, 0.0);
+#define NEAR_LD1(K, C0, NE)                                                       \
+    {                                                                            \
+        const int q = 2 * min(t0 + K * TB, (NE) - 1);                            \
+        r##K = sd[q];                                                            \
+        i##K = sd[q + 1];                                                        \
+    }
+#define NEAR_LD(C0, NE, CW)                                                                          \
+    {                                                                                               \
+        const double* sd = reinterpret_cast<const double*>(val + I.voff + (int64_t)(C0) * nr);      \
+        NEAR_LD1(0, C0, NE) NEAR_LD1(1, C0, NE) NEAR_LD1(2, C0, NE)                                 \
+        NEAR_LD1(3, C0, NE) NEAR_LD1(4, C0, NE) NEAR_LD1(5, C0, NE)                                 \
+        NEAR_LD_EXTRA(C0, NE)                                                                       \
+        if (t0 < (CW)) xc = x[colsIn[I.cptr + (C0) + t0]];                                          \
+    }
+#if MLFMA_NEAR_LOADS == 8
+#define NEAR_LD_EXTRA(C0, NE) NEAR_LD1(6, C0, NE) NEAR_LD1(7, C0, NE)
+#else
+#define NEAR_LD_EXTRA(C0, NE)
+#endif
+    NEAR_LD(0, ne, cw)
+    double* sTd = reinterpret_cast<double*>(sT);
+    while (true) {
         __syncthreads();  // the previous tile is consumed
-        double* sTd = reinterpret_cast<double*>(sT);
-#define NEAR_ST(K) \
-    if (e##K < ne) { \
-        const int c = e##K / nr; \
-        const int a = 2 * (c * ld + (e##K - c * nr)); \
-        sTd[a] = r##K; \
-        sTd[a + 1] = i##K; \
+#define NEAR_ST(K)                                            \
+    if (t0 + K * TB < ne) {                                   \
+        const int e = t0 + K * TB;                            \
+        const int c = e / nr;                                 \
+        const int a = 2 * (c * ld + (e - c * nr));            \
+        sTd[a] = r##K;                                        \
+        sTd[a + 1] = i##K;                                    \
     }
         NEAR_ST(0) NEAR_ST(1) NEAR_ST(2) NEAR_ST(3) NEAR_ST(4) NEAR_ST(5)
+#if MLFMA_NEAR_LOADS == 8
+        NEAR_ST(6) NEAR_ST(7)
+#endif
 #undef NEAR_ST
-        if (threadIdx.x < cw) sxc[threadIdx.x] = xc;
+        if (t0 < cw) sxc[t0] = xc;
         __syncthreads();
+        const int c0n = c0 + cwMax;
+        const bool more = c0n < I.nc;  // uniform over the block
+        int cwn = 0, nen = 0;
+        if (more) {
+            cwn = min(cwMax, I.nc - c0n);
+            nen = cwn * nr;
+            NEAR_LD(c0n, nen, cwn)
+        }
         if (g < G)
             for (int c = g; c < cw; c += G) {
                 const cplx<double> z = sT[c * ld + r], xx = sxc[c];
                 re += z.re * xx.re - z.im * xx.im;
                 im += z.re * xx.im + z.im * xx.re;
             }
-        if (threadIdx.x < cw && c0 + (int)threadIdx.x >= I.nself) {
-            const cplx<double>* col = sT + threadIdx.x * ld;
+        if (t0 < cw && c0 + t0 >= I.nself) {
+            const cplx<double>* col = sT + t0 * ld;
             double cr = 0, ci = 0, cr2 = 0, ci2 = 0;
             int q = 0;
             for (; q + 1 < nr; q += 2) {
@@ -375,9 +464,16 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
                 cr += z0.re * x0.re - z0.im * x0.im;
                 ci += z0.re * x0.im + z0.im * x0.re;
             }
-            part[I.cpo + c0 + threadIdx.x - I.nself] = mk(cr + cr2, ci + ci2);
+            part[I.cpo + c0 + t0 - I.nself] = mk(cr + cr2, ci + ci2);
         }
+        if (!more) break;
+        c0 = c0n;
+        cw = cwn;
+        ne = nen;
     }
+#undef NEAR_LD
+#undef NEAR_LD1
+#undef NEAR_LD_EXTRA
     sre[threadIdx.x] = (g < G) ? re : 0.0;
     sim[threadIdx.x] = (g < G) ? im : 0.0;
     __syncthreads();
@@ -487,9 +583,9 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
                                     const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t);                 \
     template void launchM2Lx<R>(M2LLevels<R>, hipStream_t);                                                         \
     template bool launchL2LFused<R>(int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,         \
-                                    const int32_t*, const int32_t*, const double*, const int32_t*, const int32_t*,  \
-                                    const double*, const int32_t*, const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, \
-                                    hipStream_t);                                                                   \
+                                    const int32_t*, const int32_t*, const double*, int, const int32_t*,             \
+                                    const int32_t*, const double*, const int32_t*, int, const cplx<R>*,             \
+                                    const cplx<R>*, cplx<R>*, size_t, hipStream_t);                                 \
     template void launchL2Tx<R>(const int64_t*, const int32_t*, int64_t, const LeafItem*, const cplx<R>*,           \
                                 const cplx<R>*, cplx<double>, const int64_t*, const int32_t*, const cplx<double>*,  \
                                 const int64_t*, cplx<double>*, int, hipStream_t);

@@ -37,9 +37,10 @@ void launchM2Lx(M2LLevels<R> lv, hipStream_t st);
 
 template <typename R>
 bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
-                    int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW,
+                    int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW, int nAtPh,
                     const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
-                    const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax, hipStream_t st);
+                    int nAtTh, const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax,
+                    hipStream_t st);
 
 // near field: kNearSym writes partial sums; kL2Tx folds them into the rows of
 // far-field leaves and kNearSum into the listed remaining rows

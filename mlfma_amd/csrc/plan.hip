@@ -340,6 +340,8 @@ void DevicePlan::build() {
             L.atThSrc = upload(I.atThSrc);
             L.atThW = upload(I.atThW);
             L.atThFlip = upload(I.atThFlip);
+            L.nAtPh = (int)I.atPhSrc.size();
+            L.nAtTh = (int)I.atThSrc.size();
             // M2M into level l: stems with non-empty children
             std::vector<int32_t> stem, ptr{0}, ch, oc;
             const auto& P = h.m2mPtr[l];
@@ -524,8 +526,8 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];
         if (!launchL2LFused<R>(C.nl2l, C.l2lParent, C.l2lChild, C.l2lOct, P.nth, P.nph, C.nth, C.nph, P.atPhPtr,
-                               P.atPhSrc, P.atPhW, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, P.phDn<R>(),
-                               Lc + P.Moff, Lc + C.Moff, ldsMax, st)) {
+                               P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, P.nAtTh,
+                               P.phDn<R>(), Lc + P.Moff, Lc + C.Moff, ldsMax, st)) {
             launchL2LPhi<R>(C.nl2l, C.l2lParent, C.l2lOct, P.nth, P.nph, C.nph, P.atPhPtr, P.atPhSrc, P.atPhW,
                             P.phDn<R>(), Lc + P.Moff, inner, st);
             launchL2LTheta<R>(C.nl2l, C.l2lChild, P.nth, C.nth, C.nph, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip,

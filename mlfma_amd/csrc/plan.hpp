@@ -37,6 +37,7 @@ struct LevelBufs {
     double *thW = nullptr, *phW = nullptr;
     int32_t *atPhPtr = nullptr, *atPhSrc = nullptr, *atThPtr = nullptr, *atThSrc = nullptr, *atThFlip = nullptr;
     double *atPhW = nullptr, *atThW = nullptr;
+    int nAtPh = 0, nAtTh = 0;  // taps of the gather-form anterpolation tables
     // M2M into this level
     int nstem = 0;
     int32_t *stemSlot = nullptr, *mPtr = nullptr, *mChild = nullptr, *mOct = nullptr;
