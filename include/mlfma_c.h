@@ -149,6 +149,15 @@ int mlfma_voltage(const mlfma_plan* plan, double* v_host);
 int mlfma_matvec(mlfma_plan* plan, const double* x, double* y, int32_t nrhs,
                  int32_t accumulate, int32_t ptr_mode, void* stream);
 
+/* y = Z^H x, or y += Z^H x: the exact adjoint of the matvec above (the
+ * reference has none; it is the transposed stage chain: S2M <-> L2T swapped,
+ * L2L and M2M transposed with conjugated phases, M2L with conjugated operators
+ * of the reversed pairs, conjugated near field).  Arguments as mlfma_matvec.
+ * MLFMA_ERR_STATE when the plan keeps stale locals (info.stale_boxes > 0 and
+ * MLFMA_FLAG_FRESH_LOCALS not set): that operator is not linear. */
+int mlfma_matvec_adjoint(mlfma_plan* plan, const double* x, double* y, int32_t nrhs,
+                         int32_t accumulate, int32_t ptr_mode, void* stream);
+
 /* Dense reference matvec over the same near-field kernel, all pairs
  * (the reference's evalDirect path, src/mlfma.cpp:84-107); for small n. */
 int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t ptr_mode,

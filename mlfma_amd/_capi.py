@@ -80,6 +80,8 @@ SIGNATURES = {
     "mlfma_voltage": (C.c_int, [C.c_void_p, dptr]),
     "mlfma_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                C.c_int32, C.c_void_p]),
+    "mlfma_matvec_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_void_p]),
     "mlfma_matvec_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "mlfma_solve": (C.c_int, [C.c_void_p, dptr, dptr, C.POINTER(SolveOpts), C.POINTER(SolveReport),
                               dptr]),

@@ -172,6 +172,7 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
     // device (each block re-reads the children's fields from L2)
     int rows = nthP;
     while (rows > 2 && (int64_t)nstem * ((nthP + rows - 1) / rows) < MLFMA_M2M_MIN_BLOCKS) rows = (rows + 1) / 2;
+    while (rows > 1 && rows * mphC > 16 * TB) rows = (rows + 1) / 2;  // at most 16 outputs per thread
     const int tp = 2 * p;
     const size_t fixed = sizeof(cplx<R>) * 2 * (size_t)rows * mphC + sizeof(R) * ((size_t)rows + nphP) * tp +
                         sizeof(int) * ((size_t)rows + nphP);
@@ -208,7 +209,7 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
 // is read with uniform loads, two pairs in flight; within a level the targets
 // of one XCD are spatially contiguous so their shared sources stay in that
 // XCD's L2.
-template <typename R>
+template <typename R, bool CONJ>
 __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
     int b = blockIdx.x, l = 0;
     while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
@@ -228,8 +229,12 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
     cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
     int q = q0;
     for (; q + 1 < q1; q += 2) {
-        const cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
-        const cplx<R> t1 = T[(int64_t)off[q + 1] * ndir + d];
+        cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
+        cplx<R> t1 = T[(int64_t)off[q + 1] * ndir + d];
+        if (CONJ) {
+            t0.im = -t0.im;
+            t1.im = -t1.im;
+        }
         const cplx<R>* m0 = M + ((int64_t)src[q] * ndir + d) * 2;
         const cplx<R>* m1 = M + ((int64_t)src[q + 1] * ndir + d) * 2;
         a0 = a0 + t0 * m0[0];
@@ -238,7 +243,8 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
         b1 = b1 + t1 * m1[1];
     }
     if (q < q1) {
-        const cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
+        cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
+        if (CONJ) t0.im = -t0.im;
         const cplx<R>* m0 = M + ((int64_t)src[q] * ndir + d) * 2;
         a0 = a0 + t0 * m0[0];
         a1 = a1 + t0 * m0[1];
@@ -249,7 +255,7 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
 }
 
 template <typename R>
-void launchM2Lx(M2LLevels<R> lv, hipStream_t st) {
+void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st) {
     int total = 0;
     for (int l = 0; l < lv.nlev; ++l) {
         lv.nch[l] = (lv.ndir[l] + TB - 1) / TB;
@@ -258,7 +264,10 @@ void launchM2Lx(M2LLevels<R> lv, hipStream_t st) {
         total += 8 * lv.per[l] * lv.nch[l];
     }
     if (total <= 0) return;
-    hipLaunchKernelGGL(kM2Lx<R>, dim3(total), dim3(TB), 0, st, lv);
+    if (conjT)
+        hipLaunchKernelGGL((kM2Lx<R, true>), dim3(total), dim3(TB), 0, st, lv);
+    else
+        hipLaunchKernelGGL((kM2Lx<R, false>), dim3(total), dim3(TB), 0, st, lv);
 }
 
 // ------------------------------------------------------------------ fused L2L
@@ -267,7 +276,7 @@ void launchM2Lx(M2LLevels<R> lv, hipStream_t st) {
 // are staged in LDS; the parent local, phase-shifted to the child centre on
 // the parent grid, is staged in LDS; the phi anterpolation goes to LDS; the
 // theta anterpolation with the pole rule is added into the child local.
-template <typename R>
+template <typename R, bool ADJ>
 __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ parent, const int32_t* __restrict__ child,
                                                 const int32_t* __restrict__ oct, int nitems, int per, int mthP,
                                                 int mphP, int nthC, int nphC, const int32_t* __restrict__ atPhPtr,
@@ -277,6 +286,8 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
                                                 const int32_t* __restrict__ atThFlip, int nAtTh,
                                                 const cplx<R>* __restrict__ phaseDn, const cplx<R>* __restrict__ Lp,
                                                 cplx<R>* __restrict__ Lc) {
+    // ADJ: the adjoint of M2M for one child, conj(phase_up) (child grid) times
+    // the anterpolated parent field; phaseDn then holds the child-grid phases
     extern __shared__ __align__(16) unsigned char smem[];
     const int item = xcdItem(blockIdx.x, per);
     if (item >= nitems) return;
@@ -302,11 +313,15 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     for (int i = threadIdx.x; i <= nphC; i += TB) tPhPtr[i] = atPhPtr[i];
     for (int i = threadIdx.x; i <= nthC; i += TB) tThPtr[i] = atThPtr[i];
     const cplx<R>* Lb = Lp + (int64_t)parent[item] * ndirP * 2;
-    const cplx<R>* ph = phaseDn + (int64_t)oct[item] * ndirP;
-    for (int d = threadIdx.x; d < ndirP; d += TB) {
-        const cplx<R> s = ph[d];
-        sP[2 * d] = s * Lb[2 * d];
-        sP[2 * d + 1] = s * Lb[2 * d + 1];
+    if (ADJ) {
+        for (int d = threadIdx.x; d < 2 * ndirP; d += TB) sP[d] = Lb[d];
+    } else {
+        const cplx<R>* ph = phaseDn + (int64_t)oct[item] * ndirP;
+        for (int d = threadIdx.x; d < ndirP; d += TB) {
+            const cplx<R> s = ph[d];
+            sP[2 * d] = s * Lb[2 * d];
+            sP[2 * d + 1] = s * Lb[2 * d + 1];
+        }
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nInner; e += TB) {
@@ -335,13 +350,19 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
             a0 = a0 + w * sI[d];
             a1 = a1 + w * sI[d + 1];
         }
+        if (ADJ) {
+            cplx<R> s = phaseDn[(int64_t)oct[item] * nthC * nphC + e];
+            s.im = -s.im;
+            a0 = s * a0;
+            a1 = s * a1;
+        }
         out[2 * e] = out[2 * e] + a0;
         out[2 * e + 1] = out[2 * e + 1] + a1;
     }
 }
 
 template <typename R>
-bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
+bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
                     int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW, int nAtPh,
                     const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
                     int nAtTh, const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax,
@@ -352,10 +373,148 @@ bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, con
                        sizeof(int) * ((size_t)nphC + 1 + nAtPh + nthC + 1 + nAtTh);
     if (lds > ldsMax) return false;
     const int per = (nitems + 7) / 8;
-    hipLaunchKernelGGL(kL2LFused<R>, dim3(8 * per), dim3(TB), lds, st, parent, child, oct, nitems, per, mthP, mphP,
-                       nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip, nAtTh, phaseDn,
-                       Lp, Lc);
+    if (adjoint)
+        hipLaunchKernelGGL((kL2LFused<R, true>), dim3(8 * per), dim3(TB), lds, st, parent, child, oct, nitems, per,
+                           mthP, mphP, nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip,
+                           nAtTh, phaseDn, Lp, Lc);
+    else
+        hipLaunchKernelGGL((kL2LFused<R, false>), dim3(8 * per), dim3(TB), lds, st, parent, child, oct, nitems, per,
+                           mthP, mphP, nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip,
+                           nAtTh, phaseDn, Lp, Lc);
     return true;
+}
+
+// ------------------------------------------------------------------ L2L adjoint
+// The adjoint of L2L (src/fmm/stem.cpp:109-131,186-244): the parent gains
+// conj(phase_down) (parent grid) times the interpolated child field, summed
+// over the children that L2L feeds.  One block per (parent, run of parent
+// theta rows); per child the theta interpolation (pole rule) goes to LDS and
+// the phi interpolation, phase and sum stay in registers.
+template <typename R, int EPT>
+__global__ __launch_bounds__(TB) void kL2LAdj(const int32_t* __restrict__ parSlot, const int32_t* __restrict__ ptr,
+                                              const int32_t* __restrict__ child, const int32_t* __restrict__ oct,
+                                              int npar, int per, int rows, int nthP, int nphP, int mthC, int mphC,
+                                              int p, const int32_t* __restrict__ thNear, const double* __restrict__ thW,
+                                              const int32_t* __restrict__ phNear, const double* __restrict__ phW,
+                                              const cplx<R>* __restrict__ phaseDn, const cplx<R>* __restrict__ Lc,
+                                              cplx<R>* __restrict__ Lp) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int item = xcdItem(blockIdx.x, per);
+    if (item >= npar) return;
+    const int j0 = blockIdx.y * rows;
+    const int nr = min(rows, nthP - j0);
+    const int ndirC = mthC * mphC;
+    const int ndirP = nthP * nphP;
+    const int nInner = nr * mphC;
+    const int nOut = nr * nphP;
+    const int tp = 2 * p;
+    cplx<R>* sC = reinterpret_cast<cplx<R>*>(smem);  // ndirC x 2
+    cplx<R>* sI = sC + 2 * ndirC;                     // rows x mphC x 2
+    R* tThW = reinterpret_cast<R*>(sI + (size_t)2 * rows * mphC);
+    R* tPhW = tThW + rows * tp;
+    int* tThNear = reinterpret_cast<int*>(tPhW + nphP * tp);
+    int* tPhNear = tThNear + rows;
+    for (int i = threadIdx.x; i < nr * tp; i += TB) tThW[i] = (R)thW[j0 * tp + i];
+    for (int i = threadIdx.x; i < nphP * tp; i += TB) tPhW[i] = (R)phW[i];
+    for (int i = threadIdx.x; i < nr; i += TB) tThNear[i] = thNear[j0 + i];
+    for (int i = threadIdx.x; i < nphP; i += TB) tPhNear[i] = phNear[i];
+    cplx<R> acc[EPT][2];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) acc[k][0] = acc[k][1] = mk<R>(0, 0);
+    for (int c = ptr[item]; c < ptr[item + 1]; ++c) {
+        __syncthreads();
+        const cplx<R>* Cb = Lc + (int64_t)child[c] * ndirC * 2;
+        for (int d = threadIdx.x; d < 2 * ndirC; d += TB) sC[d] = Cb[d];
+        __syncthreads();
+        for (int e = threadIdx.x; e < nInner; e += TB) {
+            const int jr = e / mphC, iph = e - jr * mphC;
+            const int near = tThNear[jr];
+            const int iphf = iph + (iph < mphC / 2 ? mphC / 2 : -mphC / 2);
+            cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+            for (int q = 0; q < tp; ++q) {
+                const int i = near + 1 - p + q;
+                const int f = i < 0 ? -i - 1 : (i >= mthC ? 2 * mthC - i - 1 : i);
+                const bool flip = (i != f);
+                const int dd = 2 * (f * mphC + (flip ? iphf : iph));
+                const R w = flip ? -tThW[jr * tp + q] : tThW[jr * tp + q];
+                a0 = a0 + w * sC[dd];
+                a1 = a1 + w * sC[dd + 1];
+            }
+            sI[2 * e] = a0;
+            sI[2 * e + 1] = a1;
+        }
+        __syncthreads();
+        const cplx<R>* ph = phaseDn + (int64_t)oct[c] * ndirP + (int64_t)j0 * nphP;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const int e = threadIdx.x + k * TB;
+            if (e < nOut) {
+                const int jr = e / nphP, jph = e - jr * nphP;
+                const int near = tPhNear[jph];
+                const cplx<R>* in = sI + 2 * jr * mphC;
+                cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+                for (int q = 0; q < tp; ++q) {
+                    int i = near + 1 - p + q;
+                    i = i < 0 ? i + mphC : (i >= mphC ? i - mphC : i);
+                    const R w = tPhW[jph * tp + q];
+                    a0 = a0 + w * in[2 * i];
+                    a1 = a1 + w * in[2 * i + 1];
+                }
+                cplx<R> s = ph[e];
+                s.im = -s.im;
+                acc[k][0] = acc[k][0] + s * a0;
+                acc[k][1] = acc[k][1] + s * a1;
+            }
+        }
+    }
+    cplx<R>* out = Lp + ((int64_t)parSlot[item] * nthP + j0) * nphP * 2;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int e = threadIdx.x + k * TB;
+        if (e < nOut) {
+            out[2 * e] = out[2 * e] + acc[k][0];
+            out[2 * e + 1] = out[2 * e + 1] + acc[k][1];
+        }
+    }
+}
+
+template <typename R>
+bool launchL2LAdj(int npar, const int32_t* parSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
+                  int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
+                  const int32_t* phNear, const double* phW, const cplx<R>* phaseDn, const cplx<R>* Lc, cplx<R>* Lp,
+                  size_t ldsMax, hipStream_t st) {
+    if (npar <= 0) return true;
+    int rows = nthP;
+    while (rows > 2 && (int64_t)npar * ((nthP + rows - 1) / rows) < MLFMA_M2M_MIN_BLOCKS) rows = (rows + 1) / 2;
+    while (rows > 1 && rows * nphP > 16 * TB) rows = (rows + 1) / 2;  // at most 16 outputs per thread
+    const int tp = 2 * p;
+    const size_t lds = sizeof(cplx<R>) * 2 * ((size_t)mthC * mphC + (size_t)rows * mphC) +
+                       sizeof(R) * ((size_t)rows + nphP) * tp + sizeof(int) * ((size_t)rows + nphP);
+    const int ept = (rows * nphP + TB - 1) / TB;
+    if (lds > ldsMax || ept > 16) return false;
+    const int per = (npar + 7) / 8;
+    dim3 g(8 * per, (nthP + rows - 1) / rows);
+#define ADJ_CASE(E)                                                                                                \
+    hipLaunchKernelGGL((kL2LAdj<R, E>), g, dim3(TB), lds, st, parSlot, ptr, child, oct, npar, per, rows, nthP, nphP, \
+                       mthC, mphC, p, thNear, thW, phNear, phW, phaseDn, Lc, Lp)
+    if (ept <= 1) ADJ_CASE(1);
+    else if (ept <= 2) ADJ_CASE(2);
+    else if (ept <= 4) ADJ_CASE(4);
+    else if (ept <= 8) ADJ_CASE(8);
+    else ADJ_CASE(16);
+#undef ADJ_CASE
+    return true;
+}
+
+template <typename R>
+__global__ void kZeroField(cplx<R>* __restrict__ f, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n) f[i] = mk<R>(0, 0);
+}
+
+template <typename R>
+void launchZeroField(cplx<R>* f, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(kZeroField<R>, dim3((unsigned)((n + TB - 1) / TB)), dim3(TB), 0, st, f, n);
 }
 
 // ------------------------------------------------------------------ near field
@@ -369,6 +528,7 @@ bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, con
 static_assert(MLFMA_NEAR_LOADS == 6 || MLFMA_NEAR_LOADS == 8, "near-field loads per lane: 6 or 8");
 constexpr int NT = MLFMA_NEAR_LOADS * TB;  // values per tile
 
+template <bool CONJ>
 __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
                                                const int64_t* __restrict__ perm, const cplx<double>* __restrict__ val,
                                                const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
@@ -400,7 +560,7 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     {                                                                            \
         const int q = 2 * min(t0 + K * TB, (NE) - 1);                            \
         r##K = sd[q];                                                            \
-        i##K = sd[q + 1];                                                        \
+        i##K = CONJ ? -sd[q + 1] : sd[q + 1];                                    \
     }
 #define NEAR_LD(C0, NE, CW)                                                                          \
     {                                                                                               \
@@ -517,9 +677,13 @@ __global__ __launch_bounds__(TB) void kNearSum(const int64_t* __restrict__ rows,
     }
 }
 
-void launchNearSym(const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
                    const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st) {
-    if (nitems > 0) hipLaunchKernelGGL(kNearSym, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
+    if (nitems <= 0) return;
+    if (conjVal)
+        hipLaunchKernelGGL(kNearSym<true>, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
+    else
+        hipLaunchKernelGGL(kNearSym<false>, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
 }
 
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
@@ -581,14 +745,18 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
     template bool launchM2MFused<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                     int, int, const int32_t*, const double*, const int32_t*, const double*,         \
                                     const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t);                 \
-    template void launchM2Lx<R>(M2LLevels<R>, hipStream_t);                                                         \
-    template bool launchL2LFused<R>(int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,         \
+    template void launchM2Lx<R>(M2LLevels<R>, bool, hipStream_t);                                                   \
+    template bool launchL2LFused<R>(bool, int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,   \
                                     const int32_t*, const int32_t*, const double*, int, const int32_t*,             \
                                     const int32_t*, const double*, const int32_t*, int, const cplx<R>*,             \
                                     const cplx<R>*, cplx<R>*, size_t, hipStream_t);                                 \
     template void launchL2Tx<R>(const int64_t*, const int32_t*, int64_t, const LeafItem*, const cplx<R>*,           \
                                 const cplx<R>*, cplx<double>, const int64_t*, const int32_t*, const cplx<double>*,  \
-                                const int64_t*, cplx<double>*, int, hipStream_t);
+                                const int64_t*, cplx<double>*, int, hipStream_t);                                   \
+    template bool launchL2LAdj<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
+                                  int, int, const int32_t*, const double*, const int32_t*, const double*,           \
+                                  const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t);                   \
+    template void launchZeroField<R>(cplx<R>*, int64_t, hipStream_t);
 INST(double)
 INST(float)
 #undef INST

@@ -32,11 +32,15 @@ struct M2LLevels {
     cplx<R>* L[kMaxLevels];
 };
 
+// conjT: conjugated operators (the adjoint pass; offsets then name the
+// reversed pairs)
 template <typename R>
-void launchM2Lx(M2LLevels<R> lv, hipStream_t st);
+void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st);
 
+// adjoint = true: the adjoint of M2M instead (phaseDn then holds the
+// child-grid up-phases, applied conjugated after the anterpolation)
 template <typename R>
-bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
+bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
                     int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW, int nAtPh,
                     const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
                     int nAtTh, const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax,
@@ -44,7 +48,7 @@ bool launchL2LFused(int nitems, const int32_t* parent, const int32_t* child, con
 
 // near field: kNearSym writes partial sums; kL2Tx folds them into the rows of
 // far-field leaves and kNearSum into the listed remaining rows
-void launchNearSym(const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
                    const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st);
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
                    const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st);
@@ -54,5 +58,15 @@ template <typename R>
 void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,
                 const cplx<R>* Lc, cplx<double> Ck, const int64_t* sumPtr, const int32_t* sumIdx,
                 const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st);
+
+// adjoint of L2L: parents gain conj(phase_down) * Interp(child), per parent CSR
+template <typename R>
+bool launchL2LAdj(int npar, const int32_t* parSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
+                  int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
+                  const int32_t* phNear, const double* phW, const cplx<R>* phaseDn, const cplx<R>* Lc, cplx<R>* Lp,
+                  size_t ldsMax, hipStream_t st);
+
+template <typename R>
+void launchZeroField(cplx<R>* f, int64_t n, hipStream_t st);
 
 }  // namespace mlfma

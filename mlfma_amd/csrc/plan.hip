@@ -359,6 +359,14 @@ void DevicePlan::build() {
             L.mPtr = upload(ptr);
             L.mChild = upload(ch);
             L.mOct = upload(oc);
+            {
+                // per child (parent slot, child slot, octant): the adjoint of M2M
+                std::vector<int32_t> ap;
+                for (size_t q = 0; q + 1 < ptr.size(); ++q)
+                    for (int c = ptr[q]; c < ptr[q + 1]; ++c) ap.push_back(stem[q]);
+                L.m2mAdjParent = upload(ap);
+                L.nm2mChild = (int)ap.size();
+            }
             innerLen = std::max<int64_t>(innerLen, (int64_t)L.nstem * L.nth * levels[l + 1].nph * 2);
         }
         // M2L targets with a non-empty interaction list; stale boxes otherwise
@@ -381,6 +389,16 @@ void DevicePlan::build() {
         L.lPtr = upload(ptr);
         L.lSrc = upload(sr);
         L.lOff = upload(of);
+        {
+            // the adjoint pass reads each pair reversed: operator of the negated offset
+            std::vector<int32_t> neg(of.size());
+            for (size_t q = 0; q < of.size(); ++q) {
+                int dx, dy, dz;
+                offsetVector(of[q], dx, dy, dz);
+                neg[q] = offsetIndex(-dx, -dy, -dz);
+            }
+            L.lOffNeg = upload(neg);
+        }
         L.nstale = (int)stale.size();
         L.staleSlots = upload(stale);
         // L2L into level l (from l-1)
@@ -390,6 +408,27 @@ void DevicePlan::build() {
             L.l2lParent = upload(h.l2lParent[l]);
             L.l2lOct = upload(h.l2lOct[l]);
             innerLen = std::max<int64_t>(innerLen, (int64_t)L.nl2l * levels[l - 1].nth * L.nph * 2);
+            // the same items grouped by parent (the adjoint of L2L runs per parent)
+            std::vector<int> order(L.nl2l);
+            for (int q = 0; q < L.nl2l; ++q) order[q] = q;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int a, int b) { return h.l2lParent[l][a] < h.l2lParent[l][b]; });
+            std::vector<int32_t> ps, pp{0}, pc, po;
+            for (int q = 0; q < L.nl2l; ++q) {
+                const int it = order[q];
+                if (ps.empty() || ps.back() != h.l2lParent[l][it]) {
+                    if (!ps.empty()) pp.push_back((int32_t)pc.size());
+                    ps.push_back(h.l2lParent[l][it]);
+                }
+                pc.push_back(h.l2lChild[l][it]);
+                po.push_back(h.l2lOct[l][it]);
+            }
+            if (!ps.empty()) pp.push_back((int32_t)pc.size());
+            L.nl2lPar = (int)ps.size();
+            L.l2lParSlot = upload(ps);
+            L.l2lParPtr = upload(pp);
+            L.l2lParChild = upload(pc);
+            L.l2lParOct = upload(po);
         }
     }
     inner64 = (cplx<double>*)alloc(innerLen * sizeof(cplx<double>));
@@ -514,7 +553,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
             lv.M[k] = M + L.Moff;
             lv.L[k] = Lc + L.Moff;
         }
-        launchM2Lx<R>(lv, st);
+        launchM2Lx<R>(lv, false, st);
     }
     if (flags & MLFMA_FLAG_FRESH_LOCALS)
         for (int l = 2; l <= ML; ++l) {
@@ -525,7 +564,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     for (int l = 3; l <= ML; ++l) {
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];
-        if (!launchL2LFused<R>(C.nl2l, C.l2lParent, C.l2lChild, C.l2lOct, P.nth, P.nph, C.nth, C.nph, P.atPhPtr,
+        if (!launchL2LFused<R>(false, C.nl2l, C.l2lParent, C.l2lChild, C.l2lOct, P.nth, P.nph, C.nth, C.nph, P.atPhPtr,
                                P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, P.nAtTh,
                                P.phDn<R>(), Lc + P.Moff, Lc + C.Moff, ldsMax, st)) {
             launchL2LPhi<R>(C.nl2l, C.l2lParent, C.l2lOct, P.nth, P.nph, C.nph, P.atPhPtr, P.atPhSrc, P.atPhW,
@@ -535,13 +574,85 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         }
     }
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
-    launchNearSym(nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
+    launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
     launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, Lc, Ck, nearSumPtr, nearSumIdx, nearPart, perm, y,
                   accumulate, st);
     if (timed) hipCheck(hipEventRecord(ev[7], st), "event");
+}
+
+// y (=|+=) Z^H x.  Z = near + C k R^H A S with S (S2M) the radiation
+// patterns, A = L2L . M2L . M2M and R^H (L2T) the conjugated patterns, so
+// Z^H = conj(near) + conj(C k) S^H A^H R: the patterns aggregate x into the
+// leaf locals (the S2M kernel), the adjoint of L2L carries them up, M2L with
+// conjugated operators of the reversed pairs maps locals to multipoles, the
+// adjoint of M2M carries those down, and the L2T kernel with conj(C k)
+// disaggregates them.  This is the adjoint of the linear operator: boxes
+// with an empty interaction list contribute nothing (fresh locals).
+template <typename R>
+void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st) {
+    const int ML = hp.tree.maxLevel;
+    const int p = hp.cfg.interp_order;
+    cplx<R>* M = fieldM<R>();
+    cplx<R>* Lc = fieldL<R>();
+    const cplx<R>* rp = radpat<R>();
+    launchZeroField<R>(M, fieldLen, st);
+    launchZeroField<R>(Lc, fieldLen, st);
+    launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, Lc, st);
+    for (int l = ML; l >= 3; --l) {
+        LevelBufs& P = levels[l - 1];
+        LevelBufs& C = levels[l];
+        if (!launchL2LAdj<R>(C.nl2lPar, C.l2lParSlot, C.l2lParPtr, C.l2lParChild, C.l2lParOct, P.nth, P.nph, C.nth,
+                             C.nph, p, P.thNear, P.thW, P.phNear, P.phW, P.phDn<R>(), Lc + C.Moff, Lc + P.Moff, ldsMax,
+                             st))
+            throw Error(MLFMA_ERR_ARG, "adjoint L2L: angular grid too large for one block");
+    }
+    {
+        M2LLevels<R> lv;
+        for (int l = 2; l <= ML; ++l) {
+            LevelBufs& L = levels[l];
+            if (L.ntgt == 0) continue;
+            if (lv.nlev == kMaxLevels) throw Error(MLFMA_ERR_ARG, "too many tree levels");
+            const int k = lv.nlev++;
+            lv.ndir[k] = L.ndir;
+            lv.ntgt[k] = L.ntgt;
+            lv.tgt[k] = L.tgtSlot;
+            lv.ptr[k] = L.lPtr;
+            lv.src[k] = L.lSrc;
+            lv.off[k] = L.lOffNeg;
+            lv.T[k] = L.T<R>();
+            lv.M[k] = Lc + L.Moff;  // reads the adjoint locals
+            lv.L[k] = M + L.Moff;   // writes the adjoint multipoles
+        }
+        launchM2Lx<R>(lv, true, st);
+    }
+    for (int l = 2; l < ML; ++l) {
+        LevelBufs& P = levels[l];
+        LevelBufs& C = levels[l + 1];
+        if (!launchL2LFused<R>(true, P.nm2mChild, P.m2mAdjParent, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph,
+                               P.atPhPtr, P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip,
+                               P.nAtTh, C.phUp<R>(), M + P.Moff, M + C.Moff, ldsMax, st))
+            throw Error(MLFMA_ERR_ARG, "adjoint M2M: angular grid too large for one block");
+    }
+    launchNearSym(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
+    launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st);
+    const cplx<double> Ckc{Ck.re, -Ck.im};
+    launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, M, Ckc, nearSumPtr, nearSumIdx, nearPart, perm, y,
+                  accumulate, st);
+}
+
+void DevicePlan::applyOpAdjoint(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32) {
+    if (hp.staleBoxes > 0 && !(flags & MLFMA_FLAG_FRESH_LOCALS))
+        throw Error(MLFMA_ERR_STATE,
+                    "the operator keeps stale locals (boxes with an empty interaction list) and is not linear; "
+                    "set MLFMA_FLAG_FRESH_LOCALS for an adjoint");
+    if (fp32) {
+        if (!useFp32) throw Error(MLFMA_ERR_STATE, "plan was not built for FP32");
+        applyAdjoint<float>(x, y, accumulate, st);
+    } else
+        applyAdjoint<double>(x, y, accumulate, st);
 }
 
 void DevicePlan::applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
@@ -685,6 +796,37 @@ int mlfma_matvec(mlfma_plan* plan, const double* x, double* y, int32_t nrhs, int
         d.matvec(d.xs, d.ys, nrhs, accumulate, st, fp32);
         hipCheck(hipMemcpyAsync(y, d.ys, bytes, hipMemcpyDeviceToHost, st), "y download");
         hipCheck(hipStreamSynchronize(st), "matvec");
+    } else
+        throw Error(MLFMA_ERR_ARG, "bad ptr_mode");
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_matvec_adjoint(mlfma_plan* plan, const double* x, double* y, int32_t nrhs, int32_t accumulate,
+                         int32_t ptr_mode, void* stream) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    needDevice(d);
+    if (!x || !y || nrhs < 1) throw Error(MLFMA_ERR_ARG, "bad matvec arguments");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    hipStream_t st = stream ? (hipStream_t)stream : d.stream;
+    const int64_t n = d.hp.src.n;
+    const bool fp32 = d.useFp32;
+    auto run = [&](const cplx<double>* xd, cplx<double>* yd) {
+        for (int r = 0; r < nrhs; ++r) d.applyOpAdjoint(xd + (size_t)r * n, yd + (size_t)r * n, accumulate, st, fp32);
+        hipCheck(hipGetLastError(), "adjoint matvec launch");
+    };
+    if (ptr_mode == MLFMA_PTR_DEVICE) {
+        run((const cplx<double>*)x, (cplx<double>*)y);
+        hipCheck(hipStreamSynchronize(st), "adjoint matvec");
+    } else if (ptr_mode == MLFMA_PTR_HOST) {
+        d.ensureRhs(nrhs);
+        const size_t bytes = (size_t)n * nrhs * sizeof(cplx<double>);
+        hipCheck(hipMemcpyAsync(d.xs, x, bytes, hipMemcpyHostToDevice, st), "x upload");
+        if (accumulate) hipCheck(hipMemcpyAsync(d.ys, y, bytes, hipMemcpyHostToDevice, st), "y upload");
+        run(d.xs, d.ys);
+        hipCheck(hipMemcpyAsync(y, d.ys, bytes, hipMemcpyDeviceToHost, st), "y download");
+        hipCheck(hipStreamSynchronize(st), "adjoint matvec");
     } else
         throw Error(MLFMA_ERR_ARG, "bad ptr_mode");
     return MLFMA_OK;

@@ -44,11 +44,17 @@ struct LevelBufs {
     // M2L targets of this level
     int ntgt = 0;
     int32_t *tgtSlot = nullptr, *lPtr = nullptr, *lSrc = nullptr, *lOff = nullptr;
+    int32_t* lOffNeg = nullptr;  // offsets of the reversed pairs (adjoint)
+    int32_t* m2mAdjParent = nullptr;  // per M2M child entry: its parent slot (adjoint)
+    int nm2mChild = 0;                // M2M child entries into this level
     int nstale = 0;
     int32_t* staleSlots = nullptr;
     // L2L into this level
     int nl2l = 0;
     int32_t *l2lChild = nullptr, *l2lParent = nullptr, *l2lOct = nullptr;
+    // L2L into this level grouped by parent (adjoint)
+    int nl2lPar = 0;
+    int32_t *l2lParSlot = nullptr, *l2lParPtr = nullptr, *l2lParChild = nullptr, *l2lParOct = nullptr;
 
     template <typename R>
     const cplx<R>* T() const;
@@ -119,6 +125,10 @@ struct DevicePlan {
     // y (=|+=) Z x on device vectors in input order
     template <typename R>
     void apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed);
+    // y (=|+=) Z^H x (the adjoint of the linear operator)
+    template <typename R>
+    void applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st);
+    void applyOpAdjoint(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32);
     void applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed);
     void matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int accumulate, hipStream_t st, bool fp32);
 

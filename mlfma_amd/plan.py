@@ -78,6 +78,13 @@ class Plan:
     def matvec(self, x, y=None, accumulate=False, stream=None):
         """y = Z x (or y += Z x).  numpy arrays (n,) or (nrhs, n) go through host
         buffers; torch CUDA tensors (complex128, contiguous) are used in place."""
+        return self._apply(self.lib.mlfma_matvec, x, y, accumulate, stream)
+
+    def matvec_adjoint(self, x, y=None, accumulate=False, stream=None):
+        """y = Z^H x (or y += Z^H x), same conventions as matvec."""
+        return self._apply(self.lib.mlfma_matvec_adjoint, x, y, accumulate, stream)
+
+    def _apply(self, fn, x, y, accumulate, stream):
         try:
             import torch
             is_t = isinstance(x, torch.Tensor)
@@ -90,9 +97,8 @@ class Plan:
             if y is None:
                 y = torch.empty_like(x)
             st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
-            _check(self.lib, self.lib.mlfma_matvec(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
-                                                   nrhs, 1 if accumulate else 0, A.MLFMA_PTR_DEVICE,
-                                                   C.c_void_p(st)))
+            _check(self.lib, fn(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), nrhs,
+                                1 if accumulate else 0, A.MLFMA_PTR_DEVICE, C.c_void_p(st)))
             return y
         x = _cvec(x)
         nrhs = 1 if x.ndim == 1 else x.shape[0]
@@ -102,8 +108,8 @@ class Plan:
             y = np.zeros_like(x)
         elif accumulate:
             y = _cvec(y)
-        _check(self.lib, self.lib.mlfma_matvec(self._h, C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data), nrhs,
-                                               1 if accumulate else 0, A.MLFMA_PTR_HOST, None))
+        _check(self.lib, fn(self._h, C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data), nrhs,
+                            1 if accumulate else 0, A.MLFMA_PTR_HOST, None))
         return y
 
     def matvec_direct(self, x):

@@ -166,3 +166,48 @@ def test_stage_times_and_runtime_flags():
     assert abs(sum(d.values()) - st["total"]) <= 1e-3
     with pytest.raises(Exception):
         p.set_flags(1 << 7)                # unknown flag bit
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_adjoint_matvec(name):
+    """<Z x, w> = <x, Z^H w> to round-off: the adjoint is the exact transpose
+    of the implemented operator (stage by stage), in FP64 and FP32."""
+    geom, kw = cases.make(name)
+    pw = cases.planewave_z()
+    p = Plan(geom, pw, A.make_config(device=0, flags=A.MLFMA_FLAG_FRESH_LOCALS, **kw))
+    rng = np.random.default_rng(3)
+    for trial in range(2):
+        x = G.complex_gaussian(geom.n, 20 + trial)
+        w = G.complex_gaussian(geom.n, 30 + trial)
+        zx = p.matvec(x)
+        zhw = p.matvec_adjoint(w)
+        lhs, rhs = np.vdot(w, zx), np.vdot(zhw, x)
+        scale = np.linalg.norm(zx) * np.linalg.norm(w)
+        say(name, "adjoint identity", abs(lhs - rhs) / scale)
+        assert abs(lhs - rhs) <= 1e-13 * scale
+    # linear, and the near-field part alone is conj(near)^T = conj(near)
+    x2 = G.complex_gaussian(geom.n, 40)
+    a = p.matvec_adjoint(x + 2j * x2)
+    b = p.matvec_adjoint(x) + 2j * p.matvec_adjoint(x2)
+    assert cases.relerr(a, b) <= 1e-13
+    # against the dense operator of the same kernel, which is symmetric
+    # (Galerkin with a symmetric dyadic Green's function), so that
+    # Z_dense^H w = conj(Z_dense conj(w)): within the MLFMA error
+    o = O.Oracle(geom, pw, A.make_config(device=-1, **kw))
+    e = cases.relerr(p.matvec_adjoint(w), np.conj(o.dense_matvec(np.conj(w))))
+    ef = cases.relerr(p.matvec(w), o.dense_matvec(w))
+    say(name, "adjoint vs dense", e, "forward vs dense", ef)
+    assert e <= max(3 * ef, 1e-9)
+    if rng is not None:
+        p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, flags=A.MLFMA_FLAG_FRESH_LOCALS, **kw))
+        zx = p32.matvec(x)
+        zhw = p32.matvec_adjoint(w)
+        assert abs(np.vdot(w, zx) - np.vdot(zhw, x)) <= 1e-5 * np.linalg.norm(zx) * np.linalg.norm(w)
+
+
+def test_adjoint_rejects_stale_operator():
+    geom, kw = cases.make("dipole_adaptive")
+    p = Plan(geom, cases.planewave_z(), A.make_config(device=0, **kw))
+    assert p.info()["stale_boxes"] > 0
+    with pytest.raises(Exception):
+        p.matvec_adjoint(G.complex_gaussian(geom.n, 1))
