@@ -168,6 +168,9 @@ int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t pt
                                   conjugates, stops on |g_k+1|/g0                  */
 #define MLFMA_GMRES_RESTART 1  /* GMRES(m), unitary Givens, true residual at restart */
 #define MLFMA_FGMRES_IR 2      /* FP64 flexible GMRES outer, FP32 far-field inner   */
+#define MLFMA_SOLVE_ADJOINT 0x100  /* or-ed into method: solve Z^H x = b instead
+                                      (the adjoint solve of a gradient; no
+                                      reference counterpart)                       */
 
 typedef struct mlfma_solve_opts {
     int32_t method;       /* MLFMA_GMRES_*                                          */
@@ -200,6 +203,11 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
  * direction list).  I: n complex (host); dirs: ndir x 3 (host);
  * E: ndir x 3 complex (host). */
 int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E);
+
+/* The adjoint of mlfma_farfield: I = F^H E for ndir x 3 complex fields E
+ * (host) at the unit directions dirs (host); I: n complex (host, input
+ * order).  No reference counterpart: the gradient of a far-field objective. */
+int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs, int64_t ndir, double* I);
 
 /* --------------------------------------------------------------- introspection */
 

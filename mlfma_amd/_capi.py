@@ -24,6 +24,7 @@ MLFMA_PTR_DEVICE = 1
 MLFMA_GMRES_COMPAT = 0
 MLFMA_GMRES_RESTART = 1
 MLFMA_FGMRES_IR = 2
+MLFMA_SOLVE_ADJOINT = 0x100
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_OOM", -3: "ERR_HIP", -4: "ERR_BREAKDOWN",
           -5: "ERR_NONFINITE", -6: "ERR_STATE", -7: "ERR_GEOMETRY", -8: "ERR_NOT_CONVERGED"}
@@ -86,6 +87,7 @@ SIGNATURES = {
     "mlfma_solve": (C.c_int, [C.c_void_p, dptr, dptr, C.POINTER(SolveOpts), C.POINTER(SolveReport),
                               dptr]),
     "mlfma_farfield": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
+    "mlfma_farfield_adjoint": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
     "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),
     "mlfma_plan_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mlfma_stage_times_detail": (C.c_int, [C.c_void_p, dptr, C.c_int32, C.POINTER(C.c_int32)]),

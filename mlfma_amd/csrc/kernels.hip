@@ -348,6 +348,56 @@ void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const dou
     hipLaunchKernelGGL(kFarfield, dim3(ndir), dim3(BS), 0, st, s, n, I, dirs, k, Ck, E);
 }
 
+// Adjoint of kFarfield: I_n = conj(C k) sum_d f_n(k rhat_d)^H (I - rr) E_d,
+// one thread per source (tree order); directions staged in LDS a block at a
+// time with their projected, conjugate-scaled fields.
+__global__ void kFarfieldAdj(SrcDev s, int64_t n, const cplx<double>* __restrict__ E, const double* __restrict__ dirs,
+                             int ndir, double k, cplx<double> Ckc, const int64_t* __restrict__ perm,
+                             cplx<double>* __restrict__ I) {
+    __shared__ double sr[BS][3];
+    __shared__ cplx<double> se[BS][3];
+    const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+    cplx<double> acc = mk(0.0, 0.0);
+    for (int d0 = 0; d0 < ndir; d0 += BS) {
+        __syncthreads();
+        const int d = d0 + threadIdx.x;
+        if (d < ndir) {
+            const double rh[3] = {dirs[3 * d], dirs[3 * d + 1], dirs[3 * d + 2]};
+            for (int i = 0; i < 3; ++i) {
+                cplx<double> v = mk(0.0, 0.0);
+                for (int j = 0; j < 3; ++j) v = v + ((i == j ? 1.0 : 0.0) - rh[i] * rh[j]) * E[3 * d + j];
+                se[threadIdx.x][i] = Ckc * v;
+                sr[threadIdx.x][i] = rh[i];
+            }
+        }
+        __syncthreads();
+        if (t < n) {
+            const int nd = min(BS, ndir - d0);
+            for (int q = 0; q < nd; ++q) {
+                const double kv[3] = {k * sr[q][0], k * sr[q][1], k * sr[q][2]};
+                cplx<double> f[3];
+                if (s.kind == 0) {
+                    const DipoleDev& dp = s.dip[t];
+                    const cplx<double> e = cexpi(-(dp.pos[0] * kv[0] + dp.pos[1] * kv[1] + dp.pos[2] * kv[2]));
+                    for (int i = 0; i < 3; ++i) f[i] = dp.phat[i] * e;
+                } else {
+                    rwgPlaneWave(s.rwg[t], s.tri, kv, f);
+                    for (int i = 0; i < 3; ++i) f[i] = mk(f[i].re, -f[i].im);
+                }
+                for (int i = 0; i < 3; ++i) acc = acc + mk(f[i].re, -f[i].im) * se[q][i];
+            }
+        }
+    }
+    if (t < n) I[perm[t]] = acc;
+}
+
+void launchFarfieldAdj(const SrcDev& s, int64_t n, const cplx<double>* E, const double* dirs, int ndir, double k,
+                       cplx<double> Ckc, const int64_t* perm, cplx<double>* I, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kFarfieldAdj, dim3((unsigned)((n + BS - 1) / BS)), dim3(BS), 0, st, s, n, E, dirs, ndir, k, Ckc,
+                       perm, I);
+}
+
 // ------------------------------------------------------------------ matvec stages
 
 __global__ void kGather(const cplx<double>* __restrict__ x, const int64_t* __restrict__ perm,

@@ -89,6 +89,8 @@ void launchVoltage(const SrcDev& s, int64_t n, const double* wavevec, const doub
                    cplx<double>* V, hipStream_t st);
 void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const double* dirs, int ndir, double k,
                     cplx<double> Ck, cplx<double>* E, hipStream_t st);
+void launchFarfieldAdj(const SrcDev& s, int64_t n, const cplx<double>* E, const double* dirs, int ndir, double k,
+                       cplx<double> Ckc, const int64_t* perm, cplx<double>* I, hipStream_t st);
 
 // matvec stages, templated on the real type of the far field
 void launchGather(const cplx<double>* x, const int64_t* perm, cplx<double>* xt, int64_t n, int nrhs,

@@ -130,6 +130,14 @@ struct DevicePlan {
     void applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st);
     void applyOpAdjoint(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32);
     void applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed);
+    // the operator of the running solve: Z, or Z^H for an adjoint solve
+    bool solveAdjoint = false;
+    void applySolveOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
+        if (solveAdjoint)
+            applyOpAdjoint(x, y, accumulate, st, fp32);
+        else
+            applyOp(x, y, accumulate, st, fp32, timed);
+    }
     void matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int accumulate, hipStream_t st, bool fp32);
 
     template <typename R>

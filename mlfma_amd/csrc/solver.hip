@@ -232,7 +232,7 @@ static SolveOut gmresCompat(DevicePlan& d, Krylov& K, const cplx<double>* b, cpl
     int it = 0;
     for (;;) {
         const int j = it;
-        d.applyOp(Q + (size_t)j * n, w, 0, K.st, false, false);
+        d.applySolveOp(Q + (size_t)j * n, w, 0, K.st, false, false);
         std::vector<zc> h;
         double hn;
         K.orthogonalize(Q, j + 1, w, h, hn);
@@ -299,7 +299,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
             hipCheck(hipMemcpyAsync(w, b, n * sizeof(cplx<double>), hipMemcpyDeviceToDevice, K.st), "copy");
             beta = bn;
         } else {
-            d.applyOp(x, w, 0, K.st, fp32, false);
+            d.applySolveOp(x, w, 0, K.st, fp32, false);
             K.sub(b, w, w);
             beta = K.norm(w);
         }
@@ -312,7 +312,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         g[0] = beta;
         int j = 0;
         for (; j < m && total < maxit; ++j) {
-            d.applyOp(Q + (size_t)j * n, w, 0, K.st, fp32, false);
+            d.applySolveOp(Q + (size_t)j * n, w, 0, K.st, fp32, false);
             std::vector<zc> h;
             double hn;
             K.orthogonalize(Q, j + 1, w, h, hn);
@@ -375,7 +375,7 @@ static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<do
             hipCheck(hipMemcpyAsync(r, b, n * sizeof(cplx<double>), hipMemcpyDeviceToDevice, K.st), "copy");
             res = 1.0;
         } else {
-            d.applyOp(x, ax, 0, K.st, false, false);
+            d.applySolveOp(x, ax, 0, K.st, false, false);
             K.sub(b, ax, r);
             res = K.norm(r) / bn;
         }
@@ -435,7 +435,14 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     if (!x || !opts) throw Error(MLFMA_ERR_ARG, "null argument");
     if (!opts->use_voltage && !b) throw Error(MLFMA_ERR_ARG, "b is NULL and use_voltage = 0");
     if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
-    if (opts->method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
+    const int method = opts->method & 0xff;
+    struct AdjointMode {  // cleared however the solve ends
+        DevicePlan& d;
+        ~AdjointMode() { d.solveAdjoint = false; }
+    } adjointMode{d};
+    d.solveAdjoint = (opts->method & MLFMA_SOLVE_ADJOINT) != 0;
+    if (opts->method & ~(0xff | MLFMA_SOLVE_ADJOINT)) throw Error(MLFMA_ERR_ARG, "unknown solve method bits");
+    if (method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
     hipCheck(hipSetDevice(d.device), "hipSetDevice");
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t n = d.hp.src.n;
@@ -460,18 +467,18 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     hipCheck(hipMemcpyAsync(bt, bh.data(), n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "b upload");
     Krylov K(d, st, std::max(opts->maxit, opts->restart) + 2);
     SolveOut o;
-    if (opts->method == MLFMA_GMRES_COMPAT)
+    if (method == MLFMA_GMRES_COMPAT)
         o = gmresCompat(d, K, bt, xtree, opts->maxit, opts->tol, hist);
-    else if (opts->method == MLFMA_GMRES_RESTART)
+    else if (method == MLFMA_GMRES_RESTART)
         o = gmresRestart(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, false, true,
                          hist, 0);
-    else if (opts->method == MLFMA_FGMRES_IR)
+    else if (method == MLFMA_FGMRES_IR)
         o = gmresIR(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, hist);
     else
         throw Error(MLFMA_ERR_ARG, "unknown solve method");
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // true residual with the FP64 matvec
-    d.applyOp(xtree, d.yt, 0, st, false, false);
+    d.applySolveOp(xtree, d.yt, 0, st, false, false);
     K.sub(bt, d.yt, d.yt);
     const double rn = K.norm(d.yt), bn = K.norm(bt);
     hipCheck(hipMemcpyAsync(x, xtree, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "x download");
@@ -507,6 +514,33 @@ int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_
     hipCheck(hipGetLastError(), "farfield launch");
     hipCheck(hipMemcpyAsync(E, dE, (size_t)ndir * 3 * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "E download");
     hipCheck(hipStreamSynchronize(st), "farfield");
+    d.release(dd);
+    d.release(dE);
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs, int64_t ndir, double* I) {
+    API_BEGIN
+    DevicePlan& d = Pl(plan);
+    if (!I || !dirs || !E || ndir < 0) throw Error(MLFMA_ERR_ARG, "bad farfield arguments");
+    if (ndir > INT32_MAX) throw Error(MLFMA_ERR_ARG, "too many directions");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const int64_t n = d.hp.src.n;
+    hipStream_t st = d.stream;
+    d.ensureRhs(1);
+    const int64_t nd = std::max<int64_t>(ndir, 1);
+    double* dd = (double*)d.alloc((size_t)nd * 3 * sizeof(double));
+    cplx<double>* dE = (cplx<double>*)d.alloc((size_t)nd * 3 * sizeof(cplx<double>));
+    if (ndir > 0) {
+        hipCheck(hipMemcpyAsync(dd, dirs, (size_t)ndir * 3 * sizeof(double), hipMemcpyHostToDevice, st), "dirs upload");
+        hipCheck(hipMemcpyAsync(dE, E, (size_t)ndir * 3 * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "E upload");
+    }
+    const cplx<double> Ckc{d.Ck.re, -d.Ck.im};
+    launchFarfieldAdj(d.src, n, dE, dd, (int)ndir, d.hp.k, Ckc, d.perm, d.xs, st);
+    hipCheck(hipGetLastError(), "farfield adjoint launch");
+    hipCheck(hipMemcpyAsync(I, d.xs, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "I download");
+    hipCheck(hipStreamSynchronize(st), "farfield adjoint");
     d.release(dd);
     d.release(dE);
     return MLFMA_OK;

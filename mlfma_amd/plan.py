@@ -136,9 +136,12 @@ class Plan:
         return dict(zip(["S2M", "M2M", "M2L", "L2L", "near_pairs", "near_sum", "L2T"], t.tolist()))
 
     # ---------------------------------------------------------------- solve
-    def solve(self, b=None, method=A.MLFMA_GMRES_COMPAT, restart=50, maxit=500, tol=1e-6, history=False):
+    def solve(self, b=None, method=A.MLFMA_GMRES_COMPAT, restart=50, maxit=500, tol=1e-6, history=False,
+              adjoint=False):
+        """GMRES for Z x = b (b = None: the plane-wave excitation -V), or for
+        Z^H x = b with adjoint=True.  Returns (x, report)."""
         o = A.SolveOpts()
-        o.method = method
+        o.method = method | (A.MLFMA_SOLVE_ADJOINT if adjoint else 0)
         o.restart = restart
         o.maxit = maxit
         o.use_voltage = 1 if b is None else 0
@@ -166,6 +169,16 @@ class Plan:
         _check(self.lib, self.lib.mlfma_farfield(self._h, I.ctypes.data_as(A.dptr), dirs.ctypes.data_as(A.dptr),
                                                  len(dirs), E.ctypes.data_as(A.dptr)))
         return E
+
+    def farfield_adjoint(self, E, dirs):
+        """I = F^H E: the adjoint of farfield for the same directions."""
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        E = np.ascontiguousarray(E, dtype=np.complex128).reshape(len(dirs), 3)
+        I = np.zeros(self.n, dtype=np.complex128)
+        _check(self.lib, self.lib.mlfma_farfield_adjoint(self._h, E.ctypes.data_as(A.dptr),
+                                                         dirs.ctypes.data_as(A.dptr), len(dirs),
+                                                         I.ctypes.data_as(A.dptr)))
+        return I
 
     # ---------------------------------------------------------------- introspection
     def level_boxes(self, level):
