@@ -5,13 +5,16 @@ geodesic icosphere of frequency 32 (20,480 triangles, 30,720 RWG unknowns),
 7-point triangle quadrature, leaf boxes of 0.25 wavelength (root 4, levels
 0-4), interpolation order 6, 3 digits, plane wave theta = 0 x-polarised.
 
-A step is one full matvec y = Z x (gather, S2M, M2M, M2L, L2L, L2T, near field,
-scatter) with x and y resident in HBM.  The driver contract: W untimed warmup
-steps, then K timed steps bracketed by torch.cuda.synchronize(); one JSON line.
+A step is one full matvec y = Z x (S2M, M2M, M2L, L2L, near field, L2T) with
+x and y resident in HBM.  W untimed warmup steps, then K timed steps bracketed
+by torch.cuda.synchronize() give `value`; one JSON line.
 
 The line also carries:
-  roofline      the dominant kernel's achieved algorithmic GB/s (HIP events on
-                the library's stream over the timed steps) against 8 TB/s;
+  stage_ms      per-stage device times from HIP events on the library's stream,
+                recorded over a second pass of the same K steps;
+  roofline      the near-field pair kernel's achieved algorithmic GB/s (its
+                launch time from the events of that second pass) against 8 TB/s;
+  batched       8 right-hand sides per call (each operator read shared by 8);
   cpu_baseline  the CPU oracle (FP64 restatement of the reference, one core)
                 timed on a bounded number of matvecs of the same workload;
   solve         GMRES(50) to 1e-6 in FP64 and the FP32-far-field refinement
@@ -161,11 +164,20 @@ def main():
     near_pairs_ms = 0.0
     for _ in range(args.warmup):
         plan.matvec(x, y)
-    # timed region: K matvecs with the per-stage HIP events recorded on the
-    # library's stream (their device cost is included in the throughput)
-    plan.set_flags(A.MLFMA_FLAG_STAGE_TIMES)
+    # timed region: K matvecs, uninstrumented
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.matvec(x, y)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = args.steps / elapsed
+    # the same K matvecs again with the per-stage HIP events recorded on the
+    # library's stream: stage times and the roofline kernel's launch time
+    plan.set_flags(A.MLFMA_FLAG_STAGE_TIMES)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     for _ in range(args.steps):
         plan.matvec(x, y)
         st = plan.stage_times()
@@ -173,20 +185,26 @@ def main():
             stages[k] += st[k]
         near_pairs_ms += plan.stage_times_detail()["near_pairs"]
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = args.steps / elapsed
+    with_events = args.steps / (time.perf_counter() - t1)
+    plan.set_flags(0)
     stage_ms = {k: round(v / args.steps, 4) for k, v in stages.items()}
     log("stage ms", stage_ms)
-    # the same K matvecs without events, reported beside the timed value
-    plan.set_flags(0)
+    log("matvec/s", round(value, 2), "with stage events", round(with_events, 2))
+    # batched right-hand sides: 8 vectors per call share every operator read
+    nb = 8
+    Xb = torch.from_numpy(np.stack([G.complex_gaussian(n, 50 + r) for r in range(nb)])).to(dev)
+    Yb = torch.empty_like(Xb)
+    for _ in range(2):
+        plan.matvec(Xb, Yb)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.matvec(x, y)
+    calls = max(1, args.steps // 2)
+    t2 = time.perf_counter()
+    for _ in range(calls):
+        plan.matvec(Xb, Yb)
     torch.cuda.synchronize()
-    no_events = args.steps / (time.perf_counter() - t1)
-    log("matvec/s with stage events", round(value, 2), "without", round(no_events, 2))
+    el_b = time.perf_counter() - t2
+    batched = {"nrhs": nb, "matvec_per_s": round(nb * calls / el_b, 3), "ms_per_call": round(1000 * el_b / calls, 4)}
+    log("batched", batched)
 
     # roofline of the near-field pair kernel (one launch per matvec; HIP events
     # on the library's stream right before and after that launch)
@@ -203,7 +221,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": desc, "roofline": roofline, "stage_ms": stage_ms, "setup_s": round(setup_s, 2),
-        "value_without_stage_events": round(no_events, 3),
+        "value_with_stage_events": round(with_events, 3), "batched": batched,
     }
 
     if not args.no_solve:

@@ -569,8 +569,8 @@ static void buildNearItems(HostPlan& hp) {
             for (int64_t q = t.nodes[cn].sbeg; q < t.nodes[cn].send; ++q) hp.nearCols.push_back(q);
         const int64_t ncol = (int64_t)hp.nearCols.size() - base;
         const int64_t nself = n.count();
-        for (int64_t r0 = 0; r0 < n.count(); r0 += 256) {
-            const int64_t nr = std::min<int64_t>(256, n.count() - r0);
+        for (int64_t r0 = 0; r0 < n.count(); r0 += 128) {
+            const int64_t nr = std::min<int64_t>(128, n.count() - r0);
             const int64_t cw = std::max<int64_t>(1, cap / nr);
             for (int64_t c0 = 0; c0 < ncol; c0 += cw) {
                 const int64_t c1 = std::min(ncol, c0 + cw);

@@ -160,7 +160,7 @@ struct HostPlan {
     // (src/fmm/leaf.cpp:329-383).  Every non-empty leaf heads one row block:
     // its self block (full square, zero diagonal, applied to the rows only)
     // followed by the columns of the pairs it heads.  Row blocks are cut into
-    // work items of at most 256 rows and a bounded number of entries; values
+    // work items of at most 128 rows and a bounded number of entries; values
     // are column-major per item.  Each item writes partial sums (nr for its
     // rows, nc - nself for its pair columns); nearSumPtr/nearSumIdx list the
     // partials of every tree position in a fixed order.

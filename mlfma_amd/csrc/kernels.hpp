@@ -33,7 +33,7 @@ struct LeafItem {
 // ---- near-field work item (one per row block piece; see HostPlan::NearItem)
 struct NearItem {
     int64_t rbeg;   // first row (tree position)
-    int32_t nr;     // rows, <= 256
+    int32_t nr;     // rows, <= 128
     int32_t nc;     // columns
     int32_t nself;  // leading columns of the self block (applied to the rows only)
     int32_t pad;

@@ -4,6 +4,7 @@
 // field and L2T writes), so a matvec needs no separate gather/scatter pass.
 #include "cplx_ops.hpp"
 #include "kernels_mv.hpp"
+#include "host.hpp"  // Error
 
 namespace mlfma {
 
@@ -23,10 +24,13 @@ constexpr int TB = 256;
 // ------------------------------------------------------------------ S2M
 // M_b[d] = sum_s x_s rp_b[s][d]  (src/fmm/leaf.cpp:212-230).  Block = one leaf
 // x blockDim directions; lanes stream consecutive 32-byte (theta, phi) pairs
-// of one source row, two source rows in flight per iteration.
-template <typename R>
+// of one source row.  NB right-hand sides share every pattern read: x has
+// column stride xs (input order), the fields of right-hand side r start at
+// M + r fs.  One right-hand side keeps two source rows in flight instead.
+template <typename R, int NB>
 __global__ void kS2Mx(const LeafItem* __restrict__ items, int nitems, int per, const cplx<R>* __restrict__ rp,
-                      const cplx<double>* __restrict__ x, const int64_t* __restrict__ perm, cplx<R>* __restrict__ M) {
+                      const cplx<double>* __restrict__ x, int64_t xs, const int64_t* __restrict__ perm,
+                      cplx<R>* __restrict__ M, int64_t fs) {
     const int item = xcdItem(blockIdx.x, per);
     if (item >= nitems) return;
     const LeafItem it = items[item];
@@ -34,35 +38,62 @@ __global__ void kS2Mx(const LeafItem* __restrict__ items, int nitems, int per, c
     if (d >= it.ndir) return;
     const cplx<R>* p = rp + it.rpOff + 2 * d;
     const int64_t stride = (int64_t)it.ndir * 2;
-    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
-    int q = 0;
-    for (; q + 1 < it.cnt; q += 2) {
-        const cplx<R> x0 = cast<R>(x[perm[it.sbeg + q]]);
-        const cplx<R> x1 = cast<R>(x[perm[it.sbeg + q + 1]]);
-        const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
-        const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
-        a0 = a0 + x0 * r00;
-        a1 = a1 + x0 * r01;
-        b0 = b0 + x1 * r10;
-        b1 = b1 + x1 * r11;
+    if constexpr (NB == 1) {
+        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
+        int q = 0;
+        for (; q + 1 < it.cnt; q += 2) {
+            const cplx<R> x0 = cast<R>(x[perm[it.sbeg + q]]);
+            const cplx<R> x1 = cast<R>(x[perm[it.sbeg + q + 1]]);
+            const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
+            const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
+            a0 = a0 + x0 * r00;
+            a1 = a1 + x0 * r01;
+            b0 = b0 + x1 * r10;
+            b1 = b1 + x1 * r11;
+        }
+        if (q < it.cnt) {
+            const cplx<R> x0 = cast<R>(x[perm[it.sbeg + q]]);
+            a0 = a0 + x0 * p[q * stride];
+            a1 = a1 + x0 * p[q * stride + 1];
+        }
+        M[it.fOff + 2 * d] = a0 + b0;
+        M[it.fOff + 2 * d + 1] = a1 + b1;
+    } else {
+        cplx<R> a[NB][2];
+#pragma unroll
+        for (int r = 0; r < NB; ++r) a[r][0] = a[r][1] = mk<R>(0, 0);
+        for (int q = 0; q < it.cnt; ++q) {
+            const int64_t src = perm[it.sbeg + q];
+            const cplx<R> r0 = p[q * stride], r1 = p[q * stride + 1];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                const cplx<R> xr = cast<R>(x[r * xs + src]);
+                a[r][0] = a[r][0] + xr * r0;
+                a[r][1] = a[r][1] + xr * r1;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            M[r * fs + it.fOff + 2 * d] = a[r][0];
+            M[r * fs + it.fOff + 2 * d + 1] = a[r][1];
+        }
     }
-    if (q < it.cnt) {
-        const cplx<R> x0 = cast<R>(x[perm[it.sbeg + q]]);
-        a0 = a0 + x0 * p[q * stride];
-        a1 = a1 + x0 * p[q * stride + 1];
-    }
-    M[it.fOff + 2 * d] = a0 + b0;
-    M[it.fOff + 2 * d + 1] = a1 + b1;
 }
 
 template <typename R>
 void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* x,
-                const int64_t* perm, cplx<R>* M, hipStream_t st) {
+                const int64_t* perm, cplx<R>* M, hipStream_t st, int nb, int64_t xs, int64_t fs) {
     if (nitems <= 0) return;
     const int bs = std::min(TB, ((maxndir + 63) / 64) * 64);
     const int per = (nitems + 7) / 8;
     dim3 g(8 * per, (maxndir + bs - 1) / bs);
-    hipLaunchKernelGGL(kS2Mx<R>, g, dim3(bs), 0, st, items, nitems, per, rp, x, perm, M);
+#define S2M_CASE(B) hipLaunchKernelGGL((kS2Mx<R, B>), g, dim3(bs), 0, st, items, nitems, per, rp, x, xs, perm, M, fs)
+    if (nb == 1) S2M_CASE(1);
+    else if (nb == 2) S2M_CASE(2);
+    else if (nb == 4) S2M_CASE(4);
+    else if (nb == 8) S2M_CASE(8);
+    else throw Error(MLFMA_ERR_ARG, "right-hand-side batch must be 1, 2, 4 or 8");
+#undef S2M_CASE
 }
 
 // ------------------------------------------------------------------ fused M2M
@@ -79,10 +110,12 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
                                                 int mphC, int p, const int32_t* __restrict__ thNear,
                                                 const double* __restrict__ thW, const int32_t* __restrict__ phNear,
                                                 const double* __restrict__ phW, const cplx<R>* __restrict__ phaseUp,
-                                                const cplx<R>* __restrict__ Mc, cplx<R>* __restrict__ Mp) {
+                                                const cplx<R>* __restrict__ Mc, cplx<R>* __restrict__ Mp, int64_t fs) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int item = xcdItem(blockIdx.x, per);
     if (item >= nstem) return;
+    Mc += blockIdx.z * fs;  // right-hand side blockIdx.z
+    Mp += blockIdx.z * fs;
     const int j0 = blockIdx.y * rows;  // parent theta rows [j0, j0 + nr)
     const int nr = min(rows, nthP - j0);
     const int ndirC = mthC * mphC;
@@ -166,7 +199,7 @@ template <typename R>
 bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
                     int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
                     const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
-                    size_t ldsMax, hipStream_t st) {
+                    size_t ldsMax, hipStream_t st, int nb, int64_t fs) {
     if (nstem <= 0) return true;
     // split the parent's theta rows so that there are enough blocks for the
     // device (each block re-reads the children's fields from L2)
@@ -189,10 +222,10 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
     const int ept = (nInner + TB - 1) / TB;
     if (ept > 16) return false;
     const int per = (nstem + 7) / 8;
-    dim3 g(8 * per, (nthP + rows - 1) / rows);
+    dim3 g(8 * per, (nthP + rows - 1) / rows, nb);
 #define M2M_CASE(E)                                                                                                \
     hipLaunchKernelGGL((kM2MFused<R, E>), g, dim3(TB), lds, st, stemSlot, ptr, child, oct, nstem, per, rows, cb,  \
-                       nthP, nphP, mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp)
+                       nthP, nphP, mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp, fs)
     if (ept <= 1) M2M_CASE(1);
     else if (ept <= 2) M2M_CASE(2);
     else if (ept <= 4) M2M_CASE(4);
@@ -206,10 +239,11 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
 // L_t[d] = sum over the interaction list of T_off[d] M_s[d]  (src/fmm/node.cpp:105-134),
 // quadrature weights folded into T.  One launch covers every level: a block
 // is (level, target box, chunk of 256 directions); the (source, offset) list
-// is read with uniform loads, two pairs in flight; within a level the targets
-// of one XCD are spatially contiguous so their shared sources stay in that
-// XCD's L2.
-template <typename R, bool CONJ>
+// is read with uniform loads; within a level the targets of one XCD are
+// spatially contiguous so their shared sources stay in that XCD's L2.  NB
+// right-hand sides (fields at stride lv.fs) share every operator read; one
+// right-hand side keeps two pairs in flight instead.
+template <typename R, bool CONJ, int NB>
 __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
     int b = blockIdx.x, l = 0;
     while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
@@ -226,36 +260,57 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
     const cplx<R>* T = lv.T[l];
     const cplx<R>* M = lv.M[l];
     const int q0 = lv.ptr[l][item], q1 = lv.ptr[l][item + 1];
-    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
-    int q = q0;
-    for (; q + 1 < q1; q += 2) {
-        cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
-        cplx<R> t1 = T[(int64_t)off[q + 1] * ndir + d];
-        if (CONJ) {
-            t0.im = -t0.im;
-            t1.im = -t1.im;
-        }
-        const cplx<R>* m0 = M + ((int64_t)src[q] * ndir + d) * 2;
-        const cplx<R>* m1 = M + ((int64_t)src[q + 1] * ndir + d) * 2;
-        a0 = a0 + t0 * m0[0];
-        a1 = a1 + t0 * m0[1];
-        b0 = b0 + t1 * m1[0];
-        b1 = b1 + t1 * m1[1];
-    }
-    if (q < q1) {
-        cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
-        if (CONJ) t0.im = -t0.im;
-        const cplx<R>* m0 = M + ((int64_t)src[q] * ndir + d) * 2;
-        a0 = a0 + t0 * m0[0];
-        a1 = a1 + t0 * m0[1];
-    }
     cplx<R>* out = lv.L[l] + ((int64_t)lv.tgt[l][item] * ndir + d) * 2;
-    out[0] = a0 + b0;
-    out[1] = a1 + b1;
+    if constexpr (NB == 1) {
+        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
+        int q = q0;
+        for (; q + 1 < q1; q += 2) {
+            cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
+            cplx<R> t1 = T[(int64_t)off[q + 1] * ndir + d];
+            if (CONJ) {
+                t0.im = -t0.im;
+                t1.im = -t1.im;
+            }
+            const cplx<R>* m0 = M + ((int64_t)src[q] * ndir + d) * 2;
+            const cplx<R>* m1 = M + ((int64_t)src[q + 1] * ndir + d) * 2;
+            a0 = a0 + t0 * m0[0];
+            a1 = a1 + t0 * m0[1];
+            b0 = b0 + t1 * m1[0];
+            b1 = b1 + t1 * m1[1];
+        }
+        if (q < q1) {
+            cplx<R> t0 = T[(int64_t)off[q] * ndir + d];
+            if (CONJ) t0.im = -t0.im;
+            const cplx<R>* m0 = M + ((int64_t)src[q] * ndir + d) * 2;
+            a0 = a0 + t0 * m0[0];
+            a1 = a1 + t0 * m0[1];
+        }
+        out[0] = a0 + b0;
+        out[1] = a1 + b1;
+    } else {
+        cplx<R> a[NB][2];
+#pragma unroll
+        for (int r = 0; r < NB; ++r) a[r][0] = a[r][1] = mk<R>(0, 0);
+        for (int q = q0; q < q1; ++q) {
+            cplx<R> t = T[(int64_t)off[q] * ndir + d];
+            if (CONJ) t.im = -t.im;
+            const cplx<R>* m = M + ((int64_t)src[q] * ndir + d) * 2;
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                a[r][0] = a[r][0] + t * m[r * lv.fs];
+                a[r][1] = a[r][1] + t * m[r * lv.fs + 1];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            out[r * lv.fs] = a[r][0];
+            out[r * lv.fs + 1] = a[r][1];
+        }
+    }
 }
 
 template <typename R>
-void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st) {
+void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb) {
     int total = 0;
     for (int l = 0; l < lv.nlev; ++l) {
         lv.nch[l] = (lv.ndir[l] + TB - 1) / TB;
@@ -264,10 +319,13 @@ void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st) {
         total += 8 * lv.per[l] * lv.nch[l];
     }
     if (total <= 0) return;
-    if (conjT)
-        hipLaunchKernelGGL((kM2Lx<R, true>), dim3(total), dim3(TB), 0, st, lv);
-    else
-        hipLaunchKernelGGL((kM2Lx<R, false>), dim3(total), dim3(TB), 0, st, lv);
+#define M2L_CASE(C, B) hipLaunchKernelGGL((kM2Lx<R, C, B>), dim3(total), dim3(TB), 0, st, lv)
+    if (nb == 1) { if (conjT) M2L_CASE(true, 1); else M2L_CASE(false, 1); }
+    else if (nb == 2) { if (conjT) M2L_CASE(true, 2); else M2L_CASE(false, 2); }
+    else if (nb == 4) { if (conjT) M2L_CASE(true, 4); else M2L_CASE(false, 4); }
+    else if (nb == 8) { if (conjT) M2L_CASE(true, 8); else M2L_CASE(false, 8); }
+    else throw Error(MLFMA_ERR_ARG, "right-hand-side batch must be 1, 2, 4 or 8");
+#undef M2L_CASE
 }
 
 // ------------------------------------------------------------------ fused L2L
@@ -285,12 +343,14 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
                                                 const int32_t* __restrict__ atThSrc, const double* __restrict__ atThW,
                                                 const int32_t* __restrict__ atThFlip, int nAtTh,
                                                 const cplx<R>* __restrict__ phaseDn, const cplx<R>* __restrict__ Lp,
-                                                cplx<R>* __restrict__ Lc) {
+                                                cplx<R>* __restrict__ Lc, int64_t fs) {
     // ADJ: the adjoint of M2M for one child, conj(phase_up) (child grid) times
     // the anterpolated parent field; phaseDn then holds the child-grid phases
     extern __shared__ __align__(16) unsigned char smem[];
     const int item = xcdItem(blockIdx.x, per);
     if (item >= nitems) return;
+    Lp += blockIdx.z * fs;  // right-hand side blockIdx.z
+    Lc += blockIdx.z * fs;
     const int ndirP = mthP * mphP;
     const int nInner = mthP * nphC;
     cplx<R>* sP = reinterpret_cast<cplx<R>*>(smem);
@@ -366,21 +426,22 @@ bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32
                     int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW, int nAtPh,
                     const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
                     int nAtTh, const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax,
-                    hipStream_t st) {
+                    hipStream_t st, int nb, int64_t fs) {
     if (nitems <= 0) return true;
     const size_t lds = sizeof(cplx<R>) * 2 * ((size_t)mthP * mphP + (size_t)mthP * nphC) +
                        sizeof(R) * ((size_t)nAtPh + nAtTh) +
                        sizeof(int) * ((size_t)nphC + 1 + nAtPh + nthC + 1 + nAtTh);
     if (lds > ldsMax) return false;
     const int per = (nitems + 7) / 8;
+    const dim3 g(8 * per, 1, nb);
     if (adjoint)
-        hipLaunchKernelGGL((kL2LFused<R, true>), dim3(8 * per), dim3(TB), lds, st, parent, child, oct, nitems, per,
-                           mthP, mphP, nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip,
-                           nAtTh, phaseDn, Lp, Lc);
+        hipLaunchKernelGGL((kL2LFused<R, true>), g, dim3(TB), lds, st, parent, child, oct, nitems, per, mthP, mphP,
+                           nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip, nAtTh,
+                           phaseDn, Lp, Lc, fs);
     else
-        hipLaunchKernelGGL((kL2LFused<R, false>), dim3(8 * per), dim3(TB), lds, st, parent, child, oct, nitems, per,
-                           mthP, mphP, nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip,
-                           nAtTh, phaseDn, Lp, Lc);
+        hipLaunchKernelGGL((kL2LFused<R, false>), g, dim3(TB), lds, st, parent, child, oct, nitems, per, mthP, mphP,
+                           nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip, nAtTh,
+                           phaseDn, Lp, Lc, fs);
     return true;
 }
 
@@ -647,6 +708,101 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     }
 }
 
+// The same for NB >= 2 right-hand sides (x column stride xs, partial sums of
+// right-hand side r at part + r ps): each value is read once and used 2 NB
+// times.  Items have at most 128 rows; tiles of 4 values per lane.
+constexpr int NTB = 4 * TB;
+constexpr int CWB = 64;
+
+template <bool CONJ, int NB>
+__global__ __launch_bounds__(TB) void kNearSymB(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
+                                                const int64_t* __restrict__ perm, const cplx<double>* __restrict__ val,
+                                                const cplx<double>* __restrict__ x, int64_t xs,
+                                                cplx<double>* __restrict__ part, int64_t ps) {
+    __shared__ cplx<double> sT[NTB];
+    __shared__ cplx<double> sxr[NB][128];
+    __shared__ cplx<double> sxc[NB][CWB];
+    __shared__ double sre[TB], sim[TB];
+    const NearItem I = items[blockIdx.x];
+    const int nr = I.nr;  // 1..128
+    const int ld = nr | 1;
+    const int cwMax = min(CWB, NTB / ld);
+    const int t0 = threadIdx.x;
+    for (int e = t0; e < NB * nr; e += TB) {
+        const int rh = e / nr, q = e - rh * nr;
+        sxr[rh][q] = x[rh * xs + perm[I.rbeg + q]];
+    }
+    const int G = TB / nr;
+    const int r = t0 % nr, g = t0 / nr;
+    double re[NB], im[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) re[k] = im[k] = 0.0;
+    double* sTd = reinterpret_cast<double*>(sT);
+    for (int c0 = 0; c0 < I.nc; c0 += cwMax) {
+        const int cw = min(cwMax, I.nc - c0);
+        const int ne = cw * nr;
+        const double* sd = reinterpret_cast<const double*>(val + I.voff + (int64_t)c0 * nr);
+        // four loads per lane in flight, held as plain doubles
+        const int q0 = 2 * min(t0, ne - 1), q1 = 2 * min(t0 + TB, ne - 1);
+        const int q2 = 2 * min(t0 + 2 * TB, ne - 1), q3 = 2 * min(t0 + 3 * TB, ne - 1);
+        const double v0r = sd[q0], v0i = sd[q0 + 1], v1r = sd[q1], v1i = sd[q1 + 1];
+        const double v2r = sd[q2], v2i = sd[q2 + 1], v3r = sd[q3], v3i = sd[q3 + 1];
+        __syncthreads();  // the previous tile is consumed
+#define NEARB_ST(K, VR, VI)                                   \
+    if (t0 + K * TB < ne) {                                   \
+        const int e = t0 + K * TB;                            \
+        const int c = e / nr;                                 \
+        const int a = 2 * (c * ld + (e - c * nr));            \
+        sTd[a] = VR;                                          \
+        sTd[a + 1] = CONJ ? -(VI) : (VI);                     \
+    }
+        NEARB_ST(0, v0r, v0i) NEARB_ST(1, v1r, v1i) NEARB_ST(2, v2r, v2i) NEARB_ST(3, v3r, v3i)
+#undef NEARB_ST
+        for (int e = t0; e < NB * cw; e += TB) {
+            const int rh = e / cw, c = e - rh * cw;
+            sxc[rh][c] = x[rh * xs + colsIn[I.cptr + c0 + c]];
+        }
+        __syncthreads();
+        if (g < G)
+            for (int c = g; c < cw; c += G) {
+                const cplx<double> z = sT[c * ld + r];
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const cplx<double> xx = sxc[k][c];
+                    re[k] += z.re * xx.re - z.im * xx.im;
+                    im[k] += z.re * xx.im + z.im * xx.re;
+                }
+            }
+        for (int e = t0; e < NB * cw; e += TB) {
+            const int rh = e / cw, c = e - rh * cw;
+            if (c0 + c < I.nself) continue;
+            const cplx<double>* col = sT + c * ld;
+            double cr = 0, ci = 0;
+            for (int q = 0; q < nr; ++q) {
+                const cplx<double> z = col[q], xx = sxr[rh][q];
+                cr += z.re * xx.re - z.im * xx.im;
+                ci += z.re * xx.im + z.im * xx.re;
+            }
+            part[rh * ps + I.cpo + c0 + c - I.nself] = mk(cr, ci);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        __syncthreads();
+        sre[t0] = (g < G) ? re[k] : 0.0;
+        sim[t0] = (g < G) ? im[k] : 0.0;
+        __syncthreads();
+        if (t0 < nr) {
+            double a = 0, b = 0;
+            for (int q = 0; q < G; ++q) {
+                a += sre[q * nr + t0];
+                b += sim[q * nr + t0];
+            }
+            part[k * ps + I.rpo + t0] = mk(a, b);
+        }
+    }
+}
+
 // Sum of the near-field partials of tree position t, split over the 64 lanes
 // of a wave (same lane assignment and butterfly every call: deterministic).
 __device__ __forceinline__ cplx<double> nearFold(const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
@@ -661,51 +817,78 @@ __device__ __forceinline__ cplx<double> nearFold(const int64_t* __restrict__ ptr
 }
 
 // y[perm[t]] (=|+=) near-field sum, one wave per listed row (rows that have no
-// far-field leaf; the far-field rows are finished by kL2Tx).
+// far-field leaf; the far-field rows are finished by kL2Tx); NB right-hand
+// sides (y column stride xs, partials at stride ps).
+template <int NB>
 __global__ __launch_bounds__(TB) void kNearSum(const int64_t* __restrict__ rows, int64_t nrows,
                                                const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
-                                               const cplx<double>* __restrict__ part, const int64_t* __restrict__ perm,
-                                               cplx<double>* __restrict__ y, int accumulate) {
+                                               const cplx<double>* __restrict__ part, int64_t ps,
+                                               const int64_t* __restrict__ perm, cplx<double>* __restrict__ y,
+                                               int64_t xs, int accumulate) {
     const int64_t w = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w >= nrows) return;  // uniform per wave
     const int64_t t = rows[w];
-    const cplx<double> v = nearFold(ptr, idx, part, t, lane);
-    if (lane == 0) {
-        const int64_t o = perm[t];
-        y[o] = accumulate ? y[o] + v : v;
+    const int64_t o = perm[t];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const cplx<double> v = nearFold(ptr, idx, part + k * ps, t, lane);
+        if (lane == 0) y[k * xs + o] = accumulate ? y[k * xs + o] + v : v;
     }
 }
 
 void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
-                   const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st) {
+                   const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb,
+                   int64_t xs, int64_t ps) {
     if (nitems <= 0) return;
-    if (conjVal)
-        hipLaunchKernelGGL(kNearSym<true>, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
-    else
-        hipLaunchKernelGGL(kNearSym<false>, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
+#define NEARB_CASE(C, B) \
+    hipLaunchKernelGGL((kNearSymB<C, B>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, xs, part, ps)
+    if (nb == 1) {
+        if (conjVal)
+            hipLaunchKernelGGL(kNearSym<true>, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
+        else
+            hipLaunchKernelGGL(kNearSym<false>, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
+    } else if (nb == 2) {
+        if (conjVal) NEARB_CASE(true, 2); else NEARB_CASE(false, 2);
+    } else if (nb == 4) {
+        if (conjVal) NEARB_CASE(true, 4); else NEARB_CASE(false, 4);
+    } else if (nb == 8) {
+        if (conjVal) NEARB_CASE(true, 8); else NEARB_CASE(false, 8);
+    } else
+        throw Error(MLFMA_ERR_ARG, "right-hand-side batch must be 1, 2, 4 or 8");
+#undef NEARB_CASE
 }
 
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
-                   const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st) {
+                   const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st,
+                   int nb, int64_t xs, int64_t ps) {
     if (nrows <= 0) return;
-    const int64_t nb = (nrows + TB / 64 - 1) / (TB / 64);
-    hipLaunchKernelGGL(kNearSum, dim3((unsigned)nb), dim3(TB), 0, st, rows, nrows, sumPtr, sumIdx, part, perm, y,
-                       accumulate);
+    const int64_t nbk = (nrows + TB / 64 - 1) / (TB / 64);
+#define NSUM_CASE(B)                                                                                          \
+    hipLaunchKernelGGL(kNearSum<B>, dim3((unsigned)nbk), dim3(TB), 0, st, rows, nrows, sumPtr, sumIdx, part, ps, \
+                       perm, y, xs, accumulate)
+    if (nb == 1) NSUM_CASE(1);
+    else if (nb == 2) NSUM_CASE(2);
+    else if (nb == 4) NSUM_CASE(4);
+    else if (nb == 8) NSUM_CASE(8);
+    else throw Error(MLFMA_ERR_ARG, "right-hand-side batch must be 1, 2, 4 or 8");
+#undef NSUM_CASE
 }
 
 // ------------------------------------------------------------------ L2T
 // y_s (=|+=) near_s + C k sum_d conj(rp_s[d]) . L[d]  (src/fmm/leaf.cpp:257-280,
 // with the near-field partial sums of the row folded in).  One wave per source
-// row of a far-field leaf; the row's leaf gives its local field.
-template <typename R>
+// row of a far-field leaf; the row's leaf gives its local field.  NB
+// right-hand sides share every pattern read (fields at stride fs, y at column
+// stride xs, partials at stride ps).
+template <typename R, int NB>
 __global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, const int32_t* __restrict__ rowItem,
                                             int64_t nrows, const LeafItem* __restrict__ items,
-                                            const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc,
+                                            const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc, int64_t fs,
                                             cplx<double> Ck, const int64_t* __restrict__ sumPtr,
                                             const int32_t* __restrict__ sumIdx, const cplx<double>* __restrict__ part,
-                                            const int64_t* __restrict__ perm, cplx<double>* __restrict__ y,
-                                            int accumulate) {
+                                            int64_t ps, const int64_t* __restrict__ perm, cplx<double>* __restrict__ y,
+                                            int64_t xs, int accumulate) {
     const int64_t w = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w >= nrows) return;  // uniform per wave
@@ -713,46 +896,62 @@ __global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, 
     const LeafItem it = items[rowItem[w]];
     const cplx<R>* r = rp + it.rpOff + (t - it.sbeg) * it.ndir * 2;
     const cplx<R>* Lb = Lc + it.fOff;
-    R re = 0, im = 0;
+    R re[NB], im[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) re[k] = im[k] = 0;
     for (int d = lane; d < it.ndir; d += 64) {
-        const cplx<R> v = conjmul(r[2 * d], Lb[2 * d]) + conjmul(r[2 * d + 1], Lb[2 * d + 1]);
-        re += v.re;
-        im += v.im;
+        const cplx<R> r0 = r[2 * d], r1 = r[2 * d + 1];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const cplx<R> v = conjmul(r0, Lb[k * fs + 2 * d]) + conjmul(r1, Lb[k * fs + 2 * d + 1]);
+            re[k] += v.re;
+            im[k] += v.im;
+        }
     }
-    re = wsum(re);
-    im = wsum(im);
-    const cplx<double> nearv = nearFold(sumPtr, sumIdx, part, t, lane);
-    if (lane == 0) {
-        const int64_t o = perm[t];
-        const cplx<double> v = nearv + Ck * mk((double)re, (double)im);
-        y[o] = accumulate ? y[o] + v : v;
+    const int64_t o = perm[t];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const R a = wsum(re[k]), b = wsum(im[k]);
+        const cplx<double> nearv = nearFold(sumPtr, sumIdx, part + k * ps, t, lane);
+        if (lane == 0) {
+            const cplx<double> v = nearv + Ck * mk((double)a, (double)b);
+            y[k * xs + o] = accumulate ? y[k * xs + o] + v : v;
+        }
     }
 }
 
 template <typename R>
 void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,
                 const cplx<R>* Lc, cplx<double> Ck, const int64_t* sumPtr, const int32_t* sumIdx,
-                const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st) {
+                const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st, int nb,
+                int64_t fs, int64_t xs, int64_t ps) {
     if (nrows <= 0) return;
-    const int64_t nb = (nrows + TB / 64 - 1) / (TB / 64);
-    hipLaunchKernelGGL(kL2Tx<R>, dim3((unsigned)nb), dim3(TB), 0, st, rowPos, rowItem, nrows, items, rp, Lc, Ck, sumPtr,
-                       sumIdx, part, perm, y, accumulate);
+    const int64_t nbk = (nrows + TB / 64 - 1) / (TB / 64);
+#define L2T_CASE(B)                                                                                              \
+    hipLaunchKernelGGL((kL2Tx<R, B>), dim3((unsigned)nbk), dim3(TB), 0, st, rowPos, rowItem, nrows, items, rp, Lc, fs, \
+                       Ck, sumPtr, sumIdx, part, ps, perm, y, xs, accumulate)
+    if (nb == 1) L2T_CASE(1);
+    else if (nb == 2) L2T_CASE(2);
+    else if (nb == 4) L2T_CASE(4);
+    else if (nb == 8) L2T_CASE(8);
+    else throw Error(MLFMA_ERR_ARG, "right-hand-side batch must be 1, 2, 4 or 8");
+#undef L2T_CASE
 }
 
 #define INST(R)                                                                                                     \
     template void launchS2Mx<R>(const LeafItem*, int, int, const cplx<R>*, const cplx<double>*, const int64_t*,     \
-                                cplx<R>*, hipStream_t);                                                             \
+                                cplx<R>*, hipStream_t, int, int64_t, int64_t);                                     \
     template bool launchM2MFused<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                     int, int, const int32_t*, const double*, const int32_t*, const double*,         \
-                                    const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t);                 \
-    template void launchM2Lx<R>(M2LLevels<R>, bool, hipStream_t);                                                   \
+                                    const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t, int, int64_t);   \
+    template void launchM2Lx<R>(M2LLevels<R>, bool, hipStream_t, int);                                              \
     template bool launchL2LFused<R>(bool, int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,   \
                                     const int32_t*, const int32_t*, const double*, int, const int32_t*,             \
                                     const int32_t*, const double*, const int32_t*, int, const cplx<R>*,             \
-                                    const cplx<R>*, cplx<R>*, size_t, hipStream_t);                                 \
+                                    const cplx<R>*, cplx<R>*, size_t, hipStream_t, int, int64_t);                   \
     template void launchL2Tx<R>(const int64_t*, const int32_t*, int64_t, const LeafItem*, const cplx<R>*,           \
                                 const cplx<R>*, cplx<double>, const int64_t*, const int32_t*, const cplx<double>*,  \
-                                const int64_t*, cplx<double>*, int, hipStream_t);                                   \
+                                const int64_t*, cplx<double>*, int, hipStream_t, int, int64_t, int64_t, int64_t);   \
     template bool launchL2LAdj<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                   int, int, const int32_t*, const double*, const int32_t*, const double*,           \
                                   const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t);                   \

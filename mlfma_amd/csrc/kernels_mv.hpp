@@ -5,9 +5,12 @@
 
 namespace mlfma {
 
+// Batched launchers take nb right-hand sides (1, 2, 4 or 8): x and y with
+// column stride xs (input order), field arrays at stride fs, near-field
+// partial sums at stride ps.
 template <typename R>
 void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* x,
-                const int64_t* perm, cplx<R>* M, hipStream_t st);
+                const int64_t* perm, cplx<R>* M, hipStream_t st, int nb = 1, int64_t xs = 0, int64_t fs = 0);
 
 // returns false when the level's fields do not fit the LDS limit (caller
 // then uses the two-pass kernels)
@@ -15,7 +18,7 @@ template <typename R>
 bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
                     int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
                     const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
-                    size_t ldsMax, hipStream_t st);
+                    size_t ldsMax, hipStream_t st, int nb = 1, int64_t fs = 0);
 
 // M2L work of every level, passed by value to one launch
 constexpr int kMaxLevels = 16;
@@ -30,12 +33,13 @@ struct M2LLevels {
     const cplx<R>* T[kMaxLevels];
     const cplx<R>* M[kMaxLevels];
     cplx<R>* L[kMaxLevels];
+    int64_t fs = 0;  // field stride between right-hand sides
 };
 
 // conjT: conjugated operators (the adjoint pass; offsets then name the
 // reversed pairs)
 template <typename R>
-void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st);
+void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb = 1);
 
 // adjoint = true: the adjoint of M2M instead (phaseDn then holds the
 // child-grid up-phases, applied conjugated after the anterpolation)
@@ -44,20 +48,23 @@ bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32
                     int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW, int nAtPh,
                     const int32_t* atThPtr, const int32_t* atThSrc, const double* atThW, const int32_t* atThFlip,
                     int nAtTh, const cplx<R>* phaseDn, const cplx<R>* Lp, cplx<R>* Lc, size_t ldsMax,
-                    hipStream_t st);
+                    hipStream_t st, int nb = 1, int64_t fs = 0);
 
 // near field: kNearSym writes partial sums; kL2Tx folds them into the rows of
 // far-field leaves and kNearSum into the listed remaining rows
 void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
-                   const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st);
+                   const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb = 1,
+                   int64_t xs = 0, int64_t ps = 0);
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
-                   const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st);
+                   const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st,
+                   int nb = 1, int64_t xs = 0, int64_t ps = 0);
 
 // y (=|+=) near + L2T for the rows of far-field leaves
 template <typename R>
 void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,
                 const cplx<R>* Lc, cplx<double> Ck, const int64_t* sumPtr, const int32_t* sumIdx,
-                const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st);
+                const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st,
+                int nb = 1, int64_t fs = 0, int64_t xs = 0, int64_t ps = 0);
 
 // adjoint of L2L: parents gain conj(phase_down) * Interp(child), per parent CSR
 template <typename R>

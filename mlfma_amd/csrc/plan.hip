@@ -6,12 +6,15 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "host.hpp"
 #include "kernels.hpp"
 #include "kernels_mv.hpp"
 #include "plan.hpp"
+
+#include <type_traits>
 
 namespace mlfma {
 
@@ -438,7 +441,7 @@ void DevicePlan::build() {
     std::vector<NearItem> nitemsH(h.nearItems.size());
     for (size_t q = 0; q < nitemsH.size(); ++q) {
         const auto& I = h.nearItems[q];
-        if (I.nr < 1 || I.nr > 256 || I.nc < 1 || I.nself < 0 || I.nself > I.nc)
+        if (I.nr < 1 || I.nr > 128 || I.nc < 1 || I.nself < 0 || I.nself > I.nc)
             throw Error(MLFMA_ERR_STATE, "near-field item out of range");
         nitemsH[q] = NearItem{I.rbeg, I.nr, I.nc, I.nself, 0, I.cptr, I.voff, I.rpo, I.cpo};
     }
@@ -515,29 +518,38 @@ void DevicePlan::ensureRhs(int nrhs) {
 
 // One matvec y (=|+=) Z x on input-order device vectors (single right-hand side).
 template <typename R>
-void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed) {
+void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed, int nb,
+                       int64_t xs) {
+    // nb right-hand sides (1, 2, 4 or 8) with column stride xs; their fields
+    // at stride fs in the field arrays, their near-field partials at stride ps
+    if (nb > batchCap) throw Error(MLFMA_ERR_STATE, "right-hand-side batch larger than the plan's buffers");
     const int ML = hp.tree.maxLevel;
     const int p = hp.cfg.interp_order;
+    const int64_t fs = fieldLen, ps = hp.nearParts;
     cplx<R>* M = fieldM<R>();
     cplx<R>* Lc = fieldL<R>();
     cplx<R>* inner = scratch<R>();
     const cplx<R>* rp = radpat<R>();
     if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
-    launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st);
+    launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
     if (timed) hipCheck(hipEventRecord(ev[1], st), "event");
     for (int l = ML - 1; l >= 2; --l) {
         LevelBufs& P = levels[l];
         LevelBufs& C = levels[l + 1];
         if (!launchM2MFused<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph, p, P.thNear,
-                               P.thW, P.phNear, P.phW, C.phUp<R>(), M + C.Moff, M + P.Moff, ldsMax, st)) {
-            launchM2MTheta<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, C.nth, C.nph, p, P.thNear, P.thW,
-                              C.phUp<R>(), M + C.Moff, inner, st);
-            launchM2MPhi<R>(P.nstem, P.stemSlot, P.nth, P.nph, C.nph, p, P.phNear, P.phW, inner, M + P.Moff, st);
+                               P.thW, P.phNear, P.phW, C.phUp<R>(), M + C.Moff, M + P.Moff, ldsMax, st, nb, fs)) {
+            for (int r = 0; r < nb; ++r) {
+                launchM2MTheta<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, C.nth, C.nph, p, P.thNear,
+                                  P.thW, C.phUp<R>(), M + r * fs + C.Moff, inner, st);
+                launchM2MPhi<R>(P.nstem, P.stemSlot, P.nth, P.nph, C.nph, p, P.phNear, P.phW, inner,
+                                M + r * fs + P.Moff, st);
+            }
         }
     }
     if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
     {
         M2LLevels<R> lv;
+        lv.fs = fs;
         for (int l = 2; l <= ML; ++l) {
             LevelBufs& L = levels[l];
             if (L.ntgt == 0) continue;
@@ -553,33 +565,36 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
             lv.M[k] = M + L.Moff;
             lv.L[k] = Lc + L.Moff;
         }
-        launchM2Lx<R>(lv, false, st);
+        launchM2Lx<R>(lv, false, st, nb);
     }
     if (flags & MLFMA_FLAG_FRESH_LOCALS)
-        for (int l = 2; l <= ML; ++l) {
-            LevelBufs& L = levels[l];
-            launchZeroSlots<R>(L.nstale, L.staleSlots, L.ndir, Lc + L.Moff, st);
-        }
+        for (int r = 0; r < nb; ++r)
+            for (int l = 2; l <= ML; ++l) {
+                LevelBufs& L = levels[l];
+                launchZeroSlots<R>(L.nstale, L.staleSlots, L.ndir, Lc + r * fs + L.Moff, st);
+            }
     if (timed) hipCheck(hipEventRecord(ev[3], st), "event");
     for (int l = 3; l <= ML; ++l) {
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];
         if (!launchL2LFused<R>(false, C.nl2l, C.l2lParent, C.l2lChild, C.l2lOct, P.nth, P.nph, C.nth, C.nph, P.atPhPtr,
                                P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, P.nAtTh,
-                               P.phDn<R>(), Lc + P.Moff, Lc + C.Moff, ldsMax, st)) {
-            launchL2LPhi<R>(C.nl2l, C.l2lParent, C.l2lOct, P.nth, P.nph, C.nph, P.atPhPtr, P.atPhSrc, P.atPhW,
-                            P.phDn<R>(), Lc + P.Moff, inner, st);
-            launchL2LTheta<R>(C.nl2l, C.l2lChild, P.nth, C.nth, C.nph, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip,
-                              inner, Lc + C.Moff, st);
+                               P.phDn<R>(), Lc + P.Moff, Lc + C.Moff, ldsMax, st, nb, fs)) {
+            for (int r = 0; r < nb; ++r) {
+                launchL2LPhi<R>(C.nl2l, C.l2lParent, C.l2lOct, P.nth, P.nph, C.nph, P.atPhPtr, P.atPhSrc, P.atPhW,
+                                P.phDn<R>(), Lc + r * fs + P.Moff, inner, st);
+                launchL2LTheta<R>(C.nl2l, C.l2lChild, P.nth, C.nth, C.nph, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip,
+                                  inner, Lc + r * fs + C.Moff, st);
+            }
         }
     }
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
-    launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
+    launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
-    launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st);
+    launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
     launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, Lc, Ck, nearSumPtr, nearSumIdx, nearPart, perm, y,
-                  accumulate, st);
+                  accumulate, st, nb, fs, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[7], st), "event");
 }
 
@@ -655,19 +670,52 @@ void DevicePlan::applyOpAdjoint(const cplx<double>* x, cplx<double>* y, int accu
         applyAdjoint<double>(x, y, accumulate, st);
 }
 
-void DevicePlan::applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
+void DevicePlan::applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed,
+                         int nb, int64_t xs) {
     if (fp32) {
         if (!useFp32) throw Error(MLFMA_ERR_STATE, "plan was not built for FP32");
-        apply<float>(x, y, accumulate, st, timed);
+        apply<float>(x, y, accumulate, st, timed, nb, xs);
     } else
-        apply<double>(x, y, accumulate, st, timed);
+        apply<double>(x, y, accumulate, st, timed, nb, xs);
 }
 
+// Grow the field arrays and near-field partials to hold nb right-hand sides.
+void DevicePlan::ensureBatch(int nb) {
+    if (nb <= batchCap) return;
+    hipCheck(hipStreamSynchronize(stream), "batch resize");
+    const int64_t len = std::max<int64_t>(fieldLen, 1) * nb;
+    auto regrow = [&](auto*& ptr, size_t elem) {
+        using T = std::remove_reference_t<decltype(*ptr)>;
+        if (ptr) release(ptr);
+        ptr = (T*)alloc(len * elem);
+        hipCheck(hipMemset(ptr, 0, len * elem), "memset");
+    };
+    regrow(M64, sizeof(cplx<double>));
+    regrow(L64, sizeof(cplx<double>));
+    if (useFp32) {
+        regrow(M32, sizeof(cplx<float>));
+        regrow(L32, sizeof(cplx<float>));
+    }
+    release(nearPart);
+    nearPart = (cplx<double>*)alloc(std::max<int64_t>(hp.nearParts, 1) * nb * sizeof(cplx<double>));
+    batchCap = nb;
+}
+
+// Right-hand sides go through in batches of 8, 4, 2, 1 when the operator is
+// linear; a plan that keeps stale locals (the reference's state carried from
+// one matvec to the next) applies them one after another.
 void DevicePlan::matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int accumulate, hipStream_t st, bool fp32) {
     const int64_t n = hp.src.n;
     const bool timed = (flags & MLFMA_FLAG_STAGE_TIMES) != 0;
-    for (int r = 0; r < nrhs; ++r)
-        applyOp(x + (size_t)r * n, y + (size_t)r * n, accumulate, st, fp32, timed && r == 0);
+    const bool linear = hp.staleBoxes == 0 || (flags & MLFMA_FLAG_FRESH_LOCALS);
+    int r = 0;
+    while (r < nrhs) {
+        const int left = nrhs - r;
+        const int nb = !linear ? 1 : (left >= 8 ? 8 : (left >= 4 ? 4 : (left >= 2 ? 2 : 1)));
+        if (nb > 1) ensureBatch(nb);
+        applyOp(x + (size_t)r * n, y + (size_t)r * n, accumulate, st, fp32, timed && r == 0, nb, n);
+        r += nb;
+    }
     hipCheck(hipGetLastError(), "matvec launch");
     if (timed) timesValid = true;
 }

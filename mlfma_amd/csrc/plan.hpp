@@ -5,7 +5,7 @@
 //   * multipole / local fields: per level >= 2, [box slot][direction][theta|phi];
 //   * translation operators: per level, [316 offsets][direction], weights folded;
 //   * near field: each unordered pair stored once (self blocks full), cut into
-//     work items of <= 256 rows; column-major values (C k folded).
+//     work items of <= 128 rows; column-major values (C k folded).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -124,12 +124,16 @@ struct DevicePlan {
     void ensureRhs(int nrhs);
     // y (=|+=) Z x on device vectors in input order
     template <typename R>
-    void apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed);
+    void apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed, int nb = 1,
+               int64_t xs = 0);
+    int batchCap = 1;  // right-hand sides the field arrays and partials hold
+    void ensureBatch(int nb);
     // y (=|+=) Z^H x (the adjoint of the linear operator)
     template <typename R>
     void applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st);
     void applyOpAdjoint(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32);
-    void applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed);
+    void applyOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed,
+                 int nb = 1, int64_t xs = 0);
     // the operator of the running solve: Z, or Z^H for an adjoint solve
     bool solveAdjoint = false;
     void applySolveOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
