@@ -77,6 +77,11 @@ typedef struct mlfma_planewave {
                                        reference's ClockTimes, src/clock.h:8-17);
                                        each event costs a few microseconds of
                                        device idle time */
+#define MLFMA_FLAG_M2L_GROUP 4u     /* M2L through the sibling-group kernel on
+                                       every level (default: chosen per level
+                                       from the list overlap and block count) */
+#define MLFMA_FLAG_M2L_TARGET 8u    /* M2L through the per-target kernel on
+                                       every level */
 
 /* Numerical configuration (src/config.h:50-73) plus device selection. */
 typedef struct mlfma_config {

@@ -237,6 +237,81 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
 
 // ------------------------------------------------------------------ M2L
 // L_t[d] = sum over the interaction list of T_off[d] M_s[d]  (src/fmm/node.cpp:105-134),
+// quadrature weights folded into T.  A block is (level, sibling group, chunk
+// of 256 directions): the up to eight children of one parent share most of
+// their sources, so each distinct source field is read once and applied to
+// every sibling whose list holds it, with that pair's operator (scalar
+// branches on the uniform offset table).  One launch covers every level;
+// grid z is the right-hand side (fields at stride lv.fs).  CONJ: conjugated
+// operators (the adjoint pass, whose offsets name the reversed pairs).
+template <typename R, bool CONJ>
+__global__ __launch_bounds__(TB) void kM2LGroup(M2LGroups<R> lv) {
+    int b = blockIdx.x, l = 0;
+    while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
+    b -= lv.blk[l];
+    const int nch = lv.nch[l];
+    const int item = xcdItem(b / nch, lv.per[l]);
+    const int chunk = b - (b / nch) * nch;
+    if (item >= lv.ngrp[l]) return;
+    const int ndir = lv.ndir[l];
+    const int d = chunk * TB + threadIdx.x;
+    if (d >= ndir) return;
+    const int64_t zo = (int64_t)blockIdx.z * lv.fs;
+    const int32_t* src = lv.src[l];
+    const int32_t* off = lv.off[l];
+    const cplx<R>* T = lv.T[l] + d;
+    const cplx<R>* M = lv.M[l] + zo + 2 * d;
+    const int q0 = lv.ptr[l][item], q1 = lv.ptr[l][item + 1];
+    cplx<R> acc[8][2];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k][0] = acc[k][1] = mk<R>(0, 0);
+    for (int q = q0; q < q1; ++q) {
+        const cplx<R>* m = M + (int64_t)src[q] * ndir * 2;
+        const cplx<R> m0 = m[0], m1 = m[1];
+        const int32_t* o8 = off + 8 * q;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int o = o8[k];
+            if (o >= 0) {
+                cplx<R> t = T[(int64_t)o * ndir];
+                if (CONJ) t.im = -t.im;
+                acc[k][0] = acc[k][0] + t * m0;
+                acc[k][1] = acc[k][1] + t * m1;
+            }
+        }
+    }
+    const int32_t* tg = lv.tgt[l] + 8 * item;
+    cplx<R>* L = lv.L[l] + zo + 2 * d;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int slot = tg[k];
+        if (slot >= 0) {
+            cplx<R>* out = L + (int64_t)slot * ndir * 2;
+            out[0] = acc[k][0];
+            out[1] = acc[k][1];
+        }
+    }
+}
+
+template <typename R>
+void launchM2LGroup(M2LGroups<R> lv, bool conjT, hipStream_t st, int nb) {
+    int total = 0;
+    for (int l = 0; l < lv.nlev; ++l) {
+        lv.nch[l] = (lv.ndir[l] + TB - 1) / TB;
+        lv.per[l] = (lv.ngrp[l] + 7) / 8;
+        lv.blk[l] = total;
+        total += 8 * lv.per[l] * lv.nch[l];
+    }
+    if (total <= 0) return;
+    const dim3 g(total, 1, nb);
+    if (conjT)
+        hipLaunchKernelGGL((kM2LGroup<R, true>), g, dim3(TB), 0, st, lv);
+    else
+        hipLaunchKernelGGL((kM2LGroup<R, false>), g, dim3(TB), 0, st, lv);
+}
+
+// ------------------------------------------------------------------ M2L per target
+// L_t[d] = sum over the interaction list of T_off[d] M_s[d]  (src/fmm/node.cpp:105-134),
 // quadrature weights folded into T.  One launch covers every level: a block
 // is (level, target box, chunk of 256 directions); the (source, offset) list
 // is read with uniform loads; within a level the targets of one XCD are
@@ -944,6 +1019,7 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
     template bool launchM2MFused<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                     int, int, const int32_t*, const double*, const int32_t*, const double*,         \
                                     const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t, int, int64_t);   \
+    template void launchM2LGroup<R>(M2LGroups<R>, bool, hipStream_t, int);                                          \
     template void launchM2Lx<R>(M2LLevels<R>, bool, hipStream_t, int);                                              \
     template bool launchL2LFused<R>(bool, int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,   \
                                     const int32_t*, const int32_t*, const double*, int, const int32_t*,             \

@@ -20,8 +20,29 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
                     const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
                     size_t ldsMax, hipStream_t st, int nb = 1, int64_t fs = 0);
 
-// M2L work of every level, passed by value to one launch
+// M2L work of every level (sibling groups), passed by value to one launch
 constexpr int kMaxLevels = 16;
+template <typename R>
+struct M2LGroups {
+    int nlev = 0;
+    int ndir[kMaxLevels], ngrp[kMaxLevels], nch[kMaxLevels], per[kMaxLevels], blk[kMaxLevels];
+    const int32_t* tgt[kMaxLevels];  // 8 target slots per group (-1: none)
+    const int32_t* ptr[kMaxLevels];  // per group: its source entries
+    const int32_t* src[kMaxLevels];  // source slot per entry
+    const int32_t* off[kMaxLevels];  // 8 offsets per entry (-1: not in that target's list)
+    const cplx<R>* T[kMaxLevels];
+    const cplx<R>* M[kMaxLevels];
+    cplx<R>* L[kMaxLevels];
+    int64_t fs = 0;  // field stride between right-hand sides
+};
+
+// conjT: conjugated operators (the adjoint pass; offsets then name the
+// reversed pairs); nb right-hand sides over grid z
+template <typename R>
+void launchM2LGroup(M2LGroups<R> lv, bool conjT, hipStream_t st, int nb = 1);
+
+// M2L per target box (one block per target and direction chunk), for levels
+// whose sibling lists overlap little; nb right-hand sides share the operator
 template <typename R>
 struct M2LLevels {
     int nlev = 0;
@@ -36,13 +57,9 @@ struct M2LLevels {
     int64_t fs = 0;  // field stride between right-hand sides
 };
 
-// conjT: conjugated operators (the adjoint pass; offsets then name the
-// reversed pairs)
 template <typename R>
 void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb = 1);
 
-// adjoint = true: the adjoint of M2M instead (phaseDn then holds the
-// child-grid up-phases, applied conjugated after the anterpolation)
 template <typename R>
 bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
                     int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW, int nAtPh,

@@ -6,7 +6,10 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <array>
+#include <map>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "host.hpp"
@@ -402,6 +405,57 @@ void DevicePlan::build() {
             }
             L.lOffNeg = upload(neg);
         }
+        {
+            // sibling groups for M2L: the targets of level l under one parent
+            // share most of their sources; per group, every distinct source
+            // with its offset to each of the (up to) eight sibling targets
+            std::unordered_map<int, int> gid;  // parent node -> group
+            std::vector<std::array<int32_t, 8>> gt;
+            std::vector<std::map<int32_t, std::array<int32_t, 8>>> gs;
+            for (size_t q = 0; q < tgt.size(); ++q) {
+                const int slot = tgt[q];
+                const Node& nd = t.nodes[h.slotNode[l][slot]];
+                auto it = gid.find(nd.parent);
+                int g;
+                if (it == gid.end()) {
+                    g = (int)gt.size();
+                    gid.emplace(nd.parent, g);
+                    gt.emplace_back();
+                    gt.back().fill(-1);
+                    gs.emplace_back();
+                } else
+                    g = it->second;
+                gt[g][nd.branch] = slot;
+                for (int e = ptr[q]; e < ptr[q + 1]; ++e) {
+                    auto ins = gs[g].emplace(sr[e], std::array<int32_t, 8>{});
+                    if (ins.second) ins.first->second.fill(-1);
+                    ins.first->second[nd.branch] = of[e];
+                }
+            }
+            std::vector<int32_t> gtf, gp{0}, gsrc, goff, goffn;
+            for (size_t g = 0; g < gt.size(); ++g) {
+                for (int k = 0; k < 8; ++k) gtf.push_back(gt[g][k]);
+                for (auto& kv : gs[g]) {
+                    gsrc.push_back(kv.first);
+                    for (int k = 0; k < 8; ++k) {
+                        const int o = kv.second[k];
+                        goff.push_back(o);
+                        int dx = 0, dy = 0, dz = 0;
+                        if (o >= 0) offsetVector(o, dx, dy, dz);
+                        goffn.push_back(o >= 0 ? offsetIndex(-dx, -dy, -dz) : -1);
+                    }
+                }
+                gp.push_back((int32_t)gsrc.size());
+            }
+            L.ngrp = (int)gt.size();
+            L.nGrpEntries = (int64_t)gsrc.size();
+            L.nPairs = (int64_t)sr.size();
+            L.gTgt = upload(gtf);
+            L.gPtr = upload(gp);
+            L.gSrc = upload(gsrc);
+            L.gOff = upload(goff);
+            L.gOffNeg = upload(goffn);
+        }
         L.nstale = (int)stale.size();
         L.staleSlots = upload(stale);
         // L2L into level l (from l-1)
@@ -517,6 +571,52 @@ void DevicePlan::ensureRhs(int nrhs) {
 }
 
 // One matvec y (=|+=) Z x on input-order device vectors (single right-hand side).
+// M2L of every level: levels whose sibling targets share most of their
+// sources (at least three list entries per distinct source) and that fill the
+// device go through the sibling-group kernel, the others through the
+// per-target kernel.  adjoint: conjugated operators of the reversed pairs.
+template <typename R>
+void DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb) {
+    const int ML = hp.tree.maxLevel;
+    if (ML - 1 > kMaxLevels) throw Error(MLFMA_ERR_ARG, "too many tree levels");
+    M2LGroups<R> g;
+    M2LLevels<R> t;
+    g.fs = t.fs = fieldLen;
+    for (int l = 2; l <= ML; ++l) {
+        LevelBufs& L = levels[l];
+        if (L.ntgt == 0) continue;
+        const int64_t nch = (L.ndir + 255) / 256;
+        bool group = L.nPairs >= 3 * L.nGrpEntries && (int64_t)L.ngrp * nch * nb >= 2048;
+        if (flags & MLFMA_FLAG_M2L_GROUP) group = true;
+        if (flags & MLFMA_FLAG_M2L_TARGET) group = false;
+        if (group) {
+            const int k = g.nlev++;
+            g.ndir[k] = L.ndir;
+            g.ngrp[k] = L.ngrp;
+            g.tgt[k] = L.gTgt;
+            g.ptr[k] = L.gPtr;
+            g.src[k] = L.gSrc;
+            g.off[k] = adjoint ? L.gOffNeg : L.gOff;
+            g.T[k] = L.T<R>();
+            g.M[k] = in + L.Moff;
+            g.L[k] = out + L.Moff;
+        } else {
+            const int k = t.nlev++;
+            t.ndir[k] = L.ndir;
+            t.ntgt[k] = L.ntgt;
+            t.tgt[k] = L.tgtSlot;
+            t.ptr[k] = L.lPtr;
+            t.src[k] = L.lSrc;
+            t.off[k] = adjoint ? L.lOffNeg : L.lOff;
+            t.T[k] = L.T<R>();
+            t.M[k] = in + L.Moff;
+            t.L[k] = out + L.Moff;
+        }
+    }
+    if (g.nlev) launchM2LGroup<R>(g, adjoint, st, nb);
+    if (t.nlev) launchM2Lx<R>(t, adjoint, st, nb);
+}
+
 template <typename R>
 void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed, int nb,
                        int64_t xs) {
@@ -548,24 +648,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     }
     if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
     {
-        M2LLevels<R> lv;
-        lv.fs = fs;
-        for (int l = 2; l <= ML; ++l) {
-            LevelBufs& L = levels[l];
-            if (L.ntgt == 0) continue;
-            if (lv.nlev == kMaxLevels) throw Error(MLFMA_ERR_ARG, "too many tree levels");
-            const int k = lv.nlev++;
-            lv.ndir[k] = L.ndir;
-            lv.ntgt[k] = L.ntgt;
-            lv.tgt[k] = L.tgtSlot;
-            lv.ptr[k] = L.lPtr;
-            lv.src[k] = L.lSrc;
-            lv.off[k] = L.lOff;
-            lv.T[k] = L.T<R>();
-            lv.M[k] = M + L.Moff;
-            lv.L[k] = Lc + L.Moff;
-        }
-        launchM2Lx<R>(lv, false, st, nb);
+        launchM2L<R>(M, Lc, false, st, nb);
     }
     if (flags & MLFMA_FLAG_FRESH_LOCALS)
         for (int r = 0; r < nb; ++r)
@@ -625,23 +708,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
             throw Error(MLFMA_ERR_ARG, "adjoint L2L: angular grid too large for one block");
     }
     {
-        M2LLevels<R> lv;
-        for (int l = 2; l <= ML; ++l) {
-            LevelBufs& L = levels[l];
-            if (L.ntgt == 0) continue;
-            if (lv.nlev == kMaxLevels) throw Error(MLFMA_ERR_ARG, "too many tree levels");
-            const int k = lv.nlev++;
-            lv.ndir[k] = L.ndir;
-            lv.ntgt[k] = L.ntgt;
-            lv.tgt[k] = L.tgtSlot;
-            lv.ptr[k] = L.lPtr;
-            lv.src[k] = L.lSrc;
-            lv.off[k] = L.lOffNeg;
-            lv.T[k] = L.T<R>();
-            lv.M[k] = Lc + L.Moff;  // reads the adjoint locals
-            lv.L[k] = M + L.Moff;   // writes the adjoint multipoles
-        }
-        launchM2Lx<R>(lv, true, st);
+        launchM2L<R>(Lc, M, true, st, 1);  // adjoint locals in, adjoint multipoles out
     }
     for (int l = 2; l < ML; ++l) {
         LevelBufs& P = levels[l];
@@ -921,7 +988,8 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
 int mlfma_plan_set_flags(mlfma_plan* plan, uint32_t flags) {
     API_BEGIN
     DevicePlan& d = P(plan);
-    if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES)) throw Error(MLFMA_ERR_ARG, "unknown flag bits");
+    if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES | MLFMA_FLAG_M2L_GROUP | MLFMA_FLAG_M2L_TARGET))
+        throw Error(MLFMA_ERR_ARG, "unknown flag bits");
     d.flags = flags;
     return MLFMA_OK;
     API_END

@@ -45,6 +45,11 @@ struct LevelBufs {
     int ntgt = 0;
     int32_t *tgtSlot = nullptr, *lPtr = nullptr, *lSrc = nullptr, *lOff = nullptr;
     int32_t* lOffNeg = nullptr;  // offsets of the reversed pairs (adjoint)
+    // M2L sibling groups: 8 target slots per group (-1 none), per group a CSR
+    // of distinct sources, 8 offsets per source entry (-1: not in that list)
+    int ngrp = 0;
+    int64_t nGrpEntries = 0, nPairs = 0;
+    int32_t *gTgt = nullptr, *gPtr = nullptr, *gSrc = nullptr, *gOff = nullptr, *gOffNeg = nullptr;
     int32_t* m2mAdjParent = nullptr;  // per M2M child entry: its parent slot (adjoint)
     int nm2mChild = 0;                // M2M child entries into this level
     int nstale = 0;
@@ -127,6 +132,8 @@ struct DevicePlan {
     void apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed, int nb = 1,
                int64_t xs = 0);
     int batchCap = 1;  // right-hand sides the field arrays and partials hold
+    template <typename R>
+    void launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb);
     void ensureBatch(int nb);
     // y (=|+=) Z^H x (the adjoint of the linear operator)
     template <typename R>
