@@ -82,6 +82,10 @@ typedef struct mlfma_planewave {
                                        from the list overlap and block count) */
 #define MLFMA_FLAG_M2L_TARGET 8u    /* M2L through the per-target kernel on
                                        every level */
+#define MLFMA_FLAG_ASYNC 16u        /* matvecs on device pointers return once
+                                       queued on the stream instead of after
+                                       completion (order by the stream; host
+                                       pointers always complete) */
 
 /* Numerical configuration (src/config.h:50-73) plus device selection. */
 typedef struct mlfma_config {
@@ -150,7 +154,8 @@ int mlfma_voltage(const mlfma_plan* plan, double* v_host);
  * Node::rvec.  x and y hold nrhs column vectors of n complex values each
  * (column stride n).  ptr_mode says whether x and y are host or device
  * pointers; stream is a hipStream_t (NULL = the plan's own stream).  The call
- * returns after the result is complete in y. */
+ * returns after the result is complete in y (device pointers with
+ * MLFMA_FLAG_ASYNC: once the work is queued on the stream). */
 int mlfma_matvec(mlfma_plan* plan, const double* x, double* y, int32_t nrhs,
                  int32_t accumulate, int32_t ptr_mode, void* stream);
 

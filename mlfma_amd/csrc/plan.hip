@@ -902,7 +902,7 @@ int mlfma_matvec(mlfma_plan* plan, const double* x, double* y, int32_t nrhs, int
     const bool fp32 = d.useFp32;
     if (ptr_mode == MLFMA_PTR_DEVICE) {
         d.matvec((const cplx<double>*)x, (cplx<double>*)y, nrhs, accumulate, st, fp32);
-        hipCheck(hipStreamSynchronize(st), "matvec");
+        if (!(d.flags & MLFMA_FLAG_ASYNC)) hipCheck(hipStreamSynchronize(st), "matvec");
     } else if (ptr_mode == MLFMA_PTR_HOST) {
         d.ensureRhs(nrhs);
         const size_t bytes = (size_t)n * nrhs * sizeof(cplx<double>);
@@ -933,7 +933,7 @@ int mlfma_matvec_adjoint(mlfma_plan* plan, const double* x, double* y, int32_t n
     };
     if (ptr_mode == MLFMA_PTR_DEVICE) {
         run((const cplx<double>*)x, (cplx<double>*)y);
-        hipCheck(hipStreamSynchronize(st), "adjoint matvec");
+        if (!(d.flags & MLFMA_FLAG_ASYNC)) hipCheck(hipStreamSynchronize(st), "adjoint matvec");
     } else if (ptr_mode == MLFMA_PTR_HOST) {
         d.ensureRhs(nrhs);
         const size_t bytes = (size_t)n * nrhs * sizeof(cplx<double>);
@@ -988,7 +988,8 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
 int mlfma_plan_set_flags(mlfma_plan* plan, uint32_t flags) {
     API_BEGIN
     DevicePlan& d = P(plan);
-    if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES | MLFMA_FLAG_M2L_GROUP | MLFMA_FLAG_M2L_TARGET))
+    if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES | MLFMA_FLAG_M2L_GROUP | MLFMA_FLAG_M2L_TARGET |
+                  MLFMA_FLAG_ASYNC))
         throw Error(MLFMA_ERR_ARG, "unknown flag bits");
     d.flags = flags;
     return MLFMA_OK;

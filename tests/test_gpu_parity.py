@@ -276,3 +276,20 @@ def test_m2l_kernels_agree(name):
     g, t = out[A.MLFMA_FLAG_M2L_GROUP], out[A.MLFMA_FLAG_M2L_TARGET]
     say(name, "group vs target", cases.relerr(g[0], t[0]), cases.relerr(g[1], t[1]))
     assert cases.relerr(g[0], t[0]) <= 1e-13 and cases.relerr(g[1], t[1]) <= 1e-13
+
+
+def test_async_device_matvec():
+    import torch
+    geom, kw = cases.make("rwg_fixed")
+    p = Plan(geom, cases.planewave_z(), A.make_config(device=0, **kw))
+    x = torch.from_numpy(G.complex_gaussian(geom.n, 8)).to("cuda:0")
+    y0 = p.matvec(x).cpu().numpy()
+    p.set_flags(A.MLFMA_FLAG_ASYNC)
+    ys = [p.matvec(x) for _ in range(4)]   # queued on torch's stream
+    torch.cuda.synchronize()
+    for y in ys:
+        assert cases.relerr(y.cpu().numpy(), y0) == 0.0
+    z = p.matvec_adjoint(x)
+    torch.cuda.synchronize()
+    p.set_flags(0)
+    assert cases.relerr(z.cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()) == 0.0
