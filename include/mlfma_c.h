@@ -82,6 +82,12 @@ typedef struct mlfma_planewave {
                                        from the list overlap and block count) */
 #define MLFMA_FLAG_M2L_TARGET 8u    /* M2L through the per-target kernel on
                                        every level */
+#define MLFMA_FLAG_PHYSICAL 32u     /* RWG only, fixed at creation: near-field
+                                       entries in the physical mixed-potential
+                                       form with the singular 1/R integrals done
+                                       in closed form, coincident triangles and
+                                       the diagonal included (the reference drops
+                                       them, src/sources/rwg.cpp:97,113-116) */
 #define MLFMA_FLAG_ASYNC 16u        /* matvecs on device pointers return once
                                        queued on the stream instead of after
                                        completion (order by the stream; host

@@ -117,7 +117,147 @@ __device__ __forceinline__ cplx<double> greenContract(const double ro[3], const 
     return (ir)*(e * c);
 }
 
+// ---- physical near field (no counterpart in the reference, which drops
+// coincident triangles and keeps the hypersingular dyadic kernel).
+//
+// Mixed-potential Galerkin entry with the reference's normalisation
+// (f = +-(l / 2A) rho, quadrature weights summing to 1/2, so the area factors
+// cancel):  Z_mn = l_m l_n sum_{T,T'} s s' sum_q sum_q' w w' (rho.rho' - 4/k^2)
+// e^{ikR}/R, equal to the reference's dyadic form for separated triangles
+// (integration by parts over both triangles of each RWG).  For triangles
+// closer than twice the longer edge the 1/R part of the inner integral is
+// done in closed form (Wilton, Rao, Glisson, IEEE TAP 32 (1984) 276-281;
+// Graglia, IEEE TAP 41 (1993) 1448-1455) and only (e^{ikR} - 1)/R is
+// sampled.
+
+// I = int_T dS'/|r - r'|,  Iv = int_T (r' - r)/|r - r'| dS'
+__device__ void triPotentials(const double r[3], const TriDev& T, double& I, double Iv[3]) {
+    double v[3][3];
+    for (int a = 0; a < 3; ++a) {
+        v[0][a] = T.X0[a];
+        v[1][a] = T.X0[a] + T.D0[a];
+        v[2][a] = T.X0[a] - T.D2[a];
+    }
+    const double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+    const double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int a = 0; a < 3; ++a) n[a] /= nn;
+    const double d = n[0] * (r[0] - v[0][0]) + n[1] * (r[1] - v[0][1]) + n[2] * (r[2] - v[0][2]);
+    const double ad = fabs(d);
+    const double rho[3] = {r[0] - d * n[0], r[1] - d * n[1], r[2] - d * n[2]};
+    double scale = 0;
+    I = 0;
+    double Ip[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        const double* a = v[i];
+        const double* b = v[(i + 1) % 3];
+        double l[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double len = sqrt(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+        scale = fmax(scale, len);
+        for (int c = 0; c < 3; ++c) l[c] /= len;
+        const double u[3] = {l[1] * n[2] - l[2] * n[1], l[2] * n[0] - l[0] * n[2], l[0] * n[1] - l[1] * n[0]};
+        const double lp = (b[0] - rho[0]) * l[0] + (b[1] - rho[1]) * l[1] + (b[2] - rho[2]) * l[2];
+        const double lm = (a[0] - rho[0]) * l[0] + (a[1] - rho[1]) * l[1] + (a[2] - rho[2]) * l[2];
+        const double P0 = (a[0] - rho[0]) * u[0] + (a[1] - rho[1]) * u[1] + (a[2] - rho[2]) * u[2];
+        const double R02 = P0 * P0 + d * d;
+        const double Rp = sqrt(lp * lp + R02), Rm = sqrt(lm * lm + R02);
+        // log((R+ + l+)/(R- + l-)); R + l = R0^2 / (R - l) for l < 0 avoids
+        // the cancellation near the edge's line
+        double lg = 0;
+        if (R02 > 1e-28 * scale * scale) {
+            const double ap = lp >= 0 ? Rp + lp : R02 / (Rp - lp);
+            const double am = lm >= 0 ? Rm + lm : R02 / (Rm - lm);
+            lg = log(ap / am);
+        } else if (lm > 0 || lp < 0)  // on the edge's line, outside the segment
+            lg = lm > 0 ? log(lp / lm) : log(lm / lp);
+        // (on the segment itself R0 = 0 multiplies the logarithm)
+        double at = 0;
+        if (fabs(P0) > 1e-14 * scale)
+            at = atan(P0 * lp / (R02 + ad * Rp)) - atan(P0 * lm / (R02 + ad * Rm));
+        I += P0 * lg - ad * at;
+        const double w = 0.5 * (R02 * lg + lp * Rp - lm * Rm);
+        for (int c = 0; c < 3; ++c) Ip[c] += w * u[c];
+    }
+    for (int c = 0; c < 3; ++c) Iv[c] = Ip[c] - d * n[c] * I;
+}
+
+__device__ cplx<double> entryPhys(const SrcDev& s, int64_t o, int64_t q, double k) {
+    const RwgDev& a = s.rwg[o];
+    const RwgDev& b = s.rwg[q];
+    const double k2 = 4.0 / (k * k);
+    cplx<double> acc = mk(0.0, 0.0);
+    for (int ot = 0; ot < 2; ++ot) {
+        const TriDev& TO = s.tri[a.tri[ot]];
+        for (int st = 0; st < 2; ++st) {
+            const TriDev& TS = s.tri[b.tri[st]];
+            const double sg = (ot ^ st) ? -1.0 : 1.0;
+            // closeness: centroid distance against the longer of the two triangles' edges
+            double co[3], cs[3], eo = 0, es = 0;
+            for (int c = 0; c < 3; ++c) {
+                co[c] = TO.X0[c] + (TO.D0[c] - TO.D2[c]) / 3.0;
+                cs[c] = TS.X0[c] + (TS.D0[c] - TS.D2[c]) / 3.0;
+            }
+            auto edge = [](const TriDev& T) {
+                const double e0 = T.D0[0] * T.D0[0] + T.D0[1] * T.D0[1] + T.D0[2] * T.D0[2];
+                const double e2 = T.D2[0] * T.D2[0] + T.D2[1] * T.D2[1] + T.D2[2] * T.D2[2];
+                const double e1x = T.D0[0] + T.D2[0], e1y = T.D0[1] + T.D2[1], e1z = T.D0[2] + T.D2[2];
+                return sqrt(fmax(fmax(e0, e2), e1x * e1x + e1y * e1y + e1z * e1z));
+            };
+            eo = edge(TO);
+            es = edge(TS);
+            const double dc = sqrt((co[0] - cs[0]) * (co[0] - cs[0]) + (co[1] - cs[1]) * (co[1] - cs[1]) +
+                                   (co[2] - cs[2]) * (co[2] - cs[2]));
+            const bool close = dc < 2.0 * fmax(eo, es);
+            double areaS = 0;
+            if (close) {
+                const double cx = TS.D0[1] * TS.D2[2] - TS.D0[2] * TS.D2[1];
+                const double cy = TS.D0[2] * TS.D2[0] - TS.D0[0] * TS.D2[2];
+                const double cz = TS.D0[0] * TS.D2[1] - TS.D0[1] * TS.D2[0];
+                areaS = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            }
+            cplx<double> part = mk(0.0, 0.0);
+            for (int i = 0; i < TO.nq; ++i) {
+                const double ro[3] = {TO.q[i][0], TO.q[i][1], TO.q[i][2]};
+                const double vo[3] = {ro[0] - a.xpm[ot][0], ro[1] - a.xpm[ot][1], ro[2] - a.xpm[ot][2]};
+                cplx<double> inner = mk(0.0, 0.0);
+                for (int j = 0; j < TS.nq; ++j) {
+                    const double rs[3] = {TS.q[j][0], TS.q[j][1], TS.q[j][2]};
+                    const double vs[3] = {rs[0] - b.xpm[st][0], rs[1] - b.xpm[st][1], rs[2] - b.xpm[st][2]};
+                    const double dx = ro[0] - rs[0], dy = ro[1] - rs[1], dz = ro[2] - rs[2];
+                    const double R = sqrt(dx * dx + dy * dy + dz * dz);
+                    const double h = vo[0] * vs[0] + vo[1] * vs[1] + vo[2] * vs[2] - k2;
+                    cplx<double> g;
+                    if (close) {  // (e^{ikR} - 1)/R, its R -> 0 limit ik
+                        if (R > 0) {
+                            const double sh = sin(0.5 * k * R);
+                            g = mk(-2.0 * sh * sh / R, sin(k * R) / R);
+                        } else
+                            g = mk(0.0, k);
+                    } else {
+                        const cplx<double> e = cexpi(k * R);
+                        g = mk(e.re / R, e.im / R);
+                    }
+                    inner = inner + (TS.q[j][3] * h) * g;
+                }
+                if (close) {  // int (rho_o . rho_s - 4/k^2)/R dS' / (2 A')
+                    double I, Iv[3];
+                    triPotentials(ro, TS, I, Iv);
+                    const double rx[3] = {ro[0] - b.xpm[st][0], ro[1] - b.xpm[st][1], ro[2] - b.xpm[st][2]};
+                    double sing = -k2 * I;
+                    for (int c = 0; c < 3; ++c) sing += vo[c] * (Iv[c] + rx[c] * I);
+                    inner = inner + mk(sing / (2.0 * areaS), 0.0);
+                }
+                part = part + TO.q[i][3] * inner;
+            }
+            acc = acc + sg * part;
+        }
+    }
+    return (a.leng * b.leng) * acc;
+}
+
 __device__ cplx<double> entry(const SrcDev& s, int64_t o, int64_t q, double k) {
+    if (s.physical) return entryPhys(s, o, q, k);
     if (s.kind == 0) {
         const DipoleDev& a = s.dip[o];
         const DipoleDev& b = s.dip[q];
@@ -175,7 +315,7 @@ __global__ void kNearBuild(SrcDev s, const NearItem* __restrict__ items, int nit
     const int64_t row = I.rbeg + r, col = cols[I.cptr + c];
     cplx<double> v;
     if (c < I.nself)
-        v = row == col ? mk(0.0, 0.0) : entry(s, row > col ? row : col, row > col ? col : row, k);
+        v = (row == col && !s.physical) ? mk(0.0, 0.0) : entry(s, row > col ? row : col, row > col ? col : row, k);
     else
         v = entry(s, row, col, k);
     val[e] = Ck * v;
@@ -604,7 +744,7 @@ __global__ void kDirect(SrcDev s, const cplx<double>* __restrict__ x, int64_t n,
     if (i >= n) return;
     cplx<double> acc = mk(0.0, 0.0);
     for (int64_t j = 0; j < n; ++j) {
-        if (j == i) continue;
+        if (j == i && !s.physical) continue;
         acc = acc + entry(s, i, j, k) * x[j];
     }
     y[i] = Ck * acc;

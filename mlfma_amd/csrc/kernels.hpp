@@ -63,6 +63,7 @@ struct TriDev {
 
 struct SrcDev {
     int kind;
+    int physical;  // RWG near entries in the physical (singular-integral) form
     const DipoleDev* dip;
     const RwgDev* rwg;
     const TriDev* tri;

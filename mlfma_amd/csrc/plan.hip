@@ -104,6 +104,7 @@ void DevicePlan::build() {
 
     // sources in tree order
     src.kind = h.src.kind;
+    src.physical = (flags & MLFMA_FLAG_PHYSICAL) ? 1 : 0;
     if (h.src.kind == MLFMA_SRC_DIPOLE) {
         std::vector<DipoleDev> d(n);
         for (int64_t q = 0; q < n; ++q) {
@@ -832,6 +833,8 @@ int mlfma_plan_create(const mlfma_geometry* geom, const mlfma_planewave* pw, con
     auto plan = std::make_unique<mlfma_plan>();
     plan->d = std::make_unique<DevicePlan>();
     DevicePlan& d = *plan->d;
+    if ((cfg->flags & MLFMA_FLAG_PHYSICAL) && geom->kind != MLFMA_SRC_RWG)
+        throw Error(MLFMA_ERR_ARG, "MLFMA_FLAG_PHYSICAL needs RWG sources");
     buildHostPlan(d.hp, *geom, *pw, *cfg);
     d.flags = cfg->flags;
     d.hostOnly = cfg->device < 0;
@@ -989,8 +992,10 @@ int mlfma_plan_set_flags(mlfma_plan* plan, uint32_t flags) {
     API_BEGIN
     DevicePlan& d = P(plan);
     if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES | MLFMA_FLAG_M2L_GROUP | MLFMA_FLAG_M2L_TARGET |
-                  MLFMA_FLAG_ASYNC))
+                  MLFMA_FLAG_ASYNC | MLFMA_FLAG_PHYSICAL))
         throw Error(MLFMA_ERR_ARG, "unknown flag bits");
+    if ((flags ^ d.flags) & MLFMA_FLAG_PHYSICAL)
+        throw Error(MLFMA_ERR_ARG, "MLFMA_FLAG_PHYSICAL is fixed when the plan is created");
     d.flags = flags;
     return MLFMA_OK;
     API_END
