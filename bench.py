@@ -18,7 +18,9 @@ The line also carries:
   cpu_baseline  the CPU oracle (FP64 restatement of the reference, one core)
                 timed on a bounded number of matvecs of the same workload;
   solve         GMRES(50) to 1e-6 in FP64 and the FP32-far-field refinement
-                solve, with iteration counts and true residuals.
+                solve (the reference's kernel), with iteration counts and true
+                residuals; and GMRES(50) with the physical near field plus the
+                bistatic RCS at 361 angles against the Mie series.
 """
 from __future__ import annotations
 
@@ -87,6 +89,39 @@ def near_bytes(info):
     # (FP64 complex, 16 B) is read once; the column indices, x gathers and
     # partial sums are a few percent on top and are not counted
     return info["near_nnz"] * 16
+
+
+def physical_solve(geom, pw, kw, radius=2.0):
+    """GMRES(50) to 1e-6 with the physical near field (closed-form singular
+    integrals) on the sphere, and its bistatic RCS at 361 angles (theta 0-180
+    degrees in 0.5-degree steps, phi = 0 and 90) against the Mie series."""
+    from mlfma_amd import _capi as A
+    from mlfma_amd.plan import Plan
+    from tests import _physical as P
+    t0 = time.time()
+    p = Plan(geom, pw, A.make_config(device=0, flags=A.MLFMA_FLAG_PHYSICAL, **kw))
+    setup = time.time() - t0
+    I, rep = p.solve(method=A.MLFMA_GMRES_RESTART, restart=50, maxit=3000, tol=1e-6)
+    # the same with the FP32 far field inside FP64 iterative refinement
+    p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, flags=A.MLFMA_FLAG_PHYSICAL, **kw))
+    I32, rep32 = p32.solve(method=A.MLFMA_FGMRES_IR, restart=50, maxit=3000, tol=1e-6)
+    mixed = {"method": "FP64 refinement, FP32 far field, MLFMA_FLAG_PHYSICAL", "iterations": rep32["iterations"],
+             "status": rep32["status"], "resid_true": rep32["resid_true"], "seconds": rep32["seconds"],
+             "solution_vs_fp64_rel": float(np.linalg.norm(I32 - I) / np.linalg.norm(I))}
+    th = np.radians(np.linspace(0.0, 180.0, 361))
+    se, sh = P.mie_pec_rcs(2 * np.pi * radius, th)
+    errs = []
+    for phi, ref in ((0.0, se), (np.pi / 2, sh)):
+        d = np.stack([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)], axis=1)
+        E = p.farfield(I, d)
+        errs.append(10 * np.log10(4 * np.pi * np.sum(np.abs(E) ** 2, axis=1)) - 10 * np.log10(ref))
+    errs = np.concatenate(errs)
+    return {"method": "GMRES(50), FP64, MLFMA_FLAG_PHYSICAL", "iterations": rep["iterations"],
+            "status": rep["status"], "resid_true": rep["resid_true"], "seconds": rep["seconds"],
+            "plan_setup_s": round(setup, 3),
+            "rcs_vs_mie_db": {"angles": 2 * len(th), "rms": round(float(np.sqrt(np.mean(errs ** 2))), 4),
+                              "max": round(float(np.abs(errs).max()), 4)},
+            "mixed": mixed}
 
 
 def cpu_baseline(geom, pw, kw, budget_s=20.0, y_gpu=None):
@@ -258,6 +293,8 @@ def main():
             p32.matvec(xx, yy)
         torch.cuda.synchronize()
         sol["fp32_farfield_matvec_per_s"] = round(args.steps / (time.perf_counter() - t1), 3)
+        if args.workload == "sphere30k":
+            sol["physical"] = physical_solve(geom, pw, kw)
         result["solve"] = sol
         log("solve", sol)
 
