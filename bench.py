@@ -210,6 +210,15 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
     value = args.steps / elapsed
+    # the same K matvecs replayed from a captured hipGraph
+    plan.set_flags(A.MLFMA_FLAG_ASYNC | A.MLFMA_FLAG_GRAPH)
+    plan.matvec(x, y)
+    torch.cuda.synchronize()
+    tg = time.perf_counter()
+    for _ in range(args.steps):
+        plan.matvec(x, y)
+    torch.cuda.synchronize()
+    value_graph = args.steps / (time.perf_counter() - tg)
     # the same with a completed result per call (the synchronous entry point)
     plan.set_flags(0)
     torch.cuda.synchronize()
@@ -234,7 +243,8 @@ def main():
     plan.set_flags(0)
     stage_ms = {k: round(v / args.steps, 4) for k, v in stages.items()}
     log("stage ms", stage_ms)
-    log("matvec/s", round(value, 2), "synchronous calls", round(value_sync, 2), "with stage events",
+    log("matvec/s", round(value, 2), "graph replay", round(value_graph, 2), "synchronous calls", round(value_sync, 2),
+        "with stage events",
         round(with_events, 2))
     # batched right-hand sides: 8 vectors per call share every operator read
     nb = 8
@@ -269,7 +279,8 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": desc, "roofline": roofline, "stage_ms": stage_ms, "setup_s": round(setup_s, 2),
-        "value_synchronous_calls": round(value_sync, 3), "value_with_stage_events": round(with_events, 3),
+        "value_graph_replay": round(value_graph, 3), "value_synchronous_calls": round(value_sync, 3),
+        "value_with_stage_events": round(with_events, 3),
         "batched": batched,
     }
 

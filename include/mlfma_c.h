@@ -88,6 +88,11 @@ typedef struct mlfma_planewave {
                                        in closed form, coincident triangles and
                                        the diagonal included (the reference drops
                                        them, src/sources/rwg.cpp:97,113-116) */
+#define MLFMA_FLAG_GRAPH 64u        /* capture a matvec's launch sequence into a
+                                       hipGraph once per (x, y, nrhs,
+                                       accumulate, precision) and replay it
+                                       (up to 4 cached; cleared when the flags
+                                       change; not with stage times) */
 #define MLFMA_FLAG_ASYNC 16u        /* matvecs on device pointers return once
                                        queued on the stream instead of after
                                        completion (order by the stream; host

@@ -47,10 +47,16 @@ void DevicePlan::release(void* p) {
     }
 }
 
+void DevicePlan::clearGraphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
 DevicePlan::~DevicePlan() {
     if (hostOnly) return;
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    clearGraphs();
     for (void* p : allocs) (void)hipFree(p);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
@@ -750,7 +756,8 @@ void DevicePlan::applyOp(const cplx<double>* x, cplx<double>* y, int accumulate,
 // Grow the field arrays and near-field partials to hold nb right-hand sides.
 void DevicePlan::ensureBatch(int nb) {
     if (nb <= batchCap) return;
-    hipCheck(hipStreamSynchronize(stream), "batch resize");
+    hipCheck(hipDeviceSynchronize(), "batch resize");
+    clearGraphs();  // captured graphs hold the old field addresses
     const int64_t len = std::max<int64_t>(fieldLen, 1) * nb;
     auto regrow = [&](auto*& ptr, size_t elem) {
         using T = std::remove_reference_t<decltype(*ptr)>;
@@ -776,6 +783,52 @@ void DevicePlan::matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int ac
     const int64_t n = hp.src.n;
     const bool timed = (flags & MLFMA_FLAG_STAGE_TIMES) != 0;
     const bool linear = hp.staleBoxes == 0 || (flags & MLFMA_FLAG_FRESH_LOCALS);
+    // MLFMA_FLAG_GRAPH: replay a captured launch sequence for these buffers.
+    // The capture runs on the plan's own stream (torch's default stream cannot
+    // be captured) and the graph is launched on st.  The graph holds kernel
+    // launches only (no memset or host work), its buffers are sized first.
+    if ((flags & MLFMA_FLAG_GRAPH) && !timed && linear) {
+        for (size_t i = 0; i < graphs.size(); ++i) {
+            GraphEntry& g = graphs[i];
+            if (g.x == x && g.y == y && g.nrhs == nrhs && g.accumulate == accumulate && g.fp32 == fp32) {
+                hipCheck(hipGraphLaunch(g.exec, st), "hipGraphLaunch");
+                std::rotate(graphs.begin() + i, graphs.begin() + i + 1, graphs.end());
+                return;
+            }
+        }
+        ensureBatch(nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1)));
+        // the capture stream must not start before st's earlier work
+        hipCheck(hipEventRecord(ev[8], st), "event");
+        hipCheck(hipStreamWaitEvent(stream, ev[8], 0), "wait");
+        hipCheck(hipStreamSynchronize(stream), "capture");
+        hipGraph_t graph = nullptr;
+        hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            int r = 0;
+            while (r < nrhs) {
+                const int left = nrhs - r;
+                const int nb = left >= 8 ? 8 : (left >= 4 ? 4 : (left >= 2 ? 2 : 1));
+                applyOp(x + (size_t)r * n, y + (size_t)r * n, accumulate, stream, fp32, false, nb, n);
+                r += nb;
+            }
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        hipCheck(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hipCheck(e, "hipGraphInstantiate");
+        if (graphs.size() >= 4) {
+            (void)hipGraphExecDestroy(graphs.front().exec);
+            graphs.erase(graphs.begin());
+        }
+        graphs.push_back(GraphEntry{x, y, nrhs, accumulate, fp32, exec});
+        hipCheck(hipGraphLaunch(exec, st), "hipGraphLaunch");
+        return;
+    }
     int r = 0;
     while (r < nrhs) {
         const int left = nrhs - r;
@@ -992,10 +1045,15 @@ int mlfma_plan_set_flags(mlfma_plan* plan, uint32_t flags) {
     API_BEGIN
     DevicePlan& d = P(plan);
     if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES | MLFMA_FLAG_M2L_GROUP | MLFMA_FLAG_M2L_TARGET |
-                  MLFMA_FLAG_ASYNC | MLFMA_FLAG_PHYSICAL))
+                  MLFMA_FLAG_ASYNC | MLFMA_FLAG_PHYSICAL | MLFMA_FLAG_GRAPH))
         throw Error(MLFMA_ERR_ARG, "unknown flag bits");
     if ((flags ^ d.flags) & MLFMA_FLAG_PHYSICAL)
         throw Error(MLFMA_ERR_ARG, "MLFMA_FLAG_PHYSICAL is fixed when the plan is created");
+    if (!d.hostOnly) {
+        hipCheck(hipSetDevice(d.device), "hipSetDevice");
+        hipCheck(hipStreamSynchronize(d.stream), "set flags");
+        d.clearGraphs();  // captured launch sequences depend on the flags
+    }
     d.flags = flags;
     return MLFMA_OK;
     API_END

@@ -132,6 +132,16 @@ struct DevicePlan {
     void apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed, int nb = 1,
                int64_t xs = 0);
     int batchCap = 1;  // right-hand sides the field arrays and partials hold
+    // captured matvecs (MLFMA_FLAG_GRAPH), least recently used last
+    struct GraphEntry {
+        const void* x;
+        void* y;
+        int nrhs, accumulate;
+        bool fp32;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    void clearGraphs();
     template <typename R>
     void launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb);
     void ensureBatch(int nb);
