@@ -157,6 +157,12 @@ int mlfma_plan_info(const mlfma_plan* plan, mlfma_info* out);
  * (src/solver.cpp:31-32). */
 int mlfma_voltage(const mlfma_plan* plan, double* v_host);
 
+/* The excitation V of the same sources under another plane wave of the
+ * plan's wavenumber (buildVoltage, src/sources/dipole.h:29-32 and
+ * src/sources/rwg.h:18-21, for the given incidence): n complex (host, input
+ * order).  For sweeps of incidence angles; the solve's right-hand side is -V. */
+int mlfma_excitation(mlfma_plan* plan, const mlfma_planewave* pw, double* v_host);
+
 #define MLFMA_PTR_HOST 0
 #define MLFMA_PTR_DEVICE 1
 
@@ -216,6 +222,15 @@ typedef struct mlfma_solve_report {
  * residual estimate. */
 int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_opts* opts,
                 mlfma_solve_report* report, double* resid_hist);
+
+/* GMRES(m) on nrhs right-hand sides at once (e.g. one scatterer, several
+ * incidence angles): one batched matvec per step over all of them (no
+ * reference counterpart).  b, x: nrhs x n complex (host, column stride n);
+ * opts->method must be MLFMA_GMRES_RESTART and opts->use_voltage is ignored;
+ * reports: nrhs entries (or NULL); resid_hist: nrhs x maxit (or NULL).
+ * Non-convergence is reported per right-hand side in reports[r].status. */
+int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs,
+                      const mlfma_solve_opts* opts, mlfma_solve_report* reports, double* resid_hist);
 
 /* --------------------------------------------------------------- far field */
 

@@ -84,6 +84,7 @@ SIGNATURES = {
     "mlfma_plan_destroy": (C.c_int, [C.c_void_p]),
     "mlfma_plan_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
     "mlfma_voltage": (C.c_int, [C.c_void_p, dptr]),
+    "mlfma_excitation": (C.c_int, [C.c_void_p, C.c_void_p, dptr]),
     "mlfma_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                C.c_int32, C.c_void_p]),
     "mlfma_matvec_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
@@ -91,6 +92,7 @@ SIGNATURES = {
     "mlfma_matvec_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "mlfma_solve": (C.c_int, [C.c_void_p, dptr, dptr, C.POINTER(SolveOpts), C.POINTER(SolveReport),
                               dptr]),
+    "mlfma_solve_batch": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int32, C.c_void_p, C.c_void_p, dptr]),
     "mlfma_farfield": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
     "mlfma_farfield_adjoint": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
     "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),

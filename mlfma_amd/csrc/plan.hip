@@ -946,6 +946,33 @@ int mlfma_voltage(const mlfma_plan* plan, double* v) {
     API_END
 }
 
+int mlfma_excitation(mlfma_plan* plan, const mlfma_planewave* pw, double* v) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    needDevice(d);
+    if (!pw || !v) throw Error(MLFMA_ERR_ARG, "null argument");
+    const double kn = std::sqrt(pw->khat[0] * pw->khat[0] + pw->khat[1] * pw->khat[1] + pw->khat[2] * pw->khat[2]);
+    if (!(kn > 0) || std::abs(pw->wavenum - d.hp.pw.wavenum) > 1e-12 * d.hp.pw.wavenum)
+        throw Error(MLFMA_ERR_ARG, "the plane wave needs a direction and the plan's wavenumber");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const int64_t n = d.hp.src.n;
+    d.ensureRhs(1);
+    const double wv[3] = {pw->wavenum * pw->khat[0] / kn, pw->wavenum * pw->khat[1] / kn,
+                          pw->wavenum * pw->khat[2] / kn};
+    launchVoltage(d.src, n, wv, pw->pol, pw->amplitude, d.yt, d.stream);
+    hipCheck(hipGetLastError(), "voltage launch");
+    std::vector<cplx<double>> vt(n);
+    hipCheck(hipMemcpyAsync(vt.data(), d.yt, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, d.stream), "V copy");
+    hipCheck(hipStreamSynchronize(d.stream), "excitation");
+    for (int64_t q = 0; q < n; ++q) {  // tree order -> input order
+        const int64_t i = d.hp.tree.perm[q];
+        v[2 * i] = vt[q].re;
+        v[2 * i + 1] = vt[q].im;
+    }
+    return MLFMA_OK;
+    API_END
+}
+
 int mlfma_matvec(mlfma_plan* plan, const double* x, double* y, int32_t nrhs, int32_t accumulate, int32_t ptr_mode,
                  void* stream) {
     API_BEGIN

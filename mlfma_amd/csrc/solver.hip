@@ -6,6 +6,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "plan.hpp"
@@ -173,14 +174,35 @@ struct Krylov {
         for (int i = 0; i < ncol; ++i) hout[i] = std::complex<double>(hh[i].re + hh[ncol + 1 + i].re, hh[i].im + hh[ncol + 1 + i].im);
         wnorm = std::sqrt(hh[ncol].re);
     }
+    // the two CGS passes of orthogonalize() queued without a host sync, for
+    // columns Q + i ld; readOrtho() after a stream sync returns h and |w|
+    void queueOrtho(const cplx<double>* Q, int64_t ld, int ncol, cplx<double>* w, cplx<double>* hhost) {
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL(kDotPartial, dim3(nb), dim3(KB), 0, st, Q, ld, ncol, (const cplx<double>*)w, n, partial);
+            hipLaunchKernelGGL(kReduce, dim3(ncol), dim3(KB), 0, st, partial, nb, h + (pass ? ncol + 1 : 0), 0);
+            hipLaunchKernelGGL(kUpdate, dim3(nb), dim3(KB), 0, st, Q, ld, ncol,
+                               (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), w, n,
+                               partial + (size_t)nb * (ncol + 1), pass);
+        }
+        hipLaunchKernelGGL(kReduce, dim3(1), dim3(KB), 0, st, partial + (size_t)nb * (ncol + 1), nb, h + ncol, 0);
+        hipCheck(hipMemcpyAsync(hhost, h, (2 * ncol + 1) * sizeof(cplx<double>), hipMemcpyDeviceToHost, st),
+                 "h copy");
+    }
+    static void readOrtho(const cplx<double>* hh, int ncol, std::vector<std::complex<double>>& hout, double& wnorm) {
+        hout.resize(ncol);
+        for (int i = 0; i < ncol; ++i)
+            hout[i] = std::complex<double>(hh[i].re + hh[ncol + 1 + i].re, hh[i].im + hh[ncol + 1 + i].im);
+        wnorm = std::sqrt(hh[ncol].re);
+    }
     void scale(const cplx<double>* in, double s, cplx<double>* out) {
         hipLaunchKernelGGL(kScale, dim3(nb), dim3(KB), 0, st, in, s, out, n);
     }
-    void combine(const cplx<double>* Q, int ncol, const std::vector<std::complex<double>>& y, cplx<double>* x, int acc) {
+    void combine(const cplx<double>* Q, int ncol, const std::vector<std::complex<double>>& y, cplx<double>* x, int acc,
+                 int64_t ld = 0) {
         std::vector<cplx<double>> yy(ncol);
         for (int i = 0; i < ncol; ++i) yy[i] = cplx<double>{y[i].real(), y[i].imag()};
         hipCheck(hipMemcpyAsync(h, yy.data(), ncol * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "y upload");
-        hipLaunchKernelGGL(kCombine, dim3(nb), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)h, x, n, acc);
+        hipLaunchKernelGGL(kCombine, dim3(nb), dim3(KB), 0, st, Q, ld ? ld : n, ncol, (const cplx<double>*)h, x, n, acc);
         hipCheck(hipStreamSynchronize(st), "combine");
     }
     void sub(const cplx<double>* b, const cplx<double>* a, cplx<double>* r) {
@@ -356,6 +378,141 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
     return out;
 }
 
+// y = Z x for nrhs column vectors (stride n) through the batched kernels.
+static void applyMany(DevicePlan& d, const cplx<double>* x, cplx<double>* y, int nrhs, hipStream_t st) {
+    const int64_t n = d.hp.src.n;
+    int r = 0;
+    while (r < nrhs) {
+        const int left = nrhs - r;
+        const int nb = left >= 8 ? 8 : (left >= 4 ? 4 : (left >= 2 ? 2 : 1));
+        d.applyOp(x + (size_t)r * n, y + (size_t)r * n, 0, st, false, false, nb, n);
+        r += nb;
+    }
+}
+
+// GMRES(m) on nrhs right-hand sides in lockstep: one batched matvec per step
+// over the block of the right-hand sides' current Krylov vectors (basis
+// stored [step][rhs][n]), per right-hand side classical Gram-Schmidt twice
+// and unitary Givens rotations as in gmresRestart, one host sync per step
+// for all of them.  A right-hand side that has converged stops updating
+// (its lane of the block is zero) until the others finish the cycle.
+static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
+                                        const cplx<double>* B, cplx<double>* X, int nrhs, int m, int maxit,
+                                        double tol, double* hist) {
+    const int64_t n = d.hp.src.n;
+    m = std::max(1, std::min(m, maxit));
+    const int64_t ld = (int64_t)nrhs * n;
+    cplx<double>* Q = (cplx<double>*)d.alloc((size_t)ld * (m + 1) * sizeof(cplx<double>));
+    cplx<double>* W = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
+    std::vector<cplx<double>> hh((size_t)nrhs * (2 * m + 3));
+    std::vector<SolveOut> out(nrhs);
+    std::vector<double> bn(nrhs);
+    std::vector<char> done(nrhs, 0);
+    for (int r = 0; r < nrhs; ++r) {
+        bn[r] = K[r]->norm(B + (size_t)r * n);
+        if (!std::isfinite(bn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
+        if (bn[r] == 0) done[r] = 1;
+    }
+    hipCheck(hipMemsetAsync(X, 0, (size_t)ld * sizeof(cplx<double>), st), "zero x");
+    int total = 0;
+    for (;;) {
+        // residuals r = b - A x
+        if (total == 0)
+            hipCheck(hipMemcpyAsync(W, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
+        else {
+            applyMany(d, X, W, nrhs, st);
+            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + (size_t)r * n, W + (size_t)r * n, W + (size_t)r * n);
+        }
+        std::vector<double> beta(nrhs, 0.0);
+        bool all = true;
+        for (int r = 0; r < nrhs; ++r) {
+            if (bn[r] == 0) continue;
+            beta[r] = total == 0 ? bn[r] : K[r]->norm(W + (size_t)r * n);
+            out[r].est = beta[r] / bn[r];
+            done[r] = out[r].est <= tol;
+            all = all && done[r];
+        }
+        if (all || total >= maxit) break;
+        std::vector<std::vector<std::vector<zc>>> H(nrhs);
+        std::vector<std::vector<zc>> g(nrhs, std::vector<zc>(m + 1, 0.0)), cs(nrhs, std::vector<zc>(m)),
+            sn(nrhs, std::vector<zc>(m));
+        std::vector<int> steps(nrhs, 0);
+        std::vector<char> active(nrhs, 0);
+        for (int r = 0; r < nrhs; ++r) {
+            active[r] = !done[r];
+            if (active[r]) {
+                K[r]->scale(W + (size_t)r * n, beta[r], Q + (size_t)r * n);
+                g[r][0] = beta[r];
+            } else
+                hipCheck(hipMemsetAsync(Q + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+        }
+        bool breakdown = false;
+        for (int j = 0; j < m && total < maxit; ++j) {
+            applyMany(d, Q + (size_t)j * ld, W, nrhs, st);
+            for (int r = 0; r < nrhs; ++r)
+                if (active[r])
+                    K[r]->queueOrtho(Q + (size_t)r * n, ld, j + 1, W + (size_t)r * n, hh.data() + (size_t)r * (2 * m + 3));
+            hipCheck(hipStreamSynchronize(st), "arnoldi");
+            bool any = false;
+            for (int r = 0; r < nrhs; ++r) {
+                cplx<double>* qn = Q + (size_t)(j + 1) * ld + (size_t)r * n;
+                if (!active[r]) {
+                    hipCheck(hipMemsetAsync(qn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                    continue;
+                }
+                std::vector<zc> h;
+                double hn;
+                Krylov::readOrtho(hh.data() + (size_t)r * (2 * m + 3), j + 1, h, hn);
+                h.push_back(hn);
+                for (int i = 0; i < j; ++i) {
+                    const zc a = std::conj(cs[r][i]) * h[i] + std::conj(sn[r][i]) * h[i + 1];
+                    h[i + 1] = -sn[r][i] * h[i] + cs[r][i] * h[i + 1];
+                    h[i] = a;
+                }
+                const double rho = std::sqrt(std::norm(h[j]) + std::norm(h[j + 1]));
+                if (rho == 0) {
+                    out[r].status = MLFMA_ERR_BREAKDOWN;
+                    active[r] = 0;
+                    breakdown = true;
+                    hipCheck(hipMemsetAsync(qn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                    continue;
+                }
+                cs[r][j] = h[j] / rho;
+                sn[r][j] = h[j + 1] / rho;
+                h[j] = rho;
+                h[j + 1] = 0.0;
+                H[r].push_back(h);
+                g[r][j + 1] = -sn[r][j] * g[r][j];
+                g[r][j] = std::conj(cs[r][j]) * g[r][j];
+                steps[r] = j + 1;
+                const double est = std::abs(g[r][j + 1]) / bn[r];
+                if (hist) hist[(size_t)r * maxit + total] = est;
+                ++out[r].iters;
+                if (est <= tol || hn == 0) {
+                    active[r] = 0;
+                    hipCheck(hipMemsetAsync(qn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                } else {
+                    K[r]->scale(W + (size_t)r * n, hn, qn);
+                    any = true;
+                }
+            }
+            ++total;
+            if (!any) break;
+        }
+        for (int r = 0; r < nrhs; ++r)
+            if (steps[r] > 0) {
+                const std::vector<zc> y = backSolve(H[r], g[r], steps[r]);
+                K[r]->combine(Q + (size_t)r * n, steps[r], y, X + (size_t)r * n, 1, ld);
+            }
+        if (breakdown) break;
+    }
+    for (int r = 0; r < nrhs; ++r)
+        if (out[r].status == MLFMA_OK && out[r].est > tol) out[r].status = MLFMA_ERR_NOT_CONVERGED;
+    d.release(Q);
+    d.release(W);
+    return out;
+}
+
 // Mixed precision: FP64 iterative refinement (true residual with the FP64
 // matvec) around inner GMRES(m) solves whose far field is FP32.
 static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit, double tol,
@@ -517,6 +674,58 @@ int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_
     d.release(dd);
     d.release(dE);
     return MLFMA_OK;
+    API_END
+}
+
+int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs, const mlfma_solve_opts* opts,
+                      mlfma_solve_report* reps, double* hist) {
+    API_BEGIN
+    DevicePlan& d = Pl(plan);
+    if (!b || !x || !opts || nrhs < 1) throw Error(MLFMA_ERR_ARG, "bad batch solve arguments");
+    if (opts->method != MLFMA_GMRES_RESTART) throw Error(MLFMA_ERR_ARG, "batched solves use MLFMA_GMRES_RESTART");
+    if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
+    if (d.hp.staleBoxes > 0 && !(d.flags & MLFMA_FLAG_FRESH_LOCALS))
+        throw Error(MLFMA_ERR_STATE, "batched solves need a linear operator (MLFMA_FLAG_FRESH_LOCALS)");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t n = d.hp.src.n;
+    hipStream_t st = d.stream;
+    d.ensureBatch(nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1)));
+    const size_t bytes = (size_t)n * nrhs * sizeof(cplx<double>);
+    for (size_t i = 0; i < (size_t)n * nrhs * 2; ++i)
+        if (!std::isfinite(b[i])) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
+    cplx<double>* B = (cplx<double>*)d.alloc(bytes);
+    cplx<double>* X = (cplx<double>*)d.alloc(bytes);
+    cplx<double>* R = (cplx<double>*)d.alloc(bytes);
+    hipCheck(hipMemcpyAsync(B, b, bytes, hipMemcpyHostToDevice, st), "b upload");
+    const int m = opts->restart > 0 ? opts->restart : 50;
+    std::vector<std::unique_ptr<Krylov>> K;
+    for (int r = 0; r < nrhs; ++r) K.emplace_back(new Krylov(d, st, std::min(m, opts->maxit) + 2));
+    std::vector<SolveOut> o = gmresBatch(d, K, st, B, X, nrhs, m, opts->maxit, opts->tol, hist);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // true residuals with the FP64 matvec
+    applyMany(d, X, R, nrhs, st);
+    for (int r = 0; r < nrhs; ++r) {
+        K[r]->sub(B + (size_t)r * n, R + (size_t)r * n, R + (size_t)r * n);
+        const double rn = K[r]->norm(R + (size_t)r * n), bn = K[r]->norm(B + (size_t)r * n);
+        if (reps) {
+            reps[r].iterations = o[r].iters;
+            reps[r].status = o[r].status;
+            reps[r].resid_est = o[r].est;
+            reps[r].resid_true = bn > 0 ? rn / bn : 0.0;
+            reps[r].seconds = secs;
+        }
+    }
+    hipCheck(hipMemcpyAsync(x, X, bytes, hipMemcpyDeviceToHost, st), "x download");
+    hipCheck(hipStreamSynchronize(st), "batch solve");
+    K.clear();
+    d.release(B);
+    d.release(X);
+    d.release(R);
+    int worst = MLFMA_OK;
+    for (auto& e : o)
+        if (e.status != MLFMA_OK) worst = e.status;
+    return worst == MLFMA_ERR_NOT_CONVERGED ? MLFMA_OK : worst;
     API_END
 }
 

@@ -161,6 +161,40 @@ class Plan:
             report["history"] = hist[: rep.iterations].copy()
         return x, report
 
+    def excitation(self, pw) -> np.ndarray:
+        """V of the sources under another plane wave (same wavenumber)."""
+        v = np.zeros(self.n, dtype=np.complex128)
+        _check(self.lib, self.lib.mlfma_excitation(self._h, C.byref(pw), v.ctypes.data_as(A.dptr)))
+        return v
+
+    def solve_batch(self, B, restart=50, maxit=500, tol=1e-6, history=False):
+        """GMRES(restart) on the rows of B (nrhs, n) at once, one batched
+        matvec per step.  Returns (X, [report per right-hand side])."""
+        B = np.ascontiguousarray(B, dtype=np.complex128)
+        if B.ndim != 2 or B.shape[1] != self.n:
+            raise ValueError("B must be (nrhs, n)")
+        nrhs = B.shape[0]
+        o = A.SolveOpts()
+        o.method = A.MLFMA_GMRES_RESTART
+        o.restart = restart
+        o.maxit = maxit
+        o.use_voltage = 0
+        o.tol = tol
+        reps = (A.SolveReport * nrhs)()
+        X = np.zeros_like(B)
+        hist = np.zeros((nrhs, maxit))
+        _check(self.lib, self.lib.mlfma_solve_batch(self._h, B.ctypes.data_as(A.dptr), X.ctypes.data_as(A.dptr),
+                                                    nrhs, C.byref(o), reps, hist.ctypes.data_as(A.dptr)))
+        out = []
+        for r in range(nrhs):
+            rep = reps[r]
+            d = {"iterations": rep.iterations, "status": A.ERRORS.get(rep.status, rep.status),
+                 "resid_est": rep.resid_est, "resid_true": rep.resid_true, "seconds": rep.seconds}
+            if history:
+                d["history"] = hist[r, : rep.iterations].copy()
+            out.append(d)
+        return X, out
+
     # ---------------------------------------------------------------- far field
     def farfield(self, I, dirs):
         I = _cvec(I, self.n)

@@ -62,3 +62,32 @@ def test_zero_rhs_and_errors():
         p.solve(np.full(geom.n, np.nan + 0j), method=A.MLFMA_GMRES_RESTART)
     with pytest.raises(Exception):
         p.solve(method=A.MLFMA_FGMRES_IR)   # needs an FP32 plan
+
+
+def test_batched_solve_matches_single_and_dense():
+    geom = _small_problem()
+    kw = dict(root_leng=2.0, max_node_srcs=2, max_level=-1, interp_order=3, digits=5,
+              flags=A.MLFMA_FLAG_FRESH_LOCALS)
+    pw = A.make_planewave(1.0, (0.0, 0.0, 1.0), (1.0, 0.0, 0.0), 2 * np.pi)
+    p = Plan(geom, pw, A.make_config(device=0, **kw))
+    n = geom.n
+    Z = np.stack([p.matvec(np.eye(n)[j].astype(complex)) for j in range(n)], 1)
+    B = np.stack([G.complex_gaussian(n, 40 + r) for r in range(5)])
+    B[3] = 0.0                                           # a zero right-hand side in the batch
+    X, reps = p.solve_batch(B, restart=n + 2, maxit=3 * n, tol=1e-12)
+    for r in range(5):
+        if r == 3:
+            assert np.all(X[r] == 0)
+            continue
+        assert reps[r]["status"] == "OK" and reps[r]["resid_true"] <= 1e-11, (r, reps[r])
+        assert cases.relerr(X[r], np.linalg.solve(Z, B[r])) <= 1e-8
+        xs, rs = p.solve(B[r], method=A.MLFMA_GMRES_RESTART, restart=n + 2, maxit=3 * n, tol=1e-12)
+        assert cases.relerr(X[r], xs) <= 1e-9
+    # restarted (small m) batch on a larger problem: every column converges
+    geom2, kw2 = cases.make("rwg_fixed")
+    q = Plan(geom2, cases.planewave_z(), A.make_config(device=0, flags=A.MLFMA_FLAG_PHYSICAL, **kw2))
+    B2 = np.stack([G.complex_gaussian(geom2.n, 60 + r) for r in range(8)])
+    X2, reps2 = q.solve_batch(B2, restart=30, maxit=2000, tol=1e-8)
+    for r in range(8):
+        y = q.matvec(X2[r])
+        assert cases.relerr(y, B2[r]) <= 2e-8, (r, reps2[r])
