@@ -1,0 +1,27 @@
+"""Several scatterers at once.
+
+Each plan owns its HIP stream, and the library releases the GIL during its
+calls, so batched solves of independent scatterers issued from host threads
+run concurrently on the device: small scatterers that cannot fill the GPU
+alone share it.  Every scatterer's right-hand sides go through one batched
+solve (``Plan.solve_batch``: one multi-RHS matvec per GMRES step).
+"""
+from __future__ import annotations
+
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from .plan import Plan
+
+
+def solve_many(plans: list[Plan], rhs: list[np.ndarray], restart: int = 50, maxit: int = 500, tol: float = 1e-6,
+               workers: int | None = None):
+    """Solve plans[i] X_i = rhs[i] (rhs[i]: (nrhs_i, n_i)) for all i, the
+    scatterers concurrently.  Returns [(X_i, reports_i)] in input order."""
+    if len(plans) != len(rhs):
+        raise ValueError("one right-hand-side block per plan")
+    workers = workers or min(16, len(plans))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = [ex.submit(p.solve_batch, np.asarray(b), restart, maxit, tol) for p, b in zip(plans, rhs)]
+        return [f.result() for f in futs]
