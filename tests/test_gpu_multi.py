@@ -23,7 +23,7 @@ def scatterers():
         rw = G.rwg_edges(f)
         geom = A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=rw)
         kw = dict(root_leng=4.0 * r + 0.5, max_node_srcs=0, max_level=2, interp_order=4, digits=3,
-                  flags=A.MLFMA_FLAG_PHYSICAL)
+                  flags=A.MLFMA_FLAG_PHYSICAL | A.MLFMA_FLAG_FRESH_LOCALS)
         p = Plan(geom, A.make_planewave(1.0, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2 * np.pi),
                  A.make_config(device=0, **kw))
         ang = np.radians(np.arange(3) * 60.0)
