@@ -58,6 +58,18 @@ def workload(name):
                 "levels": "0-4", "interp_order": 6, "digits": 3, "over_interp": 4.0, "quad_points": 7,
                 "excitation": "plane wave theta=0 x-pol"}
         return geom, pw, kw, desc
+    if name == "sphere200k":
+        v, f = G.icosphere(82, 4.0)
+        r = G.rwg_edges(f)
+        geom = A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=r)
+        kw = dict(root_leng=8.0, max_node_srcs=0, max_level=5, interp_order=6, digits=3, over_interp=4.0,
+                  quad_prec=2)
+        pw = A.make_planewave(1.0, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2 * np.pi)
+        desc = {"workload": "rwg_sphere_R4_icosphere82", "unknowns": int(len(r)), "triangles": int(len(f)),
+                "mean_edge_wavelengths": round(G.mesh_stats(v, f, r)["mean_edge"], 4), "leaf_wavelengths": 0.25,
+                "levels": "0-5", "interp_order": 6, "digits": 3, "over_interp": 4.0, "quad_points": 7,
+                "excitation": "plane wave theta=0 x-pol"}
+        return geom, pw, kw, desc
     if name == "dipoles4k":
         pos, mom = G.uniform_dipoles(4096, 4.0, 0)
         geom = A.GeometryArrays(A.MLFMA_SRC_DIPOLE, pos=pos, mom=mom)
@@ -102,9 +114,11 @@ def physical_solve(geom, pw, kw, radius=2.0):
     p = Plan(geom, pw, A.make_config(device=0, flags=A.MLFMA_FLAG_PHYSICAL, **kw))
     setup = time.time() - t0
     I, rep = p.solve(method=A.MLFMA_GMRES_RESTART, restart=50, maxit=3000, tol=1e-6)
+    log("physical GMRES", rep["iterations"], rep["status"], round(rep["seconds"], 3), "s")
     # the same with the FP32 far field inside FP64 iterative refinement
     p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, flags=A.MLFMA_FLAG_PHYSICAL, **kw))
     I32, rep32 = p32.solve(method=A.MLFMA_FGMRES_IR, restart=50, maxit=3000, tol=1e-6)
+    log("physical FP32-IR", rep32["iterations"], rep32["status"], round(rep32["seconds"], 3), "s")
     mixed = {"method": "FP64 refinement, FP32 far field, MLFMA_FLAG_PHYSICAL", "iterations": rep32["iterations"],
              "status": rep32["status"], "resid_true": rep32["resid_true"], "seconds": rep32["seconds"],
              "solution_vs_fp64_rel": float(np.linalg.norm(I32 - I) / np.linalg.norm(I))}
@@ -116,8 +130,12 @@ def physical_solve(geom, pw, kw, radius=2.0):
     tb = time.perf_counter()
     Xb, repb = p.solve_batch(B, restart=50, maxit=3000, tol=1e-6)
     t_batch = time.perf_counter() - tb
+    log("physical batched sweep", round(t_batch, 3), "s")
     ts = time.perf_counter()
-    singles = [p.solve(B[j], method=A.MLFMA_GMRES_RESTART, restart=50, maxit=3000, tol=1e-6) for j in range(8)]
+    singles = []
+    for j in range(8):
+        singles.append(p.solve(B[j], method=A.MLFMA_GMRES_RESTART, restart=50, maxit=3000, tol=1e-6))
+        log("physical single", j, singles[-1][1]["iterations"])
     t_single = time.perf_counter() - ts
     sweep = {"angles": 8, "batched_seconds": round(t_batch, 4), "sequential_seconds": round(t_single, 4),
              "iterations": [r["iterations"] for r in repb], "all_converged": all(r["status"] == "OK" for r in repb),
@@ -320,8 +338,8 @@ def main():
             p32.matvec(xx, yy)
         torch.cuda.synchronize()
         sol["fp32_farfield_matvec_per_s"] = round(args.steps / (time.perf_counter() - t1), 3)
-        if args.workload == "sphere30k":
-            sol["physical"] = physical_solve(geom, pw, kw)
+        if args.workload in ("sphere30k", "sphere200k"):
+            sol["physical"] = physical_solve(geom, pw, kw, radius=2.0 if args.workload == "sphere30k" else 4.0)
         result["solve"] = sol
         log("solve", sol)
 
