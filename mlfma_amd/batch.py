@@ -16,12 +16,13 @@ from .plan import Plan
 
 
 def solve_many(plans: list[Plan], rhs: list[np.ndarray], restart: int = 50, maxit: int = 500, tol: float = 1e-6,
-               workers: int | None = None):
+               workers: int | None = None, method: int | None = None):
     """Solve plans[i] X_i = rhs[i] (rhs[i]: (nrhs_i, n_i)) for all i, the
     scatterers concurrently.  Returns [(X_i, reports_i)] in input order."""
     if len(plans) != len(rhs):
         raise ValueError("one right-hand-side block per plan")
     workers = workers or min(16, len(plans))
     with ThreadPoolExecutor(max_workers=workers) as ex:
-        futs = [ex.submit(p.solve_batch, np.asarray(b), restart, maxit, tol) for p, b in zip(plans, rhs)]
+        kw = {} if method is None else {"method": method}
+        futs = [ex.submit(p.solve_batch, np.asarray(b), restart, maxit, tol, False, **kw) for p, b in zip(plans, rhs)]
         return [f.result() for f in futs]

@@ -378,14 +378,16 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
     return out;
 }
 
-// y = Z x for nrhs column vectors (stride n) through the batched kernels.
-static void applyMany(DevicePlan& d, const cplx<double>* x, cplx<double>* y, int nrhs, hipStream_t st) {
+// y = Z x for nrhs column vectors (stride n) through the batched kernels
+// (fp32: the FP32 far field).
+static void applyMany(DevicePlan& d, const cplx<double>* x, cplx<double>* y, int nrhs, hipStream_t st,
+                      bool fp32 = false) {
     const int64_t n = d.hp.src.n;
     int r = 0;
     while (r < nrhs) {
         const int left = nrhs - r;
         const int nb = left >= 8 ? 8 : (left >= 4 ? 4 : (left >= 2 ? 2 : 1));
-        d.applyOp(x + (size_t)r * n, y + (size_t)r * n, 0, st, false, false, nb, n);
+        d.applyOp(x + (size_t)r * n, y + (size_t)r * n, 0, st, fp32, false, nb, n);
         r += nb;
     }
 }
@@ -398,7 +400,9 @@ static void applyMany(DevicePlan& d, const cplx<double>* x, cplx<double>* y, int
 // (its lane of the block is zero) until the others finish the cycle.
 static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                         const cplx<double>* B, cplx<double>* X, int nrhs, int m, int maxit,
-                                        double tol, double* hist) {
+                                        const std::vector<double>& tolv, double* hist, bool fp32 = false,
+                                        int histOff = 0, int histLd = 0) {
+    if (!histLd) histLd = maxit;
     const int64_t n = d.hp.src.n;
     m = std::max(1, std::min(m, maxit));
     const int64_t ld = (int64_t)nrhs * n;
@@ -420,7 +424,7 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         if (total == 0)
             hipCheck(hipMemcpyAsync(W, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
         else {
-            applyMany(d, X, W, nrhs, st);
+            applyMany(d, X, W, nrhs, st, fp32);
             for (int r = 0; r < nrhs; ++r) K[r]->sub(B + (size_t)r * n, W + (size_t)r * n, W + (size_t)r * n);
         }
         std::vector<double> beta(nrhs, 0.0);
@@ -429,7 +433,7 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
             if (bn[r] == 0) continue;
             beta[r] = total == 0 ? bn[r] : K[r]->norm(W + (size_t)r * n);
             out[r].est = beta[r] / bn[r];
-            done[r] = out[r].est <= tol;
+            done[r] = out[r].est <= tolv[r];
             all = all && done[r];
         }
         if (all || total >= maxit) break;
@@ -448,7 +452,7 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         }
         bool breakdown = false;
         for (int j = 0; j < m && total < maxit; ++j) {
-            applyMany(d, Q + (size_t)j * ld, W, nrhs, st);
+            applyMany(d, Q + (size_t)j * ld, W, nrhs, st, fp32);
             for (int r = 0; r < nrhs; ++r)
                 if (active[r])
                     K[r]->queueOrtho(Q + (size_t)r * n, ld, j + 1, W + (size_t)r * n, hh.data() + (size_t)r * (2 * m + 3));
@@ -486,9 +490,9 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
                 g[r][j] = std::conj(cs[r][j]) * g[r][j];
                 steps[r] = j + 1;
                 const double est = std::abs(g[r][j + 1]) / bn[r];
-                if (hist) hist[(size_t)r * maxit + total] = est;
+                if (hist) hist[(size_t)r * histLd + histOff + total] = est;
                 ++out[r].iters;
-                if (est <= tol || hn == 0) {
+                if (est <= tolv[r] || hn == 0) {
                     active[r] = 0;
                     hipCheck(hipMemsetAsync(qn, 0, n * sizeof(cplx<double>), st), "zero lane");
                 } else {
@@ -507,9 +511,63 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         if (breakdown) break;
     }
     for (int r = 0; r < nrhs; ++r)
-        if (out[r].status == MLFMA_OK && out[r].est > tol) out[r].status = MLFMA_ERR_NOT_CONVERGED;
+        if (out[r].status == MLFMA_OK && out[r].est > tolv[r]) out[r].status = MLFMA_ERR_NOT_CONVERGED;
     d.release(Q);
     d.release(W);
+    return out;
+}
+
+// Batched mixed precision: FP64 residuals of all right-hand sides (batched
+// FP64 matvec) around batched inner GMRES(m) solves with the FP32 far field;
+// each right-hand side's inner tolerance is max(1e-4, tol / (2 residual)).
+static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
+                                          const cplx<double>* B, cplx<double>* X, int nrhs, int m, int maxit,
+                                          double tol, double* hist) {
+    const int64_t n = d.hp.src.n;
+    const int64_t ld = (int64_t)nrhs * n;
+    cplx<double>* R = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
+    cplx<double>* D = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
+    std::vector<SolveOut> out(nrhs);
+    std::vector<double> bn(nrhs);
+    for (int r = 0; r < nrhs; ++r) bn[r] = K[r]->norm(B + (size_t)r * n);
+    hipCheck(hipMemsetAsync(X, 0, (size_t)ld * sizeof(cplx<double>), st), "zero x");
+    int total = 0;
+    for (int outer = 0; outer < 30; ++outer) {
+        if (outer == 0)
+            hipCheck(hipMemcpyAsync(R, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
+        else {
+            applyMany(d, X, R, nrhs, st, false);
+            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + (size_t)r * n, R + (size_t)r * n, R + (size_t)r * n);
+        }
+        bool all = true;
+        std::vector<double> itol(nrhs, 1.0);
+        for (int r = 0; r < nrhs; ++r) {
+            const double res = bn[r] > 0 ? K[r]->norm(R + (size_t)r * n) / bn[r] : 0.0;
+            out[r].est = res;
+            if (res <= tol) {  // converged: its inner right-hand side is zero
+                hipCheck(hipMemsetAsync(R + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+                continue;
+            }
+            all = false;
+            itol[r] = std::max(1e-4, 0.5 * tol / res);
+        }
+        if (all || total >= maxit) break;
+        std::vector<SolveOut> in = gmresBatch(d, K, st, R, D, nrhs, m, maxit - total, itol, hist, true, total, maxit);
+        int steps = 0;
+        for (int r = 0; r < nrhs; ++r) {
+            out[r].iters += in[r].iters;
+            steps = std::max(steps, in[r].iters);
+            if (in[r].status == MLFMA_ERR_BREAKDOWN) out[r].status = MLFMA_ERR_BREAKDOWN;
+            // X += D
+            std::vector<zc> one{1.0};
+            K[r]->combine(D + (size_t)r * n, 1, one, X + (size_t)r * n, 1);
+        }
+        total += std::max(steps, 1);
+    }
+    for (int r = 0; r < nrhs; ++r)
+        if (out[r].status == MLFMA_OK && out[r].est > tol) out[r].status = MLFMA_ERR_NOT_CONVERGED;
+    d.release(R);
+    d.release(D);
     return out;
 }
 
@@ -682,7 +740,9 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     API_BEGIN
     DevicePlan& d = Pl(plan);
     if (!b || !x || !opts || nrhs < 1) throw Error(MLFMA_ERR_ARG, "bad batch solve arguments");
-    if (opts->method != MLFMA_GMRES_RESTART) throw Error(MLFMA_ERR_ARG, "batched solves use MLFMA_GMRES_RESTART");
+    if (opts->method != MLFMA_GMRES_RESTART && opts->method != MLFMA_FGMRES_IR)
+        throw Error(MLFMA_ERR_ARG, "batched solves use MLFMA_GMRES_RESTART or MLFMA_FGMRES_IR");
+    if (opts->method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
     if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
     if (d.hp.staleBoxes > 0 && !(d.flags & MLFMA_FLAG_FRESH_LOCALS))
         throw Error(MLFMA_ERR_STATE, "batched solves need a linear operator (MLFMA_FLAG_FRESH_LOCALS)");
@@ -701,7 +761,10 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     const int m = opts->restart > 0 ? opts->restart : 50;
     std::vector<std::unique_ptr<Krylov>> K;
     for (int r = 0; r < nrhs; ++r) K.emplace_back(new Krylov(d, st, std::min(m, opts->maxit) + 2));
-    std::vector<SolveOut> o = gmresBatch(d, K, st, B, X, nrhs, m, opts->maxit, opts->tol, hist);
+    std::vector<SolveOut> o =
+        opts->method == MLFMA_FGMRES_IR
+            ? gmresBatchIR(d, K, st, B, X, nrhs, m, opts->maxit, opts->tol, hist)
+            : gmresBatch(d, K, st, B, X, nrhs, m, opts->maxit, std::vector<double>(nrhs, opts->tol), hist);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // true residuals with the FP64 matvec
     applyMany(d, X, R, nrhs, st);

@@ -167,15 +167,17 @@ class Plan:
         _check(self.lib, self.lib.mlfma_excitation(self._h, C.byref(pw), v.ctypes.data_as(A.dptr)))
         return v
 
-    def solve_batch(self, B, restart=50, maxit=500, tol=1e-6, history=False):
+    def solve_batch(self, B, restart=50, maxit=500, tol=1e-6, history=False, method=A.MLFMA_GMRES_RESTART):
         """GMRES(restart) on the rows of B (nrhs, n) at once, one batched
-        matvec per step.  Returns (X, [report per right-hand side])."""
+        matvec per step (method MLFMA_FGMRES_IR on an FP32 plan: FP64
+        refinement around batched FP32-far-field GMRES).  Returns (X,
+        [report per right-hand side])."""
         B = np.ascontiguousarray(B, dtype=np.complex128)
         if B.ndim != 2 or B.shape[1] != self.n:
             raise ValueError("B must be (nrhs, n)")
         nrhs = B.shape[0]
         o = A.SolveOpts()
-        o.method = A.MLFMA_GMRES_RESTART
+        o.method = method
         o.restart = restart
         o.maxit = maxit
         o.use_voltage = 0

@@ -83,6 +83,13 @@ def test_batched_solve_matches_single_and_dense():
         assert cases.relerr(X[r], np.linalg.solve(Z, B[r])) <= 1e-8
         xs, rs = p.solve(B[r], method=A.MLFMA_GMRES_RESTART, restart=n + 2, maxit=3 * n, tol=1e-12)
         assert cases.relerr(X[r], xs) <= 1e-9
+    # batched FP64 refinement around FP32-far-field inner solves
+    p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, **kw))
+    X32, reps32 = p32.solve_batch(B, restart=n + 2, maxit=6 * n, tol=1e-9, method=A.MLFMA_FGMRES_IR)
+    for r in (0, 1, 2, 4):
+        assert reps32[r]["status"] == "OK" and reps32[r]["resid_true"] <= 1e-9, (r, reps32[r])
+        assert cases.relerr(X32[r], np.linalg.solve(Z, B[r])) <= 1e-6
+    assert np.all(X32[3] == 0)
     # restarted (small m) batch on a larger problem: every column converges
     geom2, kw2 = cases.make("rwg_fixed")
     q = Plan(geom2, cases.planewave_z(), A.make_config(device=0, flags=A.MLFMA_FLAG_PHYSICAL, **kw2))
