@@ -252,6 +252,9 @@ def main():
     y = torch.empty_like(x)
     stages = {k: 0.0 for k in ["S2M", "M2M", "M2L", "L2L", "L2T", "near", "total"]}
     near_pairs_ms = 0.0
+    # the library's autotuner picks M2M / M2L launch shapes for this plan
+    tune_before, tune_after = plan.autotune(1)
+    log("autotune ms", round(tune_before, 4), "->", round(tune_after, 4))
     for _ in range(args.warmup):
         plan.matvec(x, y)
     # timed region: K matvecs, uninstrumented, queued on torch's stream (the
@@ -305,6 +308,7 @@ def main():
     nb = 8
     Xb = torch.from_numpy(np.stack([G.complex_gaussian(n, 50 + r) for r in range(nb)])).to(dev)
     Yb = torch.empty_like(Xb)
+    tb_before, tb_after = plan.autotune(nb)
     plan.set_flags(A.MLFMA_FLAG_ASYNC)
     for _ in range(2):
         plan.matvec(Xb, Yb)
@@ -316,7 +320,8 @@ def main():
     torch.cuda.synchronize()
     el_b = time.perf_counter() - t2
     plan.set_flags(0)
-    batched = {"nrhs": nb, "matvec_per_s": round(nb * calls / el_b, 3), "ms_per_call": round(1000 * el_b / calls, 4)}
+    batched = {"nrhs": nb, "matvec_per_s": round(nb * calls / el_b, 3), "ms_per_call": round(1000 * el_b / calls, 4),
+               "autotune_ms": {"before": round(tb_before, 4), "after": round(tb_after, 4)}}
     log("batched", batched)
 
     # roofline of the near-field pair kernel (one launch per matvec; HIP events
@@ -334,6 +339,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": desc, "roofline": roofline, "stage_ms": stage_ms, "setup_s": round(setup_s, 2),
+        "autotune_ms": {"before": round(tune_before, 4), "after": round(tune_after, 4)},
         "value_graph_replay": round(value_graph, 3), "value_synchronous_calls": round(value_sync, 3),
         "value_with_stage_events": round(with_events, 3),
         "batched": batched,

@@ -261,6 +261,14 @@ int mlfma_stage_times(const mlfma_plan* plan, double* ms7);
  * *count = 7.  No reference counterpart (finer than ClockTimes). */
 int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, int32_t* count);
 
+/* Choose the plan's launch parameters for batches of nrhs (1, 2, 4 or 8)
+ * right-hand sides by timing candidates with device events: the M2M LDS
+ * budget and block target, and per level the sibling-group or per-target
+ * M2L kernel.  Mean matvec device time before and after (ms, may be NULL).
+ * Results do not depend on the choice beyond round-off.  No reference
+ * counterpart. */
+int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms_after);
+
 /* Replace the plan's MLFMA_FLAG_* set for later matvecs and solves (the flags
  * act at apply time; the plan itself does not change).  No reference
  * counterpart: the reference fixes them at compile time. */

@@ -97,6 +97,7 @@ SIGNATURES = {
     "mlfma_farfield_adjoint": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
     "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),
     "mlfma_plan_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mlfma_autotune": (C.c_int, [C.c_void_p, C.c_int32, dptr, dptr]),
     "mlfma_stage_times_detail": (C.c_int, [C.c_void_p, dptr, C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_level_boxes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), iptr, iptr, iptr,
                                     C.c_int64]),

@@ -199,19 +199,21 @@ template <typename R>
 bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
                     int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
                     const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
-                    size_t ldsMax, hipStream_t st, int nb, int64_t fs) {
+                    size_t ldsMax, hipStream_t st, int nb, int64_t fs, int ldsBudget, int minBlocks) {
     if (nstem <= 0) return true;
+    if (ldsBudget <= 0) ldsBudget = MLFMA_M2M_LDS_BUDGET;
+    if (minBlocks <= 0) minBlocks = MLFMA_M2M_MIN_BLOCKS;
     // split the parent's theta rows so that there are enough blocks for the
     // device (each block re-reads the children's fields from L2)
     int rows = nthP;
-    while (rows > 2 && (int64_t)nstem * ((nthP + rows - 1) / rows) < MLFMA_M2M_MIN_BLOCKS) rows = (rows + 1) / 2;
+    while (rows > 2 && (int64_t)nstem * ((nthP + rows - 1) / rows) < minBlocks) rows = (rows + 1) / 2;
     while (rows > 1 && rows * mphC > 16 * TB) rows = (rows + 1) / 2;  // at most 16 outputs per thread
     const int tp = 2 * p;
     const size_t fixed = sizeof(cplx<R>) * 2 * (size_t)rows * mphC + sizeof(R) * ((size_t)rows + nphP) * tp +
                         sizeof(int) * ((size_t)rows + nphP);
     const size_t perChild = sizeof(cplx<R>) * 2 * (size_t)mthC * mphC;
     // children per batch: as many as fit the LDS budget (several blocks per CU)
-    const size_t budget = std::min<size_t>(ldsMax, MLFMA_M2M_LDS_BUDGET);
+    const size_t budget = std::min<size_t>(ldsMax, (size_t)ldsBudget);
     int cb = fixed < budget ? (int)std::min<size_t>(8, (budget - fixed) / perChild) : 0;
     if (cb < 1) {
         if (fixed + perChild > ldsMax) return false;
@@ -618,10 +620,11 @@ template <typename R>
 bool launchL2LAdj(int npar, const int32_t* parSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
                   int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
                   const int32_t* phNear, const double* phW, const cplx<R>* phaseDn, const cplx<R>* Lc, cplx<R>* Lp,
-                  size_t ldsMax, hipStream_t st) {
+                  size_t ldsMax, hipStream_t st, int minBlocks) {
     if (npar <= 0) return true;
+    if (minBlocks <= 0) minBlocks = MLFMA_M2M_MIN_BLOCKS;
     int rows = nthP;
-    while (rows > 2 && (int64_t)npar * ((nthP + rows - 1) / rows) < MLFMA_M2M_MIN_BLOCKS) rows = (rows + 1) / 2;
+    while (rows > 2 && (int64_t)npar * ((nthP + rows - 1) / rows) < minBlocks) rows = (rows + 1) / 2;
     while (rows > 1 && rows * nphP > 16 * TB) rows = (rows + 1) / 2;  // at most 16 outputs per thread
     const int tp = 2 * p;
     const size_t lds = sizeof(cplx<R>) * 2 * ((size_t)mthC * mphC + (size_t)rows * mphC) +
@@ -1018,7 +1021,8 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
                                 cplx<R>*, hipStream_t, int, int64_t, int64_t);                                     \
     template bool launchM2MFused<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                     int, int, const int32_t*, const double*, const int32_t*, const double*,         \
-                                    const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t, int, int64_t);   \
+                                    const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t, int, int64_t,    \
+                                    int, int);                                                                      \
     template void launchM2LGroup<R>(M2LGroups<R>, bool, hipStream_t, int);                                          \
     template void launchM2Lx<R>(M2LLevels<R>, bool, hipStream_t, int);                                              \
     template bool launchL2LFused<R>(bool, int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,   \
@@ -1030,7 +1034,7 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
                                 const int64_t*, cplx<double>*, int, hipStream_t, int, int64_t, int64_t, int64_t);   \
     template bool launchL2LAdj<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                   int, int, const int32_t*, const double*, const int32_t*, const double*,           \
-                                  const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t);                   \
+                                  const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t, int);              \
     template void launchZeroField<R>(cplx<R>*, int64_t, hipStream_t);
 INST(double)
 INST(float)

@@ -594,6 +594,7 @@ void DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStr
         if (L.ntgt == 0) continue;
         const int64_t nch = (L.ndir + 255) / 256;
         bool group = L.nPairs >= 3 * L.nGrpEntries && (int64_t)L.ngrp * nch * nb >= 2048;
+        if (l < (int)tuneM2L.size() && tuneM2L[l]) group = tuneM2L[l] == 1;  // autotuned choice
         if (flags & MLFMA_FLAG_M2L_GROUP) group = true;
         if (flags & MLFMA_FLAG_M2L_TARGET) group = false;
         if (group) {
@@ -644,7 +645,8 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         LevelBufs& P = levels[l];
         LevelBufs& C = levels[l + 1];
         if (!launchM2MFused<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph, p, P.thNear,
-                               P.thW, P.phNear, P.phW, C.phUp<R>(), M + C.Moff, M + P.Moff, ldsMax, st, nb, fs)) {
+                               P.thW, P.phNear, P.phW, C.phUp<R>(), M + C.Moff, M + P.Moff, ldsMax, st, nb, fs,
+                               tuneM2MBudget, tuneMinBlocks)) {
             for (int r = 0; r < nb; ++r) {
                 launchM2MTheta<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, C.nth, C.nph, p, P.thNear,
                                   P.thW, C.phUp<R>(), M + r * fs + C.Moff, inner, st);
@@ -711,7 +713,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
         LevelBufs& C = levels[l];
         if (!launchL2LAdj<R>(C.nl2lPar, C.l2lParSlot, C.l2lParPtr, C.l2lParChild, C.l2lParOct, P.nth, P.nph, C.nth,
                              C.nph, p, P.thNear, P.thW, P.phNear, P.phW, P.phDn<R>(), Lc + C.Moff, Lc + P.Moff, ldsMax,
-                             st))
+                             st, tuneMinBlocks))
             throw Error(MLFMA_ERR_ARG, "adjoint L2L: angular grid too large for one block");
     }
     {
@@ -1048,6 +1050,83 @@ int mlfma_stage_times(const mlfma_plan* plan, double* ms) {
     ms[4] = el(6, 7);  // L2T
     ms[5] = el(4, 6);  // near field (pair products + fold)
     ms[6] = el(0, 7);  // total (first right-hand side)
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms_after) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    needDevice(d);
+    if (nrhs != 1 && nrhs != 2 && nrhs != 4 && nrhs != 8) throw Error(MLFMA_ERR_ARG, "nrhs must be 1, 2, 4 or 8");
+    const bool linear = d.hp.staleBoxes == 0 || (d.flags & MLFMA_FLAG_FRESH_LOCALS);
+    if (nrhs > 1 && !linear)
+        throw Error(MLFMA_ERR_STATE, "batched tuning needs a linear operator (MLFMA_FLAG_FRESH_LOCALS)");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const uint32_t saved = d.flags;
+    hipCheck(hipStreamSynchronize(d.stream), "autotune");
+    d.clearGraphs();
+    d.flags = (saved & ~(MLFMA_FLAG_GRAPH | MLFMA_FLAG_ASYNC | MLFMA_FLAG_M2L_GROUP | MLFMA_FLAG_M2L_TARGET)) |
+              MLFMA_FLAG_STAGE_TIMES;
+    const int64_t n = d.hp.src.n;
+    d.ensureRhs(nrhs);
+    if (nrhs > 1) d.ensureBatch(nrhs);
+    hipCheck(hipMemset(d.xt, 0, (size_t)n * nrhs * sizeof(cplx<double>)), "x");
+    // mean device time of events a -> b over reps matvecs after one warmup
+    auto timeStage = [&](int a, int b, int reps) {
+        d.matvec(d.xt, d.yt, nrhs, 0, d.stream, false);
+        double acc = 0;
+        for (int r = 0; r < reps; ++r) {
+            d.matvec(d.xt, d.yt, nrhs, 0, d.stream, false);
+            hipCheck(hipStreamSynchronize(d.stream), "autotune");
+            float t = 0;
+            hipCheck(hipEventElapsedTime(&t, d.ev[a], d.ev[b]), "hipEventElapsedTime");
+            acc += t;
+        }
+        return acc / reps;
+    };
+    try {
+        const double before = timeStage(0, 7, 5);
+        // M2M: LDS budget for staged children x block target for the row split
+        double best = 1e300;
+        int bb = 0, bm = 0;
+        for (int budget : {16384, 32768, 65536})
+            for (int mb : {256, 512, 1024, 2048, 4096}) {
+                d.tuneM2MBudget = budget;
+                d.tuneMinBlocks = mb;
+                const double t = timeStage(1, 2, 3);
+                if (t < best) {
+                    best = t;
+                    bb = budget;
+                    bm = mb;
+                }
+            }
+        d.tuneM2MBudget = bb;
+        d.tuneMinBlocks = bm;
+        // M2L: sibling-group or per-target kernel, level by level
+        d.tuneM2L.assign(d.hp.tree.maxLevel + 1, 0);
+        for (int l = 2; l <= d.hp.tree.maxLevel; ++l) {
+            if (d.levels[l].ntgt == 0) continue;
+            double tb = 1e300;
+            int mode = 0;
+            for (int m : {1, 2}) {
+                d.tuneM2L[l] = m;
+                const double t = timeStage(2, 3, 3);
+                if (t < tb) {
+                    tb = t;
+                    mode = m;
+                }
+            }
+            d.tuneM2L[l] = mode;
+        }
+        const double after = timeStage(0, 7, 5);
+        if (ms_before) *ms_before = before;
+        if (ms_after) *ms_after = after;
+    } catch (...) {
+        d.flags = saved;
+        throw;
+    }
+    d.flags = saved;
     return MLFMA_OK;
     API_END
 }

@@ -132,6 +132,10 @@ struct DevicePlan {
     void apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed, int nb = 1,
                int64_t xs = 0);
     int batchCap = 1;  // right-hand sides the field arrays and partials hold
+    // launch parameters chosen by mlfma_autotune (0: compile-time defaults /
+    // the per-level rule); tuneM2L[l]: 1 sibling-group, 2 per-target M2L
+    int tuneM2MBudget = 0, tuneMinBlocks = 0;
+    std::vector<int> tuneM2L;
     // captured matvecs (MLFMA_FLAG_GRAPH), least recently used last
     struct GraphEntry {
         const void* x;

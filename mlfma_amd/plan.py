@@ -123,6 +123,13 @@ class Plan:
         """Replace the MLFMA_FLAG_* set used by later matvecs and solves."""
         _check(self.lib, self.lib.mlfma_plan_set_flags(self._h, int(flags)))
 
+    def autotune(self, nrhs: int = 1):
+        """Time candidate launch parameters for batches of nrhs right-hand
+        sides and keep the fastest; returns (ms before, ms after)."""
+        a, b = C.c_double(0), C.c_double(0)
+        _check(self.lib, self.lib.mlfma_autotune(self._h, nrhs, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def stage_times(self) -> dict:
         t = np.zeros(7)
         _check(self.lib, self.lib.mlfma_stage_times(self._h, t.ctypes.data_as(A.dptr)))

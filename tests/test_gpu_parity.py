@@ -340,3 +340,20 @@ def test_excitation_other_incidence():
         e = cases.relerr(p.excitation(pw2), ref.voltage())
         say(name, "oblique excitation", e, "tolerance", tol)
         assert e <= tol
+
+
+def test_autotune_keeps_results():
+    for name in ("rwg_adaptive", "dipole_fixed"):
+        geom, kw = cases.make(name)
+        p = Plan(geom, cases.planewave_z(), A.make_config(device=0, flags=A.MLFMA_FLAG_FRESH_LOCALS, **kw))
+        x = G.complex_gaussian(geom.n, 14)
+        X = np.stack([G.complex_gaussian(geom.n, 140 + r) for r in range(4)])
+        y0, Y0 = p.matvec(x), p.matvec(X)
+        for nrhs in (1, 4):
+            before, after = p.autotune(nrhs)
+            say(name, "autotune", nrhs, before, after)
+            assert before > 0 and after > 0
+        assert cases.relerr(p.matvec(x), y0) <= 1e-13
+        assert cases.relerr(p.matvec(X), Y0) <= 1e-13
+        with pytest.raises(Exception):
+            p.autotune(3)
