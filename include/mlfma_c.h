@@ -245,6 +245,18 @@ int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_
  * order).  No reference counterpart: the gradient of a far-field objective. */
 int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs, int64_t ndir, double* I);
 
+/* Position sensitivity of the operator for dipole sources:
+ * grad[i][a] = -2 Re(lambda^H (dZ/dr_i^a) I) for complex n-vectors I and
+ * lambda (host, input order); grad: n x 3 (host).  With I = Z^-1 b and
+ * lambda = Z^-H F^H E this is the part of the gradient of
+ * J = sum_d |E(rhat_d)|^2 that flows through the solve; the far-field term
+ * 2 Re(E^H dF I) comes from mlfma_farfield_adjoint.  The tree is held
+ * fixed (a derivative, not a re-partition).  No reference counterpart: the
+ * reference computes no derivatives.  Requires a linear operator
+ * (MLFMA_FLAG_FRESH_LOCALS when boxes keep stale locals); not defined for
+ * RWG plans. */
+int mlfma_position_gradient(mlfma_plan* plan, const double* I, const double* lambda, double* grad);
+
 /* --------------------------------------------------------------- introspection */
 
 /* Per-stage device time of the last matvec run with MLFMA_FLAG_STAGE_TIMES in

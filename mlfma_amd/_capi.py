@@ -95,6 +95,7 @@ SIGNATURES = {
     "mlfma_solve_batch": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int32, C.c_void_p, C.c_void_p, dptr]),
     "mlfma_farfield": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
     "mlfma_farfield_adjoint": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
+    "mlfma_position_gradient": (C.c_int, [C.c_void_p, dptr, dptr, dptr]),
     "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),
     "mlfma_plan_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mlfma_autotune": (C.c_int, [C.c_void_p, C.c_int32, dptr, dptr]),

@@ -9,9 +9,18 @@ tensors, the same one ``torch.matmul`` follows):
   farfield  E = F I           grad_I = F^H grad_E      (mlfma_farfield_adjoint)
 
 Chained, they give the gradient of a far-field objective with respect to the
-excitation, e.g. J = sum_d |E_far(rhat_d)|^2 with E_far = F Z^-1 b.  The
-plan's geometry is fixed: gradients with respect to source positions are not
-provided.
+excitation, e.g. J = sum_d |E_far(rhat_d)|^2 with E_far = F Z^-1 b.
+
+For dipole plans ``objective_gradients`` also gives dJ/dr, the gradient with
+respect to the source positions (adjoint method, tree held fixed):
+with I = Z^-1 b, E = F I and lambda = Z^-H F^H E,
+
+  dJ = 2 Re(E^H dF I) - 2 Re(lambda^H dZ I) + 2 Re(lambda^H db).
+
+The far-field pattern of dipole i carries e^{-ik rhat.r_i}, so the first term
+is 2 Re(-ik I_i conj(u_i^a)) with u^a = F^H (rhat^a E); the second comes from
+mlfma_position_gradient; the third is present when b is the plan's own
+plane-wave excitation (b_i proportional to e^{i kvec.r_i}).
 """
 from __future__ import annotations
 
@@ -84,3 +93,47 @@ def farfield(plan: Plan, I: torch.Tensor, dirs) -> torch.Tensor:
     """E = F I at the unit directions dirs (ndir x 3), differentiable in I."""
     dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
     return _Farfield.apply(I, plan, dirs)
+
+
+def position_gradient(plan: Plan, I, lam, E, dirs, b=None, wavevector=None) -> np.ndarray:
+    """dJ/dr (n x 3) of J = sum_d |E(rhat_d)|^2 for a dipole plan from the
+    forward solution I = Z^-1 b, the far fields E = F I at dirs and the
+    adjoint solution lam = Z^-H F^H E.  With wavevector (3,) given, b is the
+    plane-wave excitation b_i = c_i e^{i wavevector.r_i} and its derivative is
+    included."""
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    E = np.asarray(E, dtype=np.complex128).reshape(len(dirs), 3)
+    I = np.asarray(I, dtype=np.complex128)
+    lam = np.asarray(lam, dtype=np.complex128)
+    k = plan.pw.wavenum
+    g = plan.position_gradient(I, lam)
+    for a in range(3):
+        u = plan.farfield_adjoint(dirs[:, a:a + 1] * E, dirs)
+        g[:, a] += 2.0 * np.real(-1j * k * I * np.conj(u))
+    if wavevector is not None:
+        b = np.asarray(b, dtype=np.complex128)
+        w = np.asarray(wavevector, dtype=np.float64)
+        g += 2.0 * np.real((np.conj(lam) * 1j * b)[:, None] * w[None, :])
+    return g
+
+
+def objective_gradients(plan: Plan, dirs, b=None, restart: int = 50, maxit: int = 500, tol: float = 1e-10):
+    """J = sum_d |E_far(rhat_d)|^2 with E_far = F Z^-1 b (b = None: the plan's
+    plane-wave excitation) and its gradients: returns (J, grad_b, grad_r)
+    with dJ = 2 Re(grad_b^H db) + grad_r . dr (grad_r: n x 3, dipole plans).
+    Two GMRES solves (forward and adjoint) with the given options."""
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    opts = dict(method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=tol)
+    wave = None
+    if b is None:
+        b = -plan.voltage()
+        wave = plan.pw.wavenum * np.asarray(plan.pw.khat[:3], dtype=np.float64)
+    I, rep = plan.solve(b, **opts)
+    if rep["status"] != "OK":
+        raise RuntimeError(f"forward solve did not converge: {rep}")
+    E = plan.farfield(I, dirs)
+    lam, rep = plan.solve(plan.farfield_adjoint(E, dirs), adjoint=True, **opts)
+    if rep["status"] != "OK":
+        raise RuntimeError(f"adjoint solve did not converge: {rep}")
+    J = float(np.sum(np.abs(E) ** 2))
+    return J, lam, position_gradient(plan, I, lam, E, dirs, b, wave)

@@ -117,6 +117,74 @@ __device__ __forceinline__ cplx<double> greenContract(const double ro[3], const 
     return (ir)*(e * c);
 }
 
+// Gradient of greenContract with respect to ro (minus that for rs):
+// E = A (vo.vs) - B (vo.R^)(vs.R^), A = e^{ikr}(1/r - 1/(k^2 r^3) + i/(k r^2)),
+// B = e^{ikr}(1/r - 3/(k^2 r^3) + 3i/(k r^2)), R = ro - rs, so
+// grad E = A' R^ (vo.vs) - B' R^ (vo.R^)(vs.R^)
+//          - B [(vo - (vo.R^)R^)(vs.R^) + (vo.R^)(vs - (vs.R^)R^)] / r.
+__device__ void greenGrad(const double ro[3], const double rs[3], const double vo[3], const double vs[3], double k,
+                          cplx<double> g[3]) {
+    const double X[3] = {ro[0] - rs[0], ro[1] - rs[1], ro[2] - rs[2]};
+    const double r = sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+    const double ir = 1.0 / r, ir2 = ir * ir, ir3 = ir2 * ir, ir4 = ir2 * ir2;
+    const double ik2 = 1.0 / (k * k), ikk = 1.0 / k;
+    const cplx<double> e = cexpi(k * r);
+    const double h[3] = {X[0] * ir, X[1] * ir, X[2] * ir};
+    const cplx<double> A = e * mk(ir - ik2 * ir3, ikk * ir2);
+    const cplx<double> B = e * mk(ir - 3.0 * ik2 * ir3, 3.0 * ikk * ir2);
+    const cplx<double> Ad = mk(0.0, k) * A + e * mk(-ir2 + 3.0 * ik2 * ir4, -2.0 * ikk * ir3);
+    const cplx<double> Bd = mk(0.0, k) * B + e * mk(-ir2 + 9.0 * ik2 * ir4, -6.0 * ikk * ir3);
+    const double vovs = vo[0] * vs[0] + vo[1] * vs[1] + vo[2] * vs[2];
+    const double voh = vo[0] * h[0] + vo[1] * h[1] + vo[2] * h[2];
+    const double vsh = vs[0] * h[0] + vs[1] * h[1] + vs[2] * h[2];
+    for (int a = 0; a < 3; ++a) {
+        const double t = ((vo[a] - voh * h[a]) * vsh + voh * (vs[a] - vsh * h[a])) * ir;
+        g[a] = (h[a] * vovs) * Ad - (h[a] * voh * vsh) * Bd - t * B;
+    }
+}
+
+// Near-field part of the dipole position gradient, -2 Re(lambda^H dZ I), one
+// block per near item; for an entry (a row, b column) with w = conj(l_a) I_b
+// (+ conj(l_b) I_a for pair columns, applied both ways) the row source gains
+// -2 Re(C k grad E . w) and the column source the opposite (E depends on
+// r_a - r_b).  Accumulated with FP64 atomics into grad (input order, n x 3);
+// colsIn are the items' columns in input order, iperm maps input order to tree
+// positions.
+__global__ void kNearGradDip(SrcDev s, const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
+                             const int64_t* __restrict__ perm, const int64_t* __restrict__ iperm,
+                             const cplx<double>* __restrict__ I,
+                             const cplx<double>* __restrict__ lam, double k, cplx<double> Ck, double* __restrict__ grad) {
+    const NearItem it = items[blockIdx.x];
+    const int64_t ne = (int64_t)it.nr * it.nc;
+    for (int64_t e = threadIdx.x; e < ne; e += BS) {
+        const int64_t c = e / it.nr, r = e - c * it.nr;
+        const int64_t ib = colsIn[it.cptr + c];
+        const int64_t a = it.rbeg + r, b = iperm[ib];
+        if (a == b) continue;
+        const DipoleDev& da = s.dip[a];
+        const DipoleDev& db = s.dip[b];
+        if (da.pos[0] == db.pos[0] && da.pos[1] == db.pos[1] && da.pos[2] == db.pos[2]) continue;
+        const int64_t ia = perm[a];
+        cplx<double> w = mk(lam[ia].re, -lam[ia].im) * I[ib];
+        if (c >= it.nself) w = w + mk(lam[ib].re, -lam[ib].im) * I[ia];
+        cplx<double> g[3];
+        greenGrad(da.pos, db.pos, da.phat, db.phat, k, g);
+        for (int q = 0; q < 3; ++q) {
+            const double v = -2.0 * ((Ck * g[q]) * w).re;
+            atomicAdd(&grad[3 * ia + q], v);
+            atomicAdd(&grad[3 * ib + q], -v);
+        }
+    }
+}
+
+void launchNearGradDip(const SrcDev& s, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+                       const int64_t* iperm, const cplx<double>* I, const cplx<double>* lam, double k, cplx<double> Ck,
+                       double* grad, hipStream_t st) {
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL(kNearGradDip, dim3(nitems), dim3(BS), 0, st, s, items, colsIn, perm, iperm, I, lam, k, Ck,
+                       grad);
+}
+
 // ---- physical near field (no counterpart in the reference, which drops
 // coincident triangles and keeps the hypersingular dyadic kernel).
 //

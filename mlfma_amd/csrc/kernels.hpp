@@ -92,6 +92,10 @@ void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const dou
                     cplx<double> Ck, cplx<double>* E, hipStream_t st);
 void launchFarfieldAdj(const SrcDev& s, int64_t n, const cplx<double>* E, const double* dirs, int ndir, double k,
                        cplx<double> Ckc, const int64_t* perm, cplx<double>* I, hipStream_t st);
+// dipole position gradient, near part: grad (n x 3, input order) += -2 Re(lambda^H dZ_near I)
+void launchNearGradDip(const SrcDev& s, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+                       const int64_t* iperm, const cplx<double>* I, const cplx<double>* lam, double k, cplx<double> Ck,
+                       double* grad, hipStream_t st);
 
 // matvec stages, templated on the real type of the far field
 void launchGather(const cplx<double>* x, const int64_t* perm, cplx<double>* xt, int64_t n, int nrhs,

@@ -1016,6 +1016,48 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
 #undef L2T_CASE
 }
 
+// Direction-weighted disaggregation for the position gradient:
+// out[i][a] = sum_d khat_d[a] conj(rp_s[d]) . F[d] for every source s of a
+// far-field leaf (i = perm[s], input order), F the leaf's field; one wave per
+// source row.  With F the forward locals L(I) this is the row derivative of
+// L2T, with F the adjoint multipoles M'(lambda) the column derivative of S2M
+// (a dipole pattern's position derivative is -ik khat times the pattern).
+__global__ __launch_bounds__(TB) void kL2TGrad(const int64_t* __restrict__ rowPos, const int32_t* __restrict__ rowItem,
+                                               int64_t nrows, const LeafItem* __restrict__ items,
+                                               const LevelDev* __restrict__ lv, const cplx<double>* __restrict__ rp,
+                                               const cplx<double>* __restrict__ F, const int64_t* __restrict__ perm,
+                                               cplx<double>* __restrict__ out) {
+    const int64_t w = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nrows) return;
+    const int64_t t = rowPos[w];
+    const LeafItem it = items[rowItem[w]];
+    const double* kh = lv[it.level].khat;
+    const cplx<double>* r = rp + it.rpOff + (t - it.sbeg) * it.ndir * 2;
+    const cplx<double>* Fb = F + it.fOff;
+    double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
+    for (int d = lane; d < it.ndir; d += 64) {
+        const cplx<double> v = conjmul(r[2 * d], Fb[2 * d]) + conjmul(r[2 * d + 1], Fb[2 * d + 1]);
+        for (int a = 0; a < 3; ++a) {
+            re[a] += kh[3 * d + a] * v.re;
+            im[a] += kh[3 * d + a] * v.im;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        const double x = wsum(re[a]), y = wsum(im[a]);
+        if (lane == 0) out[3 * perm[t] + a] = mk(x, y);
+    }
+}
+
+void launchL2TGrad(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items,
+                   const LevelDev* lv, const cplx<double>* rp, const cplx<double>* F, const int64_t* perm,
+                   cplx<double>* out, hipStream_t st) {
+    if (nrows <= 0) return;
+    const int64_t nbk = (nrows + TB / 64 - 1) / (TB / 64);
+    hipLaunchKernelGGL(kL2TGrad, dim3((unsigned)nbk), dim3(TB), 0, st, rowPos, rowItem, nrows, items, lv, rp, F, perm,
+                       out);
+}
+
 #define INST(R)                                                                                                     \
     template void launchS2Mx<R>(const LeafItem*, int, int, const cplx<R>*, const cplx<double>*, const int64_t*,     \
                                 cplx<R>*, hipStream_t, int, int64_t, int64_t);                                     \

@@ -95,4 +95,9 @@ bool launchL2LAdj(int npar, const int32_t* parSlot, const int32_t* ptr, const in
 template <typename R>
 void launchZeroField(cplx<R>* f, int64_t n, hipStream_t st);
 
+// out (n x 3 complex, input order) = sum_d khat_d conj(rp_s[d]) . F[d] per far-field source
+void launchL2TGrad(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items,
+                   const LevelDev* lv, const cplx<double>* rp, const cplx<double>* F, const int64_t* perm,
+                   cplx<double>* out, hipStream_t st);
+
 }  // namespace mlfma

@@ -9,6 +9,7 @@
 #include <memory>
 #include <vector>
 
+#include "kernels_mv.hpp"
 #include "plan.hpp"
 
 namespace mlfma {
@@ -815,6 +816,70 @@ int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs
     hipCheck(hipStreamSynchronize(st), "farfield adjoint");
     d.release(dd);
     d.release(dE);
+    return MLFMA_OK;
+    API_END
+}
+
+// grad[i][a] = -2 Re(lambda^H (dZ/dr_i^a) I) for dipole sources.  Z = near +
+// C k R^H A S depends on the positions through the near entries and the
+// patterns rp_s[d] = P_d phat_s e^{i k khat_d.(c - r_s)} of R and S (the
+// translations depend on box centres only), so with L = A S I the forward
+// locals and M' = A^H R lambda the adjoint multipoles,
+//   row part    conj(lambda_i) C k (ik)  sum_d khat_d conj(rp_i[d]) . L[d]
+//   column part I_i C k (-ik) conj(sum_d khat_d conj(rp_i[d]) . M'[d])
+// and the near part from the Green's-function gradient of each stored entry.
+int mlfma_position_gradient(mlfma_plan* plan, const double* I, const double* lambda, double* grad) {
+    API_BEGIN
+    DevicePlan& d = Pl(plan);
+    if (!I || !lambda || !grad) throw Error(MLFMA_ERR_ARG, "null argument");
+    if (d.hp.src.kind != MLFMA_SRC_DIPOLE || d.src.physical)
+        throw Error(MLFMA_ERR_ARG, "position gradients are provided for dipole sources only");
+    if (d.hp.staleBoxes > 0 && !(d.flags & MLFMA_FLAG_FRESH_LOCALS))
+        throw Error(MLFMA_ERR_STATE,
+                    "the operator keeps stale locals (boxes with an empty interaction list) and is not linear; "
+                    "set MLFMA_FLAG_FRESH_LOCALS for a position gradient");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const int64_t n = d.hp.src.n;
+    hipStream_t st = d.stream;
+    d.ensureRhs(1);
+    cplx<double>* dI = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    cplx<double>* dLam = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    cplx<double>* dY = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    cplx<double>* gRow = (cplx<double>*)d.alloc((size_t)n * 3 * sizeof(cplx<double>));
+    cplx<double>* gCol = (cplx<double>*)d.alloc((size_t)n * 3 * sizeof(cplx<double>));
+    double* gNear = (double*)d.alloc((size_t)n * 3 * sizeof(double));
+    std::vector<int64_t> ip(n);
+    for (int64_t t = 0; t < n; ++t) ip[d.hp.tree.perm[t]] = t;
+    int64_t* iperm = (int64_t*)d.alloc((size_t)n * sizeof(int64_t));
+    hipCheck(hipMemcpyAsync(iperm, ip.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st), "iperm upload");
+    hipCheck(hipMemcpyAsync(dI, I, n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "I upload");
+    hipCheck(hipMemcpyAsync(dLam, lambda, n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "lambda upload");
+    hipCheck(hipMemsetAsync(gRow, 0, (size_t)n * 3 * sizeof(cplx<double>), st), "clear");
+    hipCheck(hipMemsetAsync(gCol, 0, (size_t)n * 3 * sizeof(cplx<double>), st), "clear");
+    hipCheck(hipMemsetAsync(gNear, 0, (size_t)n * 3 * sizeof(double), st), "clear");
+    d.applyOp(dI, dY, 0, st, false, false);  // forward locals L(I) stay in the local field
+    launchL2TGrad(d.farRowPos, d.farRowItem, d.nFarRows, d.leafItems, d.levelDev, d.rp64, d.L64, d.perm, gRow, st);
+    d.applyOpAdjoint(dLam, dY, 0, st, false);  // adjoint multipoles M'(lambda) stay in the multipole field
+    launchL2TGrad(d.farRowPos, d.farRowItem, d.nFarRows, d.leafItems, d.levelDev, d.rp64, d.M64, d.perm, gCol, st);
+    launchNearGradDip(d.src, d.nearItems, d.nNearItems, d.nearColsIn, d.perm, iperm, dI, dLam, d.hp.k, d.Ck, gNear, st);
+    hipCheck(hipGetLastError(), "position gradient launch");
+    std::vector<std::complex<double>> row(3 * n), col(3 * n);
+    std::vector<double> nearg(3 * n);
+    hipCheck(hipMemcpyAsync(row.data(), gRow, 3 * n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "download");
+    hipCheck(hipMemcpyAsync(col.data(), gCol, 3 * n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "download");
+    hipCheck(hipMemcpyAsync(nearg.data(), gNear, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st), "download");
+    hipCheck(hipStreamSynchronize(st), "position gradient");
+    for (void* p : {(void*)dI, (void*)dLam, (void*)dY, (void*)gRow, (void*)gCol, (void*)gNear, (void*)iperm})
+        d.release(p);
+    const std::complex<double> Ck(d.Ck.re, d.Ck.im), ik(0.0, d.hp.k);
+    const std::complex<double>* Ih = (const std::complex<double>*)I;
+    const std::complex<double>* Lh = (const std::complex<double>*)lambda;
+    for (int64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const std::complex<double> v =
+                std::conj(Lh[i]) * Ck * ik * row[3 * i + a] - Ih[i] * Ck * ik * std::conj(col[3 * i + a]);
+            grad[3 * i + a] = nearg[3 * i + a] - 2.0 * v.real();
+        }
     return MLFMA_OK;
     API_END
 }

@@ -223,6 +223,16 @@ class Plan:
                                                          I.ctypes.data_as(A.dptr)))
         return I
 
+    def position_gradient(self, I, lam):
+        """(n, 3) array -2 Re(lam^H (dZ/dr_i^a) I) for a dipole plan: the
+        operator's sensitivity to each source position (tree held fixed)."""
+        I = _cvec(I, self.n)
+        lam = _cvec(lam, self.n)
+        g = np.zeros((self.n, 3))
+        _check(self.lib, self.lib.mlfma_position_gradient(self._h, I.ctypes.data_as(A.dptr),
+                                                          lam.ctypes.data_as(A.dptr), g.ctypes.data_as(A.dptr)))
+        return g
+
     # ---------------------------------------------------------------- introspection
     def level_boxes(self, level):
         nb = C.c_int64()
