@@ -93,6 +93,11 @@ typedef struct mlfma_planewave {
                                        accumulate, precision) and replay it
                                        (up to 4 cached; cleared when the flags
                                        change; not with stage times) */
+#define MLFMA_FLAG_NEAR_SERIAL 128u /* run the near-field pair products after
+                                       the far-field chain on the matvec's
+                                       stream instead of beside it on the
+                                       plan's second stream (stage-timed
+                                       matvecs always do) */
 #define MLFMA_FLAG_ASYNC 16u        /* matvecs on device pointers return once
                                        queued on the stream instead of after
                                        completion (order by the stream; host
@@ -275,10 +280,11 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
 
 /* Choose the plan's launch parameters for batches of nrhs (1, 2, 4 or 8)
  * right-hand sides by timing candidates with device events: the M2M LDS
- * budget and block target, and per level the sibling-group or per-target
- * M2L kernel.  Mean matvec device time before and after (ms, may be NULL).
- * Results do not depend on the choice beyond round-off.  No reference
- * counterpart. */
+ * budget and block target, per level the sibling-group or per-target M2L
+ * kernel, and whether the near-field pair products run beside the far-field
+ * chain (on a second stream of the plan) or after it.  Mean matvec device
+ * time before and after (ms, may be NULL).  Results do not depend on the
+ * choice beyond round-off.  No reference counterpart. */
 int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms_after);
 
 /* Replace the plan's MLFMA_FLAG_* set for later matvecs and solves (the flags

@@ -58,8 +58,12 @@ DevicePlan::~DevicePlan() {
     if (stream) (void)hipStreamSynchronize(stream);
     clearGraphs();
     for (void* p : allocs) (void)hipFree(p);
+    if (nearStream) (void)hipStreamSynchronize(nearStream);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
+    if (evFork) (void)hipEventDestroy(evFork);
+    if (evJoin) (void)hipEventDestroy(evJoin);
+    if (nearStream) (void)hipStreamDestroy(nearStream);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -103,6 +107,11 @@ void DevicePlan::build() {
     }
     // stage events need no system-scope fence: they order work of one stream
     for (auto& e : ev) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
+    // the near field runs beside the far-field chain on a second stream
+    // (default priority), forked and joined with events
+    hipCheck(hipStreamCreateWithFlags(&nearStream, hipStreamNonBlocking), "hipStreamCreate");
+    hipCheck(hipEventCreateWithFlags(&evFork, hipEventDisableTiming), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evJoin, hipEventDisableTiming), "hipEventCreate");
     const int64_t n = h.src.n;
     const Tree& t = h.tree;
     const int ML = t.maxLevel;
@@ -638,8 +647,21 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     cplx<R>* Lc = fieldL<R>();
     cplx<R>* inner = scratch<R>();
     const cplx<R>* rp = radpat<R>();
+    // the near-field pairs depend on x only: fork them onto the near stream
+    // so they overlap the far-field chain, before S2M (nearOverlap 1) or
+    // after it (2: the bandwidth-bound pair products then run beside the
+    // latency-bound M2M/M2L/L2L only); serial when stages are timed
+    const int overlap = (timed || (flags & MLFMA_FLAG_NEAR_SERIAL)) ? 0 : nearOverlap;
+    auto forkNear = [&]() {
+        hipCheck(hipEventRecord(evFork, st), "event");
+        hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
+        launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, nearStream, nb, xs, ps);
+        hipCheck(hipEventRecord(evJoin, nearStream), "event");
+    };
+    if (overlap == 1) forkNear();
     if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
     launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
+    if (overlap == 2) forkNear();
     if (timed) hipCheck(hipEventRecord(ev[1], st), "event");
     for (int l = ML - 1; l >= 2; --l) {
         LevelBufs& P = levels[l];
@@ -681,7 +703,10 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         }
     }
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
-    launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st, nb, xs, ps);
+    if (overlap)
+        hipCheck(hipStreamWaitEvent(st, evJoin, 0), "wait");
+    else
+        launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
@@ -705,6 +730,13 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
     cplx<R>* M = fieldM<R>();
     cplx<R>* Lc = fieldL<R>();
     const cplx<R>* rp = radpat<R>();
+    const bool overlap = nearOverlap != 0 && !(flags & MLFMA_FLAG_NEAR_SERIAL);
+    if (overlap) {
+        hipCheck(hipEventRecord(evFork, st), "event");
+        hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
+        launchNearSym(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, nearStream);
+        hipCheck(hipEventRecord(evJoin, nearStream), "event");
+    }
     launchZeroField<R>(M, fieldLen, st);
     launchZeroField<R>(Lc, fieldLen, st);
     launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, Lc, st);
@@ -727,7 +759,10 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
                                P.nAtTh, C.phUp<R>(), M + P.Moff, M + C.Moff, ldsMax, st))
             throw Error(MLFMA_ERR_ARG, "adjoint M2M: angular grid too large for one block");
     }
-    launchNearSym(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
+    if (overlap)
+        hipCheck(hipStreamWaitEvent(st, evJoin, 0), "wait");
+    else
+        launchNearSym(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st);
     const cplx<double> Ckc{Ck.re, -Ck.im};
     launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, M, Ckc, nearSumPtr, nearSumIdx, nearPart, perm, y,
@@ -1085,8 +1120,22 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
         }
         return acc / reps;
     };
+    // mean device time of whole matvecs (no stage events) over reps
+    auto timeWhole = [&](int reps) {
+        const uint32_t f = d.flags;
+        d.flags &= ~MLFMA_FLAG_STAGE_TIMES;
+        d.matvec(d.xt, d.yt, nrhs, 0, d.stream, false);
+        hipCheck(hipEventRecord(d.ev[0], d.stream), "event");
+        for (int r = 0; r < reps; ++r) d.matvec(d.xt, d.yt, nrhs, 0, d.stream, false);
+        hipCheck(hipEventRecord(d.ev[7], d.stream), "event");
+        hipCheck(hipStreamSynchronize(d.stream), "autotune");
+        d.flags = f;
+        float t = 0;
+        hipCheck(hipEventElapsedTime(&t, d.ev[0], d.ev[7]), "hipEventElapsedTime");
+        return (double)t / reps;
+    };
     try {
-        const double before = timeStage(0, 7, 5);
+        const double before = timeWhole(10);
         // M2M: LDS budget for staged children x block target for the row split
         double best = 1e300;
         int bb = 0, bm = 0;
@@ -1119,7 +1168,20 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
             }
             d.tuneM2L[l] = mode;
         }
-        const double after = timeStage(0, 7, 5);
+        // near field beside the far-field chain, or after it
+        if (!(saved & MLFMA_FLAG_NEAR_SERIAL)) {
+            double tm[3] = {0, 0, 0};
+            for (int rep = 0; rep < 2; ++rep)
+                for (int m = 0; m < 3; ++m) {
+                    d.nearOverlap = m;
+                    tm[m] += timeWhole(10);
+                }
+            int bestMode = 2;
+            for (int m = 0; m < 3; ++m)
+                if (tm[m] < tm[bestMode]) bestMode = m;
+            d.nearOverlap = bestMode;
+        }
+        const double after = timeWhole(10);
         if (ms_before) *ms_before = before;
         if (ms_after) *ms_after = after;
     } catch (...) {
@@ -1151,7 +1213,7 @@ int mlfma_plan_set_flags(mlfma_plan* plan, uint32_t flags) {
     API_BEGIN
     DevicePlan& d = P(plan);
     if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES | MLFMA_FLAG_M2L_GROUP | MLFMA_FLAG_M2L_TARGET |
-                  MLFMA_FLAG_ASYNC | MLFMA_FLAG_PHYSICAL | MLFMA_FLAG_GRAPH))
+                  MLFMA_FLAG_ASYNC | MLFMA_FLAG_PHYSICAL | MLFMA_FLAG_GRAPH | MLFMA_FLAG_NEAR_SERIAL))
         throw Error(MLFMA_ERR_ARG, "unknown flag bits");
     if ((flags ^ d.flags) & MLFMA_FLAG_PHYSICAL)
         throw Error(MLFMA_ERR_ARG, "MLFMA_FLAG_PHYSICAL is fixed when the plan is created");

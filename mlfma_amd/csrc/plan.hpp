@@ -88,6 +88,9 @@ struct DevicePlan {
     bool useFp32 = false;
     hipStream_t stream = nullptr;
     std::array<hipEvent_t, 9> ev{};
+    hipStream_t nearStream = nullptr;  // near-field pairs beside the far-field chain
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    int nearOverlap = 2;               // 0 serial, 1 fork before S2M, 2 after; chosen by mlfma_autotune
     bool timesValid = false;
     uint32_t flags = 0;  // MLFMA_FLAG_* in effect for matvecs (settable after creation)
     std::vector<void*> allocs;

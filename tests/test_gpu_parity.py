@@ -165,7 +165,7 @@ def test_stage_times_and_runtime_flags():
     assert abs(d["near_pairs"] + d["near_sum"] - st["near"]) <= 1e-3
     assert abs(sum(d.values()) - st["total"]) <= 1e-3
     with pytest.raises(Exception):
-        p.set_flags(1 << 7)                # unknown flag bit
+        p.set_flags(1 << 8)                # unknown flag bit
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -293,6 +293,33 @@ def test_async_device_matvec():
     torch.cuda.synchronize()
     p.set_flags(0)
     assert cases.relerr(z.cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()) == 0.0
+
+
+def test_near_overlap_matches_serial():
+    """The near-field pair products beside the far-field chain (the plan's
+    second stream, the default) give the bits of the serial order
+    (MLFMA_FLAG_NEAR_SERIAL): forward, batched and adjoint matvecs, eager,
+    queued on torch's stream and replayed from a graph."""
+    import torch
+    geom, kw = cases.make("rwg_adaptive")
+    base = A.MLFMA_FLAG_FRESH_LOCALS
+    p = Plan(geom, cases.planewave_z(), A.make_config(device=0, flags=base, **kw))
+    x = torch.from_numpy(G.complex_gaussian(geom.n, 31)).to("cuda:0")
+    X = torch.from_numpy(np.stack([G.complex_gaussian(geom.n, 40 + r) for r in range(6)])).to("cuda:0")
+    p.set_flags(base | A.MLFMA_FLAG_NEAR_SERIAL)
+    y0, Y0, z0 = (p.matvec(x).cpu().numpy(), p.matvec(X).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy())
+    for f in (base, base | A.MLFMA_FLAG_ASYNC, base | A.MLFMA_FLAG_GRAPH):
+        p.set_flags(f)
+        y, Y = torch.empty_like(x), torch.empty_like(X)
+        for rep in range(3):
+            p.matvec(x, y)
+            p.matvec(X, Y)
+            z = p.matvec_adjoint(x)
+            torch.cuda.synchronize()
+            say("flags", f, "rep", rep)
+            assert cases.relerr(y.cpu().numpy(), y0) == 0.0
+            assert cases.relerr(Y.cpu().numpy(), Y0) == 0.0
+            assert cases.relerr(z.cpu().numpy(), z0) == 0.0
 
 
 def test_graph_replay_matches_eager():
