@@ -93,11 +93,13 @@ typedef struct mlfma_planewave {
                                        accumulate, precision) and replay it
                                        (up to 4 cached; cleared when the flags
                                        change; not with stage times) */
-#define MLFMA_FLAG_NEAR_SERIAL 128u /* run the near-field pair products after
-                                       the far-field chain on the matvec's
-                                       stream instead of beside it on the
-                                       plan's second stream (stage-timed
-                                       matvecs always do) */
+#define MLFMA_FLAG_ONE_STREAM 128u /* run every stage in order on the
+                                       matvec's stream, instead of the
+                                       near-field pair products and the
+                                       finest level's M2L beside the
+                                       far-field chain on two more streams of
+                                       the plan (stage-timed matvecs always
+                                       run in order) */
 #define MLFMA_FLAG_ASYNC 16u        /* matvecs on device pointers return once
                                        queued on the stream instead of after
                                        completion (order by the stream; host
@@ -279,10 +281,12 @@ int mlfma_stage_times(const mlfma_plan* plan, double* ms7);
 int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, int32_t* count);
 
 /* Choose the plan's launch parameters for batches of nrhs (1, 2, 4 or 8)
- * right-hand sides by timing candidates with device events: the M2M LDS
- * budget and block target, per level the sibling-group or per-target M2L
- * kernel, and whether the near-field pair products run beside the far-field
- * chain (on a second stream of the plan) or after it.  Mean matvec device
+ * right-hand sides by timing candidates with device events: the block
+ * target of the M2M (and adjoint L2L) row split, per level the sibling-group or per-target
+ * M2L kernel, whether the near-field pair products run beside the far-field
+ * chain (on a second stream of the plan, forked before or after S2M) or after
+ * it, and whether the finest level's M2L runs beside the M2M chain (on a
+ * third stream) or in line.  Mean matvec device
  * time before and after (ms, may be NULL).  Results do not depend on the
  * choice beyond round-off.  No reference counterpart. */
 int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms_after);

@@ -24,7 +24,7 @@ MLFMA_FLAG_M2L_TARGET = 8
 MLFMA_FLAG_ASYNC = 16
 MLFMA_FLAG_PHYSICAL = 32
 MLFMA_FLAG_GRAPH = 64
-MLFMA_FLAG_NEAR_SERIAL = 128
+MLFMA_FLAG_ONE_STREAM = 128
 MLFMA_PTR_HOST = 0
 MLFMA_PTR_DEVICE = 1
 MLFMA_GMRES_COMPAT = 0

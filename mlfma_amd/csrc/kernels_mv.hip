@@ -11,9 +11,6 @@ namespace mlfma {
 constexpr int TB = 256;
 
 // launch-shape parameters (compile-time; defaults measured on MI355X)
-#ifndef MLFMA_M2M_LDS_BUDGET
-#define MLFMA_M2M_LDS_BUDGET 32768   // LDS bytes per M2M block for staged children
-#endif
 #ifndef MLFMA_M2M_MIN_BLOCKS
 #define MLFMA_M2M_MIN_BLOCKS 1024    // split parent theta rows until this many blocks
 #endif
@@ -98,16 +95,19 @@ void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* r
 
 // ------------------------------------------------------------------ fused M2M
 // One block per (parent box, run of parent theta rows) (src/fmm/stem.cpp:
-// 72-103,133-184).  The interpolation tables of the rows are staged in LDS;
-// the non-empty children are phase-shifted into LDS a batch at a time (one
-// global round trip per batch); the theta interpolation (pole rule: reflected
-// node, phi + pi, both components negated) accumulates in registers over all
-// children, goes to LDS, and the periodic phi interpolation writes the parent.
-template <typename R, int EPT>
+// 72-103,133-184).  The theta and phi interpolations do not depend on the
+// child, so the children are phase-shifted and summed on the child grid first
+// (one LDS array, one global read of every child direction) and the sum is
+// interpolated once: the same linear map as interpolating every child and
+// adding, with a different rounding order.  The interpolation tables of the
+// rows are staged in LDS; the theta interpolation (pole rule: reflected node,
+// phi + pi, both components negated) goes to LDS, and the periodic phi
+// interpolation writes the parent.
+template <typename R>
 __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stemSlot, const int32_t* __restrict__ ptr,
                                                 const int32_t* __restrict__ child, const int32_t* __restrict__ oct,
-                                                int nstem, int per, int rows, int cb, int nthP, int nphP, int mthC,
-                                                int mphC, int p, const int32_t* __restrict__ thNear,
+                                                int nstem, int per, int rows, int nthP, int nphP, int mthC, int mphC,
+                                                int p, const int32_t* __restrict__ thNear,
                                                 const double* __restrict__ thW, const int32_t* __restrict__ phNear,
                                                 const double* __restrict__ phW, const cplx<R>* __restrict__ phaseUp,
                                                 const cplx<R>* __restrict__ Mc, cplx<R>* __restrict__ Mp, int64_t fs) {
@@ -121,8 +121,8 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
     const int ndirC = mthC * mphC;
     const int nInner = nr * mphC;
     const int tp = 2 * p;
-    cplx<R>* sC = reinterpret_cast<cplx<R>*>(smem);       // cb children x ndirC x 2
-    cplx<R>* sI = sC + (size_t)cb * 2 * ndirC;             // rows x mphC x 2
+    cplx<R>* sS = reinterpret_cast<cplx<R>*>(smem);  // ndirC x 2: shifted children, summed
+    cplx<R>* sI = sS + (size_t)2 * ndirC;            // rows x mphC x 2
     R* tThW = reinterpret_cast<R*>(sI + (size_t)2 * rows * mphC);
     R* tPhW = tThW + rows * tp;
     int* tThNear = reinterpret_cast<int*>(tPhW + nphP * tp);
@@ -131,50 +131,35 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
     for (int i = threadIdx.x; i < nphP * tp; i += TB) tPhW[i] = (R)phW[i];
     for (int i = threadIdx.x; i < nr; i += TB) tThNear[i] = thNear[j0 + i];
     for (int i = threadIdx.x; i < nphP; i += TB) tPhNear[i] = phNear[i];
-    cplx<R> acc[EPT][2];
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) acc[k][0] = acc[k][1] = mk<R>(0, 0);
     const int c0 = ptr[item], c1 = ptr[item + 1];
-    for (int cbeg = c0; cbeg < c1; cbeg += cb) {
-        const int nb = min(cb, c1 - cbeg);
-        __syncthreads();  // tables staged / previous batch consumed
-        for (int e = threadIdx.x; e < nb * ndirC; e += TB) {
-            const int k = e / ndirC, d = e - k * ndirC;
-            const int c = cbeg + k;
+    for (int d = threadIdx.x; d < ndirC; d += TB) {
+        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+        for (int c = c0; c < c1; ++c) {
             const cplx<R>* Mb = Mc + (int64_t)child[c] * ndirC * 2;
             const cplx<R> s = phaseUp[(int64_t)oct[c] * ndirC + d];
-            sC[2 * e] = s * Mb[2 * d];
-            sC[2 * e + 1] = s * Mb[2 * d + 1];
+            a0 = a0 + s * Mb[2 * d];
+            a1 = a1 + s * Mb[2 * d + 1];
         }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int e = threadIdx.x + k * TB;
-            if (e < nInner) {
-                const int jr = e / mphC, iph = e - jr * mphC;
-                const int near = tThNear[jr];
-                const int iphf = iph + (iph < mphC / 2 ? mphC / 2 : -mphC / 2);
-                for (int q = 0; q < tp; ++q) {
-                    const int i = near + 1 - p + q;
-                    const int f = i < 0 ? -i - 1 : (i >= mthC ? 2 * mthC - i - 1 : i);
-                    const bool flip = (i != f);
-                    const int dd = 2 * (f * mphC + (flip ? iphf : iph));
-                    const R w = flip ? -tThW[jr * tp + q] : tThW[jr * tp + q];
-                    for (int b = 0; b < nb; ++b) {
-                        acc[k][0] = acc[k][0] + w * sC[b * 2 * ndirC + dd];
-                        acc[k][1] = acc[k][1] + w * sC[b * 2 * ndirC + dd + 1];
-                    }
-                }
-            }
-        }
+        sS[2 * d] = a0;
+        sS[2 * d + 1] = a1;
     }
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int e = threadIdx.x + k * TB;
-        if (e < nInner) {
-            sI[2 * e] = acc[k][0];
-            sI[2 * e + 1] = acc[k][1];
+    __syncthreads();
+    for (int e = threadIdx.x; e < nInner; e += TB) {
+        const int jr = e / mphC, iph = e - jr * mphC;
+        const int near = tThNear[jr];
+        const int iphf = iph + (iph < mphC / 2 ? mphC / 2 : -mphC / 2);
+        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+        for (int q = 0; q < tp; ++q) {
+            const int i = near + 1 - p + q;
+            const int f = i < 0 ? -i - 1 : (i >= mthC ? 2 * mthC - i - 1 : i);
+            const bool flip = (i != f);
+            const int dd = 2 * (f * mphC + (flip ? iphf : iph));
+            const R w = flip ? -tThW[jr * tp + q] : tThW[jr * tp + q];
+            a0 = a0 + w * sS[dd];
+            a1 = a1 + w * sS[dd + 1];
         }
+        sI[2 * e] = a0;
+        sI[2 * e + 1] = a1;
     }
     __syncthreads();
     cplx<R>* out = Mp + ((int64_t)stemSlot[item] * nthP + j0) * nphP * 2;
@@ -199,41 +184,24 @@ template <typename R>
 bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
                     int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
                     const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
-                    size_t ldsMax, hipStream_t st, int nb, int64_t fs, int ldsBudget, int minBlocks) {
+                    size_t ldsMax, hipStream_t st, int nb, int64_t fs, int minBlocks) {
     if (nstem <= 0) return true;
-    if (ldsBudget <= 0) ldsBudget = MLFMA_M2M_LDS_BUDGET;
     if (minBlocks <= 0) minBlocks = MLFMA_M2M_MIN_BLOCKS;
     // split the parent's theta rows so that there are enough blocks for the
-    // device (each block re-reads the children's fields from L2)
+    // device (each block sums the children on the child grid again)
     int rows = nthP;
     while (rows > 2 && (int64_t)nstem * ((nthP + rows - 1) / rows) < minBlocks) rows = (rows + 1) / 2;
-    while (rows > 1 && rows * mphC > 16 * TB) rows = (rows + 1) / 2;  // at most 16 outputs per thread
     const int tp = 2 * p;
-    const size_t fixed = sizeof(cplx<R>) * 2 * (size_t)rows * mphC + sizeof(R) * ((size_t)rows + nphP) * tp +
-                        sizeof(int) * ((size_t)rows + nphP);
-    const size_t perChild = sizeof(cplx<R>) * 2 * (size_t)mthC * mphC;
-    // children per batch: as many as fit the LDS budget (several blocks per CU)
-    const size_t budget = std::min<size_t>(ldsMax, (size_t)ldsBudget);
-    int cb = fixed < budget ? (int)std::min<size_t>(8, (budget - fixed) / perChild) : 0;
-    if (cb < 1) {
-        if (fixed + perChild > ldsMax) return false;
-        cb = 1;
-    }
-    const size_t lds = fixed + perChild * cb;
-    const int nInner = rows * mphC;
-    const int ept = (nInner + TB - 1) / TB;
-    if (ept > 16) return false;
+    auto ldsFor = [&](int r) {
+        return sizeof(cplx<R>) * 2 * ((size_t)mthC * mphC + (size_t)r * mphC) +
+               sizeof(R) * ((size_t)r + nphP) * tp + sizeof(int) * ((size_t)r + nphP);
+    };
+    while (rows > 1 && ldsFor(rows) > ldsMax) rows = (rows + 1) / 2;
+    if (ldsFor(rows) > ldsMax) return false;
     const int per = (nstem + 7) / 8;
     dim3 g(8 * per, (nthP + rows - 1) / rows, nb);
-#define M2M_CASE(E)                                                                                                \
-    hipLaunchKernelGGL((kM2MFused<R, E>), g, dim3(TB), lds, st, stemSlot, ptr, child, oct, nstem, per, rows, cb,  \
-                       nthP, nphP, mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp, fs)
-    if (ept <= 1) M2M_CASE(1);
-    else if (ept <= 2) M2M_CASE(2);
-    else if (ept <= 4) M2M_CASE(4);
-    else if (ept <= 8) M2M_CASE(8);
-    else M2M_CASE(16);
-#undef M2M_CASE
+    hipLaunchKernelGGL((kM2MFused<R>), g, dim3(TB), ldsFor(rows), st, stemSlot, ptr, child, oct, nstem, per, rows, nthP,
+                       nphP, mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp, fs);
     return true;
 }
 
@@ -1064,7 +1032,7 @@ void launchL2TGrad(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows,
     template bool launchM2MFused<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                     int, int, const int32_t*, const double*, const int32_t*, const double*,         \
                                     const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t, int, int64_t,    \
-                                    int, int);                                                                      \
+                                    int);                                                                      \
     template void launchM2LGroup<R>(M2LGroups<R>, bool, hipStream_t, int);                                          \
     template void launchM2Lx<R>(M2LLevels<R>, bool, hipStream_t, int);                                              \
     template bool launchL2LFused<R>(bool, int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,   \

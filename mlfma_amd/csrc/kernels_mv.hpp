@@ -13,14 +13,13 @@ void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* r
                 const int64_t* perm, cplx<R>* M, hipStream_t st, int nb = 1, int64_t xs = 0, int64_t fs = 0);
 
 // returns false when the level's fields do not fit the LDS limit (caller
-// then uses the two-pass kernels); ldsBudget / minBlocks: LDS bytes for
-// staged children and the block count to split parent rows for (0: the
-// compile-time defaults)
+// then uses the two-pass kernels); minBlocks: the block count to split parent
+// rows for (0: the compile-time default)
 template <typename R>
 bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, const int32_t* child, const int32_t* oct,
                     int nthP, int nphP, int mthC, int mphC, int p, const int32_t* thNear, const double* thW,
                     const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
-                    size_t ldsMax, hipStream_t st, int nb = 1, int64_t fs = 0, int ldsBudget = 0, int minBlocks = 0);
+                    size_t ldsMax, hipStream_t st, int nb = 1, int64_t fs = 0, int minBlocks = 0);
 
 // M2L work of every level (sibling groups), passed by value to one launch
 constexpr int kMaxLevels = 16;

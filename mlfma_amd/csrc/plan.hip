@@ -59,11 +59,13 @@ DevicePlan::~DevicePlan() {
     clearGraphs();
     for (void* p : allocs) (void)hipFree(p);
     if (nearStream) (void)hipStreamSynchronize(nearStream);
+    if (m2lStream) (void)hipStreamSynchronize(m2lStream);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
-    if (evFork) (void)hipEventDestroy(evFork);
-    if (evJoin) (void)hipEventDestroy(evJoin);
+    for (hipEvent_t e : {evFork, evJoin, evLeafM, evM2L})
+        if (e) (void)hipEventDestroy(e);
     if (nearStream) (void)hipStreamDestroy(nearStream);
+    if (m2lStream) (void)hipStreamDestroy(m2lStream);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -112,6 +114,9 @@ void DevicePlan::build() {
     hipCheck(hipStreamCreateWithFlags(&nearStream, hipStreamNonBlocking), "hipStreamCreate");
     hipCheck(hipEventCreateWithFlags(&evFork, hipEventDisableTiming), "hipEventCreate");
     hipCheck(hipEventCreateWithFlags(&evJoin, hipEventDisableTiming), "hipEventCreate");
+    hipCheck(hipStreamCreateWithFlags(&m2lStream, hipStreamNonBlocking), "hipStreamCreate");
+    hipCheck(hipEventCreateWithFlags(&evLeafM, hipEventDisableTiming), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evM2L, hipEventDisableTiming), "hipEventCreate");
     const int64_t n = h.src.n;
     const Tree& t = h.tree;
     const int ML = t.maxLevel;
@@ -592,13 +597,13 @@ void DevicePlan::ensureRhs(int nrhs) {
 // device go through the sibling-group kernel, the others through the
 // per-target kernel.  adjoint: conjugated operators of the reversed pairs.
 template <typename R>
-void DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb) {
+void DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb, int lmin, int lmax) {
     const int ML = hp.tree.maxLevel;
     if (ML - 1 > kMaxLevels) throw Error(MLFMA_ERR_ARG, "too many tree levels");
     M2LGroups<R> g;
     M2LLevels<R> t;
     g.fs = t.fs = fieldLen;
-    for (int l = 2; l <= ML; ++l) {
+    for (int l = std::max(2, lmin); l <= std::min(ML, lmax); ++l) {
         LevelBufs& L = levels[l];
         if (L.ntgt == 0) continue;
         const int64_t nch = (L.ndir + 255) / 256;
@@ -651,7 +656,8 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // so they overlap the far-field chain, before S2M (nearOverlap 1) or
     // after it (2: the bandwidth-bound pair products then run beside the
     // latency-bound M2M/M2L/L2L only); serial when stages are timed
-    const int overlap = (timed || (flags & MLFMA_FLAG_NEAR_SERIAL)) ? 0 : nearOverlap;
+    const bool serial = timed || (flags & MLFMA_FLAG_ONE_STREAM);
+    const int overlap = serial ? 0 : nearOverlap;
     auto forkNear = [&]() {
         hipCheck(hipEventRecord(evFork, st), "event");
         hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
@@ -662,13 +668,29 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
     launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
     if (overlap == 2) forkNear();
+    // M2L of the finest level needs only the leaf multipoles, and its locals
+    // are first read by the L2L into that level: it runs on its own stream
+    // beside the M2M chain and the coarser M2L
+    const bool splitM2L = !serial && m2lSplit && ML >= 2 && levels[ML].ntgt > 0;
+    bool m2lPending = false;
+    if (splitM2L) {
+        hipCheck(hipEventRecord(evLeafM, st), "event");
+        hipCheck(hipStreamWaitEvent(m2lStream, evLeafM, 0), "wait");
+        launchM2L<R>(M, Lc, false, m2lStream, nb, ML, ML);
+        hipCheck(hipEventRecord(evM2L, m2lStream), "event");
+        m2lPending = true;
+    }
+    auto joinM2L = [&]() {
+        if (m2lPending) hipCheck(hipStreamWaitEvent(st, evM2L, 0), "wait");
+        m2lPending = false;
+    };
     if (timed) hipCheck(hipEventRecord(ev[1], st), "event");
     for (int l = ML - 1; l >= 2; --l) {
         LevelBufs& P = levels[l];
         LevelBufs& C = levels[l + 1];
         if (!launchM2MFused<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph, p, P.thNear,
                                P.thW, P.phNear, P.phW, C.phUp<R>(), M + C.Moff, M + P.Moff, ldsMax, st, nb, fs,
-                               tuneM2MBudget, tuneMinBlocks)) {
+                               tuneMinBlocks)) {
             for (int r = 0; r < nb; ++r) {
                 launchM2MTheta<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, C.nth, C.nph, p, P.thNear,
                                   P.thW, C.phUp<R>(), M + r * fs + C.Moff, inner, st);
@@ -678,9 +700,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         }
     }
     if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
-    {
-        launchM2L<R>(M, Lc, false, st, nb);
-    }
+    launchM2L<R>(M, Lc, false, st, nb, 2, splitM2L ? ML - 1 : ML);
     if (flags & MLFMA_FLAG_FRESH_LOCALS)
         for (int r = 0; r < nb; ++r)
             for (int l = 2; l <= ML; ++l) {
@@ -691,6 +711,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     for (int l = 3; l <= ML; ++l) {
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];
+        if (l == ML) joinM2L();
         if (!launchL2LFused<R>(false, C.nl2l, C.l2lParent, C.l2lChild, C.l2lOct, P.nth, P.nph, C.nth, C.nph, P.atPhPtr,
                                P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, P.nAtTh,
                                P.phDn<R>(), Lc + P.Moff, Lc + C.Moff, ldsMax, st, nb, fs)) {
@@ -702,6 +723,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
             }
         }
     }
+    joinM2L();
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
     if (overlap)
         hipCheck(hipStreamWaitEvent(st, evJoin, 0), "wait");
@@ -730,7 +752,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
     cplx<R>* M = fieldM<R>();
     cplx<R>* Lc = fieldL<R>();
     const cplx<R>* rp = radpat<R>();
-    const bool overlap = nearOverlap != 0 && !(flags & MLFMA_FLAG_NEAR_SERIAL);
+    const bool overlap = nearOverlap != 0 && !(flags & MLFMA_FLAG_ONE_STREAM);
     if (overlap) {
         hipCheck(hipEventRecord(evFork, st), "event");
         hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
@@ -749,7 +771,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
             throw Error(MLFMA_ERR_ARG, "adjoint L2L: angular grid too large for one block");
     }
     {
-        launchM2L<R>(Lc, M, true, st, 1);  // adjoint locals in, adjoint multipoles out
+        launchM2L<R>(Lc, M, true, st, 1, 2, ML);  // adjoint locals in, adjoint multipoles out
     }
     for (int l = 2; l < ML; ++l) {
         LevelBufs& P = levels[l];
@@ -1136,21 +1158,17 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
     };
     try {
         const double before = timeWhole(10);
-        // M2M: LDS budget for staged children x block target for the row split
+        // M2M: block target for the parent row split
         double best = 1e300;
-        int bb = 0, bm = 0;
-        for (int budget : {16384, 32768, 65536})
-            for (int mb : {256, 512, 1024, 2048, 4096}) {
-                d.tuneM2MBudget = budget;
-                d.tuneMinBlocks = mb;
-                const double t = timeStage(1, 2, 3);
-                if (t < best) {
-                    best = t;
-                    bb = budget;
-                    bm = mb;
-                }
+        int bm = 0;
+        for (int mb : {256, 512, 1024, 2048, 4096}) {
+            d.tuneMinBlocks = mb;
+            const double t = timeStage(1, 2, 3);
+            if (t < best) {
+                best = t;
+                bm = mb;
             }
-        d.tuneM2MBudget = bb;
+        }
         d.tuneMinBlocks = bm;
         // M2L: sibling-group or per-target kernel, level by level
         d.tuneM2L.assign(d.hp.tree.maxLevel + 1, 0);
@@ -1169,7 +1187,7 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
             d.tuneM2L[l] = mode;
         }
         // near field beside the far-field chain, or after it
-        if (!(saved & MLFMA_FLAG_NEAR_SERIAL)) {
+        if (!(saved & MLFMA_FLAG_ONE_STREAM)) {
             double tm[3] = {0, 0, 0};
             for (int rep = 0; rep < 2; ++rep)
                 for (int m = 0; m < 3; ++m) {
@@ -1180,6 +1198,14 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
             for (int m = 0; m < 3; ++m)
                 if (tm[m] < tm[bestMode]) bestMode = m;
             d.nearOverlap = bestMode;
+            // finest-level M2L beside the M2M chain, or in line
+            double ts[2] = {0, 0};
+            for (int rep = 0; rep < 2; ++rep)
+                for (int m = 0; m < 2; ++m) {
+                    d.m2lSplit = m == 1;
+                    ts[m] += timeWhole(10);
+                }
+            d.m2lSplit = ts[1] <= ts[0];
         }
         const double after = timeWhole(10);
         if (ms_before) *ms_before = before;
@@ -1213,7 +1239,7 @@ int mlfma_plan_set_flags(mlfma_plan* plan, uint32_t flags) {
     API_BEGIN
     DevicePlan& d = P(plan);
     if (flags & ~(MLFMA_FLAG_FRESH_LOCALS | MLFMA_FLAG_STAGE_TIMES | MLFMA_FLAG_M2L_GROUP | MLFMA_FLAG_M2L_TARGET |
-                  MLFMA_FLAG_ASYNC | MLFMA_FLAG_PHYSICAL | MLFMA_FLAG_GRAPH | MLFMA_FLAG_NEAR_SERIAL))
+                  MLFMA_FLAG_ASYNC | MLFMA_FLAG_PHYSICAL | MLFMA_FLAG_GRAPH | MLFMA_FLAG_ONE_STREAM))
         throw Error(MLFMA_ERR_ARG, "unknown flag bits");
     if ((flags ^ d.flags) & MLFMA_FLAG_PHYSICAL)
         throw Error(MLFMA_ERR_ARG, "MLFMA_FLAG_PHYSICAL is fixed when the plan is created");

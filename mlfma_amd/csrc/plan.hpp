@@ -91,6 +91,9 @@ struct DevicePlan {
     hipStream_t nearStream = nullptr;  // near-field pairs beside the far-field chain
     hipEvent_t evFork = nullptr, evJoin = nullptr;
     int nearOverlap = 2;               // 0 serial, 1 fork before S2M, 2 after; chosen by mlfma_autotune
+    hipStream_t m2lStream = nullptr;   // finest-level M2L beside the M2M chain
+    hipEvent_t evLeafM = nullptr, evM2L = nullptr;
+    bool m2lSplit = true;              // chosen by mlfma_autotune
     bool timesValid = false;
     uint32_t flags = 0;  // MLFMA_FLAG_* in effect for matvecs (settable after creation)
     std::vector<void*> allocs;
@@ -137,7 +140,7 @@ struct DevicePlan {
     int batchCap = 1;  // right-hand sides the field arrays and partials hold
     // launch parameters chosen by mlfma_autotune (0: compile-time defaults /
     // the per-level rule); tuneM2L[l]: 1 sibling-group, 2 per-target M2L
-    int tuneM2MBudget = 0, tuneMinBlocks = 0;
+    int tuneMinBlocks = 0;
     std::vector<int> tuneM2L;
     // captured matvecs (MLFMA_FLAG_GRAPH), least recently used last
     struct GraphEntry {
@@ -150,7 +153,7 @@ struct DevicePlan {
     std::vector<GraphEntry> graphs;
     void clearGraphs();
     template <typename R>
-    void launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb);
+    void launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb, int lmin, int lmax);
     void ensureBatch(int nb);
     // y (=|+=) Z^H x (the adjoint of the linear operator)
     template <typename R>

@@ -295,18 +295,19 @@ def test_async_device_matvec():
     assert cases.relerr(z.cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()) == 0.0
 
 
-def test_near_overlap_matches_serial():
-    """The near-field pair products beside the far-field chain (the plan's
-    second stream, the default) give the bits of the serial order
-    (MLFMA_FLAG_NEAR_SERIAL): forward, batched and adjoint matvecs, eager,
-    queued on torch's stream and replayed from a graph."""
+def test_stream_overlap_matches_one_stream():
+    """The near-field pair products and the finest level's M2L beside the
+    far-field chain (the plan's second and third streams, the default) give
+    the bits of the in-order run (MLFMA_FLAG_ONE_STREAM): forward, batched
+    and adjoint matvecs, eager, queued on torch's stream and replayed from a
+    graph."""
     import torch
     geom, kw = cases.make("rwg_adaptive")
     base = A.MLFMA_FLAG_FRESH_LOCALS
     p = Plan(geom, cases.planewave_z(), A.make_config(device=0, flags=base, **kw))
     x = torch.from_numpy(G.complex_gaussian(geom.n, 31)).to("cuda:0")
     X = torch.from_numpy(np.stack([G.complex_gaussian(geom.n, 40 + r) for r in range(6)])).to("cuda:0")
-    p.set_flags(base | A.MLFMA_FLAG_NEAR_SERIAL)
+    p.set_flags(base | A.MLFMA_FLAG_ONE_STREAM)
     y0, Y0, z0 = (p.matvec(x).cpu().numpy(), p.matvec(X).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy())
     for f in (base, base | A.MLFMA_FLAG_ASYNC, base | A.MLFMA_FLAG_GRAPH):
         p.set_flags(f)
