@@ -53,4 +53,14 @@ __device__ __forceinline__ float wsum(float v) {
 // XCD b % 8; with a grid of 8 * per blocks XCD x gets items [x*per, (x+1)*per).
 __device__ __forceinline__ int xcdItem(int b, int per) { return (b & 7) * per + (b >> 3); }
 
+// The same for a grid of 8 * per * nch blocks where each item has nch
+// chunks: XCD x gets items [x*per, (x+1)*per) with all their chunks, so the
+// chunks of one item share that XCD's L2.
+__device__ __forceinline__ void xcdItemChunk(int b, int per, int nch, int& item, int& chunk) {
+    const int r = b >> 3;
+    const int q = r / nch;
+    item = (b & 7) * per + q;
+    chunk = r - q * nch;
+}
+
 }  // namespace mlfma

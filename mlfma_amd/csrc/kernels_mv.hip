@@ -131,15 +131,25 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
     for (int i = threadIdx.x; i < nphP * tp; i += TB) tPhW[i] = (R)phW[i];
     for (int i = threadIdx.x; i < nr; i += TB) tThNear[i] = thNear[j0 + i];
     for (int i = threadIdx.x; i < nphP; i += TB) tPhNear[i] = phNear[i];
-    const int c0 = ptr[item], c1 = ptr[item + 1];
+    const int c0 = ptr[item], nc = ptr[item + 1] - c0;  // at most 8 children
     for (int d = threadIdx.x; d < ndirC; d += TB) {
+        // every child's loads are issued before the first product
+        cplx<R> s[8], m0[8], m1[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < nc) {
+                const cplx<R>* Mb = Mc + (int64_t)child[c0 + k] * ndirC * 2;
+                s[k] = phaseUp[(int64_t)oct[c0 + k] * ndirC + d];
+                m0[k] = Mb[2 * d];
+                m1[k] = Mb[2 * d + 1];
+            }
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
-        for (int c = c0; c < c1; ++c) {
-            const cplx<R>* Mb = Mc + (int64_t)child[c] * ndirC * 2;
-            const cplx<R> s = phaseUp[(int64_t)oct[c] * ndirC + d];
-            a0 = a0 + s * Mb[2 * d];
-            a1 = a1 + s * Mb[2 * d + 1];
-        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < nc) {
+                a0 = a0 + s[k] * m0[k];
+                a1 = a1 + s[k] * m1[k];
+            }
         sS[2 * d] = a0;
         sS[2 * d + 1] = a1;
     }
@@ -220,8 +230,8 @@ __global__ __launch_bounds__(TB) void kM2LGroup(M2LGroups<R> lv) {
     while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
     b -= lv.blk[l];
     const int nch = lv.nch[l];
-    const int item = xcdItem(b / nch, lv.per[l]);
-    const int chunk = b - (b / nch) * nch;
+    int item, chunk;
+    xcdItemChunk(b, lv.per[l], nch, item, chunk);
     if (item >= lv.ngrp[l]) return;
     const int ndir = lv.ndir[l];
     const int d = chunk * TB + threadIdx.x;
@@ -294,8 +304,8 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
     while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
     b -= lv.blk[l];
     const int nch = lv.nch[l];
-    const int item = xcdItem(b / nch, lv.per[l]);
-    const int chunk = b - (b / nch) * nch;
+    int item, chunk;
+    xcdItemChunk(b, lv.per[l], nch, item, chunk);
     if (item >= lv.ntgt[l]) return;
     const int ndir = lv.ndir[l];
     const int d = chunk * TB + threadIdx.x;
@@ -354,6 +364,66 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
     }
 }
 
+// Single right-hand side: a block is (level, target box, chunk of 64
+// directions) and its four waves split the target's pair list (wave w takes
+// pairs w, w + 4, ...; two pairs in flight per wave), so eight independent
+// pairs are in flight per direction instead of a long dependent walk; the
+// wave partials are added in LDS in a fixed order.  Lanes past the level's
+// direction count read a clamped direction and store nothing.
+template <typename R, bool CONJ>
+__global__ __launch_bounds__(TB) void kM2Lw(M2LLevels<R> lv) {
+    int b = blockIdx.x, l = 0;
+    while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
+    b -= lv.blk[l];
+    const int nch = lv.nch[l];
+    int item, chunk;
+    xcdItemChunk(b, lv.per[l], nch, item, chunk);
+    if (item >= lv.ntgt[l]) return;  // uniform per block
+    const int ndir = lv.ndir[l];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int d = chunk * 64 + lane;
+    const int dc = d < ndir ? d : ndir - 1;
+    const int32_t* src = lv.src[l];
+    const int32_t* off = lv.off[l];
+    const cplx<R>* T = lv.T[l] + dc;
+    const cplx<R>* M = lv.M[l] + 2 * dc;
+    const int q0 = lv.ptr[l][item], q1 = lv.ptr[l][item + 1];
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
+    int q = q0 + w;
+    for (; q + 4 < q1; q += 8) {
+        cplx<R> t0 = T[(int64_t)off[q] * ndir];
+        cplx<R> t1 = T[(int64_t)off[q + 4] * ndir];
+        if (CONJ) {
+            t0.im = -t0.im;
+            t1.im = -t1.im;
+        }
+        const cplx<R>* m0 = M + (int64_t)src[q] * ndir * 2;
+        const cplx<R>* m1 = M + (int64_t)src[q + 4] * ndir * 2;
+        a0 = a0 + t0 * m0[0];
+        a1 = a1 + t0 * m0[1];
+        b0 = b0 + t1 * m1[0];
+        b1 = b1 + t1 * m1[1];
+    }
+    if (q < q1) {
+        cplx<R> t0 = T[(int64_t)off[q] * ndir];
+        if (CONJ) t0.im = -t0.im;
+        const cplx<R>* m0 = M + (int64_t)src[q] * ndir * 2;
+        a0 = a0 + t0 * m0[0];
+        a1 = a1 + t0 * m0[1];
+    }
+    __shared__ cplx<R> red[3][64][2];
+    if (w > 0) {
+        red[w - 1][lane][0] = a0 + b0;
+        red[w - 1][lane][1] = a1 + b1;
+    }
+    __syncthreads();
+    if (w == 0 && d < ndir) {
+        cplx<R>* out = lv.L[l] + ((int64_t)lv.tgt[l][item] * ndir + d) * 2;
+        out[0] = ((a0 + b0) + red[0][lane][0]) + (red[1][lane][0] + red[2][lane][0]);
+        out[1] = ((a1 + b1) + red[0][lane][1]) + (red[1][lane][1] + red[2][lane][1]);
+    }
+}
+
 template <typename R>
 void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb) {
     int total = 0;
@@ -364,6 +434,19 @@ void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb) {
         total += 8 * lv.per[l] * lv.nch[l];
     }
     if (total <= 0) return;
+    if (nb == 1) {
+        total = 0;
+        for (int l = 0; l < lv.nlev; ++l) {
+            lv.nch[l] = (lv.ndir[l] + 63) / 64;
+            lv.blk[l] = total;
+            total += 8 * lv.per[l] * lv.nch[l];
+        }
+        if (conjT)
+            hipLaunchKernelGGL((kM2Lw<R, true>), dim3(total), dim3(TB), 0, st, lv);
+        else
+            hipLaunchKernelGGL((kM2Lw<R, false>), dim3(total), dim3(TB), 0, st, lv);
+        return;
+    }
 #define M2L_CASE(C, B) hipLaunchKernelGGL((kM2Lx<R, C, B>), dim3(total), dim3(TB), 0, st, lv)
     if (nb == 1) { if (conjT) M2L_CASE(true, 1); else M2L_CASE(false, 1); }
     else if (nb == 2) { if (conjT) M2L_CASE(true, 2); else M2L_CASE(false, 2); }
@@ -934,8 +1017,9 @@ __global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, 
                                             cplx<double> Ck, const int64_t* __restrict__ sumPtr,
                                             const int32_t* __restrict__ sumIdx, const cplx<double>* __restrict__ part,
                                             int64_t ps, const int64_t* __restrict__ perm, cplx<double>* __restrict__ y,
-                                            int64_t xs, int accumulate) {
-    const int64_t w = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+                                            int64_t xs, int accumulate, int per) {
+    // rows of one XCD are contiguous, so a leaf's local field is read by one L2
+    const int64_t w = (int64_t)xcdItem(blockIdx.x, per) * (TB / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w >= nrows) return;  // uniform per wave
     const int64_t t = rowPos[w];
@@ -973,9 +1057,11 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
                 int64_t fs, int64_t xs, int64_t ps) {
     if (nrows <= 0) return;
     const int64_t nbk = (nrows + TB / 64 - 1) / (TB / 64);
-#define L2T_CASE(B)                                                                                              \
-    hipLaunchKernelGGL((kL2Tx<R, B>), dim3((unsigned)nbk), dim3(TB), 0, st, rowPos, rowItem, nrows, items, rp, Lc, fs, \
-                       Ck, sumPtr, sumIdx, part, ps, perm, y, xs, accumulate)
+    if (nbk > INT32_MAX / 2) throw Error(MLFMA_ERR_ARG, "too many far-field rows");
+    const int per = (int)((nbk + 7) / 8);
+#define L2T_CASE(B)                                                                                                   \
+    hipLaunchKernelGGL((kL2Tx<R, B>), dim3((unsigned)(8 * per)), dim3(TB), 0, st, rowPos, rowItem, nrows, items, rp, \
+                       Lc, fs, Ck, sumPtr, sumIdx, part, ps, perm, y, xs, accumulate, per)
     if (nb == 1) L2T_CASE(1);
     else if (nb == 2) L2T_CASE(2);
     else if (nb == 4) L2T_CASE(4);
