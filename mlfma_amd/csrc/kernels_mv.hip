@@ -724,7 +724,7 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
                                                const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
     __shared__ cplx<double> sT[NT];
     __shared__ cplx<double> sxr[TB], sxc[TB];
-    __shared__ double sre[TB], sim[TB];
+    __shared__ double sre[TB], sim[TB], scr[TB], sci[TB];
     const NearItem I = items[blockIdx.x];
     const int nr = I.nr;        // 1..256
     const int ld = nr | 1;      // odd leading dimension in LDS
@@ -798,23 +798,33 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
                 re += z.re * xx.re - z.im * xx.im;
                 im += z.re * xx.im + z.im * xx.re;
             }
-        if (t0 < cw && c0 + t0 >= I.nself) {
-            const cplx<double>* col = sT + t0 * ld;
-            double cr = 0, ci = 0, cr2 = 0, ci2 = 0;
-            int q = 0;
-            for (; q + 1 < nr; q += 2) {
-                const cplx<double> z0 = col[q], x0 = sxr[q], z1 = col[q + 1], x1 = sxr[q + 1];
-                cr += z0.re * x0.re - z0.im * x0.im;
-                ci += z0.re * x0.im + z0.im * x0.re;
-                cr2 += z1.re * x1.re - z1.im * x1.im;
-                ci2 += z1.re * x1.im + z1.im * x1.re;
+        // column sums of the pair columns: S = TB / cw lanes per column, each
+        // over a contiguous run of rows, added in LDS in a fixed order
+        if (c0 + cw > I.nself) {  // uniform over the block
+            const int S = TB / cw;
+            const int cc = t0 / S, sg = t0 - cc * S;
+            double cr = 0, ci = 0;
+            if (cc < cw && c0 + cc >= I.nself) {
+                const int L = (nr + S - 1) / S;
+                const int qa = min(nr, sg * L), qb = min(nr, qa + L);
+                const cplx<double>* col = sT + cc * ld;
+                for (int q = qa; q < qb; ++q) {
+                    const cplx<double> z0 = col[q], x0 = sxr[q];
+                    cr += z0.re * x0.re - z0.im * x0.im;
+                    ci += z0.re * x0.im + z0.im * x0.re;
+                }
             }
-            if (q < nr) {
-                const cplx<double> z0 = col[q], x0 = sxr[q];
-                cr += z0.re * x0.re - z0.im * x0.im;
-                ci += z0.re * x0.im + z0.im * x0.re;
+            scr[t0] = cr;
+            sci[t0] = ci;
+            __syncthreads();
+            if (t0 < cw && c0 + t0 >= I.nself) {
+                double a = 0, b = 0;
+                for (int q = 0; q < S; ++q) {
+                    a += scr[t0 * S + q];
+                    b += sci[t0 * S + q];
+                }
+                part[I.cpo + c0 + t0 - I.nself] = mk(a, b);
             }
-            part[I.cpo + c0 + t0 - I.nself] = mk(cr + cr2, ci + ci2);
         }
         if (!more) break;
         c0 = c0n;

@@ -98,6 +98,11 @@ static std::vector<cplx<R>> toR(const std::vector<cplx<double>>& v) {
     return o;
 }
 
+// flags of the events that fork and join the plan's side streams
+#ifndef MLFMA_SYNC_EVENT_FLAGS
+#define MLFMA_SYNC_EVENT_FLAGS hipEventDisableTiming
+#endif
+
 void DevicePlan::build() {
     HostPlan& h = hp;
     hipCheck(hipSetDevice(device), "hipSetDevice");
@@ -112,11 +117,11 @@ void DevicePlan::build() {
     // the near field runs beside the far-field chain on a second stream
     // (default priority), forked and joined with events
     hipCheck(hipStreamCreateWithFlags(&nearStream, hipStreamNonBlocking), "hipStreamCreate");
-    hipCheck(hipEventCreateWithFlags(&evFork, hipEventDisableTiming), "hipEventCreate");
-    hipCheck(hipEventCreateWithFlags(&evJoin, hipEventDisableTiming), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evFork, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evJoin, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
     hipCheck(hipStreamCreateWithFlags(&m2lStream, hipStreamNonBlocking), "hipStreamCreate");
-    hipCheck(hipEventCreateWithFlags(&evLeafM, hipEventDisableTiming), "hipEventCreate");
-    hipCheck(hipEventCreateWithFlags(&evM2L, hipEventDisableTiming), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evLeafM, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evM2L, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
     const int64_t n = h.src.n;
     const Tree& t = h.tree;
     const int ML = t.maxLevel;
