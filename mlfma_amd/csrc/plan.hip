@@ -62,7 +62,7 @@ DevicePlan::~DevicePlan() {
     if (m2lStream) (void)hipStreamSynchronize(m2lStream);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {evFork, evJoin, evLeafM, evM2L})
+    for (hipEvent_t e : {evFork, evJoin, evM2L, evNear})
         if (e) (void)hipEventDestroy(e);
     if (nearStream) (void)hipStreamDestroy(nearStream);
     if (m2lStream) (void)hipStreamDestroy(m2lStream);
@@ -120,8 +120,8 @@ void DevicePlan::build() {
     hipCheck(hipEventCreateWithFlags(&evFork, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
     hipCheck(hipEventCreateWithFlags(&evJoin, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
     hipCheck(hipStreamCreateWithFlags(&m2lStream, hipStreamNonBlocking), "hipStreamCreate");
-    hipCheck(hipEventCreateWithFlags(&evLeafM, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
     hipCheck(hipEventCreateWithFlags(&evM2L, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evNear, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
     const int64_t n = h.src.n;
     const Tree& t = h.tree;
     const int ML = t.maxLevel;
@@ -663,31 +663,45 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // latency-bound M2M/M2L/L2L only); serial when stages are timed
     const bool serial = timed || (flags & MLFMA_FLAG_ONE_STREAM);
     const int overlap = serial ? 0 : nearOverlap;
-    auto forkNear = [&]() {
-        hipCheck(hipEventRecord(evFork, st), "event");
-        hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
-        launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, nearStream, nb, xs, ps);
-        hipCheck(hipEventRecord(evJoin, nearStream), "event");
-    };
-    if (overlap == 1) forkNear();
-    if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
-    launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
-    if (overlap == 2) forkNear();
     // M2L of the finest level needs only the leaf multipoles, and its locals
     // are first read by the L2L into that level: it runs on its own stream
     // beside the M2M chain and the coarser M2L
     const bool splitM2L = !serial && m2lSplit && ML >= 2 && levels[ML].ntgt > 0;
-    bool m2lPending = false;
-    if (splitM2L) {
-        hipCheck(hipEventRecord(evLeafM, st), "event");
-        hipCheck(hipStreamWaitEvent(m2lStream, evLeafM, 0), "wait");
-        launchM2L<R>(M, Lc, false, m2lStream, nb, ML, ML);
-        hipCheck(hipEventRecord(evM2L, m2lStream), "event");
-        m2lPending = true;
+    // Each event record or wait on st costs the chain a few microseconds, so
+    // both side streams fork from one event (evFork after S2M; the near field
+    // forked before S2M has its own) and join through one: the M2L stream
+    // waits for the near field before it records evM2L.
+    auto launchNear = [&](hipStream_t s) {
+        launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, s, nb, xs, ps);
+    };
+    if (overlap == 1) {
+        hipCheck(hipEventRecord(evNear, st), "event");
+        hipCheck(hipStreamWaitEvent(nearStream, evNear, 0), "wait");
+        launchNear(nearStream);
+        hipCheck(hipEventRecord(evJoin, nearStream), "event");
     }
-    auto joinM2L = [&]() {
-        if (m2lPending) hipCheck(hipStreamWaitEvent(st, evM2L, 0), "wait");
-        m2lPending = false;
+    if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
+    launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
+    if (overlap == 2 || splitM2L) hipCheck(hipEventRecord(evFork, st), "event");
+    if (overlap == 2) {
+        hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
+        launchNear(nearStream);
+        hipCheck(hipEventRecord(evJoin, nearStream), "event");
+    }
+    if (splitM2L) {
+        hipCheck(hipStreamWaitEvent(m2lStream, evFork, 0), "wait");
+        launchM2L<R>(M, Lc, false, m2lStream, nb, ML, ML);
+        if (overlap) hipCheck(hipStreamWaitEvent(m2lStream, evJoin, 0), "wait");
+        hipCheck(hipEventRecord(evM2L, m2lStream), "event");
+    }
+    // st waits once for the side streams: before the L2L into the finest
+    // level when M2L was split (the near field is joined through it), else
+    // before the near-field sums
+    bool sidePending = splitM2L || overlap;
+    auto joinSide = [&]() {
+        if (!sidePending) return;
+        hipCheck(hipStreamWaitEvent(st, splitM2L ? evM2L : evJoin, 0), "wait");
+        sidePending = false;
     };
     if (timed) hipCheck(hipEventRecord(ev[1], st), "event");
     for (int l = ML - 1; l >= 2; --l) {
@@ -716,7 +730,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     for (int l = 3; l <= ML; ++l) {
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];
-        if (l == ML) joinM2L();
+        if (l == ML) joinSide();
         if (!launchL2LFused<R>(false, C.nl2l, C.l2lParent, C.l2lChild, C.l2lOct, P.nth, P.nph, C.nth, C.nph, P.atPhPtr,
                                P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, P.nAtTh,
                                P.phDn<R>(), Lc + P.Moff, Lc + C.Moff, ldsMax, st, nb, fs)) {
@@ -728,12 +742,9 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
             }
         }
     }
-    joinM2L();
+    joinSide();
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
-    if (overlap)
-        hipCheck(hipStreamWaitEvent(st, evJoin, 0), "wait");
-    else
-        launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st, nb, xs, ps);
+    if (!overlap) launchNear(st);
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");

@@ -92,7 +92,7 @@ struct DevicePlan {
     hipEvent_t evFork = nullptr, evJoin = nullptr;
     int nearOverlap = 2;               // 0 serial, 1 fork before S2M, 2 after; chosen by mlfma_autotune
     hipStream_t m2lStream = nullptr;   // finest-level M2L beside the M2M chain
-    hipEvent_t evLeafM = nullptr, evM2L = nullptr;
+    hipEvent_t evM2L = nullptr, evNear = nullptr;
     bool m2lSplit = true;              // chosen by mlfma_autotune
     bool timesValid = false;
     uint32_t flags = 0;  // MLFMA_FLAG_* in effect for matvecs (settable after creation)
