@@ -784,7 +784,9 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
         if (!launchL2LAdj<R>(C.nl2lPar, C.l2lParSlot, C.l2lParPtr, C.l2lParChild, C.l2lParOct, P.nth, P.nph, C.nth,
                              C.nph, p, P.thNear, P.thW, P.phNear, P.phW, P.phDn<R>(), Lc + C.Moff, Lc + P.Moff, ldsMax,
                              st, tuneMinBlocks))
-            throw Error(MLFMA_ERR_ARG, "adjoint L2L: angular grid too large for one block");
+            throw Error(MLFMA_ERR_ARG, "adjoint of the L2L into level " + std::to_string(l) +
+                                           ": the level's fields exceed the LDS of one block (the adjoint has no "
+                                           "two-pass path; use fewer digits or smaller leaf boxes)");
     }
     {
         launchM2L<R>(Lc, M, true, st, 1, 2, ML);  // adjoint locals in, adjoint multipoles out
@@ -795,7 +797,9 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
         if (!launchL2LFused<R>(true, P.nm2mChild, P.m2mAdjParent, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph,
                                P.atPhPtr, P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip,
                                P.nAtTh, C.phUp<R>(), M + P.Moff, M + C.Moff, ldsMax, st))
-            throw Error(MLFMA_ERR_ARG, "adjoint M2M: angular grid too large for one block");
+            throw Error(MLFMA_ERR_ARG, "adjoint of the M2M into level " + std::to_string(l) +
+                                           ": the level's fields exceed the LDS of one block (the adjoint has no "
+                                           "two-pass path; use fewer digits or smaller leaf boxes)");
     }
     if (overlap)
         hipCheck(hipStreamWaitEvent(st, evJoin, 0), "wait");
