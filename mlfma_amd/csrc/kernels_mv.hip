@@ -59,7 +59,19 @@ __global__ void kS2Mx(const LeafItem* __restrict__ items, int nitems, int per, c
         cplx<R> a[NB][2];
 #pragma unroll
         for (int r = 0; r < NB; ++r) a[r][0] = a[r][1] = mk<R>(0, 0);
-        for (int q = 0; q < it.cnt; ++q) {
+        int q = 0;
+        for (; q + 1 < it.cnt; q += 2) {  // two source rows' pattern loads in flight
+            const int64_t s0 = perm[it.sbeg + q], s1 = perm[it.sbeg + q + 1];
+            const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
+            const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                const cplx<R> x0 = cast<R>(x[r * xs + s0]), x1 = cast<R>(x[r * xs + s1]);
+                a[r][0] = a[r][0] + x0 * r00 + x1 * r10;
+                a[r][1] = a[r][1] + x0 * r01 + x1 * r11;
+            }
+        }
+        if (q < it.cnt) {
             const int64_t src = perm[it.sbeg + q];
             const cplx<R> r0 = p[q * stride], r1 = p[q * stride + 1];
 #pragma unroll
@@ -230,8 +242,10 @@ __global__ __launch_bounds__(TB) void kM2LGroup(M2LGroups<R> lv) {
     while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
     b -= lv.blk[l];
     const int nch = lv.nch[l];
-    int item, chunk;
-    xcdItemChunk(b, lv.per[l], nch, item, chunk);
+    // chunks of one item on different XCDs (measured faster for these two
+    // kernels, whose batched fields do not fit one XCD's L2 anyway)
+    const int item = xcdItem(b / nch, lv.per[l]);
+    const int chunk = b - (b / nch) * nch;
     if (item >= lv.ngrp[l]) return;
     const int ndir = lv.ndir[l];
     const int d = chunk * TB + threadIdx.x;
@@ -304,8 +318,10 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
     while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
     b -= lv.blk[l];
     const int nch = lv.nch[l];
-    int item, chunk;
-    xcdItemChunk(b, lv.per[l], nch, item, chunk);
+    // chunks of one item on different XCDs (measured faster for these two
+    // kernels, whose batched fields do not fit one XCD's L2 anyway)
+    const int item = xcdItem(b / nch, lv.per[l]);
+    const int chunk = b - (b / nch) * nch;
     if (item >= lv.ntgt[l]) return;
     const int ndir = lv.ndir[l];
     const int d = chunk * TB + threadIdx.x;
