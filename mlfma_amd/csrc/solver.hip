@@ -2,6 +2,7 @@
 // far field and the dense direct matvec.  Reference: src/solver.cpp:5-146.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -43,23 +44,23 @@ __device__ __forceinline__ void blockSum2(double& a, double& b) {
     }
 }
 
-// partial[i * nb + block] = sum over the block's rows of conj(Q_i) w, i < ncol
+// partial[i * nb + block] = sum over the block's rows of conj(Q_i) w; grid
+// (row blocks, columns): one column per block, so the ncol reductions run in
+// parallel instead of one after another
 __global__ void kDotPartial(const cplx<double>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ w,
                             int64_t n, cplx<double>* __restrict__ partial) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const int nb = gridDim.x;
-    cplx<double> wv{0, 0};
-    if (r < n) wv = w[r];
-    for (int i = 0; i < ncol; ++i) {
-        double re = 0, im = 0;
-        if (r < n) {
-            const cplx<double> q = Q[(int64_t)i * ld + r];
-            re = q.re * wv.re + q.im * wv.im;
-            im = q.re * wv.im - q.im * wv.re;
-        }
-        blockSum2(re, im);
-        if (threadIdx.x == 0) partial[(int64_t)i * nb + blockIdx.x] = cplx<double>{re, im};
+    const int i = blockIdx.y;
+    double re = 0, im = 0;
+    if (r < n) {
+        const cplx<double> wv = w[r];
+        const cplx<double> q = Q[(int64_t)i * ld + r];
+        re = q.re * wv.re + q.im * wv.im;
+        im = q.re * wv.im - q.im * wv.re;
     }
+    blockSum2(re, im);
+    if (threadIdx.x == 0) partial[(int64_t)i * nb + blockIdx.x] = cplx<double>{re, im};
 }
 
 // h[i] (+)= sum_b partial[i * nb + b]; one block per column
@@ -103,6 +104,60 @@ __global__ void kUpdate(const cplx<double>* __restrict__ Q, int64_t ld, int ncol
     }
 }
 
+// The same three kernels for nrhs right-hand sides at once (grid z or y = rhs):
+// basis vectors of right-hand side r at Q + r n (column stride ld), w at
+// W + r n, partials at partial + r ps, coefficients at h + r hs.
+__global__ void kDotPartialB(const cplx<double>* __restrict__ Q, int64_t ld, const cplx<double>* __restrict__ W,
+                             int64_t n, cplx<double>* __restrict__ partial, int64_t ps) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    const int nb = gridDim.x, i = blockIdx.y, k = blockIdx.z;
+    double re = 0, im = 0;
+    if (r < n) {
+        const cplx<double> wv = W[(int64_t)k * n + r];
+        const cplx<double> q = Q[(int64_t)i * ld + (int64_t)k * n + r];
+        re = q.re * wv.re + q.im * wv.im;
+        im = q.re * wv.im - q.im * wv.re;
+    }
+    blockSum2(re, im);
+    if (threadIdx.x == 0) partial[(int64_t)k * ps + (int64_t)i * nb + blockIdx.x] = cplx<double>{re, im};
+}
+
+__global__ void kReduceB(const cplx<double>* __restrict__ partial, int64_t ps, int nb, cplx<double>* __restrict__ h,
+                         int64_t hs) {
+    const int i = blockIdx.x, k = blockIdx.y;
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < nb; b += KB) {
+        re += partial[(int64_t)k * ps + (int64_t)i * nb + b].re;
+        im += partial[(int64_t)k * ps + (int64_t)i * nb + b].im;
+    }
+    blockSum2(re, im);
+    if (threadIdx.x == 0) h[(int64_t)k * hs + i] = cplx<double>{re, im};
+}
+
+__global__ void kUpdateB(const cplx<double>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ h,
+                         int64_t hs, cplx<double>* __restrict__ W, int64_t n, cplx<double>* __restrict__ partial,
+                         int64_t ps, int norm) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    const int k = blockIdx.y;
+    double s2 = 0;
+    if (r < n) {
+        cplx<double> a = W[(int64_t)k * n + r];
+        for (int i = 0; i < ncol; ++i) {
+            const cplx<double> q = Q[(int64_t)i * ld + (int64_t)k * n + r];
+            const cplx<double> c = h[(int64_t)k * hs + i];
+            a.re -= q.re * c.re - q.im * c.im;
+            a.im -= q.re * c.im + q.im * c.re;
+        }
+        W[(int64_t)k * n + r] = a;
+        s2 = a.re * a.re + a.im * a.im;
+    }
+    if (norm) {
+        double z = 0;
+        blockSum2(s2, z);
+        if (threadIdx.x == 0) partial[(int64_t)k * ps + blockIdx.x] = cplx<double>{s2, 0.0};
+    }
+}
+
 // out = in / s (s real, on the host)
 __global__ void kScale(const cplx<double>* __restrict__ in, double s, cplx<double>* __restrict__ out, int64_t n) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
@@ -129,6 +184,65 @@ __global__ void kSub(const cplx<double>* __restrict__ b, const cplx<double>* __r
                      int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x;
     if (i < n) r[i] = cplx<double>{b[i].re - a[i].re, b[i].im - a[i].im};
+}
+
+// ---- GMRES(m) step on the device (no host round trip per iteration)
+__device__ __forceinline__ cplx<double> cmul(cplx<double> a, cplx<double> b) {
+    return cplx<double>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx<double> cconj(cplx<double> a) { return cplx<double>{a.re, -a.im}; }
+
+// Column j of the Hessenberg matrix from the two CGS passes (hbuf: pass 1 at
+// [0, ncol), |w|^2 at [ncol], pass 2 at [ncol + 1, 2 ncol + 1)), rotated by the
+// previous Givens rotations, the new rotation, and the rotated right-hand
+// side g; stat[3 j .. 3 j + 2] = (|g_{j+1}| / bn, |w|, rho), also mirrored to
+// the host-mapped statHost.  Single thread: j + 1 <= restart length.
+__global__ void kGivens(const cplx<double>* __restrict__ hbuf, int j, double bn, cplx<double>* __restrict__ cs,
+                        cplx<double>* __restrict__ sn, cplx<double>* __restrict__ g, cplx<double>* __restrict__ H,
+                        int ldH, double* __restrict__ stat, double* __restrict__ statHost) {
+    if (threadIdx.x != 0) return;
+    const int ncol = j + 1;
+    cplx<double>* h = H + (int64_t)j * ldH;
+    for (int i = 0; i < ncol; ++i)
+        h[i] = cplx<double>{hbuf[i].re + hbuf[ncol + 1 + i].re, hbuf[i].im + hbuf[ncol + 1 + i].im};
+    const double hn = sqrt(hbuf[ncol].re);
+    h[ncol] = cplx<double>{hn, 0.0};
+    for (int i = 0; i < j; ++i) {
+        const cplx<double> a = cmul(cconj(cs[i]), h[i]), b = cmul(cconj(sn[i]), h[i + 1]);
+        const cplx<double> c = cmul(sn[i], h[i]), e = cmul(cs[i], h[i + 1]);
+        h[i + 1] = cplx<double>{-c.re + e.re, -c.im + e.im};
+        h[i] = cplx<double>{a.re + b.re, a.im + b.im};
+    }
+    const double rho = sqrt(h[j].re * h[j].re + h[j].im * h[j].im + h[ncol].re * h[ncol].re + h[ncol].im * h[ncol].im);
+    cs[j] = cplx<double>{h[j].re / rho, h[j].im / rho};
+    sn[j] = cplx<double>{h[ncol].re / rho, h[ncol].im / rho};
+    h[j] = cplx<double>{rho, 0.0};
+    h[ncol] = cplx<double>{0.0, 0.0};
+    const cplx<double> gj = g[j];
+    const cplx<double> t = cmul(sn[j], gj);
+    g[j + 1] = cplx<double>{-t.re, -t.im};
+    g[j] = cmul(cconj(cs[j]), gj);
+    const double est = hypot(g[j + 1].re, g[j + 1].im) / bn;
+    stat[3 * j] = est;
+    stat[3 * j + 1] = hn;
+    stat[3 * j + 2] = rho;
+    statHost[3 * j] = est;
+    statHost[3 * j + 1] = hn;
+    statHost[3 * j + 2] = rho;
+}
+
+// out = in / *s (s on the device)
+__global__ void kScaleDev(const cplx<double>* __restrict__ in, const double* __restrict__ s,
+                         cplx<double>* __restrict__ out, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    const double v = *s;
+    if (r < n) out[r] = cplx<double>{in[r].re / v, in[r].im / v};
+}
+
+// g = (beta, 0, ..., 0), m + 1 entries
+__global__ void kInitG(cplx<double>* __restrict__ g, double beta, int m1) {
+    const int i = blockIdx.x * KB + threadIdx.x;
+    if (i < m1) g[i] = cplx<double>{i == 0 ? beta : 0.0, 0.0};
 }
 
 // Krylov workspace and helpers bound to one plan
@@ -161,7 +275,7 @@ struct Krylov {
     void orthogonalize(const cplx<double>* Q, int ncol, cplx<double>* w, std::vector<std::complex<double>>& hout,
                        double& wnorm) {
         for (int pass = 0; pass < 2; ++pass) {
-            hipLaunchKernelGGL(kDotPartial, dim3(nb), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)w, n, partial);
+            hipLaunchKernelGGL(kDotPartial, dim3(nb, ncol), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)w, n, partial);
             hipLaunchKernelGGL(kReduce, dim3(ncol), dim3(KB), 0, st, partial, nb, h + (pass ? ncol + 1 : 0), 0);
             hipLaunchKernelGGL(kUpdate, dim3(nb), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)(h + (pass ? ncol + 1 : 0)),
                                w, n, partial + (size_t)nb * (ncol + 1), pass);
@@ -179,7 +293,7 @@ struct Krylov {
     // columns Q + i ld; readOrtho() after a stream sync returns h and |w|
     void queueOrtho(const cplx<double>* Q, int64_t ld, int ncol, cplx<double>* w, cplx<double>* hhost) {
         for (int pass = 0; pass < 2; ++pass) {
-            hipLaunchKernelGGL(kDotPartial, dim3(nb), dim3(KB), 0, st, Q, ld, ncol, (const cplx<double>*)w, n, partial);
+            hipLaunchKernelGGL(kDotPartial, dim3(nb, ncol), dim3(KB), 0, st, Q, ld, ncol, (const cplx<double>*)w, n, partial);
             hipLaunchKernelGGL(kReduce, dim3(ncol), dim3(KB), 0, st, partial, nb, h + (pass ? ncol + 1 : 0), 0);
             hipLaunchKernelGGL(kUpdate, dim3(nb), dim3(KB), 0, st, Q, ld, ncol,
                                (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), w, n,
@@ -188,6 +302,17 @@ struct Krylov {
         hipLaunchKernelGGL(kReduce, dim3(1), dim3(KB), 0, st, partial + (size_t)nb * (ncol + 1), nb, h + ncol, 0);
         hipCheck(hipMemcpyAsync(hhost, h, (2 * ncol + 1) * sizeof(cplx<double>), hipMemcpyDeviceToHost, st),
                  "h copy");
+    }
+    // the two CGS passes against Q[:, 0..ncol) into the device column h
+    void queueOrthoDev(const cplx<double>* Q, int ncol, cplx<double>* w) {
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL(kDotPartial, dim3(nb, ncol), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)w, n, partial);
+            hipLaunchKernelGGL(kReduce, dim3(ncol), dim3(KB), 0, st, partial, nb, h + (pass ? ncol + 1 : 0), 0);
+            hipLaunchKernelGGL(kUpdate, dim3(nb), dim3(KB), 0, st, Q, n, ncol,
+                               (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), w, n,
+                               partial + (size_t)nb * (ncol + 1), pass);
+        }
+        hipLaunchKernelGGL(kReduce, dim3(1), dim3(KB), 0, st, partial + (size_t)nb * (ncol + 1), nb, h + ncol, 0);
     }
     static void readOrtho(const cplx<double>* hh, int ncol, std::vector<std::complex<double>>& hout, double& wnorm) {
         hout.resize(ncol);
@@ -296,6 +421,36 @@ static SolveOut gmresCompat(DevicePlan& d, Krylov& K, const cplx<double>* b, cpl
 // Restarted GMRES(m) with unitary complex Givens rotations; convergence is
 // checked on the true residual at every restart.  With fp32 the far field of
 // every matvec runs in FP32 (used as the inner solver of the refinement loop).
+// Device state of one pipelined GMRES(m) cycle: rotations, rotated
+// right-hand side, rotated Hessenberg columns, per-step statistics (device
+// and host-mapped) and one event per step in flight.
+constexpr int kWindow = 4;
+struct GivensWork {
+    DevicePlan& d;
+    cplx<double>*cs = nullptr, *sn = nullptr, *g = nullptr, *H = nullptr;
+    double* stat = nullptr;      // device
+    double* statHost = nullptr;  // host-mapped (pinned)
+    double* statDev = nullptr;   // device address of statHost
+    std::array<hipEvent_t, kWindow> ev{};
+    GivensWork(DevicePlan& dp, int m) : d(dp) {
+        cs = (cplx<double>*)d.alloc((size_t)(m + 1) * sizeof(cplx<double>));
+        sn = (cplx<double>*)d.alloc((size_t)(m + 1) * sizeof(cplx<double>));
+        g = (cplx<double>*)d.alloc((size_t)(m + 2) * sizeof(cplx<double>));
+        H = (cplx<double>*)d.alloc((size_t)(m + 1) * (m + 1) * sizeof(cplx<double>));
+        stat = (double*)d.alloc((size_t)3 * (m + 1) * sizeof(double));
+        hipCheck(hipHostMalloc((void**)&statHost, (size_t)3 * (m + 1) * sizeof(double), hipHostMallocMapped),
+                 "hipHostMalloc");
+        hipCheck(hipHostGetDevicePointer((void**)&statDev, statHost, 0), "hipHostGetDevicePointer");
+        for (auto& e : ev) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    }
+    ~GivensWork() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (statHost) (void)hipHostFree(statHost);
+        for (void* p : {(void*)cs, (void*)sn, (void*)g, (void*)H, (void*)stat}) d.release(p);
+    }
+};
+
 static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit,
                              double tol, bool fp32, bool x0zero, double* hist, int histOff) {
     SolveOut out;
@@ -303,6 +458,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
     m = std::max(1, std::min(m, maxit));
     cplx<double>* Q = (cplx<double>*)d.alloc((size_t)n * (m + 1) * sizeof(cplx<double>));
     cplx<double>* w = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    GivensWork G(d, m);
     const double bn = K.norm(b);
     if (!(bn > 0) || !std::isfinite(bn)) {
         hipCheck(hipMemsetAsync(x, 0, n * sizeof(cplx<double>), K.st), "zero x");
@@ -330,43 +486,60 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         out.est = res;
         if (res <= tol || total >= maxit) break;
         K.scale(w, beta, Q);
-        std::vector<std::vector<zc>> H;
-        std::vector<zc> g(m + 1, 0.0), cs(m), sn(m);
-        g[0] = beta;
-        int j = 0;
-        for (; j < m && total < maxit; ++j) {
-            d.applySolveOp(Q + (size_t)j * n, w, 0, K.st, fp32, false);
-            std::vector<zc> h;
-            double hn;
-            K.orthogonalize(Q, j + 1, w, h, hn);
-            h.push_back(hn);
-            for (int i = 0; i < j; ++i) {
-                const zc a = std::conj(cs[i]) * h[i] + std::conj(sn[i]) * h[i + 1];
-                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-                h[i] = a;
+        // Pipelined cycle: each step (matvec, two CGS passes, the Givens update
+        // and the scaling of the next basis vector) is queued without a host
+        // round trip; up to kWindow steps are in flight while the host reads
+        // the residual estimates of finished steps (host-mapped) and stops
+        // queueing at the first one that meets the stopping test.  Steps queued
+        // beyond it only append columns, so the solution uses exactly the
+        // columns the synchronous algorithm would.
+        hipLaunchKernelGGL(kInitG, dim3((m + 1 + KB - 1) / KB), dim3(KB), 0, K.st, G.g, beta, m + 1);
+        int jq = 0, jc = 0, ncolUsed = -1;
+        while (true) {
+            while (ncolUsed < 0 && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
+                d.applySolveOp(Q + (size_t)jq * n, w, 0, K.st, fp32, false);
+                K.queueOrthoDev(Q, jq + 1, w);
+                hipLaunchKernelGGL(kGivens, dim3(1), dim3(64), 0, K.st, (const cplx<double>*)K.h, jq, bn, G.cs, G.sn,
+                                   G.g, G.H, m + 1, G.stat, G.statDev);
+                hipLaunchKernelGGL(kScaleDev, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<double>*)w,
+                                   (const double*)(G.stat + 3 * jq + 1), Q + (size_t)(jq + 1) * n, n);
+                hipCheck(hipEventRecord(G.ev[jq % kWindow], K.st), "event");
+                ++jq;
             }
-            const double rho = std::sqrt(std::norm(h[j]) + std::norm(h[j + 1]));
+            if (jc == jq) break;
+            hipCheck(hipEventSynchronize(G.ev[jc % kWindow]), "gmres step");
+            const double est = G.statHost[3 * jc], hn = G.statHost[3 * jc + 1], rho = G.statHost[3 * jc + 2];
             if (rho == 0) {
                 out.status = MLFMA_ERR_BREAKDOWN;
+                ncolUsed = jc;
                 break;
             }
-            cs[j] = h[j] / rho;
-            sn[j] = h[j + 1] / rho;
-            h[j] = rho;
-            h[j + 1] = 0.0;
-            H.push_back(h);
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = std::conj(cs[j]) * g[j];
             ++total;
-            const double est = std::abs(g[j + 1]) / bn;
             if (hist) hist[histOff + total - 1] = est;
+            ++jc;
             if (est <= tol || hn == 0) {
-                ++j;
+                ncolUsed = jc;
                 break;
             }
-            K.scale(w, hn, Q + (size_t)(j + 1) * n);
+            if (total >= maxit) {
+                ncolUsed = jc;
+                break;
+            }
         }
+        if (ncolUsed < 0) ncolUsed = jc;
+        hipCheck(hipStreamSynchronize(K.st), "gmres cycle");  // steps queued past the stop
+        const int j = ncolUsed;
         if (j > 0) {
+            std::vector<cplx<double>> Hh((size_t)j * (m + 1)), gh(j);
+            hipCheck(hipMemcpy(Hh.data(), G.H, Hh.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost), "H copy");
+            hipCheck(hipMemcpy(gh.data(), G.g, gh.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost), "g copy");
+            std::vector<std::vector<zc>> H(j);
+            for (int c = 0; c < j; ++c) {
+                H[c].resize(c + 1);
+                for (int i = 0; i <= c; ++i) H[c][i] = zc(Hh[(size_t)c * (m + 1) + i].re, Hh[(size_t)c * (m + 1) + i].im);
+            }
+            std::vector<zc> g(j);
+            for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
             const std::vector<zc> y = backSolve(H, g, j);
             K.combine(Q, j, y, x, 1);
         }
@@ -399,6 +572,42 @@ static void applyMany(DevicePlan& d, const cplx<double>* x, cplx<double>* y, int
 // and unitary Givens rotations as in gmresRestart, one host sync per step
 // for all of them.  A right-hand side that has converged stops updating
 // (its lane of the block is zero) until the others finish the cycle.
+// CGS2 of nrhs right-hand sides at once: one launch per kernel for all of
+// them; hhost receives, per right-hand side at stride hs, the layout of
+// Krylov::queueOrtho (pass 1, |w|^2, pass 2).
+struct BatchOrtho {
+    DevicePlan& d;
+    int nrhs, nb;
+    int64_t n, ps, hs;
+    cplx<double>*partial = nullptr, *h = nullptr;
+    BatchOrtho(DevicePlan& dp, int nr, int m) : d(dp), nrhs(nr), n(dp.hp.src.n) {
+        nb = (int)((n + KB - 1) / KB);
+        ps = (int64_t)nb * (m + 3);
+        hs = 2 * (int64_t)m + 3;
+        partial = (cplx<double>*)d.alloc((size_t)nrhs * ps * sizeof(cplx<double>));
+        h = (cplx<double>*)d.alloc((size_t)nrhs * hs * sizeof(cplx<double>));
+    }
+    ~BatchOrtho() {
+        d.release(partial);
+        d.release(h);
+    }
+    void queue(const cplx<double>* Q, int64_t ld, int ncol, cplx<double>* W, cplx<double>* hhost, hipStream_t st) {
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL(kDotPartialB, dim3(nb, ncol, nrhs), dim3(KB), 0, st, Q, ld, (const cplx<double>*)W, n,
+                               partial, ps);
+            hipLaunchKernelGGL(kReduceB, dim3(ncol, nrhs), dim3(KB), 0, st, (const cplx<double>*)partial, ps, nb,
+                               h + (pass ? ncol + 1 : 0), hs);
+            hipLaunchKernelGGL(kUpdateB, dim3(nb, nrhs), dim3(KB), 0, st, Q, ld, ncol,
+                               (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), hs, W, n,
+                               partial + (size_t)nb * (ncol + 1), ps, pass);
+        }
+        hipLaunchKernelGGL(kReduceB, dim3(1, nrhs), dim3(KB), 0, st, (const cplx<double>*)(partial + (size_t)nb * (ncol + 1)),
+                           ps, nb, h + ncol, hs);
+        hipCheck(hipMemcpyAsync(hhost, h, (size_t)nrhs * hs * sizeof(cplx<double>), hipMemcpyDeviceToHost, st),
+                 "h copy");
+    }
+};
+
 static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                         const cplx<double>* B, cplx<double>* X, int nrhs, int m, int maxit,
                                         const std::vector<double>& tolv, double* hist, bool fp32 = false,
@@ -409,7 +618,8 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
     const int64_t ld = (int64_t)nrhs * n;
     cplx<double>* Q = (cplx<double>*)d.alloc((size_t)ld * (m + 1) * sizeof(cplx<double>));
     cplx<double>* W = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
-    std::vector<cplx<double>> hh((size_t)nrhs * (2 * m + 3));
+    BatchOrtho BO(d, nrhs, m);
+    std::vector<cplx<double>> hh((size_t)nrhs * BO.hs);
     std::vector<SolveOut> out(nrhs);
     std::vector<double> bn(nrhs);
     std::vector<char> done(nrhs, 0);
@@ -454,9 +664,8 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         bool breakdown = false;
         for (int j = 0; j < m && total < maxit; ++j) {
             applyMany(d, Q + (size_t)j * ld, W, nrhs, st, fp32);
-            for (int r = 0; r < nrhs; ++r)
-                if (active[r])
-                    K[r]->queueOrtho(Q + (size_t)r * n, ld, j + 1, W + (size_t)r * n, hh.data() + (size_t)r * (2 * m + 3));
+            // all lanes at once (a finished lane's basis vectors are zero)
+            BO.queue(Q, ld, j + 1, W, hh.data(), st);
             hipCheck(hipStreamSynchronize(st), "arnoldi");
             bool any = false;
             for (int r = 0; r < nrhs; ++r) {
@@ -467,7 +676,7 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
                 }
                 std::vector<zc> h;
                 double hn;
-                Krylov::readOrtho(hh.data() + (size_t)r * (2 * m + 3), j + 1, h, hn);
+                Krylov::readOrtho(hh.data() + (size_t)r * BO.hs, j + 1, h, hn);
                 h.push_back(hn);
                 for (int i = 0; i < j; ++i) {
                     const zc a = std::conj(cs[r][i]) * h[i] + std::conj(sn[r][i]) * h[i + 1];
