@@ -186,6 +186,72 @@ __global__ void kSub(const cplx<double>* __restrict__ b, const cplx<double>* __r
     if (i < n) r[i] = cplx<double>{b[i].re - a[i].re, b[i].im - a[i].im};
 }
 
+// ---- CGS for the pipelined GMRES(m): wide dot products (RPT rows per
+// thread, grid (row blocks, columns)) and an update kernel that reduces the
+// dot partials itself (one wave per coefficient, fixed order), so a CGS pass
+// is two launches
+constexpr int RPT = 8;
+
+__global__ void kDotWide(const cplx<double>* __restrict__ Q, int64_t ld, const cplx<double>* __restrict__ w, int64_t n,
+                         cplx<double>* __restrict__ partial) {
+    const int nbd = gridDim.x, i = blockIdx.y;
+    double re = 0, im = 0;
+    for (int k = 0; k < RPT; ++k) {
+        const int64_t r = ((int64_t)blockIdx.x * RPT + k) * KB + threadIdx.x;
+        if (r < n) {
+            const cplx<double> wv = w[r];
+            const cplx<double> q = Q[(int64_t)i * ld + r];
+            re += q.re * wv.re + q.im * wv.im;
+            im += q.re * wv.im - q.im * wv.re;
+        }
+    }
+    blockSum2(re, im);
+    if (threadIdx.x == 0) partial[(int64_t)i * nbd + blockIdx.x] = cplx<double>{re, im};
+}
+
+// h_i = sum_b partial[i nbd + b] for i < ncol (block 0 also stores them to
+// hout), then w -= Q h; normPart[block] = sum |w|^2 of the block's rows when
+// norm != 0.  Dynamic LDS: ncol complex.
+__global__ void kUpdateR(const cplx<double>* __restrict__ Q, int64_t ld, int ncol,
+                         const cplx<double>* __restrict__ partial, int nbd, cplx<double>* __restrict__ hout,
+                         cplx<double>* __restrict__ w, int64_t n, double* __restrict__ normPart, int norm) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double>* sh = reinterpret_cast<cplx<double>*>(smem);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = wv; i < ncol; i += KB / 64) {
+        double re = 0, im = 0;
+        for (int b = lane; b < nbd; b += 64) {
+            re += partial[(int64_t)i * nbd + b].re;
+            im += partial[(int64_t)i * nbd + b].im;
+        }
+        re = waveSum(re);
+        im = waveSum(im);
+        if (lane == 0) {
+            sh[i] = cplx<double>{re, im};
+            if (blockIdx.x == 0) hout[i] = cplx<double>{re, im};
+        }
+    }
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    double s2 = 0;
+    if (r < n) {
+        cplx<double> a = w[r];
+        for (int i = 0; i < ncol; ++i) {
+            const cplx<double> q = Q[(int64_t)i * ld + r];
+            const cplx<double> c = sh[i];
+            a.re -= q.re * c.re - q.im * c.im;
+            a.im -= q.re * c.im + q.im * c.re;
+        }
+        w[r] = a;
+        s2 = a.re * a.re + a.im * a.im;
+    }
+    if (norm) {
+        double z = 0;
+        blockSum2(s2, z);
+        if (threadIdx.x == 0) normPart[blockIdx.x] = s2;
+    }
+}
+
 // ---- GMRES(m) step on the device (no host round trip per iteration)
 __device__ __forceinline__ cplx<double> cmul(cplx<double> a, cplx<double> b) {
     return cplx<double>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
@@ -193,19 +259,24 @@ __device__ __forceinline__ cplx<double> cmul(cplx<double> a, cplx<double> b) {
 __device__ __forceinline__ cplx<double> cconj(cplx<double> a) { return cplx<double>{a.re, -a.im}; }
 
 // Column j of the Hessenberg matrix from the two CGS passes (hbuf: pass 1 at
-// [0, ncol), |w|^2 at [ncol], pass 2 at [ncol + 1, 2 ncol + 1)), rotated by the
+// [0, ncol), pass 2 at [ncol + 1, 2 ncol + 1); |w|^2 summed from the nbn
+// per-block partials of the second pass), rotated by the
 // previous Givens rotations, the new rotation, and the rotated right-hand
 // side g; stat[3 j .. 3 j + 2] = (|g_{j+1}| / bn, |w|, rho), also mirrored to
-// the host-mapped statHost.  Single thread: j + 1 <= restart length.
-__global__ void kGivens(const cplx<double>* __restrict__ hbuf, int j, double bn, cplx<double>* __restrict__ cs,
-                        cplx<double>* __restrict__ sn, cplx<double>* __restrict__ g, cplx<double>* __restrict__ H,
-                        int ldH, double* __restrict__ stat, double* __restrict__ statHost) {
+// the host-mapped statHost.  One block; the rotations run on thread 0.
+__global__ void kGivens(const cplx<double>* __restrict__ hbuf, const double* __restrict__ normPart, int nbn, int j,
+                        double bn, cplx<double>* __restrict__ cs, cplx<double>* __restrict__ sn,
+                        cplx<double>* __restrict__ g, cplx<double>* __restrict__ H, int ldH, double* __restrict__ stat,
+                        double* __restrict__ statHost) {
+    double s2 = 0, z = 0;
+    for (int b = threadIdx.x; b < nbn; b += KB) s2 += normPart[b];
+    blockSum2(s2, z);
     if (threadIdx.x != 0) return;
     const int ncol = j + 1;
     cplx<double>* h = H + (int64_t)j * ldH;
     for (int i = 0; i < ncol; ++i)
         h[i] = cplx<double>{hbuf[i].re + hbuf[ncol + 1 + i].re, hbuf[i].im + hbuf[ncol + 1 + i].im};
-    const double hn = sqrt(hbuf[ncol].re);
+    const double hn = sqrt(s2);
     h[ncol] = cplx<double>{hn, 0.0};
     for (int i = 0; i < j; ++i) {
         const cplx<double> a = cmul(cconj(cs[i]), h[i]), b = cmul(cconj(sn[i]), h[i + 1]);
@@ -302,17 +373,6 @@ struct Krylov {
         hipLaunchKernelGGL(kReduce, dim3(1), dim3(KB), 0, st, partial + (size_t)nb * (ncol + 1), nb, h + ncol, 0);
         hipCheck(hipMemcpyAsync(hhost, h, (2 * ncol + 1) * sizeof(cplx<double>), hipMemcpyDeviceToHost, st),
                  "h copy");
-    }
-    // the two CGS passes against Q[:, 0..ncol) into the device column h
-    void queueOrthoDev(const cplx<double>* Q, int ncol, cplx<double>* w) {
-        for (int pass = 0; pass < 2; ++pass) {
-            hipLaunchKernelGGL(kDotPartial, dim3(nb, ncol), dim3(KB), 0, st, Q, n, ncol, (const cplx<double>*)w, n, partial);
-            hipLaunchKernelGGL(kReduce, dim3(ncol), dim3(KB), 0, st, partial, nb, h + (pass ? ncol + 1 : 0), 0);
-            hipLaunchKernelGGL(kUpdate, dim3(nb), dim3(KB), 0, st, Q, n, ncol,
-                               (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), w, n,
-                               partial + (size_t)nb * (ncol + 1), pass);
-        }
-        hipLaunchKernelGGL(kReduce, dim3(1), dim3(KB), 0, st, partial + (size_t)nb * (ncol + 1), nb, h + ncol, 0);
     }
     static void readOrtho(const cplx<double>* hh, int ncol, std::vector<std::complex<double>>& hout, double& wnorm) {
         hout.resize(ncol);
@@ -432,7 +492,15 @@ struct GivensWork {
     double* statHost = nullptr;  // host-mapped (pinned)
     double* statDev = nullptr;   // device address of statHost
     std::array<hipEvent_t, kWindow> ev{};
+    cplx<double>* dotPart = nullptr;  // wide dot partials, (m + 1) x nbd
+    double* normPart = nullptr;       // |w|^2 per update block
+    int nbd = 0, nbu = 0;
     GivensWork(DevicePlan& dp, int m) : d(dp) {
+        const int64_t n = d.hp.src.n;
+        nbu = (int)((n + KB - 1) / KB);
+        nbd = (int)((n + (int64_t)KB * RPT - 1) / ((int64_t)KB * RPT));
+        dotPart = (cplx<double>*)d.alloc((size_t)nbd * (m + 1) * sizeof(cplx<double>));
+        normPart = (double*)d.alloc((size_t)nbu * sizeof(double));
         cs = (cplx<double>*)d.alloc((size_t)(m + 1) * sizeof(cplx<double>));
         sn = (cplx<double>*)d.alloc((size_t)(m + 1) * sizeof(cplx<double>));
         g = (cplx<double>*)d.alloc((size_t)(m + 2) * sizeof(cplx<double>));
@@ -447,7 +515,8 @@ struct GivensWork {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (statHost) (void)hipHostFree(statHost);
-        for (void* p : {(void*)cs, (void*)sn, (void*)g, (void*)H, (void*)stat}) d.release(p);
+        for (void* p : {(void*)cs, (void*)sn, (void*)g, (void*)H, (void*)stat, (void*)dotPart, (void*)normPart})
+            d.release(p);
     }
 };
 
@@ -498,9 +567,17 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         while (true) {
             while (ncolUsed < 0 && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
                 d.applySolveOp(Q + (size_t)jq * n, w, 0, K.st, fp32, false);
-                K.queueOrthoDev(Q, jq + 1, w);
-                hipLaunchKernelGGL(kGivens, dim3(1), dim3(64), 0, K.st, (const cplx<double>*)K.h, jq, bn, G.cs, G.sn,
-                                   G.g, G.H, m + 1, G.stat, G.statDev);
+                const int ncol = jq + 1;
+                for (int pass = 0; pass < 2; ++pass) {  // CGS2
+                    hipLaunchKernelGGL(kDotWide, dim3(G.nbd, ncol), dim3(KB), 0, K.st, (const cplx<double>*)Q, n,
+                                       (const cplx<double>*)w, n, G.dotPart);
+                    hipLaunchKernelGGL(kUpdateR, dim3(G.nbu), dim3(KB), ncol * sizeof(cplx<double>), K.st,
+                                       (const cplx<double>*)Q, n, ncol, (const cplx<double>*)G.dotPart, G.nbd,
+                                       K.h + (pass ? ncol + 1 : 0), w, n, G.normPart, pass);
+                }
+                hipLaunchKernelGGL(kGivens, dim3(1), dim3(KB), 0, K.st, (const cplx<double>*)K.h,
+                                   (const double*)G.normPart, G.nbu, jq, bn, G.cs, G.sn, G.g, G.H, m + 1, G.stat,
+                                   G.statDev);
                 hipLaunchKernelGGL(kScaleDev, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<double>*)w,
                                    (const double*)(G.stat + 3 * jq + 1), Q + (size_t)(jq + 1) * n, n);
                 hipCheck(hipEventRecord(G.ev[jq % kWindow], K.st), "event");
