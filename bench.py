@@ -138,10 +138,12 @@ def physical_solve(geom, pw, kw, radius=2.0, restart=50, maxit=3000):
     t0 = time.time()
     p = Plan(geom, pw, A.make_config(device=0, flags=A.MLFMA_FLAG_PHYSICAL, **kw))
     setup = time.time() - t0
+    p.autotune(1)  # launch parameters for single right-hand sides (not timed)
     I, rep = p.solve(method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=1e-6)
     log("physical GMRES", rep["iterations"], rep["status"], round(rep["seconds"], 3), "s")
     # the same with the FP32 far field inside FP64 iterative refinement
     p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, flags=A.MLFMA_FLAG_PHYSICAL, **kw))
+    p32.autotune(1)
     I32, rep32 = p32.solve(method=A.MLFMA_FGMRES_IR, restart=restart, maxit=maxit, tol=1e-6)
     log("physical FP32-IR", rep32["iterations"], rep32["status"], round(rep32["seconds"], 3), "s")
     mixed = {"method": "FP64 refinement, FP32 far field, MLFMA_FLAG_PHYSICAL", "iterations": rep32["iterations"],
@@ -152,18 +154,20 @@ def physical_solve(geom, pw, kw, radius=2.0, restart=50, maxit=3000):
     angles = np.radians(np.arange(8) * 180.0 / 7.0)
     B = np.stack([-p.excitation(A.make_planewave(1.0, (np.cos(t), 0.0, -np.sin(t)), (np.sin(t), 0.0, np.cos(t)),
                                                  2 * np.pi)) for t in angles])
-    tb = time.perf_counter()
-    Xb, repb = p.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6)
-    t_batch = time.perf_counter() - tb
-    log("physical batched sweep", round(t_batch, 3), "s")
     ts = time.perf_counter()
     singles = []
     for j in range(8):
         singles.append(p.solve(B[j], method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=1e-6))
         log("physical single", j, singles[-1][1]["iterations"])
     t_single = time.perf_counter() - ts
+    p.autotune(8)  # launch parameters for batches of 8 (not timed)
+    tb = time.perf_counter()
+    Xb, repb = p.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6)
+    t_batch = time.perf_counter() - tb
+    log("physical batched sweep", round(t_batch, 3), "s")
     # batched and mixed precision together: FP64 refinement around batched
     # FP32-far-field GMRES, against the plain FP64 single solves above
+    p32.autotune(8)
     tm = time.perf_counter()
     Xm, repm = p32.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6, method=A.MLFMA_FGMRES_IR)
     t_mixed = time.perf_counter() - tm

@@ -302,6 +302,72 @@ __global__ void kGivens(const cplx<double>* __restrict__ hbuf, const double* __r
     statHost[3 * j + 2] = rho;
 }
 
+// kGivens for nrhs right-hand sides, one block each: lane k's CGS output at
+// hbuf + k hs (pass 1 at [0, ncol), |w|^2 at [ncol], pass 2 at [ncol + 1,
+// 2 ncol + 1)), its state at cs/sn + k (m + 1), g + k (m + 2), H + k (m + 1)^2,
+// stat/statHost + k 3 (m + 1).  Lanes with on[k] == 0 are skipped.
+__global__ void kGivensB(const cplx<double>* __restrict__ hbuf, int64_t hs, int j, int m,
+                         const double* __restrict__ bn, const int* __restrict__ on, cplx<double>* __restrict__ cs,
+                         cplx<double>* __restrict__ sn, cplx<double>* __restrict__ g, cplx<double>* __restrict__ H,
+                         double* __restrict__ stat, double* __restrict__ statHost) {
+    const int k = blockIdx.x;
+    if (threadIdx.x != 0 || !on[k]) return;
+    const cplx<double>* hb = hbuf + (int64_t)k * hs;
+    cplx<double>* c_ = cs + (int64_t)k * (m + 1);
+    cplx<double>* s_ = sn + (int64_t)k * (m + 1);
+    cplx<double>* g_ = g + (int64_t)k * (m + 2);
+    cplx<double>* h = H + ((int64_t)k * (m + 1) + j) * (m + 1);
+    const int ncol = j + 1;
+    for (int i = 0; i < ncol; ++i)
+        h[i] = cplx<double>{hb[i].re + hb[ncol + 1 + i].re, hb[i].im + hb[ncol + 1 + i].im};
+    const double hn = sqrt(hb[ncol].re);
+    h[ncol] = cplx<double>{hn, 0.0};
+    for (int i = 0; i < j; ++i) {
+        const cplx<double> a = cmul(cconj(c_[i]), h[i]), b = cmul(cconj(s_[i]), h[i + 1]);
+        const cplx<double> c = cmul(s_[i], h[i]), e = cmul(c_[i], h[i + 1]);
+        h[i + 1] = cplx<double>{-c.re + e.re, -c.im + e.im};
+        h[i] = cplx<double>{a.re + b.re, a.im + b.im};
+    }
+    const double rho = sqrt(h[j].re * h[j].re + h[j].im * h[j].im + h[ncol].re * h[ncol].re + h[ncol].im * h[ncol].im);
+    c_[j] = cplx<double>{h[j].re / rho, h[j].im / rho};
+    s_[j] = cplx<double>{h[ncol].re / rho, h[ncol].im / rho};
+    h[j] = cplx<double>{rho, 0.0};
+    h[ncol] = cplx<double>{0.0, 0.0};
+    const cplx<double> gj = g_[j];
+    const cplx<double> t = cmul(s_[j], gj);
+    g_[j + 1] = cplx<double>{-t.re, -t.im};
+    g_[j] = cmul(cconj(c_[j]), gj);
+    const double est = hypot(g_[j + 1].re, g_[j + 1].im) / bn[k];
+    double* st = stat + (int64_t)k * 3 * (m + 1) + 3 * j;
+    double* sh = statHost + (int64_t)k * 3 * (m + 1) + 3 * j;
+    st[0] = sh[0] = est;
+    st[1] = sh[1] = hn;
+    st[2] = sh[2] = rho;
+}
+
+// Q_{j+1} lane k = W lane k / |w| (that lane's statistic), zero for lanes off
+__global__ void kScaleB(const cplx<double>* __restrict__ W, const double* __restrict__ stat, int j, int m,
+                        const int* __restrict__ on, cplx<double>* __restrict__ Qn, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    const int k = blockIdx.y;
+    if (r >= n) return;
+    const int64_t o = (int64_t)k * n + r;
+    if (!on[k]) {
+        Qn[o] = cplx<double>{0.0, 0.0};
+        return;
+    }
+    const double v = stat[(int64_t)k * 3 * (m + 1) + 3 * j + 1];
+    Qn[o] = cplx<double>{W[o].re / v, W[o].im / v};
+}
+
+// g lanes = (beta_k, 0, ..., 0), m + 2 entries each
+__global__ void kInitGB(cplx<double>* __restrict__ g, const double* __restrict__ beta, int m2, int nrhs) {
+    const int i = blockIdx.x * KB + threadIdx.x;
+    if (i >= m2 * nrhs) return;
+    const int k = i / m2, q = i - k * m2;
+    g[i] = cplx<double>{q == 0 ? beta[k] : 0.0, 0.0};
+}
+
 // out = in / *s (s on the device)
 __global__ void kScaleDev(const cplx<double>* __restrict__ in, const double* __restrict__ s,
                          cplx<double>* __restrict__ out, int64_t n) {
@@ -680,8 +746,41 @@ struct BatchOrtho {
         }
         hipLaunchKernelGGL(kReduceB, dim3(1, nrhs), dim3(KB), 0, st, (const cplx<double>*)(partial + (size_t)nb * (ncol + 1)),
                            ps, nb, h + ncol, hs);
-        hipCheck(hipMemcpyAsync(hhost, h, (size_t)nrhs * hs * sizeof(cplx<double>), hipMemcpyDeviceToHost, st),
-                 "h copy");
+        if (hhost)
+            hipCheck(hipMemcpyAsync(hhost, h, (size_t)nrhs * hs * sizeof(cplx<double>), hipMemcpyDeviceToHost, st),
+                     "h copy");
+    }
+};
+
+// Device state of one pipelined batched GMRES(m) cycle (per right-hand side:
+// rotations, rotated right-hand side, rotated columns, step statistics).
+struct GivensBatch {
+    DevicePlan& d;
+    int nrhs, m;
+    cplx<double>*cs = nullptr, *sn = nullptr, *g = nullptr, *H = nullptr;
+    double *stat = nullptr, *statHost = nullptr, *statDev = nullptr, *bn = nullptr, *beta = nullptr;
+    int* on = nullptr;
+    std::array<hipEvent_t, kWindow> ev{};
+    GivensBatch(DevicePlan& dp, int nr, int m_) : d(dp), nrhs(nr), m(m_) {
+        cs = (cplx<double>*)d.alloc((size_t)nrhs * (m + 1) * sizeof(cplx<double>));
+        sn = (cplx<double>*)d.alloc((size_t)nrhs * (m + 1) * sizeof(cplx<double>));
+        g = (cplx<double>*)d.alloc((size_t)nrhs * (m + 2) * sizeof(cplx<double>));
+        H = (cplx<double>*)d.alloc((size_t)nrhs * (m + 1) * (m + 1) * sizeof(cplx<double>));
+        stat = (double*)d.alloc((size_t)nrhs * 3 * (m + 1) * sizeof(double));
+        bn = (double*)d.alloc((size_t)nrhs * sizeof(double));
+        beta = (double*)d.alloc((size_t)nrhs * sizeof(double));
+        on = (int*)d.alloc((size_t)nrhs * sizeof(int));
+        hipCheck(hipHostMalloc((void**)&statHost, (size_t)nrhs * 3 * (m + 1) * sizeof(double), hipHostMallocMapped),
+                 "hipHostMalloc");
+        hipCheck(hipHostGetDevicePointer((void**)&statDev, statHost, 0), "hipHostGetDevicePointer");
+        for (auto& e : ev) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    }
+    ~GivensBatch() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (statHost) (void)hipHostFree(statHost);
+        for (void* p : {(void*)cs, (void*)sn, (void*)g, (void*)H, (void*)stat, (void*)bn, (void*)beta, (void*)on})
+            d.release(p);
     }
 };
 
@@ -696,7 +795,7 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
     cplx<double>* Q = (cplx<double>*)d.alloc((size_t)ld * (m + 1) * sizeof(cplx<double>));
     cplx<double>* W = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
     BatchOrtho BO(d, nrhs, m);
-    std::vector<cplx<double>> hh((size_t)nrhs * BO.hs);
+    GivensBatch GB(d, nrhs, m);
     std::vector<SolveOut> out(nrhs);
     std::vector<double> bn(nrhs);
     std::vector<char> done(nrhs, 0);
@@ -725,75 +824,89 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
             all = all && done[r];
         }
         if (all || total >= maxit) break;
-        std::vector<std::vector<std::vector<zc>>> H(nrhs);
-        std::vector<std::vector<zc>> g(nrhs, std::vector<zc>(m + 1, 0.0)), cs(nrhs, std::vector<zc>(m)),
-            sn(nrhs, std::vector<zc>(m));
+        // Pipelined cycle, as in gmresRestart: every step (batched matvec,
+        // CGS2 of all lanes, per-lane Givens update, scaling) is queued without
+        // a host round trip, up to kWindow steps ahead of the host, which reads
+        // each lane's residual estimate from host-mapped memory and stops the
+        // lane at its first step that meets the test.  A lane's later steps
+        // only append columns; the solution uses exactly its columns up to the
+        // stop.
         std::vector<int> steps(nrhs, 0);
         std::vector<char> active(nrhs, 0);
+        std::vector<int> onh(nrhs, 0);
         for (int r = 0; r < nrhs; ++r) {
             active[r] = !done[r];
-            if (active[r]) {
-                K[r]->scale(W + (size_t)r * n, beta[r], Q + (size_t)r * n);
-                g[r][0] = beta[r];
-            } else
-                hipCheck(hipMemsetAsync(Q + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+            onh[r] = active[r];
+            if (active[r]) K[r]->scale(W + (size_t)r * n, beta[r], Q + (size_t)r * n);
+            else hipCheck(hipMemsetAsync(Q + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
         }
+        hipCheck(hipMemcpyAsync(GB.on, onh.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, st), "lanes");
+        hipCheck(hipMemcpyAsync(GB.bn, bn.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st), "norms");
+        hipCheck(hipMemcpyAsync(GB.beta, beta.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st), "betas");
+        hipLaunchKernelGGL(kInitGB, dim3(((m + 2) * nrhs + KB - 1) / KB), dim3(KB), 0, st, GB.g, (const double*)GB.beta,
+                           m + 2, nrhs);
         bool breakdown = false;
-        for (int j = 0; j < m && total < maxit; ++j) {
-            applyMany(d, Q + (size_t)j * ld, W, nrhs, st, fp32);
-            // all lanes at once (a finished lane's basis vectors are zero)
-            BO.queue(Q, ld, j + 1, W, hh.data(), st);
-            hipCheck(hipStreamSynchronize(st), "arnoldi");
+        int jq = 0, jc = 0;
+        bool stop = false;
+        while (true) {
+            while (!stop && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
+                applyMany(d, Q + (size_t)jq * ld, W, nrhs, st, fp32);
+                BO.queue(Q, ld, jq + 1, W, nullptr, st);
+                hipLaunchKernelGGL(kGivensB, dim3(nrhs), dim3(64), 0, st, (const cplx<double>*)BO.h, BO.hs, jq, m,
+                                   (const double*)GB.bn, (const int*)GB.on, GB.cs, GB.sn, GB.g, GB.H, GB.stat,
+                                   GB.statDev);
+                hipLaunchKernelGGL(kScaleB, dim3(K[0]->nb, nrhs), dim3(KB), 0, st, (const cplx<double>*)W,
+                                   (const double*)GB.stat, jq, m, (const int*)GB.on, Q + (size_t)(jq + 1) * ld, n);
+                hipCheck(hipEventRecord(GB.ev[jq % kWindow], st), "event");
+                ++jq;
+            }
+            if (jc == jq) break;
+            hipCheck(hipEventSynchronize(GB.ev[jc % kWindow]), "gmres step");
             bool any = false;
             for (int r = 0; r < nrhs; ++r) {
-                cplx<double>* qn = Q + (size_t)(j + 1) * ld + (size_t)r * n;
-                if (!active[r]) {
-                    hipCheck(hipMemsetAsync(qn, 0, n * sizeof(cplx<double>), st), "zero lane");
-                    continue;
-                }
-                std::vector<zc> h;
-                double hn;
-                Krylov::readOrtho(hh.data() + (size_t)r * BO.hs, j + 1, h, hn);
-                h.push_back(hn);
-                for (int i = 0; i < j; ++i) {
-                    const zc a = std::conj(cs[r][i]) * h[i] + std::conj(sn[r][i]) * h[i + 1];
-                    h[i + 1] = -sn[r][i] * h[i] + cs[r][i] * h[i + 1];
-                    h[i] = a;
-                }
-                const double rho = std::sqrt(std::norm(h[j]) + std::norm(h[j + 1]));
+                if (!active[r]) continue;
+                const double* sr = GB.statHost + (size_t)r * 3 * (m + 1) + 3 * jc;
+                const double est = sr[0], hn = sr[1], rho = sr[2];
                 if (rho == 0) {
                     out[r].status = MLFMA_ERR_BREAKDOWN;
                     active[r] = 0;
                     breakdown = true;
-                    hipCheck(hipMemsetAsync(qn, 0, n * sizeof(cplx<double>), st), "zero lane");
                     continue;
                 }
-                cs[r][j] = h[j] / rho;
-                sn[r][j] = h[j + 1] / rho;
-                h[j] = rho;
-                h[j + 1] = 0.0;
-                H[r].push_back(h);
-                g[r][j + 1] = -sn[r][j] * g[r][j];
-                g[r][j] = std::conj(cs[r][j]) * g[r][j];
-                steps[r] = j + 1;
-                const double est = std::abs(g[r][j + 1]) / bn[r];
+                steps[r] = jc + 1;
                 if (hist) hist[(size_t)r * histLd + histOff + total] = est;
                 ++out[r].iters;
-                if (est <= tolv[r] || hn == 0) {
+                if (est <= tolv[r] || hn == 0)
                     active[r] = 0;
-                    hipCheck(hipMemsetAsync(qn, 0, n * sizeof(cplx<double>), st), "zero lane");
-                } else {
-                    K[r]->scale(W + (size_t)r * n, hn, qn);
+                else
                     any = true;
-                }
             }
             ++total;
-            if (!any) break;
+            ++jc;
+            if (!any || total >= maxit) stop = true;
+            if (stop) break;  // steps queued past the stop are not read
         }
+        hipCheck(hipStreamSynchronize(st), "gmres cycle");
         for (int r = 0; r < nrhs; ++r)
             if (steps[r] > 0) {
-                const std::vector<zc> y = backSolve(H[r], g[r], steps[r]);
-                K[r]->combine(Q + (size_t)r * n, steps[r], y, X + (size_t)r * n, 1, ld);
+                const int j = steps[r];
+                std::vector<cplx<double>> Hh((size_t)j * (m + 1)), gh(j);
+                hipCheck(hipMemcpy(Hh.data(), GB.H + (size_t)r * (m + 1) * (m + 1), Hh.size() * sizeof(cplx<double>),
+                                   hipMemcpyDeviceToHost),
+                         "H copy");
+                hipCheck(hipMemcpy(gh.data(), GB.g + (size_t)r * (m + 2), gh.size() * sizeof(cplx<double>),
+                                   hipMemcpyDeviceToHost),
+                         "g copy");
+                std::vector<std::vector<zc>> H(j);
+                for (int c = 0; c < j; ++c) {
+                    H[c].resize(c + 1);
+                    for (int i = 0; i <= c; ++i)
+                        H[c][i] = zc(Hh[(size_t)c * (m + 1) + i].re, Hh[(size_t)c * (m + 1) + i].im);
+                }
+                std::vector<zc> g(j);
+                for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
+                const std::vector<zc> y = backSolve(H, g, j);
+                K[r]->combine(Q + (size_t)r * n, j, y, X + (size_t)r * n, 1, ld);
             }
         if (breakdown) break;
     }
