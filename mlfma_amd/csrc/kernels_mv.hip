@@ -738,11 +738,12 @@ template <bool CONJ>
 __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
                                                const int64_t* __restrict__ perm, const cplx<double>* __restrict__ val,
                                                const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
+    // 30 KiB of LDS per block (five blocks per CU): the row-sum reduction at
+    // the end reuses the tile's storage, the column sums reduce by shuffles
     __shared__ cplx<double> sT[NT];
-    __shared__ cplx<double> sxr[TB], sxc[TB];
-    __shared__ double sre[TB], sim[TB], scr[TB], sci[TB];
+    __shared__ cplx<double> sxr[128], sxc[TB];
     const NearItem I = items[blockIdx.x];
-    const int nr = I.nr;        // 1..256
+    const int nr = I.nr;        // 1..128
     const int ld = nr | 1;      // odd leading dimension in LDS
     const int cwMax = min(TB, NT / ld);
     if (threadIdx.x < nr) sxr[threadIdx.x] = x[perm[I.rbeg + threadIdx.x]];
@@ -814,13 +815,16 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
                 re += z.re * xx.re - z.im * xx.im;
                 im += z.re * xx.im + z.im * xx.re;
             }
-        // column sums of the pair columns: S = TB / cw lanes per column, each
-        // over a contiguous run of rows, added in LDS in a fixed order
+        // column sums of the pair columns: S lanes per column (a power of two
+        // <= 64, so a column's lanes sit in one wave), each over a contiguous
+        // run of rows, added by a butterfly in a fixed order
         if (c0 + cw > I.nself) {  // uniform over the block
-            const int S = TB / cw;
+            int S = 1;
+            while (S < 64 && 2 * S * cw <= TB) S *= 2;
             const int cc = t0 / S, sg = t0 - cc * S;
             double cr = 0, ci = 0;
-            if (cc < cw && c0 + cc >= I.nself) {
+            const bool mine = cc < cw && c0 + cc >= I.nself;
+            if (mine) {
                 const int L = (nr + S - 1) / S;
                 const int qa = min(nr, sg * L), qb = min(nr, qa + L);
                 const cplx<double>* col = sT + cc * ld;
@@ -830,17 +834,11 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
                     ci += z0.re * x0.im + z0.im * x0.re;
                 }
             }
-            scr[t0] = cr;
-            sci[t0] = ci;
-            __syncthreads();
-            if (t0 < cw && c0 + t0 >= I.nself) {
-                double a = 0, b = 0;
-                for (int q = 0; q < S; ++q) {
-                    a += scr[t0 * S + q];
-                    b += sci[t0 * S + q];
-                }
-                part[I.cpo + c0 + t0 - I.nself] = mk(a, b);
+            for (int o = S / 2; o > 0; o >>= 1) {  // every lane of the wave takes part
+                cr += __shfl_xor(cr, o, 64);
+                ci += __shfl_xor(ci, o, 64);
             }
+            if (mine && sg == 0) part[I.cpo + c0 + cc - I.nself] = mk(cr, ci);
         }
         if (!more) break;
         c0 = c0n;
@@ -850,6 +848,9 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
 #undef NEAR_LD
 #undef NEAR_LD1
 #undef NEAR_LD_EXTRA
+    double* sre = reinterpret_cast<double*>(sT);  // the tile's storage, free now
+    double* sim = sre + TB;
+    __syncthreads();  // every lane is done with the last tile
     sre[threadIdx.x] = (g < G) ? re : 0.0;
     sim[threadIdx.x] = (g < G) ? im : 0.0;
     __syncthreads();
