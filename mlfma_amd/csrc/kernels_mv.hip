@@ -15,7 +15,7 @@ constexpr int TB = 256;
 #define MLFMA_M2M_MIN_BLOCKS 1024    // split parent theta rows until this many blocks
 #endif
 #ifndef MLFMA_NEAR_LOADS
-#define MLFMA_NEAR_LOADS 6           // near-field values in flight per lane (6 or 8)
+#define MLFMA_NEAR_LOADS 6           // near-field values in flight per lane (5, 6 or 8)
 #endif
 
 // ------------------------------------------------------------------ S2M
@@ -731,7 +731,8 @@ void launchZeroField(cplx<R>* f, int64_t n, hipStream_t st) {
 // the pair columns, column sums Z^T x_rows.  Tiles of NT values are staged in
 // LDS with a leading dimension made odd (conflict-free column reads); the
 // partial sums go to a buffer that kNearSum folds in a fixed order.
-static_assert(MLFMA_NEAR_LOADS == 6 || MLFMA_NEAR_LOADS == 8, "near-field loads per lane: 6 or 8");
+static_assert(MLFMA_NEAR_LOADS == 5 || MLFMA_NEAR_LOADS == 6 || MLFMA_NEAR_LOADS == 8,
+              "near-field loads per lane: 5, 6 or 8");
 constexpr int NT = MLFMA_NEAR_LOADS * TB;  // values per tile
 
 template <bool CONJ>
@@ -750,7 +751,7 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     const int G = TB / nr;
     const int r = threadIdx.x % nr, g = threadIdx.x / nr;
     double re = 0, im = 0;
-    // Software pipeline: the six loads per lane of tile k+1 are issued before
+    // Software pipeline: the loads per lane of tile k+1 are issued before
     // the LDS work of tile k.  Values are held as plain doubles (a struct or
     // array here is left in scratch memory by the compiler); indices are
     // clamped into the tile so every load is valid.
@@ -758,7 +759,10 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     int c0 = 0;
     int cw = min(cwMax, I.nc);
     int ne = cw * nr;
-    double r0, r1, r2, r3, r4, r5, i0, i1, i2, i3, i4, i5;
+    double r0, r1, r2, r3, r4, i0, i1, i2, i3, i4;
+#if MLFMA_NEAR_LOADS >= 6
+    double r5, i5;
+#endif
 #if MLFMA_NEAR_LOADS == 8
     double r6, r7, i6, i7;
 #endif
@@ -773,12 +777,14 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     {                                                                                               \
         const double* sd = reinterpret_cast<const double*>(val + I.voff + (int64_t)(C0) * nr);      \
         NEAR_LD1(0, C0, NE) NEAR_LD1(1, C0, NE) NEAR_LD1(2, C0, NE)                                 \
-        NEAR_LD1(3, C0, NE) NEAR_LD1(4, C0, NE) NEAR_LD1(5, C0, NE)                                 \
+        NEAR_LD1(3, C0, NE) NEAR_LD1(4, C0, NE)                                                     \
         NEAR_LD_EXTRA(C0, NE)                                                                       \
         if (t0 < (CW)) xc = x[colsIn[I.cptr + (C0) + t0]];                                          \
     }
 #if MLFMA_NEAR_LOADS == 8
-#define NEAR_LD_EXTRA(C0, NE) NEAR_LD1(6, C0, NE) NEAR_LD1(7, C0, NE)
+#define NEAR_LD_EXTRA(C0, NE) NEAR_LD1(5, C0, NE) NEAR_LD1(6, C0, NE) NEAR_LD1(7, C0, NE)
+#elif MLFMA_NEAR_LOADS == 6
+#define NEAR_LD_EXTRA(C0, NE) NEAR_LD1(5, C0, NE)
 #else
 #define NEAR_LD_EXTRA(C0, NE)
 #endif
@@ -794,7 +800,10 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
         sTd[a] = r##K;                                        \
         sTd[a + 1] = i##K;                                    \
     }
-        NEAR_ST(0) NEAR_ST(1) NEAR_ST(2) NEAR_ST(3) NEAR_ST(4) NEAR_ST(5)
+        NEAR_ST(0) NEAR_ST(1) NEAR_ST(2) NEAR_ST(3) NEAR_ST(4)
+#if MLFMA_NEAR_LOADS >= 6
+        NEAR_ST(5)
+#endif
 #if MLFMA_NEAR_LOADS == 8
         NEAR_ST(6) NEAR_ST(7)
 #endif
