@@ -98,3 +98,86 @@ def test_batched_solve_matches_single_and_dense():
     for r in range(8):
         y = q.matvec(X2[r])
         assert cases.relerr(y, B2[r]) <= 2e-8, (r, reps2[r])
+
+
+def _gmres_restart_numpy(Z, b, m, maxit, tol):
+    """GMRES(m) from zero on a dense matrix (CGS2, unitary Givens rotations,
+    restarts with the true residual), returning x, the iteration count and
+    the residual estimate of every iteration: the synchronous algorithm the
+    device solver pipelines."""
+    n = len(b)
+    x = np.zeros(n, complex)
+    bn = np.linalg.norm(b)
+    hist, total = [], 0
+    while True:
+        r = b - Z @ x if total else b.copy()
+        beta = np.linalg.norm(r)
+        if beta / bn <= tol or total >= maxit:
+            return x, total, np.array(hist)
+        Q = [r / beta]
+        H = np.zeros((m + 1, m), complex)
+        cs, sn = np.zeros(m, complex), np.zeros(m, complex)
+        g = np.zeros(m + 1, complex)
+        g[0] = beta
+        j = 0
+        stop = False
+        while j < m and total < maxit:
+            w = Z @ Q[j]
+            h = np.zeros(j + 2, complex)
+            for _ in range(2):
+                c = np.array([np.vdot(q, w) for q in Q])
+                w = w - sum(ci * q for ci, q in zip(c, Q))
+                h[:j + 1] += c
+            hn = np.linalg.norm(w)
+            h[j + 1] = hn
+            for i in range(j):
+                a = np.conj(cs[i]) * h[i] + np.conj(sn[i]) * h[i + 1]
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1]
+                h[i] = a
+            rho = np.sqrt(abs(h[j]) ** 2 + abs(h[j + 1]) ** 2)
+            cs[j], sn[j] = h[j] / rho, h[j + 1] / rho
+            h[j], h[j + 1] = rho, 0
+            H[:j + 2, j] = h
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = np.conj(cs[j]) * g[j]
+            total += 1
+            est = abs(g[j + 1]) / bn
+            hist.append(est)
+            j += 1
+            if est <= tol or hn == 0:
+                stop = True
+                break
+            Q.append(w / hn)
+        y = np.linalg.solve(np.triu(H[:j, :j]), g[:j])
+        x = x + sum(yi * q for yi, q in zip(y, Q[:j]))
+
+
+def test_pipelined_gmres_matches_synchronous_algorithm():
+    """The device GMRES(m) queues steps ahead of its stopping test; its
+    iteration count and residual history must be those of the synchronous
+    algorithm on the same operator (physical RWG sphere, restart 30: several
+    cycles and stops in the middle of one), single and batched."""
+    geom, kw = cases.make("rwg_fixed")
+    p = Plan(geom, cases.planewave_z(),
+             A.make_config(device=0, **dict(kw, flags=A.MLFMA_FLAG_PHYSICAL | A.MLFMA_FLAG_FRESH_LOCALS)))
+    n = geom.n
+    Z = np.stack([p.matvec(np.eye(n)[j].astype(complex)) for j in range(n)], 1)
+    m, maxit = 30, 2000
+    mid = False
+    for seed, tol in ((4, 1e-4), (5, 1e-6), (6, 1e-7)):
+        b = G.complex_gaussian(n, seed)
+        xr, itr, hr = _gmres_restart_numpy(Z, b, m, maxit, tol)
+        x, rep = p.solve(b, method=A.MLFMA_GMRES_RESTART, restart=m, maxit=maxit, tol=tol, history=True)
+        print("pipelined", rep["iterations"], "synchronous", itr, flush=True)
+        assert rep["iterations"] == itr and itr < maxit
+        mid = mid or itr % m != 0
+        assert np.allclose(rep["history"], hr, rtol=1e-6, atol=0)
+        assert cases.relerr(x, xr) <= 1e-9
+    assert mid  # at least one stop inside a cycle
+    B = np.stack([G.complex_gaussian(n, 60 + r) for r in range(3)])
+    X, reps = p.solve_batch(B, restart=m, maxit=maxit, tol=1e-6, history=True)
+    for r in range(3):
+        xr, itr, hr = _gmres_restart_numpy(Z, B[r], m, maxit, 1e-6)
+        assert reps[r]["iterations"] == itr, (r, reps[r]["iterations"], itr)
+        assert np.allclose(reps[r]["history"][:itr], hr, rtol=1e-6, atol=0)
+        assert cases.relerr(X[r], xr) <= 1e-9
