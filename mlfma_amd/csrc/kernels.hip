@@ -744,14 +744,20 @@ __global__ void kL2LPhi(const int32_t* __restrict__ parent, const int32_t* __res
     const int ith = e / nphC, jph = e - ith * nphC;
     const int ndirP = mthP * mphP;
     const cplx<R>* Lb = Lp + (int64_t)parent[item] * ndirP * 2;
-    const cplx<R>* ph = phaseDn + (int64_t)oct[item] * ndirP;
+    // phaseDn == nullptr: no phase (the first pass of the adjoint of M2M)
+    const cplx<R>* ph = phaseDn ? phaseDn + (int64_t)oct[item] * ndirP : nullptr;
     cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
     for (int q = atPtr[jph]; q < atPtr[jph + 1]; ++q) {
         const int d = ith * mphP + atSrc[q];
         const R w = (R)atW[q];
-        const cplx<R> s = ph[d];
-        a0 = a0 + w * (s * Lb[2 * d]);
-        a1 = a1 + w * (s * Lb[2 * d + 1]);
+        if (ph) {
+            const cplx<R> s = ph[d];
+            a0 = a0 + w * (s * Lb[2 * d]);
+            a1 = a1 + w * (s * Lb[2 * d + 1]);
+        } else {
+            a0 = a0 + w * Lb[2 * d];
+            a1 = a1 + w * Lb[2 * d + 1];
+        }
     }
     cplx<R>* o = inner + ((int64_t)item * mthP * nphC + e) * 2;
     o[0] = a0;
@@ -774,7 +780,8 @@ template <typename R>
 __global__ void kL2LTheta(const int32_t* __restrict__ child, int mthP, int nthC, int nphC,
                           const int32_t* __restrict__ atPtr, const int32_t* __restrict__ atSrc,
                           const double* __restrict__ atW, const int32_t* __restrict__ atFlip,
-                          const cplx<R>* __restrict__ inner, cplx<R>* __restrict__ Lc) {
+                          const cplx<R>* __restrict__ inner, cplx<R>* __restrict__ Lc,
+                          const cplx<R>* __restrict__ postPhase, const int32_t* __restrict__ oct) {
     const int item = blockIdx.y;
     const int e = blockIdx.x * BS + threadIdx.x;
     if (e >= nthC * nphC) return;
@@ -789,6 +796,12 @@ __global__ void kL2LTheta(const int32_t* __restrict__ child, int mthP, int nthC,
         a0 = a0 + w * in[2 * d];
         a1 = a1 + w * in[2 * d + 1];
     }
+    if (postPhase) {  // the adjoint of M2M: conj of the child-grid up-phase
+        cplx<R> s = postPhase[(int64_t)oct[item] * nthC * nphC + e];
+        s.im = -s.im;
+        a0 = s * a0;
+        a1 = s * a1;
+    }
     cplx<R>* o = Lc + ((int64_t)child[item] * nthC * nphC + e) * 2;
     o[0] = o[0] + a0;
     o[1] = o[1] + a1;
@@ -797,11 +810,11 @@ __global__ void kL2LTheta(const int32_t* __restrict__ child, int mthP, int nthC,
 template <typename R>
 void launchL2LTheta(int nitems, const int32_t* child, int mthP, int nthC, int nphC, const int32_t* atPtr,
                     const int32_t* atSrc, const double* atW, const int32_t* atFlip, const cplx<R>* inner, cplx<R>* Lc,
-                    hipStream_t st) {
+                    hipStream_t st, const cplx<R>* postPhase, const int32_t* oct) {
     if (nitems <= 0) return;
     dim3 g((nthC * nphC + BS - 1) / BS, nitems);
     hipLaunchKernelGGL(kL2LTheta<R>, g, dim3(BS), 0, st, child, mthP, nthC, nphC, atPtr, atSrc, atW, atFlip, inner,
-                       Lc);
+                       Lc, postPhase, oct);
 }
 
 // Dense matvec over all pairs with the near-field kernel (the reference's
@@ -835,7 +848,8 @@ void launchDirect(const SrcDev& s, const cplx<double>* xt, int64_t n, double k, 
                                   const int32_t*, const double*, const cplx<R>*, const cplx<R>*, cplx<R>*,        \
                                   hipStream_t);                                                                   \
     template void launchL2LTheta<R>(int, const int32_t*, int, int, int, const int32_t*, const int32_t*,            \
-                                    const double*, const int32_t*, const cplx<R>*, cplx<R>*, hipStream_t);
+                                    const double*, const int32_t*, const cplx<R>*, cplx<R>*, hipStream_t,         \
+                                    const cplx<R>*, const int32_t*);
 INST(double)
 INST(float)
 #undef INST

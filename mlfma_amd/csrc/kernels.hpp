@@ -121,7 +121,8 @@ void launchL2LPhi(int nitems, const int32_t* parent, const int32_t* oct, int mth
 template <typename R>
 void launchL2LTheta(int nitems, const int32_t* child, int mthP, int nthC, int nphC, const int32_t* atPtr,
                     const int32_t* atSrc, const double* atW, const int32_t* atFlip, const cplx<R>* inner,
-                    cplx<R>* Lc, hipStream_t st);
+                    cplx<R>* Lc, hipStream_t st, const cplx<R>* postPhase = nullptr,
+                    const int32_t* oct = nullptr);  // postPhase: conj(postPhase[oct]) applied before the sum
 void launchDirect(const SrcDev& s, const cplx<double>* xt, int64_t n, double k, cplx<double> Ck, cplx<double>* yt,
                   hipStream_t st);
 

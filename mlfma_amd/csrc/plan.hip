@@ -400,6 +400,8 @@ void DevicePlan::build() {
                 L.nm2mChild = (int)ap.size();
             }
             innerLen = std::max<int64_t>(innerLen, (int64_t)L.nstem * L.nth * levels[l + 1].nph * 2);
+            // the two-pass adjoint of M2M holds one anterpolated field per child
+            innerLen = std::max<int64_t>(innerLen, (int64_t)L.nm2mChild * L.nth * levels[l + 1].nph * 2);
         }
         // M2L targets with a non-empty interaction list; stale boxes otherwise
         std::vector<int32_t> tgt, ptr{0}, sr, of, stale;
@@ -796,10 +798,16 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
         LevelBufs& C = levels[l + 1];
         if (!launchL2LFused<R>(true, P.nm2mChild, P.m2mAdjParent, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph,
                                P.atPhPtr, P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip,
-                               P.nAtTh, C.phUp<R>(), M + P.Moff, M + C.Moff, ldsMax, st))
-            throw Error(MLFMA_ERR_ARG, "adjoint of the M2M into level " + std::to_string(l) +
-                                           ": the level's fields exceed the LDS of one block (the adjoint has no "
-                                           "two-pass path; use fewer digits or smaller leaf boxes)");
+                               P.nAtTh, C.phUp<R>(), M + P.Moff, M + C.Moff, ldsMax, st)) {
+            // two passes through global memory when the parent grid does not
+            // fit one block's LDS: phi anterpolation of the parent per child,
+            // then theta anterpolation, conj child-grid phase and the sum
+            cplx<R>* inner = scratch<R>();
+            launchL2LPhi<R>(P.nm2mChild, P.m2mAdjParent, P.mOct, P.nth, P.nph, C.nph, P.atPhPtr, P.atPhSrc, P.atPhW,
+                            nullptr, M + P.Moff, inner, st);
+            launchL2LTheta<R>(P.nm2mChild, P.mChild, P.nth, C.nth, C.nph, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip,
+                              inner, M + C.Moff, st, C.phUp<R>(), P.mOct);
+        }
     }
     if (overlap)
         hipCheck(hipStreamWaitEvent(st, evJoin, 0), "wait");
