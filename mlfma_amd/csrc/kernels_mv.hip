@@ -735,9 +735,11 @@ static_assert(MLFMA_NEAR_LOADS == 5 || MLFMA_NEAR_LOADS == 6 || MLFMA_NEAR_LOADS
               "near-field loads per lane: 5, 6 or 8");
 constexpr int NT = MLFMA_NEAR_LOADS * TB;  // values per tile
 
-template <bool CONJ>
+// V: the stored value type (double; float for the FP32 inner solves of the
+// mixed-precision path, products and sums still in FP64)
+template <bool CONJ, typename V>
 __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
-                                               const int64_t* __restrict__ perm, const cplx<double>* __restrict__ val,
+                                               const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
                                                const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
     // 30 KiB of LDS per block (five blocks per CU): the row-sum reduction at
     // the end reuses the tile's storage, the column sums reduce by shuffles
@@ -770,12 +772,12 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
 #define NEAR_LD1(K, C0, NE)                                                       \
     {                                                                            \
         const int q = 2 * min(t0 + K * TB, (NE) - 1);                            \
-        r##K = sd[q];                                                            \
-        i##K = CONJ ? -sd[q + 1] : sd[q + 1];                                    \
+        r##K = (double)sd[q];                                                    \
+        i##K = CONJ ? -(double)sd[q + 1] : (double)sd[q + 1];                    \
     }
 #define NEAR_LD(C0, NE, CW)                                                                          \
     {                                                                                               \
-        const double* sd = reinterpret_cast<const double*>(val + I.voff + (int64_t)(C0) * nr);      \
+        const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)(C0) * nr);                \
         NEAR_LD1(0, C0, NE) NEAR_LD1(1, C0, NE) NEAR_LD1(2, C0, NE)                                 \
         NEAR_LD1(3, C0, NE) NEAR_LD1(4, C0, NE)                                                     \
         NEAR_LD_EXTRA(C0, NE)                                                                       \
@@ -879,9 +881,9 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
 constexpr int NTB = 4 * TB;
 constexpr int CWB = 64;
 
-template <bool CONJ, int NB>
+template <bool CONJ, int NB, typename V>
 __global__ __launch_bounds__(TB) void kNearSymB(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
-                                                const int64_t* __restrict__ perm, const cplx<double>* __restrict__ val,
+                                                const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
                                                 const cplx<double>* __restrict__ x, int64_t xs,
                                                 cplx<double>* __restrict__ part, int64_t ps) {
     __shared__ cplx<double> sT[NTB];
@@ -906,7 +908,7 @@ __global__ __launch_bounds__(TB) void kNearSymB(const NearItem* __restrict__ ite
     for (int c0 = 0; c0 < I.nc; c0 += cwMax) {
         const int cw = min(cwMax, I.nc - c0);
         const int ne = cw * nr;
-        const double* sd = reinterpret_cast<const double*>(val + I.voff + (int64_t)c0 * nr);
+        const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)c0 * nr);
         // four loads per lane in flight, held as plain doubles
         const int q0 = 2 * min(t0, ne - 1), q1 = 2 * min(t0 + TB, ne - 1);
         const int q2 = 2 * min(t0 + 2 * TB, ne - 1), q3 = 2 * min(t0 + 3 * TB, ne - 1);
@@ -1002,17 +1004,19 @@ __global__ __launch_bounds__(TB) void kNearSum(const int64_t* __restrict__ rows,
     }
 }
 
+template <typename V>
 void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
-                   const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb,
+                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb,
                    int64_t xs, int64_t ps) {
     if (nitems <= 0) return;
-#define NEARB_CASE(C, B) \
-    hipLaunchKernelGGL((kNearSymB<C, B>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, xs, part, ps)
+#define NEARB_CASE(C, B)                                                                                      \
+    hipLaunchKernelGGL((kNearSymB<C, B, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, xs, part, \
+                       ps)
     if (nb == 1) {
         if (conjVal)
-            hipLaunchKernelGGL(kNearSym<true>, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
+            hipLaunchKernelGGL((kNearSym<true, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
         else
-            hipLaunchKernelGGL(kNearSym<false>, dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
+            hipLaunchKernelGGL((kNearSym<false, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, part);
     } else if (nb == 2) {
         if (conjVal) NEARB_CASE(true, 2); else NEARB_CASE(false, 2);
     } else if (nb == 4) {
@@ -1022,6 +1026,19 @@ void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_
     } else
         throw Error(MLFMA_ERR_ARG, "right-hand-side batch must be 1, 2, 4 or 8");
 #undef NEARB_CASE
+}
+template void launchNearSym<double>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<double>*,
+                                    const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t);
+template void launchNearSym<float>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<float>*,
+                                   const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t);
+
+// FP32 copy of the near-field values (the mixed-precision inner solves)
+__global__ void kToFloat(const cplx<double>* __restrict__ in, cplx<float>* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n) out[i] = cplx<float>{(float)in[i].re, (float)in[i].im};
+}
+void launchToFloat(const cplx<double>* in, cplx<float>* out, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(kToFloat, dim3((unsigned)((n + TB - 1) / TB)), dim3(TB), 0, st, in, out, n);
 }
 
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,

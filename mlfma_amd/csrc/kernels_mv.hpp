@@ -70,9 +70,12 @@ bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32
 
 // near field: kNearSym writes partial sums; kL2Tx folds them into the rows of
 // far-field leaves and kNearSum into the listed remaining rows
+template <typename V>
 void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
-                   const cplx<double>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb = 1,
+                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb = 1,
                    int64_t xs = 0, int64_t ps = 0);
+// out = (float) in, n complex values
+void launchToFloat(const cplx<double>* in, cplx<float>* out, int64_t n, hipStream_t st);
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
                    const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st,
                    int nb = 1, int64_t xs = 0, int64_t ps = 0);

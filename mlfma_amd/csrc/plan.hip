@@ -551,6 +551,11 @@ void DevicePlan::build() {
     Ck = cplx<double>{Ckc.real(), Ckc.imag()};
     launchNearBuild(src, itemsByValue, nNearItems, colsTree, h.nearNnz, h.k, Ck, nearVal, stream);
     hipCheck(hipGetLastError(), "near build launch");
+    if (useFp32) {  // FP32 copy for the FP32 far-field path (entries computed in FP64)
+        nearVal32 = (cplx<float>*)alloc(std::max<int64_t>(h.nearNnz, 1) * sizeof(cplx<float>));
+        launchToFloat(nearVal, nearVal32, h.nearNnz, stream);
+        hipCheck(hipGetLastError(), "near copy launch");
+    }
     hipCheck(hipStreamSynchronize(stream), "near build");
     release(itemsByValue);
     release(colsTree);
@@ -674,7 +679,13 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // forked before S2M has its own) and join through one: the M2L stream
     // waits for the near field before it records evM2L.
     auto launchNear = [&](hipStream_t s) {
-        launchNearSym(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, s, nb, xs, ps);
+        // the FP32 far field (mixed-precision inner solves) reads the FP32
+        // copy of the near-field values
+        if (std::is_same<R, float>::value && nearVal32)
+            launchNearSym<float>(false, nearItems, nNearItems, nearColsIn, perm, nearVal32, x, nearPart, s, nb, xs,
+                                 ps);
+        else
+            launchNearSym<double>(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, s, nb, xs, ps);
     };
     if (overlap == 1) {
         hipCheck(hipEventRecord(evNear, st), "event");
@@ -774,7 +785,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
     if (overlap) {
         hipCheck(hipEventRecord(evFork, st), "event");
         hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
-        launchNearSym(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, nearStream);
+        launchNearSym<double>(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, nearStream);
         hipCheck(hipEventRecord(evJoin, nearStream), "event");
     }
     launchZeroField<R>(M, fieldLen, st);
@@ -812,7 +823,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
     if (overlap)
         hipCheck(hipStreamWaitEvent(st, evJoin, 0), "wait");
     else
-        launchNearSym(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
+        launchNearSym<double>(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st);
     const cplx<double> Ckc{Ck.re, -Ck.im};
     launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, M, Ckc, nearSumPtr, nearSumIdx, nearPart, perm, y,

@@ -121,6 +121,7 @@ struct DevicePlan {
     int64_t nNearOnlyRows = 0;
     size_t ldsMax = 65536;
     cplx<double>* nearVal = nullptr;
+    cplx<float>* nearVal32 = nullptr;  // FP32 copy (FP32 plans: the FP32 far-field path)
     cplx<double> Ck{0, 0};
     std::vector<std::complex<double>> Vin;  // excitation, input order
     cplx<double>*xt = nullptr, *yt = nullptr, *xs = nullptr, *ys = nullptr;
