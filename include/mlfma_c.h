@@ -65,7 +65,9 @@ typedef struct mlfma_planewave {
 } mlfma_planewave;
 
 #define MLFMA_FP64 0
-#define MLFMA_FP32 1         /* far-field stages in FP32 complex, FP32 accumulate */
+#define MLFMA_FP32 1         /* far-field stages in FP32 complex, FP32 accumulate;
+                                the FP32 path also reads an FP32 copy of the
+                                near-field values (FP64 products) */
 
 /* flags */
 #define MLFMA_FLAG_FRESH_LOCALS 1u  /* zero the locals of boxes with an empty
