@@ -210,7 +210,7 @@ int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t pt
 
 typedef struct mlfma_solve_opts {
     int32_t method;       /* MLFMA_GMRES_*                                          */
-    int32_t restart;      /* m for GMRES(m) / inner iterations for FGMRES           */
+    int32_t restart;      /* m for GMRES(m) / inner iterations for FGMRES; <= 2048 */
     int32_t maxit;        /* total iteration cap (reference: 500)                   */
     int32_t use_voltage;  /* 1: b = -V (reference), 0: b from the argument          */
     double tol;           /* reference: 1e-6                                        */

@@ -379,6 +379,7 @@ void DevicePlan::build() {
             const auto& P = h.m2mPtr[l];
             for (size_t s = 0; s + 1 < P.size(); ++s) {
                 if (P[s + 1] == P[s]) continue;
+                if (P[s + 1] - P[s] > 8) throw Error(MLFMA_ERR_STATE, "a box with more than eight children");
                 stem.push_back((int32_t)s);
                 for (int q = P[s]; q < P[s + 1]; ++q) {
                     ch.push_back(h.m2mChild[l][q]);

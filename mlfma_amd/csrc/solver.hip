@@ -1050,6 +1050,7 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     if (!x || !opts) throw Error(MLFMA_ERR_ARG, "null argument");
     if (!opts->use_voltage && !b) throw Error(MLFMA_ERR_ARG, "b is NULL and use_voltage = 0");
     if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
+    if (opts->restart > 2048) throw Error(MLFMA_ERR_ARG, "restart must be <= 2048");
     const int method = opts->method & 0xff;
     struct AdjointMode {  // cleared however the solve ends
         DevicePlan& d;
@@ -1144,6 +1145,7 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
         throw Error(MLFMA_ERR_ARG, "batched solves use MLFMA_GMRES_RESTART or MLFMA_FGMRES_IR");
     if (opts->method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
     if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
+    if (opts->restart > 2048) throw Error(MLFMA_ERR_ARG, "restart must be <= 2048");
     if (d.hp.staleBoxes > 0 && !(d.flags & MLFMA_FLAG_FRESH_LOCALS))
         throw Error(MLFMA_ERR_STATE, "batched solves need a linear operator (MLFMA_FLAG_FRESH_LOCALS)");
     hipCheck(hipSetDevice(d.device), "hipSetDevice");
