@@ -34,6 +34,8 @@ MLFMA_SOLVE_ADJOINT = 0x100
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_OOM", -3: "ERR_HIP", -4: "ERR_BREAKDOWN",
           -5: "ERR_NONFINITE", -6: "ERR_STATE", -7: "ERR_GEOMETRY", -8: "ERR_NOT_CONVERGED"}
+MLFMA_ERR_BREAKDOWN = -4
+MLFMA_ERR_NOT_CONVERGED = -8
 
 dptr = C.POINTER(C.c_double)
 iptr = C.POINTER(C.c_int32)

@@ -1108,7 +1108,12 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
         rep->resid_true = bn > 0 ? rn / bn : 0.0;
         rep->seconds = secs;
     }
-    return o.status == MLFMA_ERR_BREAKDOWN ? MLFMA_ERR_BREAKDOWN : MLFMA_OK;
+    if (o.status == MLFMA_ERR_BREAKDOWN) {
+        g_lastError = "GMRES breakdown: a zero rotation (the Krylov space stopped growing); x and the report hold "
+                      "the last iterate";
+        return MLFMA_ERR_BREAKDOWN;
+    }
+    return MLFMA_OK;
     API_END
 }
 
