@@ -666,9 +666,11 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     cplx<R>* inner = scratch<R>();
     const cplx<R>* rp = radpat<R>();
     // the near-field pairs depend on x only: fork them onto the near stream
-    // so they overlap the far-field chain, before S2M (nearOverlap 1) or
-    // after it (2: the bandwidth-bound pair products then run beside the
-    // latency-bound M2M/M2L/L2L only); serial when stages are timed
+    // so they overlap the far-field chain, before S2M (nearOverlap 1), after
+    // it (2: the bandwidth-bound pair products then run beside the
+    // latency-bound tree stages) or after the M2M chain (3: the M2M launches,
+    // with few blocks at coarse levels, then run uncontended); serial when
+    // stages are timed
     const bool serial = timed || (flags & MLFMA_FLAG_ONE_STREAM);
     const int overlap = serial ? 0 : nearOverlap;
     // M2L of the finest level needs only the leaf multipoles, and its locals
@@ -676,9 +678,10 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // beside the M2M chain and the coarser M2L
     const bool splitM2L = !serial && m2lSplit && ML >= 2 && levels[ML].ntgt > 0;
     // Each event record or wait on st costs the chain a few microseconds, so
-    // both side streams fork from one event (evFork after S2M; the near field
-    // forked before S2M has its own) and join through one: the M2L stream
-    // waits for the near field before it records evM2L.
+    // both side streams fork from one event (evFork after S2M, or after the
+    // M2M chain in mode 3; the near field forked before S2M has its own) and
+    // join through one: the M2L stream waits for the near field before it
+    // records evM2L.
     auto launchNear = [&](hipStream_t s) {
         // the FP32 far field (mixed-precision inner solves) reads the FP32
         // copy of the near-field values
@@ -696,18 +699,23 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     }
     if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
     launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
-    if (overlap == 2 || splitM2L) hipCheck(hipEventRecord(evFork, st), "event");
-    if (overlap == 2) {
-        hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
-        launchNear(nearStream);
-        hipCheck(hipEventRecord(evJoin, nearStream), "event");
-    }
-    if (splitM2L) {
-        hipCheck(hipStreamWaitEvent(m2lStream, evFork, 0), "wait");
-        launchM2L<R>(M, Lc, false, m2lStream, nb, ML, ML);
-        if (overlap) hipCheck(hipStreamWaitEvent(m2lStream, evJoin, 0), "wait");
-        hipCheck(hipEventRecord(evM2L, m2lStream), "event");
-    }
+    const bool nearSide = overlap == 2 || overlap == 3;
+    auto forkSides = [&]() {
+        if (!nearSide && !splitM2L) return;
+        hipCheck(hipEventRecord(evFork, st), "event");
+        if (nearSide) {
+            hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
+            launchNear(nearStream);
+            hipCheck(hipEventRecord(evJoin, nearStream), "event");
+        }
+        if (splitM2L) {
+            hipCheck(hipStreamWaitEvent(m2lStream, evFork, 0), "wait");
+            launchM2L<R>(M, Lc, false, m2lStream, nb, ML, ML);
+            if (overlap) hipCheck(hipStreamWaitEvent(m2lStream, evJoin, 0), "wait");
+            hipCheck(hipEventRecord(evM2L, m2lStream), "event");
+        }
+    };
+    if (overlap != 3) forkSides();
     // st waits once for the side streams: before the L2L into the finest
     // level when M2L was split (the near field is joined through it), else
     // before the near-field sums
@@ -732,6 +740,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
             }
         }
     }
+    if (overlap == 3) forkSides();
     if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
     launchM2L<R>(M, Lc, false, st, nb, 2, splitM2L ? ML - 1 : ML);
     if (flags & MLFMA_FLAG_FRESH_LOCALS)
@@ -1228,14 +1237,14 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
         }
         // near field beside the far-field chain, or after it
         if (!(saved & MLFMA_FLAG_ONE_STREAM)) {
-            double tm[3] = {0, 0, 0};
+            double tm[4] = {0, 0, 0, 0};
             for (int rep = 0; rep < 2; ++rep)
-                for (int m = 0; m < 3; ++m) {
+                for (int m = 0; m < 4; ++m) {
                     d.nearOverlap = m;
                     tm[m] += timeWhole(10);
                 }
             int bestMode = 2;
-            for (int m = 0; m < 3; ++m)
+            for (int m = 0; m < 4; ++m)
                 if (tm[m] < tm[bestMode]) bestMode = m;
             d.nearOverlap = bestMode;
             // finest-level M2L beside the M2M chain, or in line

@@ -90,7 +90,7 @@ struct DevicePlan {
     std::array<hipEvent_t, 9> ev{};
     hipStream_t nearStream = nullptr;  // near-field pairs beside the far-field chain
     hipEvent_t evFork = nullptr, evJoin = nullptr;
-    int nearOverlap = 2;               // 0 serial, 1 fork before S2M, 2 after; chosen by mlfma_autotune
+    int nearOverlap = 2;               // 0 in order, fork 1 before S2M, 2 after S2M, 3 after M2M; autotuned
     hipStream_t m2lStream = nullptr;   // finest-level M2L beside the M2M chain
     hipEvent_t evM2L = nullptr, evNear = nullptr;
     bool m2lSplit = true;              // chosen by mlfma_autotune
