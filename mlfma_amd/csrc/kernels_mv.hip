@@ -139,12 +139,23 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
     R* tPhW = tThW + rows * tp;
     int* tThNear = reinterpret_cast<int*>(tPhW + nphP * tp);
     int* tPhNear = tThNear + rows;
-    for (int i = threadIdx.x; i < nr * tp; i += TB) tThW[i] = (R)thW[j0 * tp + i];
-    for (int i = threadIdx.x; i < nphP * tp; i += TB) tPhW[i] = (R)phW[i];
-    for (int i = threadIdx.x; i < nr; i += TB) tThNear[i] = thNear[j0 + i];
-    for (int i = threadIdx.x; i < nphP; i += TB) tPhNear[i] = phNear[i];
+    // the table loads are issued first and stored to LDS after the children's
+    // sum, so that their latency overlaps the children's loads (a loop with
+    // one store per iteration waits for each load in turn)
+    constexpr int U = 2;
+    const int t = threadIdx.x;
+    double wTh[U] = {}, wPh[U] = {};
+    const int nTh = nr * tp, nPh = nphP * tp;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = t + u * TB;
+        if (i < nTh) wTh[u] = thW[j0 * tp + i];
+        if (i < nPh) wPh[u] = phW[i];
+    }
+    const int nearTh = t < nr ? thNear[j0 + t] : 0;
+    const int nearPh = t < nphP ? phNear[t] : 0;
     const int c0 = ptr[item], nc = ptr[item + 1] - c0;  // at most 8 children
-    for (int d = threadIdx.x; d < ndirC; d += TB) {
+    for (int d = t; d < ndirC; d += TB) {
         // every child's loads are issued before the first product
         cplx<R> s[8], m0[8], m1[8];
 #pragma unroll
@@ -165,12 +176,25 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
         sS[2 * d] = a0;
         sS[2 * d + 1] = a1;
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = t + u * TB;
+        if (i < nTh) tThW[i] = (R)wTh[u];
+        if (i < nPh) tPhW[i] = (R)wPh[u];
+    }
+    if (t < nr) tThNear[t] = nearTh;
+    if (t < nphP) tPhNear[t] = nearPh;
+    for (int i = t + U * TB; i < nTh; i += TB) tThW[i] = (R)thW[j0 * tp + i];
+    for (int i = t + U * TB; i < nPh; i += TB) tPhW[i] = (R)phW[i];
+    for (int i = t + TB; i < nr; i += TB) tThNear[i] = thNear[j0 + i];
+    for (int i = t + TB; i < nphP; i += TB) tPhNear[i] = phNear[i];
     __syncthreads();
     for (int e = threadIdx.x; e < nInner; e += TB) {
         const int jr = e / mphC, iph = e - jr * mphC;
         const int near = tThNear[jr];
         const int iphf = iph + (iph < mphC / 2 ? mphC / 2 : -mphC / 2);
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+#pragma unroll 4
         for (int q = 0; q < tp; ++q) {
             const int i = near + 1 - p + q;
             const int f = i < 0 ? -i - 1 : (i >= mthC ? 2 * mthC - i - 1 : i);
@@ -190,6 +214,7 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
         const int near = tPhNear[jph];
         const cplx<R>* in = sI + 2 * jr * mphC;
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+#pragma unroll 4
         for (int q = 0; q < tp; ++q) {
             int i = near + 1 - p + q;
             i = i < 0 ? i + mphC : (i >= mphC ? i - mphC : i);
@@ -505,33 +530,91 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     int* tPhSrc = tPhPtr + nphC + 1;
     int* tThPtr = tPhSrc + nAtPh;
     int* tThSrc = tThPtr + nthC + 1;              // 2 * coarse theta + flip
-    for (int i = threadIdx.x; i < nAtPh; i += TB) {
+    // Prologue: every global load of a lane (tables, the parent local and its
+    // phases, the child local the result is added to) is issued before the
+    // first LDS store; a loop with one store per iteration waits for each
+    // load in turn.  Up to U elements per lane and array go this way, the
+    // rest (large grids only) through the loops after it.
+    constexpr int U = 3, UO = 2;
+    const int t = threadIdx.x;
+    const int nOut = nthC * nphC;
+    const cplx<R>* Lb = Lp + (int64_t)parent[item] * ndirP * 2;
+    const cplx<R>* ph = ADJ ? nullptr : phaseDn + (int64_t)oct[item] * ndirP;
+    const cplx<R>* phC = ADJ ? phaseDn + (int64_t)oct[item] * nOut : nullptr;
+    cplx<R>* out = Lc + (int64_t)child[item] * nOut * 2;
+    double wPh[U] = {}, wTh[U] = {};
+    int sPh[U] = {}, sTh[U] = {}, fTh[U] = {};
+    cplx<R> f0[U] = {}, f1[U] = {}, fq[U] = {}, o0[UO] = {}, o1[UO] = {}, oq[UO] = {};
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = t + u * TB;
+        if (i < nAtPh) {
+            wPh[u] = atPhW[i];
+            sPh[u] = atPhSrc[i];
+        }
+        if (i < nAtTh) {
+            wTh[u] = atThW[i];
+            sTh[u] = atThSrc[i];
+            fTh[u] = atThFlip[i];
+        }
+        if (i < ndirP) {
+            if (!ADJ) fq[u] = ph[i];
+            f0[u] = Lb[2 * i];
+            f1[u] = Lb[2 * i + 1];
+        }
+    }
+    const int pPh = t <= nphC ? atPhPtr[t] : 0;
+    const int pTh = t <= nthC ? atThPtr[t] : 0;
+#pragma unroll
+    for (int u = 0; u < UO; ++u) {
+        const int e = t + u * TB;
+        if (e < nOut) {
+            o0[u] = out[2 * e];
+            o1[u] = out[2 * e + 1];
+            if (ADJ) oq[u] = phC[e];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = t + u * TB;
+        if (i < nAtPh) {
+            tPhW[i] = (R)wPh[u];
+            tPhSrc[i] = sPh[u];
+        }
+        if (i < nAtTh) {
+            const bool flip = fTh[u] != 0;
+            tThW[i] = (R)(flip ? -wTh[u] : wTh[u]);
+            tThSrc[i] = 2 * sTh[u] + (flip ? 1 : 0);
+        }
+        if (i < ndirP) {
+            sP[2 * i] = ADJ ? f0[u] : fq[u] * f0[u];
+            sP[2 * i + 1] = ADJ ? f1[u] : fq[u] * f1[u];
+        }
+    }
+    if (t <= nphC) tPhPtr[t] = pPh;
+    if (t <= nthC) tThPtr[t] = pTh;
+    for (int i = t + U * TB; i < nAtPh; i += TB) {
         tPhW[i] = (R)atPhW[i];
         tPhSrc[i] = atPhSrc[i];
     }
-    for (int i = threadIdx.x; i < nAtTh; i += TB) {
+    for (int i = t + U * TB; i < nAtTh; i += TB) {
         const bool flip = atThFlip[i] != 0;
         tThW[i] = (R)(flip ? -atThW[i] : atThW[i]);
         tThSrc[i] = 2 * atThSrc[i] + (flip ? 1 : 0);
     }
-    for (int i = threadIdx.x; i <= nphC; i += TB) tPhPtr[i] = atPhPtr[i];
-    for (int i = threadIdx.x; i <= nthC; i += TB) tThPtr[i] = atThPtr[i];
-    const cplx<R>* Lb = Lp + (int64_t)parent[item] * ndirP * 2;
-    if (ADJ) {
-        for (int d = threadIdx.x; d < 2 * ndirP; d += TB) sP[d] = Lb[d];
-    } else {
-        const cplx<R>* ph = phaseDn + (int64_t)oct[item] * ndirP;
-        for (int d = threadIdx.x; d < ndirP; d += TB) {
-            const cplx<R> s = ph[d];
-            sP[2 * d] = s * Lb[2 * d];
-            sP[2 * d + 1] = s * Lb[2 * d + 1];
-        }
+    for (int i = t + TB; i <= nphC; i += TB) tPhPtr[i] = atPhPtr[i];
+    for (int i = t + TB; i <= nthC; i += TB) tThPtr[i] = atThPtr[i];
+    for (int d = t + U * TB; d < ndirP; d += TB) {
+        const cplx<R> q = ADJ ? mk<R>(1, 0) : ph[d];
+        sP[2 * d] = ADJ ? Lb[2 * d] : q * Lb[2 * d];
+        sP[2 * d + 1] = ADJ ? Lb[2 * d + 1] : q * Lb[2 * d + 1];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nInner; e += TB) {
         const int ith = e / nphC, jph = e - ith * nphC;
         const cplx<R>* row = sP + 2 * ith * mphP;
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+#pragma unroll 4
         for (int q = tPhPtr[jph]; q < tPhPtr[jph + 1]; ++q) {
             const int d = 2 * tPhSrc[q];
             const R w = tPhW[q];
@@ -542,24 +625,40 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
         sI[2 * e + 1] = a1;
     }
     __syncthreads();
-    cplx<R>* out = Lc + (int64_t)child[item] * nthC * nphC * 2;
-    for (int e = threadIdx.x; e < nthC * nphC; e += TB) {
+    // theta anterpolation (pole rule) of output e, times the conjugated
+    // child-grid phase for the adjoint
+    auto anter = [&](int e, cplx<R>& a0, cplx<R>& a1, cplx<R> q) {
         const int jf = e / nphC, js = e - jf * nphC;
         const int jsf = js + (js < nphC / 2 ? nphC / 2 : -nphC / 2);
-        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
-        for (int q = tThPtr[jf]; q < tThPtr[jf + 1]; ++q) {
-            const int sf = tThSrc[q];
+        a0 = mk<R>(0, 0);
+        a1 = mk<R>(0, 0);
+#pragma unroll 4
+        for (int k = tThPtr[jf]; k < tThPtr[jf + 1]; ++k) {
+            const int sf = tThSrc[k];
             const int d = 2 * ((sf >> 1) * nphC + ((sf & 1) ? jsf : js));
-            const R w = tThW[q];
+            const R w = tThW[k];
             a0 = a0 + w * sI[d];
             a1 = a1 + w * sI[d + 1];
         }
         if (ADJ) {
-            cplx<R> s = phaseDn[(int64_t)oct[item] * nthC * nphC + e];
-            s.im = -s.im;
-            a0 = s * a0;
-            a1 = s * a1;
+            q.im = -q.im;
+            a0 = q * a0;
+            a1 = q * a1;
         }
+    };
+#pragma unroll
+    for (int u = 0; u < UO; ++u) {
+        const int e = t + u * TB;
+        if (e < nOut) {
+            cplx<R> a0, a1;
+            anter(e, a0, a1, oq[u]);
+            out[2 * e] = o0[u] + a0;
+            out[2 * e + 1] = o1[u] + a1;
+        }
+    }
+    for (int e = t + UO * TB; e < nOut; e += TB) {
+        cplx<R> a0, a1;
+        anter(e, a0, a1, ADJ ? phC[e] : mk<R>(0, 0));
         out[2 * e] = out[2 * e] + a0;
         out[2 * e + 1] = out[2 * e + 1] + a1;
     }
@@ -588,6 +687,7 @@ bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32
                            phaseDn, Lp, Lc, fs);
     return true;
 }
+
 
 // ------------------------------------------------------------------ L2L adjoint
 // The adjoint of L2L (src/fmm/stem.cpp:109-131,186-244): the parent gains
