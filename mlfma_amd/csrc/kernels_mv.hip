@@ -869,6 +869,9 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     double r6, r7, i6, i7;
 #endif
     cplx<double> xc = mk(0.0, 0.0);
+    // column indices run one tile ahead of the x gathers that use them, so
+    // that a tile's loads wait for one round trip, not two
+    int ci = t0 < cw ? colsIn[I.cptr + t0] : 0;
 #define NEAR_LD1(K, C0, NE)                                                       \
     {                                                                            \
         const int q = 2 * min(t0 + K * TB, (NE) - 1);                            \
@@ -881,7 +884,9 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
         NEAR_LD1(0, C0, NE) NEAR_LD1(1, C0, NE) NEAR_LD1(2, C0, NE)                                 \
         NEAR_LD1(3, C0, NE) NEAR_LD1(4, C0, NE)                                                     \
         NEAR_LD_EXTRA(C0, NE)                                                                       \
-        if (t0 < (CW)) xc = x[colsIn[I.cptr + (C0) + t0]];                                          \
+        if (t0 < (CW)) xc = x[ci];                                                                  \
+        const int cn = (C0) + cwMax;                                                                \
+        ci = t0 < min(cwMax, I.nc - cn) ? colsIn[I.cptr + cn + t0] : 0;                             \
     }
 #if MLFMA_NEAR_LOADS == 8
 #define NEAR_LD_EXTRA(C0, NE) NEAR_LD1(5, C0, NE) NEAR_LD1(6, C0, NE) NEAR_LD1(7, C0, NE)
@@ -1005,15 +1010,46 @@ __global__ __launch_bounds__(TB) void kNearSymB(const NearItem* __restrict__ ite
 #pragma unroll
     for (int k = 0; k < NB; ++k) re[k] = im[k] = 0.0;
     double* sTd = reinterpret_cast<double*>(sT);
-    for (int c0 = 0; c0 < I.nc; c0 += cwMax) {
-        const int cw = min(cwMax, I.nc - c0);
-        const int ne = cw * nr;
-        const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)c0 * nr);
-        // four loads per lane in flight, held as plain doubles
-        const int q0 = 2 * min(t0, ne - 1), q1 = 2 * min(t0 + TB, ne - 1);
-        const int q2 = 2 * min(t0 + 2 * TB, ne - 1), q3 = 2 * min(t0 + 3 * TB, ne - 1);
-        const double v0r = sd[q0], v0i = sd[q0 + 1], v1r = sd[q1], v1i = sd[q1 + 1];
-        const double v2r = sd[q2], v2i = sd[q2 + 1], v3r = sd[q3], v3i = sd[q3 + 1];
+    // Software pipeline: the values of tile k+1 and its x gathers are issued
+    // before the LDS work of tile k, and the column indices of the gathers
+    // run one tile further ahead, so that every wait is for loads issued one
+    // tile earlier.  Lane t0 gathers entries t0 and t0 + TB of the tile's
+    // NB x cw column values (NB cw <= 2 TB).
+    static_assert(NB * CWB <= 2 * TB, "two column gathers per lane");
+    int c0 = 0, cw = min(cwMax, I.nc), ne = cw * nr;
+    double v0r, v0i, v1r, v1i, v2r, v2i, v3r, v3i;
+    cplx<double> xg0 = mk(0.0, 0.0), xg1 = mk(0.0, 0.0);
+    int ia0 = 0, ia1 = 0;  // column indices (input order) of the next gathers
+    auto colIdx = [&](int cc0, int ccw, int e) {
+        return e < NB * ccw ? colsIn[I.cptr + cc0 + e % ccw] : 0;
+    };
+    auto loadVals = [&](int cc0, int nne) {
+        const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)cc0 * nr);
+        const int q0 = 2 * min(t0, nne - 1), q1 = 2 * min(t0 + TB, nne - 1);
+        const int q2 = 2 * min(t0 + 2 * TB, nne - 1), q3 = 2 * min(t0 + 3 * TB, nne - 1);
+        v0r = sd[q0];
+        v0i = sd[q0 + 1];
+        v1r = sd[q1];
+        v1i = sd[q1 + 1];
+        v2r = sd[q2];
+        v2i = sd[q2 + 1];
+        v3r = sd[q3];
+        v3i = sd[q3 + 1];
+    };
+    auto gather = [&](int ccw) {
+        if (t0 < NB * ccw) xg0 = x[(t0 / ccw) * xs + ia0];
+        if (t0 + TB < NB * ccw) xg1 = x[((t0 + TB) / ccw) * xs + ia1];
+    };
+    ia0 = colIdx(0, cw, t0);
+    ia1 = colIdx(0, cw, t0 + TB);
+    loadVals(0, ne);
+    gather(cw);
+    {
+        const int cn = cwMax, cwn = min(cwMax, I.nc - cn);
+        ia0 = cwn > 0 ? colIdx(cn, cwn, t0) : 0;
+        ia1 = cwn > 0 ? colIdx(cn, cwn, t0 + TB) : 0;
+    }
+    while (true) {
         __syncthreads();  // the previous tile is consumed
 #define NEARB_ST(K, VR, VI)                                   \
     if (t0 + K * TB < ne) {                                   \
@@ -1025,11 +1061,21 @@ __global__ __launch_bounds__(TB) void kNearSymB(const NearItem* __restrict__ ite
     }
         NEARB_ST(0, v0r, v0i) NEARB_ST(1, v1r, v1i) NEARB_ST(2, v2r, v2i) NEARB_ST(3, v3r, v3i)
 #undef NEARB_ST
-        for (int e = t0; e < NB * cw; e += TB) {
-            const int rh = e / cw, c = e - rh * cw;
-            sxc[rh][c] = x[rh * xs + colsIn[I.cptr + c0 + c]];
-        }
+        if (t0 < NB * cw) sxc[t0 / cw][t0 % cw] = xg0;
+        if (t0 + TB < NB * cw) sxc[(t0 + TB) / cw][(t0 + TB) % cw] = xg1;
         __syncthreads();
+        const int c0n = c0 + cwMax;
+        const bool more = c0n < I.nc;  // uniform over the block
+        int cwn = 0, nen = 0;
+        if (more) {
+            cwn = min(cwMax, I.nc - c0n);
+            nen = cwn * nr;
+            loadVals(c0n, nen);
+            gather(cwn);
+            const int cn = c0n + cwMax, cwnn = min(cwMax, I.nc - cn);
+            ia0 = cwnn > 0 ? colIdx(cn, cwnn, t0) : 0;
+            ia1 = cwnn > 0 ? colIdx(cn, cwnn, t0 + TB) : 0;
+        }
         if (g < G)
             for (int c = g; c < cw; c += G) {
                 const cplx<double> z = sT[c * ld + r];
@@ -1052,6 +1098,10 @@ __global__ __launch_bounds__(TB) void kNearSymB(const NearItem* __restrict__ ite
             }
             part[rh * ps + I.cpo + c0 + c - I.nself] = mk(cr, ci);
         }
+        if (!more) break;
+        c0 = c0n;
+        cw = cwn;
+        ne = nen;
     }
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
