@@ -14,6 +14,9 @@ constexpr int TB = 256;
 #ifndef MLFMA_M2M_MIN_BLOCKS
 #define MLFMA_M2M_MIN_BLOCKS 1024    // split parent theta rows until this many blocks
 #endif
+#ifndef MLFMA_M2L_PAIRS
+#define MLFMA_M2L_PAIRS 4            // pairs in flight per wave in the one-RHS M2L
+#endif
 #ifndef MLFMA_NEAR_LOADS
 #define MLFMA_NEAR_LOADS 6           // near-field values in flight per lane (5, 6 or 8)
 #endif
@@ -407,8 +410,9 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
 
 // Single right-hand side: a block is (level, target box, chunk of 64
 // directions) and its four waves split the target's pair list (wave w takes
-// pairs w, w + 4, ...; two pairs in flight per wave), so eight independent
-// pairs are in flight per direction instead of a long dependent walk; the
+// pairs w, w + 4, ...; MLFMA_M2L_PAIRS pairs in flight per wave), so that
+// many independent pairs are in flight per direction instead of a long
+// dependent walk of index and field loads; the
 // wave partials are added in LDS in a fixed order.  Lanes past the level's
 // direction count read a clamped direction and store nothing.
 template <typename R, bool CONJ>
@@ -421,36 +425,60 @@ __global__ __launch_bounds__(TB) void kM2Lw(M2LLevels<R> lv) {
     xcdItemChunk(b, lv.per[l], nch, item, chunk);
     if (item >= lv.ntgt[l]) return;  // uniform per block
     const int ndir = lv.ndir[l];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int d = chunk * 64 + lane;
     const int dc = d < ndir ? d : ndir - 1;
     const int32_t* src = lv.src[l];
     const int32_t* off = lv.off[l];
     const cplx<R>* T = lv.T[l] + dc;
     const cplx<R>* M = lv.M[l] + 2 * dc;
-    const int q0 = lv.ptr[l][item], q1 = lv.ptr[l][item + 1];
+    const int q0 = lv.ptr[l][item], n = lv.ptr[l][item + 1] - q0;
     cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
-    int q = q0 + w;
-    for (; q + 4 < q1; q += 8) {
-        cplx<R> t0 = T[(int64_t)off[q] * ndir];
-        cplx<R> t1 = T[(int64_t)off[q + 4] * ndir];
-        if (CONJ) {
-            t0.im = -t0.im;
-            t1.im = -t1.im;
+    // The wave's pairs q0 + w + 4 i are taken 64 at a time: lane i loads the
+    // (source, offset) of pair i once, and every pair's indices are then read
+    // from that lane (v_readlane), so that a step of P pairs waits for one
+    // round trip (the field loads), not two.  The even pairs of a step add
+    // into a, the odd ones into b.
+    constexpr int P = MLFMA_M2L_PAIRS;
+    for (int base = 0; base < n; base += 256) {
+        const int qi = base + w + 4 * lane;
+        const int mySrc = qi < n ? src[q0 + qi] : 0;
+        const int myOff = qi < n ? off[q0 + qi] : 0;
+        const int cnt = min(64, max(0, (n - base - w + 3) / 4));
+        int i = 0;
+        for (; i + P <= cnt; i += P) {
+            cplx<R> t[P], m[P][2];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int sj = __builtin_amdgcn_readlane(mySrc, i + j);
+                const int oj = __builtin_amdgcn_readlane(myOff, i + j);
+                t[j] = T[(int64_t)oj * ndir];
+                const cplx<R>* mp = M + (int64_t)sj * ndir * 2;
+                m[j][0] = mp[0];
+                m[j][1] = mp[1];
+            }
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                if (CONJ) t[j].im = -t[j].im;
+                if (j % 2 == 0) {
+                    a0 = a0 + t[j] * m[j][0];
+                    a1 = a1 + t[j] * m[j][1];
+                } else {
+                    b0 = b0 + t[j] * m[j][0];
+                    b1 = b1 + t[j] * m[j][1];
+                }
+            }
         }
-        const cplx<R>* m0 = M + (int64_t)src[q] * ndir * 2;
-        const cplx<R>* m1 = M + (int64_t)src[q + 4] * ndir * 2;
-        a0 = a0 + t0 * m0[0];
-        a1 = a1 + t0 * m0[1];
-        b0 = b0 + t1 * m1[0];
-        b1 = b1 + t1 * m1[1];
-    }
-    if (q < q1) {
-        cplx<R> t0 = T[(int64_t)off[q] * ndir];
-        if (CONJ) t0.im = -t0.im;
-        const cplx<R>* m0 = M + (int64_t)src[q] * ndir * 2;
-        a0 = a0 + t0 * m0[0];
-        a1 = a1 + t0 * m0[1];
+        for (; i < cnt; ++i) {
+            const int sj = __builtin_amdgcn_readlane(mySrc, i);
+            const int oj = __builtin_amdgcn_readlane(myOff, i);
+            cplx<R> t0 = T[(int64_t)oj * ndir];
+            if (CONJ) t0.im = -t0.im;
+            const cplx<R>* m0 = M + (int64_t)sj * ndir * 2;
+            a0 = a0 + t0 * m0[0];
+            a1 = a1 + t0 * m0[1];
+        }
     }
     __shared__ cplx<R> red[3][64][2];
     if (w > 0) {
