@@ -37,6 +37,20 @@ def planewave_x():
     return A.make_planewave(1.0, (0.0, 0.0, 1.0), (1.0, 0.0, 0.0), 2 * np.pi)
 
 
+def two_clusters(seed=11, nA=500, nB=150):
+    """Cluster A (side 0.45 about (0.5, 0.5, 0.5)) and cluster B (side 0.2
+    about (-0.65, -0.65, -0.65)) in a root box of side 2: A's level-3 boxes
+    are all neighbours of each other (empty interaction lists, with children),
+    and B reaches A only through the level-2 lists."""
+    rng = np.random.default_rng(seed)
+    pos = np.vstack([rng.uniform(0.275, 0.725, size=(nA, 3)), rng.uniform(-0.75, -0.55, size=(nB, 3))])
+    mom = rng.normal(size=(len(pos), 3))
+    return A.GeometryArrays(A.MLFMA_SRC_DIPOLE, pos=pos, mom=mom)
+
+
+TWO_CLUSTERS_KW = dict(root_leng=2.0, max_node_srcs=8, max_level=-1, interp_order=4, digits=4)
+
+
 CASES = {
     # name: (geometry factory, config kwargs)
     "dipole_fixed": (lambda: dipole_cube(600, 1.8, 0),

@@ -4,7 +4,9 @@ at all), coincident sources (the reference drops their entry,
 src/sources/dipole.h:62-70), sources exactly on the split planes of the tree
 (ties go to the lower octant, src/fmm/stem.cpp:13), a one-level tree whose
 leaves are all near each other, and a solve on an operator that is exactly
-zero (a breakdown reported as a status, not a crash)."""
+zero (a breakdown reported as a status, not a crash), and two clusters
+coupled only through level-2 lists above stems with empty lists (the
+reference drops the L2L below such stems: rows exactly zero)."""
 import numpy as np
 import pytest
 
@@ -100,3 +102,14 @@ def test_zero_operator_solve_reports_breakdown():
     assert e.value.code == A.MLFMA_ERR_BREAKDOWN and "breakdown" in str(e.value)
     # the plan stays usable
     assert np.all(p.matvec(np.ones(1, complex)) == 0)
+
+
+def test_two_clusters_no_l2l_below_empty_list_stems():
+    geom = cases.two_clusters()
+    p, x, y, yd = check(geom, dict(cases.TWO_CLUSTERS_KW), "two clusters")
+    # driven from cluster B alone, cluster A's rows are exactly zero
+    xb = np.zeros(geom.n, complex)
+    xb[500:] = x[500:]
+    yb = p.matvec(xb)
+    say("two clusters: max |y| on A", np.abs(yb[:500]).max(), "on B", np.abs(yb[500:]).max())
+    assert np.all(yb[:500] == 0) and np.abs(yb[500:]).max() > 0

@@ -148,6 +148,113 @@ def test_translation_vs_direct_series(dip):
         assert ex["psi_keys"] > 0
 
 
+def test_translation_table_from_alpha_samples(dip):
+    """Every T entry from the alpha samples by scipy's barycentric Lagrange
+    interpolation in psi = acos(khat . rhat) on the 2p-point stencil
+    (src/fmm/tables.cpp:129-213): where the stencil lies inside [0, pi] this
+    is plain interpolation of the samples; where it runs past an end the
+    reference takes the samples at the reflected indices -i-1 and 2n-1-i
+    (src/math.h:144-154) at unreflected abscissae (Q1), which the oracle must
+    reproduce exactly.  Both kinds of entry occur; 1e-11 of the level's
+    largest alpha."""
+    from scipy.interpolate import BarycentricInterpolator
+    _, kw, o = dip
+    p = kw["interp_order"]
+    dv = O.inode_distvecs()
+    dists = O.inode_distances()
+    for lv in (2, 3):
+        T, ex = o.translation(lv)
+        al = o.alpha(lv)
+        nps = ex["nps"]
+        th, ph = _grid_dirs(o.angles(lv))
+        kh = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1)
+        scale = np.abs(al).max()
+        counts = [0, 0]
+        worst = 0.0
+        for oi in range(len(dv)):
+            r = np.linalg.norm(dv[oi])
+            di = int(np.argmin(np.abs(dists - r)))
+            rh = dv[oi] / r
+            psi = np.arccos((kh[:, 0] * rh[0] + kh[:, 1] * rh[1]) + kh[:, 2] * rh[2])
+            near = np.floor((nps - 1) * psi / np.pi).astype(int)
+            for d in range(0, len(psi), 5):
+                ips = np.arange(near[d] + 1 - p, near[d] + p + 1)
+                src = np.where(ips < 0, -ips - 1, np.where(ips >= nps, 2 * nps - ips - 1, ips))
+                counts[int(np.any(src != ips))] += 1
+                ref = BarycentricInterpolator(np.pi * ips / (nps - 1), al[di][src])(psi[d])
+                worst = max(worst, abs(T[oi, d] - ref) / scale)
+        assert counts[0] > 0 and counts[1] > 0, counts
+        assert worst <= 1e-11, (lv, worst)
+
+
+def test_stale_locals_reach_only_stale_leaves():
+    """Q4/Q5 (src/fmm/node.cpp:106-108, src/fmm/stem.cpp:115): a box whose
+    interaction list is empty keeps the locals of the previous matvec, and a
+    box passes no locals down when its own list is empty.  So a matvec after
+    one on x1 differs from the first matvec on the same x2 by a term that is
+    linear in x1 (doubling x1 doubles it) and that lands only on the sources
+    of leaves (level >= 2) with an empty list; every other row is bitwise
+    the fresh result."""
+    geom, kw = cases.make("dipole_adaptive")
+    cfg = A.make_config(device=-1, **kw)
+    x1, x2 = G.complex_gaussian(geom.n, 1), G.complex_gaussian(geom.n, 2)
+    fresh = O.Oracle(geom, cases.planewave_z(), cfg).matvec(x2)
+    diffs = []
+    for scale in (1.0, 2.0):
+        o = O.Oracle(geom, cases.planewave_z(), cfg)
+        o.matvec(scale * x1)
+        diffs.append(o.matvec(x2) - fresh)
+    nodes = o.nodes()
+    has_list = set(o.pairs(0)[:, 0].tolist())
+    stale = [nd for nd in nodes if nd[4] == 1 and nd[0] >= 2 and nd[5] > 0 and nd[6] not in has_list]
+    assert stale, "the case has no stale leaves"
+    root = kw["root_leng"]
+    rows = np.zeros(geom.n, dtype=bool)
+    for nd in stale:
+        h = root / 2 ** nd[0]
+        idx = np.floor((geom.pos + root / 2) / h).astype(int)
+        rows |= np.all(idx == nd[1:4], axis=1)
+    assert rows.sum() == sum(nd[5] for nd in stale)
+    assert np.all(diffs[0][~rows] == 0) and np.all(diffs[1][~rows] == 0)
+    assert np.linalg.norm(diffs[0][rows]) > 1e-6 * np.linalg.norm(fresh)
+    assert np.linalg.norm(diffs[1] - 2 * diffs[0]) <= 1e-12 * np.linalg.norm(diffs[1])
+
+
+def _rows_under_stale_stems(o, pos, root):
+    """Rows whose box at some level >= 3 is a stem with an empty list."""
+    has_list = set(o.pairs(0)[:, 0].tolist())
+    under = np.zeros(len(pos), dtype=bool)
+    for nd in o.nodes():
+        if nd[4] == 0 and nd[0] >= 3 and nd[5] > 0 and nd[6] not in has_list:
+            h = root / 2 ** nd[0]
+            under |= np.all(np.floor((pos + root / 2) / h).astype(int) == nd[1:4], axis=1)
+    return under
+
+
+def test_no_l2l_below_a_stem_with_an_empty_list():
+    """Q5 (src/fmm/stem.cpp:115): a stem whose interaction list is empty
+    passes no locals down.  Two clusters that reach each other only through
+    the level-2 lists, with cluster A's level-3 boxes all stems with empty
+    lists: driven by sources in B alone, every row of A comes out exactly
+    zero (its only path, the L2L below level 3, is dropped) while no row of
+    the dense product there is near zero; B's own rows keep the FMM
+    accuracy."""
+    geom = cases.two_clusters()
+    kw = cases.TWO_CLUSTERS_KW
+    o = O.Oracle(geom, cases.planewave_z(), A.make_config(device=-1, **kw))
+    nA = 500
+    inA = np.arange(geom.n) < nA
+    under = _rows_under_stale_stems(o, geom.pos, kw["root_leng"])
+    assert np.all(under[inA])
+    x = np.zeros(geom.n, complex)
+    x[nA:] = np.exp(1j * np.random.default_rng(3).uniform(0, 2 * np.pi, geom.n - nA))
+    y, yd = o.matvec(x), o.dense_matvec(x)
+    assert np.all(y[inA] == 0)
+    # the coupling the tree drops is real: no row of A is near zero in the dense product
+    assert np.abs(yd[inA]).min() > 1e-2 * np.abs(yd[inA]).max()
+    assert np.linalg.norm(y[~inA] - yd[~inA]) <= 1e-2 * np.linalg.norm(yd[~inA])
+
+
 def test_fmm_matvec_vs_dense_dipoles(dip):
     # the reference's own O(N^2) check (evalDirect, src/mlfma.cpp:84-107): 3 digits
     geom, kw, o = dip
