@@ -671,7 +671,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // latency-bound tree stages) or after the M2M chain (3: the M2M launches,
     // with few blocks at coarse levels, then run uncontended); serial when
     // stages are timed
-    const bool serial = timed || (flags & MLFMA_FLAG_ONE_STREAM);
+    const bool serial = timed || (flags & MLFMA_FLAG_ONE_STREAM) || solveInOrder;
     const int overlap = serial ? 0 : nearOverlap;
     // M2L of the finest level needs only the leaf multipoles, and its locals
     // are first read by the L2L into that level: it runs on its own stream
@@ -791,7 +791,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
     cplx<R>* M = fieldM<R>();
     cplx<R>* Lc = fieldL<R>();
     const cplx<R>* rp = radpat<R>();
-    const bool overlap = nearOverlap != 0 && !(flags & MLFMA_FLAG_ONE_STREAM);
+    const bool overlap = nearOverlap != 0 && !(flags & MLFMA_FLAG_ONE_STREAM) && !solveInOrder;
     if (overlap) {
         hipCheck(hipEventRecord(evFork, st), "event");
         hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");

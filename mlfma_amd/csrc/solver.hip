@@ -264,45 +264,131 @@ __device__ __forceinline__ cplx<double> cconj(cplx<double> a) { return cplx<doub
 // previous Givens rotations, the new rotation, and the rotated right-hand
 // side g; stat[3 j .. 3 j + 2] = (|g_{j+1}| / bn, |w|, rho), also mirrored to
 // the host-mapped statHost.  One block; the rotations run on thread 0.
-__global__ void kGivens(const cplx<double>* __restrict__ hbuf, const double* __restrict__ normPart, int nbn, int j,
-                        double bn, cplx<double>* __restrict__ cs, cplx<double>* __restrict__ sn,
-                        cplx<double>* __restrict__ g, cplx<double>* __restrict__ H, int ldH, double* __restrict__ stat,
-                        double* __restrict__ statHost) {
-    double s2 = 0, z = 0;
-    for (int b = threadIdx.x; b < nbn; b += KB) s2 += normPart[b];
-    blockSum2(s2, z);
-    if (threadIdx.x != 0) return;
+// The Givens step of GMRES(m), called by every thread of one block with s2 = |w|^2
+// (valid in thread 0): the block stages the new Hessenberg column (the two
+// CGS passes added) and the previous rotations in LDS, then thread 0 applies
+// the rotations from LDS (a chain of dependent global loads otherwise) and
+// writes the column, the new rotation and the rotated right-hand side back.
+__device__ void givensStep(const cplx<double>* __restrict__ hbuf, double s2, int j, double bn,
+                           cplx<double>* __restrict__ cs, cplx<double>* __restrict__ sn, cplx<double>* __restrict__ g,
+                           cplx<double>* __restrict__ H, int ldH, double* __restrict__ stat,
+                           double* __restrict__ statHost, cplx<double>* __restrict__ sh) {
+    // sh: 3 (j + 2) complex of LDS: column, cs, sn
     const int ncol = j + 1;
-    cplx<double>* h = H + (int64_t)j * ldH;
-    for (int i = 0; i < ncol; ++i)
-        h[i] = cplx<double>{hbuf[i].re + hbuf[ncol + 1 + i].re, hbuf[i].im + hbuf[ncol + 1 + i].im};
-    const double hn = sqrt(s2);
-    h[ncol] = cplx<double>{hn, 0.0};
-    for (int i = 0; i < j; ++i) {
-        const cplx<double> a = cmul(cconj(cs[i]), h[i]), b = cmul(cconj(sn[i]), h[i + 1]);
-        const cplx<double> c = cmul(sn[i], h[i]), e = cmul(cs[i], h[i + 1]);
-        h[i + 1] = cplx<double>{-c.re + e.re, -c.im + e.im};
-        h[i] = cplx<double>{a.re + b.re, a.im + b.im};
+    cplx<double>* hc = sh;
+    cplx<double>* scs = sh + ncol + 1;
+    cplx<double>* ssn = scs + ncol;
+    for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
+        hc[i] = cplx<double>{hbuf[i].re + hbuf[ncol + 1 + i].re, hbuf[i].im + hbuf[ncol + 1 + i].im};
+        if (i < j) {
+            scs[i] = cs[i];
+            ssn[i] = sn[i];
+        }
     }
-    const double rho = sqrt(h[j].re * h[j].re + h[j].im * h[j].im + h[ncol].re * h[ncol].re + h[ncol].im * h[ncol].im);
-    cs[j] = cplx<double>{h[j].re / rho, h[j].im / rho};
-    sn[j] = cplx<double>{h[ncol].re / rho, h[ncol].im / rho};
-    h[j] = cplx<double>{rho, 0.0};
-    h[ncol] = cplx<double>{0.0, 0.0};
-    const cplx<double> gj = g[j];
-    const cplx<double> t = cmul(sn[j], gj);
-    g[j + 1] = cplx<double>{-t.re, -t.im};
-    g[j] = cmul(cconj(cs[j]), gj);
-    const double est = hypot(g[j + 1].re, g[j + 1].im) / bn;
-    stat[3 * j] = est;
-    stat[3 * j + 1] = hn;
-    stat[3 * j + 2] = rho;
-    statHost[3 * j] = est;
-    statHost[3 * j + 1] = hn;
-    statHost[3 * j + 2] = rho;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double hn = sqrt(s2);
+        hc[ncol] = cplx<double>{hn, 0.0};
+        cplx<double> hi = hc[0];
+        for (int i = 0; i < j; ++i) {
+            const cplx<double> hi1 = hc[i + 1], c = scs[i], sv = ssn[i];
+            const cplx<double> a = cmul(cconj(c), hi), b = cmul(cconj(sv), hi1);
+            const cplx<double> cc = cmul(sv, hi), e = cmul(c, hi1);
+            hc[i] = cplx<double>{a.re + b.re, a.im + b.im};
+            hi = cplx<double>{-cc.re + e.re, -cc.im + e.im};
+        }
+        hc[j] = hi;
+        const double rho = sqrt(hc[j].re * hc[j].re + hc[j].im * hc[j].im + hc[ncol].re * hc[ncol].re +
+                                hc[ncol].im * hc[ncol].im);
+        cs[j] = cplx<double>{hc[j].re / rho, hc[j].im / rho};
+        sn[j] = cplx<double>{hc[ncol].re / rho, hc[ncol].im / rho};
+        hc[j] = cplx<double>{rho, 0.0};
+        hc[ncol] = cplx<double>{0.0, 0.0};
+        const cplx<double> gj = g[j];
+        const cplx<double> t = cmul(sn[j], gj);
+        g[j + 1] = cplx<double>{-t.re, -t.im};
+        g[j] = cmul(cconj(cs[j]), gj);
+        const double est = hypot(g[j + 1].re, g[j + 1].im) / bn;
+        stat[3 * j] = est;
+        stat[3 * j + 1] = hn;
+        stat[3 * j + 2] = rho;
+        statHost[3 * j] = est;
+        statHost[3 * j + 1] = hn;
+        statHost[3 * j + 2] = rho;
+    }
+    __syncthreads();
+    cplx<double>* h = H + (int64_t)j * ldH;
+    for (int i = threadIdx.x; i <= ncol; i += blockDim.x) h[i] = hc[i];
 }
 
-// kGivens for nrhs right-hand sides, one block each: lane k's CGS output at
+// The second CGS pass (kUpdateR with the norm) and the Givens step in one
+// launch: every block publishes its |w|^2 partial (and block 0 the pass-2
+// coefficients) and counts itself in; the block that counts last, which has
+// then seen every other block's release, sums the partials in the same
+// order every time and runs the step.  No block waits for another.  The
+// counter is back at zero when the kernel ends.
+struct GivensArgs {
+    const cplx<double>* hbuf;
+    int j;
+    double bn;
+    cplx<double>*cs, *sn, *g, *H;
+    int ldH;
+    double *stat, *statHost;
+    unsigned int* counter;
+};
+
+__global__ void kUpdateRGivens(const cplx<double>* __restrict__ Q, int64_t ld, int ncol,
+                               const cplx<double>* __restrict__ partial, int nbd, cplx<double>* __restrict__ hout,
+                               cplx<double>* __restrict__ w, int64_t n, double* __restrict__ normPart, GivensArgs ga) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double>* sh = reinterpret_cast<cplx<double>*>(smem);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = wv; i < ncol; i += KB / 64) {
+        double re = 0, im = 0;
+        for (int b = lane; b < nbd; b += 64) {
+            re += partial[(int64_t)i * nbd + b].re;
+            im += partial[(int64_t)i * nbd + b].im;
+        }
+        re = waveSum(re);
+        im = waveSum(im);
+        if (lane == 0) {
+            sh[i] = cplx<double>{re, im};
+            if (blockIdx.x == 0) hout[i] = cplx<double>{re, im};
+        }
+    }
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    double s2 = 0;
+    if (r < n) {
+        cplx<double> a = w[r];
+        for (int i = 0; i < ncol; ++i) {
+            const cplx<double> q = Q[(int64_t)i * ld + r];
+            const cplx<double> c = sh[i];
+            a.re -= q.re * c.re - q.im * c.im;
+            a.im -= q.re * c.im + q.im * c.re;
+        }
+        w[r] = a;
+        s2 = a.re * a.re + a.im * a.im;
+    }
+    double z = 0;
+    blockSum2(s2, z);
+    __shared__ unsigned int rank;
+    if (threadIdx.x == 0) normPart[blockIdx.x] = s2;
+    __threadfence();  // release, by every thread: the block's partial (and block 0's coefficients)
+    __syncthreads();
+    if (threadIdx.x == 0) rank = atomicAdd(ga.counter, 1u);
+    __syncthreads();
+    if (rank != gridDim.x - 1) return;  // uniform over the block
+    __threadfence();  // acquire: every other block's partial
+    double t2 = 0, t0 = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += KB) t2 += normPart[b];
+    blockSum2(t2, t0);
+    if (threadIdx.x == 0) *ga.counter = 0;
+    __syncthreads();  // every lane is done with the coefficients in sh
+    givensStep(ga.hbuf, t2, ga.j, ga.bn, ga.cs, ga.sn, ga.g, ga.H, ga.ldH, ga.stat, ga.statHost, sh);
+}
+
+// The Givens step for nrhs right-hand sides, one block each: lane k's CGS output at
 // hbuf + k hs (pass 1 at [0, ncol), |w|^2 at [ncol], pass 2 at [ncol + 1,
 // 2 ncol + 1)), its state at cs/sn + k (m + 1), g + k (m + 2), H + k (m + 1)^2,
 // stat/statHost + k 3 (m + 1).  Lanes with on[k] == 0 are skipped.
@@ -310,39 +396,13 @@ __global__ void kGivensB(const cplx<double>* __restrict__ hbuf, int64_t hs, int 
                          const double* __restrict__ bn, const int* __restrict__ on, cplx<double>* __restrict__ cs,
                          cplx<double>* __restrict__ sn, cplx<double>* __restrict__ g, cplx<double>* __restrict__ H,
                          double* __restrict__ stat, double* __restrict__ statHost) {
+    extern __shared__ __align__(16) unsigned char smem[];
     const int k = blockIdx.x;
-    if (threadIdx.x != 0 || !on[k]) return;
+    if (!on[k]) return;  // uniform over the block
     const cplx<double>* hb = hbuf + (int64_t)k * hs;
-    cplx<double>* c_ = cs + (int64_t)k * (m + 1);
-    cplx<double>* s_ = sn + (int64_t)k * (m + 1);
-    cplx<double>* g_ = g + (int64_t)k * (m + 2);
-    cplx<double>* h = H + ((int64_t)k * (m + 1) + j) * (m + 1);
-    const int ncol = j + 1;
-    for (int i = 0; i < ncol; ++i)
-        h[i] = cplx<double>{hb[i].re + hb[ncol + 1 + i].re, hb[i].im + hb[ncol + 1 + i].im};
-    const double hn = sqrt(hb[ncol].re);
-    h[ncol] = cplx<double>{hn, 0.0};
-    for (int i = 0; i < j; ++i) {
-        const cplx<double> a = cmul(cconj(c_[i]), h[i]), b = cmul(cconj(s_[i]), h[i + 1]);
-        const cplx<double> c = cmul(s_[i], h[i]), e = cmul(c_[i], h[i + 1]);
-        h[i + 1] = cplx<double>{-c.re + e.re, -c.im + e.im};
-        h[i] = cplx<double>{a.re + b.re, a.im + b.im};
-    }
-    const double rho = sqrt(h[j].re * h[j].re + h[j].im * h[j].im + h[ncol].re * h[ncol].re + h[ncol].im * h[ncol].im);
-    c_[j] = cplx<double>{h[j].re / rho, h[j].im / rho};
-    s_[j] = cplx<double>{h[ncol].re / rho, h[ncol].im / rho};
-    h[j] = cplx<double>{rho, 0.0};
-    h[ncol] = cplx<double>{0.0, 0.0};
-    const cplx<double> gj = g_[j];
-    const cplx<double> t = cmul(s_[j], gj);
-    g_[j + 1] = cplx<double>{-t.re, -t.im};
-    g_[j] = cmul(cconj(c_[j]), gj);
-    const double est = hypot(g_[j + 1].re, g_[j + 1].im) / bn[k];
-    double* st = stat + (int64_t)k * 3 * (m + 1) + 3 * j;
-    double* sh = statHost + (int64_t)k * 3 * (m + 1) + 3 * j;
-    st[0] = sh[0] = est;
-    st[1] = sh[1] = hn;
-    st[2] = sh[2] = rho;
+    givensStep(hb, hb[j + 1].re, j, bn[k], cs + (int64_t)k * (m + 1), sn + (int64_t)k * (m + 1),
+               g + (int64_t)k * (m + 2), H + (int64_t)k * (m + 1) * (m + 1), m + 1, stat + (int64_t)k * 3 * (m + 1),
+               statHost + (int64_t)k * 3 * (m + 1), reinterpret_cast<cplx<double>*>(smem));
 }
 
 // Q_{j+1} lane k = W lane k / |w| (that lane's statistic), zero for lanes off
@@ -560,6 +620,7 @@ struct GivensWork {
     std::array<hipEvent_t, kWindow> ev{};
     cplx<double>* dotPart = nullptr;  // wide dot partials, (m + 1) x nbd
     double* normPart = nullptr;       // |w|^2 per update block
+    unsigned int* counter = nullptr;  // blocks done in kUpdateRGivens (zero between launches)
     int nbd = 0, nbu = 0;
     GivensWork(DevicePlan& dp, int m) : d(dp) {
         const int64_t n = d.hp.src.n;
@@ -567,6 +628,8 @@ struct GivensWork {
         nbd = (int)((n + (int64_t)KB * RPT - 1) / ((int64_t)KB * RPT));
         dotPart = (cplx<double>*)d.alloc((size_t)nbd * (m + 1) * sizeof(cplx<double>));
         normPart = (double*)d.alloc((size_t)nbu * sizeof(double));
+        counter = (unsigned int*)d.alloc(sizeof(unsigned int));
+        hipCheck(hipMemset(counter, 0, sizeof(unsigned int)), "hipMemset");
         cs = (cplx<double>*)d.alloc((size_t)(m + 1) * sizeof(cplx<double>));
         sn = (cplx<double>*)d.alloc((size_t)(m + 1) * sizeof(cplx<double>));
         g = (cplx<double>*)d.alloc((size_t)(m + 2) * sizeof(cplx<double>));
@@ -581,7 +644,8 @@ struct GivensWork {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (statHost) (void)hipHostFree(statHost);
-        for (void* p : {(void*)cs, (void*)sn, (void*)g, (void*)H, (void*)stat, (void*)dotPart, (void*)normPart})
+        for (void* p : {(void*)cs, (void*)sn, (void*)g, (void*)H, (void*)stat, (void*)dotPart, (void*)normPart,
+                        (void*)counter})
             d.release(p);
     }
 };
@@ -634,16 +698,20 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
             while (ncolUsed < 0 && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
                 d.applySolveOp(Q + (size_t)jq * n, w, 0, K.st, fp32, false);
                 const int ncol = jq + 1;
-                for (int pass = 0; pass < 2; ++pass) {  // CGS2
+                for (int pass = 0; pass < 2; ++pass) {  // CGS2; the second pass ends with the Givens step
                     hipLaunchKernelGGL(kDotWide, dim3(G.nbd, ncol), dim3(KB), 0, K.st, (const cplx<double>*)Q, n,
                                        (const cplx<double>*)w, n, G.dotPart);
-                    hipLaunchKernelGGL(kUpdateR, dim3(G.nbu), dim3(KB), ncol * sizeof(cplx<double>), K.st,
-                                       (const cplx<double>*)Q, n, ncol, (const cplx<double>*)G.dotPart, G.nbd,
-                                       K.h + (pass ? ncol + 1 : 0), w, n, G.normPart, pass);
+                    if (pass == 0)
+                        hipLaunchKernelGGL(kUpdateR, dim3(G.nbu), dim3(KB), ncol * sizeof(cplx<double>), K.st,
+                                           (const cplx<double>*)Q, n, ncol, (const cplx<double>*)G.dotPart, G.nbd,
+                                           K.h, w, n, G.normPart, 0);
+                    else
+                        hipLaunchKernelGGL(kUpdateRGivens, dim3(G.nbu), dim3(KB), (3 * ncol + 3) * sizeof(cplx<double>),
+                                           K.st, (const cplx<double>*)Q, n, ncol, (const cplx<double>*)G.dotPart,
+                                           G.nbd, K.h + ncol + 1, w, n, G.normPart,
+                                           GivensArgs{K.h, jq, bn, G.cs, G.sn, G.g, G.H, m + 1, G.stat, G.statDev,
+                                                      G.counter});
                 }
-                hipLaunchKernelGGL(kGivens, dim3(1), dim3(KB), 0, K.st, (const cplx<double>*)K.h,
-                                   (const double*)G.normPart, G.nbu, jq, bn, G.cs, G.sn, G.g, G.H, m + 1, G.stat,
-                                   G.statDev);
                 hipLaunchKernelGGL(kScaleDev, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<double>*)w,
                                    (const double*)(G.stat + 3 * jq + 1), Q + (size_t)(jq + 1) * n, n);
                 hipCheck(hipEventRecord(G.ev[jq % kWindow], K.st), "event");
@@ -852,7 +920,8 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
             while (!stop && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
                 applyMany(d, Q + (size_t)jq * ld, W, nrhs, st, fp32);
                 BO.queue(Q, ld, jq + 1, W, nullptr, st);
-                hipLaunchKernelGGL(kGivensB, dim3(nrhs), dim3(64), 0, st, (const cplx<double>*)BO.h, BO.hs, jq, m,
+                hipLaunchKernelGGL(kGivensB, dim3(nrhs), dim3(64), (3 * jq + 6) * sizeof(cplx<double>), st,
+                                   (const cplx<double>*)BO.h, BO.hs, jq, m,
                                    (const double*)GB.bn, (const int*)GB.on, GB.cs, GB.sn, GB.g, GB.H, GB.stat,
                                    GB.statDev);
                 hipLaunchKernelGGL(kScaleB, dim3(K[0]->nb, nrhs), dim3(KB), 0, st, (const cplx<double>*)W,
@@ -1054,9 +1123,16 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     const int method = opts->method & 0xff;
     struct AdjointMode {  // cleared however the solve ends
         DevicePlan& d;
-        ~AdjointMode() { d.solveAdjoint = false; }
+        ~AdjointMode() {
+            d.solveAdjoint = false;
+            d.solveInOrder = false;
+        }
     } adjointMode{d};
     d.solveAdjoint = (opts->method & MLFMA_SOLVE_ADJOINT) != 0;
+    // one right-hand side: the matvecs run their stages in order (the CGS
+    // launches between them leave the side streams nothing to overlap, and
+    // their fork and join events cost the chain; 2.7% per iteration at 30k)
+    d.solveInOrder = true;
     if (opts->method & ~(0xff | MLFMA_SOLVE_ADJOINT)) throw Error(MLFMA_ERR_ARG, "unknown solve method bits");
     if (method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
     hipCheck(hipSetDevice(d.device), "hipSetDevice");
