@@ -1227,7 +1227,7 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
             int mode = 0;
             for (int m : {1, 2}) {
                 d.tuneM2L[l] = m;
-                const double t = timeStage(2, 3, 3);
+                const double t = timeStage(2, 3, 8);
                 if (t < tb) {
                     tb = t;
                     mode = m;
@@ -1255,6 +1255,17 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
                     ts[m] += timeWhole(10);
                 }
             d.m2lSplit = ts[1] <= ts[0];
+            if (nrhs == 1) {
+                // between a solve's matvecs its own launches leave the side
+                // streams nothing to overlap, and their events cost the
+                // chain: one-right-hand-side solves run in order unless
+                // overlap makes the matvec clearly faster (> 2%)
+                const double tBest = timeWhole(10);
+                d.flags |= MLFMA_FLAG_ONE_STREAM;
+                const double tSerial = timeWhole(10);
+                d.flags &= ~MLFMA_FLAG_ONE_STREAM;
+                d.solveSerial = tSerial <= 1.02 * tBest;
+            }
         }
         const double after = timeWhole(10);
         if (ms_before) *ms_before = before;

@@ -164,8 +164,11 @@ struct DevicePlan {
                  int nb = 1, int64_t xs = 0);
     // the operator of the running solve: Z, or Z^H for an adjoint solve
     bool solveAdjoint = false;
-    // a one-right-hand-side solve is running: every stage on the solve's stream
+    // a one-right-hand-side solve is running with solveSerial set: every
+    // stage on the solve's stream; solveSerial is set by mlfma_autotune(1)
+    // when overlap does not make the matvec clearly faster
     bool solveInOrder = false;
+    bool solveSerial = false;
     void applySolveOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
         if (solveAdjoint)
             applyOpAdjoint(x, y, accumulate, st, fp32);

@@ -326,7 +326,9 @@ __device__ void givensStep(const cplx<double>* __restrict__ hbuf, double s2, int
 // coefficients) and counts itself in; the block that counts last, which has
 // then seen every other block's release, sums the partials in the same
 // order every time and runs the step.  No block waits for another.  The
-// counter is back at zero when the kernel ends.
+// counter is back at zero when the kernel ends.  One device fence per block
+// (by the thread that wrote the shared data): a fence releases by writing
+// back the XCD's L2.
 struct GivensArgs {
     const cplx<double>* hbuf;
     int j;
@@ -351,10 +353,7 @@ __global__ void kUpdateRGivens(const cplx<double>* __restrict__ Q, int64_t ld, i
         }
         re = waveSum(re);
         im = waveSum(im);
-        if (lane == 0) {
-            sh[i] = cplx<double>{re, im};
-            if (blockIdx.x == 0) hout[i] = cplx<double>{re, im};
-        }
+        if (lane == 0) sh[i] = cplx<double>{re, im};
     }
     __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
@@ -372,11 +371,16 @@ __global__ void kUpdateRGivens(const cplx<double>* __restrict__ Q, int64_t ld, i
     }
     double z = 0;
     blockSum2(s2, z);
+    // thread 0 alone writes what other blocks read (the partial; block 0 the
+    // coefficients), releases it and counts the block in
     __shared__ unsigned int rank;
-    if (threadIdx.x == 0) normPart[blockIdx.x] = s2;
-    __threadfence();  // release, by every thread: the block's partial (and block 0's coefficients)
-    __syncthreads();
-    if (threadIdx.x == 0) rank = atomicAdd(ga.counter, 1u);
+    if (threadIdx.x == 0) {
+        if (blockIdx.x == 0)
+            for (int i = 0; i < ncol; ++i) hout[i] = sh[i];
+        normPart[blockIdx.x] = s2;
+        __threadfence();
+        rank = atomicAdd(ga.counter, 1u);
+    }
     __syncthreads();
     if (rank != gridDim.x - 1) return;  // uniform over the block
     __threadfence();  // acquire: every other block's partial
@@ -1129,10 +1133,9 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
         }
     } adjointMode{d};
     d.solveAdjoint = (opts->method & MLFMA_SOLVE_ADJOINT) != 0;
-    // one right-hand side: the matvecs run their stages in order (the CGS
-    // launches between them leave the side streams nothing to overlap, and
-    // their fork and join events cost the chain; 2.7% per iteration at 30k)
-    d.solveInOrder = true;
+    // one right-hand side: the matvecs run their stages in order when the
+    // autotuner found that overlap does not pay (2.7% per iteration at 30k)
+    d.solveInOrder = d.solveSerial;
     if (opts->method & ~(0xff | MLFMA_SOLVE_ADJOINT)) throw Error(MLFMA_ERR_ARG, "unknown solve method bits");
     if (method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
     hipCheck(hipSetDevice(d.device), "hipSetDevice");
