@@ -289,8 +289,8 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
  * chain (on a second stream of the plan, forked before or after S2M) or after
  * it, and whether the finest level's M2L runs beside the M2M chain (on a
  * third stream) or in line; with nrhs = 1 also whether mlfma_solve runs its
- * matvecs in order (when the side streams make the matvec no more than 2%
- * faster).  Mean matvec device
+ * matvecs in order (whichever of the two makes short GMRES runs on the
+ * plan's excitation faster).  Mean matvec device
  * time before and after (ms, may be NULL).  Results do not depend on the
  * choice beyond round-off.  No reference counterpart. */
 int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms_after);

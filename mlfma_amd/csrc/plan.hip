@@ -1255,16 +1255,29 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
                     ts[m] += timeWhole(10);
                 }
             d.m2lSplit = ts[1] <= ts[0];
-            if (nrhs == 1) {
-                // between a solve's matvecs its own launches leave the side
-                // streams nothing to overlap, and their events cost the
-                // chain: one-right-hand-side solves run in order unless
-                // overlap makes the matvec clearly faster (> 2%)
-                const double tBest = timeWhole(10);
-                d.flags |= MLFMA_FLAG_ONE_STREAM;
-                const double tSerial = timeWhole(10);
-                d.flags &= ~MLFMA_FLAG_ONE_STREAM;
-                d.solveSerial = tSerial <= 1.02 * tBest;
+            if (nrhs == 1 && d.Vin.size() == (size_t)n) {
+                // a solve interleaves its own launches with the matvecs, so
+                // the stage order of one-right-hand-side solves is chosen by
+                // timing short GMRES runs both ways
+                const uint32_t f = d.flags;
+                d.flags &= ~MLFMA_FLAG_STAGE_TIMES;
+                double tOverlap = 0, tSerial = 0;
+                try {
+                    timeSolveSteps(d, 8);
+                    for (int rep = 0; rep < 2; ++rep) {
+                        d.solveInOrder = false;
+                        tOverlap += timeSolveSteps(d, 24);
+                        d.solveInOrder = true;
+                        tSerial += timeSolveSteps(d, 24);
+                    }
+                } catch (...) {
+                    d.solveInOrder = false;
+                    d.flags = f;
+                    throw;
+                }
+                d.solveInOrder = false;
+                d.flags = f;
+                d.solveSerial = tSerial < tOverlap;
             }
         }
         const double after = timeWhole(10);

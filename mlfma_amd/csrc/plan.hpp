@@ -166,7 +166,7 @@ struct DevicePlan {
     bool solveAdjoint = false;
     // a one-right-hand-side solve is running with solveSerial set: every
     // stage on the solve's stream; solveSerial is set by mlfma_autotune(1)
-    // when overlap does not make the matvec clearly faster
+    // when short GMRES runs are faster that way
     bool solveInOrder = false;
     bool solveSerial = false;
     void applySolveOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
@@ -202,6 +202,9 @@ template <>
 inline const cplx<double>* DevicePlan::radpat<double>() { return rp64; }
 template <>
 inline const cplx<float>* DevicePlan::radpat<float>() { return rp32; }
+
+// seconds of iters GMRES iterations on the plan's excitation (solver.hip)
+double timeSolveSteps(DevicePlan& d, int iters);
 
 }  // namespace mlfma
 

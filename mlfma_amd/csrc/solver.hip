@@ -1088,6 +1088,32 @@ static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<do
     return out;
 }
 
+// Seconds of `iters` GMRES(iters) iterations from x = 0 on the plan's
+// excitation: what mlfma_autotune times to choose the stage order of
+// one-right-hand-side solves.
+double timeSolveSteps(DevicePlan& d, int iters) {
+    const int64_t n = d.hp.src.n;
+    hipStream_t st = d.stream;
+    d.ensureRhs(1);
+    std::vector<cplx<double>> bh(n);
+    for (int64_t i = 0; i < n; ++i) bh[i] = cplx<double>{-d.Vin[i].real(), -d.Vin[i].imag()};
+    cplx<double>* bt = (cplx<double>*)d.alloc(n * sizeof(cplx<double>));
+    cplx<double>* xt = (cplx<double>*)d.alloc(n * sizeof(cplx<double>));
+    hipCheck(hipMemcpyAsync(bt, bh.data(), n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "b upload");
+    double secs = 0;
+    {
+        Krylov K(d, st, iters + 2);
+        hipCheck(hipStreamSynchronize(st), "autotune");
+        const auto t0 = std::chrono::steady_clock::now();
+        gmresRestart(d, K, bt, xt, iters, iters, 1e-300, false, true, nullptr, 0);
+        hipCheck(hipStreamSynchronize(st), "autotune");
+        secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    d.release(bt);
+    d.release(xt);
+    return secs;
+}
+
 }  // namespace mlfma
 
 using namespace mlfma;
