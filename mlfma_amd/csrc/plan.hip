@@ -731,7 +731,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         LevelBufs& C = levels[l + 1];
         if (!launchM2MFused<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph, p, P.thNear,
                                P.thW, P.phNear, P.phW, C.phUp<R>(), M + C.Moff, M + P.Moff, ldsMax, st, nb, fs,
-                               tuneMinBlocks)) {
+                               minBlocksAt(l))) {
             for (int r = 0; r < nb; ++r) {
                 launchM2MTheta<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, C.nth, C.nph, p, P.thNear,
                                   P.thW, C.phUp<R>(), M + r * fs + C.Moff, inner, st);
@@ -806,7 +806,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
         LevelBufs& C = levels[l];
         if (!launchL2LAdj<R>(C.nl2lPar, C.l2lParSlot, C.l2lParPtr, C.l2lParChild, C.l2lParOct, P.nth, P.nph, C.nth,
                              C.nph, p, P.thNear, P.thW, P.phNear, P.phW, P.phDn<R>(), Lc + C.Moff, Lc + P.Moff, ldsMax,
-                             st, tuneMinBlocks))
+                             st, minBlocksAt(l - 1)))
             throw Error(MLFMA_ERR_ARG, "adjoint of the L2L into level " + std::to_string(l) +
                                            ": the level's fields exceed the LDS of one block (the adjoint has no "
                                            "two-pass path; use fewer digits or smaller leaf boxes)");
@@ -1207,18 +1207,24 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
     };
     try {
         const double before = timeWhole(10);
-        // M2M: block target for the parent row split
-        double best = 1e300;
-        int bm = 0;
-        for (int mb : {256, 512, 1024, 2048, 4096}) {
-            d.tuneMinBlocks = mb;
-            const double t = timeStage(1, 2, 3);
-            if (t < best) {
-                best = t;
-                bm = mb;
+        // M2M: block target for the parent row split, level by level (the
+        // split re-sums the children in every block, so few large parents
+        // and many small ones want different targets)
+        d.tuneMinBlocks.assign(d.hp.tree.maxLevel + 1, 0);
+        for (int l = d.hp.tree.maxLevel - 1; l >= 2; --l) {
+            if (d.levels[l].nstem == 0) continue;
+            double best = 1e300;
+            int bm = 0;
+            for (int mb : {128, 256, 512, 1024, 2048, 4096}) {
+                d.tuneMinBlocks[l] = mb;
+                const double t = timeStage(1, 2, 4);
+                if (t < best) {
+                    best = t;
+                    bm = mb;
+                }
             }
+            d.tuneMinBlocks[l] = bm;
         }
-        d.tuneMinBlocks = bm;
         // M2L: sibling-group or per-target kernel, level by level
         d.tuneM2L.assign(d.hp.tree.maxLevel + 1, 0);
         for (int l = 2; l <= d.hp.tree.maxLevel; ++l) {

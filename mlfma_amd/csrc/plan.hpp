@@ -140,9 +140,12 @@ struct DevicePlan {
                int64_t xs = 0);
     int batchCap = 1;  // right-hand sides the field arrays and partials hold
     // launch parameters chosen by mlfma_autotune (0: compile-time defaults /
-    // the per-level rule); tuneM2L[l]: 1 sibling-group, 2 per-target M2L
-    int tuneMinBlocks = 0;
+    // the per-level rule); tuneMinBlocks[l]: block target of the M2M (and
+    // adjoint L2L) row split into parent level l; tuneM2L[l]: 1 sibling-group,
+    // 2 per-target M2L
+    std::vector<int> tuneMinBlocks;
     std::vector<int> tuneM2L;
+    int minBlocksAt(int l) const { return l < (int)tuneMinBlocks.size() ? tuneMinBlocks[l] : 0; }
     // captured matvecs (MLFMA_FLAG_GRAPH), least recently used last
     struct GraphEntry {
         const void* x;
