@@ -1225,21 +1225,41 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
             }
             d.tuneMinBlocks[l] = bm;
         }
-        // M2L: sibling-group or per-target kernel, level by level
+        // M2L: sibling-group or per-target kernel per level, every
+        // combination when there are at most five such levels (a level at a
+        // time from the previous choice oscillated between two settings)
         d.tuneM2L.assign(d.hp.tree.maxLevel + 1, 0);
-        for (int l = 2; l <= d.hp.tree.maxLevel; ++l) {
-            if (d.levels[l].ntgt == 0) continue;
-            double tb = 1e300;
-            int mode = 0;
-            for (int m : {1, 2}) {
-                d.tuneM2L[l] = m;
-                const double t = timeStage(2, 3, 8);
-                if (t < tb) {
-                    tb = t;
-                    mode = m;
+        {
+            std::vector<int> lv;
+            for (int l = 2; l <= d.hp.tree.maxLevel; ++l)
+                if (d.levels[l].ntgt > 0) lv.push_back(l);
+            if (lv.size() <= 5) {
+                double tb = 1e300;
+                unsigned bestMask = 0;
+                for (unsigned mask = 0; mask < (1u << lv.size()); ++mask) {
+                    for (size_t i = 0; i < lv.size(); ++i) d.tuneM2L[lv[i]] = (mask >> i & 1u) ? 2 : 1;
+                    const double t = timeStage(2, 3, 4);
+                    if (t < tb) {
+                        tb = t;
+                        bestMask = mask;
+                    }
+                }
+                for (size_t i = 0; i < lv.size(); ++i) d.tuneM2L[lv[i]] = (bestMask >> i & 1u) ? 2 : 1;
+            } else {
+                for (int l : lv) {
+                    double tb = 1e300;
+                    int mode = 0;
+                    for (int m : {1, 2}) {
+                        d.tuneM2L[l] = m;
+                        const double t = timeStage(2, 3, 8);
+                        if (t < tb) {
+                            tb = t;
+                            mode = m;
+                        }
+                    }
+                    d.tuneM2L[l] = mode;
                 }
             }
-            d.tuneM2L[l] = mode;
         }
         // near field beside the far-field chain, or after it
         if (!(saved & MLFMA_FLAG_ONE_STREAM)) {
