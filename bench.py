@@ -228,7 +228,17 @@ def cpu_baseline(geom, pw, kw, budget_s=20.0, y_gpu=None):
         o.matvec(x)
         times.append(time.time() - t)
     med = float(np.median(times))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": round(1.0 / med, 4), "unit": "matvec/s", "cores": 1, "kind": "port",
+            "host": {"cpu_model": model, "logical_cpus": os.cpu_count()},
             "gpu_vs_oracle_rel_diff": parity,
             "sample": f"{len(times)} FP64 matvecs of the same workload by the CPU oracle (single-threaded "
                       f"restatement of the reference); median {med:.3f} s; oracle setup {setup:.1f} s on {nt} threads "
