@@ -869,95 +869,71 @@ template <bool CONJ, typename V>
 __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
                                                const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
                                                const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
-    // 30 KiB of LDS per block (five blocks per CU): the row-sum reduction at
-    // the end reuses the tile's storage, the column sums reduce by shuffles
-    __shared__ cplx<double> sT[NT];
+    // Every load of a lane is 16 bytes: one FP64 value or two FP32 values,
+    // so an FP32 tile holds twice the entries in the same 24 KiB of LDS (the
+    // tile is kept in V).  30 KiB of LDS per block (five blocks per CU): the
+    // row-sum reduction at the end reuses the tile's storage, the column sums
+    // reduce by shuffles.
+    constexpr int EPL = sizeof(V) == 4 ? 2 : 1;  // values per lane and load
+    constexpr int NL = MLFMA_NEAR_LOADS;
+    constexpr int NTV = NT * EPL;
+    __shared__ cplx<V> sT[NTV];
     __shared__ cplx<double> sxr[128], sxc[TB];
     const NearItem I = items[blockIdx.x];
     const int nr = I.nr;        // 1..128
     const int ld = nr | 1;      // odd leading dimension in LDS
-    const int cwMax = min(TB, NT / ld);
+    const int cwMax = min(TB, NTV / ld);
     if (threadIdx.x < nr) sxr[threadIdx.x] = x[perm[I.rbeg + threadIdx.x]];
     const int G = TB / nr;
     const int r = threadIdx.x % nr, g = threadIdx.x / nr;
-    double re = 0, im = 0;
-    // Software pipeline: the loads per lane of tile k+1 are issued before
-    // the LDS work of tile k.  Values are held as plain doubles (a struct or
-    // array here is left in scratch memory by the compiler); indices are
-    // clamped into the tile so every load is valid.
     const int t0 = threadIdx.x;
-    int c0 = 0;
-    int cw = min(cwMax, I.nc);
-    int ne = cw * nr;
-    double r0, r1, r2, r3, r4, i0, i1, i2, i3, i4;
-#if MLFMA_NEAR_LOADS >= 6
-    double r5, i5;
-#endif
-#if MLFMA_NEAR_LOADS == 8
-    double r6, r7, i6, i7;
-#endif
+    const int ntile = (I.nc + cwMax - 1) / cwMax;
+    auto tileCw = [&](int j) { return min(cwMax, I.nc - j * cwMax); };
+    // Software pipeline: the loads of tile j+1 are issued before the LDS
+    // work of tile j.  Indices are clamped into the tile so every load is
+    // valid; the column indices of the x gathers run one tile ahead of the
+    // gathers that use them, so that a tile's loads wait for one round trip.
+    int ci = t0 < tileCw(0) ? colsIn[I.cptr + t0] : 0;
+    V vr[NL][EPL], vi[NL][EPL];
     cplx<double> xc = mk(0.0, 0.0);
-    // column indices run one tile ahead of the x gathers that use them, so
-    // that a tile's loads wait for one round trip, not two
-    int ci = t0 < cw ? colsIn[I.cptr + t0] : 0;
-#define NEAR_LD1(K, C0, NE)                                                       \
-    {                                                                            \
-        const int q = 2 * min(t0 + K * TB, (NE) - 1);                            \
-        r##K = (double)sd[q];                                                    \
-        i##K = CONJ ? -(double)sd[q + 1] : (double)sd[q + 1];                    \
-    }
-#define NEAR_LD(C0, NE, CW)                                                                          \
-    {                                                                                               \
-        const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)(C0) * nr);                \
-        NEAR_LD1(0, C0, NE) NEAR_LD1(1, C0, NE) NEAR_LD1(2, C0, NE)                                 \
-        NEAR_LD1(3, C0, NE) NEAR_LD1(4, C0, NE)                                                     \
-        NEAR_LD_EXTRA(C0, NE)                                                                       \
-        if (t0 < (CW)) xc = x[ci];                                                                  \
-        const int cn = (C0) + cwMax;                                                                \
-        ci = t0 < min(cwMax, I.nc - cn) ? colsIn[I.cptr + cn + t0] : 0;                             \
-    }
-#if MLFMA_NEAR_LOADS == 8
-#define NEAR_LD_EXTRA(C0, NE) NEAR_LD1(5, C0, NE) NEAR_LD1(6, C0, NE) NEAR_LD1(7, C0, NE)
-#elif MLFMA_NEAR_LOADS == 6
-#define NEAR_LD_EXTRA(C0, NE) NEAR_LD1(5, C0, NE)
-#else
-#define NEAR_LD_EXTRA(C0, NE)
-#endif
-    NEAR_LD(0, ne, cw)
-    double* sTd = reinterpret_cast<double*>(sT);
-    while (true) {
-        __syncthreads();  // the previous tile is consumed
-#define NEAR_ST(K)                                            \
-    if (t0 + K * TB < ne) {                                   \
-        const int e = t0 + K * TB;                            \
-        const int c = e / nr;                                 \
-        const int a = 2 * (c * ld + (e - c * nr));            \
-        sTd[a] = r##K;                                        \
-        sTd[a + 1] = i##K;                                    \
-    }
-        NEAR_ST(0) NEAR_ST(1) NEAR_ST(2) NEAR_ST(3) NEAR_ST(4)
-#if MLFMA_NEAR_LOADS >= 6
-        NEAR_ST(5)
-#endif
-#if MLFMA_NEAR_LOADS == 8
-        NEAR_ST(6) NEAR_ST(7)
-#endif
-#undef NEAR_ST
+    auto issue = [&](int j) {
+        const int c0 = j * cwMax, cw = tileCw(j), ne = cw * nr;
+        const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)c0 * nr);
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+#pragma unroll
+            for (int e = 0; e < EPL; ++e) {
+                const int q = 2 * min((t0 + k * TB) * EPL + e, ne - 1);
+                vr[k][e] = sd[q];
+                vi[k][e] = CONJ ? -sd[q + 1] : sd[q + 1];
+            }
+        if (t0 < cw) xc = x[ci];
+        const int cn = c0 + cwMax;
+        ci = t0 < min(cwMax, I.nc - cn) ? colsIn[I.cptr + cn + t0] : 0;
+    };
+    auto stage = [&](int j) {
+        const int cw = tileCw(j), ne = cw * nr;
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+#pragma unroll
+            for (int e = 0; e < EPL; ++e) {
+                const int q = (t0 + k * TB) * EPL + e;
+                if (q < ne) {
+                    const int c = q / nr;
+                    sT[c * ld + (q - c * nr)] = cplx<V>{vr[k][e], vi[k][e]};
+                }
+            }
         if (t0 < cw) sxc[t0] = xc;
-        __syncthreads();
-        const int c0n = c0 + cwMax;
-        const bool more = c0n < I.nc;  // uniform over the block
-        int cwn = 0, nen = 0;
-        if (more) {
-            cwn = min(cwMax, I.nc - c0n);
-            nen = cwn * nr;
-            NEAR_LD(c0n, nen, cwn)
-        }
+    };
+    double re = 0, im = 0;
+    auto compute = [&](int j) {
+        const int c0 = j * cwMax, cw = tileCw(j);
         if (g < G)
             for (int c = g; c < cw; c += G) {
-                const cplx<double> z = sT[c * ld + r], xx = sxc[c];
-                re += z.re * xx.re - z.im * xx.im;
-                im += z.re * xx.im + z.im * xx.re;
+                const cplx<V> z = sT[c * ld + r];
+                const cplx<double> xx = sxc[c];
+                re += (double)z.re * xx.re - (double)z.im * xx.im;
+                im += (double)z.re * xx.im + (double)z.im * xx.re;
             }
         // column sums of the pair columns: S lanes per column (a power of two
         // <= 64, so a column's lanes sit in one wave), each over a contiguous
@@ -966,32 +942,34 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
             int S = 1;
             while (S < 64 && 2 * S * cw <= TB) S *= 2;
             const int cc = t0 / S, sg = t0 - cc * S;
-            double cr = 0, ci = 0;
+            double cr = 0, cm = 0;
             const bool mine = cc < cw && c0 + cc >= I.nself;
             if (mine) {
                 const int L = (nr + S - 1) / S;
                 const int qa = min(nr, sg * L), qb = min(nr, qa + L);
-                const cplx<double>* col = sT + cc * ld;
+                const cplx<V>* col = sT + cc * ld;
                 for (int q = qa; q < qb; ++q) {
-                    const cplx<double> z0 = col[q], x0 = sxr[q];
-                    cr += z0.re * x0.re - z0.im * x0.im;
-                    ci += z0.re * x0.im + z0.im * x0.re;
+                    const cplx<V> z0 = col[q];
+                    const cplx<double> x0 = sxr[q];
+                    cr += (double)z0.re * x0.re - (double)z0.im * x0.im;
+                    cm += (double)z0.re * x0.im + (double)z0.im * x0.re;
                 }
             }
             for (int o = S / 2; o > 0; o >>= 1) {  // every lane of the wave takes part
                 cr += __shfl_xor(cr, o, 64);
-                ci += __shfl_xor(ci, o, 64);
+                cm += __shfl_xor(cm, o, 64);
             }
-            if (mine && sg == 0) part[I.cpo + c0 + cc - I.nself] = mk(cr, ci);
+            if (mine && sg == 0) part[I.cpo + c0 + cc - I.nself] = mk(cr, cm);
         }
-        if (!more) break;
-        c0 = c0n;
-        cw = cwn;
-        ne = nen;
+    };
+    issue(0);
+    for (int j = 0; j < ntile; ++j) {
+        __syncthreads();  // the previous tile is consumed
+        stage(j);
+        __syncthreads();
+        if (j + 1 < ntile) issue(j + 1);
+        compute(j);
     }
-#undef NEAR_LD
-#undef NEAR_LD1
-#undef NEAR_LD_EXTRA
     double* sre = reinterpret_cast<double*>(sT);  // the tile's storage, free now
     double* sim = sre + TB;
     __syncthreads();  // every lane is done with the last tile
