@@ -414,10 +414,11 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
 // many independent pairs are in flight per direction instead of a long
 // dependent walk of index and field loads; the
 // wave partials are added in LDS in a fixed order.  Lanes past the level's
-// direction count read a clamped direction and store nothing.
+// direction count read a clamped direction and store nothing.  bid: the
+// block's index in the M2L launch; red: 3 x 64 x 2 values of LDS.
 template <typename R, bool CONJ>
-__global__ __launch_bounds__(TB) void kM2Lw(M2LLevels<R> lv) {
-    int b = blockIdx.x, l = 0;
+__device__ __forceinline__ void m2lwBody(int bid, const M2LLevels<R>& lv, cplx<R> (*red)[64][2]) {
+    int b = bid, l = 0;
     while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
     b -= lv.blk[l];
     const int nch = lv.nch[l];
@@ -480,7 +481,6 @@ __global__ __launch_bounds__(TB) void kM2Lw(M2LLevels<R> lv) {
             a1 = a1 + t0 * m0[1];
         }
     }
-    __shared__ cplx<R> red[3][64][2];
     if (w > 0) {
         red[w - 1][lane][0] = a0 + b0;
         red[w - 1][lane][1] = a1 + b1;
@@ -491,6 +491,12 @@ __global__ __launch_bounds__(TB) void kM2Lw(M2LLevels<R> lv) {
         out[0] = ((a0 + b0) + red[0][lane][0]) + (red[1][lane][0] + red[2][lane][0]);
         out[1] = ((a1 + b1) + red[0][lane][1]) + (red[1][lane][1] + red[2][lane][1]);
     }
+}
+
+template <typename R, bool CONJ>
+__global__ __launch_bounds__(TB) void kM2Lw(M2LLevels<R> lv) {
+    __shared__ cplx<R> red[3][64][2];
+    m2lwBody<R, CONJ>(blockIdx.x, lv, red);
 }
 
 template <typename R>
@@ -864,11 +870,14 @@ static_assert(MLFMA_NEAR_LOADS == 5 || MLFMA_NEAR_LOADS == 6 || MLFMA_NEAR_LOADS
 constexpr int NT = MLFMA_NEAR_LOADS * TB;  // values per tile
 
 // V: the stored value type (double; float for the FP32 inner solves of the
-// mixed-precision path, products and sums still in FP64)
+// mixed-precision path, products and sums still in FP64).  bid: the item;
+// sT, sxr, sxc: LDS of NT x (16 / sizeof(cplx<V>)) values, 128 and TB values.
 template <bool CONJ, typename V>
-__global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
-                                               const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
-                                               const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
+__device__ __forceinline__ void nearSymBody(int bid, const NearItem* __restrict__ items,
+                                            const int32_t* __restrict__ colsIn, const int64_t* __restrict__ perm,
+                                            const cplx<V>* __restrict__ val, const cplx<double>* __restrict__ x,
+                                            cplx<double>* __restrict__ part, cplx<V>* sT, cplx<double>* sxr,
+                                            cplx<double>* sxc) {
     // Every load of a lane is 16 bytes: one FP64 value or two FP32 values,
     // so an FP32 tile holds twice the entries in the same 24 KiB of LDS (the
     // tile is kept in V).  30 KiB of LDS per block (five blocks per CU): the
@@ -877,9 +886,7 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     constexpr int EPL = sizeof(V) == 4 ? 2 : 1;  // values per lane and load
     constexpr int NL = MLFMA_NEAR_LOADS;
     constexpr int NTV = NT * EPL;
-    __shared__ cplx<V> sT[NTV];
-    __shared__ cplx<double> sxr[128], sxc[TB];
-    const NearItem I = items[blockIdx.x];
+    const NearItem I = items[bid];
     const int nr = I.nr;        // 1..128
     const int ld = nr | 1;      // odd leading dimension in LDS
     const int cwMax = min(TB, NTV / ld);
@@ -899,14 +906,28 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     auto issue = [&](int j) {
         const int c0 = j * cwMax, cw = tileCw(j), ne = cw * nr;
         const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)c0 * nr);
+        if constexpr (EPL == 1) {
+            // FP64 values are read once per matvec: non-temporal loads, so
+            // that they do not evict the patterns and fields the other stages
+            // re-read from L2 and the MALL
+            using d2 = double __attribute__((ext_vector_type(2)));
+            const d2* sv = reinterpret_cast<const d2*>(sd);
 #pragma unroll
-        for (int k = 0; k < NL; ++k)
-#pragma unroll
-            for (int e = 0; e < EPL; ++e) {
-                const int q = 2 * min((t0 + k * TB) * EPL + e, ne - 1);
-                vr[k][e] = sd[q];
-                vi[k][e] = CONJ ? -sd[q + 1] : sd[q + 1];
+            for (int k = 0; k < NL; ++k) {
+                const d2 v = __builtin_nontemporal_load(sv + min(t0 + k * TB, ne - 1));
+                vr[k][0] = v.x;
+                vi[k][0] = CONJ ? -v.y : v.y;
             }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NL; ++k)
+#pragma unroll
+                for (int e = 0; e < EPL; ++e) {
+                    const int q = 2 * min((t0 + k * TB) * EPL + e, ne - 1);
+                    vr[k][e] = sd[q];
+                    vi[k][e] = CONJ ? -sd[q + 1] : sd[q + 1];
+                }
+        }
         if (t0 < cw) xc = x[ci];
         const int cn = c0 + cwMax;
         ci = t0 < min(cwMax, I.nc - cn) ? colsIn[I.cptr + cn + t0] : 0;
@@ -984,6 +1005,15 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
         }
         part[I.rpo + threadIdx.x] = mk(a, b);
     }
+}
+
+template <bool CONJ, typename V>
+__global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
+                                               const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
+                                               const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
+    __shared__ cplx<V> sT[NT * (sizeof(V) == 4 ? 2 : 1)];
+    __shared__ cplx<double> sxr[128], sxc[TB];
+    nearSymBody<CONJ, V>(blockIdx.x, items, colsIn, perm, val, x, part, sT, sxr, sxc);
 }
 
 // The same for NB >= 2 right-hand sides (x column stride xs, partial sums of
@@ -1187,6 +1217,57 @@ template void launchNearSym<double>(bool, const NearItem*, int, const int32_t*, 
                                     const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t);
 template void launchNearSym<float>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<float>*,
                                    const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t);
+
+// The per-target M2L of one right-hand side and the near-field pairs in one
+// launch (stages in order on one stream): both read only inputs of the matvec
+// (multipoles; x) and write disjoint outputs (locals; partial sums), and the
+// cache-bound M2L leaves HBM bandwidth that the pair stream uses.  Runs of 8
+// blocks (one per XCD) alternate between the two in proportion to their
+// counts, so both run from the first wave on; an M2L block keeps its index
+// modulo 8, hence its XCD, and the LDS is one buffer sized for the larger.
+template <typename R, typename V>
+__global__ __launch_bounds__(TB) void kM2LNear(M2LLevels<R> lv, int gN, int gT, const NearItem* __restrict__ items,
+                                               int nitems, const int32_t* __restrict__ colsIn,
+                                               const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
+                                               const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
+    constexpr int EPL = sizeof(V) == 4 ? 2 : 1;
+    constexpr int SZN = NT * EPL * (int)sizeof(cplx<V>) + (128 + TB) * (int)sizeof(cplx<double>);
+    constexpr int SZM = 3 * 64 * 2 * (int)sizeof(cplx<R>);
+    __shared__ __align__(16) unsigned char sm[SZN > SZM ? SZN : SZM];
+    const int g = blockIdx.x >> 3, xx = blockIdx.x & 7;
+    const int f0 = (int)((int64_t)g * gN / gT), f1 = (int)((int64_t)(g + 1) * gN / gT);
+    if (f1 > f0) {
+        const int item = 8 * f0 + xx;
+        if (item >= nitems) return;
+        cplx<V>* sT = reinterpret_cast<cplx<V>*>(sm);
+        cplx<double>* sxr = reinterpret_cast<cplx<double>*>(sm + NT * EPL * sizeof(cplx<V>));
+        nearSymBody<false, V>(item, items, colsIn, perm, val, x, part, sT, sxr, sxr + 128);
+        return;
+    }
+    m2lwBody<R, false>(8 * (g - f0) + xx, lv, reinterpret_cast<cplx<R>(*)[64][2]>(sm));
+}
+
+template <typename R, typename V>
+void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st) {
+    int total = 0;
+    for (int l = 0; l < lv.nlev; ++l) {
+        lv.nch[l] = (lv.ndir[l] + 63) / 64;
+        lv.per[l] = (lv.ntgt[l] + 7) / 8;
+        lv.blk[l] = total;
+        total += 8 * lv.per[l] * lv.nch[l];
+    }
+    const int gM = total / 8, gN = (nitems + 7) / 8, gT = gM + gN;
+    if (gT == 0) return;
+    hipLaunchKernelGGL((kM2LNear<R, V>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gT, items, nitems, colsIn, perm, val,
+                       x, part);
+}
+template void launchM2LNear<double, double>(M2LLevels<double>, const NearItem*, int, const int32_t*, const int64_t*,
+                                            const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t);
+template void launchM2LNear<float, float>(M2LLevels<float>, const NearItem*, int, const int32_t*, const int64_t*,
+                                          const cplx<float>*, const cplx<double>*, cplx<double>*, hipStream_t);
+template void launchM2LNear<float, double>(M2LLevels<float>, const NearItem*, int, const int32_t*, const int64_t*,
+                                           const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t);
 
 // FP32 copy of the near-field values (the mixed-precision inner solves)
 __global__ void kToFloat(const cplx<double>* __restrict__ in, cplx<float>* __restrict__ out, int64_t n) {

@@ -609,8 +609,11 @@ void DevicePlan::ensureRhs(int nrhs) {
 // sources (at least three list entries per distinct source) and that fill the
 // device go through the sibling-group kernel, the others through the
 // per-target kernel.  adjoint: conjugated operators of the reversed pairs.
+// nearX (one right-hand side, forward): the near-field pairs of x go into the
+// launch of the per-target levels.
 template <typename R>
-void DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb, int lmin, int lmax) {
+bool DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb, int lmin, int lmax,
+                           const cplx<double>* nearX) {
     const int ML = hp.tree.maxLevel;
     if (ML - 1 > kMaxLevels) throw Error(MLFMA_ERR_ARG, "too many tree levels");
     M2LGroups<R> g;
@@ -649,7 +652,18 @@ void DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStr
         }
     }
     if (g.nlev) launchM2LGroup<R>(g, adjoint, st, nb);
+    if (nearX && t.nlev && nb == 1 && !adjoint && nNearItems > 0) {
+        // the FP32 far field reads the FP32 copy of the near-field values
+        if constexpr (std::is_same<R, float>::value)
+            if (nearVal32) {
+                launchM2LNear<R, float>(t, nearItems, nNearItems, nearColsIn, perm, nearVal32, nearX, nearPart, st);
+                return true;
+            }
+        launchM2LNear<R, double>(t, nearItems, nNearItems, nearColsIn, perm, nearVal, nearX, nearPart, st);
+        return true;
+    }
     if (t.nlev) launchM2Lx<R>(t, adjoint, st, nb);
+    return false;
 }
 
 template <typename R>
@@ -669,8 +683,10 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // so they overlap the far-field chain, before S2M (nearOverlap 1), after
     // it (2: the bandwidth-bound pair products then run beside the
     // latency-bound tree stages) or after the M2M chain (3: the M2M launches,
-    // with few blocks at coarse levels, then run uncontended); serial when
-    // stages are timed
+    // with few blocks at coarse levels, then run uncontended); in order (0)
+    // they go into the per-target M2L launch of one right-hand side, which
+    // leaves them HBM bandwidth, or else (stages timed, batches, no
+    // per-target level) follow the L2L
     const bool serial = timed || (flags & MLFMA_FLAG_ONE_STREAM) || solveInOrder;
     const int overlap = serial ? 0 : nearOverlap;
     // M2L of the finest level needs only the leaf multipoles, and its locals
@@ -742,7 +758,10 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     }
     if (overlap == 3) forkSides();
     if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
-    launchM2L<R>(M, Lc, false, st, nb, 2, splitM2L ? ML - 1 : ML);
+    // stages in order: the near-field pairs share the per-target M2L launch
+    // (not when stages are timed, so that each has its own events)
+    const bool nearFused =
+        launchM2L<R>(M, Lc, false, st, nb, 2, splitM2L ? ML - 1 : ML, !overlap && !timed ? x : nullptr);
     if (flags & MLFMA_FLAG_FRESH_LOCALS)
         for (int r = 0; r < nb; ++r)
             for (int l = 2; l <= ML; ++l) {
@@ -767,7 +786,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     }
     joinSide();
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
-    if (!overlap) launchNear(st);
+    if (!overlap && !nearFused) launchNear(st);
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
@@ -1261,26 +1280,28 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
                 }
             }
         }
-        // near field beside the far-field chain, or after it
+        // near field beside the far-field chain, after it or (mode 0) in the
+        // M2L launch, and the finest-level M2L beside the M2M chain or in
+        // line: the eight combinations, since the in-order modes pay only
+        // when both side streams are off
         if (!(saved & MLFMA_FLAG_ONE_STREAM)) {
-            double tm[4] = {0, 0, 0, 0};
+            double tm[4][2] = {};
             for (int rep = 0; rep < 2; ++rep)
-                for (int m = 0; m < 4; ++m) {
-                    d.nearOverlap = m;
-                    tm[m] += timeWhole(10);
-                }
-            int bestMode = 2;
+                for (int m = 0; m < 4; ++m)
+                    for (int sp = 0; sp < 2; ++sp) {
+                        d.nearOverlap = m;
+                        d.m2lSplit = sp == 1;
+                        tm[m][sp] += timeWhole(10);
+                    }
+            int bm = 2, bs = 1;
             for (int m = 0; m < 4; ++m)
-                if (tm[m] < tm[bestMode]) bestMode = m;
-            d.nearOverlap = bestMode;
-            // finest-level M2L beside the M2M chain, or in line
-            double ts[2] = {0, 0};
-            for (int rep = 0; rep < 2; ++rep)
-                for (int m = 0; m < 2; ++m) {
-                    d.m2lSplit = m == 1;
-                    ts[m] += timeWhole(10);
-                }
-            d.m2lSplit = ts[1] <= ts[0];
+                for (int sp = 0; sp < 2; ++sp)
+                    if (tm[m][sp] < tm[bm][bs]) {
+                        bm = m;
+                        bs = sp;
+                    }
+            d.nearOverlap = bm;
+            d.m2lSplit = bs == 1;
             if (nrhs == 1 && d.Vin.size() == (size_t)n) {
                 // a solve interleaves its own launches with the matvecs, so
                 // the stage order of one-right-hand-side solves is chosen by

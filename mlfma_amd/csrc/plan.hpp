@@ -90,7 +90,8 @@ struct DevicePlan {
     std::array<hipEvent_t, 9> ev{};
     hipStream_t nearStream = nullptr;  // near-field pairs beside the far-field chain
     hipEvent_t evFork = nullptr, evJoin = nullptr;
-    int nearOverlap = 2;               // 0 in order, fork 1 before S2M, 2 after S2M, 3 after M2M; autotuned
+    int nearOverlap = 2;               // 0 in order (in the M2L launch), fork 1 before S2M, 2 after S2M,
+                                       // 3 after M2M; autotuned
     hipStream_t m2lStream = nullptr;   // finest-level M2L beside the M2M chain
     hipEvent_t evM2L = nullptr, evNear = nullptr;
     bool m2lSplit = true;              // chosen by mlfma_autotune
@@ -156,8 +157,11 @@ struct DevicePlan {
     };
     std::vector<GraphEntry> graphs;
     void clearGraphs();
+    // nearX: also the near-field pairs of x (one right-hand side), in the
+    // launch of the per-target levels; returns whether they were launched
     template <typename R>
-    void launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb, int lmin, int lmax);
+    bool launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb, int lmin, int lmax,
+                   const cplx<double>* nearX = nullptr);
     void ensureBatch(int nb);
     // y (=|+=) Z^H x (the adjoint of the linear operator)
     template <typename R>
