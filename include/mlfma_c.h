@@ -286,8 +286,9 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
  * right-hand sides by timing candidates with device events: the block
  * target of the M2M (and adjoint L2L) row split, per level the sibling-group or per-target
  * M2L kernel, whether the near-field pair products run beside the far-field
- * chain (on a second stream of the plan, forked before or after S2M) or after
- * it, and whether the finest level's M2L runs beside the M2M chain (on a
+ * chain (on a second stream of the plan, forked before or after S2M or after
+ * the M2M chain) or in order (one right-hand side: in the launch of the
+ * per-target M2L levels), and whether the finest level's M2L runs beside the M2M chain (on a
  * third stream) or in line; with nrhs = 1 also whether mlfma_solve runs its
  * matvecs in order (whichever of the two makes short GMRES runs on the
  * plan's excitation faster).  Mean matvec device
