@@ -1219,13 +1219,14 @@ template void launchNearSym<float>(bool, const NearItem*, int, const int32_t*, c
                                    const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t);
 
 // The per-target M2L of one right-hand side and the near-field pairs in one
-// launch (stages in order on one stream): both read only inputs of the matvec
+// launch (stages in order on one stream; CONJ: the adjoint's conjugated
+// operators and values): both read only inputs of the matvec
 // (multipoles; x) and write disjoint outputs (locals; partial sums), and the
 // cache-bound M2L leaves HBM bandwidth that the pair stream uses.  Runs of 8
 // blocks (one per XCD) alternate between the two in proportion to their
 // counts, so both run from the first wave on; an M2L block keeps its index
 // modulo 8, hence its XCD, and the LDS is one buffer sized for the larger.
-template <typename R, typename V>
+template <typename R, typename V, bool CONJ>
 __global__ __launch_bounds__(TB) void kM2LNear(M2LLevels<R> lv, int gN, int gT, const NearItem* __restrict__ items,
                                                int nitems, const int32_t* __restrict__ colsIn,
                                                const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
@@ -1241,15 +1242,15 @@ __global__ __launch_bounds__(TB) void kM2LNear(M2LLevels<R> lv, int gN, int gT, 
         if (item >= nitems) return;
         cplx<V>* sT = reinterpret_cast<cplx<V>*>(sm);
         cplx<double>* sxr = reinterpret_cast<cplx<double>*>(sm + NT * EPL * sizeof(cplx<V>));
-        nearSymBody<false, V>(item, items, colsIn, perm, val, x, part, sT, sxr, sxr + 128);
+        nearSymBody<CONJ, V>(item, items, colsIn, perm, val, x, part, sT, sxr, sxr + 128);
         return;
     }
-    m2lwBody<R, false>(8 * (g - f0) + xx, lv, reinterpret_cast<cplx<R>(*)[64][2]>(sm));
+    m2lwBody<R, CONJ>(8 * (g - f0) + xx, lv, reinterpret_cast<cplx<R>(*)[64][2]>(sm));
 }
 
 template <typename R, typename V>
 void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
-                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st) {
+                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, bool conj) {
     int total = 0;
     for (int l = 0; l < lv.nlev; ++l) {
         lv.nch[l] = (lv.ndir[l] + 63) / 64;
@@ -1259,15 +1260,19 @@ void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int
     }
     const int gM = total / 8, gN = (nitems + 7) / 8, gT = gM + gN;
     if (gT == 0) return;
-    hipLaunchKernelGGL((kM2LNear<R, V>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gT, items, nitems, colsIn, perm, val,
-                       x, part);
+    if (conj)
+        hipLaunchKernelGGL((kM2LNear<R, V, true>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gT, items, nitems, colsIn,
+                           perm, val, x, part);
+    else
+        hipLaunchKernelGGL((kM2LNear<R, V, false>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gT, items, nitems, colsIn,
+                           perm, val, x, part);
 }
 template void launchM2LNear<double, double>(M2LLevels<double>, const NearItem*, int, const int32_t*, const int64_t*,
-                                            const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t);
+                                            const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t, bool);
 template void launchM2LNear<float, float>(M2LLevels<float>, const NearItem*, int, const int32_t*, const int64_t*,
-                                          const cplx<float>*, const cplx<double>*, cplx<double>*, hipStream_t);
+                                          const cplx<float>*, const cplx<double>*, cplx<double>*, hipStream_t, bool);
 template void launchM2LNear<float, double>(M2LLevels<float>, const NearItem*, int, const int32_t*, const int64_t*,
-                                           const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t);
+                                           const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t, bool);
 
 // FP32 copy of the near-field values (the mixed-precision inner solves)
 __global__ void kToFloat(const cplx<double>* __restrict__ in, cplx<float>* __restrict__ out, int64_t n) {

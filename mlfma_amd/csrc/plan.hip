@@ -609,8 +609,8 @@ void DevicePlan::ensureRhs(int nrhs) {
 // sources (at least three list entries per distinct source) and that fill the
 // device go through the sibling-group kernel, the others through the
 // per-target kernel.  adjoint: conjugated operators of the reversed pairs.
-// nearX (one right-hand side, forward): the near-field pairs of x go into the
-// launch of the per-target levels.
+// nearX (one right-hand side): the near-field pairs of x (conjugated values
+// for the adjoint) go into the launch of the per-target levels.
 template <typename R>
 bool DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStream_t st, int nb, int lmin, int lmax,
                            const cplx<double>* nearX) {
@@ -652,14 +652,16 @@ bool DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStr
         }
     }
     if (g.nlev) launchM2LGroup<R>(g, adjoint, st, nb);
-    if (nearX && t.nlev && nb == 1 && !adjoint && nNearItems > 0) {
+    if (nearX && t.nlev && nb == 1 && nNearItems > 0) {
         // the FP32 far field reads the FP32 copy of the near-field values
+        // (the adjoint the FP64 values)
         if constexpr (std::is_same<R, float>::value)
-            if (nearVal32) {
-                launchM2LNear<R, float>(t, nearItems, nNearItems, nearColsIn, perm, nearVal32, nearX, nearPart, st);
+            if (nearVal32 && !adjoint) {
+                launchM2LNear<R, float>(t, nearItems, nNearItems, nearColsIn, perm, nearVal32, nearX, nearPart, st,
+                                        false);
                 return true;
             }
-        launchM2LNear<R, double>(t, nearItems, nNearItems, nearColsIn, perm, nearVal, nearX, nearPart, st);
+        launchM2LNear<R, double>(t, nearItems, nNearItems, nearColsIn, perm, nearVal, nearX, nearPart, st, adjoint);
         return true;
     }
     if (t.nlev) launchM2Lx<R>(t, adjoint, st, nb);
@@ -830,9 +832,9 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
                                            ": the level's fields exceed the LDS of one block (the adjoint has no "
                                            "two-pass path; use fewer digits or smaller leaf boxes)");
     }
-    {
-        launchM2L<R>(Lc, M, true, st, 1, 2, ML);  // adjoint locals in, adjoint multipoles out
-    }
+    // adjoint locals in, adjoint multipoles out; in order, with the
+    // near-field pairs in the same launch
+    const bool nearFused = launchM2L<R>(Lc, M, true, st, 1, 2, ML, overlap ? nullptr : x);
     for (int l = 2; l < ML; ++l) {
         LevelBufs& P = levels[l];
         LevelBufs& C = levels[l + 1];
@@ -851,7 +853,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
     }
     if (overlap)
         hipCheck(hipStreamWaitEvent(st, evJoin, 0), "wait");
-    else
+    else if (!nearFused)
         launchNearSym<double>(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, st);
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st);
     const cplx<double> Ckc{Ck.re, -Ck.im};

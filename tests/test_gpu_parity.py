@@ -328,19 +328,24 @@ def test_stream_overlap_matches_one_stream():
 def test_one_stream_near_in_m2l_launch(fp32, m2l):
     """In order on one stream, the near-field pairs of one right-hand side
     share the launch of the per-target M2L levels: the bits of the default
-    run (pairs on their own stream), in FP64 and with the FP32 far field (the
-    FP32 copy of the near-field values), and with either M2L kernel."""
+    run (pairs on their own stream), forward and adjoint, in FP64 and with
+    the FP32 far field (the FP32 copy of the near-field values), and with
+    either M2L kernel."""
     import torch
     geom, kw = cases.make("rwg_adaptive")
     prec = A.MLFMA_FP32 if fp32 else A.MLFMA_FP64
-    p = Plan(geom, cases.planewave_z(), A.make_config(device=0, precision=prec, flags=m2l, **kw))
+    flags = m2l | A.MLFMA_FLAG_FRESH_LOCALS
+    p = Plan(geom, cases.planewave_z(), A.make_config(device=0, precision=prec, flags=flags, **kw))
     x = torch.from_numpy(G.complex_gaussian(geom.n, 33)).to("cuda:0")
     y0 = p.matvec(x).cpu().numpy()
-    p.set_flags(m2l | A.MLFMA_FLAG_ONE_STREAM)
+    z0 = p.matvec_adjoint(x).cpu().numpy()
+    p.set_flags(flags | A.MLFMA_FLAG_ONE_STREAM)
     for rep in range(2):
         y1 = p.matvec(x).cpu().numpy()
-        say("fp32", fp32, "m2l flag", m2l, "rep", rep, cases.relerr(y1, y0))
+        z1 = p.matvec_adjoint(x).cpu().numpy()
+        say("fp32", fp32, "m2l flag", m2l, "rep", rep, cases.relerr(y1, y0), cases.relerr(z1, z0))
         assert cases.relerr(y1, y0) == 0.0
+        assert cases.relerr(z1, z0) == 0.0
 
 
 def test_graph_replay_matches_eager():
