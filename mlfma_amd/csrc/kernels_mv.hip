@@ -1224,10 +1224,13 @@ template void launchNearSym<float>(bool, const NearItem*, int, const int32_t*, c
 // (multipoles; x) and write disjoint outputs (locals; partial sums), and the
 // cache-bound M2L leaves HBM bandwidth that the pair stream uses.  Runs of 8
 // blocks (one per XCD) alternate between the two in proportion to their
-// counts, so both run from the first wave on; an M2L block keeps its index
-// modulo 8, hence its XCD, and the LDS is one buffer sized for the larger.
+// counts, so both run from the first wave on (the pair runs spread over the
+// first gW of the gT runs: the pair items are longer than the M2L blocks, so
+// ending them earlier shortens the launch's tail); an M2L block keeps its
+// index modulo 8, hence its XCD, and the LDS is one buffer sized for the
+// larger.
 template <typename R, typename V, bool CONJ>
-__global__ __launch_bounds__(TB) void kM2LNear(M2LLevels<R> lv, int gN, int gT, const NearItem* __restrict__ items,
+__global__ __launch_bounds__(TB) void kM2LNear(M2LLevels<R> lv, int gN, int gW, const NearItem* __restrict__ items,
                                                int nitems, const int32_t* __restrict__ colsIn,
                                                const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
                                                const cplx<double>* __restrict__ x, cplx<double>* __restrict__ part) {
@@ -1236,7 +1239,8 @@ __global__ __launch_bounds__(TB) void kM2LNear(M2LLevels<R> lv, int gN, int gT, 
     constexpr int SZM = 3 * 64 * 2 * (int)sizeof(cplx<R>);
     __shared__ __align__(16) unsigned char sm[SZN > SZM ? SZN : SZM];
     const int g = blockIdx.x >> 3, xx = blockIdx.x & 7;
-    const int f0 = (int)((int64_t)g * gN / gT), f1 = (int)((int64_t)(g + 1) * gN / gT);
+    const int f0 = (int)min((int64_t)gN, (int64_t)g * gN / gW);
+    const int f1 = (int)min((int64_t)gN, (int64_t)(g + 1) * gN / gW);
     if (f1 > f0) {
         const int item = 8 * f0 + xx;
         if (item >= nitems) return;
@@ -1250,7 +1254,8 @@ __global__ __launch_bounds__(TB) void kM2LNear(M2LLevels<R> lv, int gN, int gT, 
 
 template <typename R, typename V>
 void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
-                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, bool conj) {
+                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, bool conj,
+                   int windowPct) {
     int total = 0;
     for (int l = 0; l < lv.nlev; ++l) {
         lv.nch[l] = (lv.ndir[l] + 63) / 64;
@@ -1260,19 +1265,20 @@ void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int
     }
     const int gM = total / 8, gN = (nitems + 7) / 8, gT = gM + gN;
     if (gT == 0) return;
+    const int gW = std::max(gN, (int)((int64_t)gT * std::min(100, std::max(1, windowPct)) / 100));
     if (conj)
-        hipLaunchKernelGGL((kM2LNear<R, V, true>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gT, items, nitems, colsIn,
+        hipLaunchKernelGGL((kM2LNear<R, V, true>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gW, items, nitems, colsIn,
                            perm, val, x, part);
     else
-        hipLaunchKernelGGL((kM2LNear<R, V, false>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gT, items, nitems, colsIn,
+        hipLaunchKernelGGL((kM2LNear<R, V, false>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gW, items, nitems, colsIn,
                            perm, val, x, part);
 }
 template void launchM2LNear<double, double>(M2LLevels<double>, const NearItem*, int, const int32_t*, const int64_t*,
-                                            const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t, bool);
+                                            const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t, bool, int);
 template void launchM2LNear<float, float>(M2LLevels<float>, const NearItem*, int, const int32_t*, const int64_t*,
-                                          const cplx<float>*, const cplx<double>*, cplx<double>*, hipStream_t, bool);
+                                          const cplx<float>*, const cplx<double>*, cplx<double>*, hipStream_t, bool, int);
 template void launchM2LNear<float, double>(M2LLevels<float>, const NearItem*, int, const int32_t*, const int64_t*,
-                                           const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t, bool);
+                                           const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t, bool, int);
 
 // FP32 copy of the near-field values (the mixed-precision inner solves)
 __global__ void kToFloat(const cplx<double>* __restrict__ in, cplx<float>* __restrict__ out, int64_t n) {

@@ -76,10 +76,12 @@ void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_
                    int64_t xs = 0, int64_t ps = 0);
 // the per-target M2L of one right-hand side and the near-field pairs in one
 // launch; conj: conjugated operators (lv holds the reversed pairs' offsets)
-// and conjugated values (the adjoint)
+// and conjugated values (the adjoint); windowPct: the pair blocks spread over
+// that percentage of the launch's blocks, from the first
 template <typename R, typename V>
 void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
-                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, bool conj);
+                   const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, bool conj,
+                   int windowPct);
 // out = (float) in, n complex values
 void launchToFloat(const cplx<double>* in, cplx<float>* out, int64_t n, hipStream_t st);
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,

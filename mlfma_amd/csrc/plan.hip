@@ -658,10 +658,11 @@ bool DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStr
         if constexpr (std::is_same<R, float>::value)
             if (nearVal32 && !adjoint) {
                 launchM2LNear<R, float>(t, nearItems, nNearItems, nearColsIn, perm, nearVal32, nearX, nearPart, st,
-                                        false);
+                                        false, nearWindow);
                 return true;
             }
-        launchM2LNear<R, double>(t, nearItems, nNearItems, nearColsIn, perm, nearVal, nearX, nearPart, st, adjoint);
+        launchM2LNear<R, double>(t, nearItems, nNearItems, nearColsIn, perm, nearVal, nearX, nearPart, st, adjoint,
+                                 nearWindow);
         return true;
     }
     if (t.nlev) launchM2Lx<R>(t, adjoint, st, nb);
@@ -1304,6 +1305,20 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
                     }
             d.nearOverlap = bm;
             d.m2lSplit = bs == 1;
+            if (bm == 0) {
+                // in order: how early the pair blocks of the shared M2L launch end
+                const int wins[4] = {50, 65, 80, 100};
+                double tw[4] = {0, 0, 0, 0};
+                for (int rep = 0; rep < 2; ++rep)
+                    for (int w = 0; w < 4; ++w) {
+                        d.nearWindow = wins[w];
+                        tw[w] += timeWhole(10);
+                    }
+                int bw = 3;
+                for (int w = 0; w < 4; ++w)
+                    if (tw[w] < tw[bw]) bw = w;
+                d.nearWindow = wins[bw];
+            }
             if (nrhs == 1 && d.Vin.size() == (size_t)n) {
                 // a solve interleaves its own launches with the matvecs, so
                 // the stage order of one-right-hand-side solves is chosen by

@@ -95,6 +95,7 @@ struct DevicePlan {
     hipStream_t m2lStream = nullptr;   // finest-level M2L beside the M2M chain
     hipEvent_t evM2L = nullptr, evNear = nullptr;
     bool m2lSplit = true;              // chosen by mlfma_autotune
+    int nearWindow = 100;              // % of the shared M2L/near launch the pair blocks spread over; autotuned
     bool timesValid = false;
     uint32_t flags = 0;  // MLFMA_FLAG_* in effect for matvecs (settable after creation)
     std::vector<void*> allocs;
