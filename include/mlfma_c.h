@@ -288,7 +288,7 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
  * M2L kernel, whether the near-field pair products run beside the far-field
  * chain (on a second stream of the plan, forked before or after S2M or after
  * the M2M chain) or in order (one right-hand side: in the launch of the
- * per-target M2L levels), and whether the finest level's M2L runs beside the M2M chain (on a
+ * per-target M2L levels, over an early share of its blocks), and whether the finest level's M2L runs beside the M2M chain (on a
  * third stream) or in line; with nrhs = 1 also whether mlfma_solve runs its
  * matvecs in order (whichever of the two makes short GMRES runs on the
  * plan's excitation faster).  Mean matvec device
