@@ -2,6 +2,8 @@
 // near field, L2T) for gfx950.  Vectors x and y are in input order; the tree
 // permutation is applied inside the kernels that touch them (S2M reads, near
 // field and L2T writes), so a matvec needs no separate gather/scatter pass.
+#include <algorithm>
+
 #include "cplx_ops.hpp"
 #include "kernels_mv.hpp"
 #include "host.hpp"  // Error
@@ -1265,7 +1267,8 @@ void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int
     }
     const int gM = total / 8, gN = (nitems + 7) / 8, gT = gM + gN;
     if (gT == 0) return;
-    const int gW = std::max(gN, (int)((int64_t)gT * std::min(100, std::max(1, windowPct)) / 100));
+    // gW >= max(gN, 1): at most one pair run per run, and no division by zero
+    const int gW = std::max({gN, 1, (int)((int64_t)gT * std::min(100, std::max(1, windowPct)) / 100)});
     if (conj)
         hipLaunchKernelGGL((kM2LNear<R, V, true>), dim3(8 * gT), dim3(TB), 0, st, lv, gN, gW, items, nitems, colsIn,
                            perm, val, x, part);
