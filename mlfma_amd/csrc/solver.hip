@@ -698,6 +698,11 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         // columns the synchronous algorithm would.
         hipLaunchKernelGGL(kInitG, dim3((m + 1 + KB - 1) / KB), dim3(KB), 0, K.st, G.g, beta, m + 1);
         int jq = 0, jc = 0, ncolUsed = -1;
+        // each event record costs the stream a few microseconds, so one event
+        // covers two steps (and the last step queued before the host waits);
+        // cover[s % kWindow]: the event slot that covers step s
+        std::array<int, kWindow> cover{};
+        int uncovered = 0;
         while (true) {
             while (ncolUsed < 0 && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
                 d.applySolveOp(Q + (size_t)jq * n, w, 0, K.st, fp32, false);
@@ -718,11 +723,16 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
                 }
                 hipLaunchKernelGGL(kScaleDev, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<double>*)w,
                                    (const double*)(G.stat + 3 * jq + 1), Q + (size_t)(jq + 1) * n, n);
-                hipCheck(hipEventRecord(G.ev[jq % kWindow], K.st), "event");
+                const bool more = jq + 1 < m && total + (jq + 1 - jc) < maxit && jq + 1 - jc < kWindow;
+                if ((jq & 1) || !more) {
+                    hipCheck(hipEventRecord(G.ev[jq % kWindow], K.st), "event");
+                    for (int q = uncovered; q <= jq; ++q) cover[q % kWindow] = jq % kWindow;
+                    uncovered = jq + 1;
+                }
                 ++jq;
             }
             if (jc == jq) break;
-            hipCheck(hipEventSynchronize(G.ev[jc % kWindow]), "gmres step");
+            hipCheck(hipEventSynchronize(G.ev[cover[jc % kWindow]]), "gmres step");
             const double est = G.statHost[3 * jc], hn = G.statHost[3 * jc + 1], rho = G.statHost[3 * jc + 2];
             if (rho == 0) {
                 out.status = MLFMA_ERR_BREAKDOWN;
