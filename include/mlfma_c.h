@@ -207,6 +207,10 @@ int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t pt
 #define MLFMA_SOLVE_ADJOINT 0x100  /* or-ed into method: solve Z^H x = b instead
                                       (the adjoint solve of a gradient; no
                                       reference counterpart)                       */
+#define MLFMA_SOLVE_DEVICE 0x200   /* or-ed into method (mlfma_solve): b and x are
+                                      device pointers on the plan's device; b
+                                      must be complete when the call is made
+                                      (the solve runs on the plan's stream)        */
 
 typedef struct mlfma_solve_opts {
     int32_t method;       /* MLFMA_GMRES_*                                          */
@@ -226,7 +230,8 @@ typedef struct mlfma_solve_report {
 
 /* Solve Z x = b — the drop-in for Solver::solve (src/solver.cpp:116-146)
  * followed by the currents of Solver::getCurrents.  b and x are host
- * buffers of n complex values; b may be NULL when opts->use_voltage = 1.
+ * buffers of n complex values (device buffers with MLFMA_SOLVE_DEVICE in
+ * opts->method); b may be NULL when opts->use_voltage = 1.
  * resid_hist (optional, host, maxit doubles) receives the per-iteration
  * residual estimate. */
 int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_opts* opts,
@@ -245,14 +250,21 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
 
 /* E(rhat_d) = C k (I - rhat rhat^T) sum_n I_n f_n(k rhat_d) for ndir unit
  * directions (generalises Node::printFarSols, src/fmm/node.cpp:147-178, to any
- * direction list).  I: n complex (host); dirs: ndir x 3 (host);
- * E: ndir x 3 complex (host). */
-int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E);
+ * direction list).  I: n complex (input order); dirs: ndir x 3;
+ * E: ndir x 3 complex.  ptr_mode: all three on the host or all three on the
+ * plan's device; stream as in mlfma_matvec (device pointers with
+ * MLFMA_FLAG_ASYNC: returns once queued).  The source sums are split into
+ * ranges whose partial sums are added in a fixed order (results do not
+ * depend on scheduling). */
+int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E,
+                   int32_t ptr_mode, void* stream);
 
-/* The adjoint of mlfma_farfield: I = F^H E for ndir x 3 complex fields E
- * (host) at the unit directions dirs (host); I: n complex (host, input
- * order).  No reference counterpart: the gradient of a far-field objective. */
-int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs, int64_t ndir, double* I);
+/* The adjoint of mlfma_farfield: I = F^H E for ndir x 3 complex fields E at
+ * the unit directions dirs; I: n complex (input order); ptr_mode and stream
+ * as in mlfma_farfield.  No reference counterpart: the gradient of a
+ * far-field objective. */
+int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs, int64_t ndir, double* I,
+                           int32_t ptr_mode, void* stream);
 
 /* Position sensitivity of the operator for dipole sources:
  * grad[i][a] = -2 Re(lambda^H (dZ/dr_i^a) I) for complex n-vectors I and
@@ -295,6 +307,23 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
  * time before and after (ms, may be NULL).  Results do not depend on the
  * choice beyond round-off.  No reference counterpart. */
 int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms_after);
+
+/* Launch parameters that mlfma_autotune chooses, set by hand (no reference
+ * counterpart).  Results do not depend on them beyond round-off.
+ *   MLFMA_PARAM_NEAR_WINDOW    1..100: % of the shared in-order M2L /
+ *                              near-field launch over which the near-field
+ *                              pair blocks are spread (one right-hand side)
+ *   MLFMA_PARAM_NEAR_OVERLAP   0 in order (in the M2L launch), 1 beside the
+ *                              chain forked before S2M, 2 after S2M, 3 after M2M
+ *   MLFMA_PARAM_M2L_SPLIT      1: the finest level's M2L on its own stream
+ *   MLFMA_PARAM_SOLVE_IN_ORDER 1: one-right-hand-side solves run every stage
+ *                              in order on one stream */
+#define MLFMA_PARAM_NEAR_WINDOW 1
+#define MLFMA_PARAM_NEAR_OVERLAP 2
+#define MLFMA_PARAM_M2L_SPLIT 3
+#define MLFMA_PARAM_SOLVE_IN_ORDER 4
+int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value);
+int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value);
 
 /* Replace the plan's MLFMA_FLAG_* set for later matvecs and solves (the flags
  * act at apply time; the plan itself does not change).  No reference

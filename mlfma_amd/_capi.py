@@ -31,6 +31,11 @@ MLFMA_GMRES_COMPAT = 0
 MLFMA_GMRES_RESTART = 1
 MLFMA_FGMRES_IR = 2
 MLFMA_SOLVE_ADJOINT = 0x100
+MLFMA_SOLVE_DEVICE = 0x200
+MLFMA_PARAM_NEAR_WINDOW = 1
+MLFMA_PARAM_NEAR_OVERLAP = 2
+MLFMA_PARAM_M2L_SPLIT = 3
+MLFMA_PARAM_SOLVE_IN_ORDER = 4
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_OOM", -3: "ERR_HIP", -4: "ERR_BREAKDOWN",
           -5: "ERR_NONFINITE", -6: "ERR_STATE", -7: "ERR_GEOMETRY", -8: "ERR_NOT_CONVERGED"}
@@ -96,11 +101,15 @@ SIGNATURES = {
     "mlfma_solve": (C.c_int, [C.c_void_p, dptr, dptr, C.POINTER(SolveOpts), C.POINTER(SolveReport),
                               dptr]),
     "mlfma_solve_batch": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int32, C.c_void_p, C.c_void_p, dptr]),
-    "mlfma_farfield": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
-    "mlfma_farfield_adjoint": (C.c_int, [C.c_void_p, dptr, dptr, C.c_int64, dptr]),
+    "mlfma_farfield": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
+                                 C.c_void_p]),
+    "mlfma_farfield_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
+                                         C.c_void_p]),
     "mlfma_position_gradient": (C.c_int, [C.c_void_p, dptr, dptr, dptr]),
     "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),
     "mlfma_plan_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mlfma_plan_set_param": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "mlfma_plan_get_param": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_autotune": (C.c_int, [C.c_void_p, C.c_int32, dptr, dptr]),
     "mlfma_stage_times_detail": (C.c_int, [C.c_void_p, dptr, C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_level_boxes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), iptr, iptr, iptr,

@@ -51,30 +51,52 @@ class _Matvec(torch.autograd.Function):
         return torch.from_numpy(p.matvec_adjoint(_np(g))), None
 
 
+def _require_converged(rep, what):
+    """A GMRES solve that stopped above its tolerance returns an iterate, not
+    the solution: the values and gradients built on it would be wrong, so
+    the wrappers raise instead (MLFMA_ERR_NOT_CONVERGED in the report)."""
+    if rep["status"] != "OK":
+        raise RuntimeError(f"{what} GMRES solve did not converge: status {rep['status']}, "
+                           f"{rep['iterations']} iterations, true residual {rep['resid_true']:.3e}")
+
+
+def _solve_any(plan, b, adjoint, opts):
+    """GMRES on a CUDA tensor (kept on the device) or a host tensor."""
+    if b.is_cuda:
+        return plan.solve(b.to(torch.complex128).contiguous(), adjoint=adjoint, **opts)
+    x, rep = plan.solve(_np(b), adjoint=adjoint, **opts)
+    return torch.from_numpy(x), rep
+
+
 class _Solve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, b, plan, opts):
-        ctx.plan, ctx.opts, ctx.device = plan, opts, b.device
-        x, rep = plan.solve(_np(b), **opts)
+        ctx.plan, ctx.opts = plan, opts
+        x, rep = _solve_any(plan, b, False, opts)
+        _require_converged(rep, "forward")
         ctx.report = rep
-        return torch.from_numpy(x).to(b.device)
+        return x
 
     @staticmethod
     def backward(ctx, g):
-        gb, _ = ctx.plan.solve(_np(g), adjoint=True, **ctx.opts)
-        return torch.from_numpy(gb).to(ctx.device), None, None
+        gb, rep = _solve_any(ctx.plan, g, True, ctx.opts)
+        _require_converged(rep, "adjoint")
+        return gb, None, None
 
 
 class _Farfield(torch.autograd.Function):
     @staticmethod
     def forward(ctx, I, plan, dirs):
-        ctx.plan, ctx.dirs, ctx.device = plan, dirs, I.device
-        return torch.from_numpy(plan.farfield(_np(I), dirs)).to(I.device)
+        ctx.plan, ctx.dirs = plan, dirs
+        if I.is_cuda:
+            return plan.farfield(I.to(torch.complex128).contiguous(), dirs)
+        return torch.from_numpy(plan.farfield(_np(I), dirs))
 
     @staticmethod
     def backward(ctx, g):
-        gI = ctx.plan.farfield_adjoint(_np(g), ctx.dirs)
-        return torch.from_numpy(gI).to(ctx.device), None, None
+        if g.is_cuda:
+            return ctx.plan.farfield_adjoint(g.to(torch.complex128).contiguous(), ctx.dirs), None, None
+        return torch.from_numpy(ctx.plan.farfield_adjoint(_np(g), ctx.dirs)), None, None
 
 
 def matvec(plan: Plan, x: torch.Tensor) -> torch.Tensor:
@@ -129,11 +151,9 @@ def objective_gradients(plan: Plan, dirs, b=None, restart: int = 50, maxit: int 
         b = -plan.voltage()
         wave = plan.pw.wavenum * np.asarray(plan.pw.khat[:3], dtype=np.float64)
     I, rep = plan.solve(b, **opts)
-    if rep["status"] != "OK":
-        raise RuntimeError(f"forward solve did not converge: {rep}")
+    _require_converged(rep, "forward")
     E = plan.farfield(I, dirs)
     lam, rep = plan.solve(plan.farfield_adjoint(E, dirs), adjoint=True, **opts)
-    if rep["status"] != "OK":
-        raise RuntimeError(f"adjoint solve did not converge: {rep}")
+    _require_converged(rep, "adjoint")
     J = float(np.sum(np.abs(E) ** 2))
     return J, lam, position_gradient(plan, I, lam, E, dirs, b, wave)

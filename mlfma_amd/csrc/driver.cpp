@@ -6,6 +6,8 @@
 //
 // usage: mlfma [--root DIR] [--device D] [--method compat|gmres|ir] [--restart M]
 //              [--maxit K] [--tol T] [--max-level L] [--dipoles FILE] [--fp32]
+//        mlfma --root DIR --print-config   (print the parsed config.txt and exit;
+//                                           no device is touched)
 #include <cctype>
 #include <chrono>
 #include <cmath>
@@ -25,22 +27,60 @@ namespace fs = std::filesystem;
 
 namespace {
 
-// config.txt: each line is read from its first digit on (src/config.h:16-48)
-std::vector<std::string> readConfigLines(const fs::path& p) {
+// config.txt (src/config.h:16-58): one field per line, in the order of the
+// Config constructor.  A line is read from its first digit on; a typed value
+// is extracted, the stream skips to the next digit and extracts again, so the
+// last number on the line wins (a sign is skipped like any other non-digit;
+// an int field stops at a decimal point and takes the digits after it as the
+// next number; a line without a number gives 0).  The extraction is the
+// standard stream's operator>> for the field's type, as in the reference.
+struct RefConfigFields {
+    int mode = 0, pdist = 0, qdist = 0, quadPrec = 0, digits = 0, interpOrder = 0;
+    double overInterp = 0, rootLeng = 0;
+    int maxNodeSrcs = 0;
+    bool evalDirect = false;
+    int nsrcs = 0;
+};
+
+void skipToDigit(std::istringstream& iss) {
+    char ch;
+    while (iss.get(ch))
+        if (std::isdigit(static_cast<unsigned char>(ch))) {
+            iss.unget();
+            break;
+        }
+}
+
+template <typename T>
+void readField(std::ifstream& f, T& val, int& lines) {
+    std::string line;
+    if (!std::getline(f, line)) return;
+    ++lines;
+    std::istringstream iss(line);
+    skipToDigit(iss);
+    T v{};  // a line without a number gives 0 (the reference leaves the field unset)
+    while (iss >> v) skipToDigit(iss);
+    val = v;
+}
+
+RefConfigFields readConfig(const fs::path& p) {
     std::ifstream f(p);
     if (!f) throw std::runtime_error("Unable to find file " + p.string());
-    std::vector<std::string> out;
-    std::string line;
-    while (std::getline(f, line)) {
-        size_t i = 0;
-        while (i < line.size() && !std::isdigit((unsigned char)line[i])) ++i;
-        std::istringstream iss(line.substr(i));
-        std::string tok;
-        iss >> tok;
-        out.push_back(tok.empty() ? "0" : tok);
-    }
-    if (out.size() < 11) throw std::runtime_error("config.txt needs 11 lines");
-    return out;
+    RefConfigFields c;
+    int lines = 0;
+    readField(f, c.mode, lines);
+    readField(f, c.pdist, lines);
+    readField(f, c.qdist, lines);
+    readField(f, c.quadPrec, lines);
+    readField(f, c.digits, lines);
+    readField(f, c.interpOrder, lines);
+    readField(f, c.overInterp, lines);
+    readField(f, c.rootLeng, lines);
+    readField(f, c.maxNodeSrcs, lines);
+    readField(f, c.evalDirect, lines);
+    readField(f, c.nsrcs, lines);
+    if (lines < 11) throw std::runtime_error("config.txt needs 11 lines");
+    return c;
 }
 
 std::vector<double> readRows(const fs::path& p, int cols) {
@@ -77,7 +117,7 @@ int main(int argc, char** argv) {
     int device = 0, restart = 50, maxit = 500, maxLevel = -1;
     double tol = 1.0e-6;
     std::string method = "compat", dipoles;
-    bool fp32 = false;
+    bool fp32 = false, printConfig = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -93,24 +133,31 @@ int main(int argc, char** argv) {
         else if (a == "--max-level") maxLevel = std::stoi(next());
         else if (a == "--dipoles") dipoles = next();
         else if (a == "--fp32") fp32 = true;
+        else if (a == "--print-config") printConfig = true;
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
         }
     }
     try {
+        const RefConfigFields rc = readConfig(root / "config" / "config.txt");
+        if (printConfig) {
+            std::printf("mode %d\npdist %d\nqdist %d\nquadPrec %d\ndigits %d\ninterpOrder %d\n"
+                        "overInterp %.17g\nrootLeng %.17g\nmaxNodeSrcs %d\nevalDirect %d\nnsrcs %d\n",
+                        rc.mode, rc.pdist, rc.qdist, rc.quadPrec, rc.digits, rc.interpOrder, rc.overInterp,
+                        rc.rootLeng, rc.maxNodeSrcs, rc.evalDirect ? 1 : 0, rc.nsrcs);
+            return 0;
+        }
         std::cout << " Importing sources...\n";
-        const auto cl = readConfigLines(root / "config" / "config.txt");
-        // order: mode pdist qdist quadPrec digits interpOrder overInterp rootLeng maxNodeSrcs evalDirect nsrcs
         mlfma_config cfg{};
-        cfg.quad_prec = std::stoi(cl[3]);
-        cfg.digits = std::stoi(cl[4]);
-        cfg.interp_order = std::stoi(cl[5]);
-        cfg.over_interp = std::stod(cl[6]);
-        cfg.root_leng = std::stod(cl[7]);
-        cfg.max_node_srcs = std::stoi(cl[8]);
-        const bool evalDirect = std::stoi(cl[9]) != 0;
-        const int nsrcs = std::stoi(cl[10]);
+        cfg.quad_prec = rc.quadPrec;
+        cfg.digits = rc.digits;
+        cfg.interp_order = rc.interpOrder;
+        cfg.over_interp = rc.overInterp;
+        cfg.root_leng = rc.rootLeng;
+        cfg.max_node_srcs = rc.maxNodeSrcs;
+        const bool evalDirect = rc.evalDirect;
+        const int nsrcs = rc.nsrcs;
         cfg.max_level = maxLevel;
         cfg.device = device;
         cfg.precision = fp32 ? MLFMA_FP32 : MLFMA_FP64;

@@ -2,6 +2,8 @@
 // setup generators.  Wave64 throughout; every kernel is a streaming pass whose
 // roofline is HBM (or L2 for the M2L source reads), see the launchers for the
 // per-item work each block does.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "cplx_ops.hpp"
 
@@ -502,14 +504,22 @@ void launchVoltage(const SrcDev& s, int64_t n, const double* wv, const double* p
 }
 
 // E(rhat) = C k (I - rr) sum_n I_n f_n(k rhat) with f_n = conj(F_n) (RWG) or
-// e^{-ik rhat.r} p (dipole) (src/fmm/node.cpp:147-178); one block per direction.
-__global__ void kFarfield(SrcDev s, int64_t n, const cplx<double>* __restrict__ I, const double* __restrict__ dirs,
-                          double k, cplx<double> Ck, cplx<double>* __restrict__ E) {
-    const int d = blockIdx.x;
+// e^{-ik rhat.r} p (dipole) (src/fmm/node.cpp:147-178).  The source sum of
+// each direction is split into nch ranges of the tree order (grid (ndir,
+// nch)); block (d, c) writes its three partial sums to part[(d nch + c) 3 + i]
+// and kFarfieldReduce adds them in range order, so the result does not depend
+// on scheduling.  I is read in input order through perm.
+__global__ void kFarfieldPart(SrcDev s, int64_t n, int64_t chunk, const cplx<double>* __restrict__ I,
+                              const int64_t* __restrict__ perm, const double* __restrict__ dirs, double k,
+                              cplx<double>* __restrict__ part) {
+    const int64_t d = blockIdx.x;
+    const int c = blockIdx.y, nch = gridDim.y;
     const double rh[3] = {dirs[3 * d], dirs[3 * d + 1], dirs[3 * d + 2]};
     const double kv[3] = {k * rh[0], k * rh[1], k * rh[2]};
+    const int64_t t0 = (int64_t)c * chunk;
+    const int64_t t1 = t0 + chunk < n ? t0 + chunk : n;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int64_t t = threadIdx.x; t < n; t += BS) {
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += BS) {
         cplx<double> f[3];
         if (s.kind == 0) {
             const DipoleDev& dp = s.dip[t];
@@ -519,8 +529,9 @@ __global__ void kFarfield(SrcDev s, int64_t n, const cplx<double>* __restrict__ 
             rwgPlaneWave(s.rwg[t], s.tri, kv, f);
             for (int i = 0; i < 3; ++i) f[i] = mk(f[i].re, -f[i].im);
         }
+        const cplx<double> It = I[perm[t]];
         for (int i = 0; i < 3; ++i) {
-            const cplx<double> v = I[t] * f[i];
+            const cplx<double> v = It * f[i];
             acc[2 * i] += v.re;
             acc[2 * i + 1] += v.im;
         }
@@ -532,28 +543,53 @@ __global__ void kFarfield(SrcDev s, int64_t n, const cplx<double>* __restrict__ 
         if (lane == 0) red[w][j] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        cplx<double> a[3];
-        for (int i = 0; i < 3; ++i) {
-            double re = 0, im = 0;
-            for (int q = 0; q < BS / 64; ++q) {
-                re += red[q][2 * i];
-                im += red[q][2 * i + 1];
-            }
-            a[i] = mk(re, im);
+    if (threadIdx.x < 3) {
+        const int i = threadIdx.x;
+        double re = 0, im = 0;
+        for (int q = 0; q < BS / 64; ++q) {
+            re += red[q][2 * i];
+            im += red[q][2 * i + 1];
         }
-        for (int i = 0; i < 3; ++i) {
-            cplx<double> v = mk(0.0, 0.0);
-            for (int j = 0; j < 3; ++j) v = v + ((i == j ? 1.0 : 0.0) - rh[i] * rh[j]) * a[j];
-            E[3 * d + i] = Ck * v;
-        }
+        part[(d * nch + c) * 3 + i] = mk(re, im);
     }
 }
 
-void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const double* dirs, int ndir, double k,
-                    cplx<double> Ck, cplx<double>* E, hipStream_t st) {
+__global__ void kFarfieldReduce(int64_t ndir, int nch, const cplx<double>* __restrict__ part,
+                                const double* __restrict__ dirs, cplx<double> Ck, cplx<double>* __restrict__ E) {
+    const int64_t d = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (d >= ndir) return;
+    cplx<double> a[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+    for (int c = 0; c < nch; ++c)
+        for (int i = 0; i < 3; ++i) a[i] = a[i] + part[(d * nch + c) * 3 + i];
+    const double rh[3] = {dirs[3 * d], dirs[3 * d + 1], dirs[3 * d + 2]};
+    for (int i = 0; i < 3; ++i) {
+        cplx<double> v = mk(0.0, 0.0);
+        for (int j = 0; j < 3; ++j) v = v + ((i == j ? 1.0 : 0.0) - rh[i] * rh[j]) * a[j];
+        E[3 * d + i] = Ck * v;
+    }
+}
+
+// Source ranges per direction: enough blocks to fill the device (about 4096
+// in all), at least 1024 sources per range, at most kFarPartCap partial
+// triples in all (the size of the partial-sum buffer).
+int farfieldChunks(int64_t n, int64_t ndir) {
+    if (n <= 0 || ndir <= 0) return 1;
+    int64_t nch = (4096 + ndir - 1) / ndir;
+    nch = std::min<int64_t>(nch, (n + 1023) / 1024);
+    nch = std::min<int64_t>(nch, kFarPartCap / ndir);
+    nch = std::max<int64_t>(nch, 1);
+    const int64_t chunk = (n + nch - 1) / nch;
+    return (int)((n + chunk - 1) / chunk);
+}
+
+void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const int64_t* perm, const double* dirs,
+                    int64_t ndir, double k, cplx<double> Ck, cplx<double>* part, cplx<double>* E, hipStream_t st) {
     if (ndir <= 0) return;
-    hipLaunchKernelGGL(kFarfield, dim3(ndir), dim3(BS), 0, st, s, n, I, dirs, k, Ck, E);
+    const int nch = farfieldChunks(n, ndir);
+    const int64_t chunk = n > 0 ? (n + nch - 1) / nch : 1;
+    hipLaunchKernelGGL(kFarfieldPart, dim3((unsigned)ndir, nch), dim3(BS), 0, st, s, n, chunk, I, perm, dirs, k, part);
+    hipLaunchKernelGGL(kFarfieldReduce, dim3((unsigned)((ndir + BS - 1) / BS)), dim3(BS), 0, st, ndir, nch,
+                       (const cplx<double>*)part, dirs, Ck, E);
 }
 
 // Adjoint of kFarfield: I_n = conj(C k) sum_d f_n(k rhat_d)^H (I - rr) E_d,

@@ -88,8 +88,14 @@ void launchTransl(int ndir, int nps, int p, const double* psi, const int32_t* ne
                   const cplx<double>* alpha, const double* dirWeight, cplx<double>* T, hipStream_t st);
 void launchVoltage(const SrcDev& s, int64_t n, const double* wavevec, const double* pol, double amp,
                    cplx<double>* V, hipStream_t st);
-void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const double* dirs, int ndir, double k,
-                    cplx<double> Ck, cplx<double>* E, hipStream_t st);
+// far field at ndir directions of I (input order): the source sum in
+// farfieldChunks(n, ndir) ranges per direction, whose partial sums (3 complex
+// each) go to part (ndir x farfieldChunks(n, ndir) x 3 complex), then one
+// fixed-order reduction per direction
+constexpr int64_t kFarPartCap = 65536;  // partial triples the far-field chunking uses at most
+int farfieldChunks(int64_t n, int64_t ndir);
+void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const int64_t* perm, const double* dirs,
+                    int64_t ndir, double k, cplx<double> Ck, cplx<double>* part, cplx<double>* E, hipStream_t st);
 void launchFarfieldAdj(const SrcDev& s, int64_t n, const cplx<double>* E, const double* dirs, int ndir, double k,
                        cplx<double> Ckc, const int64_t* perm, cplx<double>* I, hipStream_t st);
 // dipole position gradient, near part: grad (n x 3, input order) += -2 Re(lambda^H dZ_near I)

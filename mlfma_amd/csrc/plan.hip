@@ -1305,8 +1305,12 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
                     }
             d.nearOverlap = bm;
             d.m2lSplit = bs == 1;
-            if (bm == 0) {
-                // in order: how early the pair blocks of the shared M2L launch end
+            // how early the pair blocks of the shared M2L / near-field launch
+            // end; that launch runs only for one right-hand side with the
+            // stages in order (the in-order matvec mode, or solves run in order)
+            auto tuneWindow = [&]() {
+                const uint32_t f = d.flags;
+                d.flags |= MLFMA_FLAG_ONE_STREAM;
                 const int wins[4] = {50, 65, 80, 100};
                 double tw[4] = {0, 0, 0, 0};
                 for (int rep = 0; rep < 2; ++rep)
@@ -1314,10 +1318,16 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
                         d.nearWindow = wins[w];
                         tw[w] += timeWhole(10);
                     }
+                d.flags = f;
                 int bw = 3;
                 for (int w = 0; w < 4; ++w)
                     if (tw[w] < tw[bw]) bw = w;
                 d.nearWindow = wins[bw];
+            };
+            bool windowTuned = false;
+            if (nrhs == 1 && bm == 0) {
+                tuneWindow();
+                windowTuned = true;
             }
             if (nrhs == 1 && d.Vin.size() == (size_t)n) {
                 // a solve interleaves its own launches with the matvecs, so
@@ -1342,6 +1352,7 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
                 d.solveInOrder = false;
                 d.flags = f;
                 d.solveSerial = tSerial < tOverlap;
+                if (d.solveSerial && !windowTuned) tuneWindow();
             }
         }
         const double after = timeWhole(10);
@@ -1367,6 +1378,50 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
         float t = 0;
         hipCheck(hipEventElapsedTime(&t, d.ev[q], d.ev[q + 1]), "hipEventElapsedTime");
         ms[q] = t;
+    }
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    needDevice(d);
+    switch (param) {
+        case MLFMA_PARAM_NEAR_WINDOW:
+            if (value < 1 || value > 100) throw Error(MLFMA_ERR_ARG, "near window must be 1..100 (%)");
+            d.nearWindow = value;
+            break;
+        case MLFMA_PARAM_NEAR_OVERLAP:
+            if (value < 0 || value > 3) throw Error(MLFMA_ERR_ARG, "near overlap mode must be 0..3");
+            d.nearOverlap = value;
+            break;
+        case MLFMA_PARAM_M2L_SPLIT:
+            if (value != 0 && value != 1) throw Error(MLFMA_ERR_ARG, "M2L split must be 0 or 1");
+            d.m2lSplit = value == 1;
+            break;
+        case MLFMA_PARAM_SOLVE_IN_ORDER:
+            if (value != 0 && value != 1) throw Error(MLFMA_ERR_ARG, "solve order must be 0 or 1");
+            d.solveSerial = value == 1;
+            break;
+        default:
+            throw Error(MLFMA_ERR_ARG, "unknown parameter");
+    }
+    d.clearGraphs();
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    if (!value) throw Error(MLFMA_ERR_ARG, "null argument");
+    switch (param) {
+        case MLFMA_PARAM_NEAR_WINDOW: *value = d.nearWindow; break;
+        case MLFMA_PARAM_NEAR_OVERLAP: *value = d.nearOverlap; break;
+        case MLFMA_PARAM_M2L_SPLIT: *value = d.m2lSplit ? 1 : 0; break;
+        case MLFMA_PARAM_SOLVE_IN_ORDER: *value = d.solveSerial ? 1 : 0; break;
+        default: throw Error(MLFMA_ERR_ARG, "unknown parameter");
     }
     return MLFMA_OK;
     API_END

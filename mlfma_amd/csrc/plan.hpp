@@ -128,6 +128,8 @@ struct DevicePlan {
     std::vector<std::complex<double>> Vin;  // excitation, input order
     cplx<double>*xt = nullptr, *yt = nullptr, *xs = nullptr, *ys = nullptr;
     int rhsCap = 0;
+    cplx<double>* farPart = nullptr;  // far-field partial sums (grown on demand)
+    size_t farPartLen = 0;
 
     ~DevicePlan();
     void* alloc(size_t bytes);

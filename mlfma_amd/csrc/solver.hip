@@ -1172,30 +1172,41 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     // one right-hand side: the matvecs run their stages in order when the
     // autotuner found that overlap does not pay (2.7% per iteration at 30k)
     d.solveInOrder = d.solveSerial;
-    if (opts->method & ~(0xff | MLFMA_SOLVE_ADJOINT)) throw Error(MLFMA_ERR_ARG, "unknown solve method bits");
+    if (opts->method & ~(0xff | MLFMA_SOLVE_ADJOINT | MLFMA_SOLVE_DEVICE))
+        throw Error(MLFMA_ERR_ARG, "unknown solve method bits");
+    const bool devPtrs = (opts->method & MLFMA_SOLVE_DEVICE) != 0;
     if (method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
     hipCheck(hipSetDevice(d.device), "hipSetDevice");
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t n = d.hp.src.n;
     hipStream_t st = d.stream;
     d.ensureRhs(1);
-    // b (input order) -> tree order
-    std::vector<cplx<double>> bh(n);
-    for (int64_t i = 0; i < n; ++i) {
-        if (opts->use_voltage) {
-            bh[i].re = -d.Vin[i].real();
-            bh[i].im = -d.Vin[i].imag();
-        } else {
-            bh[i].re = b[2 * i];
-            bh[i].im = b[2 * i + 1];
-        }
-        if (!std::isfinite(bh[i].re) || !std::isfinite(bh[i].im)) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
-    }
     // Krylov vectors stay in input order: the matvec kernels apply the tree
     // permutation themselves
     cplx<double>* bt = (cplx<double>*)d.alloc(n * sizeof(cplx<double>));
     cplx<double>* xtree = (cplx<double>*)d.alloc(n * sizeof(cplx<double>));
-    hipCheck(hipMemcpyAsync(bt, bh.data(), n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "b upload");
+    if (devPtrs && !opts->use_voltage) {
+        // b on the device: ready when the call is made (the solve runs on the
+        // plan's stream); a non-finite b shows as a non-finite norm
+        hipCheck(hipMemcpyAsync(bt, b, n * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "b copy");
+    } else {
+        std::vector<cplx<double>> bh(n);
+        for (int64_t i = 0; i < n; ++i) {
+            if (opts->use_voltage) {
+                bh[i].re = -d.Vin[i].real();
+                bh[i].im = -d.Vin[i].imag();
+            } else {
+                bh[i].re = b[2 * i];
+                bh[i].im = b[2 * i + 1];
+            }
+            if (!std::isfinite(bh[i].re) || !std::isfinite(bh[i].im)) {
+                d.release(bt);
+                d.release(xtree);
+                throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
+            }
+        }
+        hipCheck(hipMemcpyAsync(bt, bh.data(), n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "b upload");
+    }
     Krylov K(d, st, std::max(opts->maxit, opts->restart) + 2);
     SolveOut o;
     if (method == MLFMA_GMRES_COMPAT)
@@ -1212,7 +1223,9 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     d.applySolveOp(xtree, d.yt, 0, st, false, false);
     K.sub(bt, d.yt, d.yt);
     const double rn = K.norm(d.yt), bn = K.norm(bt);
-    hipCheck(hipMemcpyAsync(x, xtree, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "x download");
+    hipCheck(hipMemcpyAsync(x, xtree, n * sizeof(cplx<double>), devPtrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                            st),
+             "x download");
     hipCheck(hipStreamSynchronize(st), "solve");
     d.release(bt);
     d.release(xtree);
@@ -1232,21 +1245,39 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     API_END
 }
 
-int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E) {
+int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E, int32_t ptr_mode,
+                   void* stream) {
     API_BEGIN
     DevicePlan& d = Pl(plan);
     if (!I || !dirs || !E || ndir < 0) throw Error(MLFMA_ERR_ARG, "bad farfield arguments");
+    if (ptr_mode != MLFMA_PTR_HOST && ptr_mode != MLFMA_PTR_DEVICE) throw Error(MLFMA_ERR_ARG, "bad ptr_mode");
+    if (ndir > INT32_MAX) throw Error(MLFMA_ERR_ARG, "too many directions");
     if (ndir == 0) return MLFMA_OK;
     hipCheck(hipSetDevice(d.device), "hipSetDevice");
     const int64_t n = d.hp.src.n;
-    hipStream_t st = d.stream;
+    hipStream_t st = stream ? (hipStream_t)stream : d.stream;
+    const size_t need = (size_t)ndir * farfieldChunks(n, ndir) * 3;
+    if (need > d.farPartLen) {
+        if (d.farPart) {
+            hipCheck(hipStreamSynchronize(st), "farfield");
+            d.release(d.farPart);
+        }
+        d.farPart = (cplx<double>*)d.alloc(need * sizeof(cplx<double>));
+        d.farPartLen = need;
+    }
+    if (ptr_mode == MLFMA_PTR_DEVICE) {
+        launchFarfield(d.src, n, (const cplx<double>*)I, d.perm, dirs, ndir, d.hp.k, d.Ck, d.farPart,
+                       (cplx<double>*)E, st);
+        hipCheck(hipGetLastError(), "farfield launch");
+        if (!(d.flags & MLFMA_FLAG_ASYNC)) hipCheck(hipStreamSynchronize(st), "farfield");
+        return MLFMA_OK;
+    }
     d.ensureRhs(1);
     hipCheck(hipMemcpyAsync(d.xs, I, n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "I upload");
-    launchGather(d.xs, d.perm, d.xt, n, 1, st);
     double* dd = (double*)d.alloc((size_t)ndir * 3 * sizeof(double));
     cplx<double>* dE = (cplx<double>*)d.alloc((size_t)ndir * 3 * sizeof(cplx<double>));
     hipCheck(hipMemcpyAsync(dd, dirs, (size_t)ndir * 3 * sizeof(double), hipMemcpyHostToDevice, st), "dirs upload");
-    launchFarfield(d.src, n, d.xt, dd, (int)ndir, d.hp.k, d.Ck, dE, st);
+    launchFarfield(d.src, n, d.xs, d.perm, dd, ndir, d.hp.k, d.Ck, d.farPart, dE, st);
     hipCheck(hipGetLastError(), "farfield launch");
     hipCheck(hipMemcpyAsync(E, dE, (size_t)ndir * 3 * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "E download");
     hipCheck(hipStreamSynchronize(st), "farfield");
@@ -1314,14 +1345,24 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     API_END
 }
 
-int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs, int64_t ndir, double* I) {
+int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs, int64_t ndir, double* I,
+                           int32_t ptr_mode, void* stream) {
     API_BEGIN
     DevicePlan& d = Pl(plan);
     if (!I || !dirs || !E || ndir < 0) throw Error(MLFMA_ERR_ARG, "bad farfield arguments");
+    if (ptr_mode != MLFMA_PTR_HOST && ptr_mode != MLFMA_PTR_DEVICE) throw Error(MLFMA_ERR_ARG, "bad ptr_mode");
     if (ndir > INT32_MAX) throw Error(MLFMA_ERR_ARG, "too many directions");
     hipCheck(hipSetDevice(d.device), "hipSetDevice");
     const int64_t n = d.hp.src.n;
-    hipStream_t st = d.stream;
+    hipStream_t st = stream ? (hipStream_t)stream : d.stream;
+    const cplx<double> Ckc{d.Ck.re, -d.Ck.im};
+    if (ptr_mode == MLFMA_PTR_DEVICE) {
+        launchFarfieldAdj(d.src, n, (const cplx<double>*)E, dirs, (int)ndir, d.hp.k, Ckc, d.perm, (cplx<double>*)I,
+                          st);
+        hipCheck(hipGetLastError(), "farfield adjoint launch");
+        if (!(d.flags & MLFMA_FLAG_ASYNC)) hipCheck(hipStreamSynchronize(st), "farfield adjoint");
+        return MLFMA_OK;
+    }
     d.ensureRhs(1);
     const int64_t nd = std::max<int64_t>(ndir, 1);
     double* dd = (double*)d.alloc((size_t)nd * 3 * sizeof(double));
@@ -1330,7 +1371,6 @@ int mlfma_farfield_adjoint(mlfma_plan* plan, const double* E, const double* dirs
         hipCheck(hipMemcpyAsync(dd, dirs, (size_t)ndir * 3 * sizeof(double), hipMemcpyHostToDevice, st), "dirs upload");
         hipCheck(hipMemcpyAsync(dE, E, (size_t)ndir * 3 * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "E upload");
     }
-    const cplx<double> Ckc{d.Ck.re, -d.Ck.im};
     launchFarfieldAdj(d.src, n, dE, dd, (int)ndir, d.hp.k, Ckc, d.perm, d.xs, st);
     hipCheck(hipGetLastError(), "farfield adjoint launch");
     hipCheck(hipMemcpyAsync(I, d.xs, n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "I download");

@@ -14,6 +14,7 @@ Readers and writers follow the text formats of the reference:
 from __future__ import annotations
 
 import os
+import re
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -164,21 +165,62 @@ def write_config(path: str, cfg: RefConfig):
             f.write(f"{k}: {getattr(cfg, k)}\n")
 
 
+_INT_RE = re.compile(r"\d+")
+_FLOAT_RE = re.compile(r"\d+(\.\d*)?")
+_EXP_RE = re.compile(r"[eE][+-]?\d+")
+
+
+def _skip_to_digit(s: str, i: int) -> int:
+    while i < len(s) and not s[i].isdigit():
+        i += 1
+    return i
+
+
+def _last_number(line: str, kind):
+    """The value the reference's ``operator>>`` leaves in a field of type
+    ``kind`` (int, float or bool) for one config line (src/config.h:16-48):
+    skip to the first digit, extract a value, skip to the next digit, extract
+    again, and so on; the last value wins.  An extraction starts at a digit, so
+    a sign is skipped like any other non-digit and an int field stops at a
+    decimal point.  A float whose exponent has no digits ("1e") fails and
+    leaves 0; a bool other than 0 or 1 leaves True and stops; a line without a
+    number gives 0 (the reference leaves the field unset)."""
+    val = kind(0)
+    i = _skip_to_digit(line, 0)
+    while i < len(line):
+        if kind is float:
+            m = _FLOAT_RE.match(line, i)
+            j = m.end()
+            if j < len(line) and line[j] in "eE":
+                e = _EXP_RE.match(line, j)
+                if e is None:  # "1e" / "1e+": the extraction fails and stores 0
+                    return 0.0
+                j = e.end()
+            val = float(line[i:j])
+        else:
+            m = _INT_RE.match(line, i)
+            j = m.end()
+            v = int(line[i:j])
+            if kind is bool:
+                if v not in (0, 1):
+                    return True
+                val = bool(v)
+            else:
+                val = v
+        i = _skip_to_digit(line, j)
+    return val
+
+
 def read_config(path: str) -> RefConfig:
-    """Each line is read from its first digit on (src/config.h:16-48)."""
-    vals = []
-    with open(path) as f:
-        for line in f:
-            s = line
-            i = 0
-            while i < len(s) and not s[i].isdigit():
-                i += 1
-            tok = s[i:].split()
-            vals.append(tok[0] if tok else "0")
+    """config/config.txt as the reference's Config reads it: one field per line
+    in file order, the last number on a line wins (src/config.h:16-58)."""
+    kinds = {"overInterp": float, "rootLeng": float, "evalDirect": bool}
     cfg = RefConfig()
-    for k, v in zip(_CFG_ORDER, vals):
-        t = type(getattr(cfg, k))
-        setattr(cfg, k, t(float(v)) if t is int else t(v))
+    with open(path) as f:
+        lines = f.read().split("\n")
+    for k, line in zip(_CFG_ORDER, lines):
+        v = _last_number(line, kinds.get(k, int))
+        setattr(cfg, k, int(v) if k == "evalDirect" else v)
     return cfg
 
 

@@ -33,6 +33,28 @@ def _cvec(a, n=None):
     return a
 
 
+def _is_cuda(t) -> bool:
+    try:
+        import torch
+    except Exception:  # torch absent
+        return False
+    return isinstance(t, torch.Tensor) and t.is_cuda
+
+
+def _dev_vec(t, n):
+    import torch
+    if t.dtype != torch.complex128 or not t.is_contiguous() or t.numel() != n:
+        raise ValueError("device vectors must be contiguous complex128 CUDA tensors of n entries")
+    return t
+
+
+def _dirs_tensor(dirs, device):
+    import torch
+    if isinstance(dirs, torch.Tensor):
+        return dirs.to(device=device, dtype=torch.float64).reshape(-1, 3).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)).to(device)
+
+
 class Plan:
     def __init__(self, geom: A.GeometryArrays, pw: A.PlaneWave | None = None, cfg: A.Config | None = None,
                  **cfg_kw):
@@ -119,6 +141,16 @@ class Plan:
                                                       A.MLFMA_PTR_HOST, None))
         return y
 
+    def set_param(self, param: int, value: int) -> None:
+        """Set one of the launch parameters the autotuner chooses
+        (MLFMA_PARAM_*)."""
+        _check(self.lib, self.lib.mlfma_plan_set_param(self._h, int(param), int(value)))
+
+    def get_param(self, param: int) -> int:
+        v = C.c_int32(0)
+        _check(self.lib, self.lib.mlfma_plan_get_param(self._h, int(param), C.byref(v)))
+        return v.value
+
     def set_flags(self, flags: int) -> None:
         """Replace the MLFMA_FLAG_* set used by later matvecs and solves."""
         _check(self.lib, self.lib.mlfma_plan_set_flags(self._h, int(flags)))
@@ -160,14 +192,26 @@ class Plan:
         o.use_voltage = 1 if b is None else 0
         o.tol = tol
         rep = A.SolveReport()
-        x = np.zeros(self.n, dtype=np.complex128)
         hist = np.zeros(maxit)
-        bp = None
-        if b is not None:
-            b = _cvec(b, self.n)
-            bp = b.ctypes.data_as(A.dptr)
-        _check(self.lib, self.lib.mlfma_solve(self._h, bp, x.ctypes.data_as(A.dptr), C.byref(o), C.byref(rep),
-                                              hist.ctypes.data_as(A.dptr)))
+        if _is_cuda(b):
+            # device vectors: b and x stay on the device (the solve runs on the
+            # plan's stream, so torch's queued work on b is finished first)
+            import torch
+            b = _dev_vec(b, self.n)
+            torch.cuda.current_stream(b.device).synchronize()
+            x = torch.empty_like(b)
+            o.method = o.method | A.MLFMA_SOLVE_DEVICE
+            _check(self.lib, self.lib.mlfma_solve(self._h, C.cast(C.c_void_p(b.data_ptr()), A.dptr),
+                                                  C.cast(C.c_void_p(x.data_ptr()), A.dptr), C.byref(o),
+                                                  C.byref(rep), hist.ctypes.data_as(A.dptr)))
+        else:
+            x = np.zeros(self.n, dtype=np.complex128)
+            bp = None
+            if b is not None:
+                b = _cvec(b, self.n)
+                bp = b.ctypes.data_as(A.dptr)
+            _check(self.lib, self.lib.mlfma_solve(self._h, bp, x.ctypes.data_as(A.dptr), C.byref(o), C.byref(rep),
+                                                  hist.ctypes.data_as(A.dptr)))
         report = {"iterations": rep.iterations, "status": A.ERRORS.get(rep.status, rep.status),
                   "resid_est": rep.resid_est, "resid_true": rep.resid_true, "seconds": rep.seconds}
         if history:
@@ -211,22 +255,46 @@ class Plan:
         return X, out
 
     # ---------------------------------------------------------------- far field
-    def farfield(self, I, dirs):
+    def farfield(self, I, dirs, stream=None):
+        """E = F I at the unit directions dirs (ndir x 3).  numpy arrays go
+        through host buffers; with I a CUDA complex128 tensor the directions
+        and the result are CUDA tensors too and stay on the device."""
+        if _is_cuda(I):
+            import torch
+            d = _dirs_tensor(dirs, I.device)
+            E = torch.empty((d.shape[0], 3), dtype=torch.complex128, device=I.device)
+            st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+            _check(self.lib, self.lib.mlfma_farfield(self._h, C.c_void_p(_dev_vec(I, self.n).data_ptr()),
+                                                     C.c_void_p(d.data_ptr()), d.shape[0], C.c_void_p(E.data_ptr()),
+                                                     A.MLFMA_PTR_DEVICE, C.c_void_p(st)))
+            return E
         I = _cvec(I, self.n)
         dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
         E = np.zeros((len(dirs), 3), dtype=np.complex128)
-        _check(self.lib, self.lib.mlfma_farfield(self._h, I.ctypes.data_as(A.dptr), dirs.ctypes.data_as(A.dptr),
-                                                 len(dirs), E.ctypes.data_as(A.dptr)))
+        _check(self.lib, self.lib.mlfma_farfield(self._h, C.c_void_p(I.ctypes.data), C.c_void_p(dirs.ctypes.data),
+                                                 len(dirs), C.c_void_p(E.ctypes.data), A.MLFMA_PTR_HOST, None))
         return E
 
-    def farfield_adjoint(self, E, dirs):
-        """I = F^H E: the adjoint of farfield for the same directions."""
+    def farfield_adjoint(self, E, dirs, stream=None):
+        """I = F^H E: the adjoint of farfield for the same directions (CUDA
+        tensors stay on the device, as in farfield)."""
+        if _is_cuda(E):
+            import torch
+            d = _dirs_tensor(dirs, E.device)
+            Et = E.to(torch.complex128).reshape(d.shape[0], 3).contiguous()
+            I = torch.empty(self.n, dtype=torch.complex128, device=E.device)
+            st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+            _check(self.lib, self.lib.mlfma_farfield_adjoint(self._h, C.c_void_p(Et.data_ptr()),
+                                                             C.c_void_p(d.data_ptr()), d.shape[0],
+                                                             C.c_void_p(I.data_ptr()), A.MLFMA_PTR_DEVICE,
+                                                             C.c_void_p(st)))
+            return I
         dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
         E = np.ascontiguousarray(E, dtype=np.complex128).reshape(len(dirs), 3)
         I = np.zeros(self.n, dtype=np.complex128)
-        _check(self.lib, self.lib.mlfma_farfield_adjoint(self._h, E.ctypes.data_as(A.dptr),
-                                                         dirs.ctypes.data_as(A.dptr), len(dirs),
-                                                         I.ctypes.data_as(A.dptr)))
+        _check(self.lib, self.lib.mlfma_farfield_adjoint(self._h, C.c_void_p(E.ctypes.data),
+                                                         C.c_void_p(dirs.ctypes.data), len(dirs),
+                                                         C.c_void_p(I.ctypes.data), A.MLFMA_PTR_HOST, None))
         return I
 
     def position_gradient(self, I, lam):
