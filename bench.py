@@ -43,42 +43,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+WORKLOADS = {"dipoles4k": "c1", "sphere30k": "c2", "sphere200k": "c3"}
+
+
 def workload(name):
-    from mlfma_amd import _capi as A
-    from mlfma_amd import geometry as G
-    if name == "sphere30k":
-        v, f = G.icosphere(32, 2.0)
-        r = G.rwg_edges(f)
-        geom = A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=r)
-        kw = dict(root_leng=4.0, max_node_srcs=0, max_level=4, interp_order=6, digits=3, over_interp=4.0,
-                  quad_prec=2)
-        pw = A.make_planewave(1.0, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2 * np.pi)
-        desc = {"workload": "rwg_sphere_R2_icosphere32", "unknowns": int(len(r)), "triangles": int(len(f)),
-                "mean_edge_wavelengths": round(G.mesh_stats(v, f, r)["mean_edge"], 4), "leaf_wavelengths": 0.25,
-                "levels": "0-4", "interp_order": 6, "digits": 3, "over_interp": 4.0, "quad_points": 7,
-                "excitation": "plane wave theta=0 x-pol"}
-        return geom, pw, kw, desc
-    if name == "sphere200k":
-        v, f = G.icosphere(82, 4.0)
-        r = G.rwg_edges(f)
-        geom = A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=r)
-        kw = dict(root_leng=8.0, max_node_srcs=0, max_level=5, interp_order=6, digits=3, over_interp=4.0,
-                  quad_prec=2)
-        pw = A.make_planewave(1.0, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2 * np.pi)
-        desc = {"workload": "rwg_sphere_R4_icosphere82", "unknowns": int(len(r)), "triangles": int(len(f)),
-                "mean_edge_wavelengths": round(G.mesh_stats(v, f, r)["mean_edge"], 4), "leaf_wavelengths": 0.25,
-                "levels": "0-5", "interp_order": 6, "digits": 3, "over_interp": 4.0, "quad_points": 7,
-                "excitation": "plane wave theta=0 x-pol"}
-        return geom, pw, kw, desc
-    if name == "dipoles4k":
-        pos, mom = G.uniform_dipoles(4096, 4.0, 0)
-        geom = A.GeometryArrays(A.MLFMA_SRC_DIPOLE, pos=pos, mom=mom)
-        kw = dict(root_leng=4.0, max_node_srcs=0, max_level=3, interp_order=6, digits=3, over_interp=4.0)
-        pw = A.make_planewave(1.0, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2 * np.pi)
-        desc = {"workload": "dipoles_4096_cube4", "unknowns": 4096, "leaf_wavelengths": 0.5, "levels": "0-3",
-                "interp_order": 6, "digits": 3}
-        return geom, pw, kw, desc
-    raise SystemExit(f"unknown workload {name}")
+    from mlfma_amd import configs as CF
+    if name not in WORKLOADS:
+        raise SystemExit(f"unknown workload {name}")
+    return getattr(CF, WORKLOADS[name])()
 
 
 def measured_traffic(kernel, tag):
@@ -128,95 +100,112 @@ def byte_model(info, n):
     return stages, compulsory
 
 
-def physical_solve(geom, pw, kw, radius=2.0, restart=50, maxit=3000):
-    """GMRES(restart) to 1e-6 with the physical near field (closed-form
-    singular integrals) on the sphere, and its bistatic RCS at 361 angles (theta 0-180
-    degrees in 0.5-degree steps, phi = 0 and 90) against the Mie series."""
+def solves(geom, pw, kw, desc):
+    """GMRES to 1e-6 on the workload, plain FP64 against mixed precision and
+    batched right-hand sides.
+
+    The reference's kernel drops the singular self and coincident-triangle
+    terms (src/sources/rwg.cpp:97,113-116); GMRES stalls on that operator
+    (reported: 500 iterations of GMRES(50), FP64 and mixed).  The solve-time
+    comparison therefore runs on the physical near field (closed-form
+    singular integrals; same far field): one incidence (theta = 0) in FP64
+    GMRES(m) and in mixed precision (FP64 FGMRES outer, FP32-far-field GMRES
+    inner), then 8 incidence angles (theta_j = j 180/7 degrees, phi = 0) one
+    FP64 solve at a time against one batched mixed-precision solve.  RCS at
+    722 bistatic angles (theta 0-180 in 0.5-degree steps, phi = 0 and 90) of
+    the mixed solutions against the FP64 ones, and of the FP64 theta = 0
+    solution against the Mie series."""
     from mlfma_amd import _capi as A
+    from mlfma_amd import configs as CF
     from mlfma_amd.plan import Plan
     from tests import _physical as P
-    t0 = time.time()
-    p = Plan(geom, pw, A.make_config(device=0, flags=A.MLFMA_FLAG_PHYSICAL, **kw))
-    setup = time.time() - t0
-    p.autotune(1)  # launch parameters for single right-hand sides (not timed)
-    I, rep = p.solve(method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=1e-6)
-    log("physical GMRES", rep["iterations"], rep["status"], round(rep["seconds"], 3), "s")
-    # the same with the FP32 far field inside FP64 iterative refinement
-    p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, flags=A.MLFMA_FLAG_PHYSICAL, **kw))
+    radius = {"rwg_sphere_R2_icosphere32": 2.0, "rwg_sphere_R4_icosphere82": 4.0}.get(desc["workload"])
+    restart = 100 if radius == 4.0 else 50   # ka = 8 pi lies 0.03 from an EFIE interior resonance
+    out = {}
+    # reference kernel
+    p64 = Plan(geom, pw, A.make_config(device=0, **kw))
+    p64.autotune(1)
+    x64, r64 = p64.solve(method=A.MLFMA_GMRES_RESTART, restart=50, maxit=500, tol=1e-6)
+    p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, **kw))
     p32.autotune(1)
-    I32, rep32 = p32.solve(method=A.MLFMA_FGMRES_IR, restart=restart, maxit=maxit, tol=1e-6)
-    log("physical FP32-IR", rep32["iterations"], rep32["status"], round(rep32["seconds"], 3), "s")
-    mixed = {"method": "FP64 refinement, FP32 far field, MLFMA_FLAG_PHYSICAL", "iterations": rep32["iterations"],
-             "status": rep32["status"], "resid_true": rep32["resid_true"], "seconds": rep32["seconds"],
-             "solution_vs_fp64_rel": float(np.linalg.norm(I32 - I) / np.linalg.norm(I))}
-    # eight incidence angles theta_j = j 180/7 degrees (phi = 0, theta-polarised):
-    # one batched solve against eight single solves
-    angles = np.radians(np.arange(8) * 180.0 / 7.0)
-    B = np.stack([-p.excitation(A.make_planewave(1.0, (np.cos(t), 0.0, -np.sin(t)), (np.sin(t), 0.0, np.cos(t)),
-                                                 2 * np.pi)) for t in angles])
-    ts = time.perf_counter()
-    singles = []
-    for j in range(8):
-        singles.append(p.solve(B[j], method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=1e-6))
-        log("physical single", j, singles[-1][1]["iterations"])
-    t_single = time.perf_counter() - ts
-    p.autotune(8)  # launch parameters for batches of 8 (not timed)
-    tb = time.perf_counter()
-    Xb, repb = p.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6)
-    t_batch = time.perf_counter() - tb
-    log("physical batched sweep", round(t_batch, 3), "s")
-    # batched and mixed precision together: FP64 refinement around batched
-    # FP32-far-field GMRES, against the plain FP64 single solves above
-    p32.autotune(8)
-    tm = time.perf_counter()
-    Xm, repm = p32.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6, method=A.MLFMA_FGMRES_IR)
-    t_mixed = time.perf_counter() - tm
-    log("physical batched mixed sweep", round(t_mixed, 3), "s")
+    x32, r32 = p32.solve(method=A.MLFMA_FGMRES_IR, restart=50, maxit=500, tol=1e-6)
+    keys = ["iterations", "status", "resid_true", "seconds"]
+    out["reference_kernel"] = {"fp64_gmres50": {k: r64[k] for k in keys}, "mixed_fgmres": {k: r32[k] for k in keys},
+                               "note": "the reference's operator (self and coincident-triangle terms dropped): "
+                                       "GMRES stalls; 500 iterations"}
+    del p64, p32
+    if radius is None:
+        return out
+    fl = A.MLFMA_FLAG_PHYSICAL | A.MLFMA_FLAG_FRESH_LOCALS
+    t0 = time.time()
+    q64 = Plan(geom, pw, A.make_config(device=0, flags=fl, **kw))
+    setup = time.time() - t0
+    q64.autotune(1)
+    q32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, flags=fl, **kw))
+    q32.autotune(1)
+    maxit = 6000
+    I64, s64 = q64.solve(method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=1e-6)
+    log("physical FP64", s64)
+    I32, s32 = q32.solve(method=A.MLFMA_FGMRES_IR, restart=restart, maxit=maxit, tol=1e-6)
+    log("physical mixed", s32)
     dirs = []
-    for phi in (0.0, np.pi / 2):
-        t_ = np.radians(np.linspace(0.0, 180.0, 361))
-        dirs.append(np.stack([np.sin(t_) * np.cos(phi), np.sin(t_) * np.sin(phi), np.cos(t_)], axis=1))
-    dirs = np.concatenate(dirs)
-    rcs_diff = 0.0
-    for j in range(8):
-        s64 = np.sum(np.abs(p.farfield(singles[j][0], dirs)) ** 2, axis=1)
-        smx = np.sum(np.abs(p.farfield(Xm[j], dirs)) ** 2, axis=1)
-        rcs_diff = max(rcs_diff, float(np.abs(smx - s64).max() / s64.max()))
-    sweep_mixed = {"seconds": round(t_mixed, 4), "speedup_vs_sequential_fp64": round(t_single / t_mixed, 3),
-                   "all_converged": all(r["status"] == "OK" for r in repm),
-                   "max_resid_true": max(r["resid_true"] for r in repm),
-                   "rcs_max_rel_diff_vs_fp64": rcs_diff}
-    sweep = {"angles": 8, "batched_seconds": round(t_batch, 4), "sequential_seconds": round(t_single, 4),
-             "batched_mixed": sweep_mixed,
-             "iterations": [r["iterations"] for r in repb], "all_converged": all(r["status"] == "OK" for r in repb),
-             "max_resid_true": max(r["resid_true"] for r in repb),
-             "batched_vs_sequential_solution_rel": max(float(np.linalg.norm(Xb[j] - singles[j][0]) /
-                                                             np.linalg.norm(singles[j][0])) for j in range(8))}
     th = np.radians(np.linspace(0.0, 180.0, 361))
+    for phi in (0.0, np.pi / 2):
+        dirs.append(np.stack([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)], axis=1))
+    dirs = np.concatenate(dirs)
+
+    def rcs(p, I):
+        return 4 * np.pi * np.sum(np.abs(p.farfield(I, dirs)) ** 2, axis=1)
+
+    def rcs_diff(a, b):
+        return float(np.abs(a - b).max() / np.abs(b).max())
+
     se, sh = P.mie_pec_rcs(2 * np.pi * radius, th)
-    errs = []
-    for phi, ref in ((0.0, se), (np.pi / 2, sh)):
-        d = np.stack([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)], axis=1)
-        E = p.farfield(I, d)
-        errs.append(10 * np.log10(4 * np.pi * np.sum(np.abs(E) ** 2, axis=1)) - 10 * np.log10(ref))
-    errs = np.concatenate(errs)
-    return {"method": f"GMRES({restart}), FP64, MLFMA_FLAG_PHYSICAL", "maxit": maxit, "iterations": rep["iterations"],
-            "status": rep["status"], "resid_true": rep["resid_true"], "seconds": rep["seconds"],
-            "plan_setup_s": round(setup, 3),
-            "rcs_vs_mie_db": {"angles": 2 * len(th), "rms": round(float(np.sqrt(np.mean(errs ** 2))), 4),
-                              "max": round(float(np.abs(errs).max()), 4)},
-            "mixed": mixed, "incidence_sweep": sweep}
+    mie = np.concatenate([se, sh])
+    db = 10 * np.log10(rcs(q64, I64)) - 10 * np.log10(mie)
+    single = {"fp64": {k: s64[k] for k in keys}, "mixed": {k: s32[k] for k in keys},
+              "mixed_speedup": round(s64["seconds"] / s32["seconds"], 3),
+              "mixed_rcs_rel_diff_vs_fp64": rcs_diff(rcs(q32, I32), rcs(q64, I64)),
+              "mixed_solution_rel_diff_vs_fp64": float(np.linalg.norm(I32 - I64) / np.linalg.norm(I64)),
+              "fp64_rcs_vs_mie_db": {"rms": round(float(np.sqrt(np.mean(db ** 2))), 4),
+                                     "max": round(float(np.abs(db).max()), 4)}}
+    B = np.stack([-q64.excitation(CF.planewave_xz(t)) for t in CF.c4_angles()])
+    ts = time.perf_counter()
+    seq = [q64.solve(B[j], method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=1e-6) for j in range(8)]
+    t_seq = time.perf_counter() - ts
+    log("physical 8 angles FP64 one at a time", round(t_seq, 3), [r["iterations"] for _, r in seq])
+    q32.autotune(8)
+    tm = time.perf_counter()
+    Xm, rm = q32.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6, method=A.MLFMA_FGMRES_IR)
+    t_mix = time.perf_counter() - tm
+    log("physical 8 angles batched mixed", round(t_mix, 3), [r["iterations"] for r in rm])
+    q64.autotune(8)
+    tb = time.perf_counter()
+    Xb, rb = q64.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6)
+    t_b64 = time.perf_counter() - tb
+    rd = max(rcs_diff(rcs(q32, Xm[j]), rcs(q64, seq[j][0])) for j in range(8))
+    sweep = {"angles": 8, "fp64_one_at_a_time_s": round(t_seq, 4), "batched_fp64_s": round(t_b64, 4),
+             "batched_mixed_s": round(t_mix, 4), "speedup_batched_mixed_vs_fp64": round(t_seq / t_mix, 3),
+             "all_converged": all(r["status"] == "OK" for r in rm) and all(r["status"] == "OK" for _, r in seq),
+             "max_resid_true_mixed": max(r["resid_true"] for r in rm),
+             "mixed_rcs_max_rel_diff_vs_fp64": rd,
+             "iterations_fp64": [r["iterations"] for _, r in seq], "iterations_mixed": [r["iterations"] for r in rm]}
+    out["physical"] = {"near_field": "MLFMA_FLAG_PHYSICAL (closed-form singular integrals)",
+                       "gmres": f"FP64 GMRES({restart}) / mixed FGMRES, tol 1e-6", "plan_setup_s": round(setup, 3),
+                       "single_incidence": single, "incidence_sweep": sweep}
+    return out
 
 
-def cpu_baseline(geom, pw, kw, budget_s=20.0, y_gpu=None):
+def cpu_baseline(geom, pw, kw, budget_s, y_gpu, x):
+    """The CPU oracle (FP64 restatement of the reference, one thread for the
+    matvec as the reference) on a bounded number of matvecs of the same
+    workload; its first matvec also checks the GPU's timed matvec of x."""
     from mlfma_amd import _capi as A
     from tests import _oracle as O
-    from mlfma_amd import geometry as G
     t0 = time.time()
     nt = max(1, min(16, os.cpu_count() or 1))
     o = O.Oracle(geom, pw, A.make_config(device=-1, **kw), threads=nt)
     setup = time.time() - t0
-    x = G.complex_gaussian(geom.n, 0)
     y_cpu = o.matvec(x)  # warm; also the parity check of the timed GPU matvec
     parity = None
     if y_gpu is not None:
@@ -314,7 +303,7 @@ def gradient_job():
 
 def multi_scatterer_job():
     """16 scatterers of uneven size (seed 1: sphere or 2:1:1 ellipsoid, equivalent
-    radius uniform in 0.5-3 wavelengths; icosphere frequency 12 per wavelength
+    radius uniform in 0.5-3 wavelengths; icosphere frequency 15 per wavelength
     of radius, of the short semi-axis for ellipsoids), each under 8 plane waves (theta_j = j 180/7 degrees, phi = 0,
     x-z polarisation in the plane of incidence): 128 right-hand sides.  Each
     scatterer's 8 right-hand sides go through one batched FP64-refined solve
@@ -326,29 +315,8 @@ def multi_scatterer_job():
     from mlfma_amd import geometry as G
     from mlfma_amd.plan import Plan
     from mlfma_amd.batch import solve_many
-    rng = np.random.default_rng(1)
-    shapes = []
-    for i in range(16):
-        r_eq = rng.uniform(0.5, 3.0)
-        ell = rng.uniform() < 0.5
-        if ell:
-            a = r_eq / 2 ** (1 / 3)
-            axes = (2 * a, a, a)
-            freq = max(4, int(round(12 * a)))
-            v, f = G.ellipsoid(freq, axes)
-            ext = 2 * a
-        else:
-            freq = max(4, int(round(12 * r_eq)))
-            v, f = G.icosphere(freq, r_eq)
-            ext = r_eq
-        r = G.rwg_edges(f)
-        L = 2
-        while 0.25 * 2 ** L < 2.05 * ext:
-            L += 1
-        kw = dict(root_leng=0.25 * 2 ** L, max_node_srcs=0, max_level=L, interp_order=6, digits=3, over_interp=4.0,
-                  quad_prec=2)
-        shapes.append(dict(kind="ellipsoid" if ell else "sphere", r_eq=round(float(r_eq), 3), n=int(len(r)),
-                           levels=L, geom=A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=r), kw=kw))
+    from mlfma_amd import configs as CF
+    shapes = CF.c4()
     k = 2 * np.pi
     angles = [j * np.pi / 7 for j in range(8)]
     t0 = time.time()
@@ -391,12 +359,48 @@ def multi_scatterer_job():
     print(json.dumps(out), flush=True)
 
 
+def kernel_bytes(group, info, n):
+    """Algorithmic bytes per launch of the kernel of a launch group (FP64,
+    c = 16 B): the per-unit figures times the units one launch processes.
+      S2M, L2T  32 B per (source, leaf direction) of radiation patterns, plus
+                the leaf fields (32 B per leaf direction) once
+      NEAR      16 B per stored near-field value
+      M2L       compulsory: operators (16 B per offset and direction), source
+                multipoles and target locals (32 B per box direction) once;
+                streamed: 32 B per (pair, direction) of source reads instead
+      M2L_NEAR  the M2L bytes of the per-target levels plus NEAR
+    Returns (bytes, streamed bytes)."""
+    c = 16
+    ml = info["max_level"]
+    nd = [info["nth"][l] * info["nph"][l] for l in range(ml + 1)]
+    B, P = info["boxes"], info["m2l_pairs"]
+    leaf = info["radpat_entries"] * 2 * c + info["boxes"][ml] * nd[ml] * 2 * c
+    near = info["near_nnz"] * c
+    m2l_comp = sum(316 * nd[l] * c + 2 * B[l] * nd[l] * 2 * c for l in range(2, ml + 1) if P[l] > 0)
+    m2l_str = sum(P[l] * nd[l] * 2 * c + 316 * nd[l] * c + B[l] * nd[l] * 2 * c for l in range(2, ml + 1) if P[l] > 0)
+    if group in ("S2M", "L2T"):
+        return leaf, leaf
+    if group == "NEAR":
+        return near, near
+    if group == "M2L_TARGET":
+        return m2l_comp, m2l_str
+    if group == "M2L_NEAR":
+        return m2l_comp + near, m2l_str + near
+    return None, None
+
+
+KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2Tx", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
+                "M2M": "kM2MFused", "L2L": "kL2LFused", "M2L_SIBLING": "kM2LGroup"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="sphere30k")
+    ap.add_argument("--workload", default="sphere200k",
+                    help="sphere200k (default: the 201,720-RWG sphere), sphere30k, dipoles4k, multi16, "
+                         "dipoles200k_grad")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES solves")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -441,16 +445,17 @@ def main():
     dev = torch.device("cuda:0")
     x = torch.from_numpy(G.complex_gaussian(n, 0)).to(dev)
     y = torch.empty_like(x)
-    stages = {k: 0.0 for k in ["S2M", "M2M", "M2L", "L2L", "L2T", "near", "total"]}
-    near_pairs_ms = 0.0
-    # the library's autotuner picks M2M / M2L launch shapes for this plan
+    # the library's autotuner picks launch shapes and the stage schedule
     tune_before, tune_after = plan.autotune(1)
-    log("autotune ms", round(tune_before, 4), "->", round(tune_after, 4))
+    schedule = {"near_overlap": plan.get_param(A.MLFMA_PARAM_NEAR_OVERLAP),
+                "m2l_split": plan.get_param(A.MLFMA_PARAM_M2L_SPLIT),
+                "near_window": plan.get_param(A.MLFMA_PARAM_NEAR_WINDOW)}
+    log("autotune ms", round(tune_before, 4), "->", round(tune_after, 4), schedule)
+    plan.set_flags(A.MLFMA_FLAG_ASYNC)
     for _ in range(args.warmup):
         plan.matvec(x, y)
     # timed region: K matvecs, uninstrumented, queued on torch's stream (the
     # library returns once a matvec is queued; the synchronize closes it)
-    plan.set_flags(A.MLFMA_FLAG_ASYNC)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -459,58 +464,78 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
     value = args.steps / elapsed
-    # the same with every stage in order on one stream (the default runs the
-    # near-field pairs and the finest M2L beside the far-field chain)
-    plan.set_flags(A.MLFMA_FLAG_ASYNC | A.MLFMA_FLAG_ONE_STREAM)
-    torch.cuda.synchronize()
-    t0n = time.perf_counter()
-    for _ in range(args.steps):
+    y_timed = y.cpu().numpy()
+    # the same K matvecs once per launch group with that group's launches
+    # bracketed by HIP events on the stream each runs on (same schedule)
+    groups = {}
+    for g in ["S2M", "M2M", "M2L_TARGET", "M2L_SIBLING", "M2L_NEAR", "L2L", "NEAR", "L2T"]:
+        plan.probe(g)
+        for _ in range(args.steps):
+            plan.matvec(x, y)
+        torch.cuda.synchronize()
+        ms, cnt = plan.probe_read()
+        if cnt:
+            groups[g] = {"kernel": KERNEL_NAMES[g], "launches_per_matvec": round(cnt / args.steps, 3),
+                         "avg_launch_ms": round(ms / cnt, 5), "ms_per_matvec": round(ms / args.steps, 5)}
+    plan.probe(None)
+    log("launch groups", groups)
+    # other schedules of the same matvec
+    rates = {}
+    for name, fl in [("one_stream", A.MLFMA_FLAG_ASYNC | A.MLFMA_FLAG_ONE_STREAM),
+                     ("graph_replay", A.MLFMA_FLAG_ASYNC | A.MLFMA_FLAG_GRAPH), ("synchronous_calls", 0)]:
+        plan.set_flags(fl)
         plan.matvec(x, y)
-    torch.cuda.synchronize()
-    value_serial = args.steps / (time.perf_counter() - t0n)
-    # the same K matvecs replayed from a captured hipGraph
-    plan.set_flags(A.MLFMA_FLAG_ASYNC | A.MLFMA_FLAG_GRAPH)
-    plan.matvec(x, y)
-    torch.cuda.synchronize()
-    tg = time.perf_counter()
-    for _ in range(args.steps):
-        plan.matvec(x, y)
-    torch.cuda.synchronize()
-    value_graph = args.steps / (time.perf_counter() - tg)
-    # the same with a completed result per call (the synchronous entry point)
-    plan.set_flags(0)
-    torch.cuda.synchronize()
-    t0s = time.perf_counter()
-    for _ in range(args.steps):
-        plan.matvec(x, y)
-    torch.cuda.synchronize()
-    value_sync = args.steps / (time.perf_counter() - t0s)
-    # the same K matvecs again with the per-stage HIP events recorded on the
-    # library's stream: stage times and the roofline kernel's launch time
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.matvec(x, y)
+        torch.cuda.synchronize()
+        rates[name] = round(args.steps / (time.perf_counter() - t1), 3)
+    # per-stage device times, every stage in order with events (the
+    # reference's ClockTimes)
     plan.set_flags(A.MLFMA_FLAG_STAGE_TIMES)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    stages = {k: 0.0 for k in ["S2M", "M2M", "M2L", "L2L", "L2T", "near", "total"]}
     for _ in range(args.steps):
         plan.matvec(x, y)
-        st = plan.stage_times()
-        for k in stages:
-            stages[k] += st[k]
-        near_pairs_ms += plan.stage_times_detail()["near_pairs"]
-    torch.cuda.synchronize()
-    with_events = args.steps / (time.perf_counter() - t1)
+        for k, v in plan.stage_times().items():
+            stages[k] += v
     plan.set_flags(0)
     stage_ms = {k: round(v / args.steps, 4) for k, v in stages.items()}
-    log("stage ms", stage_ms)
-    log("matvec/s", round(value, 2), "one stream", round(value_serial, 2), "graph replay", round(value_graph, 2),
-        "synchronous calls", round(value_sync, 2),
-        "with stage events",
-        round(with_events, 2))
+    log("stage ms", stage_ms, "rates", rates)
+
+    # roofline of the kernel that takes the most device time per matvec in
+    # the timed schedule (among kernels with a byte model)
+    cand = {g: v for g, v in groups.items() if kernel_bytes(g, info, n)[0]}
+    dom = max(cand, key=lambda g: cand[g]["ms_per_matvec"])
+    kb, kstream = kernel_bytes(dom, info, n)
+    avg = groups[dom]["avg_launch_ms"]
+    achieved = kb / (avg * 1e-3) / 1e9
+    tag = desc["workload"]
+    traffic = measured_traffic(KERNEL_NAMES[dom], tag)
+    roofline = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic) if traffic else None, "bytes_per_launch": int(kb),
+                "avg_launch_ms": avg, "launch_group": dom}
+    if kstream != kb:
+        roofline["streamed_bytes_per_launch"] = int(kstream)
+        roofline["streamed_frac"] = round(kstream / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    log("roofline", roofline)
+    kernels = {}
+    for g, v in groups.items():
+        b, bs = kernel_bytes(g, info, n)
+        e = dict(v)
+        if b:
+            e["bytes_per_launch"] = int(b)
+            e["frac_hbm_peak"] = round(b / (v["avg_launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        kernels[g] = e
+
     # batched right-hand sides: 8 vectors per call share every operator read
     nb = 8
     Xb = torch.from_numpy(np.stack([G.complex_gaussian(n, 50 + r) for r in range(nb)])).to(dev)
     Yb = torch.empty_like(Xb)
+    plan.set_flags(A.MLFMA_FLAG_FRESH_LOCALS)
     tb_before, tb_after = plan.autotune(nb)
-    plan.set_flags(A.MLFMA_FLAG_ASYNC)
+    plan.set_flags(A.MLFMA_FLAG_ASYNC | A.MLFMA_FLAG_FRESH_LOCALS)
     for _ in range(2):
         plan.matvec(Xb, Yb)
     torch.cuda.synchronize()
@@ -521,88 +546,38 @@ def main():
     torch.cuda.synchronize()
     el_b = time.perf_counter() - t2
     plan.set_flags(0)
-    batched = {"nrhs": nb, "matvec_per_s": round(nb * calls / el_b, 3), "ms_per_call": round(1000 * el_b / calls, 4),
-               "autotune_ms": {"before": round(tb_before, 4), "after": round(tb_after, 4)}}
+    batched = {"nrhs": nb, "matvec_per_s": round(nb * calls / el_b, 3), "ms_per_call": round(1000 * el_b / calls, 4)}
     log("batched", batched)
 
-    # roofline of the near-field pair kernel (one launch per matvec; HIP events
-    # on the library's stream right before and after that launch)
-    near_ms = near_pairs_ms / args.steps
-    achieved = near_bytes(info) / (near_ms * 1e-3) / 1e9
-    traffic = measured_traffic("kNearSym", "sphere30k_fp64") if args.workload == "sphere30k" else None
-    roofline = {"bound": "hbm", "kernel": "kNearSym", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": round(traffic) if traffic else None,
-                "bytes_per_launch": near_bytes(info), "avg_launch_ms": round(near_ms, 5)}
-
-    # streamed-byte model per stage against the stage times (in-order,
-    # event-timed matvecs) and for the whole matvec against ms_per_step
     sb, comp = byte_model(info, n)
     total_b = sum(sb.values())
     bytes_model = {
-        "stages": {k: {"bytes": int(v), "ms": stage_ms[k], "GBps": round(v / (stage_ms[k] * 1e-3) / 1e9, 1),
-                       "frac_hbm_peak": round(v / (stage_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
-                   for k, v in sb.items() if stage_ms[k] > 0},
         "matvec_streamed_bytes": int(total_b),
-        "matvec_effective_GBps": round(total_b / (ms_per_step * 1e-3) / 1e9, 1),
-        "matvec_frac_hbm_peak": round(total_b / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+        "matvec_streamed_frac_hbm_peak": round(total_b / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
         "compulsory_bytes": int(comp),
-        "compulsory_GBps": round(comp / (ms_per_step * 1e-3) / 1e9, 1),
-        "note": "M2L re-reads each source field and operator from L2/MALL, so its streamed-byte rate exceeds "
-                "the HBM peak; its HBM traffic is a fraction of the streamed bytes",
+        "compulsory_frac_hbm_peak": round(comp / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
     }
-    log("byte model", bytes_model)
 
     result = {
-        "metric": "MLFMA matvec/s (FP64, one MI355X)", "value": round(value, 3), "unit": "matvec/s",
+        "metric": "MLFMA matvec/s (FP64, one MI355X); GMRES solve s to 1e-6, mixed/batched vs plain FP64",
+        "value": round(value, 3), "unit": "matvec/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": desc, "roofline": roofline, "stage_ms": stage_ms, "setup_s": round(setup_s, 2),
-        "autotune_ms": {"before": round(tune_before, 4), "after": round(tune_after, 4)},
-        "value_one_stream": round(value_serial, 3), "value_graph_replay": round(value_graph, 3),
-        "value_synchronous_calls": round(value_sync, 3),
-        "value_with_stage_events": round(with_events, 3),
-        "batched": batched,
+        "config": desc, "roofline": roofline, "kernels": kernels, "schedule": schedule, "stage_ms": stage_ms,
+        "setup_s": round(setup_s, 2), "rates_other_schedules": rates, "batched": batched,
         "byte_model": bytes_model,
     }
 
     if not args.no_solve:
-        sol = {}
-        p64 = Plan(geom, pw, A.make_config(device=0, **kw))
-        p64.autotune(1)  # launch parameters and stage order (not timed)
-        x64, r64 = p64.solve(method=A.MLFMA_GMRES_RESTART, restart=50, maxit=500, tol=1e-6)
-        sol["fp64_gmres50"] = {k: r64[k] for k in ["iterations", "status", "resid_true", "seconds"]}
-        p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, **kw))
-        p32.autotune(1)
-        x32, r32 = p32.solve(method=A.MLFMA_FGMRES_IR, restart=50, maxit=500, tol=1e-6)
-        sol["mixed_fp32_ir"] = {k: r32[k] for k in ["iterations", "status", "resid_true", "seconds"]}
-        sol["mixed_vs_fp64_solution_rel"] = float(np.linalg.norm(x32 - x64) / max(np.linalg.norm(x64), 1e-300))
-        # FP32 far-field matvec throughput (queued on torch's stream, as the
-        # FP64 figure)
-        xx = torch.from_numpy(G.complex_gaussian(n, 0)).to(dev)
-        yy = torch.empty_like(xx)
-        p32.set_flags(A.MLFMA_FLAG_ASYNC)
-        for _ in range(3):
-            p32.matvec(xx, yy)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.steps):
-            p32.matvec(xx, yy)
-        torch.cuda.synchronize()
-        p32.set_flags(0)
-        sol["fp32_farfield_matvec_per_s"] = round(args.steps / (time.perf_counter() - t1), 3)
-        if args.workload in ("sphere30k", "sphere200k"):
-            if args.workload == "sphere30k":
-                sol["physical"] = physical_solve(geom, pw, kw, radius=2.0)
-            else:  # ka = 8 pi lies 0.03 from an EFIE interior resonance: longer restarts
-                sol["physical"] = physical_solve(geom, pw, kw, radius=4.0, restart=100, maxit=6000)
-        result["solve"] = sol
-        log("solve", sol)
+        result["solve"] = solves(geom, pw, kw, desc)
 
     if not args.no_cpu:
         log("cpu baseline ...")
-        y_gpu = plan.matvec(G.complex_gaussian(n, 0))
-        result["cpu_baseline"] = cpu_baseline(geom, pw, kw, args.cpu_budget, y_gpu)
+        x0 = G.complex_gaussian(n, 0)
+        if info["stale_boxes"]:
+            # the reference's stale locals start at zero: compare a fresh plan's first matvec
+            y_timed = Plan(geom, pw, A.make_config(device=0, **kw)).matvec(x0)
+        result["cpu_baseline"] = cpu_baseline(geom, pw, kw, args.cpu_budget, y_timed, x0)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

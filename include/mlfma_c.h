@@ -94,7 +94,8 @@ typedef struct mlfma_planewave {
                                        hipGraph once per (x, y, nrhs,
                                        accumulate, precision) and replay it
                                        (up to 4 cached; cleared when the flags
-                                       change; not with stage times) */
+                                       change; not with stage times or a
+                                       probe group) */
 #define MLFMA_FLAG_ONE_STREAM 128u /* run every stage in order on the
                                        matvec's stream, instead of the
                                        near-field pair products and the
@@ -322,8 +323,30 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
 #define MLFMA_PARAM_NEAR_OVERLAP 2
 #define MLFMA_PARAM_M2L_SPLIT 3
 #define MLFMA_PARAM_SOLVE_IN_ORDER 4
+#define MLFMA_PARAM_PROBE_GROUP 5  /* MLFMA_GROUP_* whose launches carry probe
+                                      events (0: none); see mlfma_probe_read */
 int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value);
 int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value);
+
+/* Launch groups of a matvec.  With MLFMA_PARAM_PROBE_GROUP = g, every launch
+ * of group g is bracketed by a pair of HIP events recorded on the stream the
+ * launch runs on (whatever the schedule: side streams or in order), so its
+ * device time is measured inside the schedule being timed; each record costs
+ * that stream a few microseconds.  mlfma_probe_read returns the summed
+ * milliseconds and the number of launches since the group was set or last
+ * read, and resets both.  No reference counterpart (the reference's stage
+ * timers, src/clock.h:8-17, time whole stages). */
+#define MLFMA_GROUP_S2M 1          /* kS2Mx                                      */
+#define MLFMA_GROUP_M2M 2          /* one M2M level (kM2MFused)                  */
+#define MLFMA_GROUP_M2L_TARGET 3   /* per-target M2L levels (kM2Lw / kM2Lx)      */
+#define MLFMA_GROUP_M2L_SIBLING 4  /* sibling-group M2L levels (kM2LGroup)       */
+#define MLFMA_GROUP_M2L_NEAR 5     /* per-target M2L with the near-field pairs in
+                                      the same launch (kM2LNear)                */
+#define MLFMA_GROUP_L2L 6          /* one L2L level (kL2LFused)                  */
+#define MLFMA_GROUP_NEAR 7         /* near-field pair products (kNearSym)        */
+#define MLFMA_GROUP_L2T 8          /* kL2Tx                                      */
+#define MLFMA_GROUP_LAST 8
+int mlfma_probe_read(mlfma_plan* plan, double* ms, int64_t* launches);
 
 /* Replace the plan's MLFMA_FLAG_* set for later matvecs and solves (the flags
  * act at apply time; the plan itself does not change).  No reference

@@ -36,6 +36,8 @@ MLFMA_PARAM_NEAR_WINDOW = 1
 MLFMA_PARAM_NEAR_OVERLAP = 2
 MLFMA_PARAM_M2L_SPLIT = 3
 MLFMA_PARAM_SOLVE_IN_ORDER = 4
+MLFMA_PARAM_PROBE_GROUP = 5
+GROUPS = {"S2M": 1, "M2M": 2, "M2L_TARGET": 3, "M2L_SIBLING": 4, "M2L_NEAR": 5, "L2L": 6, "NEAR": 7, "L2T": 8}
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_OOM", -3: "ERR_HIP", -4: "ERR_BREAKDOWN",
           -5: "ERR_NONFINITE", -6: "ERR_STATE", -7: "ERR_GEOMETRY", -8: "ERR_NOT_CONVERGED"}
@@ -110,6 +112,7 @@ SIGNATURES = {
     "mlfma_plan_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mlfma_plan_set_param": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "mlfma_plan_get_param": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "mlfma_probe_read": (C.c_int, [C.c_void_p, dptr, C.POINTER(C.c_int64)]),
     "mlfma_autotune": (C.c_int, [C.c_void_p, C.c_int32, dptr, dptr]),
     "mlfma_stage_times_detail": (C.c_int, [C.c_void_p, dptr, C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_level_boxes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), iptr, iptr, iptr,

@@ -52,14 +52,53 @@ void DevicePlan::clearGraphs() {
     graphs.clear();
 }
 
+// Probe events live in a ring; a pair is read (its elapsed time added) when
+// its slot is needed again or when the caller asks for the total.
+void DevicePlan::probeBegin(int group, hipStream_t s) {
+    if (probeGroup != group) return;
+    if (probeRing.empty()) {
+        probeRing.resize(256);
+        for (auto& p : probeRing) {
+            hipCheck(hipEventCreate(&p.a), "hipEventCreate");
+            hipCheck(hipEventCreate(&p.b), "hipEventCreate");
+        }
+    }
+    if (probePending == probeRing.size()) probeDrain(probeRing.size() - 1);
+    hipCheck(hipEventRecord(probeRing[probeHead].a, s), "event");
+}
+
+void DevicePlan::probeEnd(int group, hipStream_t s) {
+    if (probeGroup != group) return;
+    hipCheck(hipEventRecord(probeRing[probeHead].b, s), "event");
+    probeHead = (probeHead + 1) % probeRing.size();
+    ++probePending;
+}
+
+// read the oldest pending pairs until at most `keep` remain
+void DevicePlan::probeDrain(size_t keep) {
+    while (probePending > keep) {
+        const size_t i = (probeHead + probeRing.size() - probePending) % probeRing.size();
+        hipCheck(hipEventSynchronize(probeRing[i].b), "probe");
+        float t = 0;
+        hipCheck(hipEventElapsedTime(&t, probeRing[i].a, probeRing[i].b), "hipEventElapsedTime");
+        probeMs += t;
+        ++probeLaunches;
+        --probePending;
+    }
+}
+
 DevicePlan::~DevicePlan() {
     if (hostOnly) return;
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    clearGraphs();
-    for (void* p : allocs) (void)hipFree(p);
     if (nearStream) (void)hipStreamSynchronize(nearStream);
     if (m2lStream) (void)hipStreamSynchronize(m2lStream);
+    clearGraphs();
+    for (void* p : allocs) (void)hipFree(p);
+    for (auto& p : probeRing) {
+        if (p.a) (void)hipEventDestroy(p.a);
+        if (p.b) (void)hipEventDestroy(p.b);
+    }
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {evFork, evJoin, evM2L, evNear})
@@ -651,21 +690,32 @@ bool DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStr
             t.L[k] = out + L.Moff;
         }
     }
-    if (g.nlev) launchM2LGroup<R>(g, adjoint, st, nb);
+    if (g.nlev) {
+        probeBegin(MLFMA_GROUP_M2L_SIBLING, st);
+        launchM2LGroup<R>(g, adjoint, st, nb);
+        probeEnd(MLFMA_GROUP_M2L_SIBLING, st);
+    }
     if (nearX && t.nlev && nb == 1 && nNearItems > 0) {
         // the FP32 far field reads the FP32 copy of the near-field values
         // (the adjoint the FP64 values)
+        probeBegin(MLFMA_GROUP_M2L_NEAR, st);
         if constexpr (std::is_same<R, float>::value)
             if (nearVal32 && !adjoint) {
                 launchM2LNear<R, float>(t, nearItems, nNearItems, nearColsIn, perm, nearVal32, nearX, nearPart, st,
                                         false, nearWindow);
+                probeEnd(MLFMA_GROUP_M2L_NEAR, st);
                 return true;
             }
         launchM2LNear<R, double>(t, nearItems, nNearItems, nearColsIn, perm, nearVal, nearX, nearPart, st, adjoint,
                                  nearWindow);
+        probeEnd(MLFMA_GROUP_M2L_NEAR, st);
         return true;
     }
-    if (t.nlev) launchM2Lx<R>(t, adjoint, st, nb);
+    if (t.nlev) {
+        probeBegin(MLFMA_GROUP_M2L_TARGET, st);
+        launchM2Lx<R>(t, adjoint, st, nb);
+        probeEnd(MLFMA_GROUP_M2L_TARGET, st);
+    }
     return false;
 }
 
@@ -702,6 +752,12 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // join through one: the M2L stream waits for the near field before it
     // records evM2L.
     auto launchNear = [&](hipStream_t s) {
+        probeBegin(MLFMA_GROUP_NEAR, s);
+        struct End {
+            DevicePlan* d;
+            hipStream_t s;
+            ~End() { d->probeEnd(MLFMA_GROUP_NEAR, s); }
+        } end{this, s};
         // the FP32 far field (mixed-precision inner solves) reads the FP32
         // copy of the near-field values
         if (std::is_same<R, float>::value && nearVal32)
@@ -717,7 +773,9 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         hipCheck(hipEventRecord(evJoin, nearStream), "event");
     }
     if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
+    probeBegin(MLFMA_GROUP_S2M, st);
     launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
+    probeEnd(MLFMA_GROUP_S2M, st);
     const bool nearSide = overlap == 2 || overlap == 3;
     auto forkSides = [&]() {
         if (!nearSide && !splitM2L) return;
@@ -748,6 +806,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     for (int l = ML - 1; l >= 2; --l) {
         LevelBufs& P = levels[l];
         LevelBufs& C = levels[l + 1];
+        probeBegin(MLFMA_GROUP_M2M, st);
         if (!launchM2MFused<R>(P.nstem, P.stemSlot, P.mPtr, P.mChild, P.mOct, P.nth, P.nph, C.nth, C.nph, p, P.thNear,
                                P.thW, P.phNear, P.phW, C.phUp<R>(), M + C.Moff, M + P.Moff, ldsMax, st, nb, fs,
                                minBlocksAt(l))) {
@@ -758,6 +817,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
                                 M + r * fs + P.Moff, st);
             }
         }
+        probeEnd(MLFMA_GROUP_M2M, st);
     }
     if (overlap == 3) forkSides();
     if (timed) hipCheck(hipEventRecord(ev[2], st), "event");
@@ -776,6 +836,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];
         if (l == ML) joinSide();
+        probeBegin(MLFMA_GROUP_L2L, st);
         if (!launchL2LFused<R>(false, C.nl2l, C.l2lParent, C.l2lChild, C.l2lOct, P.nth, P.nph, C.nth, C.nph, P.atPhPtr,
                                P.atPhSrc, P.atPhW, P.nAtPh, P.atThPtr, P.atThSrc, P.atThW, P.atThFlip, P.nAtTh,
                                P.phDn<R>(), Lc + P.Moff, Lc + C.Moff, ldsMax, st, nb, fs)) {
@@ -786,6 +847,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
                                   inner, Lc + r * fs + C.Moff, st);
             }
         }
+        probeEnd(MLFMA_GROUP_L2L, st);
     }
     joinSide();
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
@@ -793,8 +855,10 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
+    probeBegin(MLFMA_GROUP_L2T, st);
     launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, Lc, Ck, nearSumPtr, nearSumIdx, nearPart, perm, y,
                   accumulate, st, nb, fs, xs, ps);
+    probeEnd(MLFMA_GROUP_L2T, st);
     if (timed) hipCheck(hipEventRecord(ev[7], st), "event");
 }
 
@@ -917,7 +981,7 @@ void DevicePlan::matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int ac
     // The capture runs on the plan's own stream (torch's default stream cannot
     // be captured) and the graph is launched on st.  The graph holds kernel
     // launches only (no memset or host work), its buffers are sized first.
-    if ((flags & MLFMA_FLAG_GRAPH) && !timed && linear) {
+    if ((flags & MLFMA_FLAG_GRAPH) && !timed && linear && probeGroup == 0) {
         for (size_t i = 0; i < graphs.size(); ++i) {
             GraphEntry& g = graphs[i];
             if (g.x == x && g.y == y && g.nrhs == nrhs && g.accumulate == accumulate && g.fp32 == fp32) {
@@ -1404,6 +1468,13 @@ int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value) {
             if (value != 0 && value != 1) throw Error(MLFMA_ERR_ARG, "solve order must be 0 or 1");
             d.solveSerial = value == 1;
             break;
+        case MLFMA_PARAM_PROBE_GROUP:
+            if (value < 0 || value > MLFMA_GROUP_LAST) throw Error(MLFMA_ERR_ARG, "unknown launch group");
+            d.probeDrain(0);
+            d.probeGroup = value;
+            d.probeMs = 0;
+            d.probeLaunches = 0;
+            break;
         default:
             throw Error(MLFMA_ERR_ARG, "unknown parameter");
     }
@@ -1421,8 +1492,24 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value) 
         case MLFMA_PARAM_NEAR_OVERLAP: *value = d.nearOverlap; break;
         case MLFMA_PARAM_M2L_SPLIT: *value = d.m2lSplit ? 1 : 0; break;
         case MLFMA_PARAM_SOLVE_IN_ORDER: *value = d.solveSerial ? 1 : 0; break;
+        case MLFMA_PARAM_PROBE_GROUP: *value = d.probeGroup; break;
         default: throw Error(MLFMA_ERR_ARG, "unknown parameter");
     }
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_probe_read(mlfma_plan* plan, double* ms, int64_t* launches) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    needDevice(d);
+    if (!ms || !launches) throw Error(MLFMA_ERR_ARG, "null argument");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    d.probeDrain(0);
+    *ms = d.probeMs;
+    *launches = d.probeLaunches;
+    d.probeMs = 0;
+    d.probeLaunches = 0;
     return MLFMA_OK;
     API_END
 }

@@ -131,6 +131,21 @@ struct DevicePlan {
     cplx<double>* farPart = nullptr;  // far-field partial sums (grown on demand)
     size_t farPartLen = 0;
 
+    // launch-group probe (MLFMA_PARAM_PROBE_GROUP): an event pair around every
+    // launch of one group, recorded on the stream that launch goes to, so
+    // that a group's device time is measured in the schedule being timed
+    int probeGroup = 0;
+    struct ProbePair {
+        hipEvent_t a = nullptr, b = nullptr;
+    };
+    std::vector<ProbePair> probeRing;
+    size_t probeHead = 0, probePending = 0;
+    double probeMs = 0;
+    int64_t probeLaunches = 0;
+    void probeBegin(int group, hipStream_t s);
+    void probeEnd(int group, hipStream_t s);
+    void probeDrain(size_t keep);
+
     ~DevicePlan();
     void* alloc(size_t bytes);
     void release(void* p);

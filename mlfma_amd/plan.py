@@ -146,6 +146,19 @@ class Plan:
         (MLFMA_PARAM_*)."""
         _check(self.lib, self.lib.mlfma_plan_set_param(self._h, int(param), int(value)))
 
+    def probe(self, group: str | int | None) -> None:
+        """Bracket every launch of one launch group (a name of _capi.GROUPS or
+        its number; None: no probe) with HIP events on the stream it runs
+        on."""
+        g = 0 if group is None else (A.GROUPS[group] if isinstance(group, str) else int(group))
+        self.set_param(A.MLFMA_PARAM_PROBE_GROUP, g)
+
+    def probe_read(self):
+        """(milliseconds, launches) of the probed group since the last read."""
+        ms, cnt = C.c_double(0), C.c_int64(0)
+        _check(self.lib, self.lib.mlfma_probe_read(self._h, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
     def get_param(self, param: int) -> int:
         v = C.c_int32(0)
         _check(self.lib, self.lib.mlfma_plan_get_param(self._h, int(param), C.byref(v)))

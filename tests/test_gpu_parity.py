@@ -340,12 +340,18 @@ def test_one_stream_near_in_m2l_launch(fp32, m2l):
     y0 = p.matvec(x).cpu().numpy()
     z0 = p.matvec_adjoint(x).cpu().numpy()
     p.set_flags(flags | A.MLFMA_FLAG_ONE_STREAM)
-    for rep in range(2):
-        y1 = p.matvec(x).cpu().numpy()
-        z1 = p.matvec_adjoint(x).cpu().numpy()
-        say("fp32", fp32, "m2l flag", m2l, "rep", rep, cases.relerr(y1, y0), cases.relerr(z1, z0))
-        assert cases.relerr(y1, y0) == 0.0
-        assert cases.relerr(z1, z0) == 0.0
+    # every spread of the pair blocks over the shared launch (the autotuner's
+    # candidates and a narrow one) gives the same bits
+    for window in (100, 80, 65, 50, 20):
+        p.set_param(A.MLFMA_PARAM_NEAR_WINDOW, window)
+        assert p.get_param(A.MLFMA_PARAM_NEAR_WINDOW) == window
+        for rep in range(2):
+            y1 = p.matvec(x).cpu().numpy()
+            z1 = p.matvec_adjoint(x).cpu().numpy()
+            say("fp32", fp32, "m2l flag", m2l, "window", window, "rep", rep, cases.relerr(y1, y0),
+                cases.relerr(z1, z0))
+            assert cases.relerr(y1, y0) == 0.0
+            assert cases.relerr(z1, z0) == 0.0
 
 
 def test_graph_replay_matches_eager():
