@@ -191,12 +191,14 @@ def test_c4_concurrent_batched_single(c4):
                                                                   **s["kw"]))
         rhs.append(np.stack([-p.excitation(CF.planewave_xz(t)) for t in CF.c4_angles()]))
         plans.append(p)
-    # batched 8-right-hand-side matvecs equal one at a time
+    # batched 8-right-hand-side matvecs equal one at a time: FP32 far field
+    # (the batched kernels sum in another order), so to FP32 round-off
     for i in (0, 5, 12):
         Y = plans[i].matvec(rhs[i])
         for r in (0, 4, 7):
             e = cases.relerr(Y[r], plans[i].matvec(rhs[i][r]))
-            assert e <= 1e-13, (i, r, e)
+            say("C4 scatterer", i, "batch lane", r, "vs single (FP32 far field)", e)
+            assert e <= 1e-6, (i, r, e)
     # a bounded number of iterations of the mixed-precision batched solve,
     # all scatterers concurrently and one after another: bitwise equal
     opts = dict(restart=20, maxit=40, tol=1e-6, method=A.MLFMA_FGMRES_IR)
