@@ -404,6 +404,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES solves")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--timed-only", action="store_true",
+                    help="only the timed matvecs and their probe passes (one schedule; for kernel profiles)")
     ap.add_argument("--mode", default="matvec", choices=["matvec", "copy"],
                     help="copy: only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -479,6 +481,11 @@ def main():
                          "avg_launch_ms": round(ms / cnt, 5), "ms_per_matvec": round(ms / args.steps, 5)}
     plan.probe(None)
     log("launch groups", groups)
+    if args.timed_only:
+        print(json.dumps({"metric": "MLFMA matvec/s (FP64, one MI355X)", "value": round(value, 3),
+                          "unit": "matvec/s", "ms_per_step": round(ms_per_step, 4), "steps": args.steps,
+                          "config": desc, "schedule": schedule, "kernels": groups}), flush=True)
+        return
     # other schedules of the same matvec
     rates = {}
     for name, fl in [("one_stream", A.MLFMA_FLAG_ASYNC | A.MLFMA_FLAG_ONE_STREAM),
