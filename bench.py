@@ -404,6 +404,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES solves")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--tuning", default=None,
+                    help="JSON file of launch parameters: written after autotuning when absent, applied instead "
+                         "of autotuning when present (profiled runs then launch only the timed schedule)")
     ap.add_argument("--timed-only", action="store_true",
                     help="only the timed matvecs and their probe passes (one schedule; for kernel profiles)")
     ap.add_argument("--mode", default="matvec", choices=["matvec", "copy"],
@@ -448,11 +451,19 @@ def main():
     x = torch.from_numpy(G.complex_gaussian(n, 0)).to(dev)
     y = torch.empty_like(x)
     # the library's autotuner picks launch shapes and the stage schedule
-    tune_before, tune_after = plan.autotune(1)
+    if args.tuning and os.path.exists(args.tuning):
+        with open(args.tuning) as f:
+            plan.apply_tuning(json.load(f)["record"])
+        tune_before = tune_after = None
+    else:
+        tune_before, tune_after = plan.autotune(1)
+        if args.tuning:
+            with open(args.tuning, "w") as f:
+                json.dump({"workload": desc["workload"], "record": plan.tuning()}, f)
     schedule = {"near_overlap": plan.get_param(A.MLFMA_PARAM_NEAR_OVERLAP),
                 "m2l_split": plan.get_param(A.MLFMA_PARAM_M2L_SPLIT),
                 "near_window": plan.get_param(A.MLFMA_PARAM_NEAR_WINDOW)}
-    log("autotune ms", round(tune_before, 4), "->", round(tune_after, 4), schedule)
+    log("autotune ms", tune_before, "->", tune_after, schedule)
     plan.set_flags(A.MLFMA_FLAG_ASYNC)
     for _ in range(args.warmup):
         plan.matvec(x, y)

@@ -348,6 +348,13 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value);
 #define MLFMA_GROUP_LAST 8
 int mlfma_probe_read(mlfma_plan* plan, double* ms, int64_t* launches);
 
+/* The launch parameters mlfma_autotune chose, as a record of int32 values
+ * (*len of them; at most cap written), and their application to a plan of
+ * the same geometry and configuration (another process, no timing runs).  No
+ * reference counterpart. */
+int mlfma_autotune_record(const mlfma_plan* plan, int32_t* rec, int32_t cap, int32_t* len);
+int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len);
+
 /* Replace the plan's MLFMA_FLAG_* set for later matvecs and solves (the flags
  * act at apply time; the plan itself does not change).  No reference
  * counterpart: the reference fixes them at compile time. */

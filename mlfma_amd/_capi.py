@@ -113,6 +113,8 @@ SIGNATURES = {
     "mlfma_plan_set_param": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "mlfma_plan_get_param": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_probe_read": (C.c_int, [C.c_void_p, dptr, C.POINTER(C.c_int64)]),
+    "mlfma_autotune_record": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]),
+    "mlfma_autotune_apply": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]),
     "mlfma_autotune": (C.c_int, [C.c_void_p, C.c_int32, dptr, dptr]),
     "mlfma_stage_times_detail": (C.c_int, [C.c_void_p, dptr, C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_level_boxes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), iptr, iptr, iptr,

@@ -1499,6 +1499,47 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value) 
     API_END
 }
 
+// Tuning record: [version, nearOverlap, m2lSplit, nearWindow, solveSerial,
+// levels L, tuneMinBlocks[0..L), tuneM2L[0..L)]
+int mlfma_autotune_record(const mlfma_plan* plan, int32_t* rec, int32_t cap, int32_t* len) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    if (!len) throw Error(MLFMA_ERR_ARG, "null argument");
+    const int L = d.hp.tree.maxLevel + 1;
+    std::vector<int32_t> r{1, d.nearOverlap, d.m2lSplit ? 1 : 0, d.nearWindow, d.solveSerial ? 1 : 0, L};
+    for (int l = 0; l < L; ++l) r.push_back(l < (int)d.tuneMinBlocks.size() ? d.tuneMinBlocks[l] : 0);
+    for (int l = 0; l < L; ++l) r.push_back(l < (int)d.tuneM2L.size() ? d.tuneM2L[l] : 0);
+    *len = (int32_t)r.size();
+    if (rec)
+        for (int32_t i = 0; i < std::min<int32_t>(cap, *len); ++i) rec[i] = r[i];
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    needDevice(d);
+    const int L = d.hp.tree.maxLevel + 1;
+    if (!rec || len != 6 + 2 * L || rec[0] != 1 || rec[5] != L)
+        throw Error(MLFMA_ERR_ARG, "tuning record does not match this plan");
+    if (rec[1] < 0 || rec[1] > 3 || rec[2] < 0 || rec[2] > 1 || rec[3] < 1 || rec[3] > 100 || rec[4] < 0 ||
+        rec[4] > 1)
+        throw Error(MLFMA_ERR_ARG, "tuning record out of range");
+    for (int l = 0; l < L; ++l)
+        if (rec[6 + l] < 0 || rec[6 + L + l] < 0 || rec[6 + L + l] > 2)
+            throw Error(MLFMA_ERR_ARG, "tuning record out of range");
+    d.nearOverlap = rec[1];
+    d.m2lSplit = rec[2] == 1;
+    d.nearWindow = rec[3];
+    d.solveSerial = rec[4] == 1;
+    d.tuneMinBlocks.assign(rec + 6, rec + 6 + L);
+    d.tuneM2L.assign(rec + 6 + L, rec + 6 + 2 * L);
+    d.clearGraphs();
+    return MLFMA_OK;
+    API_END
+}
+
 int mlfma_probe_read(mlfma_plan* plan, double* ms, int64_t* launches) {
     API_BEGIN
     DevicePlan& d = P(plan);

@@ -181,6 +181,19 @@ class Plan:
         _check(self.lib, self.lib.mlfma_autotune(self._h, nrhs, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def tuning(self) -> list:
+        """The launch parameters the autotuner chose (a list of ints)."""
+        n = C.c_int32(0)
+        _check(self.lib, self.lib.mlfma_autotune_record(self._h, None, 0, C.byref(n)))
+        buf = (C.c_int32 * n.value)()
+        _check(self.lib, self.lib.mlfma_autotune_record(self._h, buf, n.value, C.byref(n)))
+        return list(buf)
+
+    def apply_tuning(self, rec) -> None:
+        """Apply a record from tuning() of a plan of the same workload."""
+        buf = (C.c_int32 * len(rec))(*[int(v) for v in rec])
+        _check(self.lib, self.lib.mlfma_autotune_apply(self._h, buf, len(rec)))
+
     def stage_times(self) -> dict:
         t = np.zeros(7)
         _check(self.lib, self.lib.mlfma_stage_times(self._h, t.ctypes.data_as(A.dptr)))
