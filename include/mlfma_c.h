@@ -260,6 +260,21 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
 int mlfma_farfield(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E,
                    int32_t ptr_mode, void* stream);
 
+/* mlfma_farfield on device pointers that allocates nothing: no reference
+ * counterpart; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  I (n
+ * complex, input order), dirs (ndir x 3) and E (ndir x 3 complex) are on the
+ * plan's device; the work is queued on stream (NULL: the plan's stream) and
+ * the call returns without synchronising.  The source-range partial sums go
+ * to one fixed array per device (3 MiB, set up by mlfma_plan_create, held
+ * until the process ends), so ndir is limited to 65536 (MLFMA_ERR_ARG
+ * beyond; mlfma_farfield has no limit).  Every call on a device shares that
+ * device's array: calls that overlap on one device (other streams, other
+ * threads, other plans) race; order them on one stream.  Same kernels and
+ * results as mlfma_farfield. */
+int mlfma_farfield_noalloc(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E,
+                           void* stream);
+
 /* The adjoint of mlfma_farfield: I = F^H E for ndir x 3 complex fields E at
  * the unit directions dirs; I: n complex (input order); ptr_mode and stream
  * as in mlfma_farfield.  No reference counterpart: the gradient of a

@@ -107,6 +107,7 @@ SIGNATURES = {
                                  C.c_void_p]),
     "mlfma_farfield_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
                                          C.c_void_p]),
+    "mlfma_farfield_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "mlfma_position_gradient": (C.c_int, [C.c_void_p, dptr, dptr, dptr]),
     "mlfma_stage_times": (C.c_int, [C.c_void_p, dptr]),
     "mlfma_plan_set_flags": (C.c_int, [C.c_void_p, C.c_uint32]),

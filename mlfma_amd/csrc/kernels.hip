@@ -3,6 +3,8 @@
 // roofline is HBM (or L2 for the M2L source reads), see the launchers for the
 // per-item work each block does.
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "kernels.hpp"
 #include "cplx_ops.hpp"
@@ -580,6 +582,39 @@ int farfieldChunks(int64_t n, int64_t ndir) {
     nch = std::max<int64_t>(nch, 1);
     const int64_t chunk = (n + nch - 1) / nch;
     return (int)((n + chunk - 1) / chunk);
+}
+
+// Far-field partial sums of mlfma_farfield_noalloc (callers that cannot
+// allocate device memory): one array per device, sized for the most partial
+// triples farfieldChunks uses (kFarPartCap x 3 complex = 3 MiB), held until the
+// process ends.  Host code never names the array in a launch or copy: it uses
+// the address hipGetSymbolAddress gives on each device.
+static __device__ __align__(256) cplx<double> scratch[kFarPartCap * 3];
+
+namespace {
+std::mutex g_scratchMutex;
+std::vector<cplx<double>*> g_scratchPtr;  // by device ordinal
+}  // namespace
+
+hipError_t initFarfieldScratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (device < 0) return hipErrorInvalidDevice;
+    if ((size_t)device >= g_scratchPtr.size()) g_scratchPtr.resize((size_t)device + 1, nullptr);
+    if (g_scratchPtr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the caller makes the device current
+    cplx<double>* ptr = nullptr;
+    e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratchPtr[device] = ptr;
+    return hipSuccess;
+}
+
+cplx<double>* farfieldScratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    return device >= 0 && (size_t)device < g_scratchPtr.size() ? g_scratchPtr[device] : nullptr;
 }
 
 void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const int64_t* perm, const double* dirs,

@@ -96,6 +96,11 @@ constexpr int64_t kFarPartCap = 65536;  // partial triples the far-field chunkin
 int farfieldChunks(int64_t n, int64_t ndir);
 void launchFarfield(const SrcDev& s, int64_t n, const cplx<double>* I, const int64_t* perm, const double* dirs,
                     int64_t ndir, double k, cplx<double> Ck, cplx<double>* part, cplx<double>* E, hipStream_t st);
+// the per-device address of the fixed far-field partial-sum array (kFarPartCap
+// x 3 complex): initFarfieldScratch(device) once with that device current
+// (plan creation), then farfieldScratch(device) (nullptr before)
+hipError_t initFarfieldScratch(int device);
+cplx<double>* farfieldScratch(int device);
 void launchFarfieldAdj(const SrcDev& s, int64_t n, const cplx<double>* E, const double* dirs, int ndir, double k,
                        cplx<double> Ckc, const int64_t* perm, cplx<double>* I, hipStream_t st);
 // dipole position gradient, near part: grad (n x 3, input order) += -2 Re(lambda^H dZ_near I)

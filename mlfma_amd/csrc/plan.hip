@@ -145,6 +145,8 @@ static std::vector<cplx<R>> toR(const std::vector<cplx<double>>& v) {
 void DevicePlan::build() {
     HostPlan& h = hp;
     hipCheck(hipSetDevice(device), "hipSetDevice");
+    // this device's address of the fixed far-field partial-sum array (once per device)
+    hipCheck(initFarfieldScratch(device), "hipGetSymbolAddress (far-field partial sums)");
     hipCheck(hipStreamCreate(&stream), "hipStreamCreate");
     {
         int v = 0;

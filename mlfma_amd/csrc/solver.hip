@@ -8,6 +8,7 @@
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "kernels_mv.hpp"
@@ -1342,6 +1343,32 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     for (auto& e : o)
         if (e.status != MLFMA_OK) worst = e.status;
     return worst == MLFMA_ERR_NOT_CONVERGED ? MLFMA_OK : worst;
+    API_END
+}
+
+// The far field of mlfma_farfield on device pointers without allocating:
+// the source-range partial sums go to this device's fixed array (kFarPartCap
+// triples), the same kernels and chunking as mlfma_farfield.  Nothing is
+// allocated, freed or synchronised: the work is queued on the stream.
+int mlfma_farfield_noalloc(mlfma_plan* plan, const double* I, const double* dirs, int64_t ndir, double* E,
+                           void* stream) {
+    API_BEGIN
+    DevicePlan& d = Pl(plan);
+    if (!I || !dirs || !E || ndir < 0) throw Error(MLFMA_ERR_ARG, "bad farfield arguments");
+    if (ndir > kFarPartCap)
+        throw Error(MLFMA_ERR_ARG, "mlfma_farfield_noalloc takes at most " + std::to_string(kFarPartCap) +
+                                       " directions (its partial-sum array is fixed); use mlfma_farfield");
+    if (ndir == 0) return MLFMA_OK;
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    cplx<double>* part = farfieldScratch(d.device);
+    if (!part) throw Error(MLFMA_ERR_STATE, "the far-field partial-sum array of this device is not set up");
+    const int64_t n = d.hp.src.n;
+    if ((int64_t)farfieldChunks(n, ndir) * ndir > kFarPartCap)
+        throw Error(MLFMA_ERR_STATE, "far-field chunking exceeds the partial-sum array");
+    hipStream_t st = stream ? (hipStream_t)stream : d.stream;
+    launchFarfield(d.src, n, (const cplx<double>*)I, d.perm, dirs, ndir, d.hp.k, d.Ck, part, (cplx<double>*)E, st);
+    hipCheck(hipGetLastError(), "farfield launch");
+    return MLFMA_OK;
     API_END
 }
 
