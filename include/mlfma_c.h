@@ -204,7 +204,15 @@ int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t pt
 #define MLFMA_GMRES_COMPAT 0   /* src/solver.cpp: unrestarted, complex Givens without
                                   conjugates, stops on |g_k+1|/g0                  */
 #define MLFMA_GMRES_RESTART 1  /* GMRES(m), unitary Givens, true residual at restart */
-#define MLFMA_FGMRES_IR 2      /* FP64 flexible GMRES outer, FP32 far-field inner   */
+#define MLFMA_FGMRES_IR 2      /* mixed precision: FP64 flexible GMRES(outer_restart)
+                                  whose variable preconditioner is an inner GMRES
+                                  with the FP32 far field (at most `restart`
+                                  iterations or a relative inner_tol per step);
+                                  FP64 true residual at every outer restart     */
+#define MLFMA_GMRES_IR 3       /* mixed precision by iterative refinement: FP64
+                                  residual, restarted FP32-far-field GMRES(restart)
+                                  corrections (no outer Krylov space), at most 20
+                                  (30 batched) corrections                      */
 #define MLFMA_SOLVE_ADJOINT 0x100  /* or-ed into method: solve Z^H x = b instead
                                       (the adjoint solve of a gradient; no
                                       reference counterpart)                       */
@@ -216,9 +224,13 @@ int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t pt
 typedef struct mlfma_solve_opts {
     int32_t method;       /* MLFMA_GMRES_*                                          */
     int32_t restart;      /* m for GMRES(m) / inner iterations for FGMRES; <= 2048 */
-    int32_t maxit;        /* total iteration cap (reference: 500)                   */
+    int32_t maxit;        /* total iteration (matvec) cap (reference: 500)          */
     int32_t use_voltage;  /* 1: b = -V (reference), 0: b from the argument          */
     double tol;           /* reference: 1e-6                                        */
+    int32_t outer_restart;/* MLFMA_FGMRES_IR: outer steps per cycle (0: 30)         */
+    int32_t reserved;
+    double inner_tol;     /* MLFMA_FGMRES_IR: relative tolerance of each inner
+                             solve (0: 1e-1)                                        */
 } mlfma_solve_opts;
 
 typedef struct mlfma_solve_report {

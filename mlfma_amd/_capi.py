@@ -30,6 +30,7 @@ MLFMA_PTR_DEVICE = 1
 MLFMA_GMRES_COMPAT = 0
 MLFMA_GMRES_RESTART = 1
 MLFMA_FGMRES_IR = 2
+MLFMA_GMRES_IR = 3
 MLFMA_SOLVE_ADJOINT = 0x100
 MLFMA_SOLVE_DEVICE = 0x200
 MLFMA_PARAM_NEAR_WINDOW = 1
@@ -79,7 +80,8 @@ class Info(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("restart", C.c_int32), ("maxit", C.c_int32),
-                ("use_voltage", C.c_int32), ("tol", C.c_double)]
+                ("use_voltage", C.c_int32), ("tol", C.c_double), ("outer_restart", C.c_int32),
+                ("reserved", C.c_int32), ("inner_tol", C.c_double)]
 
 
 class SolveReport(C.Structure):

@@ -17,7 +17,7 @@ from .plan import Plan
 
 
 def solve_many(plans: list[Plan], rhs: list[np.ndarray], restart: int = 50, maxit: int = 500, tol: float = 1e-6,
-               workers: int | None = None, method: int | None = None):
+               workers: int | None = None, method: int | None = None, **solve_kw):
     """Solve plans[i] X_i = rhs[i] (rhs[i]: (nrhs_i, n_i)) for all i, the
     scatterers concurrently.  Returns [(X_i, reports_i)] in input order."""
     if len(plans) != len(rhs):
@@ -31,7 +31,9 @@ def solve_many(plans: list[Plan], rhs: list[np.ndarray], restart: int = 50, maxi
         for p, f in zip(plans, saved):
             p.set_flags(f | A.MLFMA_FLAG_ONE_STREAM)
         with ThreadPoolExecutor(max_workers=workers) as ex:
-            kw = {} if method is None else {"method": method}
+            kw = dict(solve_kw)
+            if method is not None:
+                kw["method"] = method
             futs = [ex.submit(p.solve_batch, np.asarray(b), restart, maxit, tol, False, **kw)
                     for p, b in zip(plans, rhs)]
             return [f.result() for f in futs]

@@ -4,7 +4,7 @@
 // file), builds the plan on the GPU, solves, and writes out/sol/curr_nq7.txt
 // (one "re im" line per unknown at %.15e, src/solver.cpp:149-165).
 //
-// usage: mlfma [--root DIR] [--device D] [--method compat|gmres|ir] [--restart M]
+// usage: mlfma [--root DIR] [--device D] [--method compat|gmres|fgmres|ir] [--restart M]
 //              [--maxit K] [--tol T] [--max-level L] [--dipoles FILE] [--fp32]
 //        mlfma --root DIR --print-config   (print the parsed config.txt and exit;
 //                                           no device is touched)
@@ -218,7 +218,10 @@ int main(int argc, char** argv) {
                       << std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count()
                       << " ms\n\n Solving...\n";
             mlfma_solve_opts o{};
-            o.method = method == "compat" ? MLFMA_GMRES_COMPAT : (method == "ir" ? MLFMA_FGMRES_IR : MLFMA_GMRES_RESTART);
+            o.method = method == "compat"   ? MLFMA_GMRES_COMPAT
+                       : method == "fgmres" ? MLFMA_FGMRES_IR
+                       : method == "ir"     ? MLFMA_GMRES_IR
+                                            : MLFMA_GMRES_RESTART;
             o.restart = restart;
             o.maxit = maxit;
             o.use_voltage = 1;

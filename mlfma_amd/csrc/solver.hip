@@ -655,8 +655,10 @@ struct GivensWork {
     }
 };
 
+// lastResidual = false (the inner solves of FGMRES): when maxit is reached the
+// solve ends without the residual of the final iterate (one matvec fewer)
 static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit,
-                             double tol, bool fp32, bool x0zero, double* hist, int histOff) {
+                             double tol, bool fp32, bool x0zero, double* hist, int histOff, bool lastResidual = true) {
     SolveOut out;
     const int64_t n = d.hp.src.n;
     m = std::max(1, std::min(m, maxit));
@@ -675,7 +677,9 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
     if (x0zero) hipCheck(hipMemsetAsync(x, 0, n * sizeof(cplx<double>), K.st), "zero x");
     int total = 0;
     double res = 1.0;
+    bool cycleMet = false;  // the last cycle stopped on its residual estimate
     for (;;) {
+        if (!lastResidual && (total >= maxit || cycleMet)) break;
         // r = b - A x
         double beta;
         if (x0zero && total == 0) {
@@ -745,6 +749,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
             ++jc;
             if (est <= tol || hn == 0) {
                 ncolUsed = jc;
+                cycleMet = true;
                 break;
             }
             if (total >= maxit) {
@@ -772,7 +777,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         if (out.status != MLFMA_OK) break;
     }
     out.iters = total;
-    if (out.status == MLFMA_OK && res > tol) out.status = MLFMA_ERR_NOT_CONVERGED;
+    if (out.status == MLFMA_OK && out.est > tol) out.status = MLFMA_ERR_NOT_CONVERGED;
     d.release(Q);
     d.release(w);
     return out;
@@ -870,7 +875,7 @@ struct GivensBatch {
 static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                         const cplx<double>* B, cplx<double>* X, int nrhs, int m, int maxit,
                                         const std::vector<double>& tolv, double* hist, bool fp32 = false,
-                                        int histOff = 0, int histLd = 0) {
+                                        int histOff = 0, int histLd = 0, bool lastResidual = true) {
     if (!histLd) histLd = maxit;
     const int64_t n = d.hp.src.n;
     m = std::max(1, std::min(m, maxit));
@@ -889,7 +894,9 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
     }
     hipCheck(hipMemsetAsync(X, 0, (size_t)ld * sizeof(cplx<double>), st), "zero x");
     int total = 0;
+    bool cycleMet = false;  // every lane of the last cycle stopped on its estimate
     for (;;) {
+        if (!lastResidual && (total >= maxit || cycleMet)) break;
         // residuals r = b - A x
         if (total == 0)
             hipCheck(hipMemcpyAsync(W, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
@@ -967,6 +974,7 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
             }
             ++total;
             ++jc;
+            if (!any) cycleMet = true;
             if (!any || total >= maxit) stop = true;
             if (stop) break;  // steps queued past the stop are not read
         }
@@ -1001,9 +1009,10 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
     return out;
 }
 
-// Batched mixed precision: FP64 residuals of all right-hand sides (batched
-// FP64 matvec) around batched inner GMRES(m) solves with the FP32 far field;
-// each right-hand side's inner tolerance is max(1e-4, tol / (2 residual)).
+// Batched iterative refinement (MLFMA_GMRES_IR): FP64 residuals of all
+// right-hand sides (batched FP64 matvec) around batched inner GMRES(m) solves
+// with the FP32 far field, at most 30 corrections; each right-hand side's
+// inner tolerance is max(1e-4, tol / (2 residual)).
 static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                           const cplx<double>* B, cplx<double>* X, int nrhs, int m, int maxit,
                                           double tol, double* hist) {
@@ -1016,7 +1025,8 @@ static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique
     for (int r = 0; r < nrhs; ++r) bn[r] = K[r]->norm(B + (size_t)r * n);
     hipCheck(hipMemsetAsync(X, 0, (size_t)ld * sizeof(cplx<double>), st), "zero x");
     int total = 0;
-    for (int outer = 0; outer < 30; ++outer) {
+    for (int outer = 0;; ++outer) {
+        // the residuals of the current iterates, also after the last correction
         if (outer == 0)
             hipCheck(hipMemcpyAsync(R, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
         else {
@@ -1035,7 +1045,7 @@ static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique
             all = false;
             itol[r] = std::max(1e-4, 0.5 * tol / res);
         }
-        if (all || total >= maxit) break;
+        if (all || total >= maxit || outer == 30) break;
         std::vector<SolveOut> in = gmresBatch(d, K, st, R, D, nrhs, m, maxit - total, itol, hist, true, total, maxit);
         int steps = 0;
         for (int r = 0; r < nrhs; ++r) {
@@ -1055,8 +1065,182 @@ static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique
     return out;
 }
 
-// Mixed precision: FP64 iterative refinement (true residual with the FP64
-// matvec) around inner GMRES(m) solves whose far field is FP32.
+// Flexible GMRES (Saad 1993) in FP64 whose preconditioner is an inner GMRES
+// solve with the FP32 far field: for each lane (right-hand side) the outer
+// step j applies z_j = M_j^-1 v_j (inner GMRES(mIn) from zero to a relative
+// innerTol, at most mIn iterations; the map changes from step to step, which
+// is why the outer method stores the z_j), w = Z z_j with the FP64 matvec,
+// classical Gram-Schmidt twice against v_0..v_j, and a unitary Givens
+// rotation; x = x_0 + sum_j y_j z_j at the end of each cycle of mOut steps.
+// Every cycle starts from the true FP64 residual, and the solve ends on it.
+// One right-hand side runs the pipelined single-vector GMRES as the inner
+// solver; several run the batched one (one batched matvec per inner step)
+// with one batched FP64 matvec and one batched CGS2 per outer step.
+// iterations counts matvecs: the inner FP32 ones and the outer FP64 ones.
+static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
+                                    const cplx<double>* B, cplx<double>* X, int nrhs, int mOut, int mIn, int maxit,
+                                    double tol, double innerTol, double* hist, int histLd) {
+    const int64_t n = d.hp.src.n;
+    const int64_t ld = (int64_t)nrhs * n;
+    mOut = std::max(1, mOut);
+    mIn = std::max(1, mIn);
+    cplx<double>* V = (cplx<double>*)d.alloc((size_t)ld * (mOut + 1) * sizeof(cplx<double>));
+    cplx<double>* Zb = (cplx<double>*)d.alloc((size_t)ld * mOut * sizeof(cplx<double>));
+    cplx<double>* W = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
+    std::unique_ptr<BatchOrtho> BO;
+    if (nrhs > 1) BO.reset(new BatchOrtho(d, nrhs, mOut));
+    std::vector<cplx<double>> hh(nrhs > 1 ? (size_t)nrhs * BO->hs : 0);
+    std::vector<SolveOut> out(nrhs);
+    std::vector<double> bn(nrhs);
+    std::vector<int> used(nrhs, 0);  // iterations (matvecs) per lane
+    for (int r = 0; r < nrhs; ++r) {
+        bn[r] = K[r]->norm(B + (size_t)r * n);
+        if (!std::isfinite(bn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
+    }
+    hipCheck(hipMemsetAsync(X, 0, (size_t)ld * sizeof(cplx<double>), st), "zero x");
+    // the FP64 operator (one right-hand side: Z or, for an adjoint solve, Z^H)
+    auto op64 = [&](const cplx<double>* x, cplx<double>* y) {
+        if (nrhs == 1)
+            d.applySolveOp(x, y, 0, st, false, false);
+        else
+            applyMany(d, x, y, nrhs, st, false);
+    };
+    int total = 0;  // matvecs of the lockstep (the most any lane has used)
+    for (int cycle = 0;; ++cycle) {
+        // true FP64 residuals R = B - Z X (into W)
+        if (cycle == 0)
+            hipCheck(hipMemcpyAsync(W, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
+        else {
+            op64(X, W);
+            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + (size_t)r * n, W + (size_t)r * n, W + (size_t)r * n);
+        }
+        std::vector<double> beta(nrhs, 0.0);
+        std::vector<char> active(nrhs, 0);
+        bool anyActive = false;
+        for (int r = 0; r < nrhs; ++r) {
+            if (bn[r] == 0 || out[r].status == MLFMA_ERR_BREAKDOWN) continue;
+            beta[r] = cycle == 0 ? bn[r] : K[r]->norm(W + (size_t)r * n);
+            out[r].est = beta[r] / bn[r];
+            active[r] = out[r].est > tol && beta[r] > 0;
+            anyActive = anyActive || active[r];
+        }
+        if (!anyActive || total >= maxit) break;
+        for (int r = 0; r < nrhs; ++r) {
+            if (active[r])
+                K[r]->scale(W + (size_t)r * n, beta[r], V + (size_t)r * n);
+            else
+                hipCheck(hipMemsetAsync(V + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+        }
+        // per lane: Hessenberg columns (rotated), rotations, rotated rhs g
+        std::vector<std::vector<std::vector<zc>>> H(nrhs);
+        std::vector<std::vector<zc>> cs(nrhs), sn(nrhs), g(nrhs);
+        std::vector<int> steps(nrhs, 0);
+        for (int r = 0; r < nrhs; ++r) g[r] = {zc(beta[r], 0.0)};
+        for (int j = 0; j < mOut && total < maxit; ++j) {
+            // z_j = M^-1 v_j: inner FP32-far-field GMRES on every lane at once
+            // (lanes that are done carry zero right-hand sides)
+            cplx<double>* Vj = V + (size_t)j * ld;
+            cplx<double>* Zj = Zb + (size_t)j * ld;
+            const int budget = std::max(1, std::min(mIn, maxit - total - 1));
+            int innerIts = 0;
+            if (nrhs == 1) {
+                SolveOut in = gmresRestart(d, *K[0], Vj, Zj, budget, budget, innerTol, true, true, hist, total, false);
+                innerIts = in.iters;
+                used[0] += in.iters;
+            } else {
+                std::vector<double> itol(nrhs, innerTol);
+                std::vector<SolveOut> in =
+                    gmresBatch(d, K, st, Vj, Zj, nrhs, budget, budget, itol, hist, true, total, histLd, false);
+                for (int r = 0; r < nrhs; ++r) {
+                    innerIts = std::max(innerIts, in[r].iters);
+                    if (active[r]) used[r] += in[r].iters;
+                }
+            }
+            total += innerIts;
+            // w = Z z_j (FP64) and CGS2 against v_0..v_j
+            op64(Zj, W);
+            ++total;
+            std::vector<std::vector<zc>> hcol(nrhs);
+            std::vector<double> wn(nrhs, 0.0);
+            if (nrhs == 1) {
+                K[0]->orthogonalize(V, j + 1, W, hcol[0], wn[0]);
+            } else {
+                BO->queue(V, ld, j + 1, W, hh.data(), st);
+                hipCheck(hipStreamSynchronize(st), "fgmres ortho");
+                for (int r = 0; r < nrhs; ++r) Krylov::readOrtho(hh.data() + (size_t)r * BO->hs, j + 1, hcol[r], wn[r]);
+            }
+            bool any = false;
+            for (int r = 0; r < nrhs; ++r) {
+                cplx<double>* Vn = V + (size_t)(j + 1) * ld + (size_t)r * n;
+                if (!active[r]) {
+                    hipCheck(hipMemsetAsync(Vn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                    continue;
+                }
+                ++used[r];
+                std::vector<zc>& h = hcol[r];
+                h.push_back(zc(wn[r], 0.0));
+                for (int i = 0; i < j; ++i) {  // previous rotations
+                    const zc a = cs[r][i] * h[i] + sn[r][i] * h[i + 1];
+                    h[i + 1] = -std::conj(sn[r][i]) * h[i] + cs[r][i] * h[i + 1];
+                    h[i] = a;
+                }
+                const double na = std::abs(h[j]), t = std::hypot(na, wn[r]);
+                zc c, s;
+                if (t == 0) {
+                    out[r].status = MLFMA_ERR_BREAKDOWN;
+                    active[r] = 0;
+                    hipCheck(hipMemsetAsync(Vn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                    continue;
+                }
+                if (na == 0) {
+                    c = 0.0;
+                    s = 1.0;
+                } else {
+                    c = na / t;
+                    s = (h[j] / na) * wn[r] / t;
+                }
+                cs[r].push_back(c);
+                sn[r].push_back(s);
+                h[j] = c * h[j] + s * h[j + 1];
+                h.pop_back();
+                H[r].push_back(h);
+                const zc gj = g[r][j];
+                g[r][j] = c * gj;
+                g[r].push_back(-std::conj(s) * gj);
+                steps[r] = j + 1;
+                const double est = std::abs(g[r][j + 1]) / bn[r];
+                out[r].est = est;
+                if (hist) hist[(size_t)r * histLd + std::min(total, std::max(histLd, 1)) - 1] = est;
+                if (est <= tol || wn[r] == 0) {
+                    active[r] = 0;
+                    hipCheck(hipMemsetAsync(Vn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                } else {
+                    K[r]->scale(W + (size_t)r * n, wn[r], Vn);
+                    any = true;
+                }
+            }
+            if (!any) break;
+        }
+        // x += Z y per lane
+        for (int r = 0; r < nrhs; ++r)
+            if (steps[r] > 0) {
+                const std::vector<zc> y = backSolve(H[r], g[r], steps[r]);
+                K[r]->combine(Zb + (size_t)r * n, steps[r], y, X + (size_t)r * n, 1, ld);
+            }
+    }
+    for (int r = 0; r < nrhs; ++r) {
+        out[r].iters = used[r];
+        if (out[r].status == MLFMA_OK && out[r].est > tol) out[r].status = MLFMA_ERR_NOT_CONVERGED;
+    }
+    d.release(V);
+    d.release(Zb);
+    d.release(W);
+    return out;
+}
+
+// Mixed precision by iterative refinement (MLFMA_GMRES_IR): FP64 residuals
+// (FP64 matvec) around restarted inner GMRES(m) solves whose far field is
+// FP32, at most 20 corrections; no outer Krylov space is kept.
 static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit, double tol,
                         double* hist) {
     SolveOut out;
@@ -1069,7 +1253,8 @@ static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<do
     int total = 0;
     double res = 1.0;
     const double inner = 1e-4;
-    for (int outer = 0; outer < 20; ++outer) {
+    for (int outer = 0;; ++outer) {
+        // the residual of the current iterate, also after the last correction
         if (outer == 0) {
             hipCheck(hipMemcpyAsync(r, b, n * sizeof(cplx<double>), hipMemcpyDeviceToDevice, K.st), "copy");
             res = 1.0;
@@ -1079,7 +1264,7 @@ static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<do
             res = K.norm(r) / bn;
         }
         out.est = res;
-        if (res <= tol || total >= maxit) break;
+        if (res <= tol || total >= maxit || outer == 20) break;
         const double itol = std::max(inner, 0.5 * tol / res);
         SolveOut s = gmresRestart(d, K, r, dx, m, maxit - total, itol, true, true, hist, total);
         total += s.iters;
@@ -1176,7 +1361,8 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     if (opts->method & ~(0xff | MLFMA_SOLVE_ADJOINT | MLFMA_SOLVE_DEVICE))
         throw Error(MLFMA_ERR_ARG, "unknown solve method bits");
     const bool devPtrs = (opts->method & MLFMA_SOLVE_DEVICE) != 0;
-    if (method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
+    if ((method == MLFMA_FGMRES_IR || method == MLFMA_GMRES_IR) && !d.useFp32)
+        throw Error(MLFMA_ERR_STATE, "mixed-precision solves need an FP32 plan");
     hipCheck(hipSetDevice(d.device), "hipSetDevice");
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t n = d.hp.src.n;
@@ -1208,14 +1394,20 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
         }
         hipCheck(hipMemcpyAsync(bt, bh.data(), n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "b upload");
     }
-    Krylov K(d, st, std::max(opts->maxit, opts->restart) + 2);
+    const int mOut = opts->outer_restart > 0 ? opts->outer_restart : 30;
+    Krylov K(d, st, std::max(std::max(opts->maxit, opts->restart), mOut) + 2);
     SolveOut o;
     if (method == MLFMA_GMRES_COMPAT)
         o = gmresCompat(d, K, bt, xtree, opts->maxit, opts->tol, hist);
     else if (method == MLFMA_GMRES_RESTART)
         o = gmresRestart(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, false, true,
                          hist, 0);
-    else if (method == MLFMA_FGMRES_IR)
+    else if (method == MLFMA_FGMRES_IR) {
+        std::vector<std::unique_ptr<Krylov>> Kv;
+        Kv.emplace_back(new Krylov(d, st, std::max(opts->restart > 0 ? opts->restart : 50, mOut) + 2));
+        o = fgmres(d, Kv, st, bt, xtree, 1, mOut, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol,
+                   opts->inner_tol > 0 ? opts->inner_tol : 1e-1, hist, opts->maxit)[0];
+    } else if (method == MLFMA_GMRES_IR)
         o = gmresIR(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, hist);
     else
         throw Error(MLFMA_ERR_ARG, "unknown solve method");
@@ -1293,9 +1485,10 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     API_BEGIN
     DevicePlan& d = Pl(plan);
     if (!b || !x || !opts || nrhs < 1) throw Error(MLFMA_ERR_ARG, "bad batch solve arguments");
-    if (opts->method != MLFMA_GMRES_RESTART && opts->method != MLFMA_FGMRES_IR)
-        throw Error(MLFMA_ERR_ARG, "batched solves use MLFMA_GMRES_RESTART or MLFMA_FGMRES_IR");
-    if (opts->method == MLFMA_FGMRES_IR && !d.useFp32) throw Error(MLFMA_ERR_STATE, "FGMRES_IR needs an FP32 plan");
+    if (opts->method != MLFMA_GMRES_RESTART && opts->method != MLFMA_FGMRES_IR && opts->method != MLFMA_GMRES_IR)
+        throw Error(MLFMA_ERR_ARG, "batched solves use MLFMA_GMRES_RESTART, MLFMA_FGMRES_IR or MLFMA_GMRES_IR");
+    if (opts->method != MLFMA_GMRES_RESTART && !d.useFp32)
+        throw Error(MLFMA_ERR_STATE, "mixed-precision solves need an FP32 plan");
     if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
     if (opts->restart > 2048) throw Error(MLFMA_ERR_ARG, "restart must be <= 2048");
     if (d.hp.staleBoxes > 0 && !(d.flags & MLFMA_FLAG_FRESH_LOCALS))
@@ -1314,9 +1507,13 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     hipCheck(hipMemcpyAsync(B, b, bytes, hipMemcpyHostToDevice, st), "b upload");
     const int m = opts->restart > 0 ? opts->restart : 50;
     std::vector<std::unique_ptr<Krylov>> K;
-    for (int r = 0; r < nrhs; ++r) K.emplace_back(new Krylov(d, st, std::min(m, opts->maxit) + 2));
+    const int mOut = opts->outer_restart > 0 ? opts->outer_restart : 30;
+    for (int r = 0; r < nrhs; ++r) K.emplace_back(new Krylov(d, st, std::max(std::min(m, opts->maxit), mOut) + 2));
     std::vector<SolveOut> o =
         opts->method == MLFMA_FGMRES_IR
+            ? fgmres(d, K, st, B, X, nrhs, mOut, m, opts->maxit, opts->tol,
+                     opts->inner_tol > 0 ? opts->inner_tol : 1e-1, hist, opts->maxit)
+        : opts->method == MLFMA_GMRES_IR
             ? gmresBatchIR(d, K, st, B, X, nrhs, m, opts->maxit, opts->tol, hist)
             : gmresBatch(d, K, st, B, X, nrhs, m, opts->maxit, std::vector<double>(nrhs, opts->tol), hist);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

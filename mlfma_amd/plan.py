@@ -208,15 +208,20 @@ class Plan:
 
     # ---------------------------------------------------------------- solve
     def solve(self, b=None, method=A.MLFMA_GMRES_COMPAT, restart=50, maxit=500, tol=1e-6, history=False,
-              adjoint=False):
+              adjoint=False, outer_restart=0, inner_tol=0.0):
         """GMRES for Z x = b (b = None: the plane-wave excitation -V), or for
-        Z^H x = b with adjoint=True.  Returns (x, report)."""
+        Z^H x = b with adjoint=True.  MLFMA_FGMRES_IR (FP32 plans): flexible
+        GMRES(outer_restart) in FP64 around inner FP32-far-field GMRES solves of
+        at most `restart` iterations to a relative inner_tol.  Returns (x,
+        report); report["iterations"] counts matvecs."""
         o = A.SolveOpts()
         o.method = method | (A.MLFMA_SOLVE_ADJOINT if adjoint else 0)
         o.restart = restart
         o.maxit = maxit
         o.use_voltage = 1 if b is None else 0
         o.tol = tol
+        o.outer_restart = outer_restart
+        o.inner_tol = inner_tol
         rep = A.SolveReport()
         hist = np.zeros(maxit)
         if _is_cuda(b):
@@ -250,11 +255,13 @@ class Plan:
         _check(self.lib, self.lib.mlfma_excitation(self._h, C.byref(pw), v.ctypes.data_as(A.dptr)))
         return v
 
-    def solve_batch(self, B, restart=50, maxit=500, tol=1e-6, history=False, method=A.MLFMA_GMRES_RESTART):
+    def solve_batch(self, B, restart=50, maxit=500, tol=1e-6, history=False, method=A.MLFMA_GMRES_RESTART,
+                    outer_restart=0, inner_tol=0.0):
         """GMRES(restart) on the rows of B (nrhs, n) at once, one batched
         matvec per step (method MLFMA_FGMRES_IR on an FP32 plan: FP64
-        refinement around batched FP32-far-field GMRES).  Returns (X,
-        [report per right-hand side])."""
+        flexible GMRES around batched FP32-far-field GMRES; MLFMA_GMRES_IR:
+        FP64 refinement around it).  Returns (X, [report per right-hand
+        side])."""
         B = np.ascontiguousarray(B, dtype=np.complex128)
         if B.ndim != 2 or B.shape[1] != self.n:
             raise ValueError("B must be (nrhs, n)")
@@ -265,6 +272,8 @@ class Plan:
         o.maxit = maxit
         o.use_voltage = 0
         o.tol = tol
+        o.outer_restart = outer_restart
+        o.inner_tol = inner_tol
         reps = (A.SolveReport * nrhs)()
         X = np.zeros_like(B)
         hist = np.zeros((nrhs, maxit))

@@ -47,10 +47,21 @@ def test_restarted_and_mixed_gmres_vs_dense_solve():
     assert rep["status"] == "OK" and rep["resid_true"] <= 1e-11
     assert cases.relerr(x, xd) <= 1e-8
     p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, **kw))
-    x32, r32 = p32.solve(b, method=A.MLFMA_FGMRES_IR, restart=n + 2, maxit=6 * n, tol=1e-9)
-    print("IR", r32, flush=True)
-    assert r32["status"] == "OK" and r32["resid_true"] <= 1e-9
-    assert cases.relerr(x32, xd) <= 1e-6
+    # the FP64 outer of the mixed solvers reaches an FP64 tolerance although
+    # every inner step runs the FP32 far field: flexible GMRES (several inner
+    # budgets and outer restarts) and iterative refinement.  (Short inner
+    # budgets with short outer cycles, e.g. GMRES(5) inside FGMRES(8),
+    # stagnate on this indefinite operator; a NumPy FGMRES on its dense matrix
+    # does the same.)
+    for method, kw_s in [(A.MLFMA_FGMRES_IR, dict(restart=n + 2)),
+                         (A.MLFMA_FGMRES_IR, dict(restart=12, outer_restart=200, inner_tol=1e-3)),
+                         (A.MLFMA_GMRES_IR, dict(restart=n + 2))]:
+        x32, r32 = p32.solve(b, method=method, maxit=20 * n, tol=1e-10, **kw_s)
+        print("mixed", method, kw_s, r32, cases.relerr(x32, xd), flush=True)
+        assert r32["status"] == "OK" and r32["resid_true"] <= 1e-10
+        # the outer estimate is the true FP64 residual of the returned x
+        assert abs(r32["resid_est"] - r32["resid_true"]) <= 1e-3 * r32["resid_true"] + 1e-15
+        assert cases.relerr(x32, xd) <= 1e-7
 
 
 def test_zero_rhs_and_errors():
@@ -83,13 +94,17 @@ def test_batched_solve_matches_single_and_dense():
         assert cases.relerr(X[r], np.linalg.solve(Z, B[r])) <= 1e-8
         xs, rs = p.solve(B[r], method=A.MLFMA_GMRES_RESTART, restart=n + 2, maxit=3 * n, tol=1e-12)
         assert cases.relerr(X[r], xs) <= 1e-9
-    # batched FP64 refinement around FP32-far-field inner solves
+    # batched FP64 flexible GMRES / refinement around FP32-far-field inner solves
     p32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, **kw))
-    X32, reps32 = p32.solve_batch(B, restart=n + 2, maxit=6 * n, tol=1e-9, method=A.MLFMA_FGMRES_IR)
-    for r in (0, 1, 2, 4):
-        assert reps32[r]["status"] == "OK" and reps32[r]["resid_true"] <= 1e-9, (r, reps32[r])
-        assert cases.relerr(X32[r], np.linalg.solve(Z, B[r])) <= 1e-6
-    assert np.all(X32[3] == 0)
+    for method, kw_s in [(A.MLFMA_FGMRES_IR, dict(restart=n + 2)),
+                         (A.MLFMA_FGMRES_IR, dict(restart=12, outer_restart=200, inner_tol=1e-3)),
+                         (A.MLFMA_GMRES_IR, dict(restart=n + 2))]:
+        X32, reps32 = p32.solve_batch(B, maxit=20 * n, tol=1e-9, method=method, **kw_s)
+        print("batched mixed", method, kw_s, [r["iterations"] for r in reps32], flush=True)
+        for r in (0, 1, 2, 4):
+            assert reps32[r]["status"] == "OK" and reps32[r]["resid_true"] <= 1e-9, (r, reps32[r])
+            assert cases.relerr(X32[r], np.linalg.solve(Z, B[r])) <= 1e-6
+        assert np.all(X32[3] == 0)
     # restarted (small m) batch on a larger problem: every column converges
     geom2, kw2 = cases.make("rwg_fixed")
     q = Plan(geom2, cases.planewave_z(), A.make_config(device=0, flags=A.MLFMA_FLAG_PHYSICAL, **kw2))
