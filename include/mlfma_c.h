@@ -221,6 +221,12 @@ int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t pt
                                       must be complete when the call is made
                                       (the solve runs on the plan's stream)        */
 
+#define MLFMA_PRECOND_NONE 0
+#define MLFMA_PRECOND_LEAF_BLOCKS 1  /* block Jacobi: the inverse of each leaf's
+                                        self block (Z restricted to the leaf's
+                                        sources), built on first use; no
+                                        reference counterpart */
+
 typedef struct mlfma_solve_opts {
     int32_t method;       /* MLFMA_GMRES_*                                          */
     int32_t restart;      /* m for GMRES(m) / inner iterations for FGMRES; <= 2048 */
@@ -228,9 +234,15 @@ typedef struct mlfma_solve_opts {
     int32_t use_voltage;  /* 1: b = -V (reference), 0: b from the argument          */
     double tol;           /* reference: 1e-6                                        */
     int32_t outer_restart;/* MLFMA_FGMRES_IR: outer steps per cycle (0: 30)         */
-    int32_t reserved;
+    int32_t precond;      /* MLFMA_PRECOND_*: right preconditioner of the GMRES(m)
+                             steps (inner steps for FGMRES_IR / GMRES_IR); not
+                             with MLFMA_GMRES_COMPAT                               */
     double inner_tol;     /* MLFMA_FGMRES_IR: relative tolerance of each inner
                              solve (0: 1e-1)                                        */
+    double shift_re;      /* solve (Z + s I) x = b with s = shift_re + i shift_im   */
+    double shift_im;      /* (0: Z x = b, the reference); the adjoint solves
+                             (Z + s I)^H x = b.  E.g. a dipole self term, which
+                             the reference drops (src/sources/dipole.h:65)          */
 } mlfma_solve_opts;
 
 typedef struct mlfma_solve_report {
@@ -258,6 +270,13 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
  * Non-convergence is reported per right-hand side in reports[r].status. */
 int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs,
                       const mlfma_solve_opts* opts, mlfma_solve_report* reports, double* resid_hist);
+
+/* y = P x (adjoint != 0: P^H x) for the preconditioner of kind
+ * MLFMA_PRECOND_* (built on first use); x, y: nrhs vectors of n complex
+ * (column stride n, input order), ptr_mode and stream as in mlfma_matvec.
+ * No reference counterpart. */
+int mlfma_precond_apply(mlfma_plan* plan, int32_t kind, const double* x, double* y, int32_t nrhs, int32_t adjoint,
+                        int32_t ptr_mode, void* stream);
 
 /* --------------------------------------------------------------- far field */
 

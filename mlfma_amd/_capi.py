@@ -31,6 +31,8 @@ MLFMA_GMRES_COMPAT = 0
 MLFMA_GMRES_RESTART = 1
 MLFMA_FGMRES_IR = 2
 MLFMA_GMRES_IR = 3
+MLFMA_PRECOND_NONE = 0
+MLFMA_PRECOND_LEAF_BLOCKS = 1
 MLFMA_SOLVE_ADJOINT = 0x100
 MLFMA_SOLVE_DEVICE = 0x200
 MLFMA_PARAM_NEAR_WINDOW = 1
@@ -81,7 +83,8 @@ class Info(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("restart", C.c_int32), ("maxit", C.c_int32),
                 ("use_voltage", C.c_int32), ("tol", C.c_double), ("outer_restart", C.c_int32),
-                ("reserved", C.c_int32), ("inner_tol", C.c_double)]
+                ("precond", C.c_int32), ("inner_tol", C.c_double),
+                ("shift_re", C.c_double), ("shift_im", C.c_double)]
 
 
 class SolveReport(C.Structure):
@@ -116,6 +119,8 @@ SIGNATURES = {
     "mlfma_plan_set_param": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "mlfma_plan_get_param": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_probe_read": (C.c_int, [C.c_void_p, dptr, C.POINTER(C.c_int64)]),
+    "mlfma_precond_apply": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_void_p]),
     "mlfma_autotune_record": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_autotune_apply": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]),
     "mlfma_autotune": (C.c_int, [C.c_void_p, C.c_int32, dptr, dptr]),

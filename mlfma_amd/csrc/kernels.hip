@@ -677,6 +677,15 @@ void launchFarfieldAdj(const SrcDev& s, int64_t n, const cplx<double>* E, const 
                        perm, I);
 }
 
+// y += s x over n complex values (the shift of a shifted solve)
+__global__ void kAxpy(const cplx<double>* __restrict__ x, cplx<double> s, cplx<double>* __restrict__ y, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (i < n) y[i] = y[i] + s * x[i];
+}
+void launchAxpy(const cplx<double>* x, cplx<double> s, cplx<double>* y, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(kAxpy, dim3((unsigned)((n + BS - 1) / BS)), dim3(BS), 0, st, x, s, y, n);
+}
+
 // ------------------------------------------------------------------ matvec stages
 
 __global__ void kGather(const cplx<double>* __restrict__ x, const int64_t* __restrict__ perm,

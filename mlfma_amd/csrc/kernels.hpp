@@ -43,6 +43,16 @@ struct NearItem {
     int64_t cpo;    // column partial sums [cpo, cpo + nc - nself)
 };
 
+// ---- one diagonal block of the leaf block-Jacobi preconditioner: the inverse
+// of a leaf's self block (rows and columns = its sources, tree positions
+// [sbeg, sbeg + cnt)), column-major cnt x cnt at off
+struct PrecBlock {
+    int64_t sbeg;
+    int32_t cnt;
+    int32_t pad;
+    int64_t off;
+};
+
 // ---- source descriptions in tree order
 struct DipoleDev {
     double pos[3];
@@ -107,6 +117,9 @@ void launchFarfieldAdj(const SrcDev& s, int64_t n, const cplx<double>* E, const 
 void launchNearGradDip(const SrcDev& s, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
                        const int64_t* iperm, const cplx<double>* I, const cplx<double>* lam, double k, cplx<double> Ck,
                        double* grad, hipStream_t st);
+
+// y += s x (n complex)
+void launchAxpy(const cplx<double>* x, cplx<double> s, cplx<double>* y, int64_t n, hipStream_t st);
 
 // matvec stages, templated on the real type of the far field
 void launchGather(const cplx<double>* x, const int64_t* perm, cplx<double>* xt, int64_t n, int nrhs,

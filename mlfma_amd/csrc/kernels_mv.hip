@@ -1283,6 +1283,58 @@ template void launchM2LNear<float, float>(M2LLevels<float>, const NearItem*, int
 template void launchM2LNear<float, double>(M2LLevels<float>, const NearItem*, int, const int32_t*, const int64_t*,
                                            const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t, bool, int);
 
+// ------------------------------------------------------------------ preconditioner
+// y = P x with P block diagonal over the leaves (the inverses of their self
+// blocks): one block per (leaf, right-hand side); the leaf's x values are
+// gathered into LDS, each lane forms rows (column-major P: lanes read
+// consecutive rows of a column).  The adjoint reads P^H.
+template <bool ADJ>
+__global__ __launch_bounds__(TB) void kBlockPrec(const PrecBlock* __restrict__ blocks, const cplx<double>* __restrict__ P,
+                                                 const int64_t* __restrict__ perm, const cplx<double>* __restrict__ x,
+                                                 cplx<double>* __restrict__ y, int64_t xs) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double>* sx = reinterpret_cast<cplx<double>*>(smem);
+    const PrecBlock B = blocks[blockIdx.x];
+    x += blockIdx.y * xs;
+    y += blockIdx.y * xs;
+    const int s = B.cnt;
+    for (int i = threadIdx.x; i < s; i += TB) sx[i] = x[perm[B.sbeg + i]];
+    __syncthreads();
+    const cplx<double>* Pb = P + B.off;
+    for (int r = threadIdx.x; r < s; r += TB) {
+        cplx<double> a = mk(0.0, 0.0);
+        if (!ADJ) {
+            for (int c = 0; c < s; ++c) a = a + Pb[(int64_t)c * s + r] * sx[c];
+        } else {
+            for (int c = 0; c < s; ++c) {
+                const cplx<double> v = Pb[(int64_t)r * s + c];
+                a = a + mk(v.re, -v.im) * sx[c];
+            }
+        }
+        y[perm[B.sbeg + r]] = a;
+    }
+}
+
+void launchBlockPrec(const PrecBlock* blocks, int nblocks, int maxcnt, const cplx<double>* P, const int64_t* perm,
+                     const cplx<double>* x, cplx<double>* y, int nb, int64_t xs, bool adjoint, hipStream_t st) {
+    if (nblocks <= 0) return;
+    const dim3 g((unsigned)nblocks, nb);
+    const size_t lds = (size_t)std::max(1, maxcnt) * sizeof(cplx<double>);
+    if (adjoint)
+        hipLaunchKernelGGL(kBlockPrec<true>, g, dim3(TB), lds, st, blocks, P, perm, x, y, xs);
+    else
+        hipLaunchKernelGGL(kBlockPrec<false>, g, dim3(TB), lds, st, blocks, P, perm, x, y, xs);
+}
+
+__global__ void kGatherValues(const cplx<double>* __restrict__ val, const int64_t* __restrict__ idx, int64_t n,
+                              cplx<double>* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i < n) out[i] = val[idx[i]];
+}
+void launchGatherValues(const cplx<double>* val, const int64_t* idx, int64_t n, cplx<double>* out, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(kGatherValues, dim3((unsigned)((n + TB - 1) / TB)), dim3(TB), 0, st, val, idx, n, out);
+}
+
 // FP32 copy of the near-field values (the mixed-precision inner solves)
 __global__ void kToFloat(const cplx<double>* __restrict__ in, cplx<float>* __restrict__ out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;

@@ -84,6 +84,12 @@ void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int
                    int windowPct);
 // out = (float) in, n complex values
 void launchToFloat(const cplx<double>* in, cplx<float>* out, int64_t n, hipStream_t st);
+// y = P x (adjoint: P^H x) for nb vectors (column stride xs, input order),
+// P = block diagonal over the preconditioner blocks; gathers the self-block
+// entries of the near-field values into packed column-major blocks
+void launchBlockPrec(const PrecBlock* blocks, int nblocks, int maxcnt, const cplx<double>* P, const int64_t* perm,
+                     const cplx<double>* x, cplx<double>* y, int nb, int64_t xs, bool adjoint, hipStream_t st);
+void launchGatherValues(const cplx<double>* val, const int64_t* idx, int64_t n, cplx<double>* out, hipStream_t st);
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
                    const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st,
                    int nb = 1, int64_t xs = 0, int64_t ps = 0);

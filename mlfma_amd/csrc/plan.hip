@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -629,6 +631,144 @@ void DevicePlan::build() {
     for (int64_t q = 0; q < n; ++q) Vin[t.perm[q]] = std::complex<double>(vt[q].re, vt[q].im);
 
     ensureRhs(1);
+}
+
+// Gauss-Jordan inversion with partial pivoting of a column-major s x s
+// complex block in place; false when a pivot falls below tiny.
+static bool invertBlock(std::complex<double>* A, int s, double tiny) {
+    std::vector<int> piv(s);
+    for (int k = 0; k < s; ++k) {
+        int p = k;
+        double best = std::abs(A[(size_t)k * s + k]);
+        for (int i = k + 1; i < s; ++i) {
+            const double v = std::abs(A[(size_t)k * s + i]);
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        if (!(best > tiny)) return false;
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < s; ++j) std::swap(A[(size_t)j * s + k], A[(size_t)j * s + p]);
+        const std::complex<double> d = 1.0 / A[(size_t)k * s + k];
+        A[(size_t)k * s + k] = 1.0;
+        for (int j = 0; j < s; ++j) A[(size_t)j * s + k] *= d;
+        for (int i = 0; i < s; ++i) {
+            if (i == k) continue;
+            const std::complex<double> f = A[(size_t)k * s + i];
+            if (f == 0.0) continue;
+            A[(size_t)k * s + i] = 0.0;
+            for (int j = 0; j < s; ++j) A[(size_t)j * s + i] -= f * A[(size_t)j * s + k];
+        }
+    }
+    for (int k = s - 1; k >= 0; --k)  // undo the row swaps as column swaps of the inverse
+        if (piv[k] != k)
+            for (int i = 0; i < s; ++i) std::swap(A[(size_t)k * s + i], A[(size_t)piv[k] * s + i]);
+    return true;
+}
+
+// The leaf block-Jacobi preconditioner: the self block of every non-empty
+// leaf (its stored near-field entries, C k folded in) is gathered on the
+// device, inverted on the host (leaves in parallel) and uploaded.  A block
+// that is singular to working precision (e.g. a one-source leaf of the
+// reference's kernel, whose diagonal is zero) is inverted with a small
+// diagonal shift, and an all-zero block gets a scaled identity.
+void DevicePlan::buildPrecond() {
+    if (precVal) return;
+    const HostPlan& h = hp;
+    const Tree& t = h.tree;
+    std::vector<PrecBlock> blocks;
+    std::vector<int64_t> idx;
+    size_t item = 0;
+    int64_t off = 0;
+    for (int id = 0; id < (int)t.nodes.size(); ++id) {
+        const Node& nd = t.nodes[id];
+        if (!nd.leaf() || nd.count() == 0) continue;
+        const int64_t cnt = nd.count();
+        if (cnt > 8192) throw Error(MLFMA_ERR_ARG, "the block preconditioner takes leaves of at most 8192 sources");
+        blocks.push_back(PrecBlock{nd.sbeg, (int32_t)cnt, 0, off});
+        const size_t base = idx.size();
+        idx.resize(base + (size_t)cnt * cnt, -1);
+        // this leaf's items come next in the item list; the first has c0 = 0
+        if (item >= h.nearItems.size() || h.nearItems[item].rbeg != nd.sbeg)
+            throw Error(MLFMA_ERR_STATE, "near-field items out of leaf order");
+        const int64_t cbase = h.nearItems[item].cptr;
+        while (item < h.nearItems.size() && h.nearItems[item].rbeg >= nd.sbeg && h.nearItems[item].rbeg < nd.send) {
+            const auto& it = h.nearItems[item];
+            const int64_t c0 = it.cptr - cbase, r0 = it.rbeg - nd.sbeg;
+            for (int j = 0; j < it.nself; ++j)
+                for (int i = 0; i < it.nr; ++i)
+                    idx[base + (size_t)(c0 + j) * cnt + (r0 + i)] = it.voff + (int64_t)j * it.nr + i;
+            ++item;
+        }
+        for (size_t q = base; q < idx.size(); ++q)
+            if (idx[q] < 0) throw Error(MLFMA_ERR_STATE, "a self-block entry is missing from the near field");
+        off += cnt * cnt;
+        precMaxCnt = std::max(precMaxCnt, (int)cnt);
+    }
+    nPrecBlocks = (int)blocks.size();
+    int64_t* dIdx = upload(idx);
+    cplx<double>* dS = (cplx<double>*)alloc(std::max<int64_t>(off, 1) * sizeof(cplx<double>));
+    launchGatherValues(nearVal, dIdx, off, dS, stream);
+    hipCheck(hipGetLastError(), "gather launch");
+    std::vector<std::complex<double>> S(off);
+    hipCheck(hipStreamSynchronize(stream), "gather");
+    if (off) hipCheck(hipMemcpy(S.data(), dS, off * sizeof(cplx<double>), hipMemcpyDeviceToHost), "block download");
+    release(dIdx);
+    // a scale for all-zero blocks: the median of the blocks' largest entries
+    std::vector<double> mx(blocks.size(), 0.0);
+    for (size_t b = 0; b < blocks.size(); ++b)
+        for (int64_t q = 0; q < (int64_t)blocks[b].cnt * blocks[b].cnt; ++q)
+            mx[b] = std::max(mx[b], std::abs(S[blocks[b].off + q]));
+    std::vector<double> nz;
+    for (double v : mx)
+        if (v > 0) nz.push_back(v);
+    std::sort(nz.begin(), nz.end());
+    const double typical = nz.empty() ? 1.0 : nz[nz.size() / 2];
+    std::atomic<size_t> next{0};
+    std::atomic<int> regularised{0};
+    auto work = [&]() {
+        std::vector<std::complex<double>> A;
+        for (size_t b; (b = next++) < blocks.size();) {
+            const int c = blocks[b].cnt;
+            std::complex<double>* B = S.data() + blocks[b].off;
+            if (mx[b] == 0) {
+                for (int64_t q = 0; q < (int64_t)c * c; ++q) B[q] = 0.0;
+                for (int i = 0; i < c; ++i) B[(size_t)i * c + i] = 1.0 / typical;
+                ++regularised;
+                continue;
+            }
+            A.assign(B, B + (size_t)c * c);
+            if (invertBlock(A.data(), c, 1e-13 * mx[b])) {
+                std::copy(A.begin(), A.end(), B);
+                continue;
+            }
+            ++regularised;
+            for (double shift : {1e-8, 1e-5, 1e-2, 1.0}) {
+                A.assign(B, B + (size_t)c * c);
+                for (int i = 0; i < c; ++i) A[(size_t)i * c + i] += shift * mx[b];
+                if (invertBlock(A.data(), c, 1e-13 * mx[b])) {
+                    std::copy(A.begin(), A.end(), B);
+                    break;
+                }
+            }
+        }
+    };
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < nt; ++i) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    precRegularised = regularised.load();
+    if (off) hipCheck(hipMemcpy(dS, S.data(), off * sizeof(cplx<double>), hipMemcpyHostToDevice), "block upload");
+    precVal = dS;
+    precBlocks = upload(blocks);
+}
+
+void DevicePlan::applyPrecond(const cplx<double>* x, cplx<double>* y, int nb, int64_t xs, bool adjoint,
+                              hipStream_t st) {
+    launchBlockPrec(precBlocks, nPrecBlocks, precMaxCnt, precVal, perm, x, y, nb, xs, adjoint, st);
 }
 
 void DevicePlan::ensureRhs(int nrhs) {
@@ -1538,6 +1678,35 @@ int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
     d.tuneMinBlocks.assign(rec + 6, rec + 6 + L);
     d.tuneM2L.assign(rec + 6 + L, rec + 6 + 2 * L);
     d.clearGraphs();
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_precond_apply(mlfma_plan* plan, int32_t kind, const double* x, double* y, int32_t nrhs, int32_t adjoint,
+                        int32_t ptr_mode, void* stream) {
+    API_BEGIN
+    DevicePlan& d = P(plan);
+    needDevice(d);
+    if (!x || !y || nrhs < 1) throw Error(MLFMA_ERR_ARG, "bad preconditioner arguments");
+    if (kind != MLFMA_PRECOND_LEAF_BLOCKS) throw Error(MLFMA_ERR_ARG, "unknown preconditioner");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    d.buildPrecond();
+    const int64_t n = d.hp.src.n;
+    hipStream_t st = stream ? (hipStream_t)stream : d.stream;
+    if (ptr_mode == MLFMA_PTR_DEVICE) {
+        d.applyPrecond((const cplx<double>*)x, (cplx<double>*)y, nrhs, n, adjoint != 0, st);
+        hipCheck(hipGetLastError(), "preconditioner launch");
+        if (!(d.flags & MLFMA_FLAG_ASYNC)) hipCheck(hipStreamSynchronize(st), "preconditioner");
+        return MLFMA_OK;
+    }
+    if (ptr_mode != MLFMA_PTR_HOST) throw Error(MLFMA_ERR_ARG, "bad ptr_mode");
+    d.ensureRhs(nrhs);
+    const size_t bytes = (size_t)n * nrhs * sizeof(cplx<double>);
+    hipCheck(hipMemcpyAsync(d.xs, x, bytes, hipMemcpyHostToDevice, st), "x upload");
+    d.applyPrecond(d.xs, d.ys, nrhs, n, adjoint != 0, st);
+    hipCheck(hipGetLastError(), "preconditioner launch");
+    hipCheck(hipMemcpyAsync(y, d.ys, bytes, hipMemcpyDeviceToHost, st), "y download");
+    hipCheck(hipStreamSynchronize(st), "preconditioner");
     return MLFMA_OK;
     API_END
 }

@@ -130,6 +130,14 @@ struct DevicePlan {
     int rhsCap = 0;
     cplx<double>* farPart = nullptr;  // far-field partial sums (grown on demand)
     size_t farPartLen = 0;
+    // leaf block-Jacobi preconditioner, built on first use: per non-empty leaf
+    // the inverse of its self block (Z restricted to the leaf's sources)
+    PrecBlock* precBlocks = nullptr;
+    int nPrecBlocks = 0, precMaxCnt = 0, precRegularised = 0;
+    cplx<double>* precVal = nullptr;
+    void buildPrecond();
+    // y = P x (P^H x when adjoint) for nb vectors of column stride xs
+    void applyPrecond(const cplx<double>* x, cplx<double>* y, int nb, int64_t xs, bool adjoint, hipStream_t st);
 
     // launch-group probe (MLFMA_PARAM_PROBE_GROUP): an event pair around every
     // launch of one group, recorded on the stream that launch goes to, so
@@ -189,16 +197,23 @@ struct DevicePlan {
                  int nb = 1, int64_t xs = 0);
     // the operator of the running solve: Z, or Z^H for an adjoint solve
     bool solveAdjoint = false;
+    // the running solve applies the block preconditioner on the right
+    bool solvePrec = false;
     // a one-right-hand-side solve is running with solveSerial set: every
     // stage on the solve's stream; solveSerial is set by mlfma_autotune(1)
     // when short GMRES runs are faster that way
     bool solveInOrder = false;
     bool solveSerial = false;
+    // the shift s of a shifted solve, (Z + s I) x = b (zero: none)
+    cplx<double> solveShift{0, 0};
+    bool shifted() const { return solveShift.re != 0 || solveShift.im != 0; }
     void applySolveOp(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool fp32, bool timed) {
         if (solveAdjoint)
             applyOpAdjoint(x, y, accumulate, st, fp32);
         else
             applyOp(x, y, accumulate, st, fp32, timed);
+        if (shifted())  // (Z + s I)^H = Z^H + conj(s) I
+            launchAxpy(x, cplx<double>{solveShift.re, solveAdjoint ? -solveShift.im : solveShift.im}, y, hp.src.n, st);
     }
     void matvec(const cplx<double>* x, cplx<double>* y, int nrhs, int accumulate, hipStream_t st, bool fp32);
 

@@ -664,6 +664,8 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
     m = std::max(1, std::min(m, maxit));
     cplx<double>* Q = (cplx<double>*)d.alloc((size_t)n * (m + 1) * sizeof(cplx<double>));
     cplx<double>* w = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    // right preconditioning (d.solvePrec): steps apply Z P, and x += P Q y
+    cplx<double>* pt = d.solvePrec ? (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>)) : nullptr;
     GivensWork G(d, m);
     const double bn = K.norm(b);
     if (!(bn > 0) || !std::isfinite(bn)) {
@@ -710,7 +712,11 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         int uncovered = 0;
         while (true) {
             while (ncolUsed < 0 && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
-                d.applySolveOp(Q + (size_t)jq * n, w, 0, K.st, fp32, false);
+                if (pt) {
+                    d.applyPrecond(Q + (size_t)jq * n, pt, 1, n, d.solveAdjoint, K.st);
+                    d.applySolveOp(pt, w, 0, K.st, fp32, false);
+                } else
+                    d.applySolveOp(Q + (size_t)jq * n, w, 0, K.st, fp32, false);
                 const int ncol = jq + 1;
                 for (int pass = 0; pass < 2; ++pass) {  // CGS2; the second pass ends with the Givens step
                     hipLaunchKernelGGL(kDotWide, dim3(G.nbd, ncol), dim3(KB), 0, K.st, (const cplx<double>*)Q, n,
@@ -772,7 +778,12 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
             std::vector<zc> g(j);
             for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
             const std::vector<zc> y = backSolve(H, g, j);
-            K.combine(Q, j, y, x, 1);
+            if (pt) {  // x += P (Q y), through w
+                K.combine(Q, j, y, w, 0);
+                d.applyPrecond(w, pt, 1, n, d.solveAdjoint, K.st);
+                K.combine(pt, 1, std::vector<zc>{1.0}, x, 1);
+            } else
+                K.combine(Q, j, y, x, 1);
         }
         if (out.status != MLFMA_OK) break;
     }
@@ -780,6 +791,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
     if (out.status == MLFMA_OK && out.est > tol) out.status = MLFMA_ERR_NOT_CONVERGED;
     d.release(Q);
     d.release(w);
+    if (pt) d.release(pt);
     return out;
 }
 
@@ -795,6 +807,7 @@ static void applyMany(DevicePlan& d, const cplx<double>* x, cplx<double>* y, int
         d.applyOp(x + (size_t)r * n, y + (size_t)r * n, 0, st, fp32, false, nb, n);
         r += nb;
     }
+    if (d.shifted()) launchAxpy(x, d.solveShift, y, (int64_t)nrhs * n, st);
 }
 
 // GMRES(m) on nrhs right-hand sides in lockstep: one batched matvec per step
@@ -882,6 +895,8 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
     const int64_t ld = (int64_t)nrhs * n;
     cplx<double>* Q = (cplx<double>*)d.alloc((size_t)ld * (m + 1) * sizeof(cplx<double>));
     cplx<double>* W = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
+    // right preconditioning (d.solvePrec): steps apply Z P, and X += P Q y
+    cplx<double>* PT = d.solvePrec ? (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>)) : nullptr;
     BatchOrtho BO(d, nrhs, m);
     GivensBatch GB(d, nrhs, m);
     std::vector<SolveOut> out(nrhs);
@@ -940,7 +955,11 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         bool stop = false;
         while (true) {
             while (!stop && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
-                applyMany(d, Q + (size_t)jq * ld, W, nrhs, st, fp32);
+                if (PT) {
+                    d.applyPrecond(Q + (size_t)jq * ld, PT, nrhs, n, false, st);
+                    applyMany(d, PT, W, nrhs, st, fp32);
+                } else
+                    applyMany(d, Q + (size_t)jq * ld, W, nrhs, st, fp32);
                 BO.queue(Q, ld, jq + 1, W, nullptr, st);
                 hipLaunchKernelGGL(kGivensB, dim3(nrhs), dim3(64), (3 * jq + 6) * sizeof(cplx<double>), st,
                                    (const cplx<double>*)BO.h, BO.hs, jq, m,
@@ -998,14 +1017,21 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
                 std::vector<zc> g(j);
                 for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
                 const std::vector<zc> y = backSolve(H, g, j);
-                K[r]->combine(Q + (size_t)r * n, j, y, X + (size_t)r * n, 1, ld);
-            }
+                K[r]->combine(Q + (size_t)r * n, j, y, PT ? W + (size_t)r * n : X + (size_t)r * n, PT ? 0 : 1, ld);
+            } else if (PT)
+                hipCheck(hipMemsetAsync(W + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+        if (PT) {  // X += P (Q y) of every lane, through W
+            d.applyPrecond(W, PT, nrhs, n, false, st);
+            for (int r = 0; r < nrhs; ++r)
+                if (steps[r] > 0) K[r]->combine(PT + (size_t)r * n, 1, std::vector<zc>{1.0}, X + (size_t)r * n, 1);
+        }
         if (breakdown) break;
     }
     for (int r = 0; r < nrhs; ++r)
         if (out[r].status == MLFMA_OK && out[r].est > tolv[r]) out[r].status = MLFMA_ERR_NOT_CONVERGED;
     d.release(Q);
     d.release(W);
+    if (PT) d.release(PT);
     return out;
 }
 
@@ -1352,8 +1378,21 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
         ~AdjointMode() {
             d.solveAdjoint = false;
             d.solveInOrder = false;
+            d.solvePrec = false;
+            d.solveShift = cplx<double>{0, 0};
         }
     } adjointMode{d};
+    if (!std::isfinite(opts->shift_re) || !std::isfinite(opts->shift_im))
+        throw Error(MLFMA_ERR_NONFINITE, "non-finite shift");
+    d.solveShift = cplx<double>{opts->shift_re, opts->shift_im};
+    if (opts->precond < 0 || opts->precond > 1) throw Error(MLFMA_ERR_ARG, "unknown preconditioner");
+    if (opts->precond && method == MLFMA_GMRES_COMPAT)
+        throw Error(MLFMA_ERR_ARG, "the reference-compatible GMRES takes no preconditioner");
+    if (opts->precond) {
+        hipCheck(hipSetDevice(d.device), "hipSetDevice");
+        d.buildPrecond();
+        d.solvePrec = true;
+    }
     d.solveAdjoint = (opts->method & MLFMA_SOLVE_ADJOINT) != 0;
     // one right-hand side: the matvecs run their stages in order when the
     // autotuner found that overlap does not pay (2.7% per iteration at 30k)
@@ -1508,6 +1547,21 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     const int m = opts->restart > 0 ? opts->restart : 50;
     std::vector<std::unique_ptr<Krylov>> K;
     const int mOut = opts->outer_restart > 0 ? opts->outer_restart : 30;
+    struct PrecMode {  // cleared however the solve ends
+        DevicePlan& d;
+        ~PrecMode() {
+            d.solvePrec = false;
+            d.solveShift = cplx<double>{0, 0};
+        }
+    } precMode{d};
+    if (!std::isfinite(opts->shift_re) || !std::isfinite(opts->shift_im))
+        throw Error(MLFMA_ERR_NONFINITE, "non-finite shift");
+    d.solveShift = cplx<double>{opts->shift_re, opts->shift_im};
+    if (opts->precond < 0 || opts->precond > 1) throw Error(MLFMA_ERR_ARG, "unknown preconditioner");
+    if (opts->precond) {
+        d.buildPrecond();
+        d.solvePrec = true;
+    }
     for (int r = 0; r < nrhs; ++r) K.emplace_back(new Krylov(d, st, std::max(std::min(m, opts->maxit), mOut) + 2));
     std::vector<SolveOut> o =
         opts->method == MLFMA_FGMRES_IR

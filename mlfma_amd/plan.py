@@ -134,6 +134,16 @@ class Plan:
                             1 if accumulate else 0, A.MLFMA_PTR_HOST, None))
         return y
 
+    def precond(self, x, adjoint=False, kind=A.MLFMA_PRECOND_LEAF_BLOCKS):
+        """y = P x (or P^H x): the leaf block-Jacobi preconditioner (host
+        arrays (n,) or (nrhs, n))."""
+        x = _cvec(x)
+        nrhs = 1 if x.ndim == 1 else x.shape[0]
+        y = np.zeros_like(x)
+        _check(self.lib, self.lib.mlfma_precond_apply(self._h, kind, C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data),
+                                                      nrhs, 1 if adjoint else 0, A.MLFMA_PTR_HOST, None))
+        return y
+
     def matvec_direct(self, x):
         x = _cvec(x, self.n)
         y = np.zeros_like(x)
@@ -208,7 +218,7 @@ class Plan:
 
     # ---------------------------------------------------------------- solve
     def solve(self, b=None, method=A.MLFMA_GMRES_COMPAT, restart=50, maxit=500, tol=1e-6, history=False,
-              adjoint=False, outer_restart=0, inner_tol=0.0):
+              adjoint=False, outer_restart=0, inner_tol=0.0, precond=0, shift=0j):
         """GMRES for Z x = b (b = None: the plane-wave excitation -V), or for
         Z^H x = b with adjoint=True.  MLFMA_FGMRES_IR (FP32 plans): flexible
         GMRES(outer_restart) in FP64 around inner FP32-far-field GMRES solves of
@@ -222,6 +232,8 @@ class Plan:
         o.tol = tol
         o.outer_restart = outer_restart
         o.inner_tol = inner_tol
+        o.precond = precond
+        o.shift_re, o.shift_im = complex(shift).real, complex(shift).imag
         rep = A.SolveReport()
         hist = np.zeros(maxit)
         if _is_cuda(b):
@@ -256,7 +268,7 @@ class Plan:
         return v
 
     def solve_batch(self, B, restart=50, maxit=500, tol=1e-6, history=False, method=A.MLFMA_GMRES_RESTART,
-                    outer_restart=0, inner_tol=0.0):
+                    outer_restart=0, inner_tol=0.0, precond=0, shift=0j):
         """GMRES(restart) on the rows of B (nrhs, n) at once, one batched
         matvec per step (method MLFMA_FGMRES_IR on an FP32 plan: FP64
         flexible GMRES around batched FP32-far-field GMRES; MLFMA_GMRES_IR:
@@ -274,6 +286,8 @@ class Plan:
         o.tol = tol
         o.outer_restart = outer_restart
         o.inner_tol = inner_tol
+        o.precond = precond
+        o.shift_re, o.shift_im = complex(shift).real, complex(shift).imag
         reps = (A.SolveReport * nrhs)()
         X = np.zeros_like(B)
         hist = np.zeros((nrhs, maxit))
