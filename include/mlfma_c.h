@@ -160,6 +160,27 @@ int mlfma_plan_create(const mlfma_geometry* geom, const mlfma_planewave* pw,
 
 int mlfma_plan_destroy(mlfma_plan* plan);
 
+/* One plan for `count` independent scatterers (no reference counterpart: the
+ * reference holds one tree per process in static state,
+ * src/fmm/node.h:93-105).  Scatterer s has geometry geoms[s] and configuration
+ * cfgs[s]; all share the plane wave, the source kind, digits, interpolation
+ * order, oversampling, quadrature, precision, flags and device, and every
+ * root side must be the largest one divided by a power of two (equal leaf
+ * boxes at equal depth below it).  Levels are aligned by box size, so every
+ * matvec stage runs once for all scatterers over the concatenated worklists
+ * of each level (one launch per stage and level) and the far-field tables
+ * are shared.  The unknowns of scatterer s follow those of scatterer s - 1
+ * (mlfma_plan_segments); the operator is block diagonal.  mlfma_matvec,
+ * mlfma_matvec_adjoint, mlfma_solve_batch (GMRES per scatterer and
+ * right-hand side), mlfma_excitation and mlfma_farfield (over all sources)
+ * take forest plans; mlfma_solve and the level introspection calls do not. */
+int mlfma_plan_create_forest(const mlfma_geometry* geoms, const mlfma_config* cfgs, int32_t count,
+                             const mlfma_planewave* pw, mlfma_plan** out);
+
+/* Unknown offsets of the plan's scatterers: offsets[0..count] (count = 1 for
+ * a single-scatterer plan); at most cap values written. */
+int mlfma_plan_segments(const mlfma_plan* plan, int64_t* offsets, int32_t cap, int32_t* count);
+
 int mlfma_plan_info(const mlfma_plan* plan, mlfma_info* out);
 
 /* Excitation vector V (Source::getVoltage, src/sources/source.h:20), host
@@ -265,9 +286,13 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
 /* GMRES(m) on nrhs right-hand sides at once (e.g. one scatterer, several
  * incidence angles): one batched matvec per step over all of them (no
  * reference counterpart).  b, x: nrhs x n complex (host, column stride n);
- * opts->method must be MLFMA_GMRES_RESTART and opts->use_voltage is ignored;
- * reports: nrhs entries (or NULL); resid_hist: nrhs x maxit (or NULL).
- * Non-convergence is reported per right-hand side in reports[r].status. */
+ * opts->method MLFMA_GMRES_RESTART, MLFMA_FGMRES_IR or MLFMA_GMRES_IR;
+ * opts->use_voltage is ignored.  Every (right-hand side, scatterer) pair is
+ * a lane with its own Krylov space and stopping test (one scatterer per plan
+ * except for forest plans, mlfma_plan_segments): reports holds
+ * nrhs x segments entries, lane r * segments + s (or NULL); resid_hist the
+ * same lanes x maxit (or NULL).  Non-convergence is reported per lane in
+ * reports[k].status. */
 int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs,
                       const mlfma_solve_opts* opts, mlfma_solve_report* reports, double* resid_hist);
 

@@ -97,6 +97,9 @@ SIGNATURES = {
     "mlfma_plan_create": (C.c_int, [C.POINTER(Geometry), C.POINTER(PlaneWave), C.POINTER(Config),
                                     C.POINTER(C.c_void_p)]),
     "mlfma_plan_destroy": (C.c_int, [C.c_void_p]),
+    "mlfma_plan_create_forest": (C.c_int, [C.POINTER(Geometry), C.POINTER(Config), C.c_int32, C.POINTER(PlaneWave),
+                                           C.POINTER(C.c_void_p)]),
+    "mlfma_plan_segments": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]),
     "mlfma_plan_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
     "mlfma_voltage": (C.c_int, [C.c_void_p, dptr]),
     "mlfma_excitation": (C.c_int, [C.c_void_p, C.c_void_p, dptr]),

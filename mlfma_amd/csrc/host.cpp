@@ -700,4 +700,179 @@ void buildHostPlan(HostPlan& hp, const mlfma_geometry& g, const mlfma_planewave&
     buildNearItems(hp);
 }
 
+// ---------------------------------------------------------------- forests
+void mergeHostPlans(HostPlan& out, std::vector<HostPlan>& parts) {
+    if (parts.empty()) throw Error(MLFMA_ERR_ARG, "a forest needs at least one scatterer");
+    const HostPlan& p0 = parts[0];
+    double rootMax = 0;
+    for (const HostPlan& p : parts) {
+        const mlfma_config &a = p.cfg, &b = p0.cfg;
+        if (p.src.kind != p0.src.kind || a.digits != b.digits || a.interp_order != b.interp_order ||
+            a.over_interp != b.over_interp || a.quad_prec != b.quad_prec || a.precision != b.precision ||
+            a.flags != b.flags || p.k != p0.k)
+            throw Error(MLFMA_ERR_ARG, "the scatterers of a forest must share the source kind, wavenumber, digits, "
+                                       "interpolation order, oversampling, quadrature, precision and flags");
+        rootMax = std::max(rootMax, p.cfg.root_leng);
+    }
+    // level shift: root_leng = rootMax / 2^shift
+    int D = 0;
+    out.levelShift.clear();
+    for (const HostPlan& p : parts) {
+        const double r = std::log2(rootMax / p.cfg.root_leng);
+        const int sh = (int)std::lround(r);
+        if (std::fabs(r - sh) > 1e-9)
+            throw Error(MLFMA_ERR_ARG, "forest root sides must be the largest one over powers of two");
+        out.levelShift.push_back(sh);
+        D = std::max(D, p.tree.maxLevel + sh);
+    }
+    if (D > 15) throw Error(MLFMA_ERR_GEOMETRY, "forest deeper than 15 levels");
+    out.forest = true;
+    out.cfg = p0.cfg;
+    out.cfg.root_leng = rootMax;
+    out.cfg.max_level = D;
+    out.pw = p0.pw;
+    out.k = p0.k;
+    out.grids.resize(D + 1);
+    for (int l = 0; l <= D; ++l) out.grids[l] = buildGrid(l, out.k, rootMax, out.cfg.digits);
+    out.interp.assign(D + 1, InterpTable());
+    for (int l = 2; l < D; ++l) out.interp[l] = buildInterp(out.grids[l + 1], out.grids[l], out.cfg.interp_order);
+    // sources, tree nodes, permutation
+    Sources& S = out.src;
+    S.kind = p0.src.kind;
+    Tree& T = out.tree;
+    T.maxLevel = D;
+    out.segOff.assign(1, 0);
+    std::vector<int> nodeBase, triBase;
+    std::vector<int64_t> srcBase;
+    for (const HostPlan& p : parts) {
+        const int64_t sb = S.n;
+        const int nb = (int)T.nodes.size(), tb = (int)S.tris.size();
+        srcBase.push_back(sb);
+        nodeBase.push_back(nb);
+        triBase.push_back(tb);
+        const Sources& s = p.src;
+        S.n += s.n;
+        S.ctr.insert(S.ctr.end(), s.ctr.begin(), s.ctr.end());
+        S.pos.insert(S.pos.end(), s.pos.begin(), s.pos.end());
+        S.phat.insert(S.phat.end(), s.phat.begin(), s.phat.end());
+        S.tris.insert(S.tris.end(), s.tris.begin(), s.tris.end());
+        for (int q = 0; q < 2; ++q) {
+            for (int32_t t : s.tri[q]) S.tri[q].push_back(t + tb);
+            S.xpm[q].insert(S.xpm[q].end(), s.xpm[q].begin(), s.xpm[q].end());
+        }
+        S.leng.insert(S.leng.end(), s.leng.begin(), s.leng.end());
+        for (Node nd : p.tree.nodes) {
+            if (nd.parent >= 0) nd.parent += nb;
+            if (nd.child0 >= 0) nd.child0 += nb;
+            nd.level += out.levelShift.back();
+            nd.sbeg += sb;
+            nd.send += sb;
+            nd.slot = -1;
+            T.nodes.push_back(nd);
+        }
+        for (int64_t q : p.tree.perm) T.perm.push_back(q + sb);
+        out.segOff.push_back(S.n);
+    }
+    // slots per forest level: non-empty boxes at their own level >= 2, in
+    // scatterer order (boxes above a scatterer's level 2 take no work)
+    out.boxes.assign(D + 1, 0);
+    out.slotNode.assign(D + 1, {});
+    for (size_t s = 0; s < parts.size(); ++s)
+        for (int l = 0; l <= D; ++l) {
+            const int own = l - out.levelShift[s];
+            if (own < 2 || own > parts[s].tree.maxLevel) continue;
+            for (int id : parts[s].slotNode[own]) {
+                Node& nd = T.nodes[nodeBase[s] + id];
+                nd.slot = out.boxes[l]++;
+                out.slotNode[l].push_back(nodeBase[s] + id);
+            }
+        }
+    auto slotOf = [&](size_t s, int own, int ownSlot) {
+        return T.nodes[nodeBase[s] + parts[s].slotNode[own][ownSlot]].slot;
+    };
+    // far-field leaves
+    out.radpatEntries = 0;
+    for (size_t s = 0; s < parts.size(); ++s)
+        for (const auto& f : parts[s].farLeaves) {
+            HostPlan::LeafItem it = f;
+            it.node = f.node + nodeBase[s];
+            it.level = f.level + out.levelShift[s];
+            it.slot = T.nodes[it.node].slot;
+            it.sbeg = f.sbeg + srcBase[s];
+            it.rpOff = out.radpatEntries;
+            out.farLeaves.push_back(it);
+            out.radpatEntries += it.cnt * out.grids[it.level].ndir();
+        }
+    // worklists per forest level
+    out.m2lPtr.assign(D + 1, {});
+    out.m2lSrc.assign(D + 1, {});
+    out.m2lOff.assign(D + 1, {});
+    out.m2lPairs.assign(D + 1, 0);
+    out.m2mPtr.assign(D + 1, {});
+    out.m2mChild.assign(D + 1, {});
+    out.m2mOct.assign(D + 1, {});
+    out.l2lChild.assign(D + 1, {});
+    out.l2lParent.assign(D + 1, {});
+    out.l2lOct.assign(D + 1, {});
+    for (int l = 2; l <= D; ++l) {
+        out.m2lPtr[l].assign(1, 0);
+        if (l < D) out.m2mPtr[l].assign(1, 0);
+        for (size_t s = 0; s < parts.size(); ++s) {
+            const HostPlan& p = parts[s];
+            const int own = l - out.levelShift[s];
+            if (own < 2 || own > p.tree.maxLevel) continue;
+            const int nslot = (int)p.slotNode[own].size();
+            // M2L: sources are slots of the same level
+            for (int q = 0; q < nslot; ++q) {
+                for (int e = p.m2lPtr[own][q]; e < p.m2lPtr[own][q + 1]; ++e) {
+                    out.m2lSrc[l].push_back(slotOf(s, own, p.m2lSrc[own][e]));
+                    out.m2lOff[l].push_back(p.m2lOff[own][e]);
+                }
+                out.m2lPtr[l].push_back((int)out.m2lSrc[l].size());
+            }
+            // M2M into this level: children at own + 1
+            if (l < D)
+                for (int q = 0; q < nslot; ++q) {
+                    if (own < (int)p.m2mPtr.size() && !p.m2mPtr[own].empty())
+                        for (int e = p.m2mPtr[own][q]; e < p.m2mPtr[own][q + 1]; ++e) {
+                            out.m2mChild[l].push_back(slotOf(s, own + 1, p.m2mChild[own][e]));
+                            out.m2mOct[l].push_back(p.m2mOct[own][e]);
+                        }
+                    out.m2mPtr[l].push_back((int)out.m2mChild[l].size());
+                }
+            // L2L into this level: parents at own - 1
+            if (own >= 3)
+                for (size_t e = 0; e < p.l2lChild[own].size(); ++e) {
+                    out.l2lChild[l].push_back(slotOf(s, own, p.l2lChild[own][e]));
+                    out.l2lParent[l].push_back(slotOf(s, own - 1, p.l2lParent[own][e]));
+                    out.l2lOct[l].push_back(p.l2lOct[own][e]);
+                }
+        }
+        out.m2lPairs[l] = (int64_t)out.m2lSrc[l].size();
+    }
+    // near field: items, columns and partial sums side by side
+    out.nearSumPtr.assign(1, 0);
+    for (size_t s = 0; s < parts.size(); ++s) {
+        const HostPlan& p = parts[s];
+        const int64_t cb = (int64_t)out.nearCols.size(), vb = out.nearNnz, pb = out.nearParts;
+        for (auto it : p.nearItems) {
+            it.rbeg += srcBase[s];
+            it.cptr += cb;
+            it.voff += vb;
+            it.rpo += pb;
+            it.cpo += pb;
+            out.nearItems.push_back(it);
+        }
+        for (int64_t c : p.nearCols) out.nearCols.push_back(c + srcBase[s]);
+        const int64_t ib = (int64_t)out.nearSumIdx.size();
+        for (int32_t q : p.nearSumIdx) out.nearSumIdx.push_back((int32_t)(q + pb));
+        for (size_t q = 1; q < p.nearSumPtr.size(); ++q) out.nearSumPtr.push_back(p.nearSumPtr[q] + ib);
+        out.nearNnz += p.nearNnz;
+        out.nearParts += p.nearParts;
+        out.staleBoxes += p.staleBoxes;
+        out.truncatedL2L += p.truncatedL2L;
+    }
+    if (out.nearParts >= ((int64_t)1 << 31)) throw Error(MLFMA_ERR_ARG, "near field too large for one plan");
+}
+
 }  // namespace mlfma

@@ -175,10 +175,22 @@ struct HostPlan {
     std::vector<int32_t> nearSumIdx;
     int64_t nearNnz = 0, nearParts = 0;
     int64_t staleBoxes = 0, truncatedL2L = 0;
+    // a forest of several scatterers (mergeHostPlans): segOff[s] .. segOff[s+1]
+    // are scatterer s's unknowns (input order), levelShift[s] maps its own
+    // level l to the forest level l + levelShift[s]
+    bool forest = false;
+    std::vector<int64_t> segOff;
+    std::vector<int> levelShift;
 };
 
 void buildHostPlan(HostPlan& hp, const mlfma_geometry& g, const mlfma_planewave& pw,
                    const mlfma_config& cfg);
+
+// One plan for several independent scatterers: their trees side by side, the
+// levels aligned by box size (root sides must be the largest one over powers
+// of two), the worklists of every level concatenated so that each stage runs
+// once for all of them; unknowns of scatterer s follow those of s - 1.
+void mergeHostPlans(HostPlan& out, std::vector<HostPlan>& parts);
 
 // Integer box coordinates and M2L offsets
 inline uint64_t boxKey(int level, int ix, int iy, int iz) {

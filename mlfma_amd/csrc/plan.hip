@@ -116,6 +116,7 @@ T* DevicePlan::upload(const std::vector<T>& v) {
     if (!v.empty()) hipCheck(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
     return d;
 }
+template int64_t* DevicePlan::upload<int64_t>(const std::vector<int64_t>&);  // used by the solver
 
 static std::vector<double> inodeDistances() {  // 15 distinct |d| of the M2L offsets (src/math.cpp:86-107)
     std::vector<double> d;
@@ -1234,6 +1235,45 @@ int mlfma_plan_create(const mlfma_geometry* geom, const mlfma_planewave* pw, con
     API_END
 }
 
+int mlfma_plan_create_forest(const mlfma_geometry* geoms, const mlfma_config* cfgs, int32_t count,
+                             const mlfma_planewave* pw, mlfma_plan** out) {
+    API_BEGIN
+    if (!geoms || !cfgs || !pw || !out || count < 1) throw Error(MLFMA_ERR_ARG, "bad forest arguments");
+    *out = nullptr;
+    auto plan = std::make_unique<mlfma_plan>();
+    plan->d = std::make_unique<DevicePlan>();
+    DevicePlan& d = *plan->d;
+    std::vector<HostPlan> parts(count);
+    for (int s = 0; s < count; ++s) {
+        if (cfgs[s].device != cfgs[0].device) throw Error(MLFMA_ERR_ARG, "the scatterers of a forest share one device");
+        if ((cfgs[s].flags & MLFMA_FLAG_PHYSICAL) && geoms[s].kind != MLFMA_SRC_RWG)
+            throw Error(MLFMA_ERR_ARG, "MLFMA_FLAG_PHYSICAL needs RWG sources");
+        buildHostPlan(parts[s], geoms[s], *pw, cfgs[s]);
+    }
+    mergeHostPlans(d.hp, parts);
+    parts.clear();
+    d.flags = cfgs[0].flags;
+    d.hostOnly = cfgs[0].device < 0;
+    d.device = cfgs[0].device;
+    if (!d.hostOnly) d.build();
+    *out = plan.release();
+    return MLFMA_OK;
+    API_END
+}
+
+int mlfma_plan_segments(const mlfma_plan* plan, int64_t* offsets, int32_t cap, int32_t* count) {
+    API_BEGIN
+    const DevicePlan& d = P(plan);
+    if (!count) throw Error(MLFMA_ERR_ARG, "null argument");
+    const HostPlan& h = d.hp;
+    std::vector<int64_t> seg = h.forest ? h.segOff : std::vector<int64_t>{0, h.src.n};
+    *count = (int32_t)seg.size() - 1;
+    if (offsets)
+        for (int32_t i = 0; i < std::min<int32_t>(cap, (int32_t)seg.size()); ++i) offsets[i] = seg[i];
+    return MLFMA_OK;
+    API_END
+}
+
 int mlfma_plan_destroy(mlfma_plan* plan) {
     API_BEGIN
     delete plan;
@@ -1748,6 +1788,7 @@ int mlfma_level_boxes(const mlfma_plan* plan, int32_t level, int64_t* nbox, int3
                       int32_t* nsrc, int64_t cap) {
     API_BEGIN
     const DevicePlan& d = P(plan);
+    if (d.hp.forest) throw Error(MLFMA_ERR_STATE, "level introspection is not defined for forest plans");
     const Tree& t = d.hp.tree;
     if (level < 0 || level > t.maxLevel) throw Error(MLFMA_ERR_ARG, "level out of range");
     const int b = t.lvlBeg[level], e = t.lvlBeg[level + 1];
@@ -1773,6 +1814,7 @@ int mlfma_level_pairs(const mlfma_plan* plan, int32_t level, int32_t kind, int64
                       int32_t* src_level, int64_t cap) {
     API_BEGIN
     const DevicePlan& d = P(plan);
+    if (d.hp.forest) throw Error(MLFMA_ERR_STATE, "level introspection is not defined for forest plans");
     const Tree& t = d.hp.tree;
     const Lists& L = d.hp.lists;
     if (level < 0 || level > t.maxLevel) throw Error(MLFMA_ERR_ARG, "level out of range");
@@ -1806,6 +1848,7 @@ int mlfma_level_pairs(const mlfma_plan* plan, int32_t level, int32_t kind, int64
 int mlfma_get_coeffs(const mlfma_plan* plan, int32_t level, int32_t which, double* out, int64_t cap) {
     API_BEGIN
     const DevicePlan& d = P(plan);
+    if (d.hp.forest) throw Error(MLFMA_ERR_STATE, "level introspection is not defined for forest plans");
     needDevice(d);
     const Tree& t = d.hp.tree;
     if (level < 2 || level > t.maxLevel) throw Error(MLFMA_ERR_ARG, "coefficients exist for levels >= 2");

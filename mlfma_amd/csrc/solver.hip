@@ -105,17 +105,22 @@ __global__ void kUpdate(const cplx<double>* __restrict__ Q, int64_t ld, int ncol
     }
 }
 
-// The same three kernels for nrhs right-hand sides at once (grid z or y = rhs):
-// basis vectors of right-hand side r at Q + r n (column stride ld), w at
-// W + r n, partials at partial + r ps, coefficients at h + r hs.
+// The same three kernels for several lanes at once (grid z or y = lane): a
+// lane is a vector segment of laneLen[k] values at laneBase[k] in every
+// column (a right-hand side, or one scatterer's unknowns of a right-hand side
+// in a forest plan); basis column i at Q + i ld, w at W, partials at
+// partial + k ps, coefficients at h + k hs.  Blocks past a lane's length
+// contribute zero.
 __global__ void kDotPartialB(const cplx<double>* __restrict__ Q, int64_t ld, const cplx<double>* __restrict__ W,
-                             int64_t n, cplx<double>* __restrict__ partial, int64_t ps) {
+                             const int64_t* __restrict__ laneBase, const int64_t* __restrict__ laneLen,
+                             cplx<double>* __restrict__ partial, int64_t ps) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const int nb = gridDim.x, i = blockIdx.y, k = blockIdx.z;
     double re = 0, im = 0;
-    if (r < n) {
-        const cplx<double> wv = W[(int64_t)k * n + r];
-        const cplx<double> q = Q[(int64_t)i * ld + (int64_t)k * n + r];
+    if (r < laneLen[k]) {
+        const int64_t o = laneBase[k] + r;
+        const cplx<double> wv = W[o];
+        const cplx<double> q = Q[(int64_t)i * ld + o];
         re = q.re * wv.re + q.im * wv.im;
         im = q.re * wv.im - q.im * wv.re;
     }
@@ -136,20 +141,22 @@ __global__ void kReduceB(const cplx<double>* __restrict__ partial, int64_t ps, i
 }
 
 __global__ void kUpdateB(const cplx<double>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ h,
-                         int64_t hs, cplx<double>* __restrict__ W, int64_t n, cplx<double>* __restrict__ partial,
-                         int64_t ps, int norm) {
+                         int64_t hs, cplx<double>* __restrict__ W, const int64_t* __restrict__ laneBase,
+                         const int64_t* __restrict__ laneLen, cplx<double>* __restrict__ partial, int64_t ps,
+                         int norm) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const int k = blockIdx.y;
     double s2 = 0;
-    if (r < n) {
-        cplx<double> a = W[(int64_t)k * n + r];
+    if (r < laneLen[k]) {
+        const int64_t o = laneBase[k] + r;
+        cplx<double> a = W[o];
         for (int i = 0; i < ncol; ++i) {
-            const cplx<double> q = Q[(int64_t)i * ld + (int64_t)k * n + r];
+            const cplx<double> q = Q[(int64_t)i * ld + o];
             const cplx<double> c = h[(int64_t)k * hs + i];
             a.re -= q.re * c.re - q.im * c.im;
             a.im -= q.re * c.im + q.im * c.re;
         }
-        W[(int64_t)k * n + r] = a;
+        W[o] = a;
         s2 = a.re * a.re + a.im * a.im;
     }
     if (norm) {
@@ -412,11 +419,12 @@ __global__ void kGivensB(const cplx<double>* __restrict__ hbuf, int64_t hs, int 
 
 // Q_{j+1} lane k = W lane k / |w| (that lane's statistic), zero for lanes off
 __global__ void kScaleB(const cplx<double>* __restrict__ W, const double* __restrict__ stat, int j, int m,
-                        const int* __restrict__ on, cplx<double>* __restrict__ Qn, int64_t n) {
+                        const int* __restrict__ on, cplx<double>* __restrict__ Qn, const int64_t* __restrict__ laneBase,
+                        const int64_t* __restrict__ laneLen) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const int k = blockIdx.y;
-    if (r >= n) return;
-    const int64_t o = (int64_t)k * n + r;
+    if (r >= laneLen[k]) return;
+    const int64_t o = laneBase[k] + r;
     if (!on[k]) {
         Qn[o] = cplx<double>{0.0, 0.0};
         return;
@@ -424,8 +432,6 @@ __global__ void kScaleB(const cplx<double>* __restrict__ W, const double* __rest
     const double v = stat[(int64_t)k * 3 * (m + 1) + 3 * j + 1];
     Qn[o] = cplx<double>{W[o].re / v, W[o].im / v};
 }
-
-// g lanes = (beta_k, 0, ..., 0), m + 2 entries each
 __global__ void kInitGB(cplx<double>* __restrict__ g, const double* __restrict__ beta, int m2, int nrhs) {
     const int i = blockIdx.x * KB + threadIdx.x;
     if (i >= m2 * nrhs) return;
@@ -455,7 +461,10 @@ struct Krylov {
     int nb;
     cplx<double>* partial = nullptr;
     cplx<double>* h = nullptr;  // device column
-    Krylov(DevicePlan& dp, hipStream_t s, int maxcol) : d(dp), st(s), n(dp.hp.src.n) {
+    // len: the vector length it works on (a lane of a batched solve); the
+    // plan's n by default
+    Krylov(DevicePlan& dp, hipStream_t s, int maxcol, int64_t len = -1)
+        : d(dp), st(s), n(len < 0 ? dp.hp.src.n : len) {
         nb = (int)((n + KB - 1) / KB);
         partial = (cplx<double>*)d.alloc((size_t)nb * (maxcol + 2) * sizeof(cplx<double>));
         h = (cplx<double>*)d.alloc((size_t)2 * (maxcol + 2) * sizeof(cplx<double>));
@@ -810,45 +819,70 @@ static void applyMany(DevicePlan& d, const cplx<double>* x, cplx<double>* y, int
     if (d.shifted()) launchAxpy(x, d.solveShift, y, (int64_t)nrhs * n, st);
 }
 
-// GMRES(m) on nrhs right-hand sides in lockstep: one batched matvec per step
-// over the block of the right-hand sides' current Krylov vectors (basis
-// stored [step][rhs][n]), per right-hand side classical Gram-Schmidt twice
-// and unitary Givens rotations as in gmresRestart, one host sync per step
-// for all of them.  A right-hand side that has converged stops updating
-// (its lane of the block is zero) until the others finish the cycle.
-// CGS2 of nrhs right-hand sides at once: one launch per kernel for all of
-// them; hhost receives, per right-hand side at stride hs, the layout of
-// Krylov::queueOrtho (pass 1, |w|^2, pass 2).
+// Lanes of a batched solve: nvec vectors of the plan (column stride n) cut
+// into the plan's segments (the scatterers of a forest plan, else one
+// segment): lane k is segment k % nseg of vector k / nseg.
+struct Lanes {
+    DevicePlan& d;
+    int nvec = 0, nseg = 0, count = 0;
+    int64_t n = 0, maxLen = 0;
+    std::vector<int64_t> base, len;
+    int64_t *dBase = nullptr, *dLen = nullptr;
+    Lanes(DevicePlan& dp, int nv) : d(dp), nvec(nv), n(dp.hp.src.n) {
+        const std::vector<int64_t> seg = dp.hp.forest ? dp.hp.segOff : std::vector<int64_t>{0, n};
+        nseg = (int)seg.size() - 1;
+        count = nvec * nseg;
+        for (int v = 0; v < nvec; ++v)
+            for (int sg = 0; sg < nseg; ++sg) {
+                base.push_back((int64_t)v * n + seg[sg]);
+                len.push_back(seg[sg + 1] - seg[sg]);
+                maxLen = std::max(maxLen, seg[sg + 1] - seg[sg]);
+            }
+        dBase = d.upload(base);
+        dLen = d.upload(len);
+    }
+    ~Lanes() {
+        d.release(dBase);
+        d.release(dLen);
+    }
+    int blocks() const { return (int)std::max<int64_t>(1, (maxLen + KB - 1) / KB); }
+};
+
+// CGS2 of every lane at once: one launch per kernel for all of them; hhost
+// receives, per lane at stride hs, the layout of Krylov::queueOrtho (pass 1,
+// |w|^2, pass 2).
 struct BatchOrtho {
     DevicePlan& d;
-    int nrhs, nb;
-    int64_t n, ps, hs;
+    const Lanes& LN;
+    int nb;
+    int64_t ps, hs;
     cplx<double>*partial = nullptr, *h = nullptr;
-    BatchOrtho(DevicePlan& dp, int nr, int m) : d(dp), nrhs(nr), n(dp.hp.src.n) {
-        nb = (int)((n + KB - 1) / KB);
+    BatchOrtho(DevicePlan& dp, const Lanes& lanes, int m) : d(dp), LN(lanes) {
+        nb = LN.blocks();
         ps = (int64_t)nb * (m + 3);
         hs = 2 * (int64_t)m + 3;
-        partial = (cplx<double>*)d.alloc((size_t)nrhs * ps * sizeof(cplx<double>));
-        h = (cplx<double>*)d.alloc((size_t)nrhs * hs * sizeof(cplx<double>));
+        partial = (cplx<double>*)d.alloc((size_t)LN.count * ps * sizeof(cplx<double>));
+        h = (cplx<double>*)d.alloc((size_t)LN.count * hs * sizeof(cplx<double>));
     }
     ~BatchOrtho() {
         d.release(partial);
         d.release(h);
     }
     void queue(const cplx<double>* Q, int64_t ld, int ncol, cplx<double>* W, cplx<double>* hhost, hipStream_t st) {
+        const int nl = LN.count;
         for (int pass = 0; pass < 2; ++pass) {
-            hipLaunchKernelGGL(kDotPartialB, dim3(nb, ncol, nrhs), dim3(KB), 0, st, Q, ld, (const cplx<double>*)W, n,
-                               partial, ps);
-            hipLaunchKernelGGL(kReduceB, dim3(ncol, nrhs), dim3(KB), 0, st, (const cplx<double>*)partial, ps, nb,
+            hipLaunchKernelGGL(kDotPartialB, dim3(nb, ncol, nl), dim3(KB), 0, st, Q, ld, (const cplx<double>*)W,
+                               (const int64_t*)LN.dBase, (const int64_t*)LN.dLen, partial, ps);
+            hipLaunchKernelGGL(kReduceB, dim3(ncol, nl), dim3(KB), 0, st, (const cplx<double>*)partial, ps, nb,
                                h + (pass ? ncol + 1 : 0), hs);
-            hipLaunchKernelGGL(kUpdateB, dim3(nb, nrhs), dim3(KB), 0, st, Q, ld, ncol,
-                               (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), hs, W, n,
-                               partial + (size_t)nb * (ncol + 1), ps, pass);
+            hipLaunchKernelGGL(kUpdateB, dim3(nb, nl), dim3(KB), 0, st, Q, ld, ncol,
+                               (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), hs, W, (const int64_t*)LN.dBase,
+                               (const int64_t*)LN.dLen, partial + (size_t)nb * (ncol + 1), ps, pass);
         }
-        hipLaunchKernelGGL(kReduceB, dim3(1, nrhs), dim3(KB), 0, st, (const cplx<double>*)(partial + (size_t)nb * (ncol + 1)),
+        hipLaunchKernelGGL(kReduceB, dim3(1, nl), dim3(KB), 0, st, (const cplx<double>*)(partial + (size_t)nb * (ncol + 1)),
                            ps, nb, h + ncol, hs);
         if (hhost)
-            hipCheck(hipMemcpyAsync(hhost, h, (size_t)nrhs * hs * sizeof(cplx<double>), hipMemcpyDeviceToHost, st),
+            hipCheck(hipMemcpyAsync(hhost, h, (size_t)nl * hs * sizeof(cplx<double>), hipMemcpyDeviceToHost, st),
                      "h copy");
     }
 };
@@ -885,25 +919,31 @@ struct GivensBatch {
     }
 };
 
+// GMRES(m) on every lane in lockstep: one batched matvec per step over the
+// block of the vectors' current Krylov columns (basis stored [step][vector][n]),
+// per lane classical Gram-Schmidt twice and unitary Givens rotations as in
+// gmresRestart.  A lane that has converged stops updating (its part of the
+// block is zero) until the others finish the cycle.
 static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
-                                        const cplx<double>* B, cplx<double>* X, int nrhs, int m, int maxit,
+                                        const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m, int maxit,
                                         const std::vector<double>& tolv, double* hist, bool fp32 = false,
                                         int histOff = 0, int histLd = 0, bool lastResidual = true) {
     if (!histLd) histLd = maxit;
     const int64_t n = d.hp.src.n;
+    const int nrhs = LN.count, nvec = LN.nvec;
     m = std::max(1, std::min(m, maxit));
-    const int64_t ld = (int64_t)nrhs * n;
+    const int64_t ld = (int64_t)nvec * n;
     cplx<double>* Q = (cplx<double>*)d.alloc((size_t)ld * (m + 1) * sizeof(cplx<double>));
     cplx<double>* W = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
     // right preconditioning (d.solvePrec): steps apply Z P, and X += P Q y
     cplx<double>* PT = d.solvePrec ? (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>)) : nullptr;
-    BatchOrtho BO(d, nrhs, m);
+    BatchOrtho BO(d, LN, m);
     GivensBatch GB(d, nrhs, m);
     std::vector<SolveOut> out(nrhs);
     std::vector<double> bn(nrhs);
     std::vector<char> done(nrhs, 0);
     for (int r = 0; r < nrhs; ++r) {
-        bn[r] = K[r]->norm(B + (size_t)r * n);
+        bn[r] = K[r]->norm(B + LN.base[r]);
         if (!std::isfinite(bn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
         if (bn[r] == 0) done[r] = 1;
     }
@@ -916,14 +956,14 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         if (total == 0)
             hipCheck(hipMemcpyAsync(W, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
         else {
-            applyMany(d, X, W, nrhs, st, fp32);
-            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + (size_t)r * n, W + (size_t)r * n, W + (size_t)r * n);
+            applyMany(d, X, W, nvec, st, fp32);
+            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + LN.base[r], W + LN.base[r], W + LN.base[r]);
         }
         std::vector<double> beta(nrhs, 0.0);
         bool all = true;
         for (int r = 0; r < nrhs; ++r) {
             if (bn[r] == 0) continue;
-            beta[r] = total == 0 ? bn[r] : K[r]->norm(W + (size_t)r * n);
+            beta[r] = total == 0 ? bn[r] : K[r]->norm(W + LN.base[r]);
             out[r].est = beta[r] / bn[r];
             done[r] = out[r].est <= tolv[r];
             all = all && done[r];
@@ -942,8 +982,8 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         for (int r = 0; r < nrhs; ++r) {
             active[r] = !done[r];
             onh[r] = active[r];
-            if (active[r]) K[r]->scale(W + (size_t)r * n, beta[r], Q + (size_t)r * n);
-            else hipCheck(hipMemsetAsync(Q + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+            if (active[r]) K[r]->scale(W + LN.base[r], beta[r], Q + LN.base[r]);
+            else hipCheck(hipMemsetAsync(Q + LN.base[r], 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
         }
         hipCheck(hipMemcpyAsync(GB.on, onh.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, st), "lanes");
         hipCheck(hipMemcpyAsync(GB.bn, bn.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st), "norms");
@@ -956,17 +996,18 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         while (true) {
             while (!stop && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
                 if (PT) {
-                    d.applyPrecond(Q + (size_t)jq * ld, PT, nrhs, n, false, st);
-                    applyMany(d, PT, W, nrhs, st, fp32);
+                    d.applyPrecond(Q + (size_t)jq * ld, PT, nvec, n, false, st);
+                    applyMany(d, PT, W, nvec, st, fp32);
                 } else
-                    applyMany(d, Q + (size_t)jq * ld, W, nrhs, st, fp32);
+                    applyMany(d, Q + (size_t)jq * ld, W, nvec, st, fp32);
                 BO.queue(Q, ld, jq + 1, W, nullptr, st);
                 hipLaunchKernelGGL(kGivensB, dim3(nrhs), dim3(64), (3 * jq + 6) * sizeof(cplx<double>), st,
                                    (const cplx<double>*)BO.h, BO.hs, jq, m,
                                    (const double*)GB.bn, (const int*)GB.on, GB.cs, GB.sn, GB.g, GB.H, GB.stat,
                                    GB.statDev);
-                hipLaunchKernelGGL(kScaleB, dim3(K[0]->nb, nrhs), dim3(KB), 0, st, (const cplx<double>*)W,
-                                   (const double*)GB.stat, jq, m, (const int*)GB.on, Q + (size_t)(jq + 1) * ld, n);
+                hipLaunchKernelGGL(kScaleB, dim3(LN.blocks(), nrhs), dim3(KB), 0, st, (const cplx<double>*)W,
+                                   (const double*)GB.stat, jq, m, (const int*)GB.on, Q + (size_t)(jq + 1) * ld,
+                                   (const int64_t*)LN.dBase, (const int64_t*)LN.dLen);
                 hipCheck(hipEventRecord(GB.ev[jq % kWindow], st), "event");
                 ++jq;
             }
@@ -1017,13 +1058,13 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
                 std::vector<zc> g(j);
                 for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
                 const std::vector<zc> y = backSolve(H, g, j);
-                K[r]->combine(Q + (size_t)r * n, j, y, PT ? W + (size_t)r * n : X + (size_t)r * n, PT ? 0 : 1, ld);
+                K[r]->combine(Q + LN.base[r], j, y, PT ? W + LN.base[r] : X + LN.base[r], PT ? 0 : 1, ld);
             } else if (PT)
-                hipCheck(hipMemsetAsync(W + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+                hipCheck(hipMemsetAsync(W + LN.base[r], 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
         if (PT) {  // X += P (Q y) of every lane, through W
-            d.applyPrecond(W, PT, nrhs, n, false, st);
+            d.applyPrecond(W, PT, nvec, n, false, st);
             for (int r = 0; r < nrhs; ++r)
-                if (steps[r] > 0) K[r]->combine(PT + (size_t)r * n, 1, std::vector<zc>{1.0}, X + (size_t)r * n, 1);
+                if (steps[r] > 0) K[r]->combine(PT + LN.base[r], 1, std::vector<zc>{1.0}, X + LN.base[r], 1);
         }
         if (breakdown) break;
     }
@@ -1040,15 +1081,16 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
 // with the FP32 far field, at most 30 corrections; each right-hand side's
 // inner tolerance is max(1e-4, tol / (2 residual)).
 static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
-                                          const cplx<double>* B, cplx<double>* X, int nrhs, int m, int maxit,
-                                          double tol, double* hist) {
+                                          const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m,
+                                          int maxit, double tol, double* hist) {
     const int64_t n = d.hp.src.n;
-    const int64_t ld = (int64_t)nrhs * n;
+    const int nrhs = LN.count, nvec = LN.nvec;
+    const int64_t ld = (int64_t)nvec * n;
     cplx<double>* R = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
     cplx<double>* D = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
     std::vector<SolveOut> out(nrhs);
     std::vector<double> bn(nrhs);
-    for (int r = 0; r < nrhs; ++r) bn[r] = K[r]->norm(B + (size_t)r * n);
+    for (int r = 0; r < nrhs; ++r) bn[r] = K[r]->norm(B + LN.base[r]);
     hipCheck(hipMemsetAsync(X, 0, (size_t)ld * sizeof(cplx<double>), st), "zero x");
     int total = 0;
     for (int outer = 0;; ++outer) {
@@ -1056,23 +1098,23 @@ static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique
         if (outer == 0)
             hipCheck(hipMemcpyAsync(R, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
         else {
-            applyMany(d, X, R, nrhs, st, false);
-            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + (size_t)r * n, R + (size_t)r * n, R + (size_t)r * n);
+            applyMany(d, X, R, nvec, st, false);
+            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + LN.base[r], R + LN.base[r], R + LN.base[r]);
         }
         bool all = true;
         std::vector<double> itol(nrhs, 1.0);
         for (int r = 0; r < nrhs; ++r) {
-            const double res = bn[r] > 0 ? K[r]->norm(R + (size_t)r * n) / bn[r] : 0.0;
+            const double res = bn[r] > 0 ? K[r]->norm(R + LN.base[r]) / bn[r] : 0.0;
             out[r].est = res;
             if (res <= tol) {  // converged: its inner right-hand side is zero
-                hipCheck(hipMemsetAsync(R + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+                hipCheck(hipMemsetAsync(R + LN.base[r], 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
                 continue;
             }
             all = false;
             itol[r] = std::max(1e-4, 0.5 * tol / res);
         }
         if (all || total >= maxit || outer == 30) break;
-        std::vector<SolveOut> in = gmresBatch(d, K, st, R, D, nrhs, m, maxit - total, itol, hist, true, total, maxit);
+        std::vector<SolveOut> in = gmresBatch(d, K, st, R, D, LN, m, maxit - total, itol, hist, true, total, maxit);
         int steps = 0;
         for (int r = 0; r < nrhs; ++r) {
             out[r].iters += in[r].iters;
@@ -1080,7 +1122,7 @@ static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique
             if (in[r].status == MLFMA_ERR_BREAKDOWN) out[r].status = MLFMA_ERR_BREAKDOWN;
             // X += D
             std::vector<zc> one{1.0};
-            K[r]->combine(D + (size_t)r * n, 1, one, X + (size_t)r * n, 1);
+            K[r]->combine(D + LN.base[r], 1, one, X + LN.base[r], 1);
         }
         total += std::max(steps, 1);
     }
@@ -1104,23 +1146,24 @@ static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique
 // with one batched FP64 matvec and one batched CGS2 per outer step.
 // iterations counts matvecs: the inner FP32 ones and the outer FP64 ones.
 static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
-                                    const cplx<double>* B, cplx<double>* X, int nrhs, int mOut, int mIn, int maxit,
-                                    double tol, double innerTol, double* hist, int histLd) {
+                                    const cplx<double>* B, cplx<double>* X, const Lanes& LN, int mOut, int mIn,
+                                    int maxit, double tol, double innerTol, double* hist, int histLd) {
     const int64_t n = d.hp.src.n;
-    const int64_t ld = (int64_t)nrhs * n;
+    const int nrhs = LN.count, nvec = LN.nvec;
+    const int64_t ld = (int64_t)nvec * n;
     mOut = std::max(1, mOut);
     mIn = std::max(1, mIn);
     cplx<double>* V = (cplx<double>*)d.alloc((size_t)ld * (mOut + 1) * sizeof(cplx<double>));
     cplx<double>* Zb = (cplx<double>*)d.alloc((size_t)ld * mOut * sizeof(cplx<double>));
     cplx<double>* W = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
     std::unique_ptr<BatchOrtho> BO;
-    if (nrhs > 1) BO.reset(new BatchOrtho(d, nrhs, mOut));
+    if (nrhs > 1) BO.reset(new BatchOrtho(d, LN, mOut));
     std::vector<cplx<double>> hh(nrhs > 1 ? (size_t)nrhs * BO->hs : 0);
     std::vector<SolveOut> out(nrhs);
     std::vector<double> bn(nrhs);
     std::vector<int> used(nrhs, 0);  // iterations (matvecs) per lane
     for (int r = 0; r < nrhs; ++r) {
-        bn[r] = K[r]->norm(B + (size_t)r * n);
+        bn[r] = K[r]->norm(B + LN.base[r]);
         if (!std::isfinite(bn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
     }
     hipCheck(hipMemsetAsync(X, 0, (size_t)ld * sizeof(cplx<double>), st), "zero x");
@@ -1129,7 +1172,7 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
         if (nrhs == 1)
             d.applySolveOp(x, y, 0, st, false, false);
         else
-            applyMany(d, x, y, nrhs, st, false);
+            applyMany(d, x, y, nvec, st, false);
     };
     int total = 0;  // matvecs of the lockstep (the most any lane has used)
     for (int cycle = 0;; ++cycle) {
@@ -1138,14 +1181,14 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
             hipCheck(hipMemcpyAsync(W, B, (size_t)ld * sizeof(cplx<double>), hipMemcpyDeviceToDevice, st), "copy");
         else {
             op64(X, W);
-            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + (size_t)r * n, W + (size_t)r * n, W + (size_t)r * n);
+            for (int r = 0; r < nrhs; ++r) K[r]->sub(B + LN.base[r], W + LN.base[r], W + LN.base[r]);
         }
         std::vector<double> beta(nrhs, 0.0);
         std::vector<char> active(nrhs, 0);
         bool anyActive = false;
         for (int r = 0; r < nrhs; ++r) {
             if (bn[r] == 0 || out[r].status == MLFMA_ERR_BREAKDOWN) continue;
-            beta[r] = cycle == 0 ? bn[r] : K[r]->norm(W + (size_t)r * n);
+            beta[r] = cycle == 0 ? bn[r] : K[r]->norm(W + LN.base[r]);
             out[r].est = beta[r] / bn[r];
             active[r] = out[r].est > tol && beta[r] > 0;
             anyActive = anyActive || active[r];
@@ -1153,9 +1196,9 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
         if (!anyActive || total >= maxit) break;
         for (int r = 0; r < nrhs; ++r) {
             if (active[r])
-                K[r]->scale(W + (size_t)r * n, beta[r], V + (size_t)r * n);
+                K[r]->scale(W + LN.base[r], beta[r], V + LN.base[r]);
             else
-                hipCheck(hipMemsetAsync(V + (size_t)r * n, 0, n * sizeof(cplx<double>), st), "zero lane");
+                hipCheck(hipMemsetAsync(V + LN.base[r], 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
         }
         // per lane: Hessenberg columns (rotated), rotations, rotated rhs g
         std::vector<std::vector<std::vector<zc>>> H(nrhs);
@@ -1176,7 +1219,7 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
             } else {
                 std::vector<double> itol(nrhs, innerTol);
                 std::vector<SolveOut> in =
-                    gmresBatch(d, K, st, Vj, Zj, nrhs, budget, budget, itol, hist, true, total, histLd, false);
+                    gmresBatch(d, K, st, Vj, Zj, LN, budget, budget, itol, hist, true, total, histLd, false);
                 for (int r = 0; r < nrhs; ++r) {
                     innerIts = std::max(innerIts, in[r].iters);
                     if (active[r]) used[r] += in[r].iters;
@@ -1197,9 +1240,9 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
             }
             bool any = false;
             for (int r = 0; r < nrhs; ++r) {
-                cplx<double>* Vn = V + (size_t)(j + 1) * ld + (size_t)r * n;
+                cplx<double>* Vn = V + (size_t)(j + 1) * ld + LN.base[r];
                 if (!active[r]) {
-                    hipCheck(hipMemsetAsync(Vn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                    hipCheck(hipMemsetAsync(Vn, 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
                     continue;
                 }
                 ++used[r];
@@ -1215,7 +1258,7 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
                 if (t == 0) {
                     out[r].status = MLFMA_ERR_BREAKDOWN;
                     active[r] = 0;
-                    hipCheck(hipMemsetAsync(Vn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                    hipCheck(hipMemsetAsync(Vn, 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
                     continue;
                 }
                 if (na == 0) {
@@ -1239,9 +1282,9 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
                 if (hist) hist[(size_t)r * histLd + std::min(total, std::max(histLd, 1)) - 1] = est;
                 if (est <= tol || wn[r] == 0) {
                     active[r] = 0;
-                    hipCheck(hipMemsetAsync(Vn, 0, n * sizeof(cplx<double>), st), "zero lane");
+                    hipCheck(hipMemsetAsync(Vn, 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
                 } else {
-                    K[r]->scale(W + (size_t)r * n, wn[r], Vn);
+                    K[r]->scale(W + LN.base[r], wn[r], Vn);
                     any = true;
                 }
             }
@@ -1251,7 +1294,7 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
         for (int r = 0; r < nrhs; ++r)
             if (steps[r] > 0) {
                 const std::vector<zc> y = backSolve(H[r], g[r], steps[r]);
-                K[r]->combine(Zb + (size_t)r * n, steps[r], y, X + (size_t)r * n, 1, ld);
+                K[r]->combine(Zb + LN.base[r], steps[r], y, X + LN.base[r], 1, ld);
             }
     }
     for (int r = 0; r < nrhs; ++r) {
@@ -1368,6 +1411,8 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
                 double* hist) {
     API_BEGIN
     DevicePlan& d = Pl(plan);
+    if (d.hp.forest)
+        throw Error(MLFMA_ERR_STATE, "a forest plan solves every scatterer separately: use mlfma_solve_batch");
     if (!x || !opts) throw Error(MLFMA_ERR_ARG, "null argument");
     if (!opts->use_voltage && !b) throw Error(MLFMA_ERR_ARG, "b is NULL and use_voltage = 0");
     if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
@@ -1444,7 +1489,8 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     else if (method == MLFMA_FGMRES_IR) {
         std::vector<std::unique_ptr<Krylov>> Kv;
         Kv.emplace_back(new Krylov(d, st, std::max(opts->restart > 0 ? opts->restart : 50, mOut) + 2));
-        o = fgmres(d, Kv, st, bt, xtree, 1, mOut, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol,
+        const Lanes one(d, 1);
+        o = fgmres(d, Kv, st, bt, xtree, one, mOut, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol,
                    opts->inner_tol > 0 ? opts->inner_tol : 1e-1, hist, opts->maxit)[0];
     } else if (method == MLFMA_GMRES_IR)
         o = gmresIR(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, hist);
@@ -1532,56 +1578,59 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     if (opts->restart > 2048) throw Error(MLFMA_ERR_ARG, "restart must be <= 2048");
     if (d.hp.staleBoxes > 0 && !(d.flags & MLFMA_FLAG_FRESH_LOCALS))
         throw Error(MLFMA_ERR_STATE, "batched solves need a linear operator (MLFMA_FLAG_FRESH_LOCALS)");
-    hipCheck(hipSetDevice(d.device), "hipSetDevice");
-    const auto t0 = std::chrono::steady_clock::now();
+    if (!std::isfinite(opts->shift_re) || !std::isfinite(opts->shift_im))
+        throw Error(MLFMA_ERR_NONFINITE, "non-finite shift");
+    if (opts->precond < 0 || opts->precond > 1) throw Error(MLFMA_ERR_ARG, "unknown preconditioner");
     const int64_t n = d.hp.src.n;
-    hipStream_t st = d.stream;
-    d.ensureBatch(nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1)));
-    const size_t bytes = (size_t)n * nrhs * sizeof(cplx<double>);
     for (size_t i = 0; i < (size_t)n * nrhs * 2; ++i)
         if (!std::isfinite(b[i])) throw Error(MLFMA_ERR_NONFINITE, "non-finite b");
+    hipCheck(hipSetDevice(d.device), "hipSetDevice");
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = d.stream;
+    struct SolveMode {  // cleared however the solve ends
+        DevicePlan& d;
+        ~SolveMode() {
+            d.solvePrec = false;
+            d.solveShift = cplx<double>{0, 0};
+        }
+    } solveMode{d};
+    d.solveShift = cplx<double>{opts->shift_re, opts->shift_im};
+    if (opts->precond) {
+        d.buildPrecond();
+        d.solvePrec = true;
+    }
+    d.ensureBatch(nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1)));
+    // one lane per right-hand side and scatterer (segment) of the plan
+    const Lanes LN(d, nrhs);
+    const size_t bytes = (size_t)n * nrhs * sizeof(cplx<double>);
     cplx<double>* B = (cplx<double>*)d.alloc(bytes);
     cplx<double>* X = (cplx<double>*)d.alloc(bytes);
     cplx<double>* R = (cplx<double>*)d.alloc(bytes);
     hipCheck(hipMemcpyAsync(B, b, bytes, hipMemcpyHostToDevice, st), "b upload");
     const int m = opts->restart > 0 ? opts->restart : 50;
-    std::vector<std::unique_ptr<Krylov>> K;
     const int mOut = opts->outer_restart > 0 ? opts->outer_restart : 30;
-    struct PrecMode {  // cleared however the solve ends
-        DevicePlan& d;
-        ~PrecMode() {
-            d.solvePrec = false;
-            d.solveShift = cplx<double>{0, 0};
-        }
-    } precMode{d};
-    if (!std::isfinite(opts->shift_re) || !std::isfinite(opts->shift_im))
-        throw Error(MLFMA_ERR_NONFINITE, "non-finite shift");
-    d.solveShift = cplx<double>{opts->shift_re, opts->shift_im};
-    if (opts->precond < 0 || opts->precond > 1) throw Error(MLFMA_ERR_ARG, "unknown preconditioner");
-    if (opts->precond) {
-        d.buildPrecond();
-        d.solvePrec = true;
-    }
-    for (int r = 0; r < nrhs; ++r) K.emplace_back(new Krylov(d, st, std::max(std::min(m, opts->maxit), mOut) + 2));
+    std::vector<std::unique_ptr<Krylov>> K;
+    for (int k = 0; k < LN.count; ++k)
+        K.emplace_back(new Krylov(d, st, std::max(std::min(m, opts->maxit), mOut) + 2, LN.len[k]));
     std::vector<SolveOut> o =
         opts->method == MLFMA_FGMRES_IR
-            ? fgmres(d, K, st, B, X, nrhs, mOut, m, opts->maxit, opts->tol,
+            ? fgmres(d, K, st, B, X, LN, mOut, m, opts->maxit, opts->tol,
                      opts->inner_tol > 0 ? opts->inner_tol : 1e-1, hist, opts->maxit)
         : opts->method == MLFMA_GMRES_IR
-            ? gmresBatchIR(d, K, st, B, X, nrhs, m, opts->maxit, opts->tol, hist)
-            : gmresBatch(d, K, st, B, X, nrhs, m, opts->maxit, std::vector<double>(nrhs, opts->tol), hist);
+            ? gmresBatchIR(d, K, st, B, X, LN, m, opts->maxit, opts->tol, hist)
+            : gmresBatch(d, K, st, B, X, LN, m, opts->maxit, std::vector<double>(LN.count, opts->tol), hist);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    // true residuals with the FP64 matvec
+    // true residuals with the FP64 matvec, per lane
     applyMany(d, X, R, nrhs, st);
-    for (int r = 0; r < nrhs; ++r) {
-        K[r]->sub(B + (size_t)r * n, R + (size_t)r * n, R + (size_t)r * n);
-        const double rn = K[r]->norm(R + (size_t)r * n), bn = K[r]->norm(B + (size_t)r * n);
+    for (int k = 0; k < LN.count; ++k) {
+        K[k]->sub(B + LN.base[k], R + LN.base[k], R + LN.base[k]);
+        const double rn = K[k]->norm(R + LN.base[k]), bn = K[k]->norm(B + LN.base[k]);
         if (reps) {
-            reps[r].iterations = o[r].iters;
-            reps[r].status = o[r].status;
-            reps[r].resid_est = o[r].est;
-            reps[r].resid_true = bn > 0 ? rn / bn : 0.0;
-            reps[r].seconds = secs;
+            reps[k].iterations = o[k].iters;
+            reps[k].status = o[k].status;
+            reps[k].resid_est = o[k].est;
+            reps[k].resid_true = bn > 0 ? rn / bn : 0.0;
+            reps[k].seconds = secs;
         }
     }
     hipCheck(hipMemcpyAsync(x, X, bytes, hipMemcpyDeviceToHost, st), "x download");

@@ -67,6 +67,36 @@ class Plan:
                                                     C.byref(self._h)))
         self.n = geom.n
 
+    @classmethod
+    def forest(cls, geoms, cfgs, pw=None):
+        """One plan for several independent scatterers (mlfma_plan_create_forest):
+        every matvec stage runs once for all of them; the unknowns of
+        scatterer s follow those of s - 1 (segments())."""
+        self = cls.__new__(cls)
+        self.lib = A.load_library()
+        self.geoms = list(geoms)
+        self.geom = None
+        self.pw = pw or A.make_planewave()
+        cfgs = list(cfgs)
+        if len(cfgs) != len(self.geoms):
+            raise ValueError("one configuration per scatterer")
+        self.cfg = cfgs[0]
+        garr = (A.Geometry * len(self.geoms))(*[g.struct for g in self.geoms])
+        carr = (A.Config * len(cfgs))(*cfgs)
+        self._h = C.c_void_p()
+        _check(self.lib, self.lib.mlfma_plan_create_forest(garr, carr, len(cfgs), C.byref(self.pw), C.byref(self._h)))
+        self.n = sum(g.n for g in self.geoms)
+        return self
+
+    def segments(self) -> np.ndarray:
+        """Unknown offsets of the plan's scatterers (count + 1 values)."""
+        cnt = C.c_int32(0)
+        _check(self.lib, self.lib.mlfma_plan_segments(self._h, None, 0, C.byref(cnt)))
+        off = np.zeros(cnt.value + 1, dtype=np.int64)
+        _check(self.lib, self.lib.mlfma_plan_segments(self._h, off.ctypes.data_as(C.POINTER(C.c_int64)), len(off),
+                                                      C.byref(cnt)))
+        return off
+
     def close(self):
         if self._h and self._h.value:
             self.lib.mlfma_plan_destroy(self._h)
@@ -272,12 +302,14 @@ class Plan:
         """GMRES(restart) on the rows of B (nrhs, n) at once, one batched
         matvec per step (method MLFMA_FGMRES_IR on an FP32 plan: FP64
         flexible GMRES around batched FP32-far-field GMRES; MLFMA_GMRES_IR:
-        FP64 refinement around it).  Returns (X, [report per right-hand
-        side])."""
+        FP64 refinement around it).  Returns (X, reports): one report per
+        right-hand side and scatterer (lane r * segments + s; one scatterer
+        unless the plan is a forest)."""
         B = np.ascontiguousarray(B, dtype=np.complex128)
         if B.ndim != 2 or B.shape[1] != self.n:
             raise ValueError("B must be (nrhs, n)")
         nrhs = B.shape[0]
+        nseg = len(self.segments()) - 1
         o = A.SolveOpts()
         o.method = method
         o.restart = restart
@@ -288,13 +320,13 @@ class Plan:
         o.inner_tol = inner_tol
         o.precond = precond
         o.shift_re, o.shift_im = complex(shift).real, complex(shift).imag
-        reps = (A.SolveReport * nrhs)()
+        reps = (A.SolveReport * (nrhs * nseg))()
         X = np.zeros_like(B)
-        hist = np.zeros((nrhs, maxit))
+        hist = np.zeros((nrhs * nseg, maxit))
         _check(self.lib, self.lib.mlfma_solve_batch(self._h, B.ctypes.data_as(A.dptr), X.ctypes.data_as(A.dptr),
                                                     nrhs, C.byref(o), reps, hist.ctypes.data_as(A.dptr)))
         out = []
-        for r in range(nrhs):
+        for r in range(nrhs * nseg):
             rep = reps[r]
             d = {"iterations": rep.iterations, "status": A.ERRORS.get(rep.status, rep.status),
                  "resid_est": rep.resid_est, "resid_true": rep.resid_true, "seconds": rep.seconds}
