@@ -415,6 +415,88 @@ def multi_scatterer_job():
     print(json.dumps(out), flush=True)
 
 
+def m2l_ceiling(args):
+    """Cache-resident gather ceiling for M2L: per level, each target sums the
+    fields of its interaction-list sources (32 B per direction) and the
+    operators of the pair offsets (16 B per direction), read in the lists'
+    order, through torch.nn.functional.embedding_bag (gather and sum, no
+    product), timed with CUDA events.  The bytes are the M2L's streamed bytes
+    (source field and operator read per pair); the rate is what this access
+    pattern reaches from the caches, the reference for kM2Lw / kM2LNear."""
+    import torch
+    import torch.nn.functional as Fn
+    from mlfma_amd import _capi as A
+    from mlfma_amd.plan import Plan
+    geom, pw, kw, desc = workload(args.workload)
+    plan = Plan(geom, pw, A.make_config(device=0, **kw))
+    info = plan.info()
+    dev = torch.device("cuda:0")
+    out = {"mode": "m2l-ceiling", "workload": desc["workload"], "levels": {}}
+    tot_b, tot_ms = 0, 0.0
+    for l in range(2, info["max_level"] + 1):
+        pairs, _ = plan.level_pairs(l, 0)
+        co, _, ns = plan.level_boxes(l)
+        nbox = len(co)
+        # pairs between non-empty boxes: the M2L worklists hold only those
+        pairs = pairs[(ns[pairs[:, 0]] > 0) & (ns[pairs[:, 1]] > 0)] if len(pairs) else pairs
+        if len(pairs) == 0:
+            continue
+        nd = info["nth"][l] * info["nph"][l]
+        order = np.argsort(pairs[:, 0], kind="stable")
+        tgt, src = pairs[order, 0], pairs[order, 1]
+        ut, counts = np.unique(tgt, return_counts=True)
+        offsets = torch.from_numpy(np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)).to(dev)
+        M = torch.randn(nbox, 4 * nd, dtype=torch.float64, device=dev)     # 32 B per direction
+        T = torch.randn(316, 2 * nd, dtype=torch.float64, device=dev)      # 16 B per direction
+        si = torch.from_numpy(src.astype(np.int64)).to(dev)
+        oi = torch.from_numpy((np.arange(len(src)) % 316).astype(np.int64)).to(dev)
+        for _ in range(3):
+            Fn.embedding_bag(si, M, offsets, mode="sum")
+            Fn.embedding_bag(oi, T, offsets, mode="sum")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record()
+        for _ in range(reps):
+            Fn.embedding_bag(si, M, offsets, mode="sum")
+            Fn.embedding_bag(oi, T, offsets, mode="sum")
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        b = len(src) * nd * (32 + 16)
+        tot_b += b
+        tot_ms += ms
+        out["levels"][l] = {"pairs": int(len(src)), "directions": nd, "streamed_bytes": int(b), "ms": round(ms, 4),
+                            "GBps": round(b / (ms * 1e-3) / 1e9, 1)}
+    out["total"] = {"streamed_bytes": int(tot_b), "ms": round(tot_ms, 4),
+                    "GBps": round(tot_b / (tot_ms * 1e-3) / 1e9, 1)}
+    print(json.dumps(out), flush=True)
+
+
+def hbm_rates():
+    """What torch's own streaming kernels reach on this device: a 1 GiB
+    device copy (read + write) and a sum over 2 GiB (read only), timed with
+    CUDA events over 20 launches each."""
+    import torch
+    dev = torch.device("cuda:0")
+    a = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device=dev)      # 1 GiB
+    b = torch.empty_like(a)
+    c = torch.ones(256 * 1024 * 1024, dtype=torch.float64, device=dev)      # 2 GiB
+    out = {"mode": "hbm"}
+    for name, fn, nbytes in [("copy_1GiB", lambda: b.copy_(a), 2 * a.numel() * 8),
+                             ("sum_2GiB", lambda: c.sum(), c.numel() * 8)]:
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        out[name] = {"bytes": nbytes, "ms": round(ms, 4), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    print(json.dumps(out), flush=True)
+
+
 def kernel_bytes(group, info, n):
     """Algorithmic bytes per launch of the kernel of a launch group (FP64,
     c = 16 B): the per-unit figures times the units one launch processes.
@@ -465,8 +547,11 @@ def main():
                          "of autotuning when present (profiled runs then launch only the timed schedule)")
     ap.add_argument("--timed-only", action="store_true",
                     help="only the timed matvecs and their probe passes (one schedule; for kernel profiles)")
-    ap.add_argument("--mode", default="matvec", choices=["matvec", "copy"],
-                    help="copy: only run a plain 256 MiB device copy (HBM counter calibration)")
+    ap.add_argument("--mode", default="matvec", choices=["matvec", "copy", "m2l-ceiling", "hbm"],
+                    help="copy: only run a plain 256 MiB device copy (HBM counter calibration); m2l-ceiling: the "
+                         "M2L access pattern (per target, the sum of its sources' fields and operators) through "
+                         "torch's gather-sum (embedding_bag) on the workload's interaction lists; hbm: torch's "
+                         "copy and sum rates on GiB arrays")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the library runs the matvec on one GPU and has no multi-GPU "
@@ -486,6 +571,14 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"mode": "copy", "bytes_read_per_launch": src.numel() * 8,
                           "bytes_written_per_launch": src.numel() * 8, "launches": 10}))
+        return
+
+    if args.mode == "m2l-ceiling":
+        m2l_ceiling(args)
+        return
+
+    if args.mode == "hbm":
+        hbm_rates()
         return
 
     if args.workload == "dipoles200k_grad":
@@ -518,7 +611,8 @@ def main():
                 json.dump({"workload": desc["workload"], "record": plan.tuning()}, f)
     schedule = {"near_overlap": plan.get_param(A.MLFMA_PARAM_NEAR_OVERLAP),
                 "m2l_split": plan.get_param(A.MLFMA_PARAM_M2L_SPLIT),
-                "near_window": plan.get_param(A.MLFMA_PARAM_NEAR_WINDOW)}
+                "near_window": plan.get_param(A.MLFMA_PARAM_NEAR_WINDOW),
+                "near_blocks_per_cu": plan.get_param(A.MLFMA_PARAM_NEAR_BLOCKS)}
     log("autotune ms", tune_before, "->", tune_after, schedule)
     plan.set_flags(A.MLFMA_FLAG_ASYNC)
     for _ in range(args.warmup):

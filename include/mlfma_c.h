@@ -371,7 +371,7 @@ int mlfma_stage_times_detail(const mlfma_plan* plan, double* ms, int32_t cap, in
  * target of the M2M (and adjoint L2L) row split, per level the sibling-group or per-target
  * M2L kernel, whether the near-field pair products run beside the far-field
  * chain (on a second stream of the plan, forked before or after S2M or after
- * the M2M chain) or in order (one right-hand side: in the launch of the
+ * the M2M chain; one block per work item or 1-4 resident blocks per CU) or in order (one right-hand side: in the launch of the
  * per-target M2L levels, over an early share of its blocks), and whether the finest level's M2L runs beside the M2M chain (on a
  * third stream) or in line; with nrhs = 1 also whether mlfma_solve runs its
  * matvecs in order (whichever of the two makes short GMRES runs on the
@@ -389,13 +389,17 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
  *                              chain forked before S2M, 2 after S2M, 3 after M2M
  *   MLFMA_PARAM_M2L_SPLIT      1: the finest level's M2L on its own stream
  *   MLFMA_PARAM_SOLVE_IN_ORDER 1: one-right-hand-side solves run every stage
- *                              in order on one stream */
+ *                              in order on one stream
+ *   MLFMA_PARAM_NEAR_BLOCKS    0..8: near-field pairs beside the chain run
+ *                              as that many resident blocks per CU looping
+ *                              over the work items (0: one block per item) */
 #define MLFMA_PARAM_NEAR_WINDOW 1
 #define MLFMA_PARAM_NEAR_OVERLAP 2
 #define MLFMA_PARAM_M2L_SPLIT 3
 #define MLFMA_PARAM_SOLVE_IN_ORDER 4
 #define MLFMA_PARAM_PROBE_GROUP 5  /* MLFMA_GROUP_* whose launches carry probe
                                       events (0: none); see mlfma_probe_read */
+#define MLFMA_PARAM_NEAR_BLOCKS 6
 int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value);
 int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value);
 

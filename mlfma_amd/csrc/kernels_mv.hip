@@ -1018,6 +1018,23 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
     nearSymBody<CONJ, V>(blockIdx.x, items, colsIn, perm, val, x, part, sT, sxr, sxc);
 }
 
+// The same with a fixed number of blocks, each taking items blockIdx.x,
+// blockIdx.x + gridDim.x, ...: launched beside the far-field chain, the near
+// field then holds only that many of a CU's LDS and wave slots, so that the
+// chain's blocks are not kept waiting for the whole near field to drain.
+template <bool CONJ, typename V>
+__global__ __launch_bounds__(TB) void kNearSymP(const NearItem* __restrict__ items, int nitems,
+                                                const int32_t* __restrict__ colsIn, const int64_t* __restrict__ perm,
+                                                const cplx<V>* __restrict__ val, const cplx<double>* __restrict__ x,
+                                                cplx<double>* __restrict__ part) {
+    __shared__ cplx<V> sT[NT * (sizeof(V) == 4 ? 2 : 1)];
+    __shared__ cplx<double> sxr[128], sxc[TB];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        nearSymBody<CONJ, V>(it, items, colsIn, perm, val, x, part, sT, sxr, sxc);
+        __syncthreads();  // the item's LDS reads are done before the next item's stores
+    }
+}
+
 // The same for NB >= 2 right-hand sides (x column stride xs, partial sums of
 // right-hand side r at part + r ps): each value is read once and used 2 NB
 // times.  Items have at most 128 rows; tiles of 4 values per lane.
@@ -1195,8 +1212,17 @@ __global__ __launch_bounds__(TB) void kNearSum(const int64_t* __restrict__ rows,
 template <typename V>
 void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
                    const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb,
-                   int64_t xs, int64_t ps) {
+                   int64_t xs, int64_t ps, int grid) {
     if (nitems <= 0) return;
+    if (nb == 1 && grid > 0 && grid < nitems) {
+        if (conjVal)
+            hipLaunchKernelGGL((kNearSymP<true, V>), dim3(grid), dim3(TB), 0, st, items, nitems, colsIn, perm, val, x,
+                               part);
+        else
+            hipLaunchKernelGGL((kNearSymP<false, V>), dim3(grid), dim3(TB), 0, st, items, nitems, colsIn, perm, val,
+                               x, part);
+        return;
+    }
 #define NEARB_CASE(C, B)                                                                                      \
     hipLaunchKernelGGL((kNearSymB<C, B, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, x, xs, part, \
                        ps)
@@ -1216,9 +1242,9 @@ void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_
 #undef NEARB_CASE
 }
 template void launchNearSym<double>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<double>*,
-                                    const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t);
+                                    const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t, int);
 template void launchNearSym<float>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<float>*,
-                                   const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t);
+                                   const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t, int);
 
 // The per-target M2L of one right-hand side and the near-field pairs in one
 // launch (stages in order on one stream; CONJ: the adjoint's conjugated

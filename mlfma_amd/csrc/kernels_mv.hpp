@@ -73,7 +73,7 @@ bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32
 template <typename V>
 void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
                    const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb = 1,
-                   int64_t xs = 0, int64_t ps = 0);
+                   int64_t xs = 0, int64_t ps = 0, int grid = 0);
 // the per-target M2L of one right-hand side and the near-field pairs in one
 // launch; conj: conjugated operators (lv holds the reversed pairs' offsets)
 // and conjugated values (the adjoint); windowPct: the pair blocks spread over

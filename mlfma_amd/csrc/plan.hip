@@ -155,6 +155,7 @@ void DevicePlan::build() {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
             ldsMax = (size_t)v;
+        hipCheck(hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
     }
     // stage events need no system-scope fence: they order work of one stream
     for (auto& e : ev) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
@@ -903,11 +904,14 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         } end{this, s};
         // the FP32 far field (mixed-precision inner solves) reads the FP32
         // copy of the near-field values
+        // beside the chain, a fixed number of blocks (nearBlocks per CU)
+        const int grid = s == nearStream ? nearGrid() : 0;
         if (std::is_same<R, float>::value && nearVal32)
             launchNearSym<float>(false, nearItems, nNearItems, nearColsIn, perm, nearVal32, x, nearPart, s, nb, xs,
-                                 ps);
+                                 ps, grid);
         else
-            launchNearSym<double>(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, s, nb, xs, ps);
+            launchNearSym<double>(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, s, nb, xs, ps,
+                                  grid);
     };
     if (overlap == 1) {
         hipCheck(hipEventRecord(evNear, st), "event");
@@ -1024,7 +1028,8 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
     if (overlap) {
         hipCheck(hipEventRecord(evFork, st), "event");
         hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
-        launchNearSym<double>(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, nearStream);
+        launchNearSym<double>(true, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, nearStream, 1, 0, 0,
+                              nearGrid());
         hipCheck(hipEventRecord(evJoin, nearStream), "event");
     }
     launchZeroField<R>(M, fieldLen, st);
@@ -1530,27 +1535,35 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
             }
         }
         // near field beside the far-field chain, after it or (mode 0) in the
-        // M2L launch, and the finest-level M2L beside the M2M chain or in
-        // line: the eight combinations, since the in-order modes pay only
-        // when both side streams are off
+        // M2L launch, the finest-level M2L beside the M2M chain or in line,
+        // and, beside the chain, the near field's blocks per CU (one per
+        // item, or 1-4 resident blocks per CU that loop over the items):
+        // every combination, since the in-order modes pay only when both
+        // side streams are off
         if (!(saved & MLFMA_FLAG_ONE_STREAM)) {
-            double tm[4][2] = {};
+            constexpr int NNB = 5;
+            double tm[4][2][NNB] = {};
             for (int rep = 0; rep < 2; ++rep)
                 for (int m = 0; m < 4; ++m)
-                    for (int sp = 0; sp < 2; ++sp) {
-                        d.nearOverlap = m;
-                        d.m2lSplit = sp == 1;
-                        tm[m][sp] += timeWhole(10);
-                    }
-            int bm = 2, bs = 1;
+                    for (int sp = 0; sp < 2; ++sp)
+                        for (int nbk = 0; nbk < (m ? NNB : 1); ++nbk) {
+                            d.nearOverlap = m;
+                            d.m2lSplit = sp == 1;
+                            d.nearBlocks = nbk;
+                            tm[m][sp][nbk] += timeWhole(10);
+                        }
+            int bm = 2, bs = 1, bk = 0;
             for (int m = 0; m < 4; ++m)
                 for (int sp = 0; sp < 2; ++sp)
-                    if (tm[m][sp] < tm[bm][bs]) {
-                        bm = m;
-                        bs = sp;
-                    }
+                    for (int nbk = 0; nbk < (m ? NNB : 1); ++nbk)
+                        if (tm[m][sp][nbk] < tm[bm][bs][bk]) {
+                            bm = m;
+                            bs = sp;
+                            bk = nbk;
+                        }
             d.nearOverlap = bm;
             d.m2lSplit = bs == 1;
+            d.nearBlocks = bk;
             // how early the pair blocks of the shared M2L / near-field launch
             // end; that launch runs only for one right-hand side with the
             // stages in order (the in-order matvec mode, or solves run in order)
@@ -1650,6 +1663,10 @@ int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value) {
             if (value != 0 && value != 1) throw Error(MLFMA_ERR_ARG, "solve order must be 0 or 1");
             d.solveSerial = value == 1;
             break;
+        case MLFMA_PARAM_NEAR_BLOCKS:
+            if (value < 0 || value > 8) throw Error(MLFMA_ERR_ARG, "near blocks per CU must be 0..8");
+            d.nearBlocks = value;
+            break;
         case MLFMA_PARAM_PROBE_GROUP:
             if (value < 0 || value > MLFMA_GROUP_LAST) throw Error(MLFMA_ERR_ARG, "unknown launch group");
             d.probeDrain(0);
@@ -1675,22 +1692,25 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value) 
         case MLFMA_PARAM_M2L_SPLIT: *value = d.m2lSplit ? 1 : 0; break;
         case MLFMA_PARAM_SOLVE_IN_ORDER: *value = d.solveSerial ? 1 : 0; break;
         case MLFMA_PARAM_PROBE_GROUP: *value = d.probeGroup; break;
+        case MLFMA_PARAM_NEAR_BLOCKS: *value = d.nearBlocks; break;
         default: throw Error(MLFMA_ERR_ARG, "unknown parameter");
     }
     return MLFMA_OK;
     API_END
 }
 
-// Tuning record: [version, nearOverlap, m2lSplit, nearWindow, solveSerial,
-// levels L, tuneMinBlocks[0..L), tuneM2L[0..L)]
+// Tuning record: [version 2, nearOverlap, m2lSplit, nearWindow, solveSerial,
+// levels L, tuneMinBlocks[0..L), tuneM2L[0..L), nearBlocks]; version 1 is the
+// same without nearBlocks (0)
 int mlfma_autotune_record(const mlfma_plan* plan, int32_t* rec, int32_t cap, int32_t* len) {
     API_BEGIN
     const DevicePlan& d = P(plan);
     if (!len) throw Error(MLFMA_ERR_ARG, "null argument");
     const int L = d.hp.tree.maxLevel + 1;
-    std::vector<int32_t> r{1, d.nearOverlap, d.m2lSplit ? 1 : 0, d.nearWindow, d.solveSerial ? 1 : 0, L};
+    std::vector<int32_t> r{2, d.nearOverlap, d.m2lSplit ? 1 : 0, d.nearWindow, d.solveSerial ? 1 : 0, L};
     for (int l = 0; l < L; ++l) r.push_back(l < (int)d.tuneMinBlocks.size() ? d.tuneMinBlocks[l] : 0);
     for (int l = 0; l < L; ++l) r.push_back(l < (int)d.tuneM2L.size() ? d.tuneM2L[l] : 0);
+    r.push_back(d.nearBlocks);
     *len = (int32_t)r.size();
     if (rec)
         for (int32_t i = 0; i < std::min<int32_t>(cap, *len); ++i) rec[i] = r[i];
@@ -1703,10 +1723,11 @@ int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
     DevicePlan& d = P(plan);
     needDevice(d);
     const int L = d.hp.tree.maxLevel + 1;
-    if (!rec || len != 6 + 2 * L || rec[0] != 1 || rec[5] != L)
-        throw Error(MLFMA_ERR_ARG, "tuning record does not match this plan");
+    const bool v1 = rec && len == 6 + 2 * L && rec[0] == 1, v2 = rec && len == 7 + 2 * L && rec[0] == 2;
+    if ((!v1 && !v2) || rec[5] != L) throw Error(MLFMA_ERR_ARG, "tuning record does not match this plan");
+    const int nearBlocks = v2 ? rec[6 + 2 * L] : 0;
     if (rec[1] < 0 || rec[1] > 3 || rec[2] < 0 || rec[2] > 1 || rec[3] < 1 || rec[3] > 100 || rec[4] < 0 ||
-        rec[4] > 1)
+        rec[4] > 1 || nearBlocks < 0 || nearBlocks > 8)
         throw Error(MLFMA_ERR_ARG, "tuning record out of range");
     for (int l = 0; l < L; ++l)
         if (rec[6 + l] < 0 || rec[6 + L + l] < 0 || rec[6 + L + l] > 2)
@@ -1717,6 +1738,7 @@ int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
     d.solveSerial = rec[4] == 1;
     d.tuneMinBlocks.assign(rec + 6, rec + 6 + L);
     d.tuneM2L.assign(rec + 6 + L, rec + 6 + 2 * L);
+    d.nearBlocks = nearBlocks;
     d.clearGraphs();
     return MLFMA_OK;
     API_END

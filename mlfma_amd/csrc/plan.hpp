@@ -96,6 +96,9 @@ struct DevicePlan {
     hipEvent_t evM2L = nullptr, evNear = nullptr;
     bool m2lSplit = true;              // chosen by mlfma_autotune
     int nearWindow = 100;              // % of the shared M2L/near launch the pair blocks spread over; autotuned
+    int nearBlocks = 0;                // near field on a side stream: blocks per CU (0: one per item); autotuned
+    int numCU = 0;                     // compute units of the device
+    int nearGrid() const { return nearBlocks > 0 ? nearBlocks * numCU : 0; }
     bool timesValid = false;
     uint32_t flags = 0;  // MLFMA_FLAG_* in effect for matvecs (settable after creation)
     std::vector<void*> allocs;

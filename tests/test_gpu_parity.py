@@ -324,6 +324,45 @@ def test_stream_overlap_matches_one_stream():
 
 
 @pytest.mark.parametrize("fp32", [False, True])
+def test_side_stream_schedules_match_one_stream(fp32):
+    """Every schedule the autotuner chooses among (near-field pairs forked
+    before S2M, after S2M or after the M2M chain; the finest level's M2L
+    beside the chain or in line; the near field as one block per work item or
+    as 1-4 resident blocks per CU looping over the items) gives the bits of
+    the in-order run, forward and adjoint, on the 30,720-RWG sphere (about
+    ten thousand near-field work items, so the looping launches take many
+    items per block)."""
+    import torch
+    from mlfma_amd import configs as CF
+    geom, pw, kw, _ = CF.c2()
+    flags = A.MLFMA_FLAG_FRESH_LOCALS
+    prec = A.MLFMA_FP32 if fp32 else A.MLFMA_FP64
+    p = Plan(geom, pw, A.make_config(device=0, precision=prec, flags=flags, **kw))
+    x = torch.from_numpy(G.complex_gaussian(geom.n, 35)).to("cuda:0")
+    p.set_flags(flags | A.MLFMA_FLAG_ONE_STREAM)
+    y0, z0 = p.matvec(x).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()
+    p.set_flags(flags)
+    for overlap in (1, 2, 3):
+        for split in (0, 1):
+            for nbk in (0, 1, 2, 4):
+                p.set_param(A.MLFMA_PARAM_NEAR_OVERLAP, overlap)
+                p.set_param(A.MLFMA_PARAM_M2L_SPLIT, split)
+                p.set_param(A.MLFMA_PARAM_NEAR_BLOCKS, nbk)
+                assert p.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == nbk
+                y1, z1 = p.matvec(x).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()
+                say("fp32", fp32, "overlap", overlap, "split", split, "near blocks/CU", nbk, cases.relerr(y1, y0),
+                    cases.relerr(z1, z0))
+                assert cases.relerr(y1, y0) == 0.0 and cases.relerr(z1, z0) == 0.0
+    # the record carries the near-field blocks and applies to a second plan
+    rec = p.tuning()
+    assert rec[0] == 2 and rec[-1] == 4
+    q = Plan(geom, pw, A.make_config(device=0, precision=prec, flags=flags, **kw))
+    q.apply_tuning(rec)
+    assert q.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == 4
+    assert cases.relerr(q.matvec(x).cpu().numpy(), y0) == 0.0
+
+
+@pytest.mark.parametrize("fp32", [False, True])
 @pytest.mark.parametrize("m2l", [0, A.MLFMA_FLAG_M2L_TARGET, A.MLFMA_FLAG_M2L_GROUP])
 def test_one_stream_near_in_m2l_launch(fp32, m2l):
     """In order on one stream, the near-field pairs of one right-hand side
