@@ -53,14 +53,12 @@ __device__ __forceinline__ float wsum(float v) {
 // XCD b % 8; with a grid of 8 * per blocks XCD x gets items [x*per, (x+1)*per).
 __device__ __forceinline__ int xcdItem(int b, int per) { return (b & 7) * per + (b >> 3); }
 
-// The same for a grid of 8 * per * nch blocks where each item has nch
-// chunks: XCD x gets items [x*per, (x+1)*per) with all their chunks, so the
-// chunks of one item share that XCD's L2.
-__device__ __forceinline__ void xcdItemChunk(int b, int per, int nch, int& item, int& chunk) {
-    const int r = b >> 3;
-    const int q = r / nch;
-    item = (b & 7) * per + q;
-    chunk = r - q * nch;
+// The value lane l (wave-uniform) holds, read by every lane (two v_readlane).
+__device__ __forceinline__ double readlaneD(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 }  // namespace mlfma

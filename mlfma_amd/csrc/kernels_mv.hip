@@ -26,41 +26,71 @@ constexpr int TB = 256;
 // ------------------------------------------------------------------ S2M
 // M_b[d] = sum_s x_s rp_b[s][d]  (src/fmm/leaf.cpp:212-230).  Block = one leaf
 // x blockDim directions; lanes stream consecutive 32-byte (theta, phi) pairs
-// of one source row.  NB right-hand sides share every pattern read: x has
-// column stride xs (input order), the fields of right-hand side r start at
-// M + r fs.  One right-hand side keeps two source rows in flight instead.
+// of one source row.  Blocks take the leaves in the order `order` (most
+// sources first, so that the longest leaves do not form the launch's tail).
+// One right-hand side: each wave gathers the x values of 64 rows at a time
+// (lane i: row i) and reads them back with v_readlane, so the row loop has no
+// dependent loads; four rows' pattern loads are in flight per step (even rows
+// add into a, odd rows into b).  Lanes past the leaf's direction count read a
+// clamped direction and store nothing.  NB right-hand sides share every
+// pattern read: x has column stride xs (input order), the fields of
+// right-hand side r start at M + r fs.
 template <typename R, int NB>
-__global__ void kS2Mx(const LeafItem* __restrict__ items, int nitems, int per, const cplx<R>* __restrict__ rp,
-                      const cplx<double>* __restrict__ x, int64_t xs, const int64_t* __restrict__ perm,
-                      cplx<R>* __restrict__ M, int64_t fs) {
-    const int item = xcdItem(blockIdx.x, per);
-    if (item >= nitems) return;
-    const LeafItem it = items[item];
+__global__ void kS2Mx(const LeafItem* __restrict__ items, const int32_t* __restrict__ order,
+                      const cplx<R>* __restrict__ rp, const cplx<double>* __restrict__ x, int64_t xs,
+                      const int64_t* __restrict__ perm, cplx<R>* __restrict__ M, int64_t fs) {
+    const LeafItem it = items[order[blockIdx.x]];
     const int d = blockIdx.y * blockDim.x + threadIdx.x;
-    if (d >= it.ndir) return;
-    const cplx<R>* p = rp + it.rpOff + 2 * d;
-    const int64_t stride = (int64_t)it.ndir * 2;
     if constexpr (NB == 1) {
+        if (blockIdx.y * blockDim.x >= it.ndir) return;  // uniform per block
+        const int lane = threadIdx.x & 63;
+        const int dc = d < it.ndir ? d : it.ndir - 1;
+        const int64_t stride = (int64_t)it.ndir * 2;
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
-        int q = 0;
-        for (; q + 1 < it.cnt; q += 2) {
-            const cplx<R> x0 = cast<R>(x[perm[it.sbeg + q]]);
-            const cplx<R> x1 = cast<R>(x[perm[it.sbeg + q + 1]]);
-            const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
-            const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
-            a0 = a0 + x0 * r00;
-            a1 = a1 + x0 * r01;
-            b0 = b0 + x1 * r10;
-            b1 = b1 + x1 * r11;
+        for (int q0 = 0; q0 < it.cnt; q0 += 64) {
+            const int nq = min(64, it.cnt - q0);
+            const cplx<double> xl = lane < nq ? x[perm[it.sbeg + q0 + lane]] : mk(0.0, 0.0);
+            const cplx<R>* p = rp + it.rpOff + 2 * dc + q0 * stride;
+            auto xq = [&](int q) { return cast<R>(mk(readlaneD(xl.re, q), readlaneD(xl.im, q))); };
+            int q = 0;
+            for (; q + 3 < nq; q += 4) {
+                const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
+                const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
+                const cplx<R> r20 = p[(q + 2) * stride], r21 = p[(q + 2) * stride + 1];
+                const cplx<R> r30 = p[(q + 3) * stride], r31 = p[(q + 3) * stride + 1];
+                const cplx<R> x0 = xq(q), x1 = xq(q + 1), x2 = xq(q + 2), x3 = xq(q + 3);
+                a0 = a0 + x0 * r00;
+                a1 = a1 + x0 * r01;
+                b0 = b0 + x1 * r10;
+                b1 = b1 + x1 * r11;
+                a0 = a0 + x2 * r20;
+                a1 = a1 + x2 * r21;
+                b0 = b0 + x3 * r30;
+                b1 = b1 + x3 * r31;
+            }
+            for (; q + 1 < nq; q += 2) {
+                const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
+                const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
+                const cplx<R> x0 = xq(q), x1 = xq(q + 1);
+                a0 = a0 + x0 * r00;
+                a1 = a1 + x0 * r01;
+                b0 = b0 + x1 * r10;
+                b1 = b1 + x1 * r11;
+            }
+            if (q < nq) {  // only in the last run of rows (the others hold 64)
+                const cplx<R> x0 = xq(q);
+                a0 = a0 + x0 * p[q * stride];
+                a1 = a1 + x0 * p[q * stride + 1];
+            }
         }
-        if (q < it.cnt) {
-            const cplx<R> x0 = cast<R>(x[perm[it.sbeg + q]]);
-            a0 = a0 + x0 * p[q * stride];
-            a1 = a1 + x0 * p[q * stride + 1];
+        if (d < it.ndir) {
+            M[it.fOff + 2 * d] = a0 + b0;
+            M[it.fOff + 2 * d + 1] = a1 + b1;
         }
-        M[it.fOff + 2 * d] = a0 + b0;
-        M[it.fOff + 2 * d + 1] = a1 + b1;
     } else {
+        if (d >= it.ndir) return;
+        const cplx<R>* p = rp + it.rpOff + 2 * d;
+        const int64_t stride = (int64_t)it.ndir * 2;
         cplx<R> a[NB][2];
 #pragma unroll
         for (int r = 0; r < NB; ++r) a[r][0] = a[r][1] = mk<R>(0, 0);
@@ -95,13 +125,13 @@ __global__ void kS2Mx(const LeafItem* __restrict__ items, int nitems, int per, c
 }
 
 template <typename R>
-void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* x,
-                const int64_t* perm, cplx<R>* M, hipStream_t st, int nb, int64_t xs, int64_t fs) {
+void launchS2Mx(const LeafItem* items, const int32_t* order, int nitems, int maxndir, const cplx<R>* rp,
+                const cplx<double>* x, const int64_t* perm, cplx<R>* M, hipStream_t st, int nb, int64_t xs,
+                int64_t fs) {
     if (nitems <= 0) return;
     const int bs = std::min(TB, ((maxndir + 63) / 64) * 64);
-    const int per = (nitems + 7) / 8;
-    dim3 g(8 * per, (maxndir + bs - 1) / bs);
-#define S2M_CASE(B) hipLaunchKernelGGL((kS2Mx<R, B>), g, dim3(bs), 0, st, items, nitems, per, rp, x, xs, perm, M, fs)
+    dim3 g(nitems, (maxndir + bs - 1) / bs);
+#define S2M_CASE(B) hipLaunchKernelGGL((kS2Mx<R, B>), g, dim3(bs), 0, st, items, order, rp, x, xs, perm, M, fs)
     if (nb == 1) S2M_CASE(1);
     else if (nb == 2) S2M_CASE(2);
     else if (nb == 4) S2M_CASE(4);
@@ -411,7 +441,12 @@ __global__ __launch_bounds__(TB) void kM2Lx(M2LLevels<R> lv) {
 }
 
 // Single right-hand side: a block is (level, target box, chunk of 64
-// directions) and its four waves split the target's pair list (wave w takes
+// directions).  Block b runs on XCD b mod 8; each XCD takes a contiguous
+// eighth of the level's (chunk, target) units in chunk-major order, so it
+// reads the operators and source fields of about one direction chunk (a
+// level's operators are read by one or two XCDs instead of all eight), and
+// consecutive blocks walk neighbouring targets, whose sources overlap in
+// its L2.  The block's four waves split the target's pair list (wave w takes
 // pairs w, w + 4, ...; MLFMA_M2L_PAIRS pairs in flight per wave), so that
 // many independent pairs are in flight per direction instead of a long
 // dependent walk of index and field loads; the
@@ -424,9 +459,10 @@ __device__ __forceinline__ void m2lwBody(int bid, const M2LLevels<R>& lv, cplx<R
     while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
     b -= lv.blk[l];
     const int nch = lv.nch[l];
-    int item, chunk;
-    xcdItemChunk(b, lv.per[l], nch, item, chunk);
-    if (item >= lv.ntgt[l]) return;  // uniform per block
+    const int u = (b & 7) * lv.per[l] + (b >> 3);  // per[l] = ceil(units / 8)
+    if (u >= lv.ntgt[l] * nch) return;            // uniform per block
+    const int chunk = u / lv.ntgt[l];
+    const int item = u - chunk * lv.ntgt[l];
     const int ndir = lv.ndir[l];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -515,8 +551,9 @@ void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb) {
         total = 0;
         for (int l = 0; l < lv.nlev; ++l) {
             lv.nch[l] = (lv.ndir[l] + 63) / 64;
+            lv.per[l] = (lv.ntgt[l] * lv.nch[l] + 7) / 8;
             lv.blk[l] = total;
-            total += 8 * lv.per[l] * lv.nch[l];
+            total += 8 * lv.per[l];
         }
         if (conjT)
             hipLaunchKernelGGL((kM2Lw<R, true>), dim3(total), dim3(TB), 0, st, lv);
@@ -1287,9 +1324,9 @@ void launchM2LNear(M2LLevels<R> lv, const NearItem* items, int nitems, const int
     int total = 0;
     for (int l = 0; l < lv.nlev; ++l) {
         lv.nch[l] = (lv.ndir[l] + 63) / 64;
-        lv.per[l] = (lv.ntgt[l] + 7) / 8;
+        lv.per[l] = (lv.ntgt[l] * lv.nch[l] + 7) / 8;
         lv.blk[l] = total;
-        total += 8 * lv.per[l] * lv.nch[l];
+        total += 8 * lv.per[l];
     }
     const int gM = total / 8, gN = (nitems + 7) / 8, gT = gM + gN;
     if (gT == 0) return;
@@ -1432,12 +1469,101 @@ __global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, 
     }
 }
 
+// One right-hand side: a wave takes two consecutive rows (a lone last row is
+// computed twice and stored once), so twice the pattern bytes are in flight
+// per round trip, and the first 64 entries of each row's near-field list are
+// loaded before the pattern stream so that their index chain overlaps it.
+// Every sum is formed in the order of kL2Tx (same bits).
+template <typename R>
+__global__ __launch_bounds__(TB) void kL2T2(const int64_t* __restrict__ rowPos, const int32_t* __restrict__ rowItem,
+                                            int64_t nrows, const LeafItem* __restrict__ items,
+                                            const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc,
+                                            cplx<double> Ck, const int64_t* __restrict__ sumPtr,
+                                            const int32_t* __restrict__ sumIdx, const cplx<double>* __restrict__ part,
+                                            const int64_t* __restrict__ perm, cplx<double>* __restrict__ y,
+                                            int accumulate, int per) {
+    const int64_t w = 2 * ((int64_t)xcdItem(blockIdx.x, per) * (TB / 64) + (threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    if (w >= nrows) return;  // uniform per wave
+    const bool two = w + 1 < nrows;
+    const int64_t w1 = two ? w + 1 : w;
+    const int64_t ta = rowPos[w], tb = rowPos[w1];
+    const LeafItem A = items[rowItem[w]], B = items[rowItem[w1]];
+    const int64_t pa = sumPtr[ta], pa1 = sumPtr[ta + 1], pb = sumPtr[tb], pb1 = sumPtr[tb + 1];
+    const int ia = pa + lane < pa1 ? sumIdx[pa + lane] : -1;
+    const int ib = pb + lane < pb1 ? sumIdx[pb + lane] : -1;
+    const cplx<double> va = ia >= 0 ? part[ia] : mk(0.0, 0.0);
+    const cplx<double> vb = ib >= 0 ? part[ib] : mk(0.0, 0.0);
+    const cplx<R>* ra = rp + A.rpOff + (ta - A.sbeg) * A.ndir * 2;
+    const cplx<R>* rb = rp + B.rpOff + (tb - B.sbeg) * B.ndir * 2;
+    const cplx<R>* La = Lc + A.fOff;
+    const cplx<R>* Lb = Lc + B.fOff;
+    R rea = 0, ima = 0, reb = 0, imb = 0;
+    if (A.ndir == B.ndir) {  // uniform: both rows' loads in one loop
+        const bool sameLeaf = A.fOff == B.fOff;
+        for (int d = lane; d < A.ndir; d += 64) {
+            const cplx<R> a0 = ra[2 * d], a1 = ra[2 * d + 1], b0 = rb[2 * d], b1 = rb[2 * d + 1];
+            const cplx<R> la0 = La[2 * d], la1 = La[2 * d + 1];
+            const cplx<R> lb0 = sameLeaf ? la0 : Lb[2 * d], lb1 = sameLeaf ? la1 : Lb[2 * d + 1];
+            const cplx<R> u = conjmul(a0, la0) + conjmul(a1, la1);
+            const cplx<R> v = conjmul(b0, lb0) + conjmul(b1, lb1);
+            rea += u.re;
+            ima += u.im;
+            reb += v.re;
+            imb += v.im;
+        }
+    } else {
+        for (int d = lane; d < A.ndir; d += 64) {
+            const cplx<R> u = conjmul(ra[2 * d], La[2 * d]) + conjmul(ra[2 * d + 1], La[2 * d + 1]);
+            rea += u.re;
+            ima += u.im;
+        }
+        for (int d = lane; d < B.ndir; d += 64) {
+            const cplx<R> v = conjmul(rb[2 * d], Lb[2 * d]) + conjmul(rb[2 * d + 1], Lb[2 * d + 1]);
+            reb += v.re;
+            imb += v.im;
+        }
+    }
+    // near-field partials past the first 64 of a row's list, as nearFold
+    double na = va.re, nai = va.im, nb = vb.re, nbi = vb.im;
+    for (int64_t q = pa + 64 + lane; q < pa1; q += 64) {
+        const cplx<double> v = part[sumIdx[q]];
+        na += v.re;
+        nai += v.im;
+    }
+    for (int64_t q = pb + 64 + lane; q < pb1; q += 64) {
+        const cplx<double> v = part[sumIdx[q]];
+        nb += v.re;
+        nbi += v.im;
+    }
+    const R sa = wsum(rea), sai = wsum(ima), sb = wsum(reb), sbi = wsum(imb);
+    const cplx<double> nearA = mk(wsum(na), wsum(nai)), nearB = mk(wsum(nb), wsum(nbi));
+    if (lane == 0) {
+        const int64_t oa = perm[ta];
+        const cplx<double> v = nearA + Ck * mk((double)sa, (double)sai);
+        y[oa] = accumulate ? y[oa] + v : v;
+        if (two) {
+            const int64_t ob = perm[tb];
+            const cplx<double> u = nearB + Ck * mk((double)sb, (double)sbi);
+            y[ob] = accumulate ? y[ob] + u : u;
+        }
+    }
+}
+
 template <typename R>
 void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,
                 const cplx<R>* Lc, cplx<double> Ck, const int64_t* sumPtr, const int32_t* sumIdx,
                 const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st, int nb,
                 int64_t fs, int64_t xs, int64_t ps) {
     if (nrows <= 0) return;
+    if (nb == 1) {
+        const int64_t nbk = (nrows + 2 * (TB / 64) - 1) / (2 * (TB / 64));
+        if (nbk > INT32_MAX / 2) throw Error(MLFMA_ERR_ARG, "too many far-field rows");
+        const int per = (int)((nbk + 7) / 8);
+        hipLaunchKernelGGL((kL2T2<R>), dim3((unsigned)(8 * per)), dim3(TB), 0, st, rowPos, rowItem, nrows, items, rp,
+                           Lc, Ck, sumPtr, sumIdx, part, perm, y, accumulate, per);
+        return;
+    }
     const int64_t nbk = (nrows + TB / 64 - 1) / (TB / 64);
     if (nbk > INT32_MAX / 2) throw Error(MLFMA_ERR_ARG, "too many far-field rows");
     const int per = (int)((nbk + 7) / 8);
@@ -1495,8 +1621,8 @@ void launchL2TGrad(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows,
 }
 
 #define INST(R)                                                                                                     \
-    template void launchS2Mx<R>(const LeafItem*, int, int, const cplx<R>*, const cplx<double>*, const int64_t*,     \
-                                cplx<R>*, hipStream_t, int, int64_t, int64_t);                                     \
+    template void launchS2Mx<R>(const LeafItem*, const int32_t*, int, int, const cplx<R>*, const cplx<double>*,    \
+                                const int64_t*, cplx<R>*, hipStream_t, int, int64_t, int64_t);                     \
     template bool launchM2MFused<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                     int, int, const int32_t*, const double*, const int32_t*, const double*,         \
                                     const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t, int, int64_t,    \

@@ -9,8 +9,9 @@ namespace mlfma {
 // column stride xs (input order), field arrays at stride fs, near-field
 // partial sums at stride ps.
 template <typename R>
-void launchS2Mx(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<double>* x,
-                const int64_t* perm, cplx<R>* M, hipStream_t st, int nb = 1, int64_t xs = 0, int64_t fs = 0);
+void launchS2Mx(const LeafItem* items, const int32_t* order, int nitems, int maxndir, const cplx<R>* rp,
+                const cplx<double>* x, const int64_t* perm, cplx<R>* M, hipStream_t st, int nb = 1, int64_t xs = 0,
+                int64_t fs = 0);
 
 // returns false when the level's fields do not fit the LDS limit (caller
 // then uses the two-pass kernels); minBlocks: the block count to split parent

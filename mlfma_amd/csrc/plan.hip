@@ -288,6 +288,14 @@ void DevicePlan::build() {
     nLeafItems = (int)items.size();
     leafItems = upload(items);
     {
+        std::vector<int32_t> order(nLeafItems);
+        for (int i = 0; i < nLeafItems; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return (int64_t)items[a].cnt * items[a].ndir > (int64_t)items[b].cnt * items[b].ndir;
+        });
+        s2mOrder = upload(order);
+    }
+    {
         std::vector<int64_t> rows;
         std::vector<int32_t> ritem;
         for (int i = 0; i < nLeafItems; ++i)
@@ -921,7 +929,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     }
     if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
     probeBegin(MLFMA_GROUP_S2M, st);
-    launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
+    launchS2Mx<R>(leafItems, s2mOrder, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
     probeEnd(MLFMA_GROUP_S2M, st);
     const bool nearSide = overlap == 2 || overlap == 3;
     auto forkSides = [&]() {
@@ -1034,7 +1042,7 @@ void DevicePlan::applyAdjoint(const cplx<double>* x, cplx<double>* y, int accumu
     }
     launchZeroField<R>(M, fieldLen, st);
     launchZeroField<R>(Lc, fieldLen, st);
-    launchS2Mx<R>(leafItems, nLeafItems, maxLeafNdir, rp, x, perm, Lc, st);
+    launchS2Mx<R>(leafItems, s2mOrder, nLeafItems, maxLeafNdir, rp, x, perm, Lc, st);
     for (int l = ML; l >= 3; --l) {
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];

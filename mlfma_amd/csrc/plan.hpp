@@ -112,6 +112,7 @@ struct DevicePlan {
     cplx<double>*M64 = nullptr, *L64 = nullptr, *inner64 = nullptr, *rp64 = nullptr;
     cplx<float>*M32 = nullptr, *L32 = nullptr, *inner32 = nullptr, *rp32 = nullptr;
     LeafItem* leafItems = nullptr;
+    int32_t* s2mOrder = nullptr;       // leaf items by source count, largest first (S2M launch order)
     int nLeafItems = 0, maxLeafNdir = 0;
     NearItem* nearItems = nullptr;     // longest first
     int nNearItems = 0;
