@@ -1,26 +1,35 @@
 """Benchmark: MLFMA matvec throughput (and GMRES solve time) on one MI355X.
 
-Workload (default): plain FP64 RWG EFIE on a PEC sphere of radius 2 wavelengths,
-geodesic icosphere of frequency 32 (20,480 triangles, 30,720 RWG unknowns),
-7-point triangle quadrature, leaf boxes of 0.25 wavelength (root 4, levels
-0-4), interpolation order 6, 3 digits, plane wave theta = 0 x-polarised.
+Workload (default, --workload sphere200k): plain FP64 RWG EFIE on a PEC
+sphere of radius 4 wavelengths, geodesic icosphere of frequency 82 (134,480
+triangles, 201,720 RWG unknowns), 7-point triangle quadrature, leaf boxes of
+0.25 wavelength (root 8, levels 0-5), interpolation order 6, 3 digits, plane
+wave theta = 0 x-polarised.  Also: sphere30k (30,720 RWG, radius 2),
+dipoles4k (the reference's 4,096-dipole case), multi16 (16 scatterers) and
+dipoles200k_grad (position gradient of 200,000 dipoles).
 
 A step is one full matvec y = Z x (S2M, M2M, M2L, L2L, near field, L2T) with
-x and y resident in HBM.  W untimed warmup steps, then K timed steps bracketed
-by torch.cuda.synchronize() give `value`; one JSON line.
+x and y resident in HBM, in the schedule the library's autotuner picks.  W
+untimed warmup steps, then K timed steps bracketed by torch.cuda.synchronize()
+give `value`; one JSON line.
 
 The line also carries:
-  stage_ms      per-stage device times from HIP events on the library's stream,
-                recorded over a second pass of the same K steps;
-  roofline      the near-field pair kernel's achieved algorithmic GB/s (its
-                launch time from the events of that second pass) against 8 TB/s;
+  kernels       per launch group (kernel), the device time of its launches from
+                HIP events recorded on the stream each launch runs on, over a
+                second pass of the same K steps in the same schedule;
+  roofline      the kernel with the most device time per matvec: algorithmic
+                bytes per launch / its average launch time against 8 TB/s, and
+                the HBM bytes per launch from the committed rocprofv3 counter
+                summary (profiles/<kernel>_<workload>_traffic.txt);
+  stage_ms      per-stage device times with every stage in order (the
+                reference's ClockTimes);
   batched       8 right-hand sides per call (each operator read shared by 8);
   cpu_baseline  the CPU oracle (FP64 restatement of the reference, one core)
                 timed on a bounded number of matvecs of the same workload;
-  solve         GMRES(50) to 1e-6 in FP64 and the FP32-far-field refinement
-                solve (the reference's kernel), with iteration counts and true
-                residuals; and GMRES(50) with the physical near field plus the
-                bistatic RCS at 361 angles against the Mie series.
+  solve         GMRES to 1e-6: the reference's kernel (stalls), then the
+                physical near field in FP64 GMRES and mixed-precision FGMRES,
+                8 incidence angles one at a time and batched, RCS against the
+                Mie series.
 """
 from __future__ import annotations
 
@@ -527,7 +536,7 @@ def kernel_bytes(group, info, n):
     return None, None
 
 
-KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2Tx", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
+KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2T2", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
                 "M2M": "kM2MFused", "L2L": "kL2LFused", "M2L_SIBLING": "kM2LGroup"}
 
 
@@ -614,6 +623,13 @@ def main():
                 "near_window": plan.get_param(A.MLFMA_PARAM_NEAR_WINDOW),
                 "near_blocks_per_cu": plan.get_param(A.MLFMA_PARAM_NEAR_BLOCKS)}
     log("autotune ms", tune_before, "->", tune_after, schedule)
+
+    def kname(g):
+        # beside the chain with resident blocks the near field runs kNearSymP
+        if g == "NEAR" and schedule["near_overlap"] and schedule["near_blocks_per_cu"]:
+            return "kNearSymP"
+        return KERNEL_NAMES[g]
+
     plan.set_flags(A.MLFMA_FLAG_ASYNC)
     for _ in range(args.warmup):
         plan.matvec(x, y)
@@ -638,7 +654,7 @@ def main():
         torch.cuda.synchronize()
         ms, cnt = plan.probe_read()
         if cnt:
-            groups[g] = {"kernel": KERNEL_NAMES[g], "launches_per_matvec": round(cnt / args.steps, 3),
+            groups[g] = {"kernel": kname(g), "launches_per_matvec": round(cnt / args.steps, 3),
                          "avg_launch_ms": round(ms / cnt, 5), "ms_per_matvec": round(ms / args.steps, 5)}
     plan.probe(None)
     log("launch groups", groups)
@@ -679,8 +695,8 @@ def main():
     avg = groups[dom]["avg_launch_ms"]
     achieved = kb / (avg * 1e-3) / 1e9
     tag = desc["workload"]
-    traffic = measured_traffic(KERNEL_NAMES[dom], tag)
-    roofline = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    traffic = measured_traffic(kname(dom), tag)
+    roofline = {"bound": "hbm", "kernel": kname(dom), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": round(traffic) if traffic else None, "bytes_per_launch": int(kb),
                 "avg_launch_ms": avg, "launch_group": dom}

@@ -576,8 +576,8 @@ void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb) {
 // are staged in LDS; the parent local, phase-shifted to the child centre on
 // the parent grid, is staged in LDS; the phi anterpolation goes to LDS; the
 // theta anterpolation with the pole rule is added into the child local.
-template <typename R, bool ADJ>
-__global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ parent, const int32_t* __restrict__ child,
+template <typename R, bool ADJ, int BS>
+__global__ __launch_bounds__(BS) void kL2LFused(const int32_t* __restrict__ parent, const int32_t* __restrict__ child,
                                                 const int32_t* __restrict__ oct, int nitems, int per, int mthP,
                                                 int mphP, int nthC, int nphC, const int32_t* __restrict__ atPhPtr,
                                                 const int32_t* __restrict__ atPhSrc, const double* __restrict__ atPhW,
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     cplx<R> f0[U] = {}, f1[U] = {}, fq[U] = {}, o0[UO] = {}, o1[UO] = {}, oq[UO] = {};
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int i = t + u * TB;
+        const int i = t + u * BS;
         if (i < nAtPh) {
             wPh[u] = atPhW[i];
             sPh[u] = atPhSrc[i];
@@ -640,7 +640,7 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     const int pTh = t <= nthC ? atThPtr[t] : 0;
 #pragma unroll
     for (int u = 0; u < UO; ++u) {
-        const int e = t + u * TB;
+        const int e = t + u * BS;
         if (e < nOut) {
             o0[u] = out[2 * e];
             o1[u] = out[2 * e + 1];
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int i = t + u * TB;
+        const int i = t + u * BS;
         if (i < nAtPh) {
             tPhW[i] = (R)wPh[u];
             tPhSrc[i] = sPh[u];
@@ -666,24 +666,24 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     }
     if (t <= nphC) tPhPtr[t] = pPh;
     if (t <= nthC) tThPtr[t] = pTh;
-    for (int i = t + U * TB; i < nAtPh; i += TB) {
+    for (int i = t + U * BS; i < nAtPh; i += BS) {
         tPhW[i] = (R)atPhW[i];
         tPhSrc[i] = atPhSrc[i];
     }
-    for (int i = t + U * TB; i < nAtTh; i += TB) {
+    for (int i = t + U * BS; i < nAtTh; i += BS) {
         const bool flip = atThFlip[i] != 0;
         tThW[i] = (R)(flip ? -atThW[i] : atThW[i]);
         tThSrc[i] = 2 * atThSrc[i] + (flip ? 1 : 0);
     }
-    for (int i = t + TB; i <= nphC; i += TB) tPhPtr[i] = atPhPtr[i];
-    for (int i = t + TB; i <= nthC; i += TB) tThPtr[i] = atThPtr[i];
-    for (int d = t + U * TB; d < ndirP; d += TB) {
+    for (int i = t + BS; i <= nphC; i += BS) tPhPtr[i] = atPhPtr[i];
+    for (int i = t + BS; i <= nthC; i += BS) tThPtr[i] = atThPtr[i];
+    for (int d = t + U * BS; d < ndirP; d += BS) {
         const cplx<R> q = ADJ ? mk<R>(1, 0) : ph[d];
         sP[2 * d] = ADJ ? Lb[2 * d] : q * Lb[2 * d];
         sP[2 * d + 1] = ADJ ? Lb[2 * d + 1] : q * Lb[2 * d + 1];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < nInner; e += TB) {
+    for (int e = threadIdx.x; e < nInner; e += BS) {
         const int ith = e / nphC, jph = e - ith * nphC;
         const cplx<R>* row = sP + 2 * ith * mphP;
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
     };
 #pragma unroll
     for (int u = 0; u < UO; ++u) {
-        const int e = t + u * TB;
+        const int e = t + u * BS;
         if (e < nOut) {
             cplx<R> a0, a1;
             anter(e, a0, a1, oq[u]);
@@ -729,7 +729,7 @@ __global__ __launch_bounds__(TB) void kL2LFused(const int32_t* __restrict__ pare
             out[2 * e + 1] = o1[u] + a1;
         }
     }
-    for (int e = t + UO * TB; e < nOut; e += TB) {
+    for (int e = t + UO * BS; e < nOut; e += BS) {
         cplx<R> a0, a1;
         anter(e, a0, a1, ADJ ? phC[e] : mk<R>(0, 0));
         out[2 * e] = out[2 * e] + a0;
@@ -750,14 +750,26 @@ bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32
     if (lds > ldsMax) return false;
     const int per = (nitems + 7) / 8;
     const dim3 g(8 * per, 1, nb);
-    if (adjoint)
-        hipLaunchKernelGGL((kL2LFused<R, true>), g, dim3(TB), lds, st, parent, child, oct, nitems, per, mthP, mphP,
-                           nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip, nAtTh,
-                           phaseDn, Lp, Lc, fs);
+    // blocks whose LDS leaves room for one or two per CU get more waves, so
+    // that the CU still has enough loads in flight
+#define L2L_LAUNCH(BSZ)                                                                                              \
+    {                                                                                                               \
+        if (adjoint)                                                                                                \
+            hipLaunchKernelGGL((kL2LFused<R, true, BSZ>), g, dim3(BSZ), lds, st, parent, child, oct, nitems, per,     \
+                               mthP, mphP, nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW,      \
+                               atThFlip, nAtTh, phaseDn, Lp, Lc, fs);                                               \
+        else                                                                                                        \
+            hipLaunchKernelGGL((kL2LFused<R, false, BSZ>), g, dim3(BSZ), lds, st, parent, child, oct, nitems, per,    \
+                               mthP, mphP, nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW,      \
+                               atThFlip, nAtTh, phaseDn, Lp, Lc, fs);                                               \
+    }
+    if (2 * lds > ldsMax)
+        L2L_LAUNCH(1024)
+    else if (3 * lds > ldsMax)
+        L2L_LAUNCH(512)
     else
-        hipLaunchKernelGGL((kL2LFused<R, false>), g, dim3(TB), lds, st, parent, child, oct, nitems, per, mthP, mphP,
-                           nthC, nphC, atPhPtr, atPhSrc, atPhW, nAtPh, atThPtr, atThSrc, atThW, atThFlip, nAtTh,
-                           phaseDn, Lp, Lc, fs);
+        L2L_LAUNCH(256)
+#undef L2L_LAUNCH
     return true;
 }
 
@@ -911,7 +923,7 @@ constexpr int NT = MLFMA_NEAR_LOADS * TB;  // values per tile
 // V: the stored value type (double; float for the FP32 inner solves of the
 // mixed-precision path, products and sums still in FP64).  bid: the item;
 // sT, sxr, sxc: LDS of NT x (16 / sizeof(cplx<V>)) values, 128 and TB values.
-template <bool CONJ, typename V>
+template <bool CONJ, typename V, int NL = MLFMA_NEAR_LOADS>
 __device__ __forceinline__ void nearSymBody(int bid, const NearItem* __restrict__ items,
                                             const int32_t* __restrict__ colsIn, const int64_t* __restrict__ perm,
                                             const cplx<V>* __restrict__ val, const cplx<double>* __restrict__ x,
@@ -923,8 +935,7 @@ __device__ __forceinline__ void nearSymBody(int bid, const NearItem* __restrict_
     // row-sum reduction at the end reuses the tile's storage, the column sums
     // reduce by shuffles.
     constexpr int EPL = sizeof(V) == 4 ? 2 : 1;  // values per lane and load
-    constexpr int NL = MLFMA_NEAR_LOADS;
-    constexpr int NTV = NT * EPL;
+    constexpr int NTV = NL * TB * EPL;
     const NearItem I = items[bid];
     const int nr = I.nr;        // 1..128
     const int ld = nr | 1;      // odd leading dimension in LDS
@@ -1059,15 +1070,18 @@ __global__ __launch_bounds__(TB) void kNearSym(const NearItem* __restrict__ item
 // blockIdx.x + gridDim.x, ...: launched beside the far-field chain, the near
 // field then holds only that many of a CU's LDS and wave slots, so that the
 // chain's blocks are not kept waiting for the whole near field to drain.
+// With few waves per CU, each lane keeps more values in flight.
+constexpr int NEAR_LOADS_P = 8;
+constexpr int NTP = NEAR_LOADS_P * TB;
 template <bool CONJ, typename V>
 __global__ __launch_bounds__(TB) void kNearSymP(const NearItem* __restrict__ items, int nitems,
                                                 const int32_t* __restrict__ colsIn, const int64_t* __restrict__ perm,
                                                 const cplx<V>* __restrict__ val, const cplx<double>* __restrict__ x,
                                                 cplx<double>* __restrict__ part) {
-    __shared__ cplx<V> sT[NT * (sizeof(V) == 4 ? 2 : 1)];
+    __shared__ cplx<V> sT[NTP * (sizeof(V) == 4 ? 2 : 1)];
     __shared__ cplx<double> sxr[128], sxc[TB];
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        nearSymBody<CONJ, V>(it, items, colsIn, perm, val, x, part, sT, sxr, sxc);
+        nearSymBody<CONJ, V, NEAR_LOADS_P>(it, items, colsIn, perm, val, x, part, sT, sxr, sxc);
         __syncthreads();  // the item's LDS reads are done before the next item's stores
     }
 }

@@ -328,10 +328,13 @@ def test_side_stream_schedules_match_one_stream(fp32):
     """Every schedule the autotuner chooses among (near-field pairs forked
     before S2M, after S2M or after the M2M chain; the finest level's M2L
     beside the chain or in line; the near field as one block per work item or
-    as 1-4 resident blocks per CU looping over the items) gives the bits of
+    as 1-4 resident blocks per CU looping over the items) gives the result of
     the in-order run, forward and adjoint, on the 30,720-RWG sphere (about
     ten thousand near-field work items, so the looping launches take many
-    items per block)."""
+    items per block): the same bits with one block per item; with resident
+    blocks, whose lanes keep more values in flight (wider tiles, so each
+    row's sum is grouped differently), equal to round-off and the same bits
+    for every stream schedule."""
     import torch
     from mlfma_amd import configs as CF
     geom, pw, kw, _ = CF.c2()
@@ -342,6 +345,7 @@ def test_side_stream_schedules_match_one_stream(fp32):
     p.set_flags(flags | A.MLFMA_FLAG_ONE_STREAM)
     y0, z0 = p.matvec(x).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()
     p.set_flags(flags)
+    resident = {}
     for overlap in (1, 2, 3):
         for split in (0, 1):
             for nbk in (0, 1, 2, 4):
@@ -350,16 +354,23 @@ def test_side_stream_schedules_match_one_stream(fp32):
                 p.set_param(A.MLFMA_PARAM_NEAR_BLOCKS, nbk)
                 assert p.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == nbk
                 y1, z1 = p.matvec(x).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()
-                say("fp32", fp32, "overlap", overlap, "split", split, "near blocks/CU", nbk, cases.relerr(y1, y0),
-                    cases.relerr(z1, z0))
-                assert cases.relerr(y1, y0) == 0.0 and cases.relerr(z1, z0) == 0.0
+                ey, ez = cases.relerr(y1, y0), cases.relerr(z1, z0)
+                say("fp32", fp32, "overlap", overlap, "split", split, "near blocks/CU", nbk, ey, ez)
+                if nbk == 0:
+                    assert ey == 0.0 and ez == 0.0
+                else:
+                    assert ey <= 1e-14 and ez <= 1e-14
+                    if "y" in resident:
+                        assert cases.relerr(y1, resident["y"]) == 0.0 and cases.relerr(z1, resident["z"]) == 0.0
+                    else:
+                        resident = {"y": y1, "z": z1}
     # the record carries the near-field blocks and applies to a second plan
     rec = p.tuning()
     assert rec[0] == 2 and rec[-1] == 4
     q = Plan(geom, pw, A.make_config(device=0, precision=prec, flags=flags, **kw))
     q.apply_tuning(rec)
     assert q.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == 4
-    assert cases.relerr(q.matvec(x).cpu().numpy(), y0) == 0.0
+    assert cases.relerr(q.matvec(x).cpu().numpy(), resident["y"]) == 0.0
 
 
 @pytest.mark.parametrize("fp32", [False, True])
