@@ -459,24 +459,43 @@ static int dirIndex(int dx, int dy, int dz) {
 //     index; a Leaf answer is returned only when the exit direction is the full
 //     direction, and is dropped otherwise, because that leaf is found along the
 //     lower-dimensional exit direction itself ("do not double count").
-Node* Model::neighborGeq(const Node* n, int dir) const {
-    if (n->isRoot()) return nullptr;
-    const int b = n->branchIdx;
-    int target = b;
+// The step of that rule for a node of branch index b: returns whether the
+// neighbour lies outside the parent; *target is the branch index of the
+// neighbour (the sibling, or the child of the parent's neighbour) and *exitDir
+// the direction the parent is asked along.
+bool neighborRule(int dir, int b, int* target, int* exitDir) {
+    int t = b;
     int ex[3] = {0, 0, 0};
     bool exits = false;
     for (int a = 0; a < 3; ++a) {
         const int d = DIRS[dir][a];
         if (!d) continue;
         const int bit = (b >> a) & 1;
-        target ^= (1 << a);
+        t ^= (1 << a);
         if ((d > 0 && bit == 1) || (d < 0 && bit == 0)) {
             ex[a] = d;
             exits = true;
         }
     }
-    if (!exits) return n->base->branches[target];
-    const int edir = dirIndex(ex[0], ex[1], ex[2]);
+    *target = t;
+    *exitDir = exits ? dirIndex(ex[0], ex[1], ex[2]) : -1;
+    return exits;
+}
+
+// Whether child c of a stem neighbour along dir faces the node (bit 0 on the
+// side a positive direction enters, bit 1 for a negative direction).
+bool neighborChildFaces(int dir, int c) {
+    for (int a = 0; a < 3; ++a) {
+        const int d = DIRS[dir][a];
+        if (d && ((c >> a) & 1) != (d > 0 ? 0 : 1)) return false;
+    }
+    return true;
+}
+
+Node* Model::neighborGeq(const Node* n, int dir) const {
+    if (n->isRoot()) return nullptr;
+    int target = 0, edir = -1;
+    if (!neighborRule(dir, n->branchIdx, &target, &edir)) return n->base->branches[target];
     Node* nb = neighborGeq(n->base, edir);
     if (nb == nullptr) return nullptr;
     if (nb->leaf) return edir == dir ? nb : nullptr;
@@ -484,8 +503,7 @@ Node* Model::neighborGeq(const Node* n, int dir) const {
 }
 
 // src/fmm/nbor.cpp:490-644: breadth-first descent into the children of the
-// neighbour that face this node (bit = 0 on the side a positive direction
-// enters, bit = 1 for a negative direction), in increasing branch order.
+// neighbour that face this node, in increasing branch order.
 std::vector<Node*> Model::neighborsLeq(const Node* n, Node* nb, int dir) const {
     std::vector<Node*> out;
     if (n->isRoot()) return out;
@@ -495,18 +513,9 @@ std::vector<Node*> Model::neighborsLeq(const Node* n, Node* nb, int dir) const {
         Node* x = q.front();
         if (x->leaf)
             out.push_back(x);
-        else {
-            for (int c = 0; c < 8; ++c) {
-                bool ok = true;
-                for (int a = 0; a < 3; ++a) {
-                    const int d = DIRS[dir][a];
-                    if (!d) continue;
-                    const int bit = (c >> a) & 1;
-                    if (bit != (d > 0 ? 0 : 1)) ok = false;
-                }
-                if (ok) q.push(x->branches[c]);
-            }
-        }
+        else
+            for (int c = 0; c < 8; ++c)
+                if (neighborChildFaces(dir, c)) q.push(x->branches[c]);
         q.pop();
     }
     return out;

@@ -178,6 +178,11 @@ struct Node {
     bool isRoot() const { return base == nullptr; }
 };
 
+// The neighbour rule of src/fmm/nbor.cpp for one (direction, branch index):
+// see oracle.cpp.  Directions in the order of FMM::Dir.
+bool neighborRule(int dir, int b, int* target, int* exitDir);
+bool neighborChildFaces(int dir, int c);
+
 struct Model {
     Config cfg;
     PlaneWave pw;

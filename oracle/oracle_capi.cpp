@@ -330,6 +330,29 @@ int orc_apply_interp(void* hp, int level, int dir, const double* in, double* out
     return 0;
 }
 
+// The neighbour rule (src/fmm/nbor.cpp) for direction dir (FMM::Dir order) and
+// branch index b: out = [exits the parent, target branch, exit direction or
+// -1, a Leaf answer is kept (1) or dropped (0)]; and the children of a stem
+// neighbour that face the node (out[0] = count, out[1..] = branches).
+int orc_nbor_rule(int dir, int b, int32_t* out) {
+    if (dir < 0 || dir >= 26 || b < 0 || b >= 8) return -1;
+    int t = 0, e = -1;
+    const bool ex = neighborRule(dir, b, &t, &e);
+    out[0] = ex ? 1 : 0;
+    out[1] = t;
+    out[2] = e;
+    out[3] = (ex && e == dir) ? 1 : 0;
+    return 0;
+}
+int orc_nbor_leq_children(int dir, int32_t* out) {
+    if (dir < 0 || dir >= 26) return -1;
+    int k = 0;
+    for (int c = 0; c < 8; ++c)
+        if (neighborChildFaces(dir, c)) out[1 + k++] = c;
+    out[0] = k;
+    return 0;
+}
+
 // Nodes in creation order: per node [level, ix, iy, iz, leaf, nsrcs, id]
 int orc_nodes(void* hp, int32_t* out, int64_t cap) {
     Model& m = static_cast<Handle*>(hp)->m;
