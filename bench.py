@@ -679,12 +679,15 @@ def main():
     # reference's ClockTimes)
     plan.set_flags(A.MLFMA_FLAG_STAGE_TIMES)
     stages = {k: 0.0 for k in ["S2M", "M2M", "M2L", "L2L", "L2T", "near", "total"]}
+    near_pairs = 0.0
     for _ in range(args.steps):
         plan.matvec(x, y)
         for k, v in plan.stage_times().items():
             stages[k] += v
+        near_pairs += plan.stage_times_detail()["near_pairs"]
     plan.set_flags(0)
     stage_ms = {k: round(v / args.steps, 4) for k, v in stages.items()}
+    near_pairs_ms = near_pairs / args.steps
     log("stage ms", stage_ms, "rates", rates)
 
     # roofline of the kernel that takes the most device time per matvec in
@@ -703,6 +706,13 @@ def main():
     if kstream != kb:
         roofline["streamed_bytes_per_launch"] = int(kstream)
         roofline["streamed_frac"] = round(kstream / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if dom == "NEAR" and near_pairs_ms > 0:
+        # the same pair products alone, every stage in order (one block per
+        # work item, kNearSym): the kernel's own rate, against the rate it
+        # keeps beside the far-field chain in the timed schedule
+        roofline["alone_in_order"] = {"kernel": "kNearSym", "launch_ms": round(near_pairs_ms, 5),
+                                      "achieved": round(kb / (near_pairs_ms * 1e-3) / 1e9, 1),
+                                      "frac": round(kb / (near_pairs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     log("roofline", roofline)
     kernels = {}
     for g, v in groups.items():
