@@ -536,7 +536,7 @@ def kernel_bytes(group, info, n):
     return None, None
 
 
-KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2T2", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
+KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2TN", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
                 "M2M": "kM2MFused", "L2L": "kL2LFused", "M2L_SIBLING": "kM2LGroup"}
 
 
@@ -556,6 +556,9 @@ def main():
                          "of autotuning when present (profiled runs then launch only the timed schedule)")
     ap.add_argument("--timed-only", action="store_true",
                     help="only the timed matvecs and their probe passes (one schedule; for kernel profiles)")
+    ap.add_argument("--fp32", action="store_true",
+                    help="with --timed-only: the plan's far field in FP32 (the inner matvec of the "
+                         "mixed-precision solves) instead of FP64")
     ap.add_argument("--mode", default="matvec", choices=["matvec", "copy", "m2l-ceiling", "hbm"],
                     help="copy: only run a plain 256 MiB device copy (HBM counter calibration); m2l-ceiling: the "
                          "M2L access pattern (per target, the sum of its sources' fields and operators) through "
@@ -599,7 +602,10 @@ def main():
     geom, pw, kw, desc = workload(args.workload)
     log("workload", desc)
     t0 = time.time()
-    plan = Plan(geom, pw, A.make_config(device=0, **kw))
+    if args.fp32 and not args.timed_only:
+        raise SystemExit("--fp32 goes with --timed-only")
+    prec = A.MLFMA_FP32 if args.fp32 else A.MLFMA_FP64
+    plan = Plan(geom, pw, A.make_config(device=0, precision=prec, **kw))
     setup_s = time.time() - t0
     info = plan.info()
     log("plan", {k: info[k] for k in ["max_level", "boxes", "m2l_pairs", "near_nnz", "radpat_entries",
@@ -659,7 +665,8 @@ def main():
     plan.probe(None)
     log("launch groups", groups)
     if args.timed_only:
-        print(json.dumps({"metric": "MLFMA matvec/s (FP64, one MI355X)", "value": round(value, 3),
+        print(json.dumps({"metric": f"MLFMA matvec/s ({'FP32 far field' if args.fp32 else 'FP64'}, one MI355X)",
+                          "value": round(value, 3),
                           "unit": "matvec/s", "ms_per_step": round(ms_per_step, 4), "steps": args.steps,
                           "config": desc, "schedule": schedule, "kernels": groups}), flush=True)
         return
