@@ -556,6 +556,8 @@ def main():
                          "of autotuning when present (profiled runs then launch only the timed schedule)")
     ap.add_argument("--timed-only", action="store_true",
                     help="only the timed matvecs and their probe passes (one schedule; for kernel profiles)")
+    ap.add_argument("--nrhs", type=int, default=1,
+                    help="with --timed-only: right-hand sides per matvec call (1, 2, 4 or 8)")
     ap.add_argument("--fp32", action="store_true",
                     help="with --timed-only: the plan's far field in FP32 (the inner matvec of the "
                          "mixed-precision solves) instead of FP64")
@@ -612,7 +614,13 @@ def main():
                                       "stale_boxes", "truncated_l2l", "device_bytes"]}, "setup s", round(setup_s, 2))
     n = geom.n
     dev = torch.device("cuda:0")
-    x = torch.from_numpy(G.complex_gaussian(n, 0)).to(dev)
+    if args.nrhs != 1 and not args.timed_only:
+        raise SystemExit("--nrhs goes with --timed-only")
+    if args.nrhs == 1:
+        x = torch.from_numpy(G.complex_gaussian(n, 0)).to(dev)
+    else:
+        x = torch.from_numpy(np.stack([G.complex_gaussian(n, r) for r in range(args.nrhs)])).to(dev)
+        plan.set_flags(A.MLFMA_FLAG_FRESH_LOCALS)
     y = torch.empty_like(x)
     # the library's autotuner picks launch shapes and the stage schedule
     if args.tuning and os.path.exists(args.tuning):
@@ -620,7 +628,7 @@ def main():
             plan.apply_tuning(json.load(f)["record"])
         tune_before = tune_after = None
     else:
-        tune_before, tune_after = plan.autotune(1)
+        tune_before, tune_after = plan.autotune(args.nrhs)
         if args.tuning:
             with open(args.tuning, "w") as f:
                 json.dump({"workload": desc["workload"], "record": plan.tuning()}, f)
@@ -636,7 +644,8 @@ def main():
             return "kNearSymP"
         return KERNEL_NAMES[g]
 
-    plan.set_flags(A.MLFMA_FLAG_ASYNC)
+    fresh = A.MLFMA_FLAG_FRESH_LOCALS if args.nrhs > 1 else 0
+    plan.set_flags(A.MLFMA_FLAG_ASYNC | fresh)
     for _ in range(args.warmup):
         plan.matvec(x, y)
     # timed region: K matvecs, uninstrumented, queued on torch's stream (the
@@ -648,7 +657,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
-    value = args.steps / elapsed
+    value = args.steps * args.nrhs / elapsed
     y_timed = y.cpu().numpy()
     # the same K matvecs once per launch group with that group's launches
     # bracketed by HIP events on the stream each runs on (same schedule)
@@ -666,7 +675,7 @@ def main():
     log("launch groups", groups)
     if args.timed_only:
         print(json.dumps({"metric": f"MLFMA matvec/s ({'FP32 far field' if args.fp32 else 'FP64'}, one MI355X)",
-                          "value": round(value, 3),
+                          "value": round(value, 3), "nrhs": args.nrhs,
                           "unit": "matvec/s", "ms_per_step": round(ms_per_step, 4), "steps": args.steps,
                           "config": desc, "schedule": schedule, "kernels": groups}), flush=True)
         return

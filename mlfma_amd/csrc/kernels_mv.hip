@@ -1228,13 +1228,187 @@ __global__ __launch_bounds__(TB) void kNearSymB(const NearItem* __restrict__ ite
     }
 }
 
+// Eight right-hand sides on the matrix cores: per item the row sums
+// Y = Z X_cols (rows x 8) and, for the pair columns, Z^T X_rows, as real
+// FP64 MFMA products (v_mfma_f64_16x16x4f64: A 16 x 4, B 4 x 16, D 16 x 16;
+// lane l holds A[l%16][l/16], B[l/16][l%16] and D[4i + l/16][l%16], i < 4).
+// The 16 columns of B are the real parts of the 8 right-hand sides followed
+// by their imaginary parts, so with B' = [-Im | Re] one accumulator takes
+// Re Z . B + Im Z . B' = [Re Y | Im Y].  Row tiles of 16 go to the waves
+// (wave w: tiles w and w + 4), column tiles of 16 of an LDS tile to waves
+// 3, 2, 1, 0 (pair columns only); the value tiles and x gathers of the next
+// LDS tile are in flight during the products of this one.  Values are read
+// from HBM once.
+// x comes interleaved (xt[i 8 + k]) and the partial sums go out interleaved
+// (part[j 8 + k]), so each gather and each row's or column's eight results
+// are one 128-byte line.
+constexpr int NTM = 1024;    // values per LDS tile (40 KiB of LDS per block: four blocks per CU)
+constexpr int CWM = 64;      // columns per LDS tile at most
+constexpr int NLM = NTM / TB;
+
+template <bool CONJ, typename V>
+__global__ __launch_bounds__(TB) void kNearSymM(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
+                                                const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
+                                                const cplx<double>* __restrict__ xt,
+                                                cplx<double>* __restrict__ part) {
+    using d4 = double __attribute__((ext_vector_type(4)));
+    constexpr int NB = 8;
+    __shared__ cplx<double> sT[NTM];
+    __shared__ cplx<double> sxr[NB][128];
+    __shared__ cplx<double> sxc[NB][CWM];
+    const NearItem I = items[blockIdx.x];
+    const int nr = I.nr;  // 1..128
+    const int ld = nr | 1;
+    const int cwMax = min(CWM, (NTM / ld) & ~3);
+    const int t0 = threadIdx.x;
+    const int lane = t0 & 63;
+    const int w = t0 >> 6;
+    const int lm = lane & 15, lk = lane >> 4;  // MFMA row / k index of this lane
+    const int rh = lm & 7;                     // right-hand side of this lane's B column
+    const bool reCol = lm < 8;                 // B column holds real parts
+    for (int e = t0; e < NB * nr; e += TB) {  // eight lanes read one row's 128 bytes
+        const int k = e & 7, q = e >> 3;
+        sxr[k][q] = xt[perm[I.rbeg + q] * NB + k];
+    }
+    const int nrt = (nr + 15) >> 4;
+    // Re Z . B and Im Z . B' in separate accumulators (two independent
+    // MFMA chains), added at the end
+    d4 accP[2], accQ[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) accP[j] = accQ[j] = d4{0, 0, 0, 0};
+    double* sTd = reinterpret_cast<double*>(sT);
+    static_assert(NB * CWM <= 2 * TB, "two column gathers per lane");
+    int c0 = 0, cw = min(cwMax, I.nc), ne = cw * nr;
+    V vr[NLM], vi[NLM];
+    cplx<double> xg0 = mk(0.0, 0.0), xg1 = mk(0.0, 0.0);
+    int64_t ia0 = 0, ia1 = 0;
+    // gather e of a tile: column e / 8, right-hand side e % 8 (one 128-byte
+    // line of xt per column)
+    auto colIdx = [&](int cc0, int ccw, int e) {
+        return e < NB * ccw ? (int64_t)colsIn[I.cptr + cc0 + (e >> 3)] * NB + (e & 7) : (int64_t)0;
+    };
+    auto loadVals = [&](int cc0, int nne) {
+        const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)cc0 * nr);
+#pragma unroll
+        for (int k = 0; k < NLM; ++k) {
+            const int q = 2 * min(t0 + k * TB, nne - 1);
+            vr[k] = sd[q];
+            vi[k] = sd[q + 1];
+        }
+    };
+    auto gather = [&](int ccw) {
+        if (t0 < NB * ccw) xg0 = xt[ia0];
+        if (t0 + TB < NB * ccw) xg1 = xt[ia1];
+    };
+    ia0 = colIdx(0, cw, t0);
+    ia1 = colIdx(0, cw, t0 + TB);
+    loadVals(0, ne);
+    gather(cw);
+    {
+        const int cn = cwMax, cwn = min(cwMax, I.nc - cn);
+        ia0 = cwn > 0 ? colIdx(cn, cwn, t0) : 0;
+        ia1 = cwn > 0 ? colIdx(cn, cwn, t0 + TB) : 0;
+    }
+    while (true) {
+        __syncthreads();  // the previous tile is consumed
+#pragma unroll
+        for (int k = 0; k < NLM; ++k) {
+            const int e = t0 + k * TB;
+            if (e < ne) {
+                const int c = e / nr;
+                const int a = 2 * (c * ld + (e - c * nr));
+                sTd[a] = (double)vr[k];
+                sTd[a + 1] = CONJ ? -(double)vi[k] : (double)vi[k];
+            }
+        }
+        if (t0 < NB * cw) sxc[t0 & 7][t0 >> 3] = xg0;
+        if (t0 + TB < NB * cw) sxc[(t0 + TB) & 7][(t0 + TB) >> 3] = xg1;
+        __syncthreads();
+        const int c0n = c0 + cwMax;
+        const bool more = c0n < I.nc;  // uniform over the block
+        int cwn = 0, nen = 0;
+        if (more) {
+            cwn = min(cwMax, I.nc - c0n);
+            nen = cwn * nr;
+            loadVals(c0n, nen);
+            gather(cwn);
+            const int cn = c0n + cwMax, cwnn = min(cwMax, I.nc - cn);
+            ia0 = cwnn > 0 ? colIdx(cn, cwnn, t0) : 0;
+            ia1 = cwnn > 0 ? colIdx(cn, cwnn, t0 + TB) : 0;
+        }
+        // row sums: k steps of 4 columns
+        for (int ks = 0; ks < cw; ks += 4) {
+            const int c = ks + lk;
+            const cplx<double> xx = c < cw ? sxc[rh][c] : mk(0.0, 0.0);
+            const double b = reCol ? xx.re : xx.im;
+            const double bp = reCol ? -xx.im : xx.re;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int rt = w + 4 * j;
+                if (rt < nrt) {  // uniform per wave
+                    const int row = 16 * rt + lm;
+                    const cplx<double> z = (c < cw && row < nr) ? sT[c * ld + row] : mk(0.0, 0.0);
+                    accP[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(z.re, b, accP[j], 0, 0, 0);
+                    accQ[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(z.im, bp, accQ[j], 0, 0, 0);
+                }
+            }
+        }
+        // column sums of the pair columns: column tile 3 - w of this LDS tile
+        // (the waves without a second row tile take the first column tiles)
+        {
+            const int ct = 16 * (3 - w);
+            if (ct < cw && c0 + ct + 15 >= I.nself) {  // uniform per wave
+                d4 cP = d4{0, 0, 0, 0}, cQ = d4{0, 0, 0, 0};
+                const int col = ct + lm;
+                for (int ks = 0; ks < nr; ks += 4) {
+                    const int row = ks + lk;
+                    const cplx<double> z = (col < cw && row < nr) ? sT[col * ld + row] : mk(0.0, 0.0);
+                    const cplx<double> xx = row < nr ? sxr[rh][row] : mk(0.0, 0.0);
+                    const double b = reCol ? xx.re : xx.im;
+                    const double bp = reCol ? -xx.im : xx.re;
+                    cP = __builtin_amdgcn_mfma_f64_16x16x4f64(z.re, b, cP, 0, 0, 0);
+                    cQ = __builtin_amdgcn_mfma_f64_16x16x4f64(z.im, bp, cQ, 0, 0, 0);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int c = ct + 4 * i + lk;
+                    if (c < cw && c0 + c >= I.nself) {
+                        double* o = reinterpret_cast<double*>(part + (I.cpo + c0 + c - I.nself) * NB + rh);
+                        o[reCol ? 0 : 1] = cP[i] + cQ[i];
+                    }
+                }
+            }
+        }
+        if (!more) break;
+        c0 = c0n;
+        cw = cwn;
+        ne = nen;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int rt = w + 4 * j;
+        if (rt < nrt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = 16 * rt + 4 * i + lk;
+                if (row < nr) {
+                    double* o = reinterpret_cast<double*>(part + (I.rpo + row) * NB + rh);
+                    o[reCol ? 0 : 1] = accP[j][i] + accQ[j][i];
+                }
+            }
+    }
+}
+
 // Sum of the near-field partials of tree position t, split over the 64 lanes
-// of a wave (same lane assignment and butterfly every call: deterministic).
+// of a wave (same lane assignment and butterfly every call: deterministic);
+// partial j at part[j * stride] (stride 8: the interleaved partials of eight
+// right-hand sides, part offset to the right-hand side).
 __device__ __forceinline__ cplx<double> nearFold(const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
-                                                 const cplx<double>* __restrict__ part, int64_t t, int lane) {
+                                                 const cplx<double>* __restrict__ part, int64_t t, int lane,
+                                                 int stride = 1) {
     double a = 0, b = 0;
     for (int64_t q = ptr[t] + lane; q < ptr[t + 1]; q += 64) {
-        const cplx<double> v = part[idx[q]];
+        const cplx<double> v = part[(int64_t)idx[q] * stride];
         a += v.re;
         b += v.im;
     }
@@ -1257,7 +1431,9 @@ __global__ __launch_bounds__(TB) void kNearSum(const int64_t* __restrict__ rows,
     const int64_t o = perm[t];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const cplx<double> v = nearFold(ptr, idx, part + k * ps, t, lane);
+        // eight right-hand sides: interleaved partials (kNearSymM)
+        const cplx<double> v =
+            NB == 8 ? nearFold(ptr, idx, part + k, t, lane, 8) : nearFold(ptr, idx, part + k * ps, t, lane);
         if (lane == 0) y[k * xs + o] = accumulate ? y[k * xs + o] + v : v;
     }
 }
@@ -1288,12 +1464,35 @@ void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_
         if (conjVal) NEARB_CASE(true, 2); else NEARB_CASE(false, 2);
     } else if (nb == 4) {
         if (conjVal) NEARB_CASE(true, 4); else NEARB_CASE(false, 4);
-    } else if (nb == 8) {
-        if (conjVal) NEARB_CASE(true, 8); else NEARB_CASE(false, 8);
     } else
-        throw Error(MLFMA_ERR_ARG, "right-hand-side batch must be 1, 2, 4 or 8");
+        throw Error(MLFMA_ERR_ARG, "near field: batch of 1, 2 or 4 (eight: launchNearSymM8)");
 #undef NEARB_CASE
 }
+template <typename V>
+void launchNearSymM8(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+                     const cplx<V>* val, const cplx<double>* xt, cplx<double>* part, hipStream_t st) {
+    if (nitems <= 0) return;
+    if (conjVal)
+        hipLaunchKernelGGL((kNearSymM<true, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, xt, part);
+    else
+        hipLaunchKernelGGL((kNearSymM<false, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, xt, part);
+}
+template void launchNearSymM8<double>(bool, const NearItem*, int, const int32_t*, const int64_t*,
+                                      const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t);
+template void launchNearSymM8<float>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<float>*,
+                                     const cplx<double>*, cplx<double>*, hipStream_t);
+
+// xt[i 8 + k] = x[k xs + i]: eight right-hand sides, each unknown's eight
+// values in one 128-byte line
+__global__ void kInterleave8(const cplx<double>* __restrict__ x, int64_t xs, int64_t n, cplx<double>* __restrict__ xt) {
+    const int64_t e = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (e >= 8 * n) return;
+    xt[e] = x[(e & 7) * xs + (e >> 3)];
+}
+void launchInterleave8(const cplx<double>* x, int64_t xs, int64_t n, cplx<double>* xt, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(kInterleave8, dim3((unsigned)((8 * n + TB - 1) / TB)), dim3(TB), 0, st, x, xs, n, xt);
+}
+
 template void launchNearSym<double>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<double>*,
                                     const cplx<double>*, cplx<double>*, hipStream_t, int, int64_t, int64_t, int);
 template void launchNearSym<float>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<float>*,
@@ -1477,7 +1676,8 @@ __global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, 
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         const R a = wsum(re[k]), b = wsum(im[k]);
-        const cplx<double> nearv = nearFold(sumPtr, sumIdx, part + k * ps, t, lane);
+        const cplx<double> nearv = NB == 8 ? nearFold(sumPtr, sumIdx, part + k, t, lane, 8)
+                                           : nearFold(sumPtr, sumIdx, part + k * ps, t, lane);
         if (lane == 0) {
             const cplx<double> v = nearv + Ck * mk((double)a, (double)b);
             y[k * xs + o] = accumulate ? y[k * xs + o] + v : v;

@@ -75,6 +75,14 @@ template <typename V>
 void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
                    const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb = 1,
                    int64_t xs = 0, int64_t ps = 0, int grid = 0);
+// eight right-hand sides on the matrix cores; xt: x interleaved by
+// launchInterleave8 (unknown-major, eight values per unknown); the partial
+// sums come out interleaved the same way (part[j 8 + k]), as the eight-RHS
+// folds (launchL2Tx, launchNearSum with nb = 8) read them
+template <typename V>
+void launchNearSymM8(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
+                     const cplx<V>* val, const cplx<double>* xt, cplx<double>* part, hipStream_t st);
+void launchInterleave8(const cplx<double>* x, int64_t xs, int64_t n, cplx<double>* xt, hipStream_t st);
 // the per-target M2L of one right-hand side and the near-field pairs in one
 // launch; conj: conjugated operators (lv holds the reversed pairs' offsets)
 // and conjugated values (the adjoint); windowPct: the pair blocks spread over

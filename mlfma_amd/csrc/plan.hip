@@ -892,6 +892,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // they go into the per-target M2L launch of one right-hand side, which
     // leaves them HBM bandwidth, or else (stages timed, batches, no
     // per-target level) follow the L2L
+    if (nb == 8) launchInterleave8(x, xs, hp.src.n, xInner, st);  // before any fork: every near launch reads it
     const bool serial = timed || (flags & MLFMA_FLAG_ONE_STREAM) || solveInOrder;
     const int overlap = serial ? 0 : nearOverlap;
     // M2L of the finest level needs only the leaf multipoles, and its locals
@@ -912,6 +913,15 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         } end{this, s};
         // the FP32 far field (mixed-precision inner solves) reads the FP32
         // copy of the near-field values
+        // eight right-hand sides: the matrix-core kernel on x interleaved
+        if (nb == 8) {
+            if (std::is_same<R, float>::value && nearVal32)
+                launchNearSymM8<float>(false, nearItems, nNearItems, nearColsIn, perm, nearVal32, xInner, nearPart,
+                                       s);
+            else
+                launchNearSymM8<double>(false, nearItems, nNearItems, nearColsIn, perm, nearVal, xInner, nearPart, s);
+            return;
+        }
         // beside the chain, a fixed number of blocks (nearBlocks per CU)
         const int grid = s == nearStream ? nearGrid() : 0;
         if (std::is_same<R, float>::value && nearVal32)
@@ -1123,6 +1133,7 @@ void DevicePlan::ensureBatch(int nb) {
     }
     release(nearPart);
     nearPart = (cplx<double>*)alloc(std::max<int64_t>(hp.nearParts, 1) * nb * sizeof(cplx<double>));
+    if (nb >= 8 && !xInner) xInner = (cplx<double>*)alloc(std::max<int64_t>(hp.src.n, 1) * 8 * sizeof(cplx<double>));
     batchCap = nb;
 }
 

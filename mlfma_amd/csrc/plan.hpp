@@ -170,6 +170,7 @@ struct DevicePlan {
     void apply(const cplx<double>* x, cplx<double>* y, int accumulate, hipStream_t st, bool timed, int nb = 1,
                int64_t xs = 0);
     int batchCap = 1;  // right-hand sides the field arrays and partials hold
+    cplx<double>* xInner = nullptr;  // eight right-hand sides interleaved (n x 8), for the 8-RHS near field
     // launch parameters chosen by mlfma_autotune (0: compile-time defaults /
     // the per-level rule); tuneMinBlocks[l]: block target of the M2M (and
     // adjoint L2L) row split into parent level l; tuneM2L[l]: 1 sibling-group,
