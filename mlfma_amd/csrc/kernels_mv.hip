@@ -1337,9 +1337,15 @@ __global__ __launch_bounds__(TB) void kNearSymM(const NearItem* __restrict__ ite
             ia1 = cwnn > 0 ? colIdx(cn, cwnn, t0 + TB) : 0;
         }
         // row sums: k steps of 4 columns
+        // (LDS indices clamped so every read is valid; entries past the
+        // tile's columns or the item's rows count as zero)
+#pragma unroll 4
         for (int ks = 0; ks < cw; ks += 4) {
             const int c = ks + lk;
-            const cplx<double> xx = c < cw ? sxc[rh][c] : mk(0.0, 0.0);
+            const bool cok = c < cw;
+            const int cc = min(c, cw - 1);
+            const cplx<double> xr = sxc[rh][cc];
+            const cplx<double> xx = cok ? xr : mk(0.0, 0.0);
             const double b = reCol ? xx.re : xx.im;
             const double bp = reCol ? -xx.im : xx.re;
 #pragma unroll
@@ -1347,9 +1353,10 @@ __global__ __launch_bounds__(TB) void kNearSymM(const NearItem* __restrict__ ite
                 const int rt = w + 4 * j;
                 if (rt < nrt) {  // uniform per wave
                     const int row = 16 * rt + lm;
-                    const cplx<double> z = (c < cw && row < nr) ? sT[c * ld + row] : mk(0.0, 0.0);
-                    accP[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(z.re, b, accP[j], 0, 0, 0);
-                    accQ[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(z.im, bp, accQ[j], 0, 0, 0);
+                    const cplx<double> zr = sT[cc * ld + min(row, nr - 1)];
+                    const bool ok = row < nr;
+                    accP[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? zr.re : 0.0, b, accP[j], 0, 0, 0);
+                    accQ[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? zr.im : 0.0, bp, accQ[j], 0, 0, 0);
                 }
             }
         }
@@ -1360,14 +1367,19 @@ __global__ __launch_bounds__(TB) void kNearSymM(const NearItem* __restrict__ ite
             if (ct < cw && c0 + ct + 15 >= I.nself) {  // uniform per wave
                 d4 cP = d4{0, 0, 0, 0}, cQ = d4{0, 0, 0, 0};
                 const int col = ct + lm;
+                const int colc = min(col, cw - 1);
+#pragma unroll 4
                 for (int ks = 0; ks < nr; ks += 4) {
                     const int row = ks + lk;
-                    const cplx<double> z = (col < cw && row < nr) ? sT[col * ld + row] : mk(0.0, 0.0);
-                    const cplx<double> xx = row < nr ? sxr[rh][row] : mk(0.0, 0.0);
+                    const bool ok = col < cw && row < nr;
+                    const int rc = min(row, nr - 1);
+                    const cplx<double> zr = sT[colc * ld + rc];
+                    const cplx<double> xr = sxr[rh][rc];
+                    const cplx<double> xx = row < nr ? xr : mk(0.0, 0.0);
                     const double b = reCol ? xx.re : xx.im;
                     const double bp = reCol ? -xx.im : xx.re;
-                    cP = __builtin_amdgcn_mfma_f64_16x16x4f64(z.re, b, cP, 0, 0, 0);
-                    cQ = __builtin_amdgcn_mfma_f64_16x16x4f64(z.im, bp, cQ, 0, 0, 0);
+                    cP = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? zr.re : 0.0, b, cP, 0, 0, 0);
+                    cQ = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? zr.im : 0.0, bp, cQ, 0, 0, 0);
                 }
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -1413,6 +1425,44 @@ __device__ __forceinline__ cplx<double> nearFold(const int64_t* __restrict__ ptr
         b += v.im;
     }
     return mk(wsum(a), wsum(b));
+}
+
+// The near-field sums of tree position t for eight right-hand sides from the
+// interleaved partials (part[j 8 + k]): lane l reads entry l / 8 of each step
+// of eight for right-hand side l % 8 (eight lanes per 128-byte line), then the
+// lanes of one right-hand side are added by a fixed butterfly.  Returns the
+// sum of right-hand side lane % 8.
+__device__ __forceinline__ cplx<double> nearFold8(const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                                  const cplx<double>* __restrict__ part, int64_t t, int lane) {
+    double a = 0, b = 0;
+    const int k = lane & 7;
+    for (int64_t q = ptr[t] + (lane >> 3); q < ptr[t + 1]; q += 8) {
+        const cplx<double> v = part[(int64_t)idx[q] * 8 + k];
+        a += v.re;
+        b += v.im;
+    }
+    for (int o = 8; o < 64; o <<= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    return mk(a, b);
+}
+
+// y[k xs + perm[t]] (=|+=) near-field sum of right-hand side k, eight
+// right-hand sides, one wave per listed row
+__global__ __launch_bounds__(TB) void kNearSum8(const int64_t* __restrict__ rows, int64_t nrows,
+                                                const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                                const cplx<double>* __restrict__ part, const int64_t* __restrict__ perm,
+                                                cplx<double>* __restrict__ y, int64_t xs, int accumulate) {
+    const int64_t w = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nrows) return;  // uniform per wave
+    const int64_t t = rows[w];
+    const cplx<double> v = nearFold8(ptr, idx, part, t, lane);
+    if (lane < 8) {
+        cplx<double>* o = y + lane * xs + perm[t];
+        *o = accumulate ? *o + v : v;
+    }
 }
 
 // y[perm[t]] (=|+=) near-field sum, one wave per listed row (rows that have no
@@ -1633,7 +1683,9 @@ void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, co
     if (nb == 1) NSUM_CASE(1);
     else if (nb == 2) NSUM_CASE(2);
     else if (nb == 4) NSUM_CASE(4);
-    else if (nb == 8) NSUM_CASE(8);
+    else if (nb == 8)  // interleaved partials of the eight-RHS near field
+        hipLaunchKernelGGL(kNearSum8, dim3((unsigned)nbk), dim3(TB), 0, st, rows, nrows, sumPtr, sumIdx, part, perm,
+                           y, xs, accumulate);
     else throw Error(MLFMA_ERR_ARG, "right-hand-side batch must be 1, 2, 4 or 8");
 #undef NSUM_CASE
 }
@@ -1787,6 +1839,105 @@ __global__ __launch_bounds__(TB) void kL2TN(const int64_t* __restrict__ rowPos, 
         }
     }
 }
+
+// Eight right-hand sides on the matrix cores: per far-field leaf
+// Y = conj(P) L, P the leaf's patterns (rows x 2 ndir complex, streamed from
+// HBM once), L its locals of the eight right-hand sides (2 ndir x 8, staged
+// in LDS as FP64).  With B = [Re L | Im L] and B' = [Im L | -Re L] (16 real
+// columns), Re P . B + Im P . B' = [Re Y | Im Y] (v_mfma_f64_16x16x4f64, the
+// two products in separate accumulators); row tiles of 16 go to the waves.
+// y already holds the near-field sums (kNearSum8 over the same rows);
+// y += C k Y.  Block = one leaf; LDS 2 ndir x 8 x 16 bytes.
+template <typename R>
+__global__ __launch_bounds__(TB) void kL2TM(const LeafItem* __restrict__ items, int nitems, int per,
+                                            const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc, int64_t fs,
+                                            cplx<double> Ck, const int64_t* __restrict__ perm,
+                                            cplx<double>* __restrict__ y, int64_t xs) {
+    using d4 = double __attribute__((ext_vector_type(4)));
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double>* sL = reinterpret_cast<cplx<double>*>(smem);  // [K][8]
+    const int item = xcdItem(blockIdx.x, per);
+    if (item >= nitems) return;
+    const LeafItem it = items[item];
+    const int K = 2 * it.ndir;
+    for (int e = threadIdx.x; e < 8 * K; e += TB) {
+        const int r = e / K, kk = e - r * K;
+        const cplx<R> v = Lc[r * fs + it.fOff + kk];
+        sL[kk * 8 + r] = mk((double)v.re, (double)v.im);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int rh = lm & 7;
+    const bool reCol = lm < 8;
+    const int nrt = (it.cnt + 15) >> 4;
+    for (int rt = w; rt < nrt; rt += 4) {  // uniform per wave
+        const int row = 16 * rt + lm;
+        const bool rowOk = row < it.cnt;
+        const cplx<R>* pr = rp + it.rpOff + (int64_t)(rowOk ? row : 0) * K;
+        d4 P = d4{0, 0, 0, 0}, Q = d4{0, 0, 0, 0};
+        // software pipeline: the pattern loads of the next U steps are issued
+        // before the products of these U (indices clamped, so every load is
+        // valid; entries past K or the leaf's rows count as zero)
+        constexpr int U = 8;
+        cplx<R> abuf[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) abuf[u] = pr[min(4 * u + lk, K - 1)];
+        for (int k0 = 0; k0 < K; k0 += 4 * U) {
+            cplx<R> nbuf[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) nbuf[u] = pr[min(k0 + 4 * U + 4 * u + lk, K - 1)];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = k0 + 4 * u + lk;
+                const bool ok = kk < K;
+                const cplx<double> l = sL[min(kk, K - 1) * 8 + rh];
+                const double ar = (ok && rowOk) ? (double)abuf[u].re : 0.0;
+                const double ai = (ok && rowOk) ? (double)abuf[u].im : 0.0;
+                const double b = reCol ? l.re : l.im;
+                const double bp = reCol ? l.im : -l.re;
+                P = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b, P, 0, 0, 0);
+                Q = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bp, Q, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) abuf[u] = nbuf[u];
+        }
+        // element i: row 16 rt + 4 i + lk, column lm (Re of RHS lm for
+        // lm < 8, Im of RHS lm - 8 otherwise); lane l ^ 8 holds the other part
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double v = P[i] + Q[i];
+            const double u = __shfl_xor(v, 8, 64);
+            const int r = 16 * rt + 4 * i + lk;
+            if (reCol && r < it.cnt) {
+                cplx<double>* o = y + rh * xs + perm[it.sbeg + r];
+                *o = *o + Ck * mk(v, u);
+            }
+        }
+    }
+}
+
+template <typename R>
+bool launchL2TM8(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<R>* Lc, int64_t fs,
+                 cplx<double> Ck, const int64_t* rowPos, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
+                 const cplx<double>* part, const int64_t* perm, cplx<double>* y, int64_t xs, int accumulate,
+                 size_t ldsMax, hipStream_t st) {
+    const size_t lds = (size_t)2 * maxndir * 8 * sizeof(cplx<double>);
+    if (lds > ldsMax) return false;
+    launchNearSum(rowPos, nrows, sumPtr, sumIdx, part, perm, y, accumulate, st, 8, xs, 0);
+    if (nitems <= 0) return true;
+    const int per = (nitems + 7) / 8;
+    hipLaunchKernelGGL((kL2TM<R>), dim3(8 * per), dim3(TB), lds, st, items, nitems, per, rp, Lc, fs, Ck, perm, y, xs);
+    return true;
+}
+template bool launchL2TM8<double>(const LeafItem*, int, int, const cplx<double>*, const cplx<double>*, int64_t,
+                                  cplx<double>, const int64_t*, int64_t, const int64_t*, const int32_t*,
+                                  const cplx<double>*, const int64_t*, cplx<double>*, int64_t, int, size_t,
+                                  hipStream_t);
+template bool launchL2TM8<float>(const LeafItem*, int, int, const cplx<float>*, const cplx<float>*, int64_t,
+                                 cplx<double>, const int64_t*, int64_t, const int64_t*, const int32_t*,
+                                 const cplx<double>*, const int64_t*, cplx<double>*, int64_t, int, size_t,
+                                 hipStream_t);
 
 template <typename R>
 void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, const LeafItem* items, const cplx<R>* rp,

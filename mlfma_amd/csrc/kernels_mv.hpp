@@ -99,6 +99,15 @@ void launchToFloat(const cplx<double>* in, cplx<float>* out, int64_t n, hipStrea
 void launchBlockPrec(const PrecBlock* blocks, int nblocks, int maxcnt, const cplx<double>* P, const int64_t* perm,
                      const cplx<double>* x, cplx<double>* y, int nb, int64_t xs, bool adjoint, hipStream_t st);
 void launchGatherValues(const cplx<double>* val, const int64_t* idx, int64_t n, cplx<double>* out, hipStream_t st);
+// eight right-hand sides: the near-field sums of the far-field rows into y
+// (interleaved partials), then y += C k conj(P) L per leaf on the matrix
+// cores; false (nothing launched) when a leaf's locals of eight right-hand
+// sides exceed the LDS limit
+template <typename R>
+bool launchL2TM8(const LeafItem* items, int nitems, int maxndir, const cplx<R>* rp, const cplx<R>* Lc, int64_t fs,
+                 cplx<double> Ck, const int64_t* rowPos, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
+                 const cplx<double>* part, const int64_t* perm, cplx<double>* y, int64_t xs, int accumulate,
+                 size_t ldsMax, hipStream_t st);
 void launchNearSum(const int64_t* rows, int64_t nrows, const int64_t* sumPtr, const int32_t* sumIdx,
                    const cplx<double>* part, const int64_t* perm, cplx<double>* y, int accumulate, hipStream_t st,
                    int nb = 1, int64_t xs = 0, int64_t ps = 0);

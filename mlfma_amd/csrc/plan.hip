@@ -1021,8 +1021,10 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
     probeBegin(MLFMA_GROUP_L2T, st);
-    launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, Lc, Ck, nearSumPtr, nearSumIdx, nearPart, perm, y,
-                  accumulate, st, nb, fs, xs, ps);
+    if (nb != 8 || !launchL2TM8<R>(leafItems, nLeafItems, maxLeafNdir, rp, Lc, fs, Ck, farRowPos, nFarRows,
+                                   nearSumPtr, nearSumIdx, nearPart, perm, y, xs, accumulate, ldsMax, st))
+        launchL2Tx<R>(farRowPos, farRowItem, nFarRows, leafItems, rp, Lc, Ck, nearSumPtr, nearSumIdx, nearPart, perm,
+                      y, accumulate, st, nb, fs, xs, ps);
     probeEnd(MLFMA_GROUP_L2T, st);
     if (timed) hipCheck(hipEventRecord(ev[7], st), "event");
 }
