@@ -1259,7 +1259,10 @@ __global__ __launch_bounds__(TB) void kNearSymM(const NearItem* __restrict__ ite
     const NearItem I = items[blockIdx.x];
     const int nr = I.nr;  // 1..128
     const int ld = nr | 1;
-    const int cwMax = min(CWM, (NTM / ld) & ~3);
+    // columns per LDS tile: a multiple of 16 (whole MFMA column tiles for the
+    // column sums) where the tile holds 16, else of 4
+    const int tcw = NTM / ld;
+    const int cwMax = min(CWM, tcw >= 16 ? tcw & ~15 : tcw & ~3);
     const int t0 = threadIdx.x;
     const int lane = t0 & 63;
     const int w = t0 >> 6;
