@@ -516,6 +516,9 @@ def kernel_bytes(group, info, n):
                 multipoles and target locals (32 B per box direction) once;
                 streamed: 32 B per (pair, direction) of source reads instead
       M2L_NEAR  the M2L bytes of the per-target levels plus NEAR
+      TREE_UP   S2M plus M2M: every multipole of levels >= 3 read once and
+                every multipole of levels 2 .. max-1 written once (32 B per
+                box direction)
     Returns (bytes, streamed bytes)."""
     c = 16
     ml = info["max_level"]
@@ -533,11 +536,14 @@ def kernel_bytes(group, info, n):
         return m2l_comp, m2l_str
     if group == "M2L_NEAR":
         return m2l_comp + near, m2l_str + near
+    if group == "TREE_UP":
+        m2m = sum((B[l + 1] * nd[l + 1] + B[l] * nd[l]) * 2 * c for l in range(2, ml))
+        return leaf + m2m, leaf + m2m
     return None, None
 
 
 KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2TN", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
-                "M2M": "kM2MFused", "L2L": "kL2LFused", "M2L_SIBLING": "kM2LGroup"}
+                "M2M": "kM2MFused", "L2L": "kL2LFused", "M2L_SIBLING": "kM2LGroup", "TREE_UP": "kTreeUp"}
 
 
 def main():
@@ -635,7 +641,8 @@ def main():
     schedule = {"near_overlap": plan.get_param(A.MLFMA_PARAM_NEAR_OVERLAP),
                 "m2l_split": plan.get_param(A.MLFMA_PARAM_M2L_SPLIT),
                 "near_window": plan.get_param(A.MLFMA_PARAM_NEAR_WINDOW),
-                "near_blocks_per_cu": plan.get_param(A.MLFMA_PARAM_NEAR_BLOCKS)}
+                "near_blocks_per_cu": plan.get_param(A.MLFMA_PARAM_NEAR_BLOCKS),
+                "tree_walk": plan.get_param(A.MLFMA_PARAM_TREE_WALK)}
     log("autotune ms", tune_before, "->", tune_after, schedule)
 
     def kname(g):
@@ -663,12 +670,14 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
+    if plan.get_param(A.MLFMA_PARAM_TREE_STALLS):
+        raise SystemExit("a tree-walk wait hit its poll bound: the timed matvecs are wrong")
     value = args.steps * args.nrhs / elapsed
     y_timed = y.cpu().numpy()
     # the same K matvecs once per launch group with that group's launches
     # bracketed by HIP events on the stream each runs on (same schedule)
     groups = {}
-    for g in ["S2M", "M2M", "M2L_TARGET", "M2L_SIBLING", "M2L_NEAR", "L2L", "NEAR", "L2T"]:
+    for g in ["TREE_UP", "S2M", "M2M", "M2L_TARGET", "M2L_SIBLING", "M2L_NEAR", "L2L", "NEAR", "L2T"]:
         plan.probe(g)
         for _ in range(args.steps):
             plan.matvec(x, y)

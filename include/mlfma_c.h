@@ -392,7 +392,17 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
  *                              in order on one stream
  *   MLFMA_PARAM_NEAR_BLOCKS    0..8: near-field pairs beside the chain run
  *                              as that many resident blocks per CU looping
- *                              over the work items (0: one block per item) */
+ *                              over the work items (0: one block per item)
+ *   MLFMA_PARAM_TREE_WALK      bit 0: the upward pass of one right-hand side
+ *                              (S2M and the M2M of every level) runs as one
+ *                              persistent launch walking a packed worklist
+ *                              of (leaf) and (level, parent box, row run)
+ *                              units, each waiting on the completion
+ *                              counters of the units that write its inputs
+ *   MLFMA_PARAM_TREE_STALLS    (get only) waits of the tree walks that hit
+ *                              their poll bound since the plan was made (a
+ *                              non-zero count means a wrong result; reading
+ *                              it waits for the plan's stream) */
 #define MLFMA_PARAM_NEAR_WINDOW 1
 #define MLFMA_PARAM_NEAR_OVERLAP 2
 #define MLFMA_PARAM_M2L_SPLIT 3
@@ -400,6 +410,8 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
 #define MLFMA_PARAM_PROBE_GROUP 5  /* MLFMA_GROUP_* whose launches carry probe
                                       events (0: none); see mlfma_probe_read */
 #define MLFMA_PARAM_NEAR_BLOCKS 6
+#define MLFMA_PARAM_TREE_WALK 7
+#define MLFMA_PARAM_TREE_STALLS 8
 int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value);
 int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value);
 
@@ -420,7 +432,8 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value);
 #define MLFMA_GROUP_L2L 6          /* one L2L level (kL2LFused)                  */
 #define MLFMA_GROUP_NEAR 7         /* near-field pair products (kNearSym)        */
 #define MLFMA_GROUP_L2T 8          /* kL2Tx                                      */
-#define MLFMA_GROUP_LAST 8
+#define MLFMA_GROUP_TREE_UP 9      /* S2M + every M2M level as one walk (kTreeUp)*/
+#define MLFMA_GROUP_LAST 9
 int mlfma_probe_read(mlfma_plan* plan, double* ms, int64_t* launches);
 
 /* The launch parameters mlfma_autotune chose, as a record of int32 values

@@ -53,6 +53,29 @@ __device__ __forceinline__ float wsum(float v) {
 // XCD b % 8; with a grid of 8 * per blocks XCD x gets items [x*per, (x+1)*per).
 __device__ __forceinline__ int xcdItem(int b, int per) { return (b & 7) * per + (b >> 3); }
 
+// Device-coherent loads and stores (relaxed atomics at agent scope: on
+// gfx950 they carry sc1 and go past the XCD's own L2, which is not coherent
+// with the other XCDs' L2), for values handed from one block to another
+// within a launch without agent-scope fences, which write back and
+// invalidate the whole L2.
+template <typename R, bool COH>
+__device__ __forceinline__ cplx<R> ldc(const cplx<R>* p) {
+    if constexpr (COH)
+        return mk<R>(__hip_atomic_load(&p->re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                     __hip_atomic_load(&p->im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    else
+        return *p;
+}
+template <typename R, bool COH>
+__device__ __forceinline__ void stc(cplx<R>* p, cplx<R> v) {
+    if constexpr (COH) {
+        __hip_atomic_store(&p->re, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->im, v.im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        *p = v;
+    }
+}
+
 // The value lane l (wave-uniform) holds, read by every lane (two v_readlane).
 __device__ __forceinline__ double readlaneD(double v, int l) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);

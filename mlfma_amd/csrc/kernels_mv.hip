@@ -35,6 +35,61 @@ constexpr int TB = 256;
 // clamped direction and store nothing.  NB right-hand sides share every
 // pattern read: x has column stride xs (input order), the fields of
 // right-hand side r start at M + r fs.
+//
+// One right-hand side, directions [d0, d0 + blockDim.x) of leaf it (d0 below
+// the leaf's direction count; uniform per block); COH: device-coherent
+// stores of the multipole.
+template <typename R, bool COH = false>
+__device__ __forceinline__ void s2mOne(const LeafItem& it, int d0, const cplx<R>* __restrict__ rp,
+                                       const cplx<double>* __restrict__ x, const int64_t* __restrict__ perm,
+                                       cplx<R>* __restrict__ M) {
+    const int d = d0 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int dc = d < it.ndir ? d : it.ndir - 1;
+    const int64_t stride = (int64_t)it.ndir * 2;
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
+    for (int q0 = 0; q0 < it.cnt; q0 += 64) {
+        const int nq = min(64, it.cnt - q0);
+        const cplx<double> xl = lane < nq ? x[perm[it.sbeg + q0 + lane]] : mk(0.0, 0.0);
+        const cplx<R>* p = rp + it.rpOff + 2 * dc + q0 * stride;
+        auto xq = [&](int q) { return cast<R>(mk(readlaneD(xl.re, q), readlaneD(xl.im, q))); };
+        int q = 0;
+        for (; q + 3 < nq; q += 4) {
+            const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
+            const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
+            const cplx<R> r20 = p[(q + 2) * stride], r21 = p[(q + 2) * stride + 1];
+            const cplx<R> r30 = p[(q + 3) * stride], r31 = p[(q + 3) * stride + 1];
+            const cplx<R> x0 = xq(q), x1 = xq(q + 1), x2 = xq(q + 2), x3 = xq(q + 3);
+            a0 = a0 + x0 * r00;
+            a1 = a1 + x0 * r01;
+            b0 = b0 + x1 * r10;
+            b1 = b1 + x1 * r11;
+            a0 = a0 + x2 * r20;
+            a1 = a1 + x2 * r21;
+            b0 = b0 + x3 * r30;
+            b1 = b1 + x3 * r31;
+        }
+        for (; q + 1 < nq; q += 2) {
+            const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
+            const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
+            const cplx<R> x0 = xq(q), x1 = xq(q + 1);
+            a0 = a0 + x0 * r00;
+            a1 = a1 + x0 * r01;
+            b0 = b0 + x1 * r10;
+            b1 = b1 + x1 * r11;
+        }
+        if (q < nq) {  // only in the last run of rows (the others hold 64)
+            const cplx<R> x0 = xq(q);
+            a0 = a0 + x0 * p[q * stride];
+            a1 = a1 + x0 * p[q * stride + 1];
+        }
+    }
+    if (d < it.ndir) {
+        stc<R, COH>(M + it.fOff + 2 * d, a0 + b0);
+        stc<R, COH>(M + it.fOff + 2 * d + 1, a1 + b1);
+    }
+}
+
 template <typename R, int NB>
 __global__ void kS2Mx(const LeafItem* __restrict__ items, const int32_t* __restrict__ order,
                       const cplx<R>* __restrict__ rp, const cplx<double>* __restrict__ x, int64_t xs,
@@ -43,50 +98,7 @@ __global__ void kS2Mx(const LeafItem* __restrict__ items, const int32_t* __restr
     const int d = blockIdx.y * blockDim.x + threadIdx.x;
     if constexpr (NB == 1) {
         if (blockIdx.y * blockDim.x >= it.ndir) return;  // uniform per block
-        const int lane = threadIdx.x & 63;
-        const int dc = d < it.ndir ? d : it.ndir - 1;
-        const int64_t stride = (int64_t)it.ndir * 2;
-        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), b0 = mk<R>(0, 0), b1 = mk<R>(0, 0);
-        for (int q0 = 0; q0 < it.cnt; q0 += 64) {
-            const int nq = min(64, it.cnt - q0);
-            const cplx<double> xl = lane < nq ? x[perm[it.sbeg + q0 + lane]] : mk(0.0, 0.0);
-            const cplx<R>* p = rp + it.rpOff + 2 * dc + q0 * stride;
-            auto xq = [&](int q) { return cast<R>(mk(readlaneD(xl.re, q), readlaneD(xl.im, q))); };
-            int q = 0;
-            for (; q + 3 < nq; q += 4) {
-                const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
-                const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
-                const cplx<R> r20 = p[(q + 2) * stride], r21 = p[(q + 2) * stride + 1];
-                const cplx<R> r30 = p[(q + 3) * stride], r31 = p[(q + 3) * stride + 1];
-                const cplx<R> x0 = xq(q), x1 = xq(q + 1), x2 = xq(q + 2), x3 = xq(q + 3);
-                a0 = a0 + x0 * r00;
-                a1 = a1 + x0 * r01;
-                b0 = b0 + x1 * r10;
-                b1 = b1 + x1 * r11;
-                a0 = a0 + x2 * r20;
-                a1 = a1 + x2 * r21;
-                b0 = b0 + x3 * r30;
-                b1 = b1 + x3 * r31;
-            }
-            for (; q + 1 < nq; q += 2) {
-                const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
-                const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
-                const cplx<R> x0 = xq(q), x1 = xq(q + 1);
-                a0 = a0 + x0 * r00;
-                a1 = a1 + x0 * r01;
-                b0 = b0 + x1 * r10;
-                b1 = b1 + x1 * r11;
-            }
-            if (q < nq) {  // only in the last run of rows (the others hold 64)
-                const cplx<R> x0 = xq(q);
-                a0 = a0 + x0 * p[q * stride];
-                a1 = a1 + x0 * p[q * stride + 1];
-            }
-        }
-        if (d < it.ndir) {
-            M[it.fOff + 2 * d] = a0 + b0;
-            M[it.fOff + 2 * d + 1] = a1 + b1;
-        }
+        s2mOne<R>(it, blockIdx.y * blockDim.x, rp, x, perm, M);
     } else {
         if (d >= it.ndir) return;
         const cplx<R>* p = rp + it.rpOff + 2 * d;
@@ -150,20 +162,19 @@ void launchS2Mx(const LeafItem* items, const int32_t* order, int nitems, int max
 // rows are staged in LDS; the theta interpolation (pole rule: reflected node,
 // phi + pi, both components negated) goes to LDS, and the periodic phi
 // interpolation writes the parent.
-template <typename R>
-__global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stemSlot, const int32_t* __restrict__ ptr,
-                                                const int32_t* __restrict__ child, const int32_t* __restrict__ oct,
-                                                int nstem, int per, int rows, int nthP, int nphP, int mthC, int mphC,
-                                                int p, const int32_t* __restrict__ thNear,
-                                                const double* __restrict__ thW, const int32_t* __restrict__ phNear,
-                                                const double* __restrict__ phW, const cplx<R>* __restrict__ phaseUp,
-                                                const cplx<R>* __restrict__ Mc, cplx<R>* __restrict__ Mp, int64_t fs) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int item = xcdItem(blockIdx.x, per);
-    if (item >= nstem) return;
-    Mc += blockIdx.z * fs;  // right-hand side blockIdx.z
-    Mp += blockIdx.z * fs;
-    const int j0 = blockIdx.y * rows;  // parent theta rows [j0, j0 + nr)
+//
+// m2mFused: the work of one block of BS threads, parent stem `item`, parent
+// theta rows [j0, j0 + rows) (clipped to the grid); smem: the LDS bytes of
+// m2mRows; COH: device-coherent loads of the children and stores of the
+// parent.
+template <typename R, int BS, bool COH = false>
+__device__ __forceinline__ void m2mFused(int item, int j0, int rows, const int32_t* __restrict__ stemSlot,
+                                         const int32_t* __restrict__ ptr, const int32_t* __restrict__ child,
+                                         const int32_t* __restrict__ oct, int nthP, int nphP, int mthC, int mphC,
+                                         int p, const int32_t* __restrict__ thNear, const double* __restrict__ thW,
+                                         const int32_t* __restrict__ phNear, const double* __restrict__ phW,
+                                         const cplx<R>* __restrict__ phaseUp, const cplx<R>* __restrict__ Mc,
+                                         cplx<R>* __restrict__ Mp, unsigned char* smem) {
     const int nr = min(rows, nthP - j0);
     const int ndirC = mthC * mphC;
     const int nInner = nr * mphC;
@@ -183,14 +194,14 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
     const int nTh = nr * tp, nPh = nphP * tp;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int i = t + u * TB;
+        const int i = t + u * BS;
         if (i < nTh) wTh[u] = thW[j0 * tp + i];
         if (i < nPh) wPh[u] = phW[i];
     }
     const int nearTh = t < nr ? thNear[j0 + t] : 0;
     const int nearPh = t < nphP ? phNear[t] : 0;
     const int c0 = ptr[item], nc = ptr[item + 1] - c0;  // at most 8 children
-    for (int d = t; d < ndirC; d += TB) {
+    for (int d = t; d < ndirC; d += BS) {
         // every child's loads are issued before the first product
         cplx<R> s[8], m0[8], m1[8];
 #pragma unroll
@@ -198,8 +209,8 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
             if (k < nc) {
                 const cplx<R>* Mb = Mc + (int64_t)child[c0 + k] * ndirC * 2;
                 s[k] = phaseUp[(int64_t)oct[c0 + k] * ndirC + d];
-                m0[k] = Mb[2 * d];
-                m1[k] = Mb[2 * d + 1];
+                m0[k] = ldc<R, COH>(Mb + 2 * d);
+                m1[k] = ldc<R, COH>(Mb + 2 * d + 1);
             }
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
 #pragma unroll
@@ -213,18 +224,18 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int i = t + u * TB;
+        const int i = t + u * BS;
         if (i < nTh) tThW[i] = (R)wTh[u];
         if (i < nPh) tPhW[i] = (R)wPh[u];
     }
     if (t < nr) tThNear[t] = nearTh;
     if (t < nphP) tPhNear[t] = nearPh;
-    for (int i = t + U * TB; i < nTh; i += TB) tThW[i] = (R)thW[j0 * tp + i];
-    for (int i = t + U * TB; i < nPh; i += TB) tPhW[i] = (R)phW[i];
-    for (int i = t + TB; i < nr; i += TB) tThNear[i] = thNear[j0 + i];
-    for (int i = t + TB; i < nphP; i += TB) tPhNear[i] = phNear[i];
+    for (int i = t + U * BS; i < nTh; i += BS) tThW[i] = (R)thW[j0 * tp + i];
+    for (int i = t + U * BS; i < nPh; i += BS) tPhW[i] = (R)phW[i];
+    for (int i = t + BS; i < nr; i += BS) tThNear[i] = thNear[j0 + i];
+    for (int i = t + BS; i < nphP; i += BS) tPhNear[i] = phNear[i];
     __syncthreads();
-    for (int e = threadIdx.x; e < nInner; e += TB) {
+    for (int e = threadIdx.x; e < nInner; e += BS) {
         const int jr = e / mphC, iph = e - jr * mphC;
         const int near = tThNear[jr];
         const int iphf = iph + (iph < mphC / 2 ? mphC / 2 : -mphC / 2);
@@ -244,7 +255,7 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
     }
     __syncthreads();
     cplx<R>* out = Mp + ((int64_t)stemSlot[item] * nthP + j0) * nphP * 2;
-    for (int e = threadIdx.x; e < nr * nphP; e += TB) {
+    for (int e = threadIdx.x; e < nr * nphP; e += BS) {
         const int jr = e / nphP, jph = e - jr * nphP;
         const int near = tPhNear[jph];
         const cplx<R>* in = sI + 2 * jr * mphC;
@@ -257,9 +268,25 @@ __global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stem
             a0 = a0 + w * in[2 * i];
             a1 = a1 + w * in[2 * i + 1];
         }
-        out[2 * e] = a0;
-        out[2 * e + 1] = a1;
+        stc<R, COH>(out + 2 * e, a0);
+        stc<R, COH>(out + 2 * e + 1, a1);
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(TB) void kM2MFused(const int32_t* __restrict__ stemSlot, const int32_t* __restrict__ ptr,
+                                                const int32_t* __restrict__ child, const int32_t* __restrict__ oct,
+                                                int nstem, int per, int rows, int nthP, int nphP, int mthC, int mphC,
+                                                int p, const int32_t* __restrict__ thNear,
+                                                const double* __restrict__ thW, const int32_t* __restrict__ phNear,
+                                                const double* __restrict__ phW, const cplx<R>* __restrict__ phaseUp,
+                                                const cplx<R>* __restrict__ Mc, cplx<R>* __restrict__ Mp, int64_t fs) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int item = xcdItem(blockIdx.x, per);
+    if (item >= nstem) return;
+    // right-hand side blockIdx.z; parent theta rows from blockIdx.y * rows
+    m2mFused<R, TB>(item, blockIdx.y * rows, rows, stemSlot, ptr, child, oct, nthP, nphP, mthC, mphC, p, thNear, thW,
+                    phNear, phW, phaseUp, Mc + blockIdx.z * fs, Mp + blockIdx.z * fs, smem);
 }
 
 template <typename R>
@@ -271,6 +298,19 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
     if (minBlocks <= 0) minBlocks = MLFMA_M2M_MIN_BLOCKS;
     // split the parent's theta rows so that there are enough blocks for the
     // device (each block sums the children on the child grid again)
+    size_t lds = 0;
+    const int rows = m2mRows<R>(nstem, nthP, nphP, mthC, mphC, p, ldsMax, minBlocks, &lds);
+    if (rows == 0) return false;
+    const int per = (nstem + 7) / 8;
+    dim3 g(8 * per, (nthP + rows - 1) / rows, nb);
+    hipLaunchKernelGGL((kM2MFused<R>), g, dim3(TB), lds, st, stemSlot, ptr, child, oct, nstem, per, rows, nthP, nphP,
+                       mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp, fs);
+    return true;
+}
+
+template <typename R>
+int m2mRows(int nstem, int nthP, int nphP, int mthC, int mphC, int p, size_t ldsMax, int minBlocks, size_t* lds) {
+    if (minBlocks <= 0) minBlocks = MLFMA_M2M_MIN_BLOCKS;
     int rows = nthP;
     while (rows > 2 && (int64_t)nstem * ((nthP + rows - 1) / rows) < minBlocks) rows = (rows + 1) / 2;
     const int tp = 2 * p;
@@ -279,12 +319,146 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
                sizeof(R) * ((size_t)r + nphP) * tp + sizeof(int) * ((size_t)r + nphP);
     };
     while (rows > 1 && ldsFor(rows) > ldsMax) rows = (rows + 1) / 2;
-    if (ldsFor(rows) > ldsMax) return false;
-    const int per = (nstem + 7) / 8;
-    dim3 g(8 * per, (nthP + rows - 1) / rows, nb);
-    hipLaunchKernelGGL((kM2MFused<R>), g, dim3(TB), ldsFor(rows), st, stemSlot, ptr, child, oct, nstem, per, rows, nthP,
-                       nphP, mthC, mphC, p, thNear, thW, phNear, phW, phaseUp, Mc, Mp, fs);
-    return true;
+    *lds = ldsFor(rows);
+    return *lds > ldsMax ? 0 : rows;
+}
+
+// ------------------------------------------------------------------ tree walks
+// Persistent kernels over a packed worklist of units of several stages and
+// levels.  A block takes units in ticket order (one atomic per unit) until
+// the list is exhausted.  The list is in dependency order: every unit waits
+// only for units of lower tickets, which running blocks hold, so the walk
+// always progresses whatever the grid size and residency.  A unit waits by
+// polling the completion counters of the units it reads from, and counts
+// its own completion once its stores are complete.  Values handed between
+// units are written and read device-coherently (ldc / stc: past the XCD's
+// L2), and the counters are relaxed agent-scope atomics; the fences are
+// workgroup-scope (order only), since an agent-scope fence writes back and
+// invalidates the whole L2 of the XCD on gfx950.  Waits are bounded: after
+// kTreeSpinLimit polls (about a tenth of a second), or once any wait of the
+// plan has stalled, a wait counts a stall and goes on, so a wrong count gives
+// wrong values, never a hang.  The last block out clears every counter but
+// the stall count, so the next launch (also inside a captured graph) starts
+// from zero.  Counters: ctr[0, nctr - 3) per producing unit group, then the
+// ticket, the exit count and the stall count.
+constexpr int kTreeSpinLimit = 1 << 16;
+
+__device__ __forceinline__ void treeWait(const int32_t* c, int need, int32_t* stalls) {
+    if (__hip_atomic_load(stalls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    for (int i = 0; __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need; ++i) {
+        if (i >= kTreeSpinLimit) {
+            __hip_atomic_fetch_add(stalls, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// s_waitcnt vmcnt(0): every memory access of the wave has completed
+__device__ __forceinline__ void waitVmem() { __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8)); }
+
+// One step of a block's walk: count the block's previous unit (done >= 0),
+// once every wave's stores of it have completed, and take the next ticket.
+// The two atomics are one thread-0 region between two barriers, and the
+// ticket is read into a scalar register (readfirstlane), so the loop around
+// the steps stays uniform.  (With a thread-0 region at the end of the loop
+// body and another at its start, and the ticket read as a vector value, the
+// structurizer turned the loop into two nested ones with thread 0 alone in
+// the outer one: the wave's other lanes went on to the next step's barriers
+// with the old ticket and thread 0 never ran again.)
+__device__ __forceinline__ int treeStep(int32_t* ctr, int done, int32_t* ticket, int* sU) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    waitVmem();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (done >= 0) __hip_atomic_fetch_add(ctr + done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *sU = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*sU);
+}
+
+// after the waits of lane-held counters: the block's device-coherent loads
+// of the data they guard come after them
+__device__ __forceinline__ void treeAcquire() {
+    __syncthreads();
+    waitVmem();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// the block's counter updates have completed before it counts its exit; the
+// last block out sees every other block's completed
+__device__ __forceinline__ void treeExit(int32_t* ctr, int nctr, int* sU) {
+    if (threadIdx.x == 0) {
+        waitVmem();
+        *sU = __hip_atomic_fetch_add(ctr + nctr - 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+              (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(*sU))
+        for (int i = threadIdx.x; i < nctr - 1; i += blockDim.x) ctr[i] = 0;
+}
+
+// The upward pass of one right-hand side: S2M units (leaf item, direction
+// chunk of BS; leaves in the S2M order, largest first), then the fused M2M
+// units (parent stem, run of parent theta rows) level by level from the
+// finest parent level.  An M2M unit waits for every S2M or M2M unit that
+// writes one of its children.
+template <typename R, int BS>
+__global__ __launch_bounds__(BS) void kTreeUp(TreeUp<R> tu) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int sU;
+    int32_t* ticket = tu.ctr + tu.nctr - 3;
+    int32_t* stalls = tu.ctr + tu.nctr - 1;
+    int done = -1;  // the counter of the block's previous unit
+    for (;;) {
+        const int u = treeStep(tu.ctr, done, ticket, &sU);
+        if (u >= tu.total) break;
+        if (u < tu.nS2M) {
+            const int item = tu.order[u / tu.s2mChunks];
+            const int d0 = (u % tu.s2mChunks) * BS;
+            const LeafItem it = tu.items[item];
+            if (d0 < it.ndir) s2mOne<R, true>(it, d0, tu.rp, tu.x, tu.perm, tu.M);
+            done = item;
+        } else {
+            int l = 0;
+            while (l + 1 < tu.nlev && u >= tu.lv[l + 1].unit0) ++l;
+            const TreeUpLevel<R>& L = tu.lv[l];
+            const int v = u - L.unit0;
+            const int stem = v / L.nchunk, chunk = v - stem * L.nchunk;
+            const int c0 = L.ptr[stem], nc = L.ptr[stem + 1] - c0;
+            if ((int)threadIdx.x < nc) {
+                const int c = L.childCtr[c0 + threadIdx.x];
+                if (c >= 0) treeWait(tu.ctr + c, c < tu.nLeaf ? tu.s2mChunks : tu.lv[l > 0 ? l - 1 : 0].nchunk, stalls);
+            }
+            treeAcquire();
+            m2mFused<R, BS, true>(stem, chunk * L.rows, L.rows, L.stemSlot, L.ptr, L.child, L.oct, L.nthP, L.nphP, L.mthC,
+                            L.mphC, tu.p, L.thNear, L.thW, L.phNear, L.phW, L.phaseUp, tu.M + L.MoffC, tu.M + L.MoffP,
+                            smem);
+            done = L.ctr0 + stem;
+        }
+    }
+    treeExit(tu.ctr, tu.nctr, &sU);
+}
+
+template <typename R>
+void launchTreeUp(const TreeUp<R>& tu, int bs, size_t lds, int grid, hipStream_t st) {
+    if (tu.total <= 0) return;
+    if (bs == 192)
+        hipLaunchKernelGGL((kTreeUp<R, 192>), dim3(grid), dim3(192), lds, st, tu);
+    else if (bs == 256)
+        hipLaunchKernelGGL((kTreeUp<R, 256>), dim3(grid), dim3(256), lds, st, tu);
+    else
+        throw Error(MLFMA_ERR_ARG, "tree walk block size must be 192 or 256");
+}
+
+template <typename R>
+int treeUpOccupancy(int bs, size_t lds) {
+    int n = 0;
+    hipError_t e = bs == 192 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTreeUp<R, 192>, 192, lds)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTreeUp<R, 256>, 256, lds);
+    if (e != hipSuccess) throw Error(MLFMA_ERR_HIP, "occupancy query of the tree walk");
+    return n;
 }
 
 // ------------------------------------------------------------------ M2L
@@ -2032,7 +2206,10 @@ void launchL2TGrad(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows,
     template bool launchL2LAdj<R>(int, const int32_t*, const int32_t*, const int32_t*, const int32_t*, int, int, int, \
                                   int, int, const int32_t*, const double*, const int32_t*, const double*,           \
                                   const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t, int);              \
-    template void launchZeroField<R>(cplx<R>*, int64_t, hipStream_t);
+    template void launchZeroField<R>(cplx<R>*, int64_t, hipStream_t);                                             \
+    template int m2mRows<R>(int, int, int, int, int, int, size_t, int, size_t*);                                   \
+    template void launchTreeUp<R>(const TreeUp<R>&, int, size_t, int, hipStream_t);                                \
+    template int treeUpOccupancy<R>(int, size_t);
 INST(double)
 INST(float)
 #undef INST

@@ -22,6 +22,51 @@ bool launchM2MFused(int nstem, const int32_t* stemSlot, const int32_t* ptr, cons
                     const int32_t* phNear, const double* phW, const cplx<R>* phaseUp, const cplx<R>* Mc, cplx<R>* Mp,
                     size_t ldsMax, hipStream_t st, int nb = 1, int64_t fs = 0, int minBlocks = 0);
 
+// parent theta rows per fused M2M block: split until nstem x row runs reaches
+// minBlocks (0: the compile-time default), then until the LDS (*lds bytes)
+// fits ldsMax; 0 when one row does not fit
+template <typename R>
+int m2mRows(int nstem, int nthP, int nphP, int mthC, int mphC, int p, size_t ldsMax, int minBlocks, size_t* lds);
+
+// The upward pass (S2M then M2M of every level) of one right-hand side as one
+// persistent launch over a packed worklist (kTreeUp); passed by value.
+constexpr int kTreeMaxLevels = 12;
+template <typename R>
+struct TreeUpLevel {  // M2M into one parent level
+    int nstem = 0, rows = 0, nchunk = 0, unit0 = 0;  // units [unit0, unit0 + nstem nchunk): (stem, row run)
+    int nthP = 0, nphP = 0, mthC = 0, mphC = 0;
+    int ctr0 = 0;                                    // completion counter of stem 0
+    const int32_t *stemSlot, *ptr, *child, *oct;
+    const int32_t* childCtr;                         // per child entry: its producer's counter (-1: none)
+    const int32_t *thNear, *phNear;
+    const double *thW, *phW;
+    const cplx<R>* phaseUp;                          // of the child level
+    int64_t MoffC = 0, MoffP = 0;                    // child / parent level fields
+};
+template <typename R>
+struct TreeUp {
+    int nlev = 0;            // M2M levels, finest parent level first
+    int nLeaf = 0;           // leaf items; counters [0, nLeaf) are theirs
+    int s2mChunks = 1;       // S2M units per leaf item (direction chunks of the block size)
+    int nS2M = 0, total = 0;
+    int p = 0;
+    int nctr = 0;
+    int32_t* ctr = nullptr;  // counters, then ticket, exit count, stall count
+    const LeafItem* items = nullptr;
+    const int32_t* order = nullptr;
+    const cplx<R>* rp = nullptr;
+    const cplx<double>* x = nullptr;
+    const int64_t* perm = nullptr;
+    cplx<R>* M = nullptr;
+    TreeUpLevel<R> lv[kTreeMaxLevels];
+};
+// bs: 192 or 256 threads; lds: the largest M2M level's; grid: blocks
+template <typename R>
+void launchTreeUp(const TreeUp<R>& tu, int bs, size_t lds, int grid, hipStream_t st);
+// resident blocks per CU of the tree walk at that block size and LDS
+template <typename R>
+int treeUpOccupancy(int bs, size_t lds);
+
 // M2L work of every level (sibling groups), passed by value to one launch
 constexpr int kMaxLevels = 16;
 template <typename R>

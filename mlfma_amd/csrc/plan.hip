@@ -573,6 +573,47 @@ void DevicePlan::build() {
     inner64 = (cplx<double>*)alloc(innerLen * sizeof(cplx<double>));
     if (useFp32) inner32 = (cplx<float>*)alloc(innerLen * sizeof(cplx<float>));
 
+    // Completion counters of the upward tree walk (kTreeUp): one per far-field
+    // leaf item (its S2M units), then one per stem of each M2M level, finest
+    // parent level first; per M2M child entry the counter of the unit group
+    // that writes that child's multipole (-1: nothing in the walk writes it)
+    if (ML - 2 > kTreeMaxLevels) throw Error(MLFMA_ERR_STATE, "too many levels for the tree walk");
+    {
+        std::vector<std::vector<int32_t>> leafOf(ML + 1), stemOf(ML + 1);
+        for (int l = 0; l <= ML; ++l) {
+            leafOf[l].assign(h.slotNode[l].size(), -1);
+            stemOf[l].assign(h.slotNode[l].size(), -1);
+        }
+        for (size_t i = 0; i < h.farLeaves.size(); ++i) leafOf[h.farLeaves[i].level][h.farLeaves[i].slot] = (int32_t)i;
+        treeCtr0.assign(ML + 1, 0);
+        int base = nLeafItems;
+        for (int l = ML - 1; l >= 2; --l) {
+            treeCtr0[l] = base;
+            const auto& P = h.m2mPtr[l];
+            int k = 0;
+            for (size_t s = 0; s + 1 < P.size(); ++s)
+                if (P[s + 1] != P[s]) stemOf[l][s] = k++;
+            if (k != levels[l].nstem) throw Error(MLFMA_ERR_STATE, "tree walk stems out of step");
+            base += k;
+        }
+        for (int l = ML - 1; l >= 2; --l) {
+            std::vector<int32_t> cc;
+            const auto& P = h.m2mPtr[l];
+            for (size_t s = 0; s + 1 < P.size(); ++s)
+                for (int q = P[s]; q < P[s + 1]; ++q) {
+                    const int c = h.m2mChild[l][q];
+                    const int li = leafOf[l + 1][c];
+                    const int si = l + 1 <= ML - 1 ? stemOf[l + 1][c] : -1;
+                    if (li >= 0 && si >= 0) throw Error(MLFMA_ERR_STATE, "a box both leaf and stem");
+                    cc.push_back(li >= 0 ? li : (si >= 0 ? treeCtr0[l + 1] + si : -1));
+                }
+            levels[l].mChildCtr = upload(cc);
+        }
+        treeCtrLen = base + 3;
+        treeCtr = (int32_t*)alloc((size_t)treeCtrLen * sizeof(int32_t));
+        hipCheck(hipMemset(treeCtr, 0, (size_t)treeCtrLen * sizeof(int32_t)), "memset");
+    }
+
     // near field
     std::vector<NearItem> nitemsH(h.nearItems.size());
     for (size_t q = 0; q < nitemsH.size(); ++q) {
@@ -938,9 +979,19 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         hipCheck(hipEventRecord(evJoin, nearStream), "event");
     }
     if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
-    probeBegin(MLFMA_GROUP_S2M, st);
-    launchS2Mx<R>(leafItems, s2mOrder, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
-    probeEnd(MLFMA_GROUP_S2M, st);
+    // one right-hand side, stages not timed: S2M and the M2M levels as one
+    // persistent walk when chosen (false: a level's fields exceed the LDS)
+    bool walkedUp = false;
+    if (nb == 1 && !timed && (treeWalk & 1)) {
+        probeBegin(MLFMA_GROUP_TREE_UP, st);
+        walkedUp = launchTreeUpWalk<R>(x, st);
+        probeEnd(MLFMA_GROUP_TREE_UP, st);
+    }
+    if (!walkedUp) {
+        probeBegin(MLFMA_GROUP_S2M, st);
+        launchS2Mx<R>(leafItems, s2mOrder, nLeafItems, maxLeafNdir, rp, x, perm, M, st, nb, xs, fs);
+        probeEnd(MLFMA_GROUP_S2M, st);
+    }
     const bool nearSide = overlap == 2 || overlap == 3;
     auto forkSides = [&]() {
         if (!nearSide && !splitM2L) return;
@@ -968,7 +1019,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         sidePending = false;
     };
     if (timed) hipCheck(hipEventRecord(ev[1], st), "event");
-    for (int l = ML - 1; l >= 2; --l) {
+    for (int l = ML - 1; l >= 2 && !walkedUp; --l) {
         LevelBufs& P = levels[l];
         LevelBufs& C = levels[l + 1];
         probeBegin(MLFMA_GROUP_M2M, st);
@@ -1027,6 +1078,74 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
                       y, accumulate, st, nb, fs, xs, ps);
     probeEnd(MLFMA_GROUP_L2T, st);
     if (timed) hipCheck(hipEventRecord(ev[7], st), "event");
+}
+
+int DevicePlan::treeGrid(bool fp32, int bs, size_t lds) {
+    for (const auto& o : treeOcc)
+        if (o[0] == (int)fp32 && o[1] == bs && o[2] == (int)lds) return o[3] * numCU;
+    const int n = fp32 ? treeUpOccupancy<float>(bs, lds) : treeUpOccupancy<double>(bs, lds);
+    if (n <= 0) throw Error(MLFMA_ERR_STATE, "tree walk does not fit a compute unit");
+    treeOcc.push_back({(int)fp32, bs, (int)lds, n});
+    return n * numCU;
+}
+
+// The upward walk of one right-hand side (kTreeUp): the worklist is the S2M
+// units of every far-field leaf (in the S2M order) and then, per parent level
+// from the finest, its M2M units with the row runs of the staged M2M launch.
+template <typename R>
+bool DevicePlan::launchTreeUpWalk(const cplx<double>* x, hipStream_t st) {
+    const int ML = hp.tree.maxLevel;
+    const int p = hp.cfg.interp_order;
+    TreeUp<R> tu;
+    const int bs = maxLeafNdir <= 192 ? 192 : 256;
+    tu.nLeaf = nLeafItems;
+    tu.s2mChunks = std::max(1, (maxLeafNdir + bs - 1) / bs);
+    tu.nS2M = nLeafItems * tu.s2mChunks;
+    tu.p = p;
+    tu.ctr = treeCtr;
+    tu.nctr = treeCtrLen;
+    tu.items = leafItems;
+    tu.order = s2mOrder;
+    tu.rp = radpat<R>();
+    tu.x = x;
+    tu.perm = perm;
+    tu.M = fieldM<R>();
+    size_t lds = 0;
+    int unit = tu.nS2M;
+    for (int l = ML - 1; l >= 2; --l) {
+        const LevelBufs& P = levels[l];
+        const LevelBufs& C = levels[l + 1];
+        TreeUpLevel<R>& L = tu.lv[tu.nlev++];
+        size_t ll = 0;
+        const int rows = P.nstem > 0 ? m2mRows<R>(P.nstem, P.nth, P.nph, C.nth, C.nph, p, ldsMax, minBlocksAt(l), &ll) : 1;
+        if (rows == 0) return false;
+        lds = std::max(lds, ll);
+        L.nstem = P.nstem;
+        L.rows = rows;
+        L.nchunk = (P.nth + rows - 1) / rows;
+        L.unit0 = unit;
+        unit += P.nstem * L.nchunk;
+        L.nthP = P.nth;
+        L.nphP = P.nph;
+        L.mthC = C.nth;
+        L.mphC = C.nph;
+        L.ctr0 = treeCtr0[l];
+        L.stemSlot = P.stemSlot;
+        L.ptr = P.mPtr;
+        L.child = P.mChild;
+        L.oct = P.mOct;
+        L.childCtr = P.mChildCtr;
+        L.thNear = P.thNear;
+        L.phNear = P.phNear;
+        L.thW = P.thW;
+        L.phW = P.phW;
+        L.phaseUp = C.phUp<R>();
+        L.MoffC = C.Moff;
+        L.MoffP = P.Moff;
+    }
+    tu.total = unit;
+    launchTreeUp<R>(tu, bs, lds, std::min(treeGrid(std::is_same<R, float>::value, bs, lds), tu.total), st);
+    return true;
 }
 
 // y (=|+=) Z^H x.  Z = near + C k R^H A S with S (S2M) the radiation
@@ -1561,30 +1680,39 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
         // item, or 1-4 resident blocks per CU that loop over the items):
         // every combination, since the in-order modes pay only when both
         // side streams are off
+        // (and, for one right-hand side, whether the upward pass runs as the
+        // persistent tree walk)
+        const int NTW = nrhs == 1 ? 2 : 1;
+        const int twKept = d.treeWalk;  // batched tuning keeps the one-RHS choice
         if (!(saved & MLFMA_FLAG_ONE_STREAM)) {
             constexpr int NNB = 5;
-            double tm[4][2][NNB] = {};
+            double tm[4][2][NNB][2] = {};
             for (int rep = 0; rep < 2; ++rep)
                 for (int m = 0; m < 4; ++m)
                     for (int sp = 0; sp < 2; ++sp)
-                        for (int nbk = 0; nbk < (m ? NNB : 1); ++nbk) {
-                            d.nearOverlap = m;
-                            d.m2lSplit = sp == 1;
-                            d.nearBlocks = nbk;
-                            tm[m][sp][nbk] += timeWhole(10);
-                        }
-            int bm = 2, bs = 1, bk = 0;
+                        for (int nbk = 0; nbk < (m ? NNB : 1); ++nbk)
+                            for (int tw = 0; tw < NTW; ++tw) {
+                                d.nearOverlap = m;
+                                d.m2lSplit = sp == 1;
+                                d.nearBlocks = nbk;
+                                d.treeWalk = NTW == 2 ? tw : twKept;
+                                tm[m][sp][nbk][tw] += timeWhole(10);
+                            }
+            int bm = 2, bs = 1, bk = 0, bt = 0;
             for (int m = 0; m < 4; ++m)
                 for (int sp = 0; sp < 2; ++sp)
                     for (int nbk = 0; nbk < (m ? NNB : 1); ++nbk)
-                        if (tm[m][sp][nbk] < tm[bm][bs][bk]) {
-                            bm = m;
-                            bs = sp;
-                            bk = nbk;
-                        }
+                        for (int tw = 0; tw < NTW; ++tw)
+                            if (tm[m][sp][nbk][tw] < tm[bm][bs][bk][bt]) {
+                                bm = m;
+                                bs = sp;
+                                bk = nbk;
+                                bt = tw;
+                            }
             d.nearOverlap = bm;
             d.m2lSplit = bs == 1;
             d.nearBlocks = bk;
+            d.treeWalk = NTW == 2 ? bt : twKept;
             // how early the pair blocks of the shared M2L / near-field launch
             // end; that launch runs only for one right-hand side with the
             // stages in order (the in-order matvec mode, or solves run in order)
@@ -1688,6 +1816,10 @@ int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value) {
             if (value < 0 || value > 8) throw Error(MLFMA_ERR_ARG, "near blocks per CU must be 0..8");
             d.nearBlocks = value;
             break;
+        case MLFMA_PARAM_TREE_WALK:
+            if (value < 0 || value > 1) throw Error(MLFMA_ERR_ARG, "tree walk must be 0 or 1");
+            d.treeWalk = value;
+            break;
         case MLFMA_PARAM_PROBE_GROUP:
             if (value < 0 || value > MLFMA_GROUP_LAST) throw Error(MLFMA_ERR_ARG, "unknown launch group");
             d.probeDrain(0);
@@ -1714,24 +1846,37 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value) 
         case MLFMA_PARAM_SOLVE_IN_ORDER: *value = d.solveSerial ? 1 : 0; break;
         case MLFMA_PARAM_PROBE_GROUP: *value = d.probeGroup; break;
         case MLFMA_PARAM_NEAR_BLOCKS: *value = d.nearBlocks; break;
+        case MLFMA_PARAM_TREE_WALK: *value = d.treeWalk; break;
+        case MLFMA_PARAM_TREE_STALLS: {
+            needDevice(d);
+            if (!d.treeCtr) throw Error(MLFMA_ERR_STATE, "no tree-walk counters");
+            hipCheck(hipSetDevice(d.device), "hipSetDevice");
+            hipCheck(hipStreamSynchronize(d.stream), "tree stalls");
+            int32_t v = 0;
+            hipCheck(hipMemcpy(&v, d.treeCtr + d.treeCtrLen - 1, sizeof(v), hipMemcpyDeviceToHost), "tree stalls");
+            *value = v;
+            break;
+        }
         default: throw Error(MLFMA_ERR_ARG, "unknown parameter");
     }
     return MLFMA_OK;
     API_END
 }
 
-// Tuning record: [version 2, nearOverlap, m2lSplit, nearWindow, solveSerial,
-// levels L, tuneMinBlocks[0..L), tuneM2L[0..L), nearBlocks]; version 1 is the
-// same without nearBlocks (0)
+// Tuning record: [version 3, nearOverlap, m2lSplit, nearWindow, solveSerial,
+// levels L, tuneMinBlocks[0..L), tuneM2L[0..L), nearBlocks, treeWalk];
+// version 2 is the same without treeWalk (0), version 1 also without
+// nearBlocks (0)
 int mlfma_autotune_record(const mlfma_plan* plan, int32_t* rec, int32_t cap, int32_t* len) {
     API_BEGIN
     const DevicePlan& d = P(plan);
     if (!len) throw Error(MLFMA_ERR_ARG, "null argument");
     const int L = d.hp.tree.maxLevel + 1;
-    std::vector<int32_t> r{2, d.nearOverlap, d.m2lSplit ? 1 : 0, d.nearWindow, d.solveSerial ? 1 : 0, L};
+    std::vector<int32_t> r{3, d.nearOverlap, d.m2lSplit ? 1 : 0, d.nearWindow, d.solveSerial ? 1 : 0, L};
     for (int l = 0; l < L; ++l) r.push_back(l < (int)d.tuneMinBlocks.size() ? d.tuneMinBlocks[l] : 0);
     for (int l = 0; l < L; ++l) r.push_back(l < (int)d.tuneM2L.size() ? d.tuneM2L[l] : 0);
     r.push_back(d.nearBlocks);
+    r.push_back(d.treeWalk);
     *len = (int32_t)r.size();
     if (rec)
         for (int32_t i = 0; i < std::min<int32_t>(cap, *len); ++i) rec[i] = r[i];
@@ -1744,11 +1889,13 @@ int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
     DevicePlan& d = P(plan);
     needDevice(d);
     const int L = d.hp.tree.maxLevel + 1;
-    const bool v1 = rec && len == 6 + 2 * L && rec[0] == 1, v2 = rec && len == 7 + 2 * L && rec[0] == 2;
-    if ((!v1 && !v2) || rec[5] != L) throw Error(MLFMA_ERR_ARG, "tuning record does not match this plan");
-    const int nearBlocks = v2 ? rec[6 + 2 * L] : 0;
+    const bool v1 = rec && len == 6 + 2 * L && rec[0] == 1, v2 = rec && len == 7 + 2 * L && rec[0] == 2,
+               v3 = rec && len == 8 + 2 * L && rec[0] == 3;
+    if ((!v1 && !v2 && !v3) || rec[5] != L) throw Error(MLFMA_ERR_ARG, "tuning record does not match this plan");
+    const int nearBlocks = v2 || v3 ? rec[6 + 2 * L] : 0;
+    const int treeWalk = v3 ? rec[7 + 2 * L] : 0;
     if (rec[1] < 0 || rec[1] > 3 || rec[2] < 0 || rec[2] > 1 || rec[3] < 1 || rec[3] > 100 || rec[4] < 0 ||
-        rec[4] > 1 || nearBlocks < 0 || nearBlocks > 8)
+        rec[4] > 1 || nearBlocks < 0 || nearBlocks > 8 || treeWalk < 0 || treeWalk > 1)
         throw Error(MLFMA_ERR_ARG, "tuning record out of range");
     for (int l = 0; l < L; ++l)
         if (rec[6 + l] < 0 || rec[6 + L + l] < 0 || rec[6 + L + l] > 2)
@@ -1760,6 +1907,7 @@ int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
     d.tuneMinBlocks.assign(rec + 6, rec + 6 + L);
     d.tuneM2L.assign(rec + 6 + L, rec + 6 + 2 * L);
     d.nearBlocks = nearBlocks;
+    d.treeWalk = treeWalk;
     d.clearGraphs();
     return MLFMA_OK;
     API_END

@@ -52,6 +52,7 @@ struct LevelBufs {
     int32_t *gTgt = nullptr, *gPtr = nullptr, *gSrc = nullptr, *gOff = nullptr, *gOffNeg = nullptr;
     int32_t* m2mAdjParent = nullptr;  // per M2M child entry: its parent slot (adjoint)
     int nm2mChild = 0;                // M2M child entries into this level
+    int32_t* mChildCtr = nullptr;     // per M2M child entry: the tree-walk counter of its writer (-1: none)
     int nstale = 0;
     int32_t* staleSlots = nullptr;
     // L2L into this level
@@ -98,6 +99,17 @@ struct DevicePlan {
     int nearWindow = 100;              // % of the shared M2L/near launch the pair blocks spread over; autotuned
     int nearBlocks = 0;                // near field on a side stream: blocks per CU (0: one per item); autotuned
     int numCU = 0;                     // compute units of the device
+    // persistent tree walks (MLFMA_PARAM_TREE_WALK bits; autotuned): 1 the
+    // upward pass of one right-hand side (S2M and every M2M level) as one
+    // launch over a packed worklist with per-unit completion counters
+    int treeWalk = 0;
+    int32_t* treeCtr = nullptr;        // counters of the walk, then ticket, exit count, stall count
+    int treeCtrLen = 0;
+    std::vector<int> treeCtr0;         // per M2M parent level: counter of its stem 0
+    std::vector<std::array<int, 4>> treeOcc;  // (float?, block size, LDS bytes, blocks per CU) seen
+    int treeGrid(bool fp32, int bs, size_t lds);
+    template <typename R>
+    bool launchTreeUpWalk(const cplx<double>* x, hipStream_t st);
     int nearGrid() const { return nearBlocks > 0 ? nearBlocks * numCU : 0; }
     bool timesValid = false;
     uint32_t flags = 0;  // MLFMA_FLAG_* in effect for matvecs (settable after creation)

@@ -366,7 +366,7 @@ def test_side_stream_schedules_match_one_stream(fp32):
                         resident = {"y": y1, "z": z1}
     # the record carries the near-field blocks and applies to a second plan
     rec = p.tuning()
-    assert rec[0] == 2 and rec[-1] == 4
+    assert rec[0] == 3 and rec[-2] == 4
     q = Plan(geom, pw, A.make_config(device=0, precision=prec, flags=flags, **kw))
     q.apply_tuning(rec)
     assert q.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == 4
