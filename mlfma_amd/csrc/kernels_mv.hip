@@ -202,23 +202,29 @@ __device__ __forceinline__ void m2mFused(int item, int j0, int rows, const int32
     const int nearPh = t < nphP ? phNear[t] : 0;
     const int c0 = ptr[item], nc = ptr[item + 1] - c0;  // at most 8 children
     for (int d = t; d < ndirC; d += BS) {
-        // every child's loads are issued before the first product
-        cplx<R> s[8], m0[8], m1[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (k < nc) {
-                const cplx<R>* Mb = Mc + (int64_t)child[c0 + k] * ndirC * 2;
-                s[k] = phaseUp[(int64_t)oct[c0 + k] * ndirC + d];
-                m0[k] = ldc<R, COH>(Mb + 2 * d);
-                m1[k] = ldc<R, COH>(Mb + 2 * d + 1);
-            }
+        // the loads of CG children are issued before their first product:
+        // all eight in the staged launch; two in the tree walk, whose
+        // registers also hold the S2M stream's (same sums, same order)
+        constexpr int CG = COH ? 2 : 8;
         cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (k < nc) {
-                a0 = a0 + s[k] * m0[k];
-                a1 = a1 + s[k] * m1[k];
-            }
+        for (int g = 0; g < 8; g += CG) {
+            cplx<R> s[CG], m0[CG], m1[CG];
+#pragma unroll
+            for (int k = 0; k < CG; ++k)
+                if (g + k < nc) {
+                    const cplx<R>* Mb = Mc + (int64_t)child[c0 + g + k] * ndirC * 2;
+                    s[k] = phaseUp[(int64_t)oct[c0 + g + k] * ndirC + d];
+                    m0[k] = ldc<R, COH>(Mb + 2 * d);
+                    m1[k] = ldc<R, COH>(Mb + 2 * d + 1);
+                }
+#pragma unroll
+            for (int k = 0; k < CG; ++k)
+                if (g + k < nc) {
+                    a0 = a0 + s[k] * m0[k];
+                    a1 = a1 + s[k] * m1[k];
+                }
+        }
         sS[2 * d] = a0;
         sS[2 * d + 1] = a1;
     }
@@ -325,22 +331,27 @@ int m2mRows(int nstem, int nthP, int nphP, int mthC, int mphC, int p, size_t lds
 
 // ------------------------------------------------------------------ tree walks
 // Persistent kernels over a packed worklist of units of several stages and
-// levels.  A block takes units in ticket order (one atomic per unit) until
-// the list is exhausted.  The list is in dependency order: every unit waits
-// only for units of lower tickets, which running blocks hold, so the walk
-// always progresses whatever the grid size and residency.  A unit waits by
-// polling the completion counters of the units it reads from, and counts
-// its own completion once its stores are complete.  Values handed between
-// units are written and read device-coherently (ldc / stc: past the XCD's
-// L2), and the counters are relaxed agent-scope atomics; the fences are
-// workgroup-scope (order only), since an agent-scope fence writes back and
-// invalidates the whole L2 of the XCD on gfx950.  Waits are bounded: after
-// kTreeSpinLimit polls (about a tenth of a second), or once any wait of the
-// plan has stalled, a wait counts a stall and goes on, so a wrong count gives
-// wrong values, never a hang.  The last block out clears every counter but
-// the stall count, so the next launch (also inside a captured graph) starts
-// from zero.  Counters: ctr[0, nctr - 3) per producing unit group, then the
-// ticket, the exit count and the stall count.
+// levels.  Units are handed out by kTreeQueues ticket counters: queue q gives
+// units q, q + 8, q + 16, ... in turn, and block b draws from queue b mod 8
+// (the blocks of one XCD, under the round-robin dispatch), so no single
+// counter serialises the walk (one address takes about one atomic per 15-20
+// ns).  Every unit waits only for units of lower index.  The lowest
+// unfinished unit therefore never waits: it is running, or it is the next
+// unit of its queue, whose blocks are all free once every lower unit is done
+// (blocks 0-7, one per queue, are dispatched first), so the walk progresses
+// whatever the grid size and residency.  A unit waits by polling the
+// completion counters of the units it reads from, and counts its own
+// completion once its stores are complete.  Values handed between units are
+// written and read device-coherently (ldc / stc: past the XCD's L2), and the
+// counters are relaxed agent-scope atomics; the fences are workgroup-scope
+// (order only), since an agent-scope fence writes back and invalidates the
+// whole L2 of the XCD on gfx950.  Waits are bounded: after kTreeSpinLimit
+// polls (about a tenth of a second), or once any wait of the plan has
+// stalled, a wait counts a stall and goes on, so a wrong count gives wrong
+// values, never a hang.  The last block out clears every counter but the
+// stall count, so the next launch (also inside a captured graph) starts from
+// zero.  Counters: ctr[0, nctr - 10) per producing unit group, then the
+// tickets, the exit count and the stall count.
 constexpr int kTreeSpinLimit = 1 << 16;
 
 __device__ __forceinline__ void treeWait(const int32_t* c, int need, int32_t* stalls) {
@@ -357,22 +368,27 @@ __device__ __forceinline__ void treeWait(const int32_t* c, int need, int32_t* st
 // s_waitcnt vmcnt(0): every memory access of the wave has completed
 __device__ __forceinline__ void waitVmem() { __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8)); }
 
-// One step of a block's walk: count the block's previous unit (done >= 0),
-// once every wave's stores of it have completed, and take the next ticket.
-// The two atomics are one thread-0 region between two barriers, and the
-// ticket is read into a scalar register (readfirstlane), so the loop around
-// the steps stays uniform.  (With a thread-0 region at the end of the loop
-// body and another at its start, and the ticket read as a vector value, the
-// structurizer turned the loop into two nested ones with thread 0 alone in
-// the outer one: the wave's other lanes went on to the next step's barriers
-// with the old ticket and thread 0 never ran again.)
-__device__ __forceinline__ int treeStep(int32_t* ctr, int done, int32_t* ticket, int* sU) {
+// One step of a block's walk: count the block's previous unit, once every
+// wave's stores of it have completed (thread i < nsig adds one to counter
+// sig(i)), and take the next unit from the block's queue.  The atomics are
+// one region between two barriers, and the unit is read into a scalar
+// register (readfirstlane), so the loop around the steps stays uniform.
+// (With one thread-0 region at the end of the loop body and another at its
+// start, and the unit read as a vector value, the structurizer turned the
+// loop into two nested ones with thread 0 alone in the outer one: the wave's
+// other lanes went on to the next step's barriers with the old unit and
+// thread 0 never ran again.)
+template <typename SIG>
+__device__ __forceinline__ int treeStep(int32_t* ctr, int nctr, int nsig, SIG sig, int* sU) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     waitVmem();
     __syncthreads();
+    if ((int)threadIdx.x < nsig) __hip_atomic_fetch_add(ctr + sig((int)threadIdx.x), 1, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x == 0) {
-        if (done >= 0) __hip_atomic_fetch_add(ctr + done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *sU = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int q = blockIdx.x % kTreeQueues;
+        *sU = kTreeQueues * __hip_atomic_fetch_add(ctr + nctr - 2 - kTreeQueues + q, 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT) + q;
     }
     __syncthreads();
     return __builtin_amdgcn_readfirstlane(*sU);
@@ -399,27 +415,33 @@ __device__ __forceinline__ void treeExit(int32_t* ctr, int nctr, int* sU) {
         for (int i = threadIdx.x; i < nctr - 1; i += blockDim.x) ctr[i] = 0;
 }
 
-// The upward pass of one right-hand side: S2M units (leaf item, direction
-// chunk of BS; leaves in the S2M order, largest first), then the fused M2M
-// units (parent stem, run of parent theta rows) level by level from the
-// finest parent level.  An M2M unit waits for every S2M or M2M unit that
-// writes one of its children.
+// The upward pass of one right-hand side: S2M units (s2mPer consecutive
+// pieces of the S2M order, a piece being a leaf item's direction chunk of
+// BS; leaves largest first), then the fused M2M units (parent stem, run of
+// parent theta rows) level by level from the finest parent level.  An M2M
+// unit waits for every S2M or M2M unit that writes one of its children.
 template <typename R, int BS>
 __global__ __launch_bounds__(BS) void kTreeUp(TreeUp<R> tu) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int sU;
-    int32_t* ticket = tu.ctr + tu.nctr - 3;
     int32_t* stalls = tu.ctr + tu.nctr - 1;
-    int done = -1;  // the counter of the block's previous unit
+    // the previous unit: S2M pieces [sig0, sig0 + nsig) or, when sig0 < 0,
+    // the counter -sig0 - 1 (an M2M stem)
+    int sig0 = 0, nsig = 0;
     for (;;) {
-        const int u = treeStep(tu.ctr, done, ticket, &sU);
+        const int u = treeStep(tu.ctr, tu.nctr, nsig,
+                               [&](int i) { return sig0 >= 0 ? tu.order[(sig0 + i) / tu.s2mChunks] : -sig0 - 1; },
+                               &sU);
         if (u >= tu.total) break;
         if (u < tu.nS2M) {
-            const int item = tu.order[u / tu.s2mChunks];
-            const int d0 = (u % tu.s2mChunks) * BS;
-            const LeafItem it = tu.items[item];
-            if (d0 < it.ndir) s2mOne<R, true>(it, d0, tu.rp, tu.x, tu.perm, tu.M);
-            done = item;
+            const int f0 = u * tu.s2mPer, f1 = min(f0 + tu.s2mPer, tu.nS2MPieces);
+            for (int f = f0; f < f1; ++f) {
+                const LeafItem it = tu.items[tu.order[f / tu.s2mChunks]];
+                const int d0 = (f % tu.s2mChunks) * BS;
+                if (d0 < it.ndir) s2mOne<R, true>(it, d0, tu.rp, tu.x, tu.perm, tu.M);
+            }
+            sig0 = f0;
+            nsig = f1 - f0;
         } else {
             int l = 0;
             while (l + 1 < tu.nlev && u >= tu.lv[l + 1].unit0) ++l;
@@ -432,10 +454,11 @@ __global__ __launch_bounds__(BS) void kTreeUp(TreeUp<R> tu) {
                 if (c >= 0) treeWait(tu.ctr + c, c < tu.nLeaf ? tu.s2mChunks : tu.lv[l > 0 ? l - 1 : 0].nchunk, stalls);
             }
             treeAcquire();
-            m2mFused<R, BS, true>(stem, chunk * L.rows, L.rows, L.stemSlot, L.ptr, L.child, L.oct, L.nthP, L.nphP, L.mthC,
-                            L.mphC, tu.p, L.thNear, L.thW, L.phNear, L.phW, L.phaseUp, tu.M + L.MoffC, tu.M + L.MoffP,
-                            smem);
-            done = L.ctr0 + stem;
+            m2mFused<R, BS, true>(stem, chunk * L.rows, L.rows, L.stemSlot, L.ptr, L.child, L.oct, L.nthP, L.nphP,
+                                  L.mthC, L.mphC, tu.p, L.thNear, L.thW, L.phNear, L.phW, L.phaseUp, tu.M + L.MoffC,
+                                  tu.M + L.MoffP, smem);
+            sig0 = -(L.ctr0 + stem) - 1;
+            nsig = 1;
         }
     }
     treeExit(tu.ctr, tu.nctr, &sU);

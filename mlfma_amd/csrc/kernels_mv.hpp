@@ -31,6 +31,7 @@ int m2mRows(int nstem, int nthP, int nphP, int mthC, int mphC, int p, size_t lds
 // The upward pass (S2M then M2M of every level) of one right-hand side as one
 // persistent launch over a packed worklist (kTreeUp); passed by value.
 constexpr int kTreeMaxLevels = 12;
+constexpr int kTreeQueues = 8;  // ticket counters; queue q hands out units q, q + 8, q + 16, ...
 template <typename R>
 struct TreeUpLevel {  // M2M into one parent level
     int nstem = 0, rows = 0, nchunk = 0, unit0 = 0;  // units [unit0, unit0 + nstem nchunk): (stem, row run)
@@ -47,11 +48,13 @@ template <typename R>
 struct TreeUp {
     int nlev = 0;            // M2M levels, finest parent level first
     int nLeaf = 0;           // leaf items; counters [0, nLeaf) are theirs
-    int s2mChunks = 1;       // S2M units per leaf item (direction chunks of the block size)
-    int nS2M = 0, total = 0;
+    int s2mChunks = 1;       // S2M pieces per leaf item (direction chunks of the block size)
+    int s2mPer = 1;          // S2M pieces per unit (consecutive in the S2M order)
+    int nS2MPieces = 0;      // nLeaf x s2mChunks
+    int nS2M = 0, total = 0; // S2M units (the first nS2M) and all units
     int p = 0;
     int nctr = 0;
-    int32_t* ctr = nullptr;  // counters, then ticket, exit count, stall count
+    int32_t* ctr = nullptr;  // counters, then kTreeQueues tickets, exit count, stall count
     const LeafItem* items = nullptr;
     const int32_t* order = nullptr;
     const cplx<R>* rp = nullptr;

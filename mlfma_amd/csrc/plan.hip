@@ -609,7 +609,7 @@ void DevicePlan::build() {
                 }
             levels[l].mChildCtr = upload(cc);
         }
-        treeCtrLen = base + 3;
+        treeCtrLen = base + kTreeQueues + 2;
         treeCtr = (int32_t*)alloc((size_t)treeCtrLen * sizeof(int32_t));
         hipCheck(hipMemset(treeCtr, 0, (size_t)treeCtrLen * sizeof(int32_t)), "memset");
     }
@@ -1090,8 +1090,9 @@ int DevicePlan::treeGrid(bool fp32, int bs, size_t lds) {
 }
 
 // The upward walk of one right-hand side (kTreeUp): the worklist is the S2M
-// units of every far-field leaf (in the S2M order) and then, per parent level
-// from the finest, its M2M units with the row runs of the staged M2M launch.
+// units (kTreeS2MPer leaf pieces each, in the S2M order) and then, per parent
+// level from the finest, its M2M units with the row runs of the staged M2M
+// launch.
 template <typename R>
 bool DevicePlan::launchTreeUpWalk(const cplx<double>* x, hipStream_t st) {
     const int ML = hp.tree.maxLevel;
@@ -1100,7 +1101,9 @@ bool DevicePlan::launchTreeUpWalk(const cplx<double>* x, hipStream_t st) {
     const int bs = maxLeafNdir <= 192 ? 192 : 256;
     tu.nLeaf = nLeafItems;
     tu.s2mChunks = std::max(1, (maxLeafNdir + bs - 1) / bs);
-    tu.nS2M = nLeafItems * tu.s2mChunks;
+    tu.nS2MPieces = nLeafItems * tu.s2mChunks;
+    tu.s2mPer = kTreeS2MPer;
+    tu.nS2M = (tu.nS2MPieces + tu.s2mPer - 1) / tu.s2mPer;
     tu.p = p;
     tu.ctr = treeCtr;
     tu.nctr = treeCtrLen;
@@ -1144,7 +1147,10 @@ bool DevicePlan::launchTreeUpWalk(const cplx<double>* x, hipStream_t st) {
         L.MoffP = P.Moff;
     }
     tu.total = unit;
-    launchTreeUp<R>(tu, bs, lds, std::min(treeGrid(std::is_same<R, float>::value, bs, lds), tu.total), st);
+    // every ticket queue that has units has a block
+    const int grid = std::max(std::min(treeGrid(std::is_same<R, float>::value, bs, lds), tu.total),
+                              std::min(tu.total, kTreeQueues));
+    launchTreeUp<R>(tu, bs, lds, grid, st);
     return true;
 }
 

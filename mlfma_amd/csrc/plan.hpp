@@ -105,6 +105,7 @@ struct DevicePlan {
     int treeWalk = 0;
     int32_t* treeCtr = nullptr;        // counters of the walk, then ticket, exit count, stall count
     int treeCtrLen = 0;
+    static constexpr int kTreeS2MPer = 4;  // leaf pieces per S2M unit of the walk
     std::vector<int> treeCtr0;         // per M2M parent level: counter of its stem 0
     std::vector<std::array<int, 4>> treeOcc;  // (float?, block size, LDS bytes, blocks per CU) seen
     int treeGrid(bool fp32, int bs, size_t lds);
