@@ -532,7 +532,7 @@ def kernel_bytes(group, info, n):
         return leaf, leaf
     if group == "NEAR":
         return near, near
-    if group == "M2L_TARGET":
+    if group in ("M2L_TARGET", "M2L_RUNS"):
         return m2l_comp, m2l_str
     if group == "M2L_NEAR":
         return m2l_comp + near, m2l_str + near
@@ -543,7 +543,8 @@ def kernel_bytes(group, info, n):
 
 
 KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2TN", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
-                "M2M": "kM2MFused", "L2L": "kL2LFused", "M2L_SIBLING": "kM2LGroup", "TREE_UP": "kTreeUp"}
+                "M2M": "kM2MFused", "L2L": "kL2LFused", "M2L_SIBLING": "kM2LGroup", "TREE_UP": "kTreeUp",
+                "M2L_RUNS": "kM2LT"}
 
 
 def main():
@@ -677,7 +678,7 @@ def main():
     # the same K matvecs once per launch group with that group's launches
     # bracketed by HIP events on the stream each runs on (same schedule)
     groups = {}
-    for g in ["TREE_UP", "S2M", "M2M", "M2L_TARGET", "M2L_SIBLING", "M2L_NEAR", "L2L", "NEAR", "L2T"]:
+    for g in ["TREE_UP", "S2M", "M2M", "M2L_RUNS", "M2L_TARGET", "M2L_SIBLING", "M2L_NEAR", "L2L", "NEAR", "L2T"]:
         plan.probe(g)
         for _ in range(args.steps):
             plan.matvec(x, y)

@@ -80,8 +80,9 @@ typedef struct mlfma_planewave {
                                        each event costs a few microseconds of
                                        device idle time */
 #define MLFMA_FLAG_M2L_GROUP 4u     /* M2L through the sibling-group kernel on
-                                       every level (default: chosen per level
-                                       from the list overlap and block count) */
+                                       every level (default: the target-run
+                                       kernel, or the autotuned choice per
+                                       level) */
 #define MLFMA_FLAG_M2L_TARGET 8u    /* M2L through the per-target kernel on
                                        every level */
 #define MLFMA_FLAG_PHYSICAL 32u     /* RWG only, fixed at creation: near-field
@@ -433,7 +434,8 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value);
 #define MLFMA_GROUP_NEAR 7         /* near-field pair products (kNearSym)        */
 #define MLFMA_GROUP_L2T 8          /* kL2Tx                                      */
 #define MLFMA_GROUP_TREE_UP 9      /* S2M + every M2M level as one walk (kTreeUp)*/
-#define MLFMA_GROUP_LAST 9
+#define MLFMA_GROUP_M2L_RUNS 10    /* M2L levels over target runs (kM2LT)        */
+#define MLFMA_GROUP_LAST 10
 int mlfma_probe_read(mlfma_plan* plan, double* ms, int64_t* launches);
 
 /* The launch parameters mlfma_autotune chose, as a record of int32 values

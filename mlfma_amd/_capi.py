@@ -44,7 +44,7 @@ MLFMA_PARAM_NEAR_BLOCKS = 6
 MLFMA_PARAM_TREE_WALK = 7
 MLFMA_PARAM_TREE_STALLS = 8
 GROUPS = {"S2M": 1, "M2M": 2, "M2L_TARGET": 3, "M2L_SIBLING": 4, "M2L_NEAR": 5, "L2L": 6, "NEAR": 7, "L2T": 8,
-          "TREE_UP": 9}
+          "TREE_UP": 9, "M2L_RUNS": 10}
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_OOM", -3: "ERR_HIP", -4: "ERR_BREAKDOWN",
           -5: "ERR_NONFINITE", -6: "ERR_STATE", -7: "ERR_GEOMETRY", -8: "ERR_NOT_CONVERGED"}

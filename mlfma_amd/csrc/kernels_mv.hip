@@ -767,6 +767,123 @@ void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb) {
 #undef M2L_CASE
 }
 
+// ------------------------------------------------------------------ M2L over target runs
+// L_t[d] = sum over the interaction list of T_off[d] M_s[d]  (src/fmm/node.cpp:105-134).
+// Block = (level, run of consecutive targets, chunk of W directions).  The
+// run's distinct sources (at most 256 boxes for at most 128 targets; each is
+// used by many of the run's targets) and the 316 operators
+// are staged in LDS for the chunk, W directions x 2 components of a source
+// being one contiguous 64-byte read; the targets' sums then read only LDS,
+// so each source field and operator is read from the caches once per run
+// and chunk instead of once per pair.  Lane (target, direction): each lane
+// sums its target's pairs in list order.  Within a level, the units of one
+// XCD are a contiguous range of chunks, so its L2 holds the lines (4
+// directions of a source box in 128 bytes) that neighbouring chunks share.
+// Grid z: right-hand side (fields at stride lv.fs).  CONJ: conjugated
+// operators (the adjoint pass, packed with the reversed pairs' offsets).
+template <typename R, bool CONJ>
+__global__ __launch_bounds__(TB) void kM2LT(M2LTiles<R> lv) {
+    constexpr int W = m2ltDirs<R>();
+    extern __shared__ __align__(16) unsigned char smem[];
+    int b = blockIdx.x, l = 0;
+    while (l + 1 < lv.nlev && b >= lv.blk[l + 1]) ++l;
+    b -= lv.blk[l];
+    const int nrun = lv.nrun[l];
+    const int u = xcdItem(b, lv.per[l]);
+    if (u >= nrun * lv.nch[l]) return;  // uniform per block
+    const int chunk = u / nrun, g = u - chunk * nrun;
+    const int ndir = lv.ndir[l];
+    const int d0 = chunk * W;
+    const cplx<R>* M = lv.M[l] + blockIdx.z * lv.fs;
+    cplx<R>* L = lv.L[l] + blockIdx.z * lv.fs;
+    const int s0 = lv.rsrc[l][g], ns = lv.rsrc[l][g + 1] - s0;
+    cplx<R>* sM = reinterpret_cast<cplx<R>*>(smem);  // ns x W x 2
+    cplx<R>* sT = sM + (size_t)ns * W * 2;           // kM2LOffsets x W
+    const int32_t* slots = lv.slot[l] + s0;
+    for (int e = threadIdx.x; e < ns * W * 2; e += TB) {
+        const int i = e / (2 * W), r = e - i * (2 * W);
+        const int d = d0 + (r >> 1);
+        sM[e] = d < ndir ? M[((int64_t)slots[i] * ndir + d) * 2 + (r & 1)] : mk<R>(0, 0);
+    }
+    const cplx<R>* T = lv.T[l];
+    for (int e = threadIdx.x; e < kM2LOffsets * W; e += TB) {
+        const int o = e / W, d = d0 + (e - o * W);
+        cplx<R> t = d < ndir ? T[(int64_t)o * ndir + d] : mk<R>(0, 0);
+        if (CONJ) t.im = -t.im;
+        sT[e] = t;
+    }
+    __syncthreads();
+    const int w = threadIdx.x % W;
+    const int d = d0 + w;
+    const int k1 = lv.run[l][g + 1];
+    const int32_t* ptr = lv.ptr[l];
+    const int32_t* pk = lv.pk[l];
+    for (int k = lv.run[l][g] + threadIdx.x / W; k < k1; k += TB / W) {
+        const int q0 = ptr[k], q1 = ptr[k + 1];
+        cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0);
+        int q = q0;
+        for (; q + 3 < q1; q += 4) {  // four pairs' index and LDS loads in flight
+            int v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = pk[q + j];
+            cplx<R> tt[4], mm[4][2];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                tt[j] = sT[(v[j] & 511) * W + w];
+                const cplx<R>* m = sM + ((v[j] >> 9) * W + w) * 2;
+                mm[j][0] = m[0];
+                mm[j][1] = m[1];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a0 = a0 + tt[j] * mm[j][0];
+                a1 = a1 + tt[j] * mm[j][1];
+            }
+        }
+        for (; q + 1 < q1; q += 2) {
+            const int v0 = pk[q], v1 = pk[q + 1];
+            const cplx<R> t0 = sT[(v0 & 511) * W + w], t1 = sT[(v1 & 511) * W + w];
+            const cplx<R>* m0 = sM + ((v0 >> 9) * W + w) * 2;
+            const cplx<R>* m1 = sM + ((v1 >> 9) * W + w) * 2;
+            a0 = a0 + t0 * m0[0];
+            a1 = a1 + t0 * m0[1];
+            a0 = a0 + t1 * m1[0];
+            a1 = a1 + t1 * m1[1];
+        }
+        if (q < q1) {
+            const int v0 = pk[q];
+            const cplx<R> t0 = sT[(v0 & 511) * W + w];
+            const cplx<R>* m0 = sM + ((v0 >> 9) * W + w) * 2;
+            a0 = a0 + t0 * m0[0];
+            a1 = a1 + t0 * m0[1];
+        }
+        if (d < ndir) {
+            cplx<R>* out = L + ((int64_t)lv.tgt[l][k] * ndir + d) * 2;
+            out[0] = a0;
+            out[1] = a1;
+        }
+    }
+}
+
+template <typename R>
+void launchM2LT(M2LTiles<R> lv, bool conjT, hipStream_t st, int nb) {
+    constexpr int W = m2ltDirs<R>();
+    int total = 0;
+    for (int l = 0; l < lv.nlev; ++l) {
+        lv.nch[l] = (lv.ndir[l] + W - 1) / W;
+        lv.per[l] = (lv.nrun[l] * lv.nch[l] + 7) / 8;
+        lv.blk[l] = total;
+        total += 8 * lv.per[l];
+    }
+    if (total <= 0) return;
+    const dim3 g(total, 1, nb);
+    const size_t lds = m2ltLds<R>(lv.maxSrc);
+    if (conjT)
+        hipLaunchKernelGGL((kM2LT<R, true>), g, dim3(TB), lds, st, lv);
+    else
+        hipLaunchKernelGGL((kM2LT<R, false>), g, dim3(TB), lds, st, lv);
+}
+
 // ------------------------------------------------------------------ fused L2L
 // One block per child box whose parent has a non-empty interaction list
 // (src/fmm/stem.cpp:109-131,186-244).  The gather-form anterpolation tables
@@ -2219,6 +2336,7 @@ void launchL2TGrad(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows,
                                     int);                                                                      \
     template void launchM2LGroup<R>(M2LGroups<R>, bool, hipStream_t, int);                                          \
     template void launchM2Lx<R>(M2LLevels<R>, bool, hipStream_t, int);                                              \
+    template void launchM2LT<R>(M2LTiles<R>, bool, hipStream_t, int);                                               \
     template bool launchL2LFused<R>(bool, int, const int32_t*, const int32_t*, const int32_t*, int, int, int, int,   \
                                     const int32_t*, const int32_t*, const double*, int, const int32_t*,             \
                                     const int32_t*, const double*, const int32_t*, int, const cplx<R>*,             \

@@ -110,6 +110,38 @@ struct M2LLevels {
 template <typename R>
 void launchM2Lx(M2LLevels<R> lv, bool conjT, hipStream_t st, int nb = 1);
 
+// M2L over target runs (kM2LT): per level, runs of consecutive targets and,
+// per run, the distinct source boxes of their lists (slots, in one CSR);
+// each pair is packed as (index of its source in the run's list) << 9 |
+// offset.  A block stages the run's sources and the 316 operators for a few
+// directions in LDS and forms every target of the run from there.
+constexpr int kM2LOffsets = 316;
+template <typename R>
+struct M2LTiles {
+    int nlev = 0;
+    int ndir[kMaxLevels], nrun[kMaxLevels], nch[kMaxLevels], per[kMaxLevels], blk[kMaxLevels];
+    const int32_t* tgt[kMaxLevels];   // target slots
+    const int32_t* ptr[kMaxLevels];   // per target: its pairs
+    const int32_t* pk[kMaxLevels];    // packed pairs
+    const int32_t* run[kMaxLevels];   // per run: first target (nrun + 1)
+    const int32_t* rsrc[kMaxLevels];  // per run: first entry of its source list (nrun + 1)
+    const int32_t* slot[kMaxLevels];  // source slots of every run's list
+    const cplx<R>* T[kMaxLevels];
+    const cplx<R>* M[kMaxLevels];
+    cplx<R>* L[kMaxLevels];
+    int64_t fs = 0;
+    int maxSrc = 0;                   // longest run source list (LDS size)
+};
+// directions per block: 2 in FP64, 4 in FP32 (64 bytes of each source)
+template <typename R>
+constexpr int m2ltDirs() { return sizeof(R) == 8 ? 2 : 4; }
+template <typename R>
+size_t m2ltLds(int maxSrc) {
+    return sizeof(cplx<R>) * (size_t)m2ltDirs<R>() * (2 * (size_t)maxSrc + kM2LOffsets);
+}
+template <typename R>
+void launchM2LT(M2LTiles<R> lv, bool conjT, hipStream_t st, int nb = 1);
+
 template <typename R>
 bool launchL2LFused(bool adjoint, int nitems, const int32_t* parent, const int32_t* child, const int32_t* oct, int mthP, int mphP,
                     int nthC, int nphC, const int32_t* atPhPtr, const int32_t* atPhSrc, const double* atPhW, int nAtPh,

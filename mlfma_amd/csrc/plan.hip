@@ -486,6 +486,50 @@ void DevicePlan::build() {
                 neg[q] = offsetIndex(-dx, -dy, -dz);
             }
             L.lOffNeg = upload(neg);
+            // runs of consecutive targets for the run-tiled M2L (kM2LT): a
+            // run grows while its distinct sources stay within
+            // kM2LRunSources (their LDS) and it has at most kM2LRunTargets
+            // targets; pairs packed as (source index in the run) << 9 | offset
+            std::vector<int32_t> runT{0}, runS{0}, slots, pk(of.size()), pkn(of.size());
+            std::unordered_map<int32_t, int32_t> loc;
+            int maxSrc = 0;
+            size_t r0 = 0;  // first target of the open run
+            auto closeRun = [&](size_t k) {
+                std::vector<int32_t> srcs;
+                srcs.reserve(loc.size());
+                for (auto& kv : loc) srcs.push_back(kv.first);
+                std::sort(srcs.begin(), srcs.end());
+                for (size_t i = 0; i < srcs.size(); ++i) loc[srcs[i]] = (int32_t)i;
+                for (size_t t2 = r0; t2 < k; ++t2)
+                    for (int q = ptr[t2]; q < ptr[t2 + 1]; ++q) {
+                        pk[q] = loc[sr[q]] << 9 | of[q];
+                        pkn[q] = loc[sr[q]] << 9 | neg[q];
+                    }
+                slots.insert(slots.end(), srcs.begin(), srcs.end());
+                maxSrc = std::max(maxSrc, (int)srcs.size());
+                runT.push_back((int32_t)k);
+                runS.push_back((int32_t)slots.size());
+                loc.clear();
+                r0 = k;
+            };
+            for (size_t k = 0; k < tgt.size(); ++k) {
+                int fresh = 0;
+                for (int q = ptr[k]; q < ptr[k + 1]; ++q) fresh += loc.count(sr[q]) ? 0 : 1;
+                if (k > r0 && ((int)(loc.size() + fresh) > kM2LRunSources || (int)(k - r0) >= kM2LRunTargets))
+                    closeRun(k);
+                for (int q = ptr[k]; q < ptr[k + 1]; ++q) loc.emplace(sr[q], 0);
+            }
+            if (tgt.size() > r0) closeRun(tgt.size());
+            for (int32_t v : of)
+                if (v < 0 || v >= kM2LOffsets) throw Error(MLFMA_ERR_STATE, "M2L offset out of range");
+            if (maxSrc >= (1 << 22)) throw Error(MLFMA_ERR_STATE, "M2L run too large to pack");
+            L.nrun = (int)runT.size() - 1;
+            L.runMaxSrc = maxSrc;
+            L.runTgt = upload(runT);
+            L.runSrc = upload(runS);
+            L.runSlot = upload(slots);
+            L.lPk = upload(pk);
+            L.lPkNeg = upload(pkn);
         }
         {
             // sibling groups for M2L: the targets of level l under one parent
@@ -850,16 +894,33 @@ bool DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStr
     if (ML - 1 > kMaxLevels) throw Error(MLFMA_ERR_ARG, "too many tree levels");
     M2LGroups<R> g;
     M2LLevels<R> t;
-    g.fs = t.fs = fieldLen;
+    M2LTiles<R> r;
+    g.fs = t.fs = r.fs = fieldLen;
     for (int l = std::max(2, lmin); l <= std::min(ML, lmax); ++l) {
         LevelBufs& L = levels[l];
         if (L.ntgt == 0) continue;
-        const int64_t nch = (L.ndir + 255) / 256;
-        bool group = L.nPairs >= 3 * L.nGrpEntries && (int64_t)L.ngrp * nch * nb >= 2048;
-        if (l < (int)tuneM2L.size() && tuneM2L[l]) group = tuneM2L[l] == 1;  // autotuned choice
-        if (flags & MLFMA_FLAG_M2L_GROUP) group = true;
-        if (flags & MLFMA_FLAG_M2L_TARGET) group = false;
-        if (group) {
+        // kernel per level: the target runs unless tuned or forced otherwise
+        int mode = L.nrun > 0 ? 3 : 2;
+        if (l < (int)tuneM2L.size() && tuneM2L[l]) mode = tuneM2L[l];  // autotuned choice
+        if (mode == 3 && L.nrun == 0) mode = 2;
+        if (flags & MLFMA_FLAG_M2L_GROUP) mode = 1;
+        if (flags & MLFMA_FLAG_M2L_TARGET) mode = 2;
+        const bool group = mode == 1;
+        if (mode == 3) {
+            const int k = r.nlev++;
+            r.ndir[k] = L.ndir;
+            r.nrun[k] = L.nrun;
+            r.tgt[k] = L.tgtSlot;
+            r.ptr[k] = L.lPtr;
+            r.pk[k] = adjoint ? L.lPkNeg : L.lPk;
+            r.run[k] = L.runTgt;
+            r.rsrc[k] = L.runSrc;
+            r.slot[k] = L.runSlot;
+            r.T[k] = L.T<R>();
+            r.M[k] = in + L.Moff;
+            r.L[k] = out + L.Moff;
+            r.maxSrc = std::max(r.maxSrc, L.runMaxSrc);
+        } else if (group) {
             const int k = g.nlev++;
             g.ndir[k] = L.ndir;
             g.ngrp[k] = L.ngrp;
@@ -887,6 +948,12 @@ bool DevicePlan::launchM2L(const cplx<R>* in, cplx<R>* out, bool adjoint, hipStr
         probeBegin(MLFMA_GROUP_M2L_SIBLING, st);
         launchM2LGroup<R>(g, adjoint, st, nb);
         probeEnd(MLFMA_GROUP_M2L_SIBLING, st);
+    }
+    if (r.nlev) {
+        if (m2ltLds<R>(r.maxSrc) > ldsMax) throw Error(MLFMA_ERR_STATE, "M2L run sources exceed the LDS");
+        probeBegin(MLFMA_GROUP_M2L_RUNS, st);
+        launchM2LT<R>(r, adjoint, st, nb);
+        probeEnd(MLFMA_GROUP_M2L_RUNS, st);
     }
     if (nearX && t.nlev && nb == 1 && nNearItems > 0) {
         // the FP32 far field reads the FP32 copy of the near-field values
@@ -1644,31 +1711,35 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
             }
             d.tuneMinBlocks[l] = bm;
         }
-        // M2L: sibling-group or per-target kernel per level, every
-        // combination when there are at most five such levels (a level at a
-        // time from the previous choice oscillated between two settings)
+        // M2L: sibling-group (1), per-target (2) or target-run (3) kernel
+        // per level, every combination when there are at most five such
+        // levels (a level at a time from the previous choice oscillated
+        // between two settings)
         d.tuneM2L.assign(d.hp.tree.maxLevel + 1, 0);
         {
             std::vector<int> lv;
             for (int l = 2; l <= d.hp.tree.maxLevel; ++l)
                 if (d.levels[l].ntgt > 0) lv.push_back(l);
+            auto modes = [&](int l) { return d.levels[l].nrun > 0 ? 3 : 2; };
             if (lv.size() <= 5) {
+                int combos = 1;
+                for (int l : lv) combos *= modes(l);
                 double tb = 1e300;
-                unsigned bestMask = 0;
-                for (unsigned mask = 0; mask < (1u << lv.size()); ++mask) {
-                    for (size_t i = 0; i < lv.size(); ++i) d.tuneM2L[lv[i]] = (mask >> i & 1u) ? 2 : 1;
+                int best = 0;
+                for (int c = 0; c < combos; ++c) {
+                    for (int i = 0, v = c; i < (int)lv.size(); v /= modes(lv[i]), ++i) d.tuneM2L[lv[i]] = 1 + v % modes(lv[i]);
                     const double t = timeStage(2, 3, 4);
                     if (t < tb) {
                         tb = t;
-                        bestMask = mask;
+                        best = c;
                     }
                 }
-                for (size_t i = 0; i < lv.size(); ++i) d.tuneM2L[lv[i]] = (bestMask >> i & 1u) ? 2 : 1;
+                for (int i = 0, v = best; i < (int)lv.size(); v /= modes(lv[i]), ++i) d.tuneM2L[lv[i]] = 1 + v % modes(lv[i]);
             } else {
                 for (int l : lv) {
                     double tb = 1e300;
                     int mode = 0;
-                    for (int m : {1, 2}) {
+                    for (int m = 1; m <= modes(l); ++m) {
                         d.tuneM2L[l] = m;
                         const double t = timeStage(2, 3, 8);
                         if (t < tb) {
@@ -1904,7 +1975,7 @@ int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
         rec[4] > 1 || nearBlocks < 0 || nearBlocks > 8 || treeWalk < 0 || treeWalk > 1)
         throw Error(MLFMA_ERR_ARG, "tuning record out of range");
     for (int l = 0; l < L; ++l)
-        if (rec[6 + l] < 0 || rec[6 + L + l] < 0 || rec[6 + L + l] > 2)
+        if (rec[6 + l] < 0 || rec[6 + L + l] < 0 || rec[6 + L + l] > 3)
             throw Error(MLFMA_ERR_ARG, "tuning record out of range");
     d.nearOverlap = rec[1];
     d.m2lSplit = rec[2] == 1;

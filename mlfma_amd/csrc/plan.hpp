@@ -45,6 +45,10 @@ struct LevelBufs {
     int ntgt = 0;
     int32_t *tgtSlot = nullptr, *lPtr = nullptr, *lSrc = nullptr, *lOff = nullptr;
     int32_t* lOffNeg = nullptr;  // offsets of the reversed pairs (adjoint)
+    // M2L target runs (kM2LT): per run its first target and its source list
+    int nrun = 0, runMaxSrc = 0;
+    int32_t *runTgt = nullptr, *runSrc = nullptr, *runSlot = nullptr;
+    int32_t *lPk = nullptr, *lPkNeg = nullptr;  // per pair: (source index in its run) << 9 | offset
     // M2L sibling groups: 8 target slots per group (-1 none), per group a CSR
     // of distinct sources, 8 offsets per source entry (-1: not in that list)
     int ngrp = 0;
@@ -106,6 +110,10 @@ struct DevicePlan {
     int32_t* treeCtr = nullptr;        // counters of the walk, then ticket, exit count, stall count
     int treeCtrLen = 0;
     static constexpr int kTreeS2MPer = 4;  // leaf pieces per S2M unit of the walk
+    // M2L target runs: at most this many distinct sources (64 LDS bytes each
+    // per block) and targets per run
+    static constexpr int kM2LRunSources = 256;
+    static constexpr int kM2LRunTargets = 128;
     std::vector<int> treeCtr0;         // per M2M parent level: counter of its stem 0
     std::vector<std::array<int, 4>> treeOcc;  // (float?, block size, LDS bytes, blocks per CU) seen
     int treeGrid(bool fp32, int bs, size_t lds);
@@ -187,7 +195,7 @@ struct DevicePlan {
     // launch parameters chosen by mlfma_autotune (0: compile-time defaults /
     // the per-level rule); tuneMinBlocks[l]: block target of the M2M (and
     // adjoint L2L) row split into parent level l; tuneM2L[l]: 1 sibling-group,
-    // 2 per-target M2L
+    // 2 per-target, 3 target-run M2L
     std::vector<int> tuneMinBlocks;
     std::vector<int> tuneM2L;
     int minBlocksAt(int l) const { return l < (int)tuneMinBlocks.size() ? tuneMinBlocks[l] : 0; }

@@ -254,8 +254,9 @@ def test_batched_stale_plan_stays_sequential():
 
 @pytest.mark.parametrize("name", NAMES)
 def test_m2l_kernels_agree(name):
-    """Both M2L kernels (per target, per sibling group) against the oracle,
-    single and batched right-hand sides, and the adjoint identity with each."""
+    """The three M2L kernels (per target, per sibling group, target runs with
+    their sources in LDS: the default) against the oracle, single and batched
+    right-hand sides, and the adjoint identity with each."""
     geom, kw = cases.make(name)
     pw = cases.planewave_z()
     x = G.complex_gaussian(geom.n, 61)
@@ -265,7 +266,7 @@ def test_m2l_kernels_agree(name):
     yo = ref.matvec(x)
     tol = max(1e-10, 10 * spread(geom, kw, x))
     out = {}
-    for flag in (A.MLFMA_FLAG_M2L_GROUP, A.MLFMA_FLAG_M2L_TARGET):
+    for flag in (A.MLFMA_FLAG_M2L_GROUP, A.MLFMA_FLAG_M2L_TARGET, 0):
         p = Plan(geom, pw, A.make_config(device=0, flags=A.MLFMA_FLAG_FRESH_LOCALS | flag, **kw))
         y = p.matvec(x)
         assert cases.relerr(y, yo) <= tol
@@ -273,9 +274,11 @@ def test_m2l_kernels_agree(name):
         out[flag] = (y, Yb)
         zhw = p.matvec_adjoint(w)
         assert abs(np.vdot(w, y) - np.vdot(zhw, x)) <= 1e-13 * np.linalg.norm(y) * np.linalg.norm(w)
-    g, t = out[A.MLFMA_FLAG_M2L_GROUP], out[A.MLFMA_FLAG_M2L_TARGET]
-    say(name, "group vs target", cases.relerr(g[0], t[0]), cases.relerr(g[1], t[1]))
+    g, t, r = out[A.MLFMA_FLAG_M2L_GROUP], out[A.MLFMA_FLAG_M2L_TARGET], out[0]
+    say(name, "group vs target", cases.relerr(g[0], t[0]), cases.relerr(g[1], t[1]), "runs vs target",
+        cases.relerr(r[0], t[0]), cases.relerr(r[1], t[1]))
     assert cases.relerr(g[0], t[0]) <= 1e-13 and cases.relerr(g[1], t[1]) <= 1e-13
+    assert cases.relerr(r[0], t[0]) <= 1e-13 and cases.relerr(r[1], t[1]) <= 1e-13
 
 
 def test_async_device_matvec():
