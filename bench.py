@@ -507,8 +507,10 @@ def hbm_rates():
 
 
 def kernel_bytes(group, info, n):
-    """Algorithmic bytes per launch of the kernel of a launch group (FP64,
-    c = 16 B): the per-unit figures times the units one launch processes.
+    """Algorithmic bytes per matvec of the launches of a launch group (FP64,
+    c = 16 B): the per-unit figures times the units the group's launches
+    process in one matvec (one launch for S2M, L2T, NEAR, TREE_UP; one per
+    level for M2M and L2L; the M2L levels in one or two launches).
       S2M, L2T  32 B per (source, leaf direction) of radiation patterns, plus
                 the leaf fields (32 B per leaf direction) once
       NEAR      16 B per stored near-field value
@@ -516,9 +518,11 @@ def kernel_bytes(group, info, n):
                 multipoles and target locals (32 B per box direction) once;
                 streamed: 32 B per (pair, direction) of source reads instead
       M2L_NEAR  the M2L bytes of the per-target levels plus NEAR
-      TREE_UP   S2M plus M2M: every multipole of levels >= 3 read once and
-                every multipole of levels 2 .. max-1 written once (32 B per
-                box direction)
+      M2M       per parent level: the child multipoles read and the parent
+                multipoles written once (32 B per box direction)
+      L2L       per child level: the parent locals read, the child locals
+                read and written once (32 B per box direction)
+      TREE_UP   S2M plus M2M
     Returns (bytes, streamed bytes)."""
     c = 16
     ml = info["max_level"]
@@ -536,8 +540,13 @@ def kernel_bytes(group, info, n):
         return m2l_comp, m2l_str
     if group == "M2L_NEAR":
         return m2l_comp + near, m2l_str + near
+    m2m = sum((B[l + 1] * nd[l + 1] + B[l] * nd[l]) * 2 * c for l in range(2, ml))
+    if group == "M2M":
+        return m2m, m2m
+    if group == "L2L":
+        l2l = sum((B[l - 1] * nd[l - 1] + 2 * B[l] * nd[l]) * 2 * c for l in range(3, ml + 1))
+        return l2l, l2l
     if group == "TREE_UP":
-        m2m = sum((B[l + 1] * nd[l + 1] + B[l] * nd[l]) * 2 * c for l in range(2, ml))
         return leaf + m2m, leaf + m2m
     return None, None
 
@@ -728,7 +737,8 @@ def main():
     dom = max(cand, key=lambda g: cand[g]["ms_per_matvec"])
     kb, kstream = kernel_bytes(dom, info, n)
     avg = groups[dom]["avg_launch_ms"]
-    achieved = kb / (avg * 1e-3) / 1e9
+    per = groups[dom]["ms_per_matvec"]  # = avg for a group of one launch per matvec
+    achieved = kb / (per * 1e-3) / 1e9
     tag = desc["workload"]
     traffic = measured_traffic(kname(dom), tag)
     roofline = {"bound": "hbm", "kernel": kname(dom), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -737,7 +747,7 @@ def main():
                 "avg_launch_ms": avg, "launch_group": dom}
     if kstream != kb:
         roofline["streamed_bytes_per_launch"] = int(kstream)
-        roofline["streamed_frac"] = round(kstream / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        roofline["streamed_frac"] = round(kstream / (per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     if dom == "NEAR" and near_pairs_ms > 0:
         # the same pair products alone, every stage in order (one block per
         # work item, kNearSym): the kernel's own rate, against the rate it
@@ -751,8 +761,8 @@ def main():
         b, bs = kernel_bytes(g, info, n)
         e = dict(v)
         if b:
-            e["bytes_per_launch"] = int(b)
-            e["frac_hbm_peak"] = round(b / (v["avg_launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            e["bytes_per_matvec"] = int(b)
+            e["frac_hbm_peak"] = round(b / (v["ms_per_matvec"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         kernels[g] = e
 
     # batched right-hand sides: 8 vectors per call share every operator read
