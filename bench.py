@@ -660,7 +660,7 @@ def main():
         # near-field fold); one: beside the chain with resident blocks the
         # near field runs kNearSymP
         if args.nrhs == 8 and g in ("NEAR", "L2T"):
-            return {"NEAR": "kNearSymM", "L2T": "kNearSum8+kL2TM"}[g]
+            return {"NEAR": "kNearSymW", "L2T": "kNearSum8+kL2TM"}[g]
         if args.nrhs > 1 and g == "L2T":
             return "kL2Tx"
         if g == "NEAR" and args.nrhs == 1 and schedule["near_overlap"] and schedule["near_blocks_per_cu"]:

@@ -1542,192 +1542,6 @@ __global__ __launch_bounds__(TB) void kNearSymB(const NearItem* __restrict__ ite
     }
 }
 
-// Eight right-hand sides on the matrix cores: per item the row sums
-// Y = Z X_cols (rows x 8) and, for the pair columns, Z^T X_rows, as real
-// FP64 MFMA products (v_mfma_f64_16x16x4f64: A 16 x 4, B 4 x 16, D 16 x 16;
-// lane l holds A[l%16][l/16], B[l/16][l%16] and D[4i + l/16][l%16], i < 4).
-// The 16 columns of B are the real parts of the 8 right-hand sides followed
-// by their imaginary parts, so with B' = [-Im | Re] one accumulator takes
-// Re Z . B + Im Z . B' = [Re Y | Im Y].  Row tiles of 16 go to the waves
-// (wave w: tiles w and w + 4), column tiles of 16 of an LDS tile to waves
-// 3, 2, 1, 0 (pair columns only); the value tiles and x gathers of the next
-// LDS tile are in flight during the products of this one.  Values are read
-// from HBM once.
-// x comes interleaved (xt[i 8 + k]) and the partial sums go out interleaved
-// (part[j 8 + k]), so each gather and each row's or column's eight results
-// are one 128-byte line.
-constexpr int NTM = 1024;    // values per LDS tile (40 KiB of LDS per block: four blocks per CU)
-constexpr int CWM = 64;      // columns per LDS tile at most
-constexpr int NLM = NTM / TB;
-
-template <bool CONJ, typename V>
-__global__ __launch_bounds__(TB) void kNearSymM(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
-                                                const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
-                                                const cplx<double>* __restrict__ xt,
-                                                cplx<double>* __restrict__ part) {
-    using d4 = double __attribute__((ext_vector_type(4)));
-    constexpr int NB = 8;
-    __shared__ cplx<double> sT[NTM];
-    __shared__ cplx<double> sxr[NB][128];
-    __shared__ cplx<double> sxc[NB][CWM];
-    const NearItem I = items[blockIdx.x];
-    const int nr = I.nr;  // 1..128
-    const int ld = nr | 1;
-    // columns per LDS tile: a multiple of 16 (whole MFMA column tiles for the
-    // column sums) where the tile holds 16, else of 4
-    const int tcw = NTM / ld;
-    const int cwMax = min(CWM, tcw >= 16 ? tcw & ~15 : tcw & ~3);
-    const int t0 = threadIdx.x;
-    const int lane = t0 & 63;
-    const int w = t0 >> 6;
-    const int lm = lane & 15, lk = lane >> 4;  // MFMA row / k index of this lane
-    const int rh = lm & 7;                     // right-hand side of this lane's B column
-    const bool reCol = lm < 8;                 // B column holds real parts
-    for (int e = t0; e < NB * nr; e += TB) {  // eight lanes read one row's 128 bytes
-        const int k = e & 7, q = e >> 3;
-        sxr[k][q] = xt[perm[I.rbeg + q] * NB + k];
-    }
-    const int nrt = (nr + 15) >> 4;
-    // Re Z . B and Im Z . B' in separate accumulators (two independent
-    // MFMA chains), added at the end
-    d4 accP[2], accQ[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) accP[j] = accQ[j] = d4{0, 0, 0, 0};
-    double* sTd = reinterpret_cast<double*>(sT);
-    static_assert(NB * CWM <= 2 * TB, "two column gathers per lane");
-    int c0 = 0, cw = min(cwMax, I.nc), ne = cw * nr;
-    V vr[NLM], vi[NLM];
-    cplx<double> xg0 = mk(0.0, 0.0), xg1 = mk(0.0, 0.0);
-    int64_t ia0 = 0, ia1 = 0;
-    // gather e of a tile: column e / 8, right-hand side e % 8 (one 128-byte
-    // line of xt per column)
-    auto colIdx = [&](int cc0, int ccw, int e) {
-        return e < NB * ccw ? (int64_t)colsIn[I.cptr + cc0 + (e >> 3)] * NB + (e & 7) : (int64_t)0;
-    };
-    auto loadVals = [&](int cc0, int nne) {
-        const V* sd = reinterpret_cast<const V*>(val + I.voff + (int64_t)cc0 * nr);
-#pragma unroll
-        for (int k = 0; k < NLM; ++k) {
-            const int q = 2 * min(t0 + k * TB, nne - 1);
-            vr[k] = sd[q];
-            vi[k] = sd[q + 1];
-        }
-    };
-    auto gather = [&](int ccw) {
-        if (t0 < NB * ccw) xg0 = xt[ia0];
-        if (t0 + TB < NB * ccw) xg1 = xt[ia1];
-    };
-    ia0 = colIdx(0, cw, t0);
-    ia1 = colIdx(0, cw, t0 + TB);
-    loadVals(0, ne);
-    gather(cw);
-    {
-        const int cn = cwMax, cwn = min(cwMax, I.nc - cn);
-        ia0 = cwn > 0 ? colIdx(cn, cwn, t0) : 0;
-        ia1 = cwn > 0 ? colIdx(cn, cwn, t0 + TB) : 0;
-    }
-    while (true) {
-        __syncthreads();  // the previous tile is consumed
-#pragma unroll
-        for (int k = 0; k < NLM; ++k) {
-            const int e = t0 + k * TB;
-            if (e < ne) {
-                const int c = e / nr;
-                const int a = 2 * (c * ld + (e - c * nr));
-                sTd[a] = (double)vr[k];
-                sTd[a + 1] = CONJ ? -(double)vi[k] : (double)vi[k];
-            }
-        }
-        if (t0 < NB * cw) sxc[t0 & 7][t0 >> 3] = xg0;
-        if (t0 + TB < NB * cw) sxc[(t0 + TB) & 7][(t0 + TB) >> 3] = xg1;
-        __syncthreads();
-        const int c0n = c0 + cwMax;
-        const bool more = c0n < I.nc;  // uniform over the block
-        int cwn = 0, nen = 0;
-        if (more) {
-            cwn = min(cwMax, I.nc - c0n);
-            nen = cwn * nr;
-            loadVals(c0n, nen);
-            gather(cwn);
-            const int cn = c0n + cwMax, cwnn = min(cwMax, I.nc - cn);
-            ia0 = cwnn > 0 ? colIdx(cn, cwnn, t0) : 0;
-            ia1 = cwnn > 0 ? colIdx(cn, cwnn, t0 + TB) : 0;
-        }
-        // row sums: k steps of 4 columns
-        // (LDS indices clamped so every read is valid; entries past the
-        // tile's columns or the item's rows count as zero)
-#pragma unroll 4
-        for (int ks = 0; ks < cw; ks += 4) {
-            const int c = ks + lk;
-            const bool cok = c < cw;
-            const int cc = min(c, cw - 1);
-            const cplx<double> xr = sxc[rh][cc];
-            const cplx<double> xx = cok ? xr : mk(0.0, 0.0);
-            const double b = reCol ? xx.re : xx.im;
-            const double bp = reCol ? -xx.im : xx.re;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int rt = w + 4 * j;
-                if (rt < nrt) {  // uniform per wave
-                    const int row = 16 * rt + lm;
-                    const cplx<double> zr = sT[cc * ld + min(row, nr - 1)];
-                    const bool ok = row < nr;
-                    accP[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? zr.re : 0.0, b, accP[j], 0, 0, 0);
-                    accQ[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? zr.im : 0.0, bp, accQ[j], 0, 0, 0);
-                }
-            }
-        }
-        // column sums of the pair columns: column tile 3 - w of this LDS tile
-        // (the waves without a second row tile take the first column tiles)
-        {
-            const int ct = 16 * (3 - w);
-            if (ct < cw && c0 + ct + 15 >= I.nself) {  // uniform per wave
-                d4 cP = d4{0, 0, 0, 0}, cQ = d4{0, 0, 0, 0};
-                const int col = ct + lm;
-                const int colc = min(col, cw - 1);
-#pragma unroll 4
-                for (int ks = 0; ks < nr; ks += 4) {
-                    const int row = ks + lk;
-                    const bool ok = col < cw && row < nr;
-                    const int rc = min(row, nr - 1);
-                    const cplx<double> zr = sT[colc * ld + rc];
-                    const cplx<double> xr = sxr[rh][rc];
-                    const cplx<double> xx = row < nr ? xr : mk(0.0, 0.0);
-                    const double b = reCol ? xx.re : xx.im;
-                    const double bp = reCol ? -xx.im : xx.re;
-                    cP = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? zr.re : 0.0, b, cP, 0, 0, 0);
-                    cQ = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? zr.im : 0.0, bp, cQ, 0, 0, 0);
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int c = ct + 4 * i + lk;
-                    if (c < cw && c0 + c >= I.nself) {
-                        double* o = reinterpret_cast<double*>(part + (I.cpo + c0 + c - I.nself) * NB + rh);
-                        o[reCol ? 0 : 1] = cP[i] + cQ[i];
-                    }
-                }
-            }
-        }
-        if (!more) break;
-        c0 = c0n;
-        cw = cwn;
-        ne = nen;
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int rt = w + 4 * j;
-        if (rt < nrt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int row = 16 * rt + 4 * i + lk;
-                if (row < nr) {
-                    double* o = reinterpret_cast<double*>(part + (I.rpo + row) * NB + rh);
-                    o[reCol ? 0 : 1] = accP[j][i] + accQ[j][i];
-                }
-            }
-    }
-}
-
 // Sum of the near-field partials of tree position t, split over the 64 lanes
 // of a wave (same lane assignment and butterfly every call: deterministic);
 // partial j at part[j * stride] (stride 8: the interleaved partials of eight
@@ -1798,7 +1612,7 @@ __global__ __launch_bounds__(TB) void kNearSum(const int64_t* __restrict__ rows,
     const int64_t o = perm[t];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        // eight right-hand sides: interleaved partials (kNearSymM)
+        // eight right-hand sides: interleaved partials (kNearSymW)
         const cplx<double> v =
             NB == 8 ? nearFold(ptr, idx, part + k, t, lane, 8) : nearFold(ptr, idx, part + k * ps, t, lane);
         if (lane == 0) y[k * xs + o] = accumulate ? y[k * xs + o] + v : v;
@@ -1835,14 +1649,158 @@ void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_
         throw Error(MLFMA_ERR_ARG, "near field: batch of 1, 2 or 4 (eight: launchNearSymM8)");
 #undef NEARB_CASE
 }
+// Eight right-hand sides on the matrix cores (v_mfma_f64_16x16x4f64: A
+// 16 x 4, B 4 x 16, D 16 x 16; lane l holds A[l%16][l/16], B[l/16][l%16] and
+// D[4i + l/16][l%16], i < 4).  The 16 columns of B are the real parts of the
+// eight right-hand sides followed by their imaginary parts, so with
+// B' = [-Im | Re] one accumulator takes Re Z . B + Im Z . B' = [Re Y | Im Y].
+// x comes interleaved (xt[i 8 + k]) and the partial sums go out interleaved
+// (part[j 8 + k]), so each gather and each row's or column's eight results
+// are one 128-byte line.  Values are read from memory once.
+//
+// Wave-tiled: every wave of the block takes its own
+// 16-column tiles of the item (tiles w, w + 4, ...) and, per tile, every
+// 16-row tile of the pass (rows 0-63, then 64-127 for taller items): it
+// loads the 16 x 16 block of values straight from memory in the MFMA A
+// layout of the row sums (lane l: row l % 16, column 4 s + l / 16 of the
+// block; four runs of 256 contiguous bytes per step), adds Z X_cols to the
+// pass's row sums it keeps in registers, and for pair columns passes the
+// block through its own LDS square to the transposed layout and adds
+// Z^T X_rows to the tile's column sums (the second pass adds to the first's,
+// which the same lane wrote).  No block barrier in the loop: the waves
+// stream independently, the next block's values in flight during this
+// block's products.  The row sums of the four waves are added in LDS in a
+// fixed order at the end of each pass.  At most 168 registers per lane
+// (three waves per SIMD: the compiler keeps two values in scratch).
+template <bool CONJ, typename V>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void kNearSymW(const NearItem* __restrict__ items, const int32_t* __restrict__ colsIn,
+                                                const int64_t* __restrict__ perm, const cplx<V>* __restrict__ val,
+                                                const cplx<double>* __restrict__ xt,
+                                                cplx<double>* __restrict__ part) {
+    using d4 = double __attribute__((ext_vector_type(4)));
+    constexpr int NB = 8, RT = 4;                  // right-hand sides, row tiles per pass
+    __shared__ cplx<double> sxr[128 * NB];         // x of the item's rows, interleaved
+    __shared__ double sZ[TB / 64][2][16][17];      // per wave: one 16 x 16 block (re, im), for the transpose
+    __shared__ double sRed[64 * 16];               // row sums of a pass, [row][re 0-7 | im 0-7]
+    const NearItem I = items[blockIdx.x];
+    const int nr = I.nr, nc = I.nc;
+    const int t0 = threadIdx.x, lane = t0 & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t0 >> 6);
+    const int lm = lane & 15, lk = lane >> 4;
+    const int rh = lm & 7;
+    const bool reCol = lm < 8;
+    for (int e = t0; e < NB * nr; e += TB) sxr[e] = xt[perm[I.rbeg + (e >> 3)] * NB + (e & 7)];
+    __syncthreads();
+    const int nrt = (nr + 15) >> 4, nct = (nc + 15) >> 4;
+    const cplx<V>* Zb = val + I.voff;
+    // lane's values of block (rt, ct), step s: row rt 16 + lm, column ct 16 + 4 s + lk (zero outside)
+    auto loadBlock = [&](int rt, int ct, cplx<double>* z) {
+        const int r = rt * 16 + lm;
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int c = ct * 16 + 4 * s4 + lk;
+            const bool ok = r < nr && c < nc;
+            const cplx<V> v = Zb[(int64_t)min(c, nc - 1) * nr + min(r, nr - 1)];
+            z[s4] = ok ? mk((double)v.re, CONJ ? -(double)v.im : (double)v.im) : mk(0.0, 0.0);
+        }
+    };
+    for (int pass = 0; pass * RT < nrt; ++pass) {  // uniform
+        const int rt0 = pass * RT, rt1 = min(nrt, rt0 + RT);
+        d4 acc[RT];  // both products of a row tile into one accumulator
+#pragma unroll
+        for (int j = 0; j < RT; ++j) acc[j] = d4{0, 0, 0, 0};
+        for (int ct = w; ct < nct; ct += TB / 64) {
+            // B of the row sums: x of the tile's columns
+            double bR[4], bRp[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int c = ct * 16 + 4 * s4 + lk;
+                const cplx<double> xv = c < nc ? xt[(int64_t)colsIn[I.cptr + c] * NB + rh] : mk(0.0, 0.0);
+                bR[s4] = reCol ? xv.re : xv.im;
+                bRp[s4] = reCol ? -xv.im : xv.re;
+            }
+            const bool pairs = ct * 16 + 15 >= I.nself;  // uniform per wave
+            d4 cA = d4{0, 0, 0, 0};
+            cplx<double> z[4], zn[4];
+            loadBlock(rt0, ct, z);
+#pragma unroll
+            for (int j = 0; j < RT; ++j) {
+                const int rt = rt0 + j;
+                if (rt < rt1) {  // uniform
+                    if (rt + 1 < rt1) loadBlock(rt + 1, ct, zn);
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) {
+                        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(z[s4].re, bR[s4], acc[j], 0, 0, 0);
+                        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(z[s4].im, bRp[s4], acc[j], 0, 0, 0);
+                    }
+                    if (pairs) {
+                        // transpose through the wave's LDS square: element (row lm, column 4 s + lk)
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; ++s4) {
+                            sZ[w][0][lm][4 * s4 + lk] = z[s4].re;
+                            sZ[w][1][lm][4 * s4 + lk] = z[s4].im;
+                        }
+                        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wave's LDS writes are done
+                        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; ++s4) {
+                            // A: Z^T, element (column lm, row 4 s + lk) of the block; B: x of that row
+                            const double ar = sZ[w][0][4 * s4 + lk][lm], ai = sZ[w][1][4 * s4 + lk][lm];
+                            const int r = rt * 16 + 4 * s4 + lk;
+                            const cplx<double> xv = r < nr ? sxr[r * NB + rh] : mk(0.0, 0.0);
+                            const double b = reCol ? xv.re : xv.im, bp = reCol ? -xv.im : xv.re;
+                            cA = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b, cA, 0, 0, 0);
+                            cA = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bp, cA, 0, 0, 0);
+                        }
+                        __builtin_amdgcn_s_waitcnt(0xc07f);  // the reads are done before the next block's writes
+                        __builtin_amdgcn_wave_barrier();
+                    }
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) z[s4] = zn[s4];
+                }
+            }
+            if (pairs)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int c = ct * 16 + 4 * i + lk;
+                    if (c < nc && c >= I.nself) {
+                        double* o = reinterpret_cast<double*>(part + (I.cpo + c - I.nself) * NB + rh) + (reCol ? 0 : 1);
+                        *o = pass ? *o + cA[i] : cA[i];
+                    }
+                }
+        }
+        // the pass's row sums of the four waves, added in wave order
+        for (int wv = 0; wv < TB / 64; ++wv) {
+            if (w == wv)
+#pragma unroll
+                for (int j = 0; j < RT; ++j)
+                    if (rt0 + j < rt1)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int row = 16 * j + 4 * i + lk;  // within the pass
+                            const double v = acc[j][i];
+                            sRed[row * 16 + lm] = wv ? sRed[row * 16 + lm] + v : v;
+                        }
+            __syncthreads();
+        }
+        const int prow = min(64, nr - 64 * pass);
+        for (int e = t0; e < prow * 16; e += TB) {
+            const int row = 64 * pass + (e >> 4), n = e & 15;
+            double* o = reinterpret_cast<double*>(part + (I.rpo + row) * NB + (n & 7));
+            o[n < 8 ? 0 : 1] = sRed[e];
+        }
+        __syncthreads();  // sRed is reused by the next pass
+    }
+}
+
 template <typename V>
 void launchNearSymM8(bool conjVal, const NearItem* items, int nitems, const int32_t* colsIn, const int64_t* perm,
                      const cplx<V>* val, const cplx<double>* xt, cplx<double>* part, hipStream_t st) {
     if (nitems <= 0) return;
     if (conjVal)
-        hipLaunchKernelGGL((kNearSymM<true, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, xt, part);
+        hipLaunchKernelGGL((kNearSymW<true, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, xt, part);
     else
-        hipLaunchKernelGGL((kNearSymM<false, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, xt, part);
+        hipLaunchKernelGGL((kNearSymW<false, V>), dim3(nitems), dim3(TB), 0, st, items, colsIn, perm, val, xt, part);
 }
 template void launchNearSymM8<double>(bool, const NearItem*, int, const int32_t*, const int64_t*,
                                       const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t);
