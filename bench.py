@@ -424,6 +424,89 @@ def multi_scatterer_job():
     print(json.dumps(out), flush=True)
 
 
+def c4_stalls_job():
+    """Why some c4 ellipsoids stall: an interior resonance of the EFIE or the
+    mesh.  (1) Every c4 ellipsoid under theta = 0 incidence with the
+    multi-scatterer solver (FP64 FGMRES(30) around FP32-far-field GMRES(30)
+    to 0.1, tol 1e-6, at most 6,000 iterations).  (2) Every one that stalls
+    again with (a) the 13-point triangle rule, (b) a mesh 1.5x finer (the same
+    2:1 mapping of an icosphere, smaller triangles) and (c) the same mesh
+    scaled by 0.98, 0.99, 1.01 and 1.02 (the same discretisation at another
+    electrical size).  A stall that survives the finer rule and mesh and
+    goes away with a 1-2% size change is a resonance; one that follows the
+    mesh is the discretisation.  (3) Dense: the smallest singular value of the
+    physical EFIE matrix (one-leaf plan: every pair in the near field) and the
+    FP64 GMRES(100) iterations over a size sweep, for a sphere of radius 0.6
+    across its TM_2 interior resonance (kr = 3.870, first zero of
+    (x j_2(x))') as the calibration, and for the smallest ellipsoid."""
+    from mlfma_amd import _capi as A
+    from mlfma_amd import configs as CF
+    from mlfma_amd.plan import Plan
+    flags = A.MLFMA_FLAG_PHYSICAL | A.MLFMA_FLAG_FRESH_LOCALS
+    pw0 = CF.planewave_xz(0.0)
+    opts = dict(restart=30, outer_restart=30, inner_tol=0.1, maxit=6000, tol=1e-6, method=A.MLFMA_FGMRES_IR)
+
+    def solve(sh):
+        p = Plan(sh["geom"], pw0, A.make_config(device=0, precision=A.MLFMA_FP32, flags=flags, **sh["kw"]))
+        _, rep = p.solve(-p.excitation(pw0), **opts)
+        p.close()
+        return {"unknowns": sh["n"], "iterations": rep["iterations"], "status": rep["status"],
+                "resid_true": float("%.3g" % rep["resid_true"])}
+
+    shapes = CF.c4()
+    ells = [(i, sh) for i, sh in enumerate(shapes) if sh["kind"] == "ellipsoid"]
+    first = {}
+    for i, sh in ells:
+        first[i] = solve(sh)
+        log("c4 scatterer", i, "r_eq", round(sh["r_eq"], 3), first[i])
+    variants = {}
+    for i, sh in ells:
+        if first[i]["status"] == "OK":
+            continue
+        v = {"c4": first[i]}
+        v["rule13"] = solve(CF.c4_scatterer("ellipsoid", sh["r_eq"], quad_prec=3))
+        v["mesh1.5x"] = solve(CF.c4_scatterer("ellipsoid", sh["r_eq"], mesh_scale=1.5))
+        for s in (0.98, 0.99, 1.01, 1.02):
+            v["size%.2f" % s] = solve(CF.c4_scatterer("ellipsoid", sh["r_eq"], scale=s))
+        log("stalled scatterer", i, v)
+        variants[str(i)] = {"r_eq": round(sh["r_eq"], 4), **v}
+
+    def dense_sweep(kind, r_eq, scales):
+        out = []
+        for s in scales:
+            sh = CF.c4_scatterer(kind, r_eq, scale=s)
+            n = sh["n"]
+            kw = dict(sh["kw"], max_node_srcs=n, max_level=0)
+            p = Plan(sh["geom"], pw0, A.make_config(device=0, flags=flags, **kw))
+            Z = np.empty((n, n), dtype=np.complex128)
+            for j0 in range(0, n, 8):
+                j1 = min(n, j0 + 8)
+                E = np.zeros((j1 - j0, n), dtype=np.complex128)
+                E[np.arange(j1 - j0), np.arange(j0, j1)] = 1.0
+                Z[:, j0:j1] = p.matvec(E).T
+            sv = np.linalg.svd(Z, compute_uv=False)
+            _, rep = p.solve(-p.excitation(pw0), method=A.MLFMA_GMRES_RESTART, restart=100, maxit=3000, tol=1e-6)
+            p.close()
+            extent = r_eq if kind == "sphere" else 2 * r_eq / 2 ** (1 / 3)
+            row = {"scale": round(s, 4), "k_times_semi_axis": round(2 * np.pi * extent * s, 4), "unknowns": n,
+                   "sigma_min_over_max": float("%.4g" % (sv[-1] / sv[0])), "gmres100_iterations": rep["iterations"],
+                   "status": rep["status"]}
+            log("dense", kind, r_eq, row)
+            out.append(row)
+        return out
+
+    sphere = dense_sweep("sphere", 0.6, [0.96 + 0.01 * j for j in range(14)])
+    small = min((sh["n"], i) for i, sh in ells)[1]
+    ellipsoid = dense_sweep("ellipsoid", shapes[small]["r_eq"], [0.96 + 0.01 * j for j in range(9)])
+    print(json.dumps({"mode": "c4-stalls", "solver": "FGMRES(30) around FP32-far-field GMRES(30) to 0.1, tol 1e-6, "
+                      "maxit 6000, theta=0 incidence, physical near field",
+                      "ellipsoids": {str(i): {"r_eq": round(sh["r_eq"], 4), **first[i]} for i, sh in ells},
+                      "stalled_variants": variants,
+                      "dense_sphere_r0.6": sphere,
+                      "dense_smallest_ellipsoid": {"scatterer": small, "r_eq": round(shapes[small]["r_eq"], 4),
+                                                   "sweep": ellipsoid}}), flush=True)
+
+
 def m2l_ceiling(args):
     """Cache-resident gather ceiling for M2L: per level, each target sums the
     fields of its interaction-list sources (32 B per direction) and the
@@ -577,11 +660,12 @@ def main():
     ap.add_argument("--fp32", action="store_true",
                     help="with --timed-only: the plan's far field in FP32 (the inner matvec of the "
                          "mixed-precision solves) instead of FP64")
-    ap.add_argument("--mode", default="matvec", choices=["matvec", "copy", "m2l-ceiling", "hbm"],
+    ap.add_argument("--mode", default="matvec", choices=["matvec", "copy", "m2l-ceiling", "hbm", "c4-stalls"],
                     help="copy: only run a plain 256 MiB device copy (HBM counter calibration); m2l-ceiling: the "
                          "M2L access pattern (per target, the sum of its sources' fields and operators) through "
                          "torch's gather-sum (embedding_bag) on the workload's interaction lists; hbm: torch's "
-                         "copy and sum rates on GiB arrays")
+                         "copy and sum rates on GiB arrays; c4-stalls: the stalled c4 ellipsoids against finer "
+                         "rules and meshes and 1-2% size changes, and dense singular values over size sweeps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the library runs the matvec on one GPU and has no multi-GPU "
@@ -609,6 +693,10 @@ def main():
 
     if args.mode == "hbm":
         hbm_rates()
+        return
+
+    if args.mode == "c4-stalls":
+        c4_stalls_job()
         return
 
     if args.workload == "dipoles200k_grad":

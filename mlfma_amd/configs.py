@@ -70,32 +70,44 @@ def c3():
     return _sphere(82, 4.0, 8.0, 5, "rwg_sphere_R4_icosphere82")
 
 
+def c4_scatterer(kind, r_eq, mesh_per_wavelength=15.0, scale=1.0, mesh_scale=1.0, quad_prec=2):
+    """One scatterer of the c4 family: a sphere of radius r_eq (icosphere of
+    frequency round(m r_eq)) or a 2:1:1 ellipsoid of equivalent radius r_eq
+    (semi-axes 2a, a, a with a = r_eq / 2^(1/3); an icosphere of frequency
+    round(m a) mapped onto it), m = mesh_per_wavelength x mesh_scale.  The
+    mesh is then scaled by ``scale`` (same triangles, another electrical
+    size).  Fixed depth: root = 0.25 x 2^levels >= 2.05 x the largest
+    semi-axis.  Returns a dict with kind, r_eq, geom, kw, n and levels."""
+    m = mesh_per_wavelength * mesh_scale
+    if kind == "ellipsoid":
+        a = r_eq / 2 ** (1 / 3)
+        v, f = G.ellipsoid(max(4, int(round(m * a))), (2 * a, a, a))
+        ext = 2 * a
+    else:
+        v, f = G.icosphere(max(4, int(round(m * r_eq))), r_eq)
+        ext = r_eq
+    v = v * scale
+    ext *= scale
+    r = G.rwg_edges(f)
+    L = 2
+    while 0.25 * 2 ** L < 2.05 * ext:
+        L += 1
+    kw = dict(root_leng=0.25 * 2 ** L, max_node_srcs=0, max_level=L, interp_order=6, digits=3,
+              over_interp=4.0, quad_prec=quad_prec)
+    return dict(kind=kind, r_eq=float(r_eq), n=int(len(r)), levels=L,
+                geom=A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=r), kw=kw)
+
+
 def c4(mesh_per_wavelength=15.0):
-    """The 16 scatterers: list of dicts with kind, r_eq, geom, kw (fixed depth,
-    root = 0.25 * 2^levels >= 2.05 x the largest semi-axis) and n.  Spheres are
-    icospheres of frequency round(m r_eq); ellipsoids (semi-axes 2a, a, a with
-    a = r_eq / 2^(1/3)) map an icosphere of frequency round(m a) onto the
-    ellipsoid (m = mesh_per_wavelength)."""
+    """The 16 scatterers (seed 1): each a sphere or a 2:1:1 ellipsoid with
+    probability 1/2, equivalent radius uniform in 0.5-3 wavelengths; built by
+    c4_scatterer."""
     rng = np.random.default_rng(1)
     out = []
     for _ in range(16):
         r_eq = rng.uniform(0.5, 3.0)
         ell = rng.uniform() < 0.5
-        if ell:
-            a = r_eq / 2 ** (1 / 3)
-            v, f = G.ellipsoid(max(4, int(round(mesh_per_wavelength * a))), (2 * a, a, a))
-            ext = 2 * a
-        else:
-            v, f = G.icosphere(max(4, int(round(mesh_per_wavelength * r_eq))), r_eq)
-            ext = r_eq
-        r = G.rwg_edges(f)
-        L = 2
-        while 0.25 * 2 ** L < 2.05 * ext:
-            L += 1
-        kw = dict(root_leng=0.25 * 2 ** L, max_node_srcs=0, max_level=L, interp_order=6, digits=3,
-                  over_interp=4.0, quad_prec=2)
-        out.append(dict(kind="ellipsoid" if ell else "sphere", r_eq=float(r_eq), n=int(len(r)), levels=L,
-                        geom=A.GeometryArrays(A.MLFMA_SRC_RWG, verts=v, faces=f, rwgs=r), kw=kw))
+        out.append(c4_scatterer("ellipsoid" if ell else "sphere", r_eq, mesh_per_wavelength))
     return out
 
 
