@@ -740,7 +740,8 @@ def main():
                 "m2l_split": plan.get_param(A.MLFMA_PARAM_M2L_SPLIT),
                 "near_window": plan.get_param(A.MLFMA_PARAM_NEAR_WINDOW),
                 "near_blocks_per_cu": plan.get_param(A.MLFMA_PARAM_NEAR_BLOCKS),
-                "tree_walk": plan.get_param(A.MLFMA_PARAM_TREE_WALK)}
+                "tree_walk": plan.get_param(A.MLFMA_PARAM_TREE_WALK),
+                "near_split_pct": plan.get_param(A.MLFMA_PARAM_NEAR_SPLIT)}
     log("autotune ms", tune_before, "->", tune_after, schedule)
 
     def kname(g):
@@ -829,10 +830,14 @@ def main():
     achieved = kb / (per * 1e-3) / 1e9
     tag = desc["workload"]
     traffic = measured_traffic(kname(dom), tag)
+    # (a group of several launches per matvec: the per-launch bytes are the
+    # mean over its launches, so bytes / average launch time = bytes per
+    # matvec / its launches' time per matvec)
+    nl = groups[dom]["launches_per_matvec"]
     roofline = {"bound": "hbm", "kernel": kname(dom), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": round(traffic) if traffic else None, "bytes_per_launch": int(kb),
-                "avg_launch_ms": avg, "launch_group": dom}
+                "traffic": round(traffic) if traffic else None, "bytes_per_launch": int(kb / nl),
+                "avg_launch_ms": avg, "launches_per_matvec": nl, "bytes_per_matvec": int(kb), "launch_group": dom}
     if kstream != kb:
         roofline["streamed_bytes_per_launch"] = int(kstream)
         roofline["streamed_frac"] = round(kstream / (per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)

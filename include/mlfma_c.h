@@ -403,7 +403,11 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
  *   MLFMA_PARAM_TREE_STALLS    (get only) waits of the tree walks that hit
  *                              their poll bound since the plan was made (a
  *                              non-zero count means a wrong result; reading
- *                              it waits for the plan's stream) */
+ *                              it waits for the plan's stream)
+ *   MLFMA_PARAM_NEAR_SPLIT     0..100: beside the chain, % of the near-field
+ *                              work items forked at the NEAR_OVERLAP point;
+ *                              the rest fork after the M2L, beside the L2L
+ *                              launches (100: one launch) */
 #define MLFMA_PARAM_NEAR_WINDOW 1
 #define MLFMA_PARAM_NEAR_OVERLAP 2
 #define MLFMA_PARAM_M2L_SPLIT 3
@@ -413,6 +417,7 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
 #define MLFMA_PARAM_NEAR_BLOCKS 6
 #define MLFMA_PARAM_TREE_WALK 7
 #define MLFMA_PARAM_TREE_STALLS 8
+#define MLFMA_PARAM_NEAR_SPLIT 9
 int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value);
 int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value);
 

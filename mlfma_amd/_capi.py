@@ -43,6 +43,7 @@ MLFMA_PARAM_PROBE_GROUP = 5
 MLFMA_PARAM_NEAR_BLOCKS = 6
 MLFMA_PARAM_TREE_WALK = 7
 MLFMA_PARAM_TREE_STALLS = 8
+MLFMA_PARAM_NEAR_SPLIT = 9
 GROUPS = {"S2M": 1, "M2M": 2, "M2L_TARGET": 3, "M2L_SIBLING": 4, "M2L_NEAR": 5, "L2L": 6, "NEAR": 7, "L2T": 8,
           "TREE_UP": 9, "M2L_RUNS": 10}
 

@@ -1624,13 +1624,16 @@ void launchNearSym(bool conjVal, const NearItem* items, int nitems, const int32_
                    const cplx<V>* val, const cplx<double>* x, cplx<double>* part, hipStream_t st, int nb,
                    int64_t xs, int64_t ps, int grid) {
     if (nitems <= 0) return;
-    if (nb == 1 && grid > 0 && grid < nitems) {
+    if (nb == 1 && grid > 0) {
+        // resident blocks whenever asked for (the kernel, not the item count,
+        // fixes how each row's sum is grouped)
+        const int g = std::min(grid, nitems);
         if (conjVal)
-            hipLaunchKernelGGL((kNearSymP<true, V>), dim3(grid), dim3(TB), 0, st, items, nitems, colsIn, perm, val, x,
+            hipLaunchKernelGGL((kNearSymP<true, V>), dim3(g), dim3(TB), 0, st, items, nitems, colsIn, perm, val, x,
                                part);
         else
-            hipLaunchKernelGGL((kNearSymP<false, V>), dim3(grid), dim3(TB), 0, st, items, nitems, colsIn, perm, val,
-                               x, part);
+            hipLaunchKernelGGL((kNearSymP<false, V>), dim3(g), dim3(TB), 0, st, items, nitems, colsIn, perm, val, x,
+                               part);
         return;
     }
 #define NEARB_CASE(C, B)                                                                                      \

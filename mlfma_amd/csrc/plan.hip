@@ -103,7 +103,7 @@ DevicePlan::~DevicePlan() {
     }
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {evFork, evJoin, evM2L, evNear})
+    for (hipEvent_t e : {evFork, evJoin, evM2L, evNear, evNearB, evJoinB})
         if (e) (void)hipEventDestroy(e);
     if (nearStream) (void)hipStreamDestroy(nearStream);
     if (m2lStream) (void)hipStreamDestroy(m2lStream);
@@ -167,6 +167,8 @@ void DevicePlan::build() {
     hipCheck(hipStreamCreateWithFlags(&m2lStream, hipStreamNonBlocking), "hipStreamCreate");
     hipCheck(hipEventCreateWithFlags(&evM2L, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
     hipCheck(hipEventCreateWithFlags(&evNear, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evNearB, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
+    hipCheck(hipEventCreateWithFlags(&evJoinB, MLFMA_SYNC_EVENT_FLAGS), "hipEventCreate");
     const int64_t n = h.src.n;
     const Tree& t = h.tree;
     const int ML = t.maxLevel;
@@ -1012,7 +1014,12 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
     // M2M chain in mode 3; the near field forked before S2M has its own) and
     // join through one: the M2L stream waits for the near field before it
     // records evM2L.
-    auto launchNear = [&](hipStream_t s) {
+    // beside the chain the work items [0, nearCut) fork at the overlap point
+    // and the rest after the M2L (every item writes its own partial sums)
+    const int nearCut = overlap && nearSplit < 100 ? (int)((int64_t)nNearItems * nearSplit / 100) : nNearItems;
+    auto launchNear = [&](hipStream_t s, int i0, int cnt) {
+        if (cnt <= 0) return;
+        const NearItem* items = nearItems + i0;
         probeBegin(MLFMA_GROUP_NEAR, s);
         struct End {
             DevicePlan* d;
@@ -1024,25 +1031,25 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         // eight right-hand sides: the matrix-core kernel on x interleaved
         if (nb == 8) {
             if (std::is_same<R, float>::value && nearVal32)
-                launchNearSymM8<float>(false, nearItems, nNearItems, nearColsIn, perm, nearVal32, xInner, nearPart,
+                launchNearSymM8<float>(false, items, cnt, nearColsIn, perm, nearVal32, xInner, nearPart,
                                        s);
             else
-                launchNearSymM8<double>(false, nearItems, nNearItems, nearColsIn, perm, nearVal, xInner, nearPart, s);
+                launchNearSymM8<double>(false, items, cnt, nearColsIn, perm, nearVal, xInner, nearPart, s);
             return;
         }
         // beside the chain, a fixed number of blocks (nearBlocks per CU)
         const int grid = s == nearStream ? nearGrid() : 0;
         if (std::is_same<R, float>::value && nearVal32)
-            launchNearSym<float>(false, nearItems, nNearItems, nearColsIn, perm, nearVal32, x, nearPart, s, nb, xs,
+            launchNearSym<float>(false, items, cnt, nearColsIn, perm, nearVal32, x, nearPart, s, nb, xs,
                                  ps, grid);
         else
-            launchNearSym<double>(false, nearItems, nNearItems, nearColsIn, perm, nearVal, x, nearPart, s, nb, xs, ps,
+            launchNearSym<double>(false, items, cnt, nearColsIn, perm, nearVal, x, nearPart, s, nb, xs, ps,
                                   grid);
     };
     if (overlap == 1) {
         hipCheck(hipEventRecord(evNear, st), "event");
         hipCheck(hipStreamWaitEvent(nearStream, evNear, 0), "wait");
-        launchNear(nearStream);
+        launchNear(nearStream, 0, nearCut);
         hipCheck(hipEventRecord(evJoin, nearStream), "event");
     }
     if (timed) hipCheck(hipEventRecord(ev[0], st), "event");
@@ -1065,7 +1072,7 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         hipCheck(hipEventRecord(evFork, st), "event");
         if (nearSide) {
             hipCheck(hipStreamWaitEvent(nearStream, evFork, 0), "wait");
-            launchNear(nearStream);
+            launchNear(nearStream, 0, nearCut);
             hipCheck(hipEventRecord(evJoin, nearStream), "event");
         }
         if (splitM2L) {
@@ -1115,6 +1122,13 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
                 launchZeroSlots<R>(L.nstale, L.staleSlots, L.ndir, Lc + r * fs + L.Moff, st);
             }
     if (timed) hipCheck(hipEventRecord(ev[3], st), "event");
+    const bool nearLate = nearCut < nNearItems;
+    if (nearLate) {
+        hipCheck(hipEventRecord(evNearB, st), "event");
+        hipCheck(hipStreamWaitEvent(nearStream, evNearB, 0), "wait");
+        launchNear(nearStream, nearCut, nNearItems - nearCut);
+        hipCheck(hipEventRecord(evJoinB, nearStream), "event");
+    }
     for (int l = 3; l <= ML; ++l) {
         LevelBufs& P = levels[l - 1];
         LevelBufs& C = levels[l];
@@ -1133,8 +1147,9 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
         probeEnd(MLFMA_GROUP_L2L, st);
     }
     joinSide();
+    if (nearLate) hipCheck(hipStreamWaitEvent(st, evJoinB, 0), "wait");
     if (timed) hipCheck(hipEventRecord(ev[4], st), "event");
-    if (!overlap && !nearFused) launchNear(st);
+    if (!overlap && !nearFused) launchNear(st, 0, nNearItems);
     if (timed) hipCheck(hipEventRecord(ev[5], st), "event");
     launchNearSum(nearOnlyRows, nNearOnlyRows, nearSumPtr, nearSumIdx, nearPart, perm, y, accumulate, st, nb, xs, ps);
     if (timed) hipCheck(hipEventRecord(ev[6], st), "event");
@@ -1762,34 +1777,40 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
         const int NTW = nrhs == 1 ? 2 : 1;
         const int twKept = d.treeWalk;  // batched tuning keeps the one-RHS choice
         if (!(saved & MLFMA_FLAG_ONE_STREAM)) {
-            constexpr int NNB = 5;
-            double tm[4][2][NNB][2] = {};
+            constexpr int NNB = 5, NSL = 2;
+            const int splits[NSL] = {100, 60};  // near items forked first (%), the rest beside the L2L
+            double tm[4][2][NNB][2][NSL] = {};
             for (int rep = 0; rep < 2; ++rep)
                 for (int m = 0; m < 4; ++m)
                     for (int sp = 0; sp < 2; ++sp)
                         for (int nbk = 0; nbk < (m ? NNB : 1); ++nbk)
-                            for (int tw = 0; tw < NTW; ++tw) {
-                                d.nearOverlap = m;
-                                d.m2lSplit = sp == 1;
-                                d.nearBlocks = nbk;
-                                d.treeWalk = NTW == 2 ? tw : twKept;
-                                tm[m][sp][nbk][tw] += timeWhole(10);
-                            }
-            int bm = 2, bs = 1, bk = 0, bt = 0;
+                            for (int tw = 0; tw < NTW; ++tw)
+                                for (int sl = 0; sl < (m ? NSL : 1); ++sl) {
+                                    d.nearOverlap = m;
+                                    d.m2lSplit = sp == 1;
+                                    d.nearBlocks = nbk;
+                                    d.treeWalk = NTW == 2 ? tw : twKept;
+                                    d.nearSplit = splits[sl];
+                                    tm[m][sp][nbk][tw][sl] += timeWhole(10);
+                                }
+            int bm = 2, bs = 1, bk = 0, bt = 0, bl = 0;
             for (int m = 0; m < 4; ++m)
                 for (int sp = 0; sp < 2; ++sp)
                     for (int nbk = 0; nbk < (m ? NNB : 1); ++nbk)
                         for (int tw = 0; tw < NTW; ++tw)
-                            if (tm[m][sp][nbk][tw] < tm[bm][bs][bk][bt]) {
-                                bm = m;
-                                bs = sp;
-                                bk = nbk;
-                                bt = tw;
-                            }
+                            for (int sl = 0; sl < (m ? NSL : 1); ++sl)
+                                if (tm[m][sp][nbk][tw][sl] < tm[bm][bs][bk][bt][bl]) {
+                                    bm = m;
+                                    bs = sp;
+                                    bk = nbk;
+                                    bt = tw;
+                                    bl = sl;
+                                }
             d.nearOverlap = bm;
             d.m2lSplit = bs == 1;
             d.nearBlocks = bk;
             d.treeWalk = NTW == 2 ? bt : twKept;
+            d.nearSplit = splits[bl];
             // how early the pair blocks of the shared M2L / near-field launch
             // end; that launch runs only for one right-hand side with the
             // stages in order (the in-order matvec mode, or solves run in order)
@@ -1897,6 +1918,10 @@ int mlfma_plan_set_param(mlfma_plan* plan, int32_t param, int32_t value) {
             if (value < 0 || value > 1) throw Error(MLFMA_ERR_ARG, "tree walk must be 0 or 1");
             d.treeWalk = value;
             break;
+        case MLFMA_PARAM_NEAR_SPLIT:
+            if (value < 0 || value > 100) throw Error(MLFMA_ERR_ARG, "near split must be 0..100 (%)");
+            d.nearSplit = value;
+            break;
         case MLFMA_PARAM_PROBE_GROUP:
             if (value < 0 || value > MLFMA_GROUP_LAST) throw Error(MLFMA_ERR_ARG, "unknown launch group");
             d.probeDrain(0);
@@ -1924,6 +1949,7 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value) 
         case MLFMA_PARAM_PROBE_GROUP: *value = d.probeGroup; break;
         case MLFMA_PARAM_NEAR_BLOCKS: *value = d.nearBlocks; break;
         case MLFMA_PARAM_TREE_WALK: *value = d.treeWalk; break;
+        case MLFMA_PARAM_NEAR_SPLIT: *value = d.nearSplit; break;
         case MLFMA_PARAM_TREE_STALLS: {
             needDevice(d);
             if (!d.treeCtr) throw Error(MLFMA_ERR_STATE, "no tree-walk counters");
@@ -1940,20 +1966,21 @@ int mlfma_plan_get_param(const mlfma_plan* plan, int32_t param, int32_t* value) 
     API_END
 }
 
-// Tuning record: [version 3, nearOverlap, m2lSplit, nearWindow, solveSerial,
-// levels L, tuneMinBlocks[0..L), tuneM2L[0..L), nearBlocks, treeWalk];
-// version 2 is the same without treeWalk (0), version 1 also without
-// nearBlocks (0)
+// Tuning record: [version 4, nearOverlap, m2lSplit, nearWindow, solveSerial,
+// levels L, tuneMinBlocks[0..L), tuneM2L[0..L), nearBlocks, treeWalk,
+// nearSplit]; version 3 is the same without nearSplit (100), version 2 also
+// without treeWalk (0), version 1 also without nearBlocks (0)
 int mlfma_autotune_record(const mlfma_plan* plan, int32_t* rec, int32_t cap, int32_t* len) {
     API_BEGIN
     const DevicePlan& d = P(plan);
     if (!len) throw Error(MLFMA_ERR_ARG, "null argument");
     const int L = d.hp.tree.maxLevel + 1;
-    std::vector<int32_t> r{3, d.nearOverlap, d.m2lSplit ? 1 : 0, d.nearWindow, d.solveSerial ? 1 : 0, L};
+    std::vector<int32_t> r{4, d.nearOverlap, d.m2lSplit ? 1 : 0, d.nearWindow, d.solveSerial ? 1 : 0, L};
     for (int l = 0; l < L; ++l) r.push_back(l < (int)d.tuneMinBlocks.size() ? d.tuneMinBlocks[l] : 0);
     for (int l = 0; l < L; ++l) r.push_back(l < (int)d.tuneM2L.size() ? d.tuneM2L[l] : 0);
     r.push_back(d.nearBlocks);
     r.push_back(d.treeWalk);
+    r.push_back(d.nearSplit);
     *len = (int32_t)r.size();
     if (rec)
         for (int32_t i = 0; i < std::min<int32_t>(cap, *len); ++i) rec[i] = r[i];
@@ -1967,12 +1994,15 @@ int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
     needDevice(d);
     const int L = d.hp.tree.maxLevel + 1;
     const bool v1 = rec && len == 6 + 2 * L && rec[0] == 1, v2 = rec && len == 7 + 2 * L && rec[0] == 2,
-               v3 = rec && len == 8 + 2 * L && rec[0] == 3;
-    if ((!v1 && !v2 && !v3) || rec[5] != L) throw Error(MLFMA_ERR_ARG, "tuning record does not match this plan");
-    const int nearBlocks = v2 || v3 ? rec[6 + 2 * L] : 0;
-    const int treeWalk = v3 ? rec[7 + 2 * L] : 0;
+               v3 = rec && len == 8 + 2 * L && rec[0] == 3, v4 = rec && len == 9 + 2 * L && rec[0] == 4;
+    if ((!v1 && !v2 && !v3 && !v4) || rec[5] != L)
+        throw Error(MLFMA_ERR_ARG, "tuning record does not match this plan");
+    const int nearBlocks = v2 || v3 || v4 ? rec[6 + 2 * L] : 0;
+    const int treeWalk = v3 || v4 ? rec[7 + 2 * L] : 0;
+    const int nearSplit = v4 ? rec[8 + 2 * L] : 100;
     if (rec[1] < 0 || rec[1] > 3 || rec[2] < 0 || rec[2] > 1 || rec[3] < 1 || rec[3] > 100 || rec[4] < 0 ||
-        rec[4] > 1 || nearBlocks < 0 || nearBlocks > 8 || treeWalk < 0 || treeWalk > 1)
+        rec[4] > 1 || nearBlocks < 0 || nearBlocks > 8 || treeWalk < 0 || treeWalk > 1 || nearSplit < 0 ||
+        nearSplit > 100)
         throw Error(MLFMA_ERR_ARG, "tuning record out of range");
     for (int l = 0; l < L; ++l)
         if (rec[6 + l] < 0 || rec[6 + L + l] < 0 || rec[6 + L + l] > 3)
@@ -1985,6 +2015,7 @@ int mlfma_autotune_apply(mlfma_plan* plan, const int32_t* rec, int32_t len) {
     d.tuneM2L.assign(rec + 6 + L, rec + 6 + 2 * L);
     d.nearBlocks = nearBlocks;
     d.treeWalk = treeWalk;
+    d.nearSplit = nearSplit;
     d.clearGraphs();
     return MLFMA_OK;
     API_END

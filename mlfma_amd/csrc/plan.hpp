@@ -102,6 +102,11 @@ struct DevicePlan {
     bool m2lSplit = true;              // chosen by mlfma_autotune
     int nearWindow = 100;              // % of the shared M2L/near launch the pair blocks spread over; autotuned
     int nearBlocks = 0;                // near field on a side stream: blocks per CU (0: one per item); autotuned
+    // near field on a side stream: % of its work items in the launch forked
+    // at the nearOverlap point; the rest fork after the M2L (beside the L2L
+    // chain) and join before the near-field sums.  100: one launch; autotuned
+    int nearSplit = 100;
+    hipEvent_t evNearB = nullptr, evJoinB = nullptr;
     int numCU = 0;                     // compute units of the device
     // persistent tree walks (MLFMA_PARAM_TREE_WALK bits; autotuned): 1 the
     // upward pass of one right-hand side (S2M and every M2M level) as one

@@ -302,28 +302,35 @@ def test_stream_overlap_matches_one_stream():
     """The near-field pair products and the finest level's M2L beside the
     far-field chain (the plan's second and third streams, the default) give
     the bits of the in-order run (MLFMA_FLAG_ONE_STREAM): forward, batched
-    and adjoint matvecs, eager, queued on torch's stream and replayed from a
-    graph."""
+    (6 and 8 right-hand sides) and adjoint matvecs, eager, queued on torch's
+    stream and replayed from a graph, with the near field in one launch and
+    split between the fork point and the L2L."""
     import torch
     geom, kw = cases.make("rwg_adaptive")
     base = A.MLFMA_FLAG_FRESH_LOCALS
     p = Plan(geom, cases.planewave_z(), A.make_config(device=0, flags=base, **kw))
     x = torch.from_numpy(G.complex_gaussian(geom.n, 31)).to("cuda:0")
     X = torch.from_numpy(np.stack([G.complex_gaussian(geom.n, 40 + r) for r in range(6)])).to("cuda:0")
+    X8 = torch.from_numpy(np.stack([G.complex_gaussian(geom.n, 50 + r) for r in range(8)])).to("cuda:0")
     p.set_flags(base | A.MLFMA_FLAG_ONE_STREAM)
     y0, Y0, z0 = (p.matvec(x).cpu().numpy(), p.matvec(X).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy())
-    for f in (base, base | A.MLFMA_FLAG_ASYNC, base | A.MLFMA_FLAG_GRAPH):
-        p.set_flags(f)
-        y, Y = torch.empty_like(x), torch.empty_like(X)
-        for rep in range(3):
-            p.matvec(x, y)
-            p.matvec(X, Y)
-            z = p.matvec_adjoint(x)
-            torch.cuda.synchronize()
-            say("flags", f, "rep", rep)
-            assert cases.relerr(y.cpu().numpy(), y0) == 0.0
-            assert cases.relerr(Y.cpu().numpy(), Y0) == 0.0
-            assert cases.relerr(z.cpu().numpy(), z0) == 0.0
+    W0 = p.matvec(X8).cpu().numpy()
+    for near_split in (100, 50):
+        p.set_param(A.MLFMA_PARAM_NEAR_SPLIT, near_split)
+        for f in (base, base | A.MLFMA_FLAG_ASYNC, base | A.MLFMA_FLAG_GRAPH):
+            p.set_flags(f)
+            y, Y, W = torch.empty_like(x), torch.empty_like(X), torch.empty_like(X8)
+            for rep in range(3):
+                p.matvec(x, y)
+                p.matvec(X, Y)
+                p.matvec(X8, W)
+                z = p.matvec_adjoint(x)
+                torch.cuda.synchronize()
+                say("near split", near_split, "flags", f, "rep", rep)
+                assert cases.relerr(y.cpu().numpy(), y0) == 0.0
+                assert cases.relerr(Y.cpu().numpy(), Y0) == 0.0
+                assert cases.relerr(W.cpu().numpy(), W0) == 0.0
+                assert cases.relerr(z.cpu().numpy(), z0) == 0.0
 
 
 @pytest.mark.parametrize("fp32", [False, True])
@@ -331,7 +338,8 @@ def test_side_stream_schedules_match_one_stream(fp32):
     """Every schedule the autotuner chooses among (near-field pairs forked
     before S2M, after S2M or after the M2M chain; the finest level's M2L
     beside the chain or in line; the near field as one block per work item or
-    as 1-4 resident blocks per CU looping over the items) gives the result of
+    as 1-4 resident blocks per CU looping over the items, all at once or
+    split between the fork point and the L2L) gives the result of
     the in-order run, forward and adjoint, on the 30,720-RWG sphere (about
     ten thousand near-field work items, so the looping launches take many
     items per block): the same bits with one block per item; with resident
@@ -352,27 +360,36 @@ def test_side_stream_schedules_match_one_stream(fp32):
     for overlap in (1, 2, 3):
         for split in (0, 1):
             for nbk in (0, 1, 2, 4):
-                p.set_param(A.MLFMA_PARAM_NEAR_OVERLAP, overlap)
-                p.set_param(A.MLFMA_PARAM_M2L_SPLIT, split)
-                p.set_param(A.MLFMA_PARAM_NEAR_BLOCKS, nbk)
-                assert p.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == nbk
-                y1, z1 = p.matvec(x).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()
-                ey, ez = cases.relerr(y1, y0), cases.relerr(z1, z0)
-                say("fp32", fp32, "overlap", overlap, "split", split, "near blocks/CU", nbk, ey, ez)
-                if nbk == 0:
-                    assert ey == 0.0 and ez == 0.0
-                else:
-                    assert ey <= 1e-14 and ez <= 1e-14
-                    if "y" in resident:
-                        assert cases.relerr(y1, resident["y"]) == 0.0 and cases.relerr(z1, resident["z"]) == 0.0
+                # near items split between the fork point and the L2L
+                for near_split in (100, 60, 0):
+                    p.set_param(A.MLFMA_PARAM_NEAR_OVERLAP, overlap)
+                    p.set_param(A.MLFMA_PARAM_M2L_SPLIT, split)
+                    p.set_param(A.MLFMA_PARAM_NEAR_BLOCKS, nbk)
+                    p.set_param(A.MLFMA_PARAM_NEAR_SPLIT, near_split)
+                    assert p.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == nbk
+                    assert p.get_param(A.MLFMA_PARAM_NEAR_SPLIT) == near_split
+                    y1, z1 = p.matvec(x).cpu().numpy(), p.matvec_adjoint(x).cpu().numpy()
+                    ey, ez = cases.relerr(y1, y0), cases.relerr(z1, z0)
+                    say("fp32", fp32, "overlap", overlap, "split", split, "near blocks/CU", nbk, "near split",
+                        near_split, ey, ez)
+                    if nbk == 0:
+                        assert ey == 0.0 and ez == 0.0
                     else:
-                        resident = {"y": y1, "z": z1}
-    # the record carries the near-field blocks and applies to a second plan
+                        assert ey <= 1e-14 and ez <= 1e-14
+                        if "y" in resident:
+                            assert cases.relerr(y1, resident["y"]) == 0.0
+                            assert cases.relerr(z1, resident["z"]) == 0.0
+                        else:
+                            resident = {"y": y1, "z": z1}
+    with pytest.raises(Exception):
+        p.set_param(A.MLFMA_PARAM_NEAR_SPLIT, 101)
+    # the record carries the near-field blocks and split and applies to a
+    # second plan
     rec = p.tuning()
-    assert rec[0] == 3 and rec[-2] == 4
+    assert rec[0] == 4 and rec[-3] == 4 and rec[-1] == 0
     q = Plan(geom, pw, A.make_config(device=0, precision=prec, flags=flags, **kw))
     q.apply_tuning(rec)
-    assert q.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == 4
+    assert q.get_param(A.MLFMA_PARAM_NEAR_BLOCKS) == 4 and q.get_param(A.MLFMA_PARAM_NEAR_SPLIT) == 0
     assert cases.relerr(q.matvec(x).cpu().numpy(), resident["y"]) == 0.0
 
 

@@ -90,7 +90,7 @@ def test_walk_every_schedule_sphere30k(fp32):
     assert p.get_param(A.MLFMA_PARAM_TREE_STALLS) == 0
     # the record carries the choice and applies to a second plan
     rec = p.tuning()
-    assert rec[0] == 3 and rec[-1] == 1
+    assert rec[0] == 4 and rec[-2] == 1
     q = Plan(geom, pw, A.make_config(device=0, precision=prec, flags=flags, **kw))
     q.apply_tuning(rec)
     assert q.get_param(A.MLFMA_PARAM_TREE_WALK) == 1
