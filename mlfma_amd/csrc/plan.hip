@@ -1680,11 +1680,12 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
     if (nrhs > 1) d.ensureBatch(nrhs);
     hipCheck(hipMemset(d.xt, 0, (size_t)n * nrhs * sizeof(cplx<double>)), "x");
     // mean device time of events a -> b over reps matvecs after one warmup
+    // (the matvec the plan's callers run: the FP32 far field on an FP32 plan)
     auto timeStage = [&](int a, int b, int reps) {
-        d.matvec(d.xt, d.yt, nrhs, 0, d.stream, false);
+        d.matvec(d.xt, d.yt, nrhs, 0, d.stream, d.useFp32);
         double acc = 0;
         for (int r = 0; r < reps; ++r) {
-            d.matvec(d.xt, d.yt, nrhs, 0, d.stream, false);
+            d.matvec(d.xt, d.yt, nrhs, 0, d.stream, d.useFp32);
             hipCheck(hipStreamSynchronize(d.stream), "autotune");
             float t = 0;
             hipCheck(hipEventElapsedTime(&t, d.ev[a], d.ev[b]), "hipEventElapsedTime");
@@ -1696,9 +1697,9 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
     auto timeWhole = [&](int reps) {
         const uint32_t f = d.flags;
         d.flags &= ~MLFMA_FLAG_STAGE_TIMES;
-        d.matvec(d.xt, d.yt, nrhs, 0, d.stream, false);
+        d.matvec(d.xt, d.yt, nrhs, 0, d.stream, d.useFp32);
         hipCheck(hipEventRecord(d.ev[0], d.stream), "event");
-        for (int r = 0; r < reps; ++r) d.matvec(d.xt, d.yt, nrhs, 0, d.stream, false);
+        for (int r = 0; r < reps; ++r) d.matvec(d.xt, d.yt, nrhs, 0, d.stream, d.useFp32);
         hipCheck(hipEventRecord(d.ev[7], d.stream), "event");
         hipCheck(hipStreamSynchronize(d.stream), "autotune");
         d.flags = f;

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels_mv.hpp"
@@ -111,21 +112,60 @@ __global__ void kUpdate(const cplx<double>* __restrict__ Q, int64_t ld, int ncol
 // in a forest plan); basis column i at Q + i ld, w at W, partials at
 // partial + k ps, coefficients at h + k hs.  Blocks past a lane's length
 // contribute zero.
-__global__ void kDotPartialB(const cplx<double>* __restrict__ Q, int64_t ld, const cplx<double>* __restrict__ W,
-                             const int64_t* __restrict__ laneBase, const int64_t* __restrict__ laneLen,
-                             cplx<double>* __restrict__ partial, int64_t ps) {
-    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
-    const int nb = gridDim.x, i = blockIdx.y, k = blockIdx.z;
-    double re = 0, im = 0;
-    if (r < laneLen[k]) {
-        const int64_t o = laneBase[k] + r;
-        const cplx<double> wv = W[o];
-        const cplx<double> q = Q[(int64_t)i * ld + o];
-        re = q.re * wv.re + q.im * wv.im;
-        im = q.re * wv.im - q.im * wv.re;
+// Dot products of every basis column with w for several lanes at once: grid
+// (row blocks, lanes); a block reads its RB x KB rows of w once (RB rows per
+// thread, KB apart) and then those rows of every column, each column's sum
+// reduced over the wave and then over the block's waves in LDS, in a fixed
+// order: partial[k ps + i nbd + block] (nbd = gridDim.x).  The basis may be
+// FP32 (the Krylov vectors of the solves with the FP32 far field); products
+// and sums are FP64.  Dynamic LDS: ncol (KB / 64) complex.
+constexpr int RB = 4;
+template <typename TQ>
+__global__ __launch_bounds__(KB) void kDotManyB(const cplx<TQ>* __restrict__ Q, int64_t ld, int ncol,
+                                                const cplx<double>* __restrict__ W,
+                                                const int64_t* __restrict__ laneBase,
+                                                const int64_t* __restrict__ laneLen, cplx<double>* __restrict__ partial,
+                                                int64_t ps) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<double>* sh = reinterpret_cast<cplx<double>*>(smem);  // [ncol][KB / 64]
+    constexpr int NW = KB / 64;
+    const int nbd = gridDim.x, k = blockIdx.y;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t len = laneLen[k], base = laneBase[k];
+    cplx<double> w[RB];
+    int64_t off[RB];
+    bool ok[RB];
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+        const int64_t r = ((int64_t)blockIdx.x * RB + t) * KB + threadIdx.x;
+        ok[t] = r < len;
+        off[t] = base + (ok[t] ? r : 0);
+        w[t] = ok[t] ? W[off[t]] : cplx<double>{0.0, 0.0};
     }
-    blockSum2(re, im);
-    if (threadIdx.x == 0) partial[(int64_t)k * ps + (int64_t)i * nb + blockIdx.x] = cplx<double>{re, im};
+#pragma unroll 2
+    for (int i = 0; i < ncol; ++i) {
+        const cplx<TQ>* q = Q + (int64_t)i * ld;
+        double re = 0, im = 0;
+#pragma unroll
+        for (int t = 0; t < RB; ++t) {
+            const cplx<TQ> v = ok[t] ? q[off[t]] : cplx<TQ>{0, 0};
+            const double vr = (double)v.re, vi = (double)v.im;
+            re += vr * w[t].re + vi * w[t].im;
+            im += vr * w[t].im - vi * w[t].re;
+        }
+        re = waveSum(re);
+        im = waveSum(im);
+        if (lane == 0) sh[i * NW + wv] = cplx<double>{re, im};
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ncol; i += KB) {
+        double re = 0, im = 0;
+        for (int v = 0; v < NW; ++v) {
+            re += sh[i * NW + v].re;
+            im += sh[i * NW + v].im;
+        }
+        partial[(int64_t)k * ps + (int64_t)i * nbd + blockIdx.x] = cplx<double>{re, im};
+    }
 }
 
 __global__ void kReduceB(const cplx<double>* __restrict__ partial, int64_t ps, int nb, cplx<double>* __restrict__ h,
@@ -140,7 +180,8 @@ __global__ void kReduceB(const cplx<double>* __restrict__ partial, int64_t ps, i
     if (threadIdx.x == 0) h[(int64_t)k * hs + i] = cplx<double>{re, im};
 }
 
-__global__ void kUpdateB(const cplx<double>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ h,
+template <typename TQ>
+__global__ void kUpdateB(const cplx<TQ>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ h,
                          int64_t hs, cplx<double>* __restrict__ W, const int64_t* __restrict__ laneBase,
                          const int64_t* __restrict__ laneLen, cplx<double>* __restrict__ partial, int64_t ps,
                          int norm) {
@@ -151,10 +192,11 @@ __global__ void kUpdateB(const cplx<double>* __restrict__ Q, int64_t ld, int nco
         const int64_t o = laneBase[k] + r;
         cplx<double> a = W[o];
         for (int i = 0; i < ncol; ++i) {
-            const cplx<double> q = Q[(int64_t)i * ld + o];
+            const cplx<TQ> q = Q[(int64_t)i * ld + o];
+            const double qr = (double)q.re, qi = (double)q.im;
             const cplx<double> c = h[(int64_t)k * hs + i];
-            a.re -= q.re * c.re - q.im * c.im;
-            a.im -= q.re * c.im + q.im * c.re;
+            a.re -= qr * c.re - qi * c.im;
+            a.im -= qr * c.im + qi * c.re;
         }
         W[o] = a;
         s2 = a.re * a.re + a.im * a.im;
@@ -166,6 +208,17 @@ __global__ void kUpdateB(const cplx<double>* __restrict__ Q, int64_t ld, int nco
     }
 }
 
+// out = in / s for FP32 or FP64 out; widen: out = in (FP32 to FP64)
+template <typename TQ>
+__global__ void kScaleTo(const cplx<double>* __restrict__ in, double s, cplx<TQ>* __restrict__ out, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    if (r < n) out[r] = cplx<TQ>{(TQ)(in[r].re / s), (TQ)(in[r].im / s)};
+}
+__global__ void kWiden(const cplx<float>* __restrict__ in, cplx<double>* __restrict__ out, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
+    if (r < n) out[r] = cplx<double>{(double)in[r].re, (double)in[r].im};
+}
+
 // out = in / s (s real, on the host)
 __global__ void kScale(const cplx<double>* __restrict__ in, double s, cplx<double>* __restrict__ out, int64_t n) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
@@ -173,16 +226,18 @@ __global__ void kScale(const cplx<double>* __restrict__ in, double s, cplx<doubl
 }
 
 // x = (accumulate ? x : 0) + Q y
-__global__ void kCombine(const cplx<double>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ y,
+template <typename TQ>
+__global__ void kCombine(const cplx<TQ>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ y,
                          cplx<double>* __restrict__ x, int64_t n, int accumulate) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     if (r >= n) return;
     cplx<double> a = accumulate ? x[r] : cplx<double>{0, 0};
     for (int i = 0; i < ncol; ++i) {
-        const cplx<double> q = Q[(int64_t)i * ld + r];
+        const cplx<TQ> q = Q[(int64_t)i * ld + r];
+        const double qr = (double)q.re, qi = (double)q.im;
         const cplx<double> c = y[i];
-        a.re += q.re * c.re - q.im * c.im;
-        a.im += q.re * c.im + q.im * c.re;
+        a.re += qr * c.re - qi * c.im;
+        a.im += qr * c.im + qi * c.re;
     }
     x[r] = a;
 }
@@ -418,19 +473,20 @@ __global__ void kGivensB(const cplx<double>* __restrict__ hbuf, int64_t hs, int 
 }
 
 // Q_{j+1} lane k = W lane k / |w| (that lane's statistic), zero for lanes off
+template <typename TQ>
 __global__ void kScaleB(const cplx<double>* __restrict__ W, const double* __restrict__ stat, int j, int m,
-                        const int* __restrict__ on, cplx<double>* __restrict__ Qn, const int64_t* __restrict__ laneBase,
+                        const int* __restrict__ on, cplx<TQ>* __restrict__ Qn, const int64_t* __restrict__ laneBase,
                         const int64_t* __restrict__ laneLen) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const int k = blockIdx.y;
     if (r >= laneLen[k]) return;
     const int64_t o = laneBase[k] + r;
     if (!on[k]) {
-        Qn[o] = cplx<double>{0.0, 0.0};
+        Qn[o] = cplx<TQ>{0, 0};
         return;
     }
     const double v = stat[(int64_t)k * 3 * (m + 1) + 3 * j + 1];
-    Qn[o] = cplx<double>{W[o].re / v, W[o].im / v};
+    Qn[o] = cplx<TQ>{(TQ)(W[o].re / v), (TQ)(W[o].im / v)};
 }
 __global__ void kInitGB(cplx<double>* __restrict__ g, const double* __restrict__ beta, int m2, int nrhs) {
     const int i = blockIdx.x * KB + threadIdx.x;
@@ -523,13 +579,19 @@ struct Krylov {
     void scale(const cplx<double>* in, double s, cplx<double>* out) {
         hipLaunchKernelGGL(kScale, dim3(nb), dim3(KB), 0, st, in, s, out, n);
     }
-    void combine(const cplx<double>* Q, int ncol, const std::vector<std::complex<double>>& y, cplx<double>* x, int acc,
+    template <typename TQ>
+    void combine(const cplx<TQ>* Q, int ncol, const std::vector<std::complex<double>>& y, cplx<double>* x, int acc,
                  int64_t ld = 0) {
         std::vector<cplx<double>> yy(ncol);
         for (int i = 0; i < ncol; ++i) yy[i] = cplx<double>{y[i].real(), y[i].imag()};
         hipCheck(hipMemcpyAsync(h, yy.data(), ncol * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "y upload");
-        hipLaunchKernelGGL(kCombine, dim3(nb), dim3(KB), 0, st, Q, ld ? ld : n, ncol, (const cplx<double>*)h, x, n, acc);
+        hipLaunchKernelGGL(kCombine<TQ>, dim3(nb), dim3(KB), 0, st, Q, ld ? ld : n, ncol, (const cplx<double>*)h, x, n,
+                           acc);
         hipCheck(hipStreamSynchronize(st), "combine");
+    }
+    template <typename TQ>
+    void scaleTo(const cplx<double>* in, double s, cplx<TQ>* out) {
+        hipLaunchKernelGGL(kScaleTo<TQ>, dim3(nb), dim3(KB), 0, st, in, s, out, n);
     }
     void sub(const cplx<double>* b, const cplx<double>* a, cplx<double>* r) {
         hipLaunchKernelGGL(kSub, dim3(nb), dim3(KB), 0, st, b, a, r, n);
@@ -854,11 +916,12 @@ struct Lanes {
 struct BatchOrtho {
     DevicePlan& d;
     const Lanes& LN;
-    int nb;
+    int nb, nbd;  // row blocks of the update (KB rows) and of the dot products (RB KB rows)
     int64_t ps, hs;
     cplx<double>*partial = nullptr, *h = nullptr;
     BatchOrtho(DevicePlan& dp, const Lanes& lanes, int m) : d(dp), LN(lanes) {
         nb = LN.blocks();
+        nbd = (nb + RB - 1) / RB;
         ps = (int64_t)nb * (m + 3);
         hs = 2 * (int64_t)m + 3;
         partial = (cplx<double>*)d.alloc((size_t)LN.count * ps * sizeof(cplx<double>));
@@ -868,14 +931,16 @@ struct BatchOrtho {
         d.release(partial);
         d.release(h);
     }
-    void queue(const cplx<double>* Q, int64_t ld, int ncol, cplx<double>* W, cplx<double>* hhost, hipStream_t st) {
+    template <typename TQ>
+    void queue(const cplx<TQ>* Q, int64_t ld, int ncol, cplx<double>* W, cplx<double>* hhost, hipStream_t st) {
         const int nl = LN.count;
         for (int pass = 0; pass < 2; ++pass) {
-            hipLaunchKernelGGL(kDotPartialB, dim3(nb, ncol, nl), dim3(KB), 0, st, Q, ld, (const cplx<double>*)W,
-                               (const int64_t*)LN.dBase, (const int64_t*)LN.dLen, partial, ps);
-            hipLaunchKernelGGL(kReduceB, dim3(ncol, nl), dim3(KB), 0, st, (const cplx<double>*)partial, ps, nb,
+            hipLaunchKernelGGL(kDotManyB<TQ>, dim3(nbd, nl), dim3(KB), (size_t)ncol * (KB / 64) * sizeof(cplx<double>),
+                               st, Q, ld, ncol, (const cplx<double>*)W, (const int64_t*)LN.dBase,
+                               (const int64_t*)LN.dLen, partial, ps);
+            hipLaunchKernelGGL(kReduceB, dim3(ncol, nl), dim3(KB), 0, st, (const cplx<double>*)partial, ps, nbd,
                                h + (pass ? ncol + 1 : 0), hs);
-            hipLaunchKernelGGL(kUpdateB, dim3(nb, nl), dim3(KB), 0, st, Q, ld, ncol,
+            hipLaunchKernelGGL(kUpdateB<TQ>, dim3(nb, nl), dim3(KB), 0, st, Q, ld, ncol,
                                (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), hs, W, (const int64_t*)LN.dBase,
                                (const int64_t*)LN.dLen, partial + (size_t)nb * (ncol + 1), ps, pass);
         }
@@ -924,17 +989,26 @@ struct GivensBatch {
 // per lane classical Gram-Schmidt twice and unitary Givens rotations as in
 // gmresRestart.  A lane that has converged stops updating (its part of the
 // block is zero) until the others finish the cycle.
-static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
-                                        const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m, int maxit,
-                                        const std::vector<double>& tolv, double* hist, bool fp32 = false,
-                                        int histOff = 0, int histLd = 0, bool lastResidual = true) {
+// With the FP32 far field (fp32) the Krylov basis is stored in FP32 (TQ =
+// float): these solves are preconditioners or refinement steps whose
+// operator carries FP32 rounding anyway, and the basis sweeps of the
+// orthogonalization move half the bytes; each basis vector is widened to
+// FP64 for its matvec, and the products and sums stay FP64.
+template <typename TQ>
+static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
+                                         const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m, int maxit,
+                                         const std::vector<double>& tolv, double* hist, bool fp32, int histOff,
+                                         int histLd, bool lastResidual) {
+    constexpr bool narrow = std::is_same<TQ, float>::value;
     if (!histLd) histLd = maxit;
     const int64_t n = d.hp.src.n;
     const int nrhs = LN.count, nvec = LN.nvec;
     m = std::max(1, std::min(m, maxit));
     const int64_t ld = (int64_t)nvec * n;
-    cplx<double>* Q = (cplx<double>*)d.alloc((size_t)ld * (m + 1) * sizeof(cplx<double>));
+    cplx<TQ>* Q = (cplx<TQ>*)d.alloc((size_t)ld * (m + 1) * sizeof(cplx<TQ>));
     cplx<double>* W = (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>));
+    // the FP64 copy of the basis vector a step multiplies (FP32 basis)
+    cplx<double>* X64 = narrow ? (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>)) : nullptr;
     // right preconditioning (d.solvePrec): steps apply Z P, and X += P Q y
     cplx<double>* PT = d.solvePrec ? (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>)) : nullptr;
     BatchOrtho BO(d, LN, m);
@@ -982,8 +1056,8 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         for (int r = 0; r < nrhs; ++r) {
             active[r] = !done[r];
             onh[r] = active[r];
-            if (active[r]) K[r]->scale(W + LN.base[r], beta[r], Q + LN.base[r]);
-            else hipCheck(hipMemsetAsync(Q + LN.base[r], 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
+            if (active[r]) K[r]->scaleTo<TQ>(W + LN.base[r], beta[r], Q + LN.base[r]);
+            else hipCheck(hipMemsetAsync(Q + LN.base[r], 0, LN.len[r] * sizeof(cplx<TQ>), st), "zero lane");
         }
         hipCheck(hipMemcpyAsync(GB.on, onh.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, st), "lanes");
         hipCheck(hipMemcpyAsync(GB.bn, bn.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st), "norms");
@@ -995,17 +1069,24 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         bool stop = false;
         while (true) {
             while (!stop && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
+                const cplx<double>* qj;
+                if constexpr (narrow) {
+                    hipLaunchKernelGGL(kWiden, dim3((unsigned)((ld + KB - 1) / KB)), dim3(KB), 0, st,
+                                       (const cplx<float>*)(Q + (size_t)jq * ld), X64, ld);
+                    qj = X64;
+                } else
+                    qj = Q + (size_t)jq * ld;
                 if (PT) {
-                    d.applyPrecond(Q + (size_t)jq * ld, PT, nvec, n, false, st);
+                    d.applyPrecond(qj, PT, nvec, n, false, st);
                     applyMany(d, PT, W, nvec, st, fp32);
                 } else
-                    applyMany(d, Q + (size_t)jq * ld, W, nvec, st, fp32);
-                BO.queue(Q, ld, jq + 1, W, nullptr, st);
+                    applyMany(d, qj, W, nvec, st, fp32);
+                BO.queue<TQ>(Q, ld, jq + 1, W, nullptr, st);
                 hipLaunchKernelGGL(kGivensB, dim3(nrhs), dim3(64), (3 * jq + 6) * sizeof(cplx<double>), st,
                                    (const cplx<double>*)BO.h, BO.hs, jq, m,
                                    (const double*)GB.bn, (const int*)GB.on, GB.cs, GB.sn, GB.g, GB.H, GB.stat,
                                    GB.statDev);
-                hipLaunchKernelGGL(kScaleB, dim3(LN.blocks(), nrhs), dim3(KB), 0, st, (const cplx<double>*)W,
+                hipLaunchKernelGGL(kScaleB<TQ>, dim3(LN.blocks(), nrhs), dim3(KB), 0, st, (const cplx<double>*)W,
                                    (const double*)GB.stat, jq, m, (const int*)GB.on, Q + (size_t)(jq + 1) * ld,
                                    (const int64_t*)LN.dBase, (const int64_t*)LN.dLen);
                 hipCheck(hipEventRecord(GB.ev[jq % kWindow], st), "event");
@@ -1058,7 +1139,7 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
                 std::vector<zc> g(j);
                 for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
                 const std::vector<zc> y = backSolve(H, g, j);
-                K[r]->combine(Q + LN.base[r], j, y, PT ? W + LN.base[r] : X + LN.base[r], PT ? 0 : 1, ld);
+                K[r]->combine<TQ>(Q + LN.base[r], j, y, PT ? W + LN.base[r] : X + LN.base[r], PT ? 0 : 1, ld);
             } else if (PT)
                 hipCheck(hipMemsetAsync(W + LN.base[r], 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
         if (PT) {  // X += P (Q y) of every lane, through W
@@ -1072,8 +1153,18 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
         if (out[r].status == MLFMA_OK && out[r].est > tolv[r]) out[r].status = MLFMA_ERR_NOT_CONVERGED;
     d.release(Q);
     d.release(W);
+    if (X64) d.release(X64);
     if (PT) d.release(PT);
     return out;
+}
+
+static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
+                                        const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m, int maxit,
+                                        const std::vector<double>& tolv, double* hist, bool fp32 = false,
+                                        int histOff = 0, int histLd = 0, bool lastResidual = true) {
+    if (fp32)
+        return gmresBatchT<float>(d, K, st, B, X, LN, m, maxit, tolv, hist, fp32, histOff, histLd, lastResidual);
+    return gmresBatchT<double>(d, K, st, B, X, LN, m, maxit, tolv, hist, fp32, histOff, histLd, lastResidual);
 }
 
 // Batched iterative refinement (MLFMA_GMRES_IR): FP64 residuals of all
