@@ -1807,8 +1807,6 @@ void launchNearSymM8(bool conjVal, const NearItem* items, int nitems, const int3
 }
 template void launchNearSymM8<double>(bool, const NearItem*, int, const int32_t*, const int64_t*,
                                       const cplx<double>*, const cplx<double>*, cplx<double>*, hipStream_t);
-template void launchNearSymM8<float>(bool, const NearItem*, int, const int32_t*, const int64_t*, const cplx<float>*,
-                                     const cplx<double>*, cplx<double>*, hipStream_t);
 
 // xt[i 8 + k] = x[k xs + i]: eight right-hand sides, each unknown's eight
 // values in one 128-byte line

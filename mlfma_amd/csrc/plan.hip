@@ -1026,17 +1026,16 @@ void DevicePlan::apply(const cplx<double>* x, cplx<double>* y, int accumulate, h
             hipStream_t s;
             ~End() { d->probeEnd(MLFMA_GROUP_NEAR, s); }
         } end{this, s};
-        // the FP32 far field (mixed-precision inner solves) reads the FP32
-        // copy of the near-field values
-        // eight right-hand sides: the matrix-core kernel on x interleaved
+        // eight right-hand sides: the matrix-core kernel on x interleaved,
+        // with the FP64 values also under the FP32 far field (the FP32-value
+        // instance needs more registers than the three-waves-per-SIMD budget
+        // and runs slower: 0.62 against 0.44 ms per pass at 201,720 RWGs)
         if (nb == 8) {
-            if (std::is_same<R, float>::value && nearVal32)
-                launchNearSymM8<float>(false, items, cnt, nearColsIn, perm, nearVal32, xInner, nearPart,
-                                       s);
-            else
-                launchNearSymM8<double>(false, items, cnt, nearColsIn, perm, nearVal, xInner, nearPart, s);
+            launchNearSymM8<double>(false, items, cnt, nearColsIn, perm, nearVal, xInner, nearPart, s);
             return;
         }
+        // the FP32 far field (mixed-precision inner solves) reads the FP32
+        // copy of the near-field values
         // beside the chain, a fixed number of blocks (nearBlocks per CU)
         const int grid = s == nearStream ? nearGrid() : 0;
         if (std::is_same<R, float>::value && nearVal32)
