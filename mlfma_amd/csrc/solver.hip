@@ -112,24 +112,27 @@ __global__ void kUpdate(const cplx<double>* __restrict__ Q, int64_t ld, int ncol
 // in a forest plan); basis column i at Q + i ld, w at W, partials at
 // partial + k ps, coefficients at h + k hs.  Blocks past a lane's length
 // contribute zero.
-// Dot products of every basis column with w for several lanes at once: grid
-// (row blocks, lanes); a block reads its RB x KB rows of w once (RB rows per
-// thread, KB apart) and then those rows of every column, each column's sum
-// reduced over the wave and then over the block's waves in LDS, in a fixed
-// order: partial[k ps + i nbd + block] (nbd = gridDim.x).  The basis may be
-// FP32 (the Krylov vectors of the solves with the FP32 far field); products
-// and sums are FP64.  Dynamic LDS: ncol (KB / 64) complex.
+// Dot products of the basis columns with w for several lanes at once: grid
+// (row blocks, lanes, column groups of cg); a block reads its RB x KB rows
+// of w once (RB rows per thread, KB apart) and then those rows of each of
+// its columns, each column's sum reduced over the wave and then over the
+// block's waves in LDS, in a fixed order: partial[k ps + i nbd + block]
+// (nbd = gridDim.x).  The basis may be FP32 (the Krylov vectors of the
+// solves with the FP32 far field); products and sums are FP64.  Dynamic
+// LDS: cg (KB / 64) complex.
 constexpr int RB = 4;
+constexpr int kDotCols = 8;  // columns per block of a one-vector solve
 template <typename TQ>
 __global__ __launch_bounds__(KB) void kDotManyB(const cplx<TQ>* __restrict__ Q, int64_t ld, int ncol,
                                                 const cplx<double>* __restrict__ W,
                                                 const int64_t* __restrict__ laneBase,
                                                 const int64_t* __restrict__ laneLen, cplx<double>* __restrict__ partial,
-                                                int64_t ps) {
+                                                int64_t ps, int cg) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<double>* sh = reinterpret_cast<cplx<double>*>(smem);  // [ncol][KB / 64]
+    cplx<double>* sh = reinterpret_cast<cplx<double>*>(smem);  // [cg][KB / 64]
     constexpr int NW = KB / 64;
     const int nbd = gridDim.x, k = blockIdx.y;
+    const int c0 = blockIdx.z * cg, c1 = min(ncol, c0 + cg);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t len = laneLen[k], base = laneBase[k];
     cplx<double> w[RB];
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(KB) void kDotManyB(const cplx<TQ>* __restrict__ Q, 
         w[t] = ok[t] ? W[off[t]] : cplx<double>{0.0, 0.0};
     }
 #pragma unroll 2
-    for (int i = 0; i < ncol; ++i) {
+    for (int i = c0; i < c1; ++i) {
         const cplx<TQ>* q = Q + (int64_t)i * ld;
         double re = 0, im = 0;
 #pragma unroll
@@ -155,16 +158,16 @@ __global__ __launch_bounds__(KB) void kDotManyB(const cplx<TQ>* __restrict__ Q, 
         }
         re = waveSum(re);
         im = waveSum(im);
-        if (lane == 0) sh[i * NW + wv] = cplx<double>{re, im};
+        if (lane == 0) sh[(i - c0) * NW + wv] = cplx<double>{re, im};
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < ncol; i += KB) {
+    for (int i = threadIdx.x; i < c1 - c0; i += KB) {
         double re = 0, im = 0;
         for (int v = 0; v < NW; ++v) {
             re += sh[i * NW + v].re;
             im += sh[i * NW + v].im;
         }
-        partial[(int64_t)k * ps + (int64_t)i * nbd + blockIdx.x] = cplx<double>{re, im};
+        partial[(int64_t)k * ps + (int64_t)(c0 + i) * nbd + blockIdx.x] = cplx<double>{re, im};
     }
 }
 
@@ -249,33 +252,15 @@ __global__ void kSub(const cplx<double>* __restrict__ b, const cplx<double>* __r
     if (i < n) r[i] = cplx<double>{b[i].re - a[i].re, b[i].im - a[i].im};
 }
 
-// ---- CGS for the pipelined GMRES(m): wide dot products (RPT rows per
-// thread, grid (row blocks, columns)) and an update kernel that reduces the
-// dot partials itself (one wave per coefficient, fixed order), so a CGS pass
-// is two launches
-constexpr int RPT = 8;
-
-__global__ void kDotWide(const cplx<double>* __restrict__ Q, int64_t ld, const cplx<double>* __restrict__ w, int64_t n,
-                         cplx<double>* __restrict__ partial) {
-    const int nbd = gridDim.x, i = blockIdx.y;
-    double re = 0, im = 0;
-    for (int k = 0; k < RPT; ++k) {
-        const int64_t r = ((int64_t)blockIdx.x * RPT + k) * KB + threadIdx.x;
-        if (r < n) {
-            const cplx<double> wv = w[r];
-            const cplx<double> q = Q[(int64_t)i * ld + r];
-            re += q.re * wv.re + q.im * wv.im;
-            im += q.re * wv.im - q.im * wv.re;
-        }
-    }
-    blockSum2(re, im);
-    if (threadIdx.x == 0) partial[(int64_t)i * nbd + blockIdx.x] = cplx<double>{re, im};
-}
-
+// ---- CGS for the pipelined GMRES(m): the dot products of every column in
+// one launch (kDotManyB over a single lane) and an update kernel that
+// reduces the dot partials itself (one wave per coefficient, fixed order),
+// so a CGS pass is two launches
 // h_i = sum_b partial[i nbd + b] for i < ncol (block 0 also stores them to
 // hout), then w -= Q h; normPart[block] = sum |w|^2 of the block's rows when
 // norm != 0.  Dynamic LDS: ncol complex.
-__global__ void kUpdateR(const cplx<double>* __restrict__ Q, int64_t ld, int ncol,
+template <typename TQ>
+__global__ void kUpdateR(const cplx<TQ>* __restrict__ Q, int64_t ld, int ncol,
                          const cplx<double>* __restrict__ partial, int nbd, cplx<double>* __restrict__ hout,
                          cplx<double>* __restrict__ w, int64_t n, double* __restrict__ normPart, int norm) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -300,10 +285,11 @@ __global__ void kUpdateR(const cplx<double>* __restrict__ Q, int64_t ld, int nco
     if (r < n) {
         cplx<double> a = w[r];
         for (int i = 0; i < ncol; ++i) {
-            const cplx<double> q = Q[(int64_t)i * ld + r];
+            const cplx<TQ> q = Q[(int64_t)i * ld + r];
+            const double qr = (double)q.re, qi = (double)q.im;
             const cplx<double> c = sh[i];
-            a.re -= q.re * c.re - q.im * c.im;
-            a.im -= q.re * c.im + q.im * c.re;
+            a.re -= qr * c.re - qi * c.im;
+            a.im -= qr * c.im + qi * c.re;
         }
         w[r] = a;
         s2 = a.re * a.re + a.im * a.im;
@@ -332,17 +318,21 @@ __device__ __forceinline__ cplx<double> cconj(cplx<double> a) { return cplx<doub
 // CGS passes added) and the previous rotations in LDS, then thread 0 applies
 // the rotations from LDS (a chain of dependent global loads otherwise) and
 // writes the column, the new rotation and the rotated right-hand side back.
+// pass2: the second pass's coefficients when the caller holds them in LDS
+// (at sh, so each is replaced in place by the column entry), else null (at
+// hbuf + ncol + 1).
 __device__ void givensStep(const cplx<double>* __restrict__ hbuf, double s2, int j, double bn,
                            cplx<double>* __restrict__ cs, cplx<double>* __restrict__ sn, cplx<double>* __restrict__ g,
                            cplx<double>* __restrict__ H, int ldH, double* __restrict__ stat,
-                           double* __restrict__ statHost, cplx<double>* __restrict__ sh) {
+                           double* __restrict__ statHost, cplx<double>* sh, const cplx<double>* pass2 = nullptr) {
     // sh: 3 (j + 2) complex of LDS: column, cs, sn
     const int ncol = j + 1;
     cplx<double>* hc = sh;
     cplx<double>* scs = sh + ncol + 1;
     cplx<double>* ssn = scs + ncol;
     for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
-        hc[i] = cplx<double>{hbuf[i].re + hbuf[ncol + 1 + i].re, hbuf[i].im + hbuf[ncol + 1 + i].im};
+        const cplx<double> b = pass2 ? pass2[i] : hbuf[ncol + 1 + i];
+        hc[i] = cplx<double>{hbuf[i].re + b.re, hbuf[i].im + b.im};
         if (i < j) {
             scs[i] = cs[i];
             ssn[i] = sn[i];
@@ -385,13 +375,15 @@ __device__ void givensStep(const cplx<double>* __restrict__ hbuf, double s2, int
 }
 
 // The second CGS pass (kUpdateR with the norm) and the Givens step in one
-// launch: every block publishes its |w|^2 partial (and block 0 the pass-2
-// coefficients) and counts itself in; the block that counts last, which has
-// then seen every other block's release, sums the partials in the same
-// order every time and runs the step.  No block waits for another.  The
-// counter is back at zero when the kernel ends.  One device fence per block
-// (by the thread that wrote the shared data): a fence releases by writing
-// back the XCD's L2.
+// launch: every block publishes its |w|^2 partial and counts itself in; the
+// block that counts last sums the partials in the same order every time and
+// runs the step with the pass-2 coefficients it reduced into its own LDS
+// (every block forms the same ones).  No block waits for another.  The
+// counter is back at zero when the kernel ends.  The partials are the only
+// data handed between blocks: each goes to memory as an agent-scope store
+// that completes (s_waitcnt vmcnt(0)) before its block's agent-scope count,
+// and the last block reads them with agent-scope loads, so no block needs a
+// device fence (which writes back or invalidates its XCD's whole L2).
 struct GivensArgs {
     const cplx<double>* hbuf;
     int j;
@@ -402,7 +394,8 @@ struct GivensArgs {
     unsigned int* counter;
 };
 
-__global__ void kUpdateRGivens(const cplx<double>* __restrict__ Q, int64_t ld, int ncol,
+template <typename TQ>
+__global__ void kUpdateRGivens(const cplx<TQ>* __restrict__ Q, int64_t ld, int ncol,
                                const cplx<double>* __restrict__ partial, int nbd, cplx<double>* __restrict__ hout,
                                cplx<double>* __restrict__ w, int64_t n, double* __restrict__ normPart, GivensArgs ga) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -424,35 +417,36 @@ __global__ void kUpdateRGivens(const cplx<double>* __restrict__ Q, int64_t ld, i
     if (r < n) {
         cplx<double> a = w[r];
         for (int i = 0; i < ncol; ++i) {
-            const cplx<double> q = Q[(int64_t)i * ld + r];
+            const cplx<TQ> q = Q[(int64_t)i * ld + r];
+            const double qr = (double)q.re, qi = (double)q.im;
             const cplx<double> c = sh[i];
-            a.re -= q.re * c.re - q.im * c.im;
-            a.im -= q.re * c.im + q.im * c.re;
+            a.re -= qr * c.re - qi * c.im;
+            a.im -= qr * c.im + qi * c.re;
         }
         w[r] = a;
         s2 = a.re * a.re + a.im * a.im;
     }
     double z = 0;
     blockSum2(s2, z);
-    // thread 0 alone writes what other blocks read (the partial; block 0 the
-    // coefficients), releases it and counts the block in
+    // thread 0 alone writes what other blocks read (the partial) and counts
+    // the block in once the store has completed
     __shared__ unsigned int rank;
     if (threadIdx.x == 0) {
         if (blockIdx.x == 0)
             for (int i = 0; i < ncol; ++i) hout[i] = sh[i];
-        normPart[blockIdx.x] = s2;
-        __threadfence();
-        rank = atomicAdd(ga.counter, 1u);
+        __hip_atomic_store(normPart + blockIdx.x, s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
+        rank = __hip_atomic_fetch_add(ga.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     if (rank != gridDim.x - 1) return;  // uniform over the block
-    __threadfence();  // acquire: every other block's partial
     double t2 = 0, t0 = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += KB) t2 += normPart[b];
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += KB)
+        t2 += __hip_atomic_load(normPart + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     blockSum2(t2, t0);
     if (threadIdx.x == 0) *ga.counter = 0;
-    __syncthreads();  // every lane is done with the coefficients in sh
-    givensStep(ga.hbuf, t2, ga.j, ga.bn, ga.cs, ga.sn, ga.g, ga.H, ga.ldH, ga.stat, ga.statHost, sh);
+    __syncthreads();
+    givensStep(ga.hbuf, t2, ga.j, ga.bn, ga.cs, ga.sn, ga.g, ga.H, ga.ldH, ga.stat, ga.statHost, sh, sh);
 }
 
 // The Givens step for nrhs right-hand sides, one block each: lane k's CGS output at
@@ -496,11 +490,12 @@ __global__ void kInitGB(cplx<double>* __restrict__ g, const double* __restrict__
 }
 
 // out = in / *s (s on the device)
+template <typename TQ>
 __global__ void kScaleDev(const cplx<double>* __restrict__ in, const double* __restrict__ s,
-                         cplx<double>* __restrict__ out, int64_t n) {
+                         cplx<TQ>* __restrict__ out, int64_t n) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const double v = *s;
-    if (r < n) out[r] = cplx<double>{in[r].re / v, in[r].im / v};
+    if (r < n) out[r] = cplx<TQ>{(TQ)(in[r].re / v), (TQ)(in[r].im / v)};
 }
 
 // g = (beta, 0, ..., 0), m + 1 entries
@@ -698,10 +693,13 @@ struct GivensWork {
     double* normPart = nullptr;       // |w|^2 per update block
     unsigned int* counter = nullptr;  // blocks done in kUpdateRGivens (zero between launches)
     int nbd = 0, nbu = 0;
+    int64_t *lane0 = nullptr, *laneN = nullptr;  // the vector as the one lane of kDotManyB
     GivensWork(DevicePlan& dp, int m) : d(dp) {
         const int64_t n = d.hp.src.n;
         nbu = (int)((n + KB - 1) / KB);
-        nbd = (int)((n + (int64_t)KB * RPT - 1) / ((int64_t)KB * RPT));
+        nbd = (int)((n + (int64_t)KB * RB - 1) / ((int64_t)KB * RB));
+        lane0 = d.upload(std::vector<int64_t>{0});
+        laneN = d.upload(std::vector<int64_t>{n});
         dotPart = (cplx<double>*)d.alloc((size_t)nbd * (m + 1) * sizeof(cplx<double>));
         normPart = (double*)d.alloc((size_t)nbu * sizeof(double));
         counter = (unsigned int*)d.alloc(sizeof(unsigned int));
@@ -721,20 +719,26 @@ struct GivensWork {
             if (e) (void)hipEventDestroy(e);
         if (statHost) (void)hipHostFree(statHost);
         for (void* p : {(void*)cs, (void*)sn, (void*)g, (void*)H, (void*)stat, (void*)dotPart, (void*)normPart,
-                        (void*)counter})
+                        (void*)counter, (void*)lane0, (void*)laneN})
             d.release(p);
     }
 };
 
 // lastResidual = false (the inner solves of FGMRES): when maxit is reached the
 // solve ends without the residual of the final iterate (one matvec fewer)
-static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit,
-                             double tol, bool fp32, bool x0zero, double* hist, int histOff, bool lastResidual = true) {
+// With the FP32 far field (fp32) the basis is stored in FP32, as in the
+// batched solver (TQ = float; each basis vector widened to FP64 for its
+// matvec, products and sums in FP64).
+template <typename TQ>
+static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit,
+                              double tol, bool fp32, bool x0zero, double* hist, int histOff, bool lastResidual) {
+    constexpr bool narrow = std::is_same<TQ, float>::value;
     SolveOut out;
     const int64_t n = d.hp.src.n;
     m = std::max(1, std::min(m, maxit));
-    cplx<double>* Q = (cplx<double>*)d.alloc((size_t)n * (m + 1) * sizeof(cplx<double>));
+    cplx<TQ>* Q = (cplx<TQ>*)d.alloc((size_t)n * (m + 1) * sizeof(cplx<TQ>));
     cplx<double>* w = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
+    cplx<double>* x64 = narrow ? (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>)) : nullptr;
     // right preconditioning (d.solvePrec): steps apply Z P, and x += P Q y
     cplx<double>* pt = d.solvePrec ? (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>)) : nullptr;
     GivensWork G(d, m);
@@ -744,6 +748,8 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         hipCheck(hipStreamSynchronize(K.st), "sync");
         d.release(Q);
         d.release(w);
+        if (x64) d.release(x64);
+        if (pt) d.release(pt);
         out.status = bn == 0 ? MLFMA_OK : MLFMA_ERR_NONFINITE;
         return out;
     }
@@ -766,7 +772,7 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         res = beta / bn;
         out.est = res;
         if (res <= tol || total >= maxit) break;
-        K.scale(w, beta, Q);
+        K.scaleTo<TQ>(w, beta, Q);
         // Pipelined cycle: each step (matvec, two CGS passes, the Givens update
         // and the scaling of the next basis vector) is queued without a host
         // round trip; up to kWindow steps are in flight while the host reads
@@ -783,27 +789,37 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
         int uncovered = 0;
         while (true) {
             while (ncolUsed < 0 && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
+                const cplx<double>* qj;
+                if constexpr (narrow) {
+                    hipLaunchKernelGGL(kWiden, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<float>*)(Q + (size_t)jq * n),
+                                       x64, n);
+                    qj = x64;
+                } else
+                    qj = Q + (size_t)jq * n;
                 if (pt) {
-                    d.applyPrecond(Q + (size_t)jq * n, pt, 1, n, d.solveAdjoint, K.st);
+                    d.applyPrecond(qj, pt, 1, n, d.solveAdjoint, K.st);
                     d.applySolveOp(pt, w, 0, K.st, fp32, false);
                 } else
-                    d.applySolveOp(Q + (size_t)jq * n, w, 0, K.st, fp32, false);
+                    d.applySolveOp(qj, w, 0, K.st, fp32, false);
                 const int ncol = jq + 1;
                 for (int pass = 0; pass < 2; ++pass) {  // CGS2; the second pass ends with the Givens step
-                    hipLaunchKernelGGL(kDotWide, dim3(G.nbd, ncol), dim3(KB), 0, K.st, (const cplx<double>*)Q, n,
-                                       (const cplx<double>*)w, n, G.dotPart);
+                    // one vector: column groups of kDotCols fill the device
+                    hipLaunchKernelGGL(kDotManyB<TQ>, dim3(G.nbd, 1, (ncol + kDotCols - 1) / kDotCols), dim3(KB),
+                                       (size_t)kDotCols * (KB / 64) * sizeof(cplx<double>), K.st, (const cplx<TQ>*)Q,
+                                       n, ncol, (const cplx<double>*)w, (const int64_t*)G.lane0,
+                                       (const int64_t*)G.laneN, G.dotPart, (int64_t)0, kDotCols);
                     if (pass == 0)
-                        hipLaunchKernelGGL(kUpdateR, dim3(G.nbu), dim3(KB), ncol * sizeof(cplx<double>), K.st,
-                                           (const cplx<double>*)Q, n, ncol, (const cplx<double>*)G.dotPart, G.nbd,
+                        hipLaunchKernelGGL(kUpdateR<TQ>, dim3(G.nbu), dim3(KB), ncol * sizeof(cplx<double>), K.st,
+                                           (const cplx<TQ>*)Q, n, ncol, (const cplx<double>*)G.dotPart, G.nbd,
                                            K.h, w, n, G.normPart, 0);
                     else
-                        hipLaunchKernelGGL(kUpdateRGivens, dim3(G.nbu), dim3(KB), (3 * ncol + 3) * sizeof(cplx<double>),
-                                           K.st, (const cplx<double>*)Q, n, ncol, (const cplx<double>*)G.dotPart,
-                                           G.nbd, K.h + ncol + 1, w, n, G.normPart,
+                        hipLaunchKernelGGL(kUpdateRGivens<TQ>, dim3(G.nbu), dim3(KB),
+                                           (3 * ncol + 3) * sizeof(cplx<double>), K.st, (const cplx<TQ>*)Q, n, ncol,
+                                           (const cplx<double>*)G.dotPart, G.nbd, K.h + ncol + 1, w, n, G.normPart,
                                            GivensArgs{K.h, jq, bn, G.cs, G.sn, G.g, G.H, m + 1, G.stat, G.statDev,
                                                       G.counter});
                 }
-                hipLaunchKernelGGL(kScaleDev, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<double>*)w,
+                hipLaunchKernelGGL(kScaleDev<TQ>, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<double>*)w,
                                    (const double*)(G.stat + 3 * jq + 1), Q + (size_t)(jq + 1) * n, n);
                 const bool more = jq + 1 < m && total + (jq + 1 - jc) < maxit && jq + 1 - jc < kWindow;
                 if ((jq & 1) || !more) {
@@ -850,11 +866,11 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
             for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
             const std::vector<zc> y = backSolve(H, g, j);
             if (pt) {  // x += P (Q y), through w
-                K.combine(Q, j, y, w, 0);
+                K.combine<TQ>(Q, j, y, w, 0);
                 d.applyPrecond(w, pt, 1, n, d.solveAdjoint, K.st);
                 K.combine(pt, 1, std::vector<zc>{1.0}, x, 1);
             } else
-                K.combine(Q, j, y, x, 1);
+                K.combine<TQ>(Q, j, y, x, 1);
         }
         if (out.status != MLFMA_OK) break;
     }
@@ -862,8 +878,15 @@ static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cp
     if (out.status == MLFMA_OK && out.est > tol) out.status = MLFMA_ERR_NOT_CONVERGED;
     d.release(Q);
     d.release(w);
+    if (x64) d.release(x64);
     if (pt) d.release(pt);
     return out;
+}
+
+static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit,
+                             double tol, bool fp32, bool x0zero, double* hist, int histOff, bool lastResidual = true) {
+    if (fp32) return gmresRestartT<float>(d, K, b, x, m, maxit, tol, fp32, x0zero, hist, histOff, lastResidual);
+    return gmresRestartT<double>(d, K, b, x, m, maxit, tol, fp32, x0zero, hist, histOff, lastResidual);
 }
 
 // y = Z x for nrhs column vectors (stride n) through the batched kernels
@@ -937,7 +960,7 @@ struct BatchOrtho {
         for (int pass = 0; pass < 2; ++pass) {
             hipLaunchKernelGGL(kDotManyB<TQ>, dim3(nbd, nl), dim3(KB), (size_t)ncol * (KB / 64) * sizeof(cplx<double>),
                                st, Q, ld, ncol, (const cplx<double>*)W, (const int64_t*)LN.dBase,
-                               (const int64_t*)LN.dLen, partial, ps);
+                               (const int64_t*)LN.dLen, partial, ps, ncol);
             hipLaunchKernelGGL(kReduceB, dim3(ncol, nl), dim3(KB), 0, st, (const cplx<double>*)partial, ps, nbd,
                                h + (pass ? ncol + 1 : 0), hs);
             hipLaunchKernelGGL(kUpdateB<TQ>, dim3(nb, nl), dim3(KB), 0, st, Q, ld, ncol,
