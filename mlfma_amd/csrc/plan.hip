@@ -1844,11 +1844,11 @@ int mlfma_autotune(mlfma_plan* plan, int32_t nrhs, double* ms_before, double* ms
                 double tOverlap = 0, tSerial = 0;
                 try {
                     timeSolveSteps(d, 8);
-                    for (int rep = 0; rep < 2; ++rep) {
+                    for (int rep = 0; rep < 3; ++rep) {
                         d.solveInOrder = false;
-                        tOverlap += timeSolveSteps(d, 24);
+                        tOverlap += timeSolveSteps(d, 40);
                         d.solveInOrder = true;
-                        tSerial += timeSolveSteps(d, 24);
+                        tSerial += timeSolveSteps(d, 40);
                     }
                 } catch (...) {
                     d.solveInOrder = false;

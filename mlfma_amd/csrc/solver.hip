@@ -120,6 +120,10 @@ __global__ void kUpdate(const cplx<double>* __restrict__ Q, int64_t ld, int ncol
 // (nbd = gridDim.x).  The basis may be FP32 (the Krylov vectors of the
 // solves with the FP32 far field); products and sums are FP64.  Dynamic
 // LDS: cg (KB / 64) complex.
+//   wnorm: the blocks of column group 0 also write |w|^2 of their rows as the
+//          partial of column ncol (the norm before the pass);
+//   reorth: per lane, 0 when the second pass is skipped: the block writes
+//          zero partials without reading the basis.
 constexpr int RB = 4;
 constexpr int kDotCols = 8;  // columns per block of a one-vector solve
 template <typename TQ>
@@ -127,12 +131,17 @@ __global__ __launch_bounds__(KB) void kDotManyB(const cplx<TQ>* __restrict__ Q, 
                                                 const cplx<double>* __restrict__ W,
                                                 const int64_t* __restrict__ laneBase,
                                                 const int64_t* __restrict__ laneLen, cplx<double>* __restrict__ partial,
-                                                int64_t ps, int cg) {
+                                                int64_t ps, int cg, int wnorm, const int* __restrict__ reorth) {
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<double>* sh = reinterpret_cast<cplx<double>*>(smem);  // [cg][KB / 64]
     constexpr int NW = KB / 64;
     const int nbd = gridDim.x, k = blockIdx.y;
     const int c0 = blockIdx.z * cg, c1 = min(ncol, c0 + cg);
+    if (reorth && !reorth[k]) {  // uniform over the block
+        for (int i = c0 + (int)threadIdx.x; i < c1; i += KB)
+            partial[(int64_t)k * ps + (int64_t)i * nbd + blockIdx.x] = cplx<double>{0.0, 0.0};
+        return;
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t len = laneLen[k], base = laneBase[k];
     cplx<double> w[RB];
@@ -160,6 +169,15 @@ __global__ __launch_bounds__(KB) void kDotManyB(const cplx<TQ>* __restrict__ Q, 
         im = waveSum(im);
         if (lane == 0) sh[(i - c0) * NW + wv] = cplx<double>{re, im};
     }
+    __shared__ double shn[NW];
+    const bool norm = wnorm && blockIdx.z == 0;  // uniform over the block
+    if (norm) {
+        double s2 = 0;
+#pragma unroll
+        for (int t = 0; t < RB; ++t) s2 += w[t].re * w[t].re + w[t].im * w[t].im;
+        s2 = waveSum(s2);
+        if (lane == 0) shn[wv] = s2;
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < c1 - c0; i += KB) {
         double re = 0, im = 0;
@@ -168,6 +186,31 @@ __global__ __launch_bounds__(KB) void kDotManyB(const cplx<TQ>* __restrict__ Q, 
             im += sh[i * NW + v].im;
         }
         partial[(int64_t)k * ps + (int64_t)(c0 + i) * nbd + blockIdx.x] = cplx<double>{re, im};
+    }
+    if (norm && threadIdx.x == 0) {
+        double s2 = 0;
+        for (int v = 0; v < NW; ++v) s2 += shn[v];
+        partial[(int64_t)k * ps + (int64_t)ncol * nbd + blockIdx.x] = cplx<double>{s2, 0.0};
+    }
+}
+
+// Reorthogonalization test of classical Gram-Schmidt (Daniel, Gragg,
+// Kaufman and Stewart 1976): the second pass runs only when the first one
+// cancelled more than half of |w|^2, |w_after|^2 < |w_before|^2 / 2.  One
+// block per lane: |w_before|^2 from nb0 partials at before + k ps0 (stride
+// sb between partials), |w_after|^2 from nb1 doubles or complex real parts
+// at after + k ps1 (stride sa); reorth[k] = test, norm1[k] = |w_after|^2.
+__global__ void kReorthTest(const double* __restrict__ before, int64_t ps0, int nb0, int sb,
+                            const double* __restrict__ after, int64_t ps1, int nb1, int sa, int* __restrict__ reorth,
+                            double* __restrict__ norm1) {
+    const int k = blockIdx.x;
+    double t0 = 0, t1 = 0;
+    for (int b = threadIdx.x; b < nb0; b += KB) t0 += before[(int64_t)k * ps0 + (int64_t)b * sb];
+    for (int b = threadIdx.x; b < nb1; b += KB) t1 += after[(int64_t)k * ps1 + (int64_t)b * sa];
+    blockSum2(t0, t1);
+    if (threadIdx.x == 0) {
+        reorth[k] = t1 < 0.5 * t0 ? 1 : 0;
+        norm1[k] = t1;
     }
 }
 
@@ -183,15 +226,21 @@ __global__ void kReduceB(const cplx<double>* __restrict__ partial, int64_t ps, i
     if (threadIdx.x == 0) h[(int64_t)k * hs + i] = cplx<double>{re, im};
 }
 
+// reorth: per lane, 0 when the pass is skipped (w stays; its norm partials
+// are still written)
 template <typename TQ>
 __global__ void kUpdateB(const cplx<TQ>* __restrict__ Q, int64_t ld, int ncol, const cplx<double>* __restrict__ h,
                          int64_t hs, cplx<double>* __restrict__ W, const int64_t* __restrict__ laneBase,
                          const int64_t* __restrict__ laneLen, cplx<double>* __restrict__ partial, int64_t ps,
-                         int norm) {
+                         int norm, const int* __restrict__ reorth) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const int k = blockIdx.y;
+    const bool skip = reorth && !reorth[k];
     double s2 = 0;
-    if (r < laneLen[k]) {
+    if (r < laneLen[k] && skip) {
+        const cplx<double> a = W[laneBase[k] + r];
+        s2 = a.re * a.re + a.im * a.im;
+    } else if (r < laneLen[k]) {
         const int64_t o = laneBase[k] + r;
         cplx<double> a = W[o];
         for (int i = 0; i < ncol; ++i) {
@@ -397,9 +446,20 @@ struct GivensArgs {
 template <typename TQ>
 __global__ void kUpdateRGivens(const cplx<TQ>* __restrict__ Q, int64_t ld, int ncol,
                                const cplx<double>* __restrict__ partial, int nbd, cplx<double>* __restrict__ hout,
-                               cplx<double>* __restrict__ w, int64_t n, double* __restrict__ normPart, GivensArgs ga) {
+                               cplx<double>* __restrict__ w, int64_t n, double* __restrict__ normPart, GivensArgs ga,
+                               const int* __restrict__ reorth, const double* __restrict__ norm1) {
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<double>* sh = reinterpret_cast<cplx<double>*>(smem);
+    if (!*reorth) {  // uniform: the first pass was enough; w and its norm stand
+        if (blockIdx.x != 0) return;
+        for (int i = threadIdx.x; i < ncol; i += KB) {
+            sh[i] = cplx<double>{0.0, 0.0};
+            hout[i] = sh[i];
+        }
+        __syncthreads();
+        givensStep(ga.hbuf, *norm1, ga.j, ga.bn, ga.cs, ga.sn, ga.g, ga.H, ga.ldH, ga.stat, ga.statHost, sh, sh);
+        return;
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = wv; i < ncol; i += KB / 64) {
         double re = 0, im = 0;
@@ -694,12 +754,16 @@ struct GivensWork {
     unsigned int* counter = nullptr;  // blocks done in kUpdateRGivens (zero between launches)
     int nbd = 0, nbu = 0;
     int64_t *lane0 = nullptr, *laneN = nullptr;  // the vector as the one lane of kDotManyB
+    int* reorth = nullptr;                       // the second CGS pass runs (kReorthTest)
+    double* norm1 = nullptr;                     // |w|^2 after the first pass
     GivensWork(DevicePlan& dp, int m) : d(dp) {
         const int64_t n = d.hp.src.n;
         nbu = (int)((n + KB - 1) / KB);
         nbd = (int)((n + (int64_t)KB * RB - 1) / ((int64_t)KB * RB));
         lane0 = d.upload(std::vector<int64_t>{0});
         laneN = d.upload(std::vector<int64_t>{n});
+        reorth = (int*)d.alloc(sizeof(int));
+        norm1 = (double*)d.alloc(sizeof(double));
         dotPart = (cplx<double>*)d.alloc((size_t)nbd * (m + 1) * sizeof(cplx<double>));
         normPart = (double*)d.alloc((size_t)nbu * sizeof(double));
         counter = (unsigned int*)d.alloc(sizeof(unsigned int));
@@ -719,7 +783,7 @@ struct GivensWork {
             if (e) (void)hipEventDestroy(e);
         if (statHost) (void)hipHostFree(statHost);
         for (void* p : {(void*)cs, (void*)sn, (void*)g, (void*)H, (void*)stat, (void*)dotPart, (void*)normPart,
-                        (void*)counter, (void*)lane0, (void*)laneN})
+                        (void*)counter, (void*)lane0, (void*)laneN, (void*)reorth, (void*)norm1})
             d.release(p);
     }
 };
@@ -802,22 +866,30 @@ static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, c
                 } else
                     d.applySolveOp(qj, w, 0, K.st, fp32, false);
                 const int ncol = jq + 1;
-                for (int pass = 0; pass < 2; ++pass) {  // CGS2; the second pass ends with the Givens step
+                // classical Gram-Schmidt; the second pass (which ends with
+                // the Givens step) reorthogonalizes only when the first one
+                // cancelled more than half of |w|^2 (kReorthTest)
+                for (int pass = 0; pass < 2; ++pass) {
                     // one vector: column groups of kDotCols fill the device
                     hipLaunchKernelGGL(kDotManyB<TQ>, dim3(G.nbd, 1, (ncol + kDotCols - 1) / kDotCols), dim3(KB),
                                        (size_t)kDotCols * (KB / 64) * sizeof(cplx<double>), K.st, (const cplx<TQ>*)Q,
                                        n, ncol, (const cplx<double>*)w, (const int64_t*)G.lane0,
-                                       (const int64_t*)G.laneN, G.dotPart, (int64_t)0, kDotCols);
-                    if (pass == 0)
+                                       (const int64_t*)G.laneN, G.dotPart, (int64_t)0, kDotCols, pass == 0 ? 1 : 0,
+                                       pass ? (const int*)G.reorth : nullptr);
+                    if (pass == 0) {
                         hipLaunchKernelGGL(kUpdateR<TQ>, dim3(G.nbu), dim3(KB), ncol * sizeof(cplx<double>), K.st,
                                            (const cplx<TQ>*)Q, n, ncol, (const cplx<double>*)G.dotPart, G.nbd,
-                                           K.h, w, n, G.normPart, 0);
-                    else
+                                           K.h, w, n, G.normPart, 1);
+                        hipLaunchKernelGGL(kReorthTest, dim3(1), dim3(KB), 0, K.st,
+                                           (const double*)(G.dotPart + (size_t)ncol * G.nbd), (int64_t)0, G.nbd, 2,
+                                           (const double*)G.normPart, (int64_t)0, G.nbu, 1, G.reorth, G.norm1);
+                    } else
                         hipLaunchKernelGGL(kUpdateRGivens<TQ>, dim3(G.nbu), dim3(KB),
                                            (3 * ncol + 3) * sizeof(cplx<double>), K.st, (const cplx<TQ>*)Q, n, ncol,
                                            (const cplx<double>*)G.dotPart, G.nbd, K.h + ncol + 1, w, n, G.normPart,
                                            GivensArgs{K.h, jq, bn, G.cs, G.sn, G.g, G.H, m + 1, G.stat, G.statDev,
-                                                      G.counter});
+                                                      G.counter},
+                                           (const int*)G.reorth, (const double*)G.norm1);
                 }
                 hipLaunchKernelGGL(kScaleDev<TQ>, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<double>*)w,
                                    (const double*)(G.stat + 3 * jq + 1), Q + (size_t)(jq + 1) * n, n);
@@ -942,6 +1014,8 @@ struct BatchOrtho {
     int nb, nbd;  // row blocks of the update (KB rows) and of the dot products (RB KB rows)
     int64_t ps, hs;
     cplx<double>*partial = nullptr, *h = nullptr;
+    int* reorth = nullptr;     // per lane: the second CGS pass runs
+    double* norm1 = nullptr;   // per lane: |w|^2 after the first pass
     BatchOrtho(DevicePlan& dp, const Lanes& lanes, int m) : d(dp), LN(lanes) {
         nb = LN.blocks();
         nbd = (nb + RB - 1) / RB;
@@ -949,23 +1023,36 @@ struct BatchOrtho {
         hs = 2 * (int64_t)m + 3;
         partial = (cplx<double>*)d.alloc((size_t)LN.count * ps * sizeof(cplx<double>));
         h = (cplx<double>*)d.alloc((size_t)LN.count * hs * sizeof(cplx<double>));
+        reorth = (int*)d.alloc((size_t)LN.count * sizeof(int));
+        norm1 = (double*)d.alloc((size_t)LN.count * sizeof(double));
     }
     ~BatchOrtho() {
         d.release(partial);
         d.release(h);
+        d.release(reorth);
+        d.release(norm1);
     }
+    // CGS with the second pass per lane only when the first one cancelled
+    // more than half of |w|^2 (kReorthTest); a skipped pass leaves zero
+    // coefficients, so the layout of hhost is the same
     template <typename TQ>
     void queue(const cplx<TQ>* Q, int64_t ld, int ncol, cplx<double>* W, cplx<double>* hhost, hipStream_t st) {
         const int nl = LN.count;
+        cplx<double>* normPart = partial + (size_t)nb * (ncol + 1);
         for (int pass = 0; pass < 2; ++pass) {
             hipLaunchKernelGGL(kDotManyB<TQ>, dim3(nbd, nl), dim3(KB), (size_t)ncol * (KB / 64) * sizeof(cplx<double>),
                                st, Q, ld, ncol, (const cplx<double>*)W, (const int64_t*)LN.dBase,
-                               (const int64_t*)LN.dLen, partial, ps, ncol);
-            hipLaunchKernelGGL(kReduceB, dim3(ncol, nl), dim3(KB), 0, st, (const cplx<double>*)partial, ps, nbd,
-                               h + (pass ? ncol + 1 : 0), hs);
+                               (const int64_t*)LN.dLen, partial, ps, ncol, pass == 0 ? 1 : 0,
+                               pass ? (const int*)reorth : nullptr);
+            // pass 1 also sums |w_before|^2 (column ncol) into h[ncol]
+            hipLaunchKernelGGL(kReduceB, dim3(pass ? ncol : ncol + 1, nl), dim3(KB), 0, st,
+                               (const cplx<double>*)partial, ps, nbd, h + (pass ? ncol + 1 : 0), hs);
             hipLaunchKernelGGL(kUpdateB<TQ>, dim3(nb, nl), dim3(KB), 0, st, Q, ld, ncol,
                                (const cplx<double>*)(h + (pass ? ncol + 1 : 0)), hs, W, (const int64_t*)LN.dBase,
-                               (const int64_t*)LN.dLen, partial + (size_t)nb * (ncol + 1), ps, pass);
+                               (const int64_t*)LN.dLen, normPart, ps, 1, pass ? (const int*)reorth : nullptr);
+            if (pass == 0)
+                hipLaunchKernelGGL(kReorthTest, dim3(nl), dim3(KB), 0, st, (const double*)(h + ncol), 2 * hs, 1, 2,
+                                   (const double*)normPart, 2 * ps, nb, 2, reorth, norm1);
         }
         hipLaunchKernelGGL(kReduceB, dim3(1, nl), dim3(KB), 0, st, (const cplx<double>*)(partial + (size_t)nb * (ncol + 1)),
                            ps, nb, h + ncol, hs);
@@ -1484,7 +1571,9 @@ double timeSolveSteps(DevicePlan& d, int iters) {
         Krylov K(d, st, iters + 2);
         hipCheck(hipStreamSynchronize(st), "autotune");
         const auto t0 = std::chrono::steady_clock::now();
-        gmresRestart(d, K, bt, xt, iters, iters, 1e-300, false, true, nullptr, 0);
+        // the steps the plan's one-vector solves run: with the FP32 far
+        // field on an FP32 plan (the inner solves of the mixed method)
+        gmresRestart(d, K, bt, xt, iters, iters, 1e-300, d.useFp32, true, nullptr, 0);
         hipCheck(hipStreamSynchronize(st), "autotune");
         secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
