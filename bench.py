@@ -634,7 +634,7 @@ def kernel_bytes(group, info, n):
     return None, None
 
 
-KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2TN", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
+KERNEL_NAMES = {"S2M": "kS2Mx", "L2T": "kL2TS", "NEAR": "kNearSym", "M2L_TARGET": "kM2Lw", "M2L_NEAR": "kM2LNear",
                 "M2M": "kM2MFused", "L2L": "kL2LFused", "M2L_SIBLING": "kM2LGroup", "TREE_UP": "kTreeUp",
                 "M2L_RUNS": "kM2LT"}
 

@@ -2013,106 +2013,76 @@ __global__ __launch_bounds__(TB) void kL2Tx(const int64_t* __restrict__ rowPos, 
     }
 }
 
-// One right-hand side: a wave takes NR consecutive rows (rows past the end
-// are computed as the last row and not stored), so NR rows' pattern bytes are
-// in flight per round trip, and the first 64 entries of each row's near-field
-// list are loaded before the pattern stream so that their index chain
-// overlaps it.  Every row's sums are formed in the order of kL2Tx (same
-// bits).
-template <typename R, int NR>
-__global__ __launch_bounds__(TB) void kL2TN(const int64_t* __restrict__ rowPos, const int32_t* __restrict__ rowItem,
+// One right-hand side, 16 lanes per row: the wave's four rows stream their
+// patterns (two complex per lane and load: 256 contiguous bytes per row and
+// instruction in FP64, 128 in FP32, four loads in flight) against the leaf's
+// local; each row's sum and its near-field partials are reduced over its 16
+// lanes.  Two rows per wave with lanes over the directions (three rounds of
+// 64 over the leaf level's 162 directions, the last one part-idle, and a
+// 64-lane reduction per row) took 217 / 142 us (FP64 / FP32) at 201,720
+// RWGs against 204 / 111 us; eight lanes per row 209 / 110 us, eight loads
+// in flight 214 / 117 us.
+template <typename R>
+__global__ __launch_bounds__(TB) void kL2TS(const int64_t* __restrict__ rowPos, const int32_t* __restrict__ rowItem,
                                             int64_t nrows, const LeafItem* __restrict__ items,
                                             const cplx<R>* __restrict__ rp, const cplx<R>* __restrict__ Lc,
                                             cplx<double> Ck, const int64_t* __restrict__ sumPtr,
                                             const int32_t* __restrict__ sumIdx, const cplx<double>* __restrict__ part,
                                             const int64_t* __restrict__ perm, cplx<double>* __restrict__ y,
                                             int accumulate, int per) {
-    const int64_t w = NR * ((int64_t)xcdItem(blockIdx.x, per) * (TB / 64) + (threadIdx.x >> 6));
-    const int lane = threadIdx.x & 63;
-    if (w >= nrows) return;  // uniform per wave
-    int64_t t[NR], p0[NR], p1[NR];
-    const cplx<R>* r[NR];
-    const cplx<R>* L[NR];
-    int nd[NR];
-#pragma unroll
-    for (int j = 0; j < NR; ++j) {
-        const int64_t wj = w + j < nrows ? w + j : nrows - 1;
-        t[j] = rowPos[wj];
-        const LeafItem I = items[rowItem[wj]];
-        nd[j] = I.ndir;
-        r[j] = rp + I.rpOff + (t[j] - I.sbeg) * I.ndir * 2;
-        L[j] = Lc + I.fOff;
+    const int lane = threadIdx.x & 63, g = lane >> 4, s = lane & 15;
+    const int64_t w4 = 4 * ((int64_t)xcdItem(blockIdx.x, per) * (TB / 64) + (threadIdx.x >> 6));
+    if (w4 >= nrows) return;  // uniform per wave
+    const int64_t row = w4 + g;
+    const bool ok = row < nrows;
+    const int64_t rr = ok ? row : nrows - 1;
+    const int64_t t = rowPos[rr];
+    const LeafItem I = items[rowItem[rr]];
+    const int ne = 2 * I.ndir;  // complex entries of the row (even)
+    const cplx<R>* r = rp + I.rpOff + (t - I.sbeg) * ne;
+    const cplx<R>* L = Lc + I.fOff;
+    const int64_t p0 = sumPtr[t], p1 = sumPtr[t + 1];
+    double nre = 0, nim = 0;
+    for (int64_t q = p0 + s; q < p1; q += 16) {
+        const cplx<double> v = part[sumIdx[q]];
+        nre += v.re;
+        nim += v.im;
     }
+    R re = 0, im = 0;
+    constexpr int U = 4;
+    int e = 2 * s;
+    for (; e + 32 * (U - 1) < ne; e += 32 * U) {
+        cplx<R> a[U][2], m[U][2];
 #pragma unroll
-    for (int j = 0; j < NR; ++j) {
-        p0[j] = sumPtr[t[j]];
-        p1[j] = sumPtr[t[j] + 1];
-    }
-    double nre[NR], nim[NR];
-    {
-        int ix[NR];
-#pragma unroll
-        for (int j = 0; j < NR; ++j) ix[j] = p0[j] + lane < p1[j] ? sumIdx[p0[j] + lane] : -1;
-#pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            const cplx<double> v = ix[j] >= 0 ? part[ix[j]] : mk(0.0, 0.0);
-            nre[j] = v.re;
-            nim[j] = v.im;
-        }
-    }
-    R re[NR], im[NR];
-#pragma unroll
-    for (int j = 0; j < NR; ++j) re[j] = im[j] = 0;
-    bool uniform = true;  // every row at the first row's level: one loop
-#pragma unroll
-    for (int j = 1; j < NR; ++j) uniform = uniform && nd[j] == nd[0];
-    if (uniform) {
-        bool same[NR];
-#pragma unroll
-        for (int j = 0; j < NR; ++j) same[j] = L[j] == L[0];
-        for (int d = lane; d < nd[0]; d += 64) {
-            const cplx<R> l0 = L[0][2 * d], l1 = L[0][2 * d + 1];
-            cplx<R> a[NR][2], m[NR][2];
-#pragma unroll
-            for (int j = 0; j < NR; ++j) {
-                a[j][0] = r[j][2 * d];
-                a[j][1] = r[j][2 * d + 1];
-                m[j][0] = same[j] ? l0 : L[j][2 * d];
-                m[j][1] = same[j] ? l1 : L[j][2 * d + 1];
-            }
-#pragma unroll
-            for (int j = 0; j < NR; ++j) {
-                const cplx<R> u = conjmul(a[j][0], m[j][0]) + conjmul(a[j][1], m[j][1]);
-                re[j] += u.re;
-                im[j] += u.im;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < NR; ++j)
-            for (int d = lane; d < nd[j]; d += 64) {
-                const cplx<R> u = conjmul(r[j][2 * d], L[j][2 * d]) + conjmul(r[j][2 * d + 1], L[j][2 * d + 1]);
-                re[j] += u.re;
-                im[j] += u.im;
-            }
-    }
-    // near-field partials past the first 64 of a row's list, as nearFold
-#pragma unroll
-    for (int j = 0; j < NR; ++j)
-        for (int64_t q = p0[j] + 64 + lane; q < p1[j]; q += 64) {
-            const cplx<double> v = part[sumIdx[q]];
-            nre[j] += v.re;
-            nim[j] += v.im;
+        for (int u = 0; u < U; ++u) {
+            a[u][0] = r[e + 32 * u];
+            a[u][1] = r[e + 32 * u + 1];
+            m[u][0] = L[e + 32 * u];
+            m[u][1] = L[e + 32 * u + 1];
         }
 #pragma unroll
-    for (int j = 0; j < NR; ++j) {
-        const R sa = wsum(re[j]), sai = wsum(im[j]);
-        const cplx<double> nearv = mk(wsum(nre[j]), wsum(nim[j]));
-        if (lane == 0 && w + j < nrows) {
-            const int64_t o = perm[t[j]];
-            const cplx<double> v = nearv + Ck * mk((double)sa, (double)sai);
-            y[o] = accumulate ? y[o] + v : v;
+        for (int u = 0; u < U; ++u) {
+            const cplx<R> v = conjmul(a[u][0], m[u][0]) + conjmul(a[u][1], m[u][1]);
+            re += v.re;
+            im += v.im;
         }
+    }
+    for (; e < ne; e += 32) {
+        const cplx<R> v = conjmul(r[e], L[e]) + conjmul(r[e + 1], L[e + 1]);
+        re += v.re;
+        im += v.im;
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+        re += __shfl_xor(re, o, 64);
+        im += __shfl_xor(im, o, 64);
+        nre += __shfl_xor(nre, o, 64);
+        nim += __shfl_xor(nim, o, 64);
+    }
+    if (s == 0 && ok) {
+        const int64_t o = perm[t];
+        const cplx<double> v = mk(nre, nim) + Ck * mk((double)re, (double)im);
+        y[o] = accumulate ? y[o] + v : v;
     }
 }
 
@@ -2222,11 +2192,10 @@ void launchL2Tx(const int64_t* rowPos, const int32_t* rowItem, int64_t nrows, co
                 int64_t fs, int64_t xs, int64_t ps) {
     if (nrows <= 0) return;
     if (nb == 1) {
-        constexpr int NR = 2;  // four rows per wave measured slower (FP32 at 201k: 170 against 135 us)
-        const int64_t nbk = (nrows + NR * (TB / 64) - 1) / (NR * (TB / 64));
+        const int64_t nbk = (nrows + 4 * (TB / 64) - 1) / (4 * (TB / 64));
         if (nbk > INT32_MAX / 2) throw Error(MLFMA_ERR_ARG, "too many far-field rows");
         const int per = (int)((nbk + 7) / 8);
-        hipLaunchKernelGGL((kL2TN<R, NR>), dim3((unsigned)(8 * per)), dim3(TB), 0, st, rowPos, rowItem, nrows, items,
+        hipLaunchKernelGGL((kL2TS<R>), dim3((unsigned)(8 * per)), dim3(TB), 0, st, rowPos, rowItem, nrows, items,
                            rp, Lc, Ck, sumPtr, sumIdx, part, perm, y, accumulate, per);
         return;
     }
