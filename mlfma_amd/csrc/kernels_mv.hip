@@ -90,6 +90,7 @@ __device__ __forceinline__ void s2mOne(const LeafItem& it, int d0, const cplx<R>
     }
 }
 
+constexpr int S2M_XROWS = 32;  // rows per staged run of x (NB > 1)
 template <typename R, int NB>
 __global__ void kS2Mx(const LeafItem* __restrict__ items, const int32_t* __restrict__ order,
                       const cplx<R>* __restrict__ rp, const cplx<double>* __restrict__ x, int64_t xs,
@@ -100,39 +101,54 @@ __global__ void kS2Mx(const LeafItem* __restrict__ items, const int32_t* __restr
         if (blockIdx.y * blockDim.x >= it.ndir) return;  // uniform per block
         s2mOne<R>(it, blockIdx.y * blockDim.x, rp, x, perm, M);
     } else {
-        if (d >= it.ndir) return;
-        const cplx<R>* p = rp + it.rpOff + 2 * d;
+        // the x values of a run of S2M_XROWS rows for the NB right-hand sides
+        // are staged in LDS (one coalesced gather per run) and read back as
+        // broadcasts; every thread takes part in the staging (lanes past the
+        // direction count read a clamped direction and store nothing)
+        __shared__ cplx<double> sx[S2M_XROWS * NB];
+        if (blockIdx.y * blockDim.x >= it.ndir) return;  // uniform per block
+        const int dc = d < it.ndir ? d : it.ndir - 1;
+        const cplx<R>* p = rp + it.rpOff + 2 * dc;
         const int64_t stride = (int64_t)it.ndir * 2;
         cplx<R> a[NB][2];
 #pragma unroll
         for (int r = 0; r < NB; ++r) a[r][0] = a[r][1] = mk<R>(0, 0);
-        int q = 0;
-        for (; q + 1 < it.cnt; q += 2) {  // two source rows' pattern loads in flight
-            const int64_t s0 = perm[it.sbeg + q], s1 = perm[it.sbeg + q + 1];
-            const cplx<R> r00 = p[q * stride], r01 = p[q * stride + 1];
-            const cplx<R> r10 = p[(q + 1) * stride], r11 = p[(q + 1) * stride + 1];
+        for (int q0 = 0; q0 < it.cnt; q0 += S2M_XROWS) {
+            const int nq = min(S2M_XROWS, it.cnt - q0);
+            if (q0) __syncthreads();  // the previous run's reads are done
+            for (int e = threadIdx.x; e < nq * NB; e += blockDim.x) {
+                const int q = e / NB, r = e - q * NB;
+                sx[e] = x[r * xs + perm[it.sbeg + q0 + q]];
+            }
+            __syncthreads();
+            const cplx<R>* pq = p + q0 * stride;
+            int q = 0;
+            for (; q + 1 < nq; q += 2) {  // two source rows' pattern loads in flight
+                const cplx<R> r00 = pq[q * stride], r01 = pq[q * stride + 1];
+                const cplx<R> r10 = pq[(q + 1) * stride], r11 = pq[(q + 1) * stride + 1];
 #pragma unroll
-            for (int r = 0; r < NB; ++r) {
-                const cplx<R> x0 = cast<R>(x[r * xs + s0]), x1 = cast<R>(x[r * xs + s1]);
-                a[r][0] = a[r][0] + x0 * r00 + x1 * r10;
-                a[r][1] = a[r][1] + x0 * r01 + x1 * r11;
+                for (int r = 0; r < NB; ++r) {
+                    const cplx<R> x0 = cast<R>(sx[q * NB + r]), x1 = cast<R>(sx[(q + 1) * NB + r]);
+                    a[r][0] = a[r][0] + x0 * r00 + x1 * r10;
+                    a[r][1] = a[r][1] + x0 * r01 + x1 * r11;
+                }
+            }
+            if (q < nq) {
+                const cplx<R> r0 = pq[q * stride], r1 = pq[q * stride + 1];
+#pragma unroll
+                for (int r = 0; r < NB; ++r) {
+                    const cplx<R> xr = cast<R>(sx[q * NB + r]);
+                    a[r][0] = a[r][0] + xr * r0;
+                    a[r][1] = a[r][1] + xr * r1;
+                }
             }
         }
-        if (q < it.cnt) {
-            const int64_t src = perm[it.sbeg + q];
-            const cplx<R> r0 = p[q * stride], r1 = p[q * stride + 1];
+        if (d < it.ndir)
 #pragma unroll
             for (int r = 0; r < NB; ++r) {
-                const cplx<R> xr = cast<R>(x[r * xs + src]);
-                a[r][0] = a[r][0] + xr * r0;
-                a[r][1] = a[r][1] + xr * r1;
+                M[r * fs + it.fOff + 2 * d] = a[r][0];
+                M[r * fs + it.fOff + 2 * d + 1] = a[r][1];
             }
-        }
-#pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            M[r * fs + it.fOff + 2 * d] = a[r][0];
-            M[r * fs + it.fOff + 2 * d + 1] = a[r][1];
-        }
     }
 }
 
