@@ -1692,7 +1692,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void kN
     constexpr int NB = 8, RT = 4;                  // right-hand sides, row tiles per pass
     __shared__ cplx<double> sxr[128 * NB];         // x of the item's rows, interleaved
     __shared__ double sZ[TB / 64][2][16][17];      // per wave: one 16 x 16 block (re, im), for the transpose
-    __shared__ double sRed[64 * 16];               // row sums of a pass, [row][re 0-7 | im 0-7]
+    __shared__ double sRed[RT * 16 * 16];          // row sums of a pass, [row][re 0-7 | im 0-7]
     const NearItem I = items[blockIdx.x];
     const int nr = I.nr, nc = I.nc;
     const int t0 = threadIdx.x, lane = t0 & 63;
@@ -1794,9 +1794,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void kN
                         }
             __syncthreads();
         }
-        const int prow = min(64, nr - 64 * pass);
+        const int prow = min(RT * 16, nr - RT * 16 * pass);
         for (int e = t0; e < prow * 16; e += TB) {
-            const int row = 64 * pass + (e >> 4), n = e & 15;
+            const int row = RT * 16 * pass + (e >> 4), n = e & 15;
             double* o = reinterpret_cast<double*>(part + (I.rpo + row) * NB + (n & 7));
             o[n < 8 ? 0 : 1] = sRed[e];
         }
