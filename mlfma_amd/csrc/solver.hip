@@ -530,17 +530,18 @@ __global__ void kGivensB(const cplx<double>* __restrict__ hbuf, int64_t hs, int 
 template <typename TQ>
 __global__ void kScaleB(const cplx<double>* __restrict__ W, const double* __restrict__ stat, int j, int m,
                         const int* __restrict__ on, cplx<TQ>* __restrict__ Qn, const int64_t* __restrict__ laneBase,
-                        const int64_t* __restrict__ laneLen) {
+                        const int64_t* __restrict__ laneLen, cplx<double>* __restrict__ wide) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const int k = blockIdx.y;
     if (r >= laneLen[k]) return;
     const int64_t o = laneBase[k] + r;
-    if (!on[k]) {
-        Qn[o] = cplx<TQ>{0, 0};
-        return;
+    cplx<TQ> q{0, 0};
+    if (on[k]) {
+        const double v = stat[(int64_t)k * 3 * (m + 1) + 3 * j + 1];
+        q = cplx<TQ>{(TQ)(W[o].re / v), (TQ)(W[o].im / v)};
     }
-    const double v = stat[(int64_t)k * 3 * (m + 1) + 3 * j + 1];
-    Qn[o] = cplx<TQ>{(TQ)(W[o].re / v), (TQ)(W[o].im / v)};
+    Qn[o] = q;
+    if (wide) wide[o] = cplx<double>{(double)q.re, (double)q.im};
 }
 __global__ void kInitGB(cplx<double>* __restrict__ g, const double* __restrict__ beta, int m2, int nrhs) {
     const int i = blockIdx.x * KB + threadIdx.x;
@@ -550,12 +551,18 @@ __global__ void kInitGB(cplx<double>* __restrict__ g, const double* __restrict__
 }
 
 // out = in / *s (s on the device)
+// wide (FP32 basis): also the stored vector widened to FP64, the input of
+// the next step's matvec
 template <typename TQ>
 __global__ void kScaleDev(const cplx<double>* __restrict__ in, const double* __restrict__ s,
-                         cplx<TQ>* __restrict__ out, int64_t n) {
+                         cplx<TQ>* __restrict__ out, int64_t n, cplx<double>* __restrict__ wide) {
     const int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x;
     const double v = *s;
-    if (r < n) out[r] = cplx<TQ>{(TQ)(in[r].re / v), (TQ)(in[r].im / v)};
+    if (r < n) {
+        const cplx<TQ> q{(TQ)(in[r].re / v), (TQ)(in[r].im / v)};
+        out[r] = q;
+        if (wide) wide[r] = cplx<double>{(double)q.re, (double)q.im};
+    }
 }
 
 // g = (beta, 0, ..., 0), m + 1 entries
@@ -853,10 +860,12 @@ static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, c
         int uncovered = 0;
         while (true) {
             while (ncolUsed < 0 && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
+                // FP32 basis: x64 holds column jq widened (the first column's
+                // here, the others' from the scaling of the step before)
                 const cplx<double>* qj;
                 if constexpr (narrow) {
-                    hipLaunchKernelGGL(kWiden, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<float>*)(Q + (size_t)jq * n),
-                                       x64, n);
+                    if (jq == 0)
+                        hipLaunchKernelGGL(kWiden, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<float>*)Q, x64, n);
                     qj = x64;
                 } else
                     qj = Q + (size_t)jq * n;
@@ -892,7 +901,7 @@ static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, c
                                            (const int*)G.reorth, (const double*)G.norm1);
                 }
                 hipLaunchKernelGGL(kScaleDev<TQ>, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<double>*)w,
-                                   (const double*)(G.stat + 3 * jq + 1), Q + (size_t)(jq + 1) * n, n);
+                                   (const double*)(G.stat + 3 * jq + 1), Q + (size_t)(jq + 1) * n, n, x64);
                 const bool more = jq + 1 < m && total + (jq + 1 - jc) < maxit && jq + 1 - jc < kWindow;
                 if ((jq & 1) || !more) {
                     hipCheck(hipEventRecord(G.ev[jq % kWindow], K.st), "event");
@@ -1179,10 +1188,13 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
         bool stop = false;
         while (true) {
             while (!stop && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
+                // FP32 basis: X64 holds column jq widened (the first column's
+                // here, the others' from the scaling of the step before)
                 const cplx<double>* qj;
                 if constexpr (narrow) {
-                    hipLaunchKernelGGL(kWiden, dim3((unsigned)((ld + KB - 1) / KB)), dim3(KB), 0, st,
-                                       (const cplx<float>*)(Q + (size_t)jq * ld), X64, ld);
+                    if (jq == 0)
+                        hipLaunchKernelGGL(kWiden, dim3((unsigned)((ld + KB - 1) / KB)), dim3(KB), 0, st,
+                                           (const cplx<float>*)Q, X64, ld);
                     qj = X64;
                 } else
                     qj = Q + (size_t)jq * ld;
@@ -1198,7 +1210,7 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
                                    GB.statDev);
                 hipLaunchKernelGGL(kScaleB<TQ>, dim3(LN.blocks(), nrhs), dim3(KB), 0, st, (const cplx<double>*)W,
                                    (const double*)GB.stat, jq, m, (const int*)GB.on, Q + (size_t)(jq + 1) * ld,
-                                   (const int64_t*)LN.dBase, (const int64_t*)LN.dLen);
+                                   (const int64_t*)LN.dBase, (const int64_t*)LN.dLen, X64);
                 hipCheck(hipEventRecord(GB.ev[jq % kWindow], st), "event");
                 ++jq;
             }
