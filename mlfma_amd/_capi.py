@@ -89,7 +89,8 @@ class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("restart", C.c_int32), ("maxit", C.c_int32),
                 ("use_voltage", C.c_int32), ("tol", C.c_double), ("outer_restart", C.c_int32),
                 ("precond", C.c_int32), ("inner_tol", C.c_double),
-                ("shift_re", C.c_double), ("shift_im", C.c_double)]
+                ("shift_re", C.c_double), ("shift_im", C.c_double),
+                ("deflate", C.c_int32)]
 
 
 class SolveReport(C.Structure):

@@ -12,6 +12,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "dense.hpp"
 #include "kernels_mv.hpp"
 #include "plan.hpp"
 
@@ -550,6 +551,37 @@ __global__ void kInitGB(cplx<double>* __restrict__ g, const double* __restrict__
     g[i] = cplx<double>{q == 0 ? beta[k] : 0.0, 0.0};
 }
 
+// out column c = sum_j Q column j * P(j, c) for c < nout (P column-major,
+// leading dimension ldp, FP64): the basis recombination of a deflated
+// restart.  One thread per row, eight output columns at a time; sums in FP64.
+template <typename TQ>
+__global__ void kRecombine(const cplx<TQ>* __restrict__ Q, int64_t ld, int nin, const cplx<double>* __restrict__ P,
+                           int ldp, int nout, cplx<TQ>* __restrict__ out, int64_t ldo, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c0 = 0; c0 < nout; c0 += 8) {
+        const int cn = min(8, nout - c0);
+        double ar[8] = {}, ai[8] = {};
+        for (int j = 0; j < nin; ++j) {
+            const cplx<TQ> q = Q[(int64_t)j * ld + i];
+            const double qr = (double)q.re, qi = (double)q.im;
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (c < cn) {
+                    const cplx<double> p = P[(int64_t)(c0 + c) * ldp + j];
+                    ar[c] += qr * p.re - qi * p.im;
+                    ai[c] += qr * p.im + qi * p.re;
+                }
+        }
+        for (int c = 0; c < cn; ++c) {
+            cplx<TQ> v;
+            v.re = (TQ)ar[c];
+            v.im = (TQ)ai[c];
+            out[(int64_t)(c0 + c) * ldo + i] = v;
+        }
+    }
+}
+
 // out = in / *s (s on the device)
 // wide (FP32 basis): also the stored vector widened to FP64, the input of
 // the next step's matvec
@@ -800,27 +832,43 @@ struct GivensWork {
 // With the FP32 far field (fp32) the basis is stored in FP32, as in the
 // batched solver (TQ = float; each basis vector widened to FP64 for its
 // matvec, products and sums in FP64).
+// deflate = k > 0: deflated restarts (GMRES-DR, dense.hpp): a cycle that runs
+// all m steps without meeting the tolerance hands its k harmonic Ritz vectors
+// of smallest modulus and its residual direction to the next cycle, which
+// starts at step k with its first k Hessenberg columns already triangular
+// (identity rotations) and its first k inputs Zk kept on the device.  The
+// stopping test still uses the true residual at every restart; a restart
+// whose small dense problem fails (or a cycle that ended early) restarts
+// plainly from the residual.
 template <typename TQ>
 static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit,
-                              double tol, bool fp32, bool x0zero, double* hist, int histOff, bool lastResidual) {
+                              double tol, bool fp32, bool x0zero, double* hist, int histOff, bool lastResidual,
+                              int deflate) {
     constexpr bool narrow = std::is_same<TQ, float>::value;
     SolveOut out;
     const int64_t n = d.hp.src.n;
     m = std::max(1, std::min(m, maxit));
+    const int kd = deflate > 0 && deflate + 1 < m ? deflate : 0;
     cplx<TQ>* Q = (cplx<TQ>*)d.alloc((size_t)n * (m + 1) * sizeof(cplx<TQ>));
     cplx<double>* w = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
     cplx<double>* x64 = narrow ? (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>)) : nullptr;
     // right preconditioning (d.solvePrec): steps apply Z P, and x += P Q y
     cplx<double>* pt = d.solvePrec ? (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>)) : nullptr;
+    // deflation: the kept inputs, the next cycle's first basis vectors
+    // (written beside Q, then copied in) and the two small matrices
+    cplx<TQ>* Zk = kd ? (cplx<TQ>*)d.alloc((size_t)n * kd * sizeof(cplx<TQ>)) : nullptr;
+    cplx<TQ>* Vn = kd ? (cplx<TQ>*)d.alloc((size_t)n * (kd + 1) * sizeof(cplx<TQ>)) : nullptr;
+    cplx<double>* dP = kd ? (cplx<double>*)d.alloc((size_t)(2 * m + 1) * (kd + 1) * sizeof(cplx<double>)) : nullptr;
+    auto releaseAll = [&]() {
+        for (void* p : {(void*)Q, (void*)w, (void*)x64, (void*)pt, (void*)Zk, (void*)Vn, (void*)dP})
+            if (p) d.release(p);
+    };
     GivensWork G(d, m);
     const double bn = K.norm(b);
     if (!(bn > 0) || !std::isfinite(bn)) {
         hipCheck(hipMemsetAsync(x, 0, n * sizeof(cplx<double>), K.st), "zero x");
         hipCheck(hipStreamSynchronize(K.st), "sync");
-        d.release(Q);
-        d.release(w);
-        if (x64) d.release(x64);
-        if (pt) d.release(pt);
+        releaseAll();
         out.status = bn == 0 ? MLFMA_OK : MLFMA_ERR_NONFINITE;
         return out;
     }
@@ -828,6 +876,7 @@ static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, c
     int total = 0;
     double res = 1.0;
     bool cycleMet = false;  // the last cycle stopped on its residual estimate
+    dense::DrState dr;      // the deflated start of the next cycle (dr.k > 0)
     for (;;) {
         if (!lastResidual && (total >= maxit || cycleMet)) break;
         // r = b - A x
@@ -843,7 +892,29 @@ static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, c
         res = beta / bn;
         out.est = res;
         if (res <= tol || total >= maxit) break;
-        K.scaleTo<TQ>(w, beta, Q);
+        // k0: the cycle's first step (the deflated columns before it)
+        const int k0 = dr.k;
+        if (k0 > 0) {
+            // Q[0..k0] and Zk hold the deflated block (written at the
+            // restart); its triangular columns, identity rotations and the
+            // rotated right-hand side go to the Givens state
+            std::vector<cplx<double>> Hh((size_t)k0 * (m + 1), cplx<double>{0, 0}), gh(m + 2, cplx<double>{0, 0});
+            std::vector<cplx<double>> ch(k0, cplx<double>{1, 0}), sh(k0, cplx<double>{0, 0});
+            for (int c = 0; c < k0; ++c)
+                for (int i = 0; i <= c; ++i)
+                    Hh[(size_t)c * (m + 1) + i] = cplx<double>{dr.Rk(i, c).real(), dr.Rk(i, c).imag()};
+            for (int i = 0; i <= k0; ++i) gh[i] = cplx<double>{dr.g[i].real(), dr.g[i].imag()};
+            hipCheck(hipMemcpyAsync(G.H, Hh.data(), Hh.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, K.st),
+                     "H upload");
+            hipCheck(hipMemcpyAsync(G.g, gh.data(), gh.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, K.st),
+                     "g upload");
+            hipCheck(hipMemcpyAsync(G.cs, ch.data(), k0 * sizeof(cplx<double>), hipMemcpyHostToDevice, K.st),
+                     "cs upload");
+            hipCheck(hipMemcpyAsync(G.sn, sh.data(), k0 * sizeof(cplx<double>), hipMemcpyHostToDevice, K.st),
+                     "sn upload");
+            hipCheck(hipStreamSynchronize(K.st), "deflated start");
+        } else
+            K.scaleTo<TQ>(w, beta, Q);
         // Pipelined cycle: each step (matvec, two CGS passes, the Givens update
         // and the scaling of the next basis vector) is queued without a host
         // round trip; up to kWindow steps are in flight while the host reads
@@ -851,21 +922,23 @@ static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, c
         // queueing at the first one that meets the stopping test.  Steps queued
         // beyond it only append columns, so the solution uses exactly the
         // columns the synchronous algorithm would.
-        hipLaunchKernelGGL(kInitG, dim3((m + 1 + KB - 1) / KB), dim3(KB), 0, K.st, G.g, beta, m + 1);
-        int jq = 0, jc = 0, ncolUsed = -1;
+        if (k0 == 0)
+            hipLaunchKernelGGL(kInitG, dim3((m + 1 + KB - 1) / KB), dim3(KB), 0, K.st, G.g, beta, m + 1);
+        int jq = k0, jc = k0, ncolUsed = -1;
         // each event record costs the stream a few microseconds, so one event
         // covers two steps (and the last step queued before the host waits);
         // cover[s % kWindow]: the event slot that covers step s
         std::array<int, kWindow> cover{};
-        int uncovered = 0;
+        int uncovered = k0;
         while (true) {
             while (ncolUsed < 0 && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
                 // FP32 basis: x64 holds column jq widened (the first column's
                 // here, the others' from the scaling of the step before)
                 const cplx<double>* qj;
                 if constexpr (narrow) {
-                    if (jq == 0)
-                        hipLaunchKernelGGL(kWiden, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<float>*)Q, x64, n);
+                    if (jq == k0)
+                        hipLaunchKernelGGL(kWiden, dim3(K.nb), dim3(KB), 0, K.st,
+                                           (const cplx<float>*)(Q + (size_t)k0 * n), x64, n);
                     qj = x64;
                 } else
                     qj = Q + (size_t)jq * n;
@@ -934,8 +1007,8 @@ static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, c
         if (ncolUsed < 0) ncolUsed = jc;
         hipCheck(hipStreamSynchronize(K.st), "gmres cycle");  // steps queued past the stop
         const int j = ncolUsed;
+        std::vector<cplx<double>> Hh((size_t)j * (m + 1)), gh(j + 1);
         if (j > 0) {
-            std::vector<cplx<double>> Hh((size_t)j * (m + 1)), gh(j);
             hipCheck(hipMemcpy(Hh.data(), G.H, Hh.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost), "H copy");
             hipCheck(hipMemcpy(gh.data(), G.g, gh.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost), "g copy");
             std::vector<std::vector<zc>> H(j);
@@ -946,28 +1019,74 @@ static SolveOut gmresRestartT(DevicePlan& d, Krylov& K, const cplx<double>* b, c
             std::vector<zc> g(j);
             for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
             const std::vector<zc> y = backSolve(H, g, j);
-            if (pt) {  // x += P (Q y), through w
-                K.combine<TQ>(Q, j, y, w, 0);
+            // dst (+)= Z y: the kept inputs Zk for the first k0 columns, the
+            // basis vectors after them
+            auto addTo = [&](cplx<double>* dst, int acc) {
+                if (k0 > 0) {
+                    K.combine<TQ>(Zk, k0, std::vector<zc>(y.begin(), y.begin() + k0), dst, acc);
+                    if (j > k0)
+                        K.combine<TQ>(Q + (size_t)k0 * n, j - k0, std::vector<zc>(y.begin() + k0, y.end()), dst, 1);
+                } else
+                    K.combine<TQ>(Q, j, y, dst, acc);
+            };
+            if (pt) {  // x += P (Z y), through w
+                addTo(w, 0);
                 d.applyPrecond(w, pt, 1, n, d.solveAdjoint, K.st);
                 K.combine(pt, 1, std::vector<zc>{1.0}, x, 1);
             } else
-                K.combine<TQ>(Q, j, y, x, 1);
+                addTo(x, 1);
         }
         if (out.status != MLFMA_OK) break;
+        // deflated restart after a full cycle that did not meet the tolerance
+        dense::DrState nd;
+        bool kept = false;
+        if (kd && j == m && !cycleMet && total < maxit) {
+            std::vector<cplx<double>> ch(m), sh(m);
+            hipCheck(hipMemcpy(ch.data(), G.cs, m * sizeof(cplx<double>), hipMemcpyDeviceToHost), "cs copy");
+            hipCheck(hipMemcpy(sh.data(), G.sn, m * sizeof(cplx<double>), hipMemcpyDeviceToHost), "sn copy");
+            dense::Mat R(m, m);
+            std::vector<zc> cs(m), sn(m);
+            for (int c = 0; c < m; ++c) {
+                for (int i = 0; i <= c; ++i) R(i, c) = zc(Hh[(size_t)c * (m + 1) + i].re, Hh[(size_t)c * (m + 1) + i].im);
+                cs[c] = zc(ch[c].re, ch[c].im);
+                sn[c] = zc(sh[c].re, sh[c].im);
+            }
+            kept = dense::drRestart(R, cs, sn, zc(gh[m].re, gh[m].im), k0, dr.Qhat, kd, nd) && nd.defect < 1e-8;
+        }
+        if (kept) {
+            // Zk = Q[0..m) Pk, then Q[0..kd] = Q[0..m] U (through Vn)
+            std::vector<cplx<double>> ph((size_t)m * kd + (size_t)(m + 1) * (kd + 1));
+            for (int c = 0; c < kd; ++c)
+                for (int i = 0; i < m; ++i) ph[(size_t)c * m + i] = cplx<double>{nd.Pk(i, c).real(), nd.Pk(i, c).imag()};
+            cplx<double>* pu = ph.data() + (size_t)m * kd;
+            for (int c = 0; c <= kd; ++c)
+                for (int i = 0; i <= m; ++i)
+                    pu[(size_t)c * (m + 1) + i] = cplx<double>{nd.U(i, c).real(), nd.U(i, c).imag()};
+            hipCheck(hipMemcpyAsync(dP, ph.data(), ph.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, K.st),
+                     "restart upload");
+            hipLaunchKernelGGL(kRecombine<TQ>, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<TQ>*)Q, n, m,
+                               (const cplx<double>*)dP, m, kd, Zk, n, n);
+            hipLaunchKernelGGL(kRecombine<TQ>, dim3(K.nb), dim3(KB), 0, K.st, (const cplx<TQ>*)Q, n, m + 1,
+                               (const cplx<double>*)(dP + (size_t)m * kd), m + 1, kd + 1, Vn, n, n);
+            hipCheck(hipMemcpyAsync(Q, Vn, (size_t)n * (kd + 1) * sizeof(cplx<TQ>), hipMemcpyDeviceToDevice, K.st),
+                     "restart basis");
+            hipCheck(hipStreamSynchronize(K.st), "deflated restart");
+            dr = std::move(nd);
+        } else
+            dr = dense::DrState{};
     }
     out.iters = total;
     if (out.status == MLFMA_OK && out.est > tol) out.status = MLFMA_ERR_NOT_CONVERGED;
-    d.release(Q);
-    d.release(w);
-    if (x64) d.release(x64);
-    if (pt) d.release(pt);
+    releaseAll();
     return out;
 }
 
 static SolveOut gmresRestart(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit,
-                             double tol, bool fp32, bool x0zero, double* hist, int histOff, bool lastResidual = true) {
-    if (fp32) return gmresRestartT<float>(d, K, b, x, m, maxit, tol, fp32, x0zero, hist, histOff, lastResidual);
-    return gmresRestartT<double>(d, K, b, x, m, maxit, tol, fp32, x0zero, hist, histOff, lastResidual);
+                             double tol, bool fp32, bool x0zero, double* hist, int histOff, bool lastResidual = true,
+                             int deflate = 0) {
+    if (fp32)
+        return gmresRestartT<float>(d, K, b, x, m, maxit, tol, fp32, x0zero, hist, histOff, lastResidual, deflate);
+    return gmresRestartT<double>(d, K, b, x, m, maxit, tol, fp32, x0zero, hist, histOff, lastResidual, deflate);
 }
 
 // y = Z x for nrhs column vectors (stride n) through the batched kernels
@@ -1524,7 +1643,7 @@ static std::vector<SolveOut> fgmres(DevicePlan& d, std::vector<std::unique_ptr<K
 // (FP64 matvec) around restarted inner GMRES(m) solves whose far field is
 // FP32, at most 20 corrections; no outer Krylov space is kept.
 static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<double>* x, int m, int maxit, double tol,
-                        double* hist) {
+                        double* hist, int deflate) {
     SolveOut out;
     const int64_t n = d.hp.src.n;
     cplx<double>* r = (cplx<double>*)d.alloc((size_t)n * sizeof(cplx<double>));
@@ -1548,7 +1667,7 @@ static SolveOut gmresIR(DevicePlan& d, Krylov& K, const cplx<double>* b, cplx<do
         out.est = res;
         if (res <= tol || total >= maxit || outer == 20) break;
         const double itol = std::max(inner, 0.5 * tol / res);
-        SolveOut s = gmresRestart(d, K, r, dx, m, maxit - total, itol, true, true, hist, total);
+        SolveOut s = gmresRestart(d, K, r, dx, m, maxit - total, itol, true, true, hist, total, true, deflate);
         total += s.iters;
         // x += dx
         std::vector<zc> one{1.0};
@@ -1633,6 +1752,11 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
     if (opts->maxit < 1 || !(opts->tol > 0)) throw Error(MLFMA_ERR_ARG, "maxit must be >= 1 and tol > 0");
     if (opts->restart > 2048) throw Error(MLFMA_ERR_ARG, "restart must be <= 2048");
     const int method = opts->method & 0xff;
+    if (opts->deflate < 0 || opts->deflate > 64) throw Error(MLFMA_ERR_ARG, "deflate must be in [0, 64]");
+    if (opts->deflate && method != MLFMA_GMRES_RESTART && method != MLFMA_GMRES_IR)
+        throw Error(MLFMA_ERR_ARG, "deflated restarts are for MLFMA_GMRES_RESTART and MLFMA_GMRES_IR");
+    if (opts->deflate && opts->deflate + 1 >= (opts->restart > 0 ? opts->restart : 50))
+        throw Error(MLFMA_ERR_ARG, "deflate must be smaller than restart - 1");
     struct AdjointMode {  // cleared however the solve ends
         DevicePlan& d;
         ~AdjointMode() {
@@ -1700,7 +1824,7 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
         o = gmresCompat(d, K, bt, xtree, opts->maxit, opts->tol, hist);
     else if (method == MLFMA_GMRES_RESTART)
         o = gmresRestart(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, false, true,
-                         hist, 0);
+                         hist, 0, true, opts->deflate);
     else if (method == MLFMA_FGMRES_IR) {
         std::vector<std::unique_ptr<Krylov>> Kv;
         Kv.emplace_back(new Krylov(d, st, std::max(opts->restart > 0 ? opts->restart : 50, mOut) + 2));
@@ -1708,7 +1832,8 @@ int mlfma_solve(mlfma_plan* plan, const double* b, double* x, const mlfma_solve_
         o = fgmres(d, Kv, st, bt, xtree, one, mOut, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol,
                    opts->inner_tol > 0 ? opts->inner_tol : 1e-1, hist, opts->maxit)[0];
     } else if (method == MLFMA_GMRES_IR)
-        o = gmresIR(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, hist);
+        o = gmresIR(d, K, bt, xtree, opts->restart > 0 ? opts->restart : 50, opts->maxit, opts->tol, hist,
+                    opts->deflate);
     else
         throw Error(MLFMA_ERR_ARG, "unknown solve method");
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1785,6 +1910,8 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     API_BEGIN
     DevicePlan& d = Pl(plan);
     if (!b || !x || !opts || nrhs < 1) throw Error(MLFMA_ERR_ARG, "bad batch solve arguments");
+    if (opts->deflate != 0)
+        throw Error(MLFMA_ERR_ARG, "deflated restarts are built for one right-hand side (mlfma_solve)");
     if (opts->method != MLFMA_GMRES_RESTART && opts->method != MLFMA_FGMRES_IR && opts->method != MLFMA_GMRES_IR)
         throw Error(MLFMA_ERR_ARG, "batched solves use MLFMA_GMRES_RESTART, MLFMA_FGMRES_IR or MLFMA_GMRES_IR");
     if (opts->method != MLFMA_GMRES_RESTART && !d.useFp32)

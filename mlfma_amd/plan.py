@@ -248,12 +248,14 @@ class Plan:
 
     # ---------------------------------------------------------------- solve
     def solve(self, b=None, method=A.MLFMA_GMRES_COMPAT, restart=50, maxit=500, tol=1e-6, history=False,
-              adjoint=False, outer_restart=0, inner_tol=0.0, precond=0, shift=0j):
+              adjoint=False, outer_restart=0, inner_tol=0.0, precond=0, shift=0j, deflate=0):
         """GMRES for Z x = b (b = None: the plane-wave excitation -V), or for
         Z^H x = b with adjoint=True.  MLFMA_FGMRES_IR (FP32 plans): flexible
         GMRES(outer_restart) in FP64 around inner FP32-far-field GMRES solves of
-        at most `restart` iterations to a relative inner_tol.  Returns (x,
-        report); report["iterations"] counts matvecs."""
+        at most `restart` iterations to a relative inner_tol.  deflate = k > 0
+        (MLFMA_GMRES_RESTART, MLFMA_GMRES_IR): deflated restarts keeping k
+        harmonic Ritz vectors (GMRES-DR).  Returns (x, report);
+        report["iterations"] counts matvecs."""
         o = A.SolveOpts()
         o.method = method | (A.MLFMA_SOLVE_ADJOINT if adjoint else 0)
         o.restart = restart
@@ -264,6 +266,7 @@ class Plan:
         o.inner_tol = inner_tol
         o.precond = precond
         o.shift_re, o.shift_im = complex(shift).real, complex(shift).imag
+        o.deflate = deflate
         rep = A.SolveReport()
         hist = np.zeros(maxit)
         if _is_cuda(b):
@@ -298,7 +301,7 @@ class Plan:
         return v
 
     def solve_batch(self, B, restart=50, maxit=500, tol=1e-6, history=False, method=A.MLFMA_GMRES_RESTART,
-                    outer_restart=0, inner_tol=0.0, precond=0, shift=0j):
+                    outer_restart=0, inner_tol=0.0, precond=0, shift=0j, deflate=0):
         """GMRES(restart) on the rows of B (nrhs, n) at once, one batched
         matvec per step (method MLFMA_FGMRES_IR on an FP32 plan: FP64
         flexible GMRES around batched FP32-far-field GMRES; MLFMA_GMRES_IR:
@@ -320,6 +323,7 @@ class Plan:
         o.inner_tol = inner_tol
         o.precond = precond
         o.shift_re, o.shift_im = complex(shift).real, complex(shift).imag
+        o.deflate = deflate  # deflated restarts: single right-hand side only (rejected here)
         reps = (A.SolveReport * (nrhs * nseg))()
         X = np.zeros_like(B)
         hist = np.zeros((nrhs * nseg, maxit))
