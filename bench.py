@@ -156,6 +156,9 @@ def solves(geom, pw, kw, desc):
     maxit = 6000
     I64, s64 = q64.solve(method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=1e-6)
     log("physical FP64", s64)
+    # deflated restarts: 20 harmonic Ritz vectors kept per restart
+    Idr, sdr = q64.solve(method=A.MLFMA_GMRES_RESTART, restart=restart, maxit=maxit, tol=1e-6, deflate=20)
+    log("physical FP64 GMRES-DR", sdr)
     mixed = dict(method=A.MLFMA_FGMRES_IR, restart=30, outer_restart=30, inner_tol=0.1)
     I32, s32 = q32.solve(maxit=maxit, tol=1e-6, **mixed)
     log("physical mixed", s32)
@@ -175,6 +178,9 @@ def solves(geom, pw, kw, desc):
     mie = np.concatenate([se, sh])
     db = 10 * np.log10(rcs(q64, I64)) - 10 * np.log10(mie)
     single = {"fp64": {k: s64[k] for k in keys}, "mixed": {k: s32[k] for k in keys},
+              "fp64_deflated": dict({k: sdr[k] for k in keys}, deflate=20,
+                                    speedup_vs_fp64=round(s64["seconds"] / sdr["seconds"], 3),
+                                    solution_rel_diff_vs_fp64=float(np.linalg.norm(Idr - I64) / np.linalg.norm(I64))),
               "mixed_speedup": round(s64["seconds"] / s32["seconds"], 3),
               "mixed_rcs_rel_diff_vs_fp64": rcs_diff(rcs(q32, I32), rcs(q64, I64)),
               "mixed_solution_rel_diff_vs_fp64": float(np.linalg.norm(I32 - I64) / np.linalg.norm(I64)),
@@ -203,7 +209,8 @@ def solves(geom, pw, kw, desc):
              "iterations_fp64": [r["iterations"] for _, r in seq], "iterations_mixed": [r["iterations"] for r in rm]}
     out["physical"] = {"near_field": "MLFMA_FLAG_PHYSICAL (closed-form singular integrals)",
                        "gmres": f"FP64 GMRES({restart}); mixed: FP64 FGMRES(30) around FP32-far-field GMRES(30) "
-                                f"to 1e-1 per step; tol 1e-6", "plan_setup_s": round(setup, 3),
+                                f"to 1e-1 per step; deflated: FP64 GMRES-DR({restart}, 20); tol 1e-6",
+                       "plan_setup_s": round(setup, 3),
                        "single_incidence": single, "incidence_sweep": sweep}
     return out
 
@@ -507,6 +514,62 @@ def c4_stalls_job():
                                                    "sweep": ellipsoid}}), flush=True)
 
 
+def deflation_job(args):
+    """Deflated restarts (GMRES-DR) on the physical near field: the workload
+    sphere under theta = 0 incidence, FP64 GMRES(m) against GMRES-DR(m, k)
+    for a few (m, k), the mixed solvers (FP64 FGMRES around FP32-far-field
+    GMRES; FP64 refinement around FP32-far-field GMRES-DR), then the seed-1
+    c4 ellipsoids under theta = 0 in FP64 GMRES(100) and GMRES-DR(100, 20),
+    tol 1e-6.  Iterations count matvecs of the Krylov loops; seconds are the
+    solver's wall time (the host's small dense problem of every restart
+    included)."""
+    from mlfma_amd import _capi as A
+    from mlfma_amd import configs as CF
+    from mlfma_amd.plan import Plan
+    geom, pw, kw, desc = workload(args.workload)
+    fl = A.MLFMA_FLAG_PHYSICAL | A.MLFMA_FLAG_FRESH_LOCALS
+    keys = ["iterations", "status", "resid_true", "seconds"]
+    out = {"mode": "deflation", "workload": desc["workload"], "unknowns": int(geom.n), "tol": 1e-6, "solves": {}}
+    q64 = Plan(geom, pw, A.make_config(device=0, flags=fl, **kw))
+    q64.autotune(1)
+    q32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, flags=fl, **kw))
+    q32.autotune(1)
+    runs = [("fp64_gmres100", q64, dict(method=A.MLFMA_GMRES_RESTART, restart=100)),
+            ("fp64_gmres_dr100_20", q64, dict(method=A.MLFMA_GMRES_RESTART, restart=100, deflate=20)),
+            ("fp64_gmres_dr60_15", q64, dict(method=A.MLFMA_GMRES_RESTART, restart=60, deflate=15)),
+            ("fp64_gmres_dr100_30", q64, dict(method=A.MLFMA_GMRES_RESTART, restart=100, deflate=30)),
+            ("fp64_gmres_dr40_10", q64, dict(method=A.MLFMA_GMRES_RESTART, restart=40, deflate=10)),
+            ("mixed_fgmres30_30", q32, dict(method=A.MLFMA_FGMRES_IR, restart=30, outer_restart=30, inner_tol=0.1)),
+            ("mixed_ir_dr100_20", q32, dict(method=A.MLFMA_GMRES_IR, restart=100, deflate=20)),
+            ("mixed_ir_dr60_15", q32, dict(method=A.MLFMA_GMRES_IR, restart=60, deflate=15))]
+    ref = None
+    for name, p, o in runs:
+        x, r = p.solve(maxit=6000, tol=1e-6, **o)
+        if ref is None:
+            ref = x
+        e = {k: r[k] for k in keys}
+        e["solution_rel_diff_vs_fp64_gmres"] = float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+        out["solves"][name] = e
+        log(name, e)
+    del q64, q32
+    pw0 = CF.planewave_xz(0.0)
+    ells = {}
+    for i, sh in enumerate(CF.c4()):
+        if sh["kind"] != "ellipsoid":
+            continue
+        p = Plan(sh["geom"], pw0, A.make_config(device=0, flags=fl, **sh["kw"]))
+        b = -p.excitation(pw0)
+        e = {"r_eq": round(sh["r_eq"], 4), "unknowns": sh["n"]}
+        for name, o in [("gmres100", dict(restart=100)), ("gmres_dr100_20", dict(restart=100, deflate=20))]:
+            _, r = p.solve(b, method=A.MLFMA_GMRES_RESTART, maxit=6000, tol=1e-6, **o)
+            e[name] = {k: r[k] for k in keys}
+        p.close()
+        log("c4 ellipsoid", i, e)
+        ells[str(i)] = e
+    out["c4_ellipsoids_theta0_fp64"] = ells
+    print(json.dumps(out), flush=True)
+
+
 def m2l_ceiling(args):
     """Cache-resident gather ceiling for M2L: per level, each target sums the
     fields of its interaction-list sources (32 B per direction) and the
@@ -660,12 +723,15 @@ def main():
     ap.add_argument("--fp32", action="store_true",
                     help="with --timed-only: the plan's far field in FP32 (the inner matvec of the "
                          "mixed-precision solves) instead of FP64")
-    ap.add_argument("--mode", default="matvec", choices=["matvec", "copy", "m2l-ceiling", "hbm", "c4-stalls"],
+    ap.add_argument("--mode", default="matvec", choices=["matvec", "copy", "m2l-ceiling", "hbm", "c4-stalls",
+                                                              "deflation"],
                     help="copy: only run a plain 256 MiB device copy (HBM counter calibration); m2l-ceiling: the "
                          "M2L access pattern (per target, the sum of its sources' fields and operators) through "
                          "torch's gather-sum (embedding_bag) on the workload's interaction lists; hbm: torch's "
                          "copy and sum rates on GiB arrays; c4-stalls: the stalled c4 ellipsoids against finer "
-                         "rules and meshes and 1-2% size changes, and dense singular values over size sweeps")
+                         "rules and meshes and 1-2% size changes, and dense singular values over size sweeps; "
+                         "deflation: GMRES(m) against deflated restarts (GMRES-DR) on the physical near field of "
+                         "the workload and of the c4 ellipsoids")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the library runs the matvec on one GPU and has no multi-GPU "
@@ -697,6 +763,10 @@ def main():
 
     if args.mode == "c4-stalls":
         c4_stalls_job()
+        return
+
+    if args.mode == "deflation":
+        deflation_job(args)
         return
 
     if args.workload == "dipoles200k_grad":
