@@ -267,11 +267,11 @@ typedef struct mlfma_solve_opts {
                              the reference drops (src/sources/dipole.h:65)          */
     int32_t deflate;      /* k > 0: deflated restarts (GMRES-DR, Morgan 2002) for
                              MLFMA_GMRES_RESTART and the inner solves of
-                             MLFMA_GMRES_IR: each restart keeps the k harmonic
-                             Ritz vectors of smallest modulus (k + 1 < restart,
-                             k <= 64; single right-hand side only).  0: plain
-                             restarts.  No reference counterpart (the
-                             reference's GMRES never restarts)                     */
+                             MLFMA_GMRES_IR, single and batched: each restart
+                             keeps the k harmonic Ritz vectors of smallest
+                             modulus (k + 1 < restart, k <= 64; not with
+                             MLFMA_FGMRES_IR).  0: plain restarts.  No reference
+                             counterpart (the reference's GMRES never restarts) */
 } mlfma_solve_opts;
 
 typedef struct mlfma_solve_report {

@@ -3,12 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -1222,6 +1224,23 @@ struct GivensBatch {
     }
 };
 
+// f(i) for i in [0, count) on up to 16 host threads
+template <typename F>
+static void parallelFor(int count, F f) {
+    const int nt = std::max(1, std::min({count, 16, (int)std::thread::hardware_concurrency()}));
+    if (nt <= 1) {
+        for (int i = 0; i < count; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&]() {
+            for (int i = next++; i < count; i = next++) f(i);
+        });
+    for (auto& t : th) t.join();
+}
+
 // GMRES(m) on every lane in lockstep: one batched matvec per step over the
 // block of the vectors' current Krylov columns (basis stored [step][vector][n]),
 // per lane classical Gram-Schmidt twice and unitary Givens rotations as in
@@ -1232,11 +1251,16 @@ struct GivensBatch {
 // operator carries FP32 rounding anyway, and the basis sweeps of the
 // orthogonalization move half the bytes; each basis vector is widened to
 // FP64 for its matvec, and the products and sums stay FP64.
+// deflate = k > 0: deflated restarts per lane as in gmresRestartT (the host
+// problems of the lanes on parallel threads, the recombination launched per
+// lane).  The lanes stay in lockstep: a cycle starts at step k only when
+// every unfinished lane holds a deflated start, otherwise every lane
+// restarts plainly from its residual.
 template <typename TQ>
 static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                          const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m, int maxit,
                                          const std::vector<double>& tolv, double* hist, bool fp32, int histOff,
-                                         int histLd, bool lastResidual) {
+                                         int histLd, bool lastResidual, int deflate) {
     constexpr bool narrow = std::is_same<TQ, float>::value;
     if (!histLd) histLd = maxit;
     const int64_t n = d.hp.src.n;
@@ -1251,6 +1275,12 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
     cplx<double>* PT = d.solvePrec ? (cplx<double>*)d.alloc((size_t)ld * sizeof(cplx<double>)) : nullptr;
     BatchOrtho BO(d, LN, m);
     GivensBatch GB(d, nrhs, m);
+    const int kd = deflate > 0 && deflate + 1 < m ? deflate : 0;
+    const size_t pSz = (size_t)m * kd + (size_t)(m + 1) * (kd + 1);  // per lane: Pk, then U
+    cplx<TQ>* Zk = kd ? (cplx<TQ>*)d.alloc((size_t)ld * kd * sizeof(cplx<TQ>)) : nullptr;
+    cplx<TQ>* Vn = kd ? (cplx<TQ>*)d.alloc((size_t)ld * (kd + 1) * sizeof(cplx<TQ>)) : nullptr;
+    cplx<double>* dP = kd ? (cplx<double>*)d.alloc(pSz * nrhs * sizeof(cplx<double>)) : nullptr;
+    std::vector<dense::DrState> dr(nrhs);
     std::vector<SolveOut> out(nrhs);
     std::vector<double> bn(nrhs);
     std::vector<char> done(nrhs, 0);
@@ -1281,6 +1311,13 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
             all = all && done[r];
         }
         if (all || total >= maxit) break;
+        // the cycle's first step: k when every unfinished lane has a deflated
+        // start, else 0 (every lane restarts plainly)
+        int k0 = kd;
+        for (int r = 0; r < nrhs; ++r)
+            if (!done[r] && dr[r].k != kd) k0 = 0;
+        if (k0 == 0)
+            for (auto& e : dr) e = dense::DrState{};
         // Pipelined cycle, as in gmresRestart: every step (batched matvec,
         // CGS2 of all lanes, per-lane Givens update, scaling) is queued without
         // a host round trip, up to kWindow steps ahead of the host, which reads
@@ -1294,16 +1331,48 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
         for (int r = 0; r < nrhs; ++r) {
             active[r] = !done[r];
             onh[r] = active[r];
-            if (active[r]) K[r]->scaleTo<TQ>(W + LN.base[r], beta[r], Q + LN.base[r]);
-            else hipCheck(hipMemsetAsync(Q + LN.base[r], 0, LN.len[r] * sizeof(cplx<TQ>), st), "zero lane");
+            if (k0 == 0) {
+                if (active[r]) K[r]->scaleTo<TQ>(W + LN.base[r], beta[r], Q + LN.base[r]);
+                else hipCheck(hipMemsetAsync(Q + LN.base[r], 0, LN.len[r] * sizeof(cplx<TQ>), st), "zero lane");
+            } else if (!active[r])
+                for (int c = 0; c <= k0; ++c)
+                    hipCheck(hipMemsetAsync(Q + (size_t)c * ld + LN.base[r], 0, LN.len[r] * sizeof(cplx<TQ>), st),
+                             "zero lane");
         }
         hipCheck(hipMemcpyAsync(GB.on, onh.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, st), "lanes");
         hipCheck(hipMemcpyAsync(GB.bn, bn.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st), "norms");
         hipCheck(hipMemcpyAsync(GB.beta, beta.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st), "betas");
         hipLaunchKernelGGL(kInitGB, dim3(((m + 2) * nrhs + KB - 1) / KB), dim3(KB), 0, st, GB.g, (const double*)GB.beta,
                            m + 2, nrhs);
+        std::vector<cplx<double>> hH, hg, hc, hs;  // deflated starts (alive until the sync below)
+        if (k0 > 0) {
+            // per active lane: its triangular columns, identity rotations and
+            // rotated right-hand side (Q[0..k0] and Zk hold its block)
+            hH.assign((size_t)nrhs * (m + 1) * (m + 1), cplx<double>{0, 0});
+            hg.assign((size_t)nrhs * (m + 2), cplx<double>{0, 0});
+            hc.assign((size_t)nrhs * (m + 1), cplx<double>{1, 0});
+            hs.assign((size_t)nrhs * (m + 1), cplx<double>{0, 0});
+            for (int r = 0; r < nrhs; ++r) {
+                if (!active[r]) continue;
+                cplx<double>* Hr = hH.data() + (size_t)r * (m + 1) * (m + 1);
+                for (int c = 0; c < k0; ++c)
+                    for (int i = 0; i <= c; ++i)
+                        Hr[(size_t)c * (m + 1) + i] = cplx<double>{dr[r].Rk(i, c).real(), dr[r].Rk(i, c).imag()};
+                for (int i = 0; i <= k0; ++i)
+                    hg[(size_t)r * (m + 2) + i] = cplx<double>{dr[r].g[i].real(), dr[r].g[i].imag()};
+            }
+            hipCheck(hipMemcpyAsync(GB.H, hH.data(), hH.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
+                     "H upload");
+            hipCheck(hipMemcpyAsync(GB.g, hg.data(), hg.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
+                     "g upload");
+            hipCheck(hipMemcpyAsync(GB.cs, hc.data(), hc.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
+                     "cs upload");
+            hipCheck(hipMemcpyAsync(GB.sn, hs.data(), hs.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
+                     "sn upload");
+            hipCheck(hipStreamSynchronize(st), "deflated start");
+        }
         bool breakdown = false;
-        int jq = 0, jc = 0;
+        int jq = k0, jc = k0;
         bool stop = false;
         while (true) {
             while (!stop && jq < m && total + (jq - jc) < maxit && jq - jc < kWindow) {
@@ -1311,9 +1380,9 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
                 // here, the others' from the scaling of the step before)
                 const cplx<double>* qj;
                 if constexpr (narrow) {
-                    if (jq == 0)
+                    if (jq == k0)
                         hipLaunchKernelGGL(kWiden, dim3((unsigned)((ld + KB - 1) / KB)), dim3(KB), 0, st,
-                                           (const cplx<float>*)Q, X64, ld);
+                                           (const cplx<float>*)(Q + (size_t)k0 * ld), X64, ld);
                     qj = X64;
                 } else
                     qj = Q + (size_t)jq * ld;
@@ -1361,10 +1430,18 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
             if (stop) break;  // steps queued past the stop are not read
         }
         hipCheck(hipStreamSynchronize(st), "gmres cycle");
+        // lanes that ran the whole cycle without meeting their tolerance
+        // restart deflated: their Givens state, read with the update below
+        std::vector<dense::Mat> Rl(nrhs);
+        std::vector<std::vector<zc>> csl(nrhs), snl(nrhs);
+        std::vector<zc> gml(nrhs);
+        std::vector<char> want(nrhs, 0);
+        for (int r = 0; r < nrhs; ++r)
+            want[r] = kd && steps[r] == m && active[r] && !breakdown && total < maxit;
         for (int r = 0; r < nrhs; ++r)
             if (steps[r] > 0) {
                 const int j = steps[r];
-                std::vector<cplx<double>> Hh((size_t)j * (m + 1)), gh(j);
+                std::vector<cplx<double>> Hh((size_t)j * (m + 1)), gh(j + 1);
                 hipCheck(hipMemcpy(Hh.data(), GB.H + (size_t)r * (m + 1) * (m + 1), Hh.size() * sizeof(cplx<double>),
                                    hipMemcpyDeviceToHost),
                          "H copy");
@@ -1380,7 +1457,31 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
                 std::vector<zc> g(j);
                 for (int i = 0; i < j; ++i) g[i] = zc(gh[i].re, gh[i].im);
                 const std::vector<zc> y = backSolve(H, g, j);
-                K[r]->combine<TQ>(Q + LN.base[r], j, y, PT ? W + LN.base[r] : X + LN.base[r], PT ? 0 : 1, ld);
+                cplx<double>* dst = PT ? W + LN.base[r] : X + LN.base[r];
+                if (k0 > 0) {  // the first k0 inputs are the kept vectors Zk
+                    K[r]->combine<TQ>(Zk + LN.base[r], k0, std::vector<zc>(y.begin(), y.begin() + k0), dst,
+                                      PT ? 0 : 1, ld);
+                    if (j > k0)
+                        K[r]->combine<TQ>(Q + (size_t)k0 * ld + LN.base[r], j - k0,
+                                          std::vector<zc>(y.begin() + k0, y.end()), dst, 1, ld);
+                } else
+                    K[r]->combine<TQ>(Q + LN.base[r], j, y, dst, PT ? 0 : 1, ld);
+                if (want[r]) {
+                    std::vector<cplx<double>> ch(m), sh(m);
+                    hipCheck(hipMemcpy(ch.data(), GB.cs + (size_t)r * (m + 1), m * sizeof(cplx<double>),
+                                       hipMemcpyDeviceToHost), "cs copy");
+                    hipCheck(hipMemcpy(sh.data(), GB.sn + (size_t)r * (m + 1), m * sizeof(cplx<double>),
+                                       hipMemcpyDeviceToHost), "sn copy");
+                    Rl[r] = dense::Mat(m, m);
+                    csl[r].resize(m);
+                    snl[r].resize(m);
+                    for (int c = 0; c < m; ++c) {
+                        for (int i = 0; i <= c; ++i) Rl[r](i, c) = H[c][i];
+                        csl[r][c] = zc(ch[c].re, ch[c].im);
+                        snl[r][c] = zc(sh[c].re, sh[c].im);
+                    }
+                    gml[r] = zc(gh[m].re, gh[m].im);
+                }
             } else if (PT)
                 hipCheck(hipMemsetAsync(W + LN.base[r], 0, LN.len[r] * sizeof(cplx<double>), st), "zero lane");
         if (PT) {  // X += P (Q y) of every lane, through W
@@ -1389,23 +1490,75 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
                 if (steps[r] > 0) K[r]->combine(PT + LN.base[r], 1, std::vector<zc>{1.0}, X + LN.base[r], 1);
         }
         if (breakdown) break;
+        if (kd) {
+            // the lanes' restart problems on host threads, then per kept lane
+            // Zk = Q[0..m) Pk and Q[0..kd] = Q[0..m] U (through Vn)
+            std::vector<dense::DrState> nd(nrhs);
+            std::vector<char> kept(nrhs, 0);
+            parallelFor(nrhs, [&](int r) {
+                if (want[r])
+                    kept[r] = dense::drRestart(Rl[r], csl[r], snl[r], gml[r], k0, dr[r].Qhat, kd, nd[r]) &&
+                              nd[r].defect < 1e-8;
+            });
+            std::vector<cplx<double>> ph(pSz * nrhs);
+            bool any = false;
+            for (int r = 0; r < nrhs; ++r) {
+                if (!kept[r]) {
+                    dr[r] = dense::DrState{};
+                    continue;
+                }
+                any = true;
+                cplx<double>* pk = ph.data() + pSz * r;
+                cplx<double>* pu = pk + (size_t)m * kd;
+                for (int c = 0; c < kd; ++c)
+                    for (int i = 0; i < m; ++i)
+                        pk[(size_t)c * m + i] = cplx<double>{nd[r].Pk(i, c).real(), nd[r].Pk(i, c).imag()};
+                for (int c = 0; c <= kd; ++c)
+                    for (int i = 0; i <= m; ++i)
+                        pu[(size_t)c * (m + 1) + i] = cplx<double>{nd[r].U(i, c).real(), nd[r].U(i, c).imag()};
+            }
+            if (any) {
+                hipCheck(hipMemcpyAsync(dP, ph.data(), ph.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
+                         "restart upload");
+                for (int r = 0; r < nrhs; ++r) {
+                    if (!kept[r]) continue;
+                    const int64_t len = LN.len[r];
+                    const dim3 g((unsigned)((len + KB - 1) / KB));
+                    const cplx<double>* pk = dP + pSz * r;
+                    hipLaunchKernelGGL(kRecombine<TQ>, g, dim3(KB), 0, st, (const cplx<TQ>*)(Q + LN.base[r]), ld, m,
+                                       pk, m, kd, Zk + LN.base[r], ld, len);
+                    hipLaunchKernelGGL(kRecombine<TQ>, g, dim3(KB), 0, st, (const cplx<TQ>*)(Q + LN.base[r]), ld,
+                                       m + 1, pk + (size_t)m * kd, m + 1, kd + 1, Vn + LN.base[r], ld, len);
+                }
+                for (int r = 0; r < nrhs; ++r) {
+                    if (!kept[r]) continue;
+                    for (int c = 0; c <= kd; ++c)
+                        hipCheck(hipMemcpyAsync(Q + (size_t)c * ld + LN.base[r], Vn + (size_t)c * ld + LN.base[r],
+                                                LN.len[r] * sizeof(cplx<TQ>), hipMemcpyDeviceToDevice, st),
+                                 "restart basis");
+                    dr[r] = std::move(nd[r]);
+                }
+                hipCheck(hipStreamSynchronize(st), "deflated restart");
+            }
+        }
     }
     for (int r = 0; r < nrhs; ++r)
         if (out[r].status == MLFMA_OK && out[r].est > tolv[r]) out[r].status = MLFMA_ERR_NOT_CONVERGED;
-    d.release(Q);
-    d.release(W);
-    if (X64) d.release(X64);
-    if (PT) d.release(PT);
+    for (void* p : {(void*)Q, (void*)W, (void*)X64, (void*)PT, (void*)Zk, (void*)Vn, (void*)dP})
+        if (p) d.release(p);
     return out;
 }
 
 static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                         const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m, int maxit,
                                         const std::vector<double>& tolv, double* hist, bool fp32 = false,
-                                        int histOff = 0, int histLd = 0, bool lastResidual = true) {
+                                        int histOff = 0, int histLd = 0, bool lastResidual = true,
+                                        int deflate = 0) {
     if (fp32)
-        return gmresBatchT<float>(d, K, st, B, X, LN, m, maxit, tolv, hist, fp32, histOff, histLd, lastResidual);
-    return gmresBatchT<double>(d, K, st, B, X, LN, m, maxit, tolv, hist, fp32, histOff, histLd, lastResidual);
+        return gmresBatchT<float>(d, K, st, B, X, LN, m, maxit, tolv, hist, fp32, histOff, histLd, lastResidual,
+                                  deflate);
+    return gmresBatchT<double>(d, K, st, B, X, LN, m, maxit, tolv, hist, fp32, histOff, histLd, lastResidual,
+                               deflate);
 }
 
 // Batched iterative refinement (MLFMA_GMRES_IR): FP64 residuals of all
@@ -1414,7 +1567,7 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
 // inner tolerance is max(1e-4, tol / (2 residual)).
 static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                           const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m,
-                                          int maxit, double tol, double* hist) {
+                                          int maxit, double tol, double* hist, int deflate) {
     const int64_t n = d.hp.src.n;
     const int nrhs = LN.count, nvec = LN.nvec;
     const int64_t ld = (int64_t)nvec * n;
@@ -1446,7 +1599,8 @@ static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique
             itol[r] = std::max(1e-4, 0.5 * tol / res);
         }
         if (all || total >= maxit || outer == 30) break;
-        std::vector<SolveOut> in = gmresBatch(d, K, st, R, D, LN, m, maxit - total, itol, hist, true, total, maxit);
+        std::vector<SolveOut> in =
+            gmresBatch(d, K, st, R, D, LN, m, maxit - total, itol, hist, true, total, maxit, true, deflate);
         int steps = 0;
         for (int r = 0; r < nrhs; ++r) {
             out[r].iters += in[r].iters;
@@ -1910,8 +2064,11 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
     API_BEGIN
     DevicePlan& d = Pl(plan);
     if (!b || !x || !opts || nrhs < 1) throw Error(MLFMA_ERR_ARG, "bad batch solve arguments");
-    if (opts->deflate != 0)
-        throw Error(MLFMA_ERR_ARG, "deflated restarts are built for one right-hand side (mlfma_solve)");
+    if (opts->deflate < 0 || opts->deflate > 64) throw Error(MLFMA_ERR_ARG, "deflate must be in [0, 64]");
+    if (opts->deflate && opts->method == MLFMA_FGMRES_IR)
+        throw Error(MLFMA_ERR_ARG, "deflated restarts are for MLFMA_GMRES_RESTART and MLFMA_GMRES_IR");
+    if (opts->deflate && opts->deflate + 1 >= (opts->restart > 0 ? opts->restart : 50))
+        throw Error(MLFMA_ERR_ARG, "deflate must be smaller than restart - 1");
     if (opts->method != MLFMA_GMRES_RESTART && opts->method != MLFMA_FGMRES_IR && opts->method != MLFMA_GMRES_IR)
         throw Error(MLFMA_ERR_ARG, "batched solves use MLFMA_GMRES_RESTART, MLFMA_FGMRES_IR or MLFMA_GMRES_IR");
     if (opts->method != MLFMA_GMRES_RESTART && !d.useFp32)
@@ -1959,8 +2116,9 @@ int mlfma_solve_batch(mlfma_plan* plan, const double* b, double* x, int32_t nrhs
             ? fgmres(d, K, st, B, X, LN, mOut, m, opts->maxit, opts->tol,
                      opts->inner_tol > 0 ? opts->inner_tol : 1e-1, hist, opts->maxit)
         : opts->method == MLFMA_GMRES_IR
-            ? gmresBatchIR(d, K, st, B, X, LN, m, opts->maxit, opts->tol, hist)
-            : gmresBatch(d, K, st, B, X, LN, m, opts->maxit, std::vector<double>(LN.count, opts->tol), hist);
+            ? gmresBatchIR(d, K, st, B, X, LN, m, opts->maxit, opts->tol, hist, opts->deflate)
+            : gmresBatch(d, K, st, B, X, LN, m, opts->maxit, std::vector<double>(LN.count, opts->tol), hist, false,
+                         0, 0, true, opts->deflate);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // true residuals with the FP64 matvec, per lane
     applyMany(d, X, R, nrhs, st);

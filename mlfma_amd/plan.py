@@ -323,7 +323,7 @@ class Plan:
         o.inner_tol = inner_tol
         o.precond = precond
         o.shift_re, o.shift_im = complex(shift).real, complex(shift).imag
-        o.deflate = deflate  # deflated restarts: single right-hand side only (rejected here)
+        o.deflate = deflate  # deflated restarts (GMRES-DR) per lane; not with MLFMA_FGMRES_IR
         reps = (A.SolveReport * (nrhs * nseg))()
         X = np.zeros_like(B)
         hist = np.zeros((nrhs * nseg, maxit))

@@ -9,7 +9,8 @@ residual direction appended.  The device keeps the same subspaces in another
 basis (a triangular leading block, so its Givens pipeline runs unchanged):
 iteration counts, residual histories and solutions must agree.  Then the
 FP32-basis path (iterative refinement), right preconditioning, the adjoint,
-argument errors, and a resonant scatterer where plain restarts stall."""
+argument errors, the batched solver (lanes in lockstep) against single
+solves, and a resonant scatterer where plain restarts stall."""
 import numpy as np
 import pytest
 
@@ -139,6 +140,27 @@ def test_deflated_fp32_basis_preconditioned_and_adjoint():
     assert cases.relerr(x32, xd) <= 1e-6
 
 
+def test_batched_deflation_matches_single_solves():
+    geom, kw, p, Z = _sphere_plan()
+    n = geom.n
+    B = np.stack([G.complex_gaussian(n, 20 + r) for r in range(3)])
+    X, reps = p.solve_batch(B, restart=20, maxit=4000, tol=1e-7, deflate=5)
+    for r in range(3):
+        x, rep = p.solve(B[r], method=A.MLFMA_GMRES_RESTART, restart=20, maxit=4000, tol=1e-7, deflate=5)
+        print("batched lane", r, reps[r]["iterations"], "single", rep["iterations"], flush=True)
+        assert reps[r]["status"] == "OK" and reps[r]["resid_true"] <= 1e-7
+        assert abs(reps[r]["iterations"] - rep["iterations"]) <= 2
+        assert cases.relerr(X[r], x) <= 1e-6
+        assert cases.relerr(X[r], np.linalg.solve(Z, B[r])) <= 1e-5
+    p32 = Plan(geom, cases.planewave_z(), A.make_config(
+        device=0, precision=A.MLFMA_FP32, **dict(kw, flags=A.MLFMA_FLAG_PHYSICAL | A.MLFMA_FLAG_FRESH_LOCALS)))
+    X32, r32 = p32.solve_batch(B, method=A.MLFMA_GMRES_IR, restart=20, maxit=6000, tol=1e-9, deflate=6)
+    print("batched IR + DR", [r["iterations"] for r in r32], flush=True)
+    for r in range(3):
+        assert r32[r]["status"] == "OK" and r32[r]["resid_true"] <= 1e-9
+        assert cases.relerr(X32[r], np.linalg.solve(Z, B[r])) <= 1e-6
+
+
 def test_deflation_argument_errors():
     geom, kw = cases.make("rwg_fixed")
     p = Plan(geom, cases.planewave_z(), A.make_config(device=0, **dict(kw, flags=A.MLFMA_FLAG_FRESH_LOCALS)))
@@ -148,8 +170,10 @@ def test_deflation_argument_errors():
             p.solve(b, method=A.MLFMA_GMRES_RESTART, maxit=50, **bad)
     with pytest.raises(Exception):
         p.solve(b, method=A.MLFMA_GMRES_COMPAT, maxit=50, deflate=4)
+    with pytest.raises(Exception):  # the flexible outer keeps plain restarts
+        p.solve_batch(np.stack([b, b]), restart=20, maxit=50, deflate=4, method=A.MLFMA_FGMRES_IR)
     with pytest.raises(Exception):
-        p.solve_batch(np.stack([b, b]), restart=20, maxit=50, deflate=4)
+        p.solve_batch(np.stack([b, b]), restart=10, maxit=50, deflate=9)
     # still usable after the refusals
     x, rep = p.solve(b, method=A.MLFMA_GMRES_RESTART, restart=20, maxit=20, tol=1e-3, deflate=4)
     assert np.isfinite(rep["resid_true"])
