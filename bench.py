@@ -519,17 +519,32 @@ def deflation_job(args):
     sphere under theta = 0 incidence, FP64 GMRES(m) against GMRES-DR(m, k)
     for a few (m, k), the mixed solvers (FP64 FGMRES around FP32-far-field
     GMRES; FP64 refinement around FP32-far-field GMRES-DR), then the seed-1
-    c4 ellipsoids under theta = 0 in FP64 GMRES(100) and GMRES-DR(100, 20),
-    tol 1e-6.  Iterations count matvecs of the Krylov loops; seconds are the
-    solver's wall time (the host's small dense problem of every restart
-    included)."""
+    eight incidence angles batched (FP64 GMRES(100) and GMRES-DR(100, 20);
+    mixed FGMRES and refinement around GMRES-DR), the sixteen c4 scatterers
+    x 8 angles with refinement around FP32-far-field GMRES-DR (one forest
+    plan; 16 concurrent plans), then the seed-1 c4 ellipsoids under
+    theta = 0 in FP64 GMRES(100) and GMRES-DR(100, 20), tol 1e-6.
+    Iterations count matvecs of the Krylov loops; seconds are the solver's
+    wall time (the host's small dense problem of every restart included).
+    --parts picks the parts (single, batched, c4, ellipsoids)."""
     from mlfma_amd import _capi as A
     from mlfma_amd import configs as CF
     from mlfma_amd.plan import Plan
     geom, pw, kw, desc = workload(args.workload)
     fl = A.MLFMA_FLAG_PHYSICAL | A.MLFMA_FLAG_FRESH_LOCALS
     keys = ["iterations", "status", "resid_true", "seconds"]
+    parts = set(args.parts.split(","))
     out = {"mode": "deflation", "workload": desc["workload"], "unknowns": int(geom.n), "tol": 1e-6, "solves": {}}
+    if parts & {"single", "batched"}:
+        deflation_sphere(A, CF, Plan, geom, pw, kw, fl, keys, out, parts)
+    if "c4" in parts:
+        deflation_c4(A, CF, Plan, fl, out)
+    if "ellipsoids" in parts:
+        deflation_ellipsoids(A, CF, Plan, fl, keys, out)
+    print(json.dumps(out), flush=True)
+
+
+def deflation_sphere(A, CF, Plan, geom, pw, kw, fl, keys, out, parts):
     q64 = Plan(geom, pw, A.make_config(device=0, flags=fl, **kw))
     q64.autotune(1)
     q32 = Plan(geom, pw, A.make_config(device=0, precision=A.MLFMA_FP32, flags=fl, **kw))
@@ -543,7 +558,7 @@ def deflation_job(args):
             ("mixed_ir_dr100_20", q32, dict(method=A.MLFMA_GMRES_IR, restart=100, deflate=20)),
             ("mixed_ir_dr60_15", q32, dict(method=A.MLFMA_GMRES_IR, restart=60, deflate=15))]
     ref = None
-    for name, p, o in runs:
+    for name, p, o in (runs if "single" in parts else []):
         x, r = p.solve(maxit=6000, tol=1e-6, **o)
         if ref is None:
             ref = x
@@ -551,7 +566,66 @@ def deflation_job(args):
         e["solution_rel_diff_vs_fp64_gmres"] = float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
         out["solves"][name] = e
         log(name, e)
-    del q64, q32
+    if "batched" not in parts:
+        return
+    # eight incidence angles batched (one multi-RHS matvec per step)
+    B = np.stack([-q64.excitation(CF.planewave_xz(t)) for t in CF.c4_angles()])
+    q64.autotune(8)
+    q32.autotune(8)
+    out["batched_8_angles"] = {}
+    for name, p, o in [("fp64_gmres100", q64, dict(method=A.MLFMA_GMRES_RESTART, restart=100)),
+                       ("fp64_gmres_dr100_20", q64, dict(method=A.MLFMA_GMRES_RESTART, restart=100, deflate=20)),
+                       ("mixed_fgmres30_30", q32, dict(method=A.MLFMA_FGMRES_IR, restart=30, outer_restart=30,
+                                                       inner_tol=0.1)),
+                       ("mixed_ir_dr60_15", q32, dict(method=A.MLFMA_GMRES_IR, restart=60, deflate=15)),
+                       ("mixed_ir_dr100_20", q32, dict(method=A.MLFMA_GMRES_IR, restart=100, deflate=20))]:
+        t = time.perf_counter()
+        _, reps = p.solve_batch(B, maxit=6000, tol=1e-6, **o)
+        e = {"seconds": round(time.perf_counter() - t, 4), "iterations": [r["iterations"] for r in reps],
+             "converged": sum(r["status"] == "OK" for r in reps), "max_resid_true": max(r["resid_true"] for r in reps)}
+        out["batched_8_angles"][name] = e
+        log("8 angles", name, e)
+
+
+def deflation_c4(A, CF, Plan, fl, out):
+    # sixteen c4 scatterers x 8 angles: one forest plan and 16 concurrent
+    # plans, FP64 refinement around FP32-far-field GMRES-DR
+    from mlfma_amd.batch import solve_many
+    shapes = CF.c4()
+    cfgs = [A.make_config(device=0, precision=A.MLFMA_FP32, flags=fl, **sh["kw"]) for sh in shapes]
+    pwc = CF.planewave_theta0()
+    F = Plan.forest([sh["geom"] for sh in shapes], cfgs, pwc)
+    off = F.segments()
+    Bc = np.stack([-F.excitation(CF.planewave_xz(t)) for t in CF.c4_angles()])
+    plans = [Plan(sh["geom"], pwc, c) for sh, c in zip(shapes, cfgs)]
+    rhs = [np.ascontiguousarray(Bc[:, off[i]:off[i + 1]]) for i in range(len(shapes))]
+    F.autotune(8)
+    for p in plans:
+        p.autotune(8)
+    nseg = len(shapes)
+    out["c4_16_scatterers_x_8_angles"] = {}
+    for name, o in [("ir_dr60_15", dict(method=A.MLFMA_GMRES_IR, restart=60, deflate=15)),
+                    ("ir_dr100_20", dict(method=A.MLFMA_GMRES_IR, restart=100, deflate=20)),
+                    ("fp64_dr100_20", dict(method=A.MLFMA_GMRES_RESTART, restart=100, deflate=20))]:
+        t = time.perf_counter()
+        _, rf = F.solve_batch(Bc, maxit=6000, tol=1e-6, **o)
+        tf = time.perf_counter() - t
+        t = time.perf_counter()
+        rc = solve_many(plans, rhs, maxit=6000, tol=1e-6, **o)
+        tc = time.perf_counter() - t
+        conv_f = sum(all(rf[a * nseg + i]["status"] == "OK" for a in range(8)) for i in range(nseg))
+        conv_c = sum(all(r["status"] == "OK" for r in reps) for _, reps in rc)
+        e = {"forest_s": round(tf, 3), "concurrent_plans_s": round(tc, 3), "forest_converged_scatterers": conv_f,
+             "concurrent_converged_scatterers": conv_c,
+             "max_iterations": max(r["iterations"] for r in rf)}
+        out["c4_16_scatterers_x_8_angles"][name] = e
+        log("c4 x 8", name, e)
+    for p in plans:
+        p.close()
+    F.close()
+
+
+def deflation_ellipsoids(A, CF, Plan, fl, keys, out):
     pw0 = CF.planewave_xz(0.0)
     ells = {}
     for i, sh in enumerate(CF.c4()):
@@ -567,7 +641,6 @@ def deflation_job(args):
         log("c4 ellipsoid", i, e)
         ells[str(i)] = e
     out["c4_ellipsoids_theta0_fp64"] = ells
-    print(json.dumps(out), flush=True)
 
 
 def m2l_ceiling(args):
@@ -723,6 +796,8 @@ def main():
     ap.add_argument("--fp32", action="store_true",
                     help="with --timed-only: the plan's far field in FP32 (the inner matvec of the "
                          "mixed-precision solves) instead of FP64")
+    ap.add_argument("--parts", default="single,batched,c4,ellipsoids",
+                    help="with --mode deflation: which parts to run (single, batched, c4, ellipsoids)")
     ap.add_argument("--mode", default="matvec", choices=["matvec", "copy", "m2l-ceiling", "hbm", "c4-stalls",
                                                               "deflation"],
                     help="copy: only run a plain 256 MiB device copy (HBM counter calibration); m2l-ceiling: the "

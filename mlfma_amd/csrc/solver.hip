@@ -1253,9 +1253,13 @@ static void parallelFor(int count, F f) {
 // FP64 for its matvec, and the products and sums stay FP64.
 // deflate = k > 0: deflated restarts per lane as in gmresRestartT (the host
 // problems of the lanes on parallel threads, the recombination launched per
-// lane).  The lanes stay in lockstep: a cycle starts at step k only when
-// every unfinished lane holds a deflated start, otherwise every lane
-// restarts plainly from its residual.
+// lane).  The lanes stay in lockstep: once any unfinished lane holds a
+// deflated start, the cycle starts at step k for all, and a lane without one
+// (it stopped early on its estimate, or its restart problem failed) restarts
+// plainly inside that cycle: zero basis vectors and kept inputs in columns
+// 0..k-1 with identity Hessenberg columns and a zero right-hand side there
+// (their coefficients come out exactly zero), its residual direction in
+// column k; its next deflated restart uses columns k..m only.
 template <typename TQ>
 static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                          const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m, int maxit,
@@ -1311,13 +1315,16 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
             all = all && done[r];
         }
         if (all || total >= maxit) break;
-        // the cycle's first step: k when every unfinished lane has a deflated
-        // start, else 0 (every lane restarts plainly)
-        int k0 = kd;
+        // the cycle's first step: k when an unfinished lane has a deflated
+        // start; the other unfinished lanes start plainly at column k
+        int k0 = 0;
         for (int r = 0; r < nrhs; ++r)
-            if (!done[r] && dr[r].k != kd) k0 = 0;
-        if (k0 == 0)
-            for (auto& e : dr) e = dense::DrState{};
+            if (!done[r] && dr[r].k == kd && kd) k0 = kd;
+        std::vector<char> embed(nrhs, 0);
+        for (int r = 0; r < nrhs; ++r) {
+            embed[r] = k0 > 0 && !done[r] && dr[r].k != kd;
+            if (k0 == 0 || embed[r]) dr[r] = dense::DrState{};
+        }
         // Pipelined cycle, as in gmresRestart: every step (batched matvec,
         // CGS2 of all lanes, per-lane Givens update, scaling) is queued without
         // a host round trip, up to kWindow steps ahead of the host, which reads
@@ -1334,10 +1341,17 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
             if (k0 == 0) {
                 if (active[r]) K[r]->scaleTo<TQ>(W + LN.base[r], beta[r], Q + LN.base[r]);
                 else hipCheck(hipMemsetAsync(Q + LN.base[r], 0, LN.len[r] * sizeof(cplx<TQ>), st), "zero lane");
-            } else if (!active[r])
+            } else if (!active[r] || embed[r]) {
                 for (int c = 0; c <= k0; ++c)
                     hipCheck(hipMemsetAsync(Q + (size_t)c * ld + LN.base[r], 0, LN.len[r] * sizeof(cplx<TQ>), st),
                              "zero lane");
+                if (embed[r]) {
+                    for (int c = 0; c < k0; ++c)
+                        hipCheck(hipMemsetAsync(Zk + (size_t)c * ld + LN.base[r], 0, LN.len[r] * sizeof(cplx<TQ>),
+                                                st), "zero lane");
+                    K[r]->scaleTo<TQ>(W + LN.base[r], beta[r], Q + (size_t)k0 * ld + LN.base[r]);
+                }
+            }
         }
         hipCheck(hipMemcpyAsync(GB.on, onh.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, st), "lanes");
         hipCheck(hipMemcpyAsync(GB.bn, bn.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st), "norms");
@@ -1355,6 +1369,11 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
             for (int r = 0; r < nrhs; ++r) {
                 if (!active[r]) continue;
                 cplx<double>* Hr = hH.data() + (size_t)r * (m + 1) * (m + 1);
+                if (embed[r]) {  // plain start at column k0
+                    for (int c = 0; c < k0; ++c) Hr[(size_t)c * (m + 1) + c] = cplx<double>{1, 0};
+                    hg[(size_t)r * (m + 2) + k0] = cplx<double>{beta[r], 0};
+                    continue;
+                }
                 for (int c = 0; c < k0; ++c)
                     for (int i = 0; i <= c; ++i)
                         Hr[(size_t)c * (m + 1) + i] = cplx<double>{dr[r].Rk(i, c).real(), dr[r].Rk(i, c).imag()};
@@ -1437,7 +1456,8 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
         std::vector<zc> gml(nrhs);
         std::vector<char> want(nrhs, 0);
         for (int r = 0; r < nrhs; ++r)
-            want[r] = kd && steps[r] == m && active[r] && !breakdown && total < maxit;
+            want[r] = kd && steps[r] == m && active[r] && !breakdown && total < maxit &&
+                      (!embed[r] || kd + 1 < m - k0);
         for (int r = 0; r < nrhs; ++r)
             if (steps[r] > 0) {
                 const int j = steps[r];
@@ -1467,18 +1487,20 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
                 } else
                     K[r]->combine<TQ>(Q + LN.base[r], j, y, dst, PT ? 0 : 1, ld);
                 if (want[r]) {
+                    // an embedded plain start: its cycle is columns e..m
+                    const int e = embed[r] ? k0 : 0, mm = m - e;
                     std::vector<cplx<double>> ch(m), sh(m);
                     hipCheck(hipMemcpy(ch.data(), GB.cs + (size_t)r * (m + 1), m * sizeof(cplx<double>),
                                        hipMemcpyDeviceToHost), "cs copy");
                     hipCheck(hipMemcpy(sh.data(), GB.sn + (size_t)r * (m + 1), m * sizeof(cplx<double>),
                                        hipMemcpyDeviceToHost), "sn copy");
-                    Rl[r] = dense::Mat(m, m);
-                    csl[r].resize(m);
-                    snl[r].resize(m);
-                    for (int c = 0; c < m; ++c) {
-                        for (int i = 0; i <= c; ++i) Rl[r](i, c) = H[c][i];
-                        csl[r][c] = zc(ch[c].re, ch[c].im);
-                        snl[r][c] = zc(sh[c].re, sh[c].im);
+                    Rl[r] = dense::Mat(mm, mm);
+                    csl[r].resize(mm);
+                    snl[r].resize(mm);
+                    for (int c = 0; c < mm; ++c) {
+                        for (int i = 0; i <= c; ++i) Rl[r](i, c) = H[c + e][i + e];
+                        csl[r][c] = zc(ch[c + e].re, ch[c + e].im);
+                        snl[r][c] = zc(sh[c + e].re, sh[c + e].im);
                     }
                     gml[r] = zc(gh[m].re, gh[m].im);
                 }
@@ -1497,7 +1519,8 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
             std::vector<char> kept(nrhs, 0);
             parallelFor(nrhs, [&](int r) {
                 if (want[r])
-                    kept[r] = dense::drRestart(Rl[r], csl[r], snl[r], gml[r], k0, dr[r].Qhat, kd, nd[r]) &&
+                    kept[r] = dense::drRestart(Rl[r], csl[r], snl[r], gml[r], embed[r] ? 0 : k0, dr[r].Qhat, kd,
+                                               nd[r]) &&
                               nd[r].defect < 1e-8;
             });
             std::vector<cplx<double>> ph(pSz * nrhs);
@@ -1508,14 +1531,15 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
                     continue;
                 }
                 any = true;
+                const int mm = m - (embed[r] ? k0 : 0);  // the lane's cycle columns
                 cplx<double>* pk = ph.data() + pSz * r;
-                cplx<double>* pu = pk + (size_t)m * kd;
+                cplx<double>* pu = pk + (size_t)mm * kd;
                 for (int c = 0; c < kd; ++c)
-                    for (int i = 0; i < m; ++i)
-                        pk[(size_t)c * m + i] = cplx<double>{nd[r].Pk(i, c).real(), nd[r].Pk(i, c).imag()};
+                    for (int i = 0; i < mm; ++i)
+                        pk[(size_t)c * mm + i] = cplx<double>{nd[r].Pk(i, c).real(), nd[r].Pk(i, c).imag()};
                 for (int c = 0; c <= kd; ++c)
-                    for (int i = 0; i <= m; ++i)
-                        pu[(size_t)c * (m + 1) + i] = cplx<double>{nd[r].U(i, c).real(), nd[r].U(i, c).imag()};
+                    for (int i = 0; i <= mm; ++i)
+                        pu[(size_t)c * (mm + 1) + i] = cplx<double>{nd[r].U(i, c).real(), nd[r].U(i, c).imag()};
             }
             if (any) {
                 hipCheck(hipMemcpyAsync(dP, ph.data(), ph.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
@@ -1524,11 +1548,13 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
                     if (!kept[r]) continue;
                     const int64_t len = LN.len[r];
                     const dim3 g((unsigned)((len + KB - 1) / KB));
+                    const int e = embed[r] ? k0 : 0, mm = m - e;
+                    const cplx<TQ>* q0 = Q + (size_t)e * ld + LN.base[r];
                     const cplx<double>* pk = dP + pSz * r;
-                    hipLaunchKernelGGL(kRecombine<TQ>, g, dim3(KB), 0, st, (const cplx<TQ>*)(Q + LN.base[r]), ld, m,
-                                       pk, m, kd, Zk + LN.base[r], ld, len);
-                    hipLaunchKernelGGL(kRecombine<TQ>, g, dim3(KB), 0, st, (const cplx<TQ>*)(Q + LN.base[r]), ld,
-                                       m + 1, pk + (size_t)m * kd, m + 1, kd + 1, Vn + LN.base[r], ld, len);
+                    hipLaunchKernelGGL(kRecombine<TQ>, g, dim3(KB), 0, st, q0, ld, mm, pk, mm, kd, Zk + LN.base[r],
+                                       ld, len);
+                    hipLaunchKernelGGL(kRecombine<TQ>, g, dim3(KB), 0, st, q0, ld, mm + 1, pk + (size_t)mm * kd,
+                                       mm + 1, kd + 1, Vn + LN.base[r], ld, len);
                 }
                 for (int r = 0; r < nrhs; ++r) {
                     if (!kept[r]) continue;
