@@ -234,7 +234,8 @@ int mlfma_matvec_direct(mlfma_plan* plan, const double* x, double* y, int32_t pt
 #define MLFMA_GMRES_IR 3       /* mixed precision by iterative refinement: FP64
                                   residual, restarted FP32-far-field GMRES(restart)
                                   corrections (no outer Krylov space), at most 20
-                                  (30 batched) corrections                      */
+                                  (30 batched, each inner solve at most 30 x
+                                  restart steps) corrections                    */
 #define MLFMA_SOLVE_ADJOINT 0x100  /* or-ed into method: solve Z^H x = b instead
                                       (the adjoint solve of a gradient; no
                                       reference counterpart)                       */

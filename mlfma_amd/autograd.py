@@ -105,10 +105,11 @@ def matvec(plan: Plan, x: torch.Tensor) -> torch.Tensor:
 
 
 def solve(plan: Plan, b: torch.Tensor, method: int = A.MLFMA_GMRES_RESTART, restart: int = 50,
-          maxit: int = 500, tol: float = 1e-8) -> torch.Tensor:
+          maxit: int = 500, tol: float = 1e-8, deflate: int = 0) -> torch.Tensor:
     """x = Z^-1 b by GMRES, differentiable in b (the backward pass is an
-    adjoint GMRES solve with the same options)."""
-    return _Solve.apply(b, plan, dict(method=method, restart=restart, maxit=maxit, tol=tol))
+    adjoint GMRES solve with the same options; deflate = k: deflated
+    restarts keeping k harmonic Ritz vectors)."""
+    return _Solve.apply(b, plan, dict(method=method, restart=restart, maxit=maxit, tol=tol, deflate=deflate))
 
 
 def farfield(plan: Plan, I: torch.Tensor, dirs) -> torch.Tensor:

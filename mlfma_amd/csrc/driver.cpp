@@ -5,7 +5,7 @@
 // (one "re im" line per unknown at %.15e, src/solver.cpp:149-165).
 //
 // usage: mlfma [--root DIR] [--device D] [--method compat|gmres|fgmres|ir] [--restart M]
-//              [--maxit K] [--tol T] [--max-level L] [--dipoles FILE] [--fp32]
+//              [--deflate K] [--maxit K] [--tol T] [--max-level L] [--dipoles FILE] [--fp32]
 //        mlfma --root DIR --print-config   (print the parsed config.txt and exit;
 //                                           no device is touched)
 #include <cctype>
@@ -114,7 +114,7 @@ void check(int rc, const char* what) {
 
 int main(int argc, char** argv) {
     fs::path root = ".";
-    int device = 0, restart = 50, maxit = 500, maxLevel = -1;
+    int device = 0, restart = 50, maxit = 500, maxLevel = -1, deflate = 0;
     double tol = 1.0e-6;
     std::string method = "compat", dipoles;
     bool fp32 = false, printConfig = false;
@@ -128,6 +128,7 @@ int main(int argc, char** argv) {
         else if (a == "--device") device = std::stoi(next());
         else if (a == "--method") method = next();
         else if (a == "--restart") restart = std::stoi(next());
+        else if (a == "--deflate") deflate = std::stoi(next());
         else if (a == "--maxit") maxit = std::stoi(next());
         else if (a == "--tol") tol = std::stod(next());
         else if (a == "--max-level") maxLevel = std::stoi(next());
@@ -223,6 +224,7 @@ int main(int argc, char** argv) {
                        : method == "ir"     ? MLFMA_GMRES_IR
                                             : MLFMA_GMRES_RESTART;
             o.restart = restart;
+            o.deflate = deflate;  // gmres / ir: harmonic Ritz vectors kept per restart
             o.maxit = maxit;
             o.use_voltage = 1;
             o.tol = tol;

@@ -1589,8 +1589,8 @@ static std::vector<SolveOut> gmresBatch(DevicePlan& d, std::vector<std::unique_p
 
 // Batched iterative refinement (MLFMA_GMRES_IR): FP64 residuals of all
 // right-hand sides (batched FP64 matvec) around batched inner GMRES(m) solves
-// with the FP32 far field, at most 30 corrections; each right-hand side's
-// inner tolerance is max(1e-4, tol / (2 residual)).
+// with the FP32 far field (at most 30 m steps each), at most 30 corrections;
+// each right-hand side's inner tolerance is max(1e-4, tol / (2 residual)).
 static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique_ptr<Krylov>>& K, hipStream_t st,
                                           const cplx<double>* B, cplx<double>* X, const Lanes& LN, int m,
                                           int maxit, double tol, double* hist, int deflate) {
@@ -1625,8 +1625,10 @@ static std::vector<SolveOut> gmresBatchIR(DevicePlan& d, std::vector<std::unique
             itol[r] = std::max(1e-4, 0.5 * tol / res);
         }
         if (all || total >= maxit || outer == 30) break;
-        std::vector<SolveOut> in =
-            gmresBatch(d, K, st, R, D, LN, m, maxit - total, itol, hist, true, total, maxit, true, deflate);
+        // each correction's inner solve at most 30 m steps: a lane that
+        // stalls then cannot spend the whole budget of the others' corrections
+        std::vector<SolveOut> in = gmresBatch(d, K, st, R, D, LN, m, std::min(maxit - total, 30 * m), itol, hist,
+                                              true, total, maxit, true, deflate);
         int steps = 0;
         for (int r = 0; r < nrhs; ++r) {
             out[r].iters += in[r].iters;
