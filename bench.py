@@ -200,8 +200,14 @@ def solves(geom, pw, kw, desc):
     tb = time.perf_counter()
     Xb, rb = q64.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6)
     t_b64 = time.perf_counter() - tb
+    tb = time.perf_counter()
+    _, rbd = q64.solve_batch(B, restart=restart, maxit=maxit, tol=1e-6, deflate=20)
+    t_b64d = time.perf_counter() - tb
+    log("physical 8 angles batched FP64 GMRES-DR", round(t_b64d, 3), [r["iterations"] for r in rbd])
     rd = max(rcs_diff(rcs(q32, Xm[j]), rcs(q64, seq[j][0])) for j in range(8))
     sweep = {"angles": 8, "fp64_one_at_a_time_s": round(t_seq, 4), "batched_fp64_s": round(t_b64, 4),
+             "batched_fp64_deflated_s": round(t_b64d, 4),
+             "batched_fp64_deflated_converged": all(r["status"] == "OK" for r in rbd),
              "batched_mixed_s": round(t_mix, 4), "speedup_batched_mixed_vs_fp64": round(t_seq / t_mix, 3),
              "all_converged": all(r["status"] == "OK" for r in rm) and all(r["status"] == "OK" for _, r in seq),
              "max_resid_true_mixed": max(r["resid_true"] for r in rm),

@@ -1361,33 +1361,37 @@ static std::vector<SolveOut> gmresBatchT(DevicePlan& d, std::vector<std::unique_
         std::vector<cplx<double>> hH, hg, hc, hs;  // deflated starts (alive until the sync below)
         if (k0 > 0) {
             // per active lane: its triangular columns, identity rotations and
-            // rotated right-hand side (Q[0..k0] and Zk hold its block)
-            hH.assign((size_t)nrhs * (m + 1) * (m + 1), cplx<double>{0, 0});
-            hg.assign((size_t)nrhs * (m + 2), cplx<double>{0, 0});
-            hc.assign((size_t)nrhs * (m + 1), cplx<double>{1, 0});
-            hs.assign((size_t)nrhs * (m + 1), cplx<double>{0, 0});
+            // rotated right-hand side (Q[0..k0] and Zk hold its block); only
+            // each lane's first k0 columns, k0 rotations and k0 + 1 entries
+            // of g travel (the steps write the rest; kInitGB zeroed g)
+            const size_t hc0 = (size_t)k0 * (m + 1);
+            hH.assign((size_t)nrhs * hc0, cplx<double>{0, 0});
+            hg.assign((size_t)nrhs * (k0 + 1), cplx<double>{0, 0});
+            hc.assign((size_t)nrhs * k0, cplx<double>{1, 0});
+            hs.assign((size_t)nrhs * k0, cplx<double>{0, 0});
             for (int r = 0; r < nrhs; ++r) {
                 if (!active[r]) continue;
-                cplx<double>* Hr = hH.data() + (size_t)r * (m + 1) * (m + 1);
+                cplx<double>* Hr = hH.data() + (size_t)r * hc0;
                 if (embed[r]) {  // plain start at column k0
                     for (int c = 0; c < k0; ++c) Hr[(size_t)c * (m + 1) + c] = cplx<double>{1, 0};
-                    hg[(size_t)r * (m + 2) + k0] = cplx<double>{beta[r], 0};
+                    hg[(size_t)r * (k0 + 1) + k0] = cplx<double>{beta[r], 0};
                     continue;
                 }
                 for (int c = 0; c < k0; ++c)
                     for (int i = 0; i <= c; ++i)
                         Hr[(size_t)c * (m + 1) + i] = cplx<double>{dr[r].Rk(i, c).real(), dr[r].Rk(i, c).imag()};
                 for (int i = 0; i <= k0; ++i)
-                    hg[(size_t)r * (m + 2) + i] = cplx<double>{dr[r].g[i].real(), dr[r].g[i].imag()};
+                    hg[(size_t)r * (k0 + 1) + i] = cplx<double>{dr[r].g[i].real(), dr[r].g[i].imag()};
             }
-            hipCheck(hipMemcpyAsync(GB.H, hH.data(), hH.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
-                     "H upload");
-            hipCheck(hipMemcpyAsync(GB.g, hg.data(), hg.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
-                     "g upload");
-            hipCheck(hipMemcpyAsync(GB.cs, hc.data(), hc.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
-                     "cs upload");
-            hipCheck(hipMemcpyAsync(GB.sn, hs.data(), hs.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, st),
-                     "sn upload");
+            const size_t z = sizeof(cplx<double>);
+            hipCheck(hipMemcpy2DAsync(GB.H, (size_t)(m + 1) * (m + 1) * z, hH.data(), hc0 * z, hc0 * z, nrhs,
+                                      hipMemcpyHostToDevice, st), "H upload");
+            hipCheck(hipMemcpy2DAsync(GB.g, (size_t)(m + 2) * z, hg.data(), (k0 + 1) * z, (k0 + 1) * z, nrhs,
+                                      hipMemcpyHostToDevice, st), "g upload");
+            hipCheck(hipMemcpy2DAsync(GB.cs, (size_t)(m + 1) * z, hc.data(), k0 * z, k0 * z, nrhs,
+                                      hipMemcpyHostToDevice, st), "cs upload");
+            hipCheck(hipMemcpy2DAsync(GB.sn, (size_t)(m + 1) * z, hs.data(), k0 * z, k0 * z, nrhs,
+                                      hipMemcpyHostToDevice, st), "sn upload");
             hipCheck(hipStreamSynchronize(st), "deflated start");
         }
         bool breakdown = false;
