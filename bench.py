@@ -541,7 +541,7 @@ def deflation_job(args):
     keys = ["iterations", "status", "resid_true", "seconds"]
     parts = set(args.parts.split(","))
     out = {"mode": "deflation", "workload": desc["workload"], "unknowns": int(geom.n), "tol": 1e-6, "solves": {}}
-    if parts & {"single", "batched"}:
+    if parts & {"single", "batched", "dr"}:
         deflation_sphere(A, CF, Plan, geom, pw, kw, fl, keys, out, parts)
     if "c4" in parts:
         deflation_c4(A, CF, Plan, fl, out)
@@ -563,8 +563,10 @@ def deflation_sphere(A, CF, Plan, geom, pw, kw, fl, keys, out, parts):
             ("mixed_fgmres30_30", q32, dict(method=A.MLFMA_FGMRES_IR, restart=30, outer_restart=30, inner_tol=0.1)),
             ("mixed_ir_dr100_20", q32, dict(method=A.MLFMA_GMRES_IR, restart=100, deflate=20)),
             ("mixed_ir_dr60_15", q32, dict(method=A.MLFMA_GMRES_IR, restart=60, deflate=15))]
+    if "single" not in parts:  # "dr": the deflated FP64 solve alone (for kernel profiles)
+        runs = [r for r in runs if "dr" in parts and r[0] == "fp64_gmres_dr100_20"]
     ref = None
-    for name, p, o in (runs if "single" in parts else []):
+    for name, p, o in runs:
         x, r = p.solve(maxit=6000, tol=1e-6, **o)
         if ref is None:
             ref = x
@@ -803,7 +805,8 @@ def main():
                     help="with --timed-only: the plan's far field in FP32 (the inner matvec of the "
                          "mixed-precision solves) instead of FP64")
     ap.add_argument("--parts", default="single,batched,c4,ellipsoids",
-                    help="with --mode deflation: which parts to run (single, batched, c4, ellipsoids)")
+                    help="with --mode deflation: which parts to run (single, batched, c4, ellipsoids; dr: "
+                         "only the FP64 GMRES-DR(100, 20) solve of the workload)")
     ap.add_argument("--mode", default="matvec", choices=["matvec", "copy", "m2l-ceiling", "hbm", "c4-stalls",
                                                               "deflation"],
                     help="copy: only run a plain 256 MiB device copy (HBM counter calibration); m2l-ceiling: the "
