@@ -235,11 +235,14 @@ inline bool hessEigvals(Mat T, std::vector<zc>& w) {
             w[0] = T(0, 0);
             break;
         }
+        // deflation test in the 1-norm of the real and imaginary parts
+        // (|re| + |im|, as LAPACK's complex QR does: no square roots)
+        auto cabs1 = [](zc z) { return std::abs(z.real()) + std::abs(z.imag()); };
         int l = hi;
         for (; l > 0; --l) {
-            double s = std::abs(T(l - 1, l - 1)) + std::abs(T(l, l));
+            double s = cabs1(T(l - 1, l - 1)) + cabs1(T(l, l));
             if (s == 0) s = tnorm;
-            if (std::abs(T(l, l - 1)) <= eps * s) {
+            if (cabs1(T(l, l - 1)) <= eps * s) {
                 T(l, l - 1) = 0.0;
                 break;
             }
